@@ -1,0 +1,92 @@
+/* C ABI of libpalmtracer_amd.so: the PALMTracer detection + fitting path on one MI355X (gfx950).
+ *
+ * The first block is a drop-in for the reference's exported hot-path entry points
+ * (tmonseigne/PALMTracer-cpp, CPU/main.cpp and CPU/PALMTracer.def): same names, argument types,
+ * argument meaning, return values and failure behaviour (0 / no rows on failure, never an
+ * exception across the boundary). All buffers are caller-owned host memory, borrowed for the call.
+ *
+ * The second block has no counterpart in the reference: entry points over device pointers (for
+ * callers that keep their stacks in HBM), options and diagnostics.
+ */
+#ifndef PALMTRACER_AMD_H
+#define PALMTRACER_AMD_H
+
+#include <stdint.h>
+
+#if defined(_WIN32) || defined(_WIN64)
+#define PT_EXPORT __declspec(dllexport)
+#else
+#define PT_EXPORT __attribute__((visibility("default")))
+#endif
+#define PT_CALL
+#ifdef __cplusplus
+#define PT_API extern "C" PT_EXPORT
+#else
+#define PT_API PT_EXPORT
+#endif
+
+/* ---- reference entry points --------------------------------------------------------------- */
+
+/* Replaces Localization (CPU/main.cpp:112-121 -> lib/Palm.cpp:48-70).
+ * input: planes x height x width u16, x fastest. output: outputSize doubles; rows of 18 columns
+ * [Id, Plane, Index, Channel, X, Y, Z, Intensity, SigmaX, SigmaY, Theta, QualityXY, QualityZ,
+ *  Amplitude, Offset, SegIntensity, Surface, Circularity] (lib/Constants.hpp:31-51), plane-major,
+ * each plane sorted by (Y, X). At most (outputSize / 18) / planes rows per plane, kept in label
+ * order (lib/fit/Fitting.cpp:44-58). fit: 0 none, 1 sigma fixed, 2 sigma free, 3 sigmaX/sigmaY,
+ * 4 sigmaX/sigmaY/theta; fit_params = [roi, sigmaX, sigmaY, theta]. watershedRatio <= 0 keeps the
+ * maxima separate (reference value 0); (int)ratio >= 1 enables basin fusion.
+ * Returns the number of doubles written. */
+PT_API uint64_t PT_CALL Localization(const uint16_t* input, double* output, uint64_t outputSize, uint64_t height,
+                                     uint64_t width, uint64_t planes, double thresholdVal, double watershedRatio,
+                                     uint64_t fit, const double* fit_params);
+
+/* Replaces Fitting (CPU/main.cpp:141-148 -> lib/Palm.cpp:73-119). positions: nPoints rows
+ * [Id, Plane(1-based), X, Y, SegIntensity, Surface] (lib/Constants.hpp:56-64). Rows are written in
+ * input order; outputSize is ignored as in the reference. fit in 1..4. Returns 18 * nPoints. */
+PT_API uint64_t PT_CALL Fitting(const uint16_t* input, const double* positions, uint64_t nPoints, double* output,
+                                uint64_t outputSize, uint64_t height, uint64_t width, uint64_t planes, uint64_t fit,
+                                const double* fit_params);
+
+/* Replaces Wavelett (CPU/main.cpp:257-284): level 1 = W0, 2 = W1, 3 (or more) = residual V2,
+ * 0 is read as 1. output: planes x height x width doubles. */
+PT_API void PT_CALL Wavelett(const uint16_t* input, double* output, uint64_t height, uint64_t width, uint64_t planes,
+                             uint64_t levelOutput);
+
+/* ---- no counterpart in the reference ------------------------------------------------------ */
+
+/* Localization over device memory: d_input (planes x height x width u16) and d_output
+ * (outputCapDoubles doubles) are HBM pointers; maxPerPlane is the per-plane row capacity. */
+PT_API uint64_t PT_CALL pt_localization_device(const uint16_t* d_input, double* d_output, uint64_t outputCapDoubles,
+                                               uint64_t height, uint64_t width, uint64_t planes, double thresholdVal,
+                                               double watershedRatio, uint64_t fit, const double* fit_params,
+                                               uint64_t maxPerPlane);
+
+/* Wavelett over device memory. Returns 1 on success, 0 on failure. */
+PT_API int PT_CALL pt_wavelet_device(const uint16_t* d_input, double* d_output, uint64_t height, uint64_t width,
+                                     uint64_t planes, uint64_t levelOutput);
+
+/* Watershed label map of one host frame (watershedRatio 0): labels[i] = raster index + 1 of the
+ * first local maximum of the pixel's label, 0 below threshold. Returns 1 on success. */
+PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint64_t width, double thresholdVal,
+                                   uint32_t* labels);
+
+/* Options: "fit_fp64" (1: FP64 model and sums, 0: FP32), "max_iter" (LM iteration cap, 75),
+ * "chunk_pixels" (frames per device chunk x pixels). Returns 1 on success. */
+PT_API int PT_CALL pt_set_option(const char* name, double value);
+
+/* Counters of the last call: frames, chunks, rows, frames re-flooded sequentially, multi-label
+ * components, mask pixels, status. Writes min(n, 7) values, returns 7. */
+PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n);
+
+/* Seeded synthetic stack on the device (benchmark / test input): d_out receives frames x height x
+ * width u16. params (15 doubles): bg photons, PSF sigma (px), photon median, photon sigma_log, camera
+ * offset (ADU), gain (ADU/photon), astigmatic flag, s0 (px), c (nm), d (nm), zmin, zmax (nm),
+ * sCMOS flag, offset sd, gain sd. Returns 1 on success. */
+PT_API int PT_CALL pt_synth_device(uint16_t* d_out, uint64_t seed, int width, int height, int frames, int emitters,
+                                   const double* params);
+
+/* Ground truth of pt_synth_device: frames x emitters x [x, y, sigma_x, sigma_y, photons] floats. */
+PT_API int PT_CALL pt_synth_emitters_device(float* d_out, uint64_t seed, int width, int height, int frames,
+                                            int emitters, const double* params);
+
+#endif

@@ -1,0 +1,64 @@
+// Device-resident pipeline of the PALMTracer detection + fitting path (host side).
+//
+// Mirrors the reference's orchestration (lib/Palm.cpp Palm::Processing / FitLocalizations /
+// AutoThreshold, lib/localization/FrameProcessor.cpp) over chunks of frames: every stage of a chunk
+// is one launch over all its frames, and all intermediate state stays in HBM.
+#pragma once
+#include "pt_kernels.h"
+
+#include <mutex>
+#include <vector>
+
+namespace pt {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t n);
+    void release();
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct RunStats {
+    uint64_t frames = 0, chunks = 0, rois = 0, flaggedFrames = 0, multiComponents = 0, maskPixels = 0;
+    uint32_t status = 0;
+};
+
+struct Options {
+    bool fitFp64 = true;           // FP64 model/Jacobian (reference arithmetic) or FP32
+    int maxIter = kMaxIterDefault; // Fitting::N_ITER
+    size_t chunkPixels = size_t(64) << 20;
+};
+
+class Engine {
+public:
+    static Engine& get();
+    std::mutex mu;
+    Options opt;
+    RunStats last;
+    hipStream_t stream = nullptr;
+
+    // Localization over a device-resident stack; rows written to d_out (capacity outCapRows).
+    // Returns the number of doubles written (a multiple of 18).
+    uint64_t localize(const uint16_t* d_in, uint64_t planes, int H, int W, double thr, double ratio, uint64_t fit,
+                      const double* fitParams, uint64_t maxPerPlane, double* d_out, uint64_t outCapRows);
+    // Fit-only over caller positions (6 columns each), rows in input order.
+    uint64_t fit_positions(const uint16_t* d_in, uint64_t planes, int H, int W, const double* hostPos, uint64_t n,
+                           uint64_t fit, const double* fitParams, double* d_out);
+    void wavelet(const uint16_t* d_in, uint64_t planes, int H, int W, int level, double* d_out);
+    // Label map of one frame after the watershed (labels = first-maximum raster index + 1).
+    void debug_labels(const uint16_t* d_frame, int H, int W, double thr, uint32_t* d_labels);
+
+    DevBuf in, out, rows, rois;
+
+private:
+    Engine();
+    DetectBuffers b_{};
+    DevBuf P_, lab_, fin_, aux_, mask_, seed_, perFrame_, regions_, multi_, scratch_, fb_, counters_, flag_;
+    void prepare(size_t F, size_t npx);
+    void zero_chunk(size_t F);
+    uint32_t detect_chunk(const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio, uint64_t maxPoints);
+    FitParams fit_params(uint64_t fit, const double* fp) const;
+};
+
+} // namespace pt

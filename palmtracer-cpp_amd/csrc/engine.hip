@@ -1,0 +1,278 @@
+// Host orchestration of the device pipeline. Reference: lib/Palm.cpp, lib/localization/FrameProcessor.cpp.
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+namespace pt {
+
+namespace {
+__global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Fit jobs from caller rows [Id, Plane, X, Y, SegIntensity, Surface] (lib/Palm.cpp:105-109).
+__global__ void positions_to_rois(const double* __restrict__ pos, uint64_t n, uint64_t planes, Roi* __restrict__ rois,
+                                  uint32_t* __restrict__ bad)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const double* p = pos + i * kInCols;
+        Roi r;
+        r.x = p[kInX];
+        r.y = p[kInY];
+        r.seg = p[kInSegIntensity];
+        r.surf = p[kInSurface];
+        const double pl = p[kInPlane];
+        const bool ok = pl >= 1.0 && pl < (double)planes + 1.0;
+        r.frame = ok ? (uint32_t)((uint64_t)pl - 1) : 0u;
+        if (!ok) { r.x = -2.0; atomicOr(bad, 1u); }  // outside the ROI border test: sentinel row
+        r.slot = 0;
+        rois[i] = r;
+    }
+}
+
+__global__ void masked_labels(const uint32_t* __restrict__ fin, const uint32_t* __restrict__ P, uint32_t* __restrict__ out,
+                              size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = P[i] ? fin[i] : 0u;
+}
+
+__global__ void positions_ids(const double* __restrict__ pos, const uint32_t* __restrict__ local, uint64_t n,
+                              double* __restrict__ rows)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        rows[i * kCols + kId] = pos[i * kInCols + kInId];
+        rows[i * kCols + kPlane] = pos[i * kInCols + kInPlane];
+        rows[i * kCols + kIndex] = (double)local[i];
+    }
+}
+} // namespace
+
+void DevBuf::ensure(size_t n)
+{
+    if (n <= bytes) return;
+    release();
+    n = std::max<size_t>(n, 256);
+    PT_CHECK(hipMalloc(&p, n));
+    bytes = n;
+}
+
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+Engine& Engine::get()
+{
+    static Engine* e = new Engine();  // lives until process exit
+    return *e;
+}
+
+Engine::Engine()
+{
+    PT_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    if (const char* v = std::getenv("PALMTRACER_FIT_PRECISION")) opt.fitFp64 = std::strcmp(v, "fp32") != 0;
+    if (const char* v = std::getenv("PALMTRACER_MAX_ITER")) opt.maxIter = std::max(1, std::atoi(v));
+    if (const char* v = std::getenv("PALMTRACER_CHUNK_PIXELS")) opt.chunkPixels = std::max<size_t>(1, std::strtoull(v, nullptr, 10));
+}
+
+void Engine::prepare(size_t F, size_t npx)
+{
+    const size_t n = F * npx;
+    P_.ensure(n * 4);
+    lab_.ensure(n * 4);
+    fin_.ensure(n * 4);
+    aux_.ensure(n * sizeof(PixAux));
+    mask_.ensure(n * 4);
+    seed_.ensure(n * 4);
+    perFrame_.ensure((F + 1) * 4 * 9);
+    const size_t regionCap = npx / 4 + 2;  // roots are pairwise non-adjacent: at most ceil(H/2)*ceil(W/2)
+    regions_.ensure(F * regionCap * sizeof(Region));
+    const size_t multiCap = n / 64 + 1024;
+    multi_.ensure(multiCap * 8);
+    const size_t scratchWords = n / 8 + 65536;
+    scratch_.ensure(scratchWords * 8);
+    const size_t fbSlots = std::clamp<size_t>((size_t(1) << 26) / std::max<size_t>(npx, 1), 4, 64);
+    fb_.ensure(fbSlots * fb_slot_words(npx) * 4);
+    counters_.ensure(kNumCounters * 4);
+    flag_.ensure((F + 1) * 4);
+    uint32_t* pf = perFrame_.as<uint32_t>();
+    const size_t s = F + 1;
+    b_.P = P_.as<uint32_t>();
+    b_.lab = lab_.as<uint32_t>();
+    b_.fin = fin_.as<uint32_t>();
+    b_.aux = aux_.as<PixAux>();
+    b_.maskList = mask_.as<uint32_t>();
+    b_.seedList = seed_.as<uint32_t>();
+    b_.maskCnt = pf;
+    b_.seedCnt = pf + s;
+    b_.rootCnt = pf + 2 * s;
+    b_.keep = pf + 3 * s;
+    b_.found = pf + 4 * s;
+    b_.frameFlag = pf + 5 * s;
+    b_.roiBase = pf + 6 * s;
+    b_.regions = regions_.as<Region>();
+    b_.multiList = multi_.as<uint64_t>();
+    b_.scratch = scratch_.as<uint64_t>();
+    b_.fbScratch = fb_.as<uint32_t>();
+    b_.counters = counters_.as<uint32_t>();
+    b_.flagList = flag_.as<uint32_t>();
+    b_.frames = F;
+    b_.npx = npx;
+    b_.regionCap = regionCap;
+    b_.scratchWords = scratchWords;
+    b_.multiCap = multiCap;
+    b_.fbSlots = fbSlots;
+}
+
+void Engine::zero_chunk(size_t F)
+{
+    // Per-frame counters (6 arrays of F+1 words) and the chunk counters; a fill kernel rather than a
+    // memset so that the sequence can be captured in a graph.
+    fill_u32_kernel<<<64, 256, 0, stream>>>(b_.maskCnt, 6 * (F + 1), 0u);
+    fill_u32_kernel<<<1, 64, 0, stream>>>(b_.counters, kNumCounters, 0u);
+    PT_CHECK(hipGetLastError());
+}
+
+FitParams Engine::fit_params(uint64_t fit, const double* fp) const
+{
+    FitParams p{};
+    p.mode = (fit >= 1 && fit <= 4) ? (int)fit : 0;
+    long roi = 7;
+    if (fp) {
+        const double r = fp[0];
+        roi = (r == r && r >= 1 && r < 1e6) ? (long)r : 1;  // size_t conversion of fitParams[0]
+        p.sigmaX = fp[1];
+        p.sigmaY = fp[2];
+        p.theta = fp[3];
+    }
+    if (roi % 2 == 0) roi--;                                // Fitting.cpp:37
+    p.roi = (int)std::max<long>(roi, 1);
+    p.maxIter = opt.maxIter;
+    return p;
+}
+
+uint32_t Engine::detect_chunk(const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio, uint64_t maxPoints)
+{
+    const size_t npx = (size_t)W * H;
+    zero_chunk(F);
+    launch_atrous_detect(d_frames, b_, W, H, (int)F, threshold_to_pmin(thr), stream);
+    if ((int)ratio > 0) throw Error("watershed fusion (watershedRatio >= 1) is not available in this build");
+    rois.ensure(F * (npx / 4 + 2) * sizeof(Roi));
+    launch_segment(b_, W, H, (int)F, maxPoints, rois.as<Roi>(), stream);
+    uint32_t tail[2] = {0, 0};
+    PT_CHECK(hipMemcpyAsync(&tail[0], b_.roiBase + F, 4, hipMemcpyDeviceToHost, stream));
+    PT_CHECK(hipMemcpyAsync(&tail[1], b_.counters + kStatus, 4, hipMemcpyDeviceToHost, stream));
+    uint32_t ctr[kNumCounters];
+    PT_CHECK(hipMemcpyAsync(ctr, b_.counters, sizeof(ctr), hipMemcpyDeviceToHost, stream));
+    PT_CHECK(hipStreamSynchronize(stream));
+    last.flaggedFrames += ctr[kFlagCnt];
+    last.multiComponents += ctr[kMultiCnt];
+    last.status |= tail[1];
+    if (tail[1]) throw Error("segmentation status " + std::to_string(tail[1]));
+    return tail[0];
+}
+
+uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, double thr, double ratio, uint64_t fit,
+                          const double* fitParams, uint64_t maxPerPlane, double* d_out, uint64_t outCapRows)
+{
+    last = RunStats{};
+    if (planes == 0 || maxPerPlane == 0 || H <= 0 || W <= 0) return 0;
+    if (fit == 5) throw Error("fit mode 5 (spline PSF) is not part of this library");
+    const size_t npx = (size_t)W * H;
+    const size_t F = std::max<size_t>(1, std::min<size_t>(planes, std::min<size_t>(65535, opt.chunkPixels / npx)));
+    prepare(F, npx);
+    const FitParams fp = fit_params(fit, fitParams);
+    uint64_t rowsDone = 0;
+    for (uint64_t z0 = 0; z0 < planes; z0 += F) {
+        const size_t f = std::min<uint64_t>(F, planes - z0);
+        const uint16_t* frames = d_in + z0 * npx;
+        const uint32_t n = detect_chunk(frames, f, H, W, thr, ratio, maxPerPlane);
+        if (rowsDone + n > outCapRows) throw Error("output capacity exceeded");
+        rows.ensure((size_t)std::max<uint32_t>(n, 1) * kCols * 8);
+        if (n) {
+            launch_fit(frames, npx, W, H, rois.as<Roi>(), nullptr, n, fp, rows.as<double>(), opt.fitFp64, stream);
+            launch_assemble(rows.as<double>(), b_.roiBase, (int)f, rowsDone, z0, d_out + rowsDone * kCols, stream);
+        }
+        rowsDone += n;
+        last.chunks++;
+    }
+    PT_CHECK(hipStreamSynchronize(stream));
+    last.frames = planes;
+    last.rois = rowsDone;
+    return rowsDone * kCols;
+}
+
+uint64_t Engine::fit_positions(const uint16_t* d_in, uint64_t planes, int H, int W, const double* hostPos, uint64_t n,
+                               uint64_t fit, const double* fitParams, double* d_out)
+{
+    if (n == 0) return 0;
+    if (fit < 1 || fit > 4) throw Error("Fitting supports the Gaussian modes 1-4");
+    if (n > 0xFFFFFFFFull) throw Error("too many positions");
+    // Per-plane local index in input order (lib/Palm.cpp:79-84).
+    std::vector<uint32_t> local(n);
+    std::vector<uint64_t> counts(planes + 1, 0);
+    for (uint64_t i = 0; i < n; ++i) {
+        const double pl = hostPos[i * kInCols + kInPlane];
+        const uint64_t k = (pl >= 1.0 && pl < (double)planes + 1.0) ? (uint64_t)pl - 1 : planes;
+        local[i] = (uint32_t)++counts[k];
+    }
+    DevBuf dpos, dloc, dbad;
+    dpos.ensure(n * kInCols * 8);
+    dloc.ensure(n * 4);
+    dbad.ensure(4);
+    rois.ensure(n * sizeof(Roi));
+    PT_CHECK(hipMemcpyAsync(dpos.p, hostPos, n * kInCols * 8, hipMemcpyHostToDevice, stream));
+    PT_CHECK(hipMemcpyAsync(dloc.p, local.data(), n * 4, hipMemcpyHostToDevice, stream));
+    fill_u32_kernel<<<1, 64, 0, stream>>>(dbad.as<uint32_t>(), 1, 0u);
+    const unsigned g = std::min<uint64_t>(4096, (n + 255) / 256);
+    positions_to_rois<<<g, 256, 0, stream>>>(dpos.as<double>(), n, planes, rois.as<Roi>(), dbad.as<uint32_t>());
+    PT_CHECK(hipGetLastError());
+    const FitParams fp = fit_params(fit, fitParams);
+    launch_fit(d_in, (size_t)W * H, W, H, rois.as<Roi>(), nullptr, (uint32_t)n, fp, d_out, opt.fitFp64, stream);
+    positions_ids<<<g, 256, 0, stream>>>(dpos.as<double>(), dloc.as<uint32_t>(), n, d_out);
+    PT_CHECK(hipGetLastError());
+    uint32_t bad = 0;
+    PT_CHECK(hipMemcpyAsync(&bad, dbad.p, 4, hipMemcpyDeviceToHost, stream));
+    PT_CHECK(hipStreamSynchronize(stream));
+    if (bad) std::fprintf(stderr, "[WARNING] Fitting: positions with a plane outside [1, %llu] got failure rows.\n",
+                          (unsigned long long)planes);
+    dpos.release();
+    dloc.release();
+    dbad.release();
+    return n * kCols;
+}
+
+void Engine::wavelet(const uint16_t* d_in, uint64_t planes, int H, int W, int level, double* d_out)
+{
+    launch_atrous_plane(d_in, d_out, W, H, (int)planes, level, stream);
+    PT_CHECK(hipStreamSynchronize(stream));
+}
+
+void Engine::debug_labels(const uint16_t* d_frame, int H, int W, double thr, uint32_t* d_labels)
+{
+    const size_t npx = (size_t)W * H;
+    prepare(1, npx);
+    zero_chunk(1);
+    launch_atrous_detect(d_frame, b_, W, H, 1, threshold_to_pmin(thr), stream);
+    rois.ensure((npx / 4 + 2) * sizeof(Roi));
+    launch_segment(b_, W, H, 1, ~0ull, rois.as<Roi>(), stream);
+    // Labels only where P > 0 (fin is not written elsewhere).
+    masked_labels<<<256, 256, 0, stream>>>(b_.fin, b_.P, d_labels, npx);
+    PT_CHECK(hipGetLastError());
+    uint32_t ctr[kNumCounters];
+    PT_CHECK(hipMemcpyAsync(ctr, b_.counters, sizeof(ctr), hipMemcpyDeviceToHost, stream));
+    PT_CHECK(hipStreamSynchronize(stream));
+    last = RunStats{};
+    last.flaggedFrames = ctr[kFlagCnt];
+    last.multiComponents = ctr[kMultiCnt];
+    last.status = ctr[kStatus];
+}
+
+} // namespace pt

@@ -1,0 +1,175 @@
+// C ABI of the library: the reference's exported entry points (CPU/main.cpp, CPU/PALMTracer.def) with
+// their exact signatures, plus device-pointer and diagnostic entry points declared in
+// include/palmtracer.h. No exception crosses this boundary; failures print one line on stderr and
+// return 0 (or NaN for AutoThreshold), as the reference's callers expect from a failed call.
+#include "engine.h"
+#include "../../include/palmtracer.h"
+
+#include <cstdio>
+#include <dlfcn.h>
+
+using pt::Engine;
+
+namespace {
+
+template <typename F>
+auto guarded(const char* name, F&& f, decltype(f()) onError) -> decltype(f())
+{
+    try {
+        return f();
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[palmtracer] %s: %s\n", name, e.what());
+    } catch (...) {
+        std::fprintf(stderr, "[palmtracer] %s: unknown error\n", name);
+    }
+    return onError;
+}
+
+struct HostToDevice {
+    pt::DevBuf& buf;
+    const uint16_t* dptr = nullptr;
+    HostToDevice(pt::DevBuf& b, const uint16_t* host, size_t n, hipStream_t s) : buf(b)
+    {
+        buf.ensure(n * 2);
+        PT_CHECK(hipMemcpyAsync(buf.p, host, n * 2, hipMemcpyHostToDevice, s));
+        dptr = buf.as<uint16_t>();
+    }
+};
+
+} // namespace
+
+extern "C" {
+
+PT_API uint64_t PT_CALL Localization(const uint16_t* input, double* output, uint64_t outputSize, uint64_t height,
+                                     uint64_t width, uint64_t planes, double thresholdVal, double watershedRatio,
+                                     uint64_t fit, const double* fit_params)
+{
+    return guarded("Localization", [&]() -> uint64_t {
+        Engine& e = Engine::get();
+        std::lock_guard<std::mutex> lk(e.mu);
+        const uint64_t maxPer = planes == 0 ? 0 : (outputSize / pt::kCols) / planes;  // Palm.cpp:51
+        if (maxPer == 0 || height == 0 || width == 0) return 0;
+        const size_t npx = (size_t)height * width;
+        HostToDevice h2d(e.in, input, npx * planes, e.stream);
+        const uint64_t capRows = outputSize / pt::kCols;
+        const uint64_t rowsCap = std::min<uint64_t>(capRows, planes * std::min<uint64_t>(maxPer, npx / 4 + 2));
+        e.out.ensure(std::max<uint64_t>(rowsCap, 1) * pt::kCols * 8);
+        const uint64_t n = e.localize(h2d.dptr, planes, (int)height, (int)width, thresholdVal, watershedRatio, fit,
+                                      fit_params, maxPer, e.out.as<double>(), rowsCap);
+        if (n) PT_CHECK(hipMemcpyAsync(output, e.out.p, n * 8, hipMemcpyDeviceToHost, e.stream));
+        PT_CHECK(hipStreamSynchronize(e.stream));
+        return n;
+    }, 0);
+}
+
+PT_API uint64_t PT_CALL Fitting(const uint16_t* input, const double* positions, uint64_t nPoints, double* output,
+                                uint64_t outputSize, uint64_t height, uint64_t width, uint64_t planes, uint64_t fit,
+                                const double* fit_params)
+{
+    (void)outputSize;  // the reference ignores it too (lib/Palm.cpp:74)
+    return guarded("Fitting", [&]() -> uint64_t {
+        if (nPoints == 0) return 0;
+        Engine& e = Engine::get();
+        std::lock_guard<std::mutex> lk(e.mu);
+        const size_t npx = (size_t)height * width;
+        HostToDevice h2d(e.in, input, npx * planes, e.stream);
+        e.out.ensure(nPoints * pt::kCols * 8);
+        const uint64_t n = e.fit_positions(h2d.dptr, planes, (int)height, (int)width, positions, nPoints, fit,
+                                           fit_params, e.out.as<double>());
+        PT_CHECK(hipMemcpyAsync(output, e.out.p, n * 8, hipMemcpyDeviceToHost, e.stream));
+        PT_CHECK(hipStreamSynchronize(e.stream));
+        return n;
+    }, 0);
+}
+
+PT_API void PT_CALL Wavelett(const uint16_t* input, double* output, uint64_t height, uint64_t width, uint64_t planes,
+                             uint64_t levelOutput)
+{
+    guarded("Wavelett", [&]() -> int {
+        if (planes == 0 || height == 0 || width == 0) return 0;
+        if (levelOutput == 0) levelOutput = 1;                   // CPU/main.cpp:262
+        const int level = (int)std::min<uint64_t>(levelOutput - 1, 2);
+        Engine& e = Engine::get();
+        std::lock_guard<std::mutex> lk(e.mu);
+        const size_t npx = (size_t)height * width;
+        // Bounded device footprint: planes in chunks.
+        const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(planes, (size_t(256) << 20) / (npx * 10)));
+        for (uint64_t z0 = 0; z0 < planes; z0 += chunk) {
+            const uint64_t nz = std::min(chunk, planes - z0);
+            HostToDevice h2d(e.in, input + z0 * npx, nz * npx, e.stream);
+            e.out.ensure(nz * npx * 8);
+            e.wavelet(h2d.dptr, nz, (int)height, (int)width, level, e.out.as<double>());
+            PT_CHECK(hipMemcpyAsync(output + z0 * npx, e.out.p, nz * npx * 8, hipMemcpyDeviceToHost, e.stream));
+            PT_CHECK(hipStreamSynchronize(e.stream));
+        }
+        return 0;
+    }, 0);
+}
+
+// ----- device-pointer entry points (no counterpart in the reference) -----
+
+PT_API uint64_t PT_CALL pt_localization_device(const uint16_t* d_input, double* d_output, uint64_t outputCapDoubles,
+                                               uint64_t height, uint64_t width, uint64_t planes, double thresholdVal,
+                                               double watershedRatio, uint64_t fit, const double* fit_params,
+                                               uint64_t maxPerPlane)
+{
+    return guarded("pt_localization_device", [&]() -> uint64_t {
+        Engine& e = Engine::get();
+        std::lock_guard<std::mutex> lk(e.mu);
+        return e.localize(d_input, planes, (int)height, (int)width, thresholdVal, watershedRatio, fit, fit_params,
+                          maxPerPlane, d_output, outputCapDoubles / pt::kCols);
+    }, 0);
+}
+
+PT_API int PT_CALL pt_wavelet_device(const uint16_t* d_input, double* d_output, uint64_t height, uint64_t width,
+                                     uint64_t planes, uint64_t levelOutput)
+{
+    return guarded("pt_wavelet_device", [&]() -> int {
+        Engine& e = Engine::get();
+        std::lock_guard<std::mutex> lk(e.mu);
+        if (levelOutput == 0) levelOutput = 1;
+        e.wavelet(d_input, planes, (int)height, (int)width, (int)std::min<uint64_t>(levelOutput - 1, 2), d_output);
+        return 1;
+    }, 0);
+}
+
+PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint64_t width, double thresholdVal,
+                                   uint32_t* labels)
+{
+    return guarded("pt_debug_labels", [&]() -> int {
+        Engine& e = Engine::get();
+        std::lock_guard<std::mutex> lk(e.mu);
+        const size_t npx = (size_t)height * width;
+        HostToDevice h2d(e.in, image, npx, e.stream);
+        e.out.ensure(npx * 4);
+        e.debug_labels(h2d.dptr, (int)height, (int)width, thresholdVal, e.out.as<uint32_t>());
+        PT_CHECK(hipMemcpyAsync(labels, e.out.p, npx * 4, hipMemcpyDeviceToHost, e.stream));
+        PT_CHECK(hipStreamSynchronize(e.stream));
+        return 1;
+    }, 0);
+}
+
+PT_API int PT_CALL pt_set_option(const char* name, double value)
+{
+    return guarded("pt_set_option", [&]() -> int {
+        Engine& e = Engine::get();
+        std::lock_guard<std::mutex> lk(e.mu);
+        const std::string n(name ? name : "");
+        if (n == "fit_fp64") e.opt.fitFp64 = value != 0.0;
+        else if (n == "max_iter") e.opt.maxIter = std::max(1, (int)value);
+        else if (n == "chunk_pixels") e.opt.chunkPixels = (size_t)std::max(1.0, value);
+        else throw pt::Error("unknown option '" + n + "'");
+        return 1;
+    }, 0);
+}
+
+PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n)
+{
+    Engine& e = Engine::get();
+    const uint64_t v[7] = {e.last.frames, e.last.chunks, e.last.rois, e.last.flaggedFrames, e.last.multiComponents,
+                           e.last.maskPixels, e.last.status};
+    for (uint64_t i = 0; i < n && i < 7; ++i) out[i] = v[i];
+    return 7;
+}
+
+} // extern "C"

@@ -1,0 +1,464 @@
+// Per-ROI Gaussian Levenberg-Marquardt fit on gfx950.
+//
+// Reference: lib/fit/GaussianFit.cpp:48-295 (Fit, lm, gain, getError, computeR2),
+// lib/fit/GaussianModel.cpp:19-52 (Evaluate), lib/fit/Fitting.cpp:61-175 (fill, preCompute, initP),
+// lib/signal_processing/Matrix.cpp:297-355 (InverseGaussJordan).
+//
+// Schedule: one ROI per lane. A lane walks its ROI's pixels in the reference's raster order, so
+// every sum is accumulated in the reference's order; with T = double the only differences from the
+// reference are the device exp/sin/cos. With T = float the model, its Jacobian, alpha, beta and the
+// squared error are evaluated and accumulated in FP32, while parameters, the damped normal equations
+// (Gauss-Jordan with row equilibration and partial pivoting, as the reference), the gain ratio and
+// R^2 stay in FP64. The control flow (lambda schedule, clamps, stopping rules, sentinels) is the
+// reference's, step for step.
+#include "pt_common.h"
+#include "pt_kernels.h"
+
+namespace pt {
+namespace {
+
+constexpr double kSqrt2 = 1.41421356237309504880;
+constexpr double kPi = 3.14159265358979323846;
+
+template <typename T> __device__ inline T t_exp(T v);
+template <> __device__ inline float t_exp<float>(float v) { return expf(v); }
+template <> __device__ inline double t_exp<double>(double v) { return exp(v); }
+
+template <int MODE> struct NPar { static constexpr int v = MODE == 1 ? 4 : MODE == 2 ? 5 : MODE == 3 ? 6 : 7; };
+
+// Double to size_t as the reference's x86-64 build converts (truncation; wraps for negatives;
+// 2^63 for NaN and out-of-range values).
+__device__ inline uint64_t to_size(double v)
+{
+    if (!(v == v) || v >= 9223372036854775808.0 || v <= -9223372036854775808.0) return 1ull << 63;
+    return (uint64_t)(long long)v;
+}
+
+// GaussianModel: isotropic (modes 1, 2) or rotated anisotropic (modes 3, 4).
+template <typename T, int MODE>
+struct Model {
+    T off, amp, sx, sy, sx2, sy2, c, s, c2, s2, cs;
+    __device__ Model(const double* p, double sigma, double theta)
+    {
+        off = (T)p[0];
+        amp = (T)p[1];
+        if constexpr (MODE <= 2) {
+            sx = sy = (T)(MODE == 1 ? sigma : p[MODE == 1 ? 0 : 4]);
+            sx2 = sy2 = sx * sx;
+            c = 1; s = 0; c2 = 1; s2 = 0; cs = 0;
+        } else {
+            const double th = MODE == 3 ? theta : p[MODE == 3 ? 0 : 6];
+            sx = (T)p[4];
+            sy = (T)p[5];
+            sx2 = sx * sx;
+            sy2 = sy * sy;
+            c = (T)cos(th);
+            s = (T)sin(th);
+            c2 = c * c; s2 = s * s; cs = c * s;
+        }
+    }
+    __device__ T eval(T x, T y, T* g) const
+    {
+        const T X2 = x * x, Y2 = y * y;
+        if constexpr (MODE <= 2) {
+            const T E = t_exp<T>(-(X2 + Y2) / (T(2) * sx2));
+            const T P = amp * E;
+            if (g) {
+                g[0] = 1;
+                g[1] = E;
+                g[2] = P * x / sx2;
+                g[3] = P * y / sx2;
+                if constexpr (MODE == 2) g[4] = P / (sx * sx2) * (X2 + Y2);
+            }
+            return P + off;
+        } else {
+        const T XC = x * c, XS = x * s, YC = y * c, YS = y * s;
+        const T F1 = (XC - YS) / sx, F2 = (XS + YC) / sy;
+        const T E = t_exp<T>(-((F1 * F1) + (F2 * F2)));
+        const T P = amp * E;
+        if (g) {
+            const T XY = x * y, F3 = T(2) * XY * cs, F4 = (T(2) * XY / sx2) - (T(2) * XY / sy2);
+            g[0] = 1;
+            g[1] = E;
+            g[2] = P * (-(((T(-2) * c2 * x) + (T(2) * y * cs)) / sx2) - (((T(-2) * s2 * x) - (T(2) * y * cs)) / sy2));
+            g[3] = P * ((((T(2) * s2 * y) - (T(2) * x * cs)) / sx2) - (((T(-2) * c2 * y) - (T(2) * x * cs)) / sy2));
+            g[4] = P * (T(2) * ((XC * XC) - F3 + (YS * YS)) / (sx * sx2));
+            g[5] = P * (T(2) * ((XS * XS) + F3 + (YC * YC)) / (sy * sy2));
+            if constexpr (MODE == 4) g[6] = P * ((T(2) * cs * (((X2 - Y2) / sx2) + ((Y2 - X2) / sy2))) - (s2 * F4) + (c2 * F4));
+        }
+        return P + off;
+        }
+    }
+};
+
+// InverseGaussJordan (Matrix.cpp:297-355) on an NP x NP matrix in registers. Returns false on the
+// reference's domain_error.
+template <int N>
+__device__ inline bool gauss_jordan(double (&a)[N][N], double (&o)[N][N])
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) o[i][j] = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double scale = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            if (!isfinite(a[i][j])) return false;
+            const double v = fabs(a[i][j]);
+            if (scale < v) scale = v;
+        }
+        if (scale == 0.0) return false;
+#pragma unroll
+        for (int j = 0; j < N; ++j) a[i][j] /= scale;
+        o[i][i] = 1.0 / scale;
+    }
+    const double tol = 2.220446049250313e-16 * (double)N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        int pr = i;
+#pragma unroll
+        for (int j = i + 1; j < N; ++j)
+            if (fabs(a[j][i]) > fabs(a[pr][i])) pr = j;
+        // Bring the pivot row to row i (register arrays: select by index, no dynamic addressing).
+        double piv = 0.0;
+#pragma unroll
+        for (int j = i; j < N; ++j) if (j == pr) piv = a[j][i];
+        if (!isfinite(piv) || fabs(piv) <= tol) return false;
+        if (pr != i) {
+#pragma unroll
+            for (int j = i + 1; j < N; ++j) {
+                if (j != pr) continue;
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    double t = a[i][k]; a[i][k] = a[j][k]; a[j][k] = t;
+                    t = o[i][k]; o[i][k] = o[j][k]; o[j][k] = t;
+                }
+            }
+        }
+        const double pinv = 1.0 / a[i][i];
+#pragma unroll
+        for (int k = 0; k < N; ++k) { a[i][k] *= pinv; o[i][k] *= pinv; }
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            if (j == i) continue;
+            const double fct = a[j][i];
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                a[j][k] -= fct * a[i][k];
+                o[j][k] -= fct * o[i][k];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            if (!isfinite(a[i][j]) || !isfinite(o[i][j])) return false;
+    return true;
+}
+
+// Fitting::fill (Fitting.cpp:61-106).
+__device__ inline void fill_row(int mode, double* pt, double x, double y, double z, double seg, double surf, double offset,
+                                double intensity, double sx, double sy, double theta, double rssXY, double rssZ)
+{
+    pt[kId] = 0; pt[kPlane] = 0; pt[kIndex] = 0; pt[kChannel] = -1;
+    pt[kX] = x; pt[kY] = y; pt[kZ] = z;
+    if (mode == 0) pt[kIntensity] = seg;
+    else pt[kIntensity] = (intensity > kEpsilon && sx > kEpsilon && sy > kEpsilon) ? 2.0 * kPi * intensity * sx * sy : -1;
+    pt[kSigmaX] = sx; pt[kSigmaY] = sy; pt[kTheta] = theta;
+    pt[kQualityXY] = rssXY; pt[kQualityZ] = rssZ;
+    pt[kAmplitude] = intensity; pt[kOffset] = offset;
+    pt[kSegIntensity] = seg; pt[kSurface] = surf;
+    pt[kCircularity] = (sx > kEpsilon && sy > kEpsilon) ? (sy < sx ? sy : sx) / (sx < sy ? sy : sx) : 1;
+}
+
+struct Patch {
+    const uint16_t* img;
+    int W, half, roi;
+    uint64_t ux, uy;
+    __device__ double v(int i, int j) const { return (double)img[(uy + i) * (uint64_t)W + (ux + j)]; }
+};
+
+// GaussianFit::lm: model, residuals, alpha (upper triangle), beta and chi^2 at p.
+template <typename T, int MODE>
+__device__ inline T lm_pass(const Patch& pa, const double* p, double sigma, double theta, T (&al)[NPar<MODE>::v][NPar<MODE>::v],
+                            T (&be)[NPar<MODE>::v])
+{
+    constexpr int NP = NPar<MODE>::v;
+#pragma unroll
+    for (int a = 0; a < NP; ++a) {
+        be[a] = 0;
+#pragma unroll
+        for (int b = 0; b < NP; ++b) al[a][b] = 0;
+    }
+    T err = 0;
+    const Model<T, MODE> m(p, sigma, theta);
+    bool circular = false;
+    if constexpr (MODE == 4) circular = fabs(p[4] - p[5]) <= 2.220446049250313e-16;
+    const T px = (T)p[2], py = (T)p[3];
+    for (int i = -pa.half; i <= pa.half; ++i) {
+        for (int j = -pa.half; j <= pa.half; ++j) {
+            T g[NP];
+            const T X = (T)(double)(pa.ux + j), Y = (T)(double)(pa.uy + i);
+            const T hat = m.eval(X - px, Y - py, g);
+            const T d = (T)pa.v(i, j) - hat;
+            err += d * d;
+            if constexpr (MODE == 4) if (circular) g[NP - 1] = 0;
+#pragma unroll
+            for (int a = 0; a < NP; ++a) {
+                be[a] += g[a] * d;
+#pragma unroll
+                for (int b = a; b < NP; ++b) al[a][b] += g[a] * g[b];
+            }
+        }
+    }
+    return err;
+}
+
+// GaussianFit::getError at pTry.
+template <typename T, int MODE>
+__device__ inline T lm_error(const Patch& pa, const double* p, double sigma, double theta)
+{
+    const Model<T, MODE> m(p, sigma, theta);
+    const T px = (T)p[2], py = (T)p[3];
+    T res = 0;
+    for (int i = -pa.half; i <= pa.half; ++i)
+        for (int j = -pa.half; j <= pa.half; ++j) {
+            const T X = (T)(double)(pa.ux + j), Y = (T)(double)(pa.uy + i);
+            const T dx = (T)pa.v(i, j) - m.eval(X - px, Y - py, nullptr);
+            res += dx * dx;
+        }
+    return res;
+}
+
+// GaussianFit::computeR2 at p (squared Pearson correlation of data and model; NaN -> 0).
+template <typename T, int MODE>
+__device__ inline double lm_r2(const Patch& pa, const double* p, double sigma, double theta)
+{
+    const Model<T, MODE> m(p, sigma, theta);
+    const T px = (T)p[2], py = (T)p[3];
+    const double n = (double)(pa.roi * pa.roi);
+    double sd = 0, sm = 0;
+    for (int i = -pa.half; i <= pa.half; ++i)
+        for (int j = -pa.half; j <= pa.half; ++j) {
+            const T X = (T)(double)(pa.ux + j), Y = (T)(double)(pa.uy + i);
+            sd = sd + pa.v(i, j);
+            sm = sm + (double)m.eval(X - px, Y - py, nullptr);
+        }
+    const double muD = sd / n, muM = sm / n;
+    double t1 = 0, t2 = 0, t3 = 0;
+    for (int i = -pa.half; i <= pa.half; ++i)
+        for (int j = -pa.half; j <= pa.half; ++j) {
+            const T X = (T)(double)(pa.ux + j), Y = (T)(double)(pa.uy + i);
+            const double dd = pa.v(i, j) - muD;
+            const double hh = (double)m.eval(X - px, Y - py, nullptr) - muM;
+            t1 = t1 + dd * hh;
+            t2 = t2 + dd * dd;
+            t3 = t3 + hh * hh;
+        }
+    const double r = t1 / sqrt(t2 * t3);
+    return isnan(r) ? 0.0 : r * r;
+}
+
+template <typename T, int MODE>
+__device__ void fit_one(const uint16_t* img, int W, int H, const Roi& r, const FitParams& fp, double* row, uint32_t* iters)
+{
+    constexpr int NP = NPar<MODE>::v;
+    const int roi = fp.roi, half = roi / 2;
+    const double zero = 0.0;
+    const uint64_t ux = to_size(r.x), uy = to_size(r.y);
+    const uint64_t h64 = (uint64_t)half;
+    // Border check (GaussianFit.cpp:51-56), with the reference's unsigned arithmetic.
+    if (h64 > ux || ux >= (uint64_t)W - h64 || h64 > uy || uy >= (uint64_t)H - h64) {
+        fill_row(MODE, row, r.x, r.y, zero, r.seg, r.surf, -1, -1, -1, -1, -1, -1, -1);
+        return;
+    }
+    const Patch pa{img, W, half, roi, ux, uy};
+    double minI = 1.79769313486231570815e308, maxI = -1.79769313486231570815e308;
+    for (int i = -half; i <= half; ++i)
+        for (int j = -half; j <= half; ++j) {
+            const double v = pa.v(i, j);
+            if (v < minI) minI = v;
+            if (maxI < v) maxI = v;
+        }
+    // initP (Fitting.cpp:126-175)
+    double p[NP], lo[NP], hi[NP], pTry[NP], dP[NP];
+    p[0] = minI; p[1] = maxI - minI; p[2] = r.x; p[3] = r.y;
+    lo[0] = 0; lo[1] = 0; lo[2] = r.x - 1; lo[3] = r.y - 1;
+    hi[0] = 65535.0; hi[1] = 65535.0; hi[2] = r.x + 1; hi[3] = r.y + 1;
+    if constexpr (MODE == 2) { p[4] = fp.sigmaX; lo[4] = 0.1; hi[4] = 20.0; }
+    if constexpr (MODE >= 3) {
+        p[4] = kSqrt2 * fp.sigmaX; p[5] = kSqrt2 * fp.sigmaY;
+        lo[4] = lo[5] = 0.1; hi[4] = hi[5] = 20.0;
+    }
+    if constexpr (MODE == 4) { p[6] = fp.theta; lo[6] = -150; hi[6] = 150; }
+
+    T al[NP][NP], be[NP];
+    double err = (double)lm_pass<T, MODE>(pa, p, fp.sigmaX, fp.theta, al, be);
+    bool bad = !isfinite(err);
+#pragma unroll
+    for (int a = 0; a < NP; ++a) bad |= !isfinite((double)be[a]);
+    if (bad) {
+        fill_row(MODE, row, r.x, r.y, zero, r.seg, r.surf, -1, -1, -1, -1, -1, -1, -1);
+        return;
+    }
+    auto max_abs_beta = [&]() {
+        double m = fabs((double)be[0]);
+#pragma unroll
+        for (int a = 1; a < NP; ++a) { const double v = fabs((double)be[a]); if (m < v) m = v; }
+        return m;
+    };
+    auto rel_step = [&]() {
+        double m = 0.0;
+#pragma unroll
+        for (int a = 0; a < NP; ++a) {
+            const double ap = fabs(p[a]);
+            const double q = fabs(dP[a]) / (ap < 1e-12 ? 1e-12 : ap);
+            if (m < q) m = q;
+        }
+        return m;
+    };
+    double lambda = 1e-2;
+    double m = max_abs_beta();
+    uint32_t nIt = 0;
+    for (int it = (m < 1e-6) ? fp.maxIter : 0; it < fp.maxIter; ++it) {
+        ++nIt;
+        double A[NP][NP], Ai[NP][NP];
+#pragma unroll
+        for (int a = 0; a < NP; ++a)
+#pragma unroll
+            for (int b = 0; b < NP; ++b) A[a][b] = (double)(a <= b ? al[a][b] : al[b][a]);
+#pragma unroll
+        for (int a = 0; a < NP; ++a) A[a][a] = A[a][a] == 0.0 ? 1.0 : (1.0 + lambda) * A[a][a];
+        if (!gauss_jordan<NP>(A, Ai)) {
+            if (lambda < 1e7) { lambda = fmin(lambda * 11, 1e7); continue; }
+            fill_row(MODE, row, r.x, r.y, zero, r.seg, r.surf, -1, -1, -1, -1, -1, -1, -1);
+            if (iters) *iters = nIt;
+            return;
+        }
+        bool nonfinite = false;
+#pragma unroll
+        for (int a = 0; a < NP; ++a) {
+            double d = 0.0;
+#pragma unroll
+            for (int b = 0; b < NP; ++b) d += Ai[a][b] * (double)be[b];
+            dP[a] = d;
+            nonfinite |= !isfinite(d);
+            const double v = p[a] + d;
+            pTry[a] = v < lo[a] ? lo[a] : (hi[a] < v ? hi[a] : v);
+        }
+        if (nonfinite) {
+            fill_row(MODE, row, r.x, r.y, zero, r.seg, r.surf, -1, -1, -1, -1, -1, -1, -1);
+            if (iters) *iters = nIt;
+            return;
+        }
+        bool tiny = true;
+#pragma unroll
+        for (int a = 0; a < NP; ++a) { dP[a] = pTry[a] - p[a]; tiny &= fabs(dP[a]) < 1e-15; }
+        if (tiny) {
+            if (lambda >= 1e7) break;
+            lambda = fmin(lambda * 11, 1e7);
+            continue;
+        }
+        // gain = 2 dP.beta - dP.alpha.dP (GaussianFit.cpp:248-259)
+        double quad = 0.0, lin = 0.0;
+#pragma unroll
+        for (int a = 0; a < NP; ++a) {
+            double ti = 0.0;
+#pragma unroll
+            for (int b = 0; b < NP; ++b) ti += (double)(a <= b ? al[a][b] : al[b][a]) * dP[b];
+            quad += dP[a] * ti;
+            lin += dP[a] * (double)be[a];
+        }
+        const double gain = (2.0 * lin) - quad;
+        if (gain <= 0.0) {
+            if (lambda >= 1e7) break;
+            lambda = fmin(lambda * 11, 1e7);
+            m = rel_step();
+            if (m < 1e-6 && it >= 2) break;
+            continue;
+        }
+        m = 0.0;
+#pragma unroll
+        for (int a = 0; a < NP; ++a) m = m + dP[a] * ((lambda * dP[a]) + (double)be[a]);
+        const double errTry = (double)lm_error<T, MODE>(pa, pTry, fp.sigmaX, fp.theta);
+        const double rho = (err - errTry) / (2.0 * m);
+        if (rho > 1e-2) {
+#pragma unroll
+            for (int a = 0; a < NP; ++a) p[a] = pTry[a];
+            err = (double)lm_pass<T, MODE>(pa, p, fp.sigmaX, fp.theta, al, be);
+            lambda = fmax(lambda / 9, 1e-7);
+        } else {
+            if (lambda >= 1e7) break;
+            lambda = fmin(lambda * 11, 1e7);
+        }
+        m = rel_step();
+        if (it >= 2) {
+            if (m < 1e-6) break;
+            if (err / (double)(roi * roi) < 1e-5) break;
+            if (max_abs_beta() < 1e-6) break;
+        }
+    }
+    if (iters) *iters = nIt;
+    const double off = p[0], inten = p[1], x = p[2], y = p[3];
+    const double rss = lm_r2<T, MODE>(pa, p, fp.sigmaX, fp.theta);
+    if (rss < kEpsilon) {
+        fill_row(MODE, row, x, y, zero, r.seg, r.surf, -1, -1, -1, -1, -1, -1, -1);
+        return;
+    }
+    double sx = fp.sigmaX, sy = fp.sigmaY, th = fp.theta;
+    if constexpr (MODE == 1) { sx = sy = fp.sigmaX; th = 0; }
+    if constexpr (MODE == 2) { sx = sy = p[4]; th = 0; }
+    if constexpr (MODE >= 3) { sx = p[4] / kSqrt2; sy = p[5] / kSqrt2; }
+    if constexpr (MODE == 4) th = p[6];
+    fill_row(MODE, row, x, y, zero, r.seg, r.surf, off, inten, sx, sy, th, rss, -1);
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(64) fit_kernel(const uint16_t* __restrict__ frames, size_t npx, int W, int H,
+                                                 const Roi* __restrict__ rois, const uint32_t* __restrict__ count,
+                                                 uint32_t nHost, FitParams fp, double* __restrict__ rows,
+                                                 uint32_t* __restrict__ iters)
+{
+    const uint32_t n = count ? *count : nHost;
+    for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i < n; i += gridDim.x * 64) {
+        const Roi r = rois[i];
+        double row[kCols];
+        if constexpr (MODE == 0) {
+            fill_row(0, row, r.x, r.y, 0.0, r.seg, r.surf, 0, 0, -1, -1, -1, -1, -1);
+        } else {
+            fit_one<T, MODE == 0 ? 1 : MODE>(frames + (size_t)r.frame * npx, W, H, r, fp, row, iters ? iters + i : nullptr);
+        }
+        double* o = rows + (size_t)i * kCols;
+#pragma unroll
+        for (int c = 0; c < kCols; ++c) o[c] = row[c];
+    }
+}
+
+template <typename T>
+void launch_fit_t(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, const uint32_t* d_count,
+                  uint32_t nHost, const FitParams& fp, double* d_rows, uint32_t* d_iters, hipStream_t s)
+{
+    const unsigned grid = d_count ? 4096u : std::max(1u, std::min(65535u, ceil_div(nHost, 64)));
+    switch (fp.mode) {
+    case 1: fit_kernel<T, 1><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_iters); break;
+    case 2: fit_kernel<T, 2><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_iters); break;
+    case 3: fit_kernel<T, 3><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_iters); break;
+    case 4: fit_kernel<T, 4><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_iters); break;
+    default: fit_kernel<T, 0><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_iters); break;
+    }
+    PT_CHECK(hipGetLastError());
+}
+
+} // namespace
+
+void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, const uint32_t* d_count,
+                uint32_t nHost, const FitParams& fp, double* d_rows, bool fp64, hipStream_t s, uint32_t* d_iters)
+{
+    if (fp64) launch_fit_t<double>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_iters, s);
+    else launch_fit_t<float>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_iters, s);
+}
+
+} // namespace pt

@@ -1,0 +1,90 @@
+// Device buffers and kernel launchers of the detection + fitting path.
+#pragma once
+#include "pt_common.h"
+
+namespace pt {
+
+// Per-region accumulator, one slot per root seed of a frame (exact integer sums of P = 65536*W1).
+struct Region {
+    uint32_t root;     // raster index of the region's first local maximum (its label identity)
+    uint32_t surf;     // pixel count
+    uint64_t sP;       // sum P
+    uint64_t sPx;      // sum P * column
+    uint64_t sPy;      // sum P * row
+};
+
+// One fit job: the segmentation info the fit reads (SegmentationInfo.hpp:32-71) and its frame.
+struct Roi {
+    double x, y, seg, surf;
+    uint32_t frame;
+    uint32_t slot;     // output row position within the frame (before the (Y, X) sort)
+};
+
+// Per-pixel side record, used sparsely: union-find component data at component roots and the
+// region slot at root seeds.
+struct PixAux {
+    uint32_t cmin, cmax;  // smallest / largest seed label in the component (labels = root seed + 1)
+    uint32_t csize;       // component pixel count
+    uint32_t rslot;       // region slot of a root seed
+};
+
+// Buffers of one chunk of F frames of npx pixels. Per-pixel arrays are F*npx long; per-frame
+// lists live in frame segments of capacity npx.
+struct DetectBuffers {
+    uint32_t* P = nullptr;        // 65536 * W1 where above threshold, else 0 (dense)
+    uint32_t* lab = nullptr;      // union-find parent / component root
+    uint32_t* fin = nullptr;      // watershed label (root seed index + 1)
+    PixAux* aux = nullptr;
+    uint32_t* maskList = nullptr; // per frame: pixels with P > 0
+    uint32_t* seedList = nullptr; // per frame: local maxima
+    uint32_t* maskCnt = nullptr;  // [F]
+    uint32_t* seedCnt = nullptr;  // [F]
+    uint32_t* rootCnt = nullptr;  // [F]
+    uint32_t* keep = nullptr;     // [F] regions kept for fitting
+    uint32_t* found = nullptr;    // [F] regions with surface > VOLUME_MIN (before truncation)
+    uint32_t* roiBase = nullptr;  // [F+1] exclusive scan of keep
+    uint32_t* frameFlag = nullptr;// [F] frame needs the exact sequential watershed
+    Region* regions = nullptr;    // F * regionCap
+    uint64_t* multiList = nullptr;// (frame << 32 | component root) of multi-label components
+    uint64_t* scratch = nullptr;  // flood work space (u64 words)
+    uint32_t* fbScratch = nullptr;// exact-fallback work space: fbSlots * fb_slot_words(npx) u32
+    uint32_t* counters = nullptr; // [kNumCounters], see Counter
+    uint32_t* flagList = nullptr; // [F]
+    size_t frames = 0, npx = 0, regionCap = 0, scratchWords = 0, multiCap = 0, fbSlots = 0;
+};
+
+// Device counters of one chunk (u32 words; kScratchUsed is a 64-bit counter on words 2-3).
+enum Counter : int { kMultiCnt = 0, kStatus = 1, kScratchUsed = 2, kFlagCnt = 4, kFbNext = 5, kNumCounters = 8 };
+// kStatus bits: 1 = region slots exhausted, 2 = unlabelled mask pixel (internal error).
+
+__host__ __device__ inline size_t fb_slot_words(size_t npx) { return 5 * ((npx + 1) & ~size_t(1)); }
+
+// A trous
+void launch_atrous_plane(const uint16_t* d_in, double* d_out, int W, int H, int planes, int level, hipStream_t s);
+void launch_atrous_detect(const uint16_t* d_in, const DetectBuffers& b, int W, int H, int frames, int64_t tmin,
+                          hipStream_t s);
+int64_t threshold_to_pmin(double thr);
+
+// Segmentation (ratio <= 0: watershed without fusion). Produces ROIs for frames [0, F) at
+// d_rois + roiBase[f], with per-frame truncation to maxPoints in label order.
+void launch_segment(const DetectBuffers& b, int W, int H, int frames, uint64_t maxPoints, Roi* d_rois,
+                    hipStream_t s);
+
+// Gaussian LM fit of nRoi ROIs (count read from device memory *d_count when non-null).
+struct FitParams {
+    int mode;        // 1..4 (0: no fit, rows carry the segmentation only)
+    int roi;         // odd ROI side
+    int maxIter;     // Fitting::N_ITER (75 in the reference)
+    double sigmaX, sigmaY, theta;
+};
+void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, const uint32_t* d_count,
+                uint32_t nRoiHost, const FitParams& fp, double* d_rows, bool fp64, hipStream_t s,
+                uint32_t* d_iters = nullptr);
+
+// Per-frame (Y, X) sort and row assembly into the caller's layout.
+void launch_assemble(const double* d_rows, const uint32_t* d_roiBase, int frames, uint64_t idBase, uint64_t planeBase,
+                     double* d_out, hipStream_t s);
+
+void launch_exclusive_scan(const uint32_t* d_in, uint32_t* d_out, int n, hipStream_t s);
+
+} // namespace pt

@@ -1,0 +1,642 @@
+// Segmentation2D on gfx950: local maxima, watershed labels, region statistics, ROI list.
+//
+// Reference: lib/localization/Segmentation2D.cpp (getLocalMax :294-333, watershed :336-385,
+// ComputeStatistics :144-175, FilterData :88-141), lib/fit/Fitting.cpp:44-58 (per-frame capacity).
+//
+// The reference floods every frame from one std::priority_queue whose keys age with a global pop
+// counter (key = -W1 + it * 1e-7, label claimed at push time). Labels can only differ between
+// pixels of one 8-connected component of the mask, so the frame is split:
+//   1. connected components of the mask by lock-free union-find (root = smallest raster index);
+//   2. every local maximum gets its getLocalMax label, identified here by the raster index of the
+//      first maximum of its plateau chain (+1), which orders labels exactly as the reference does;
+//   3. a component whose maxima all share one label is labelled wholesale;
+//   4. a component with several labels is flooded by one thread with a private heap that follows
+//      libstdc++'s push/pop sequence, using the component-local pop count as the age. At every
+//      pop the thread proves that the popped entry precedes every other entry of the component in
+//      the frame-global order too, whatever the other components' pops did to the counter
+//      (the global gap between two pushes is at least the local gap and at most the frame's mask
+//      size). If the proof fails (near-equal potentials, exact ties), the frame is flagged;
+//   5. flagged frames are re-flooded by an exact sequential restatement of the reference
+//      (raster-order seeds, one frame-wide heap, global counter).
+// Statistics are exact integer sums of P = 65536 * W1, so they equal the reference's FP64
+// raster-order sums whenever those are exact (sums below 2^53 ulps); larger sums are recomputed in
+// raster order in FP64.
+#include "pt_common.h"
+#include "pt_kernels.h"
+
+namespace pt {
+namespace {
+
+constexpr uint32_t kVisited = 0x80000000u;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int NT = 256;
+
+__device__ inline uint32_t ld_relaxed(const uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline bool is_local_max(const uint32_t* Pf, int W, int H, int idx)
+{
+    const uint32_t v = Pf[idx];
+    if (v == 0) return false;
+    const int y = idx / W, x = idx - y * W;
+    for (int dy = -1; dy <= 1; ++dy) {
+        const int yy = y + dy;
+        if (yy < 0 || yy >= H) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int xx = x + dx;
+            if ((dx | dy) == 0 || xx < 0 || xx >= W) continue;
+            if (v < Pf[yy * W + xx]) return false;
+        }
+    }
+    return true;
+}
+
+// A pixel that the watershed would seed: labelled with some 8-neighbour (inside the frame) that is
+// unlabelled (Segmentation2D.cpp:340-359). Unlabelled = below threshold or label 0.
+__device__ inline bool has_unlabelled_nbr(const uint32_t* Pf, const uint32_t* finf, int W, int H, int idx)
+{
+    const int y = idx / W, x = idx - y * W;
+    for (int dy = -1; dy <= 1; ++dy) {
+        const int yy = y + dy;
+        if (yy < 0 || yy >= H) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int xx = x + dx;
+            if ((dx | dy) == 0 || xx < 0 || xx >= W) continue;
+            const int q = yy * W + xx;
+            if (Pf[q] == 0 || finf[q] == 0) return true;
+        }
+    }
+    return false;
+}
+
+__device__ inline uint32_t uf_find(const uint32_t* lab, uint32_t x)
+{
+    uint32_t p = ld_relaxed(lab + x);
+    while (p != x) {
+        x = p;
+        p = ld_relaxed(lab + x);
+    }
+    return x;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Heap with the move sequence of libstdc++'s std::priority_queue<Pixel> (bits/stl_heap.h), whose
+// Pixel::operator< is reversed (lib/Structs.hpp:26): the top holds the smallest key.
+struct DHeap {
+    double* k;
+    uint64_t* v;
+    uint32_t n;
+
+    __device__ void sift_up(uint32_t hole, double key, uint64_t val)
+    {
+        while (hole > 0) {
+            const uint32_t parent = (hole - 1) / 2;
+            if (!(k[parent] > key)) break;
+            k[hole] = k[parent];
+            v[hole] = v[parent];
+            hole = parent;
+        }
+        k[hole] = key;
+        v[hole] = val;
+    }
+    __device__ void push(double key, uint64_t val) { sift_up(n++, key, val); }
+    __device__ uint64_t pop()
+    {
+        const uint64_t top = v[0];
+        if (n > 1) {
+            const uint32_t len = n - 1;
+            const double vk = k[len];
+            const uint64_t vv = v[len];
+            uint32_t hole = 0, second = 0;
+            while (second < (len - 1) / 2) {
+                second = 2 * (second + 1);
+                if (k[second] > k[second - 1]) second--;
+                k[hole] = k[second];
+                v[hole] = v[second];
+                hole = second;
+            }
+            if ((len & 1) == 0 && second == (len - 2) / 2) {
+                second = 2 * (second + 1);
+                k[hole] = k[second - 1];
+                v[hole] = v[second - 1];
+                hole = second - 1;
+            }
+            sift_up(hole, vk, vv);
+        }
+        n--;
+        return top;
+    }
+};
+
+// Reference key of a push at pop count `it`: -output + it * EPSILON, rounded per operation
+// (no fused multiply-add), Segmentation2D.cpp:378.
+__device__ inline double age_key(uint32_t p, uint32_t it)
+{
+    const double o = (double)p * kPScale;
+    return it == 0 ? -o : __dadd_rn(-o, __dmul_rn((double)it, kEpsilon));
+}
+
+__device__ inline void flag_frame(uint32_t* frameFlag, uint32_t* counters, uint32_t* flagList, int f)
+{
+    if (atomicExch(&frameFlag[f], 1u) == 0u) {
+        const uint32_t k = atomicAdd(&counters[kFlagCnt], 1u);
+        flagList[k] = (uint32_t)f;   // k < F: each frame is listed once
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(NT) cc_union_kernel(uint32_t* __restrict__ lab, const uint32_t* __restrict__ P,
+                                                      const uint32_t* __restrict__ maskList, const uint32_t* __restrict__ maskCnt,
+                                                      PixAux* __restrict__ aux, int W, int H, size_t npx)
+{
+    const int f = blockIdx.y;
+    const uint32_t cnt = maskCnt[f];
+    uint32_t* lb = lab + (size_t)f * npx;
+    const uint32_t* Pf = P + (size_t)f * npx;
+    for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < cnt; k += gridDim.x * NT) {
+        const uint32_t p = maskList[(size_t)f * npx + k];
+        aux[(size_t)f * npx + p] = PixAux{kNone, 0u, 0u, kNone};
+        const int y = p / W, x = p - y * W;
+        // Earlier 8-neighbours: W, NW, N, NE.
+        const int dys[4] = {0, -1, -1, -1}, dxs[4] = {-1, -1, 0, 1};
+        for (int j = 0; j < 4; ++j) {
+            const int yy = y + dys[j], xx = x + dxs[j];
+            if (yy < 0 || xx < 0 || xx >= W) continue;
+            const uint32_t q = (uint32_t)(yy * W + xx);
+            if (Pf[q] == 0) continue;
+            uint32_t a = p, b = q;
+            while (true) {
+                a = uf_find(lb, a);
+                b = uf_find(lb, b);
+                if (a == b) break;
+                if (a > b) { const uint32_t t = a; a = b; b = t; }
+                const uint32_t old = atomicMin(&lb[b], a);
+                if (old == b) break;
+                b = old;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(NT) cc_flatten_kernel(uint32_t* __restrict__ lab, const uint32_t* __restrict__ maskList,
+                                                        const uint32_t* __restrict__ maskCnt, PixAux* __restrict__ aux,
+                                                        size_t npx)
+{
+    const int f = blockIdx.y;
+    const uint32_t cnt = maskCnt[f];
+    uint32_t* lb = lab + (size_t)f * npx;
+    for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < cnt; k += gridDim.x * NT) {
+        const uint32_t p = maskList[(size_t)f * npx + k];
+        const uint32_t r = uf_find(lb, p);
+        lb[p] = r;
+        atomicAdd(&aux[(size_t)f * npx + r].csize, 1u);
+    }
+}
+
+// getLocalMax labels (Segmentation2D.cpp:309-327): a maximum inherits the label of the first
+// earlier maximum among (y-1,x-1), (y-1,x), (y-1,x+1), (y,x-1); otherwise it opens a new label.
+__global__ void __launch_bounds__(NT) seed_label_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ lab,
+                                                        uint32_t* __restrict__ fin, PixAux* __restrict__ aux,
+                                                        const uint32_t* __restrict__ seedList, const uint32_t* __restrict__ seedCnt,
+                                                        uint32_t* __restrict__ rootCnt, Region* __restrict__ regions,
+                                                        size_t regionCap, uint32_t* __restrict__ counters, int W, int H,
+                                                        size_t npx)
+{
+    const int f = blockIdx.y;
+    const uint32_t cnt = seedCnt[f];
+    const uint32_t* Pf = P + (size_t)f * npx;
+    for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < cnt; k += gridDim.x * NT) {
+        const uint32_t s = seedList[(size_t)f * npx + k];
+        uint32_t r = s;
+        while (true) {
+            const int y = r / W, x = r - y * W;
+            const int dys[4] = {-1, -1, -1, 0}, dxs[4] = {-1, 0, 1, -1};
+            bool moved = false;
+            for (int j = 0; j < 4 && !moved; ++j) {
+                const int yy = y + dys[j], xx = x + dxs[j];
+                if (yy < 0 || xx < 0 || xx >= W) continue;
+                const int q = yy * W + xx;
+                if (is_local_max(Pf, W, H, q)) { r = (uint32_t)q; moved = true; }
+            }
+            if (!moved) break;
+        }
+        const uint32_t L = r + 1;
+        fin[(size_t)f * npx + s] = L;
+        const uint32_t c = lab[(size_t)f * npx + s];
+        atomicMin(&aux[(size_t)f * npx + c].cmin, L);
+        atomicMax(&aux[(size_t)f * npx + c].cmax, L);
+        if (r == s) {
+            const uint32_t slot = atomicAdd(&rootCnt[f], 1u);
+            aux[(size_t)f * npx + s].rslot = slot;
+            if (slot < regionCap) regions[(size_t)f * regionCap + slot] = Region{s, 0u, 0ull, 0ull, 0ull};
+            else atomicOr(&counters[kStatus], 1u);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(NT) label_assign_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ lab,
+                                                          uint32_t* __restrict__ fin, const PixAux* __restrict__ aux,
+                                                          const uint32_t* __restrict__ maskList, const uint32_t* __restrict__ maskCnt,
+                                                          uint64_t* __restrict__ multiList, size_t multiCap,
+                                                          uint32_t* __restrict__ counters, uint32_t* __restrict__ frameFlag,
+                                                          uint32_t* __restrict__ flagList, int W, int H, size_t npx)
+{
+    const int f = blockIdx.y;
+    const uint32_t cnt = maskCnt[f];
+    const uint32_t* Pf = P + (size_t)f * npx;
+    for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < cnt; k += gridDim.x * NT) {
+        const uint32_t p = maskList[(size_t)f * npx + k];
+        const uint32_t c = lab[(size_t)f * npx + p];
+        const PixAux a = aux[(size_t)f * npx + c];
+        if (a.cmin == kNone) { flag_frame(frameFlag, counters, flagList, f); continue; }  // cannot happen
+        if (a.cmin == a.cmax) { fin[(size_t)f * npx + p] = a.cmin; continue; }
+        if (!is_local_max(Pf, W, H, p)) fin[(size_t)f * npx + p] = 0u;
+        if (p == c) {
+            const uint32_t j = atomicAdd(&counters[kMultiCnt], 1u);
+            if (j < multiCap) multiList[j] = ((uint64_t)f << 32) | c;
+            else flag_frame(frameFlag, counters, flagList, f);
+        }
+    }
+}
+
+// One thread floods one multi-label component (see file header, step 4).
+__device__ void flood_one(const uint32_t* __restrict__ P, uint32_t* __restrict__ lab, uint32_t* __restrict__ fin,
+                          const PixAux* __restrict__ aux, const uint32_t* __restrict__ maskCnt, uint64_t e,
+                          uint64_t* __restrict__ scratch, size_t scratchWords, uint32_t* __restrict__ counters,
+                          uint32_t* __restrict__ frameFlag, uint32_t* __restrict__ flagList, int W, int H, size_t npx)
+{
+    const int f = (int)(e >> 32);
+    const uint32_t c = (uint32_t)e;
+    if (__hip_atomic_load(&frameFlag[f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    const uint32_t* Pf = P + (size_t)f * npx;
+    uint32_t* lb = lab + (size_t)f * npx;
+    uint32_t* fn = fin + (size_t)f * npx;
+    const uint32_t n = aux[(size_t)f * npx + c].csize;
+    const uint64_t words = (uint64_t)(n + 1) / 2 + 2ull * n;
+    const uint64_t off = atomicAdd((unsigned long long*)&counters[kScratchUsed], (unsigned long long)words);
+    if (off + words > scratchWords) { flag_frame(frameFlag, counters, flagList, f); return; }
+    uint32_t* pix = (uint32_t*)(scratch + off);
+    DHeap h{(double*)(scratch + off + (n + 1) / 2), scratch + off + (n + 1) / 2 + n, 0u};
+
+    // Collect the component (breadth first from its root); mark visited in lab's top bit.
+    uint32_t tail = 0, head = 0;
+    pix[tail++] = c;
+    lb[c] = c | kVisited;
+    while (head < tail) {
+        const uint32_t p = pix[head++];
+        const int y = p / W, x = p - y * W;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int yy = y + dy;
+            if (yy < 0 || yy >= H) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int xx = x + dx;
+                if ((dx | dy) == 0 || xx < 0 || xx >= W) continue;
+                const uint32_t q = (uint32_t)(yy * W + xx);
+                if (Pf[q] != 0 && lb[q] == c && tail < n) {
+                    lb[q] = c | kVisited;
+                    pix[tail++] = q;
+                }
+            }
+        }
+    }
+    if (tail != n) { flag_frame(frameFlag, counters, flagList, f); return; }
+
+    // Seeds (pixels already labelled: the component's local maxima) in raster order: gather them
+    // in the heap value array, insertion-sort by index, then push like the reference's seed scan.
+    uint32_t ns = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        if (fn[pix[i]] != 0) h.v[ns++] = pix[i];
+    for (uint32_t i = 1; i < ns; ++i) {
+        const uint64_t x = h.v[i];
+        uint32_t j = i;
+        while (j > 0 && h.v[j - 1] > x) { h.v[j] = h.v[j - 1]; --j; }
+        h.v[j] = x;
+    }
+    // Move the sorted seed list to the pixel array (no longer needed) before pushing.
+    for (uint32_t i = 0; i < ns; ++i) pix[i] = (uint32_t)h.v[i];
+    for (uint32_t i = 0; i < ns; ++i) {
+        const uint32_t s = pix[i];
+        if (has_unlabelled_nbr(Pf, fn, W, H, (int)s)) h.push(age_key(Pf[s], 0), (uint64_t)s);
+    }
+
+    const double eps = kEpsilon, delta = 1e-9;
+    const double nmax = (double)maskCnt[f];
+    uint32_t it = 0;
+    bool ok = true;
+    while (h.n > 0 && ok) {
+        const uint64_t av = h.pop();
+        const uint32_t a = (uint32_t)av, ta = (uint32_t)(av >> 32);
+        const double oa = (double)Pf[a] * kPScale;
+        for (uint32_t i = 0; i < h.n; ++i) {
+            const uint64_t bv = h.v[i];
+            const uint32_t b = (uint32_t)bv, tb = (uint32_t)(bv >> 32);
+            const double ob = (double)Pf[b] * kPScale;
+            if (ta == tb) { if (oa == ob) { ok = false; break; } }
+            else if (ta < tb) { if (!(ob - oa < (double)(tb - ta) * eps - delta)) { ok = false; break; } }
+            else if (!(oa - ob > nmax * eps + delta)) { ok = false; break; }
+        }
+        if (!ok) break;
+        ++it;
+        const uint32_t la = fn[a];
+        const int y = a / W, x = a - y * W;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int yy = y + dy;
+            if (yy < 0 || yy >= H) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int xx = x + dx;
+                if ((dx | dy) == 0 || xx < 0 || xx >= W) continue;
+                const uint32_t q = (uint32_t)(yy * W + xx);
+                if (Pf[q] != 0 && fn[q] == 0) {
+                    if (h.n >= n) { ok = false; break; }   // cannot happen: each pixel is pushed once
+                    h.push(age_key(Pf[q], it), (uint64_t)q | ((uint64_t)it << 32));
+                    fn[q] = la;
+                }
+            }
+        }
+    }
+    if (!ok) flag_frame(frameFlag, counters, flagList, f);
+}
+
+__global__ void __launch_bounds__(64) flood_kernel(const uint32_t* __restrict__ P, uint32_t* __restrict__ lab,
+                                                   uint32_t* __restrict__ fin, const PixAux* __restrict__ aux,
+                                                   const uint32_t* __restrict__ maskCnt, const uint64_t* __restrict__ multiList,
+                                                   size_t multiCap, uint64_t* __restrict__ scratch, size_t scratchWords,
+                                                   uint32_t* __restrict__ counters, uint32_t* __restrict__ frameFlag,
+                                                   uint32_t* __restrict__ flagList, int W, int H, size_t npx)
+{
+    const uint32_t nMulti = min((size_t)counters[kMultiCnt], multiCap);
+    for (uint32_t t = blockIdx.x * 64 + threadIdx.x; t < nMulti; t += gridDim.x * 64)
+        flood_one(P, lab, fin, aux, maskCnt, multiList[t], scratch, scratchWords, counters, frameFlag, flagList, W, H, npx);
+}
+
+// Exact sequential restatement of Segmentation2D::watershed for one flagged frame per workgroup
+// (frames taken from flagList through a shared cursor). Work space per slot: 5 * npx words.
+__global__ void __launch_bounds__(NT) fallback_kernel(const uint32_t* __restrict__ P, uint32_t* __restrict__ fin,
+                                                      const uint32_t* __restrict__ maskList, const uint32_t* __restrict__ maskCnt,
+                                                      const uint32_t* __restrict__ flagList, uint32_t* __restrict__ counters,
+                                                      uint32_t* __restrict__ fbScratch, int W, int H, size_t npx)
+{
+    __shared__ uint32_t sFrame, sWave[NT / 64], sTotal;
+    const size_t npx2 = (npx + 1) & ~size_t(1);
+    uint32_t* seeds = fbScratch + (size_t)blockIdx.x * fb_slot_words(npx);
+    double* hk = (double*)(seeds + npx2);
+    uint64_t* hv = (uint64_t*)(seeds + 3 * npx2);
+    while (true) {
+        if (threadIdx.x == 0) {
+            const uint32_t i = atomicAdd(&counters[kFbNext], 1u);
+            sFrame = i < counters[kFlagCnt] ? flagList[i] : kNone;
+        }
+        __syncthreads();
+        const uint32_t f = sFrame;
+        if (f == kNone) return;
+        const uint32_t* Pf = P + (size_t)f * npx;
+        uint32_t* fn = fin + (size_t)f * npx;
+        // 1. Labels back to getLocalMax's state: maxima keep theirs, other mask pixels 0.
+        const uint32_t cnt = maskCnt[f];
+        for (uint32_t k = threadIdx.x; k < cnt; k += NT) {
+            const uint32_t p = maskList[(size_t)f * npx + k];
+            if (!is_local_max(Pf, W, H, (int)p)) fn[p] = 0u;
+        }
+        __syncthreads();
+        // 2. Seeds to push, in raster order (block-wide ordered compaction).
+        uint32_t total = 0;
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        for (size_t base = 0; base < npx; base += NT) {
+            const size_t i = base + threadIdx.x;
+            const bool sel = i < npx && Pf[i] != 0 && fn[i] != 0 && has_unlabelled_nbr(Pf, fn, W, H, (int)i);
+            const uint64_t m = __ballot(sel);
+            if (lane == 0) sWave[wv] = __popcll(m);
+            __syncthreads();
+            uint32_t before = total;
+            for (int j = 0; j < wv; ++j) before += sWave[j];
+            before += __popcll(m & ((1ull << lane) - 1ull));
+            if (sel) seeds[before] = (uint32_t)i;
+            if (threadIdx.x == 0) {
+                uint32_t s = 0;
+                for (int j = 0; j < NT / 64; ++j) s += sWave[j];
+                sTotal = s;
+            }
+            __syncthreads();
+            total += sTotal;
+            __syncthreads();
+        }
+        // 3. The flood itself, sequential as in the reference.
+        if (threadIdx.x == 0) {
+            DHeap h{hk, hv, 0u};
+            for (uint32_t i = 0; i < total; ++i) h.push(age_key(Pf[seeds[i]], 0), (uint64_t)seeds[i]);
+            uint32_t it = 0;
+            while (h.n > 0) {
+                const uint32_t a = (uint32_t)h.pop();
+                ++it;
+                const uint32_t la = fn[a];
+                const int y = a / W, x = a - y * W;
+                for (int dy = -1; dy <= 1; ++dy) {
+                    const int yy = y + dy;
+                    if (yy < 0 || yy >= H) continue;
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int xx = x + dx;
+                        if ((dx | dy) == 0 || xx < 0 || xx >= W) continue;
+                        const uint32_t q = (uint32_t)(yy * W + xx);
+                        if (Pf[q] != 0 && fn[q] == 0 && h.n < cnt) {
+                            h.push(age_key(Pf[q], it), (uint64_t)q);
+                            fn[q] = la;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(NT) region_stats_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ fin,
+                                                          const PixAux* __restrict__ aux, const uint32_t* __restrict__ maskList,
+                                                          const uint32_t* __restrict__ maskCnt, Region* __restrict__ regions,
+                                                          size_t regionCap, uint32_t* __restrict__ counters, int W, size_t npx)
+{
+    const int f = blockIdx.y;
+    const uint32_t cnt = maskCnt[f];
+    for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < cnt; k += gridDim.x * NT) {
+        const uint32_t p = maskList[(size_t)f * npx + k];
+        const uint32_t L = fin[(size_t)f * npx + p];
+        if (L == 0 || L > npx) { atomicOr(&counters[kStatus], 2u); continue; }
+        const uint32_t slot = aux[(size_t)f * npx + (L - 1)].rslot;
+        if (slot >= regionCap) { atomicOr(&counters[kStatus], 2u); continue; }
+        Region* r = regions + (size_t)f * regionCap + slot;
+        const uint64_t v = P[(size_t)f * npx + p];
+        const uint32_t y = p / W, x = p - y * W;
+        atomicAdd(&r->surf, 1u);
+        atomicAdd((unsigned long long*)&r->sP, (unsigned long long)v);
+        atomicAdd((unsigned long long*)&r->sPx, (unsigned long long)(v * x));
+        atomicAdd((unsigned long long*)&r->sPy, (unsigned long long)(v * y));
+    }
+}
+
+__device__ inline uint32_t block_sum(uint32_t v, uint32_t* sh)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    uint32_t t = 0;
+    for (int j = 0; j < (int)(blockDim.x >> 6); ++j) t += sh[j];
+    return t;
+}
+
+// FilterData (surface > VOLUME_MIN) and the per-frame capacity of Fitting::FitAll.
+__global__ void __launch_bounds__(NT) region_count_kernel(const Region* __restrict__ regions, size_t regionCap,
+                                                          const uint32_t* __restrict__ rootCnt, uint32_t* __restrict__ found,
+                                                          uint32_t* __restrict__ keep, uint64_t maxPoints)
+{
+    __shared__ uint32_t sh[NT / 64];
+    const int f = blockIdx.x;
+    const uint32_t n = min((size_t)rootCnt[f], regionCap);
+    uint32_t c = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += NT) c += (double)regions[(size_t)f * regionCap + i].surf > kVolumeMin;
+    const uint32_t t = block_sum(c, sh);
+    if (threadIdx.x == 0) {
+        found[f] = t;
+        keep[f] = (uint32_t)min((uint64_t)t, maxPoints);
+    }
+}
+
+// Sum over one region of the frame in raster order, in FP64 as ComputeStatistics does (used when
+// an exact integer sum reaches 2^53 and the reference's FP64 accumulation may have rounded).
+__device__ void raster_sums(const uint32_t* Pf, const uint32_t* fn, int W, size_t npx, uint32_t L, double& sx,
+                            double& sy, double& s)
+{
+    sx = sy = s = 0.0;
+    for (size_t i = 0; i < npx; ++i) {
+        if (fn[i] != L) continue;
+        const double v = (double)Pf[i] * kPScale;
+        const double y = (double)(i / W), x = (double)(i % W);
+        sx = __dadd_rn(sx, __dmul_rn(v, x));
+        sy = __dadd_rn(sy, __dmul_rn(v, y));
+        s = __dadd_rn(s, v);
+    }
+}
+
+__global__ void __launch_bounds__(NT) roi_emit_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ fin,
+                                                      const Region* __restrict__ regions, size_t regionCap,
+                                                      const uint32_t* __restrict__ rootCnt, const uint32_t* __restrict__ found,
+                                                      const uint32_t* __restrict__ keep, const uint32_t* __restrict__ roiBase,
+                                                      Roi* __restrict__ rois, int W, size_t npx)
+{
+    __shared__ uint32_t sWave[NT / 64];
+    __shared__ uint32_t sBase;
+    const int f = blockIdx.x;
+    const uint32_t n = min((size_t)rootCnt[f], regionCap);
+    const uint32_t kf = keep[f];
+    const bool truncate = found[f] > kf;
+    const Region* rg = regions + (size_t)f * regionCap;
+    if (threadIdx.x == 0) sBase = 0;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (uint32_t b0 = 0; b0 < n; b0 += NT) {
+        const uint32_t i = b0 + threadIdx.x;
+        Region r{};
+        bool sel = false;
+        if (i < n) {
+            r = rg[i];
+            sel = (double)r.surf > kVolumeMin;
+        }
+        uint32_t pos = 0;
+        if (truncate) {
+            // Keep the kf survivors with the smallest labels (first local maxima in raster order).
+            if (sel) {
+                uint32_t rank = 0;
+                for (uint32_t j = 0; j < n; ++j) {
+                    const Region q = rg[j];
+                    rank += ((double)q.surf > kVolumeMin) && q.root < r.root;
+                }
+                pos = rank;
+                sel = rank < kf;
+            }
+        } else {
+            const uint64_t m = __ballot(sel);
+            if (lane == 0) sWave[wv] = __popcll(m);
+            __syncthreads();
+            pos = sBase + __popcll(m & ((1ull << lane) - 1ull));
+            for (int j = 0; j < wv; ++j) pos += sWave[j];
+            __syncthreads();
+            if (threadIdx.x == 0) for (int j = 0; j < NT / 64; ++j) sBase += sWave[j];
+            __syncthreads();
+        }
+        if (sel) {
+            double sx, sy, s;
+            const double lim = 9007199254740992.0; // 2^53
+            if ((double)r.sPx >= lim || (double)r.sPy >= lim || (double)r.sP >= lim) {
+                raster_sums(P + (size_t)f * npx, fin + (size_t)f * npx, W, npx, r.root + 1, sx, sy, s);
+            } else {
+                sx = (double)r.sPx * kPScale;
+                sy = (double)r.sPy * kPScale;
+                s = (double)r.sP * kPScale;
+            }
+            Roi o;
+            o.x = sx / s;
+            o.y = sy / s;
+            o.seg = s;
+            o.surf = (double)r.surf;
+            o.frame = (uint32_t)f;
+            o.slot = pos;
+            rois[roiBase[f] + pos] = o;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(1024) exclusive_scan_kernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int n)
+{
+    // Single workgroup: each thread scans a contiguous segment, then the segment totals.
+    __shared__ uint32_t seg[1024];
+    const int per = (n + 1023) / 1024;
+    const int b = threadIdx.x * per, e = min(n, b + per);
+    uint32_t s = 0;
+    for (int i = b; i < e; ++i) s += in[i];
+    seg[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < 1024; ++i) { const uint32_t t = seg[i]; seg[i] = acc; acc += t; }
+        out[n] = acc;
+    }
+    __syncthreads();
+    uint32_t acc = seg[threadIdx.x];
+    for (int i = b; i < e; ++i) { const uint32_t t = in[i]; out[i] = acc; acc += t; }
+}
+
+} // namespace
+
+void launch_exclusive_scan(const uint32_t* d_in, uint32_t* d_out, int n, hipStream_t s)
+{
+    exclusive_scan_kernel<<<1, 1024, 0, s>>>(d_in, d_out, n);
+    PT_CHECK(hipGetLastError());
+}
+
+void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoints, Roi* d_rois, hipStream_t s)
+{
+    const size_t npx = b.npx;
+    const unsigned bpf = (unsigned)std::min<size_t>(64, std::max<size_t>(1, npx / 16384));
+    const dim3 g(bpf, F);
+    cc_union_kernel<<<g, NT, 0, s>>>(b.lab, b.P, b.maskList, b.maskCnt, b.aux, W, H, npx);
+    cc_flatten_kernel<<<g, NT, 0, s>>>(b.lab, b.maskList, b.maskCnt, b.aux, npx);
+    seed_label_kernel<<<g, NT, 0, s>>>(b.P, b.lab, b.fin, b.aux, b.seedList, b.seedCnt, b.rootCnt, b.regions,
+                                       b.regionCap, b.counters, W, H, npx);
+    label_assign_kernel<<<g, NT, 0, s>>>(b.P, b.lab, b.fin, b.aux, b.maskList, b.maskCnt, b.multiList, b.multiCap,
+                                         b.counters, b.frameFlag, b.flagList, W, H, npx);
+    flood_kernel<<<2048, 64, 0, s>>>(b.P, b.lab, b.fin, b.aux, b.maskCnt, b.multiList, b.multiCap,
+                                                         b.scratch, b.scratchWords, b.counters, b.frameFlag, b.flagList,
+                                                         W, H, npx);
+    fallback_kernel<<<(unsigned)b.fbSlots, NT, 0, s>>>(b.P, b.fin, b.maskList, b.maskCnt, b.flagList, b.counters,
+                                                        b.fbScratch, W, H, npx);
+    region_stats_kernel<<<g, NT, 0, s>>>(b.P, b.fin, b.aux, b.maskList, b.maskCnt, b.regions, b.regionCap, b.counters,
+                                         W, npx);
+    region_count_kernel<<<F, NT, 0, s>>>(b.regions, b.regionCap, b.rootCnt, b.found, b.keep, maxPoints);
+    exclusive_scan_kernel<<<1, 1024, 0, s>>>(b.keep, b.roiBase, F);
+    roi_emit_kernel<<<F, NT, 0, s>>>(b.P, b.fin, b.regions, b.regionCap, b.rootCnt, b.found, b.keep, b.roiBase, d_rois,
+                                     W, npx);
+    PT_CHECK(hipGetLastError());
+}
+
+} // namespace pt

@@ -1,0 +1,169 @@
+"""ctypes binding of libpalmtracer_amd.so, shaped like the reference's Python front end.
+
+The reference library (tmonseigne/PALMTracer-cpp) is driven from Python through ctypes with host
+numpy buffers (CPU/main.cpp comments :106-107, :229-230). The functions below call the same C entry
+points with the same argument meaning; `*_device` variants take torch tensors that live in HBM.
+
+The library must have been built (palmtracer-cpp_amd/build.py); there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "libpalmtracer_amd.so")
+COLUMNS = ["Id", "Plane", "Index", "Channel", "X", "Y", "Z", "Integrated Intensity", "Sigma X", "Sigma Y", "Theta",
+           "MSE XY", "MSE Z", "Intensity 0", "Intensity Offset", "Intensity", "Surface", "Circularity"]
+NCOL = 18
+
+_u64, _f64, _i32, _vp = ctypes.c_uint64, ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+_lib = None
+
+# Default synthetic camera (single-molecule stacks of the benchmark configurations).
+SYNTH_2D = dict(bg=20.0, sigma=1.3, median=2000.0, sigma_log=0.5, offset=100.0, gain=2.0, astig=0, s0=1.2, c=250.0,
+                d=400.0, zmin=-600.0, zmax=600.0, scmos=0, sd_offset=3.0, sd_gain=0.1)
+
+
+def lib():
+    """Load the shared library once; raise if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run palmtracer-cpp_amd/build.py (no CPU fallback exists)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.Localization.restype = _u64
+        L.Localization.argtypes = [_vp, _vp, _u64, _u64, _u64, _u64, _f64, _f64, _u64, _vp]
+        L.Fitting.restype = _u64
+        L.Fitting.argtypes = [_vp, _vp, _u64, _vp, _u64, _u64, _u64, _u64, _u64, _vp]
+        L.Wavelett.restype = None
+        L.Wavelett.argtypes = [_vp, _vp, _u64, _u64, _u64, _u64]
+        L.pt_localization_device.restype = _u64
+        L.pt_localization_device.argtypes = [_vp, _vp, _u64, _u64, _u64, _u64, _f64, _f64, _u64, _vp, _u64]
+        L.pt_wavelet_device.restype = _i32
+        L.pt_wavelet_device.argtypes = [_vp, _vp, _u64, _u64, _u64, _u64]
+        L.pt_debug_labels.restype = _i32
+        L.pt_debug_labels.argtypes = [_vp, _u64, _u64, _f64, _vp]
+        L.pt_set_option.restype = _i32
+        L.pt_set_option.argtypes = [ctypes.c_char_p, _f64]
+        L.pt_last_stats.restype = _i32
+        L.pt_last_stats.argtypes = [_vp, _u64]
+        L.pt_synth_device.restype = _i32
+        L.pt_synth_device.argtypes = [_vp, _u64, _i32, _i32, _i32, _i32, _vp]
+        L.pt_synth_emitters_device.restype = _i32
+        L.pt_synth_emitters_device.argtypes = [_vp, _u64, _i32, _i32, _i32, _i32, _vp]
+        _lib = L
+    return _lib
+
+
+def _stack(stack):
+    s = np.ascontiguousarray(stack, dtype=np.uint16)
+    if s.ndim == 2:
+        s = s[None]
+    if s.ndim != 3:
+        raise ValueError("stack must be (planes, height, width) or (height, width)")
+    return s
+
+
+def _params(fit_params):
+    p = np.zeros(4, np.float64)
+    fp = np.asarray(fit_params, np.float64).ravel()
+    p[:min(4, fp.size)] = fp[:4]
+    return p
+
+
+def default_capacity(planes, height, width):
+    """Output size (doubles) the reference's tests allocate: ((h*w)/5 + 1) * planes * 18."""
+    return ((height * width) // 5 + 1) * planes * NCOL
+
+
+def localization(stack, threshold, watershed_ratio=0.0, fit=2, fit_params=(7, 1.3, 1.3, 0), output_size=None):
+    """Localization (CPU/main.cpp:112-121): rows (n, 18) in the reference's column order."""
+    s = _stack(stack)
+    z, h, w = s.shape
+    cap = default_capacity(z, h, w) if output_size is None else int(output_size)
+    out = np.empty(max(cap, 1), np.float64)
+    p = _params(fit_params)
+    n = lib().Localization(s.ctypes.data, out.ctypes.data, cap, h, w, z, float(threshold), float(watershed_ratio),
+                           int(fit), p.ctypes.data)
+    return out[:n].reshape(-1, NCOL).copy()
+
+
+def fitting(stack, positions, fit=2, fit_params=(7, 1.3, 1.3, 0)):
+    """Fitting (CPU/main.cpp:141-148): positions (n, 6) = [Id, Plane, X, Y, SegIntensity, Surface]."""
+    s = _stack(stack)
+    z, h, w = s.shape
+    pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 6)
+    out = np.empty(max(pos.shape[0] * NCOL, 1), np.float64)
+    p = _params(fit_params)
+    n = lib().Fitting(s.ctypes.data, pos.ctypes.data, pos.shape[0], out.ctypes.data, out.size, h, w, z, int(fit),
+                      p.ctypes.data)
+    return out[:n].reshape(-1, NCOL).copy()
+
+
+def wavelett(stack, level=2):
+    """Wavelett (CPU/main.cpp:257-284): one plane per frame, level 1 = W0, 2 = W1, 3 = V2."""
+    s = _stack(stack)
+    z, h, w = s.shape
+    out = np.empty((z, h, w), np.float64)
+    lib().Wavelett(s.ctypes.data, out.ctypes.data, h, w, z, int(level))
+    return out
+
+
+def debug_labels(frame, threshold):
+    f = np.ascontiguousarray(frame, np.uint16)
+    h, w = f.shape
+    out = np.zeros((h, w), np.uint32)
+    if not lib().pt_debug_labels(f.ctypes.data, h, w, float(threshold), out.ctypes.data):
+        raise RuntimeError("pt_debug_labels failed")
+    return out
+
+
+def set_option(name, value):
+    if not lib().pt_set_option(name.encode(), float(value)):
+        raise ValueError(f"option {name!r} rejected")
+
+
+def last_stats():
+    v = (ctypes.c_uint64 * 7)()
+    lib().pt_last_stats(v, 7)
+    keys = ["frames", "chunks", "rows", "flagged_frames", "multi_components", "mask_pixels", "status"]
+    return dict(zip(keys, list(v)))
+
+
+def _synth_params(cfg):
+    c = dict(SYNTH_2D)
+    c.update(cfg or {})
+    order = ["bg", "sigma", "median", "sigma_log", "offset", "gain", "astig", "s0", "c", "d", "zmin", "zmax", "scmos",
+             "sd_offset", "sd_gain"]
+    return np.array([float(c[k]) for k in order], np.float64)
+
+
+def synth_device(out, seed, emitters, **cfg):
+    """Fill the torch uint16 tensor `out` (frames, height, width) on the GPU with a seeded stack."""
+    f, h, w = out.shape
+    p = _synth_params(cfg)
+    if not lib().pt_synth_device(out.data_ptr(), int(seed), w, h, f, int(emitters), p.ctypes.data):
+        raise RuntimeError("pt_synth_device failed")
+    return out
+
+
+def synth_emitters_device(out, seed, height, width, **cfg):
+    """Ground-truth emitters into the torch float32 tensor `out` (frames, emitters, 5)."""
+    f, e, _ = out.shape
+    p = _synth_params(cfg)
+    if not lib().pt_synth_emitters_device(out.data_ptr(), int(seed), width, height, f, e, p.ctypes.data):
+        raise RuntimeError("pt_synth_emitters_device failed")
+    return out
+
+
+def localization_device(d_stack, d_out, threshold, watershed_ratio=0.0, fit=2, fit_params=(7, 1.3, 1.3, 0),
+                        max_per_plane=None):
+    """Localization over HBM-resident torch tensors; returns the number of rows written to d_out."""
+    z, h, w = d_stack.shape
+    cap = d_out.numel()
+    mpp = (cap // NCOL) // max(z, 1) if max_per_plane is None else int(max_per_plane)
+    p = _params(fit_params)
+    n = lib().pt_localization_device(d_stack.data_ptr(), d_out.data_ptr(), cap, h, w, z, float(threshold),
+                                     float(watershed_ratio), int(fit), p.ctypes.data, mpp)
+    return n // NCOL

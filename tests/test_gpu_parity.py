@@ -1,0 +1,190 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle and the reference's golden data."""
+import numpy as np
+import pytest
+
+from helpers import ID, INDEX, PLANE, QXY, SEG, SURF, SX, SY, X, Y, match_rows, rank_labels, theta_to_degrees
+
+pytestmark = pytest.mark.gpu
+FIT_PARAMS_CSV = (7.0, 1.0, 2.0, 0.0)
+FIT_PARAMS_SYN = (7.0, 1.3, 1.3, 0.0)
+
+
+@pytest.fixture(autouse=True)
+def fp64_default(pt_gpu):
+    pt_gpu.set_option("fit_fp64", 1)
+    pt_gpu.set_option("max_iter", 75)
+    yield
+
+
+def _stacks(ref_inputs, ref_vectors):
+    return {
+        "stack": (ref_inputs["stack"], 103.6),
+        "small": (ref_vectors["small"], float(ref_vectors["small_autothreshold"][0])),
+        "dense": (ref_vectors["dense"], float(ref_vectors["dense_autothreshold"][0])),
+    }
+
+
+# ------------------------------------------------------------------ a trous -----------------------
+@pytest.mark.parametrize("level", [0, 1, 2, 3, 4])
+def test_wavelett_bit_exact(pt_gpu, oracle_mod, ref_inputs, level):
+    st = ref_inputs["stack"]
+    got = pt_gpu.wavelett(st, level)
+    exp = oracle_mod.wavelett(st, level)
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (9, 1), (1, 9), (13, 7), (65, 33), (3, 700), (300, 2)])
+def test_wavelett_shapes(pt_gpu, oracle_mod, shape):
+    rng = np.random.default_rng(sum(shape))
+    st = rng.integers(0, 65536, (2,) + shape, dtype=np.uint16)
+    for level in (1, 2, 3):
+        assert np.array_equal(pt_gpu.wavelett(st, level), oracle_mod.wavelett(st, level))
+
+
+def test_wavelett_saturated_extremes(pt_gpu, oracle_mod):
+    st = np.zeros((1, 40, 40), np.uint16)
+    st[0, ::2, ::3] = 65535
+    for level in (1, 2, 3):
+        assert np.array_equal(pt_gpu.wavelett(st, level), oracle_mod.wavelett(st, level))
+
+
+# ------------------------------------------------------------------ watershed labels --------------
+@pytest.mark.parametrize("name", ["stack", "small", "dense"])
+def test_labels_bit_exact(pt_gpu, oracle_mod, ref_inputs, ref_vectors, name):
+    st, thr = _stacks(ref_inputs, ref_vectors)[name]
+    for f in range(st.shape[0]):
+        w1 = oracle_mod.atrous(st[f])[1]
+        exp = oracle_mod.watershed_labels(w1, thr)
+        got = pt_gpu.debug_labels(st[f], thr)
+        assert np.array_equal(got > 0, exp > 0), f"mask differs in frame {f}"
+        assert np.array_equal(rank_labels(got), exp), f"labels differ in frame {f}"
+
+
+def test_labels_threshold_edges(pt_gpu, oracle_mod, ref_vectors):
+    """Thresholds placed exactly on W1 values: the strict comparison must match bit for bit."""
+    st = ref_vectors["dense"]
+    w1 = oracle_mod.atrous(st[0])[1]
+    vals = np.unique(w1[w1 > 5])
+    for thr in (vals[len(vals) // 3], vals[len(vals) // 2], -1.0, 0.0):
+        exp = oracle_mod.watershed_labels(w1, float(thr))
+        got = pt_gpu.debug_labels(st[0], float(thr))
+        assert np.array_equal(rank_labels(got), exp)
+
+
+# ------------------------------------------------------------------ localization ------------------
+@pytest.mark.parametrize("name", ["stack", "small", "dense"])
+def test_localization_no_fit_bit_exact(pt_gpu, oracle_mod, ref_inputs, ref_vectors, name):
+    st, thr = _stacks(ref_inputs, ref_vectors)[name]
+    got = pt_gpu.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN)
+    exp = oracle_mod.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN)
+    assert got.shape == exp.shape
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("fit", [1, 2, 3, 4])
+@pytest.mark.parametrize("name", ["small", "dense"])
+def test_localization_fp64_matches_oracle(pt_gpu, oracle_mod, ref_vectors, name, fit):
+    st, thr = ref_vectors[name], float(ref_vectors[f"{name}_autothreshold"][0])
+    got = pt_gpu.localization(st, thr, 0.0, fit, FIT_PARAMS_SYN)
+    exp = oracle_mod.localization(st, thr, 0.0, fit, FIT_PARAMS_SYN)
+    assert got.shape == exp.shape
+    # FP64 arithmetic in the reference's order; only the device exp/sin/cos differ in the last ulp.
+    assert match_rows(got, exp, atol=1e-7, rtol=1e-7) == []
+
+
+@pytest.mark.parametrize("fit", [0, 1, 2, 3, 4])
+def test_localization_matches_golden_csv(pt_gpu, ref_inputs, ref_csv, fit):
+    rows = pt_gpu.localization(ref_inputs["stack"], 103.6, 0.0, fit, FIT_PARAMS_CSV)
+    exp = ref_csv[f"stack-localizations-103.6_True_{fit}_1.0_0.0_7"]
+    assert match_rows(theta_to_degrees(rows, fit), exp, atol=1e-3) == []
+
+
+def test_quadrant_matches_golden_csv(pt_gpu, ref_inputs, ref_csv):
+    rows = pt_gpu.localization(ref_inputs["stack_quadrant"], 103.6, 0.0, 4, FIT_PARAMS_CSV)
+    exp = ref_csv["stack_quadrant-localizations-103.6_True_4_1.0_0.0_7"]
+    assert match_rows(theta_to_degrees(rows, 4), exp, atol=1e-5) == []
+
+
+@pytest.mark.parametrize("fit", [1, 2, 3, 4])
+def test_localization_fp32_within_tolerance(pt_gpu, oracle_mod, ref_vectors, fit):
+    """FP32 model/Jacobian: positions within 1e-3 px, amplitude/offset/sigma within 1e-4 relative."""
+    st, thr = ref_vectors["small"], float(ref_vectors["small_autothreshold"][0])
+    pt_gpu.set_option("fit_fp64", 0)
+    got = pt_gpu.localization(st, thr, 0.0, fit, FIT_PARAMS_SYN)
+    exp = oracle_mod.localization(st, thr, 0.0, fit, FIT_PARAMS_SYN)
+    assert got.shape == exp.shape
+    ok = (exp[:, 13] > 0) & (got[:, 13] > 0)
+    assert ok.mean() > 0.9
+    bad = match_rows(got, exp, atol=1e-3, rtol=0.0, cols=[X, Y])
+    assert len(bad) <= max(1, len(exp) // 200), bad[:5]
+
+
+def test_per_plane_capacity(pt_gpu, oracle_mod, ref_inputs):
+    st = ref_inputs["stack"]
+    z = st.shape[0]
+    for cap in (2 * 18 * z + 1, 18 * z, 5 * 18 * z):
+        got = pt_gpu.localization(st, 103.6, 0.0, 0, FIT_PARAMS_CSV, output_size=cap)
+        exp = oracle_mod.localization(st, 103.6, 0.0, 0, FIT_PARAMS_CSV, output_size=cap)
+        assert np.array_equal(got, exp)
+
+
+def test_empty_inputs(pt_gpu):
+    assert pt_gpu.localization(np.zeros((3, 12, 16), np.uint16), 0.0, 0.0, 0).shape == (0, 18)
+    assert pt_gpu.localization(np.zeros((2, 16, 16), np.uint16), 0.0, 0.0, 2, output_size=0).shape == (0, 18)
+    assert pt_gpu.fitting(np.zeros((1, 16, 16), np.uint16), np.zeros((0, 6)), 2).shape == (0, 18)
+
+
+def test_frames_are_independent(pt_gpu, ref_inputs):
+    """Empty frames between useful ones change nothing (Tests_CPU/test_Palm.cpp:160-215)."""
+    st = ref_inputs["stack"]
+    single = pt_gpu.localization(st[:1], 103.6, 0.0, 4, FIT_PARAMS_CSV)
+    z = np.zeros((5,) + st.shape[1:], np.uint16)
+    z[1] = st[0]
+    z[3] = st[0]
+    got = pt_gpu.localization(z, 103.6, 0.0, 4, FIT_PARAMS_CSV)
+    assert got.shape[0] == 2 * single.shape[0]
+    n = single.shape[0]
+    assert np.array_equal(got[:, ID], np.arange(1, 2 * n + 1))
+    assert np.array_equal(got[:, PLANE], np.repeat([2, 4], n))
+    assert np.array_equal(got[:n, 3:], single[:, 3:]) and np.array_equal(got[n:, 3:], single[:, 3:])
+
+
+# ------------------------------------------------------------------ fit only ----------------------
+@pytest.mark.parametrize("fit", [1, 2, 3, 4])
+def test_fitting_matches_oracle(pt_gpu, oracle_mod, ref_vectors, fit):
+    st = ref_vectors["dense"]
+    pos = ref_vectors["dense_positions"]
+    got = pt_gpu.fitting(st, pos, fit, FIT_PARAMS_SYN)
+    exp = oracle_mod.fitting(st, pos, fit, FIT_PARAMS_SYN)
+    assert np.array_equal(got[:, [ID, PLANE, INDEX]], exp[:, [ID, PLANE, INDEX]])
+    assert np.allclose(got, exp, atol=1e-7, rtol=1e-7, equal_nan=True)
+
+
+def test_fitting_matches_localization(pt_gpu, ref_inputs):
+    st = ref_inputs["stack"]
+    det = pt_gpu.localization(st, 103.6, 0.0, 0, FIT_PARAMS_CSV)
+    exp = pt_gpu.localization(st, 103.6, 0.0, 2, FIT_PARAMS_CSV)
+    pos = np.column_stack([2 * det[:, ID], det[:, PLANE], det[:, X], det[:, Y], det[:, SEG], det[:, SURF]])
+    out = pt_gpu.fitting(st, pos, 2, FIT_PARAMS_CSV)
+    a, e = out.copy(), exp.copy()
+    a[:, [ID, INDEX]] = 0
+    e[:, [ID, INDEX]] = 0
+    assert match_rows(a, e, atol=1e-8) == []
+
+
+# ------------------------------------------------------------------ device generator + full path --
+def test_synthetic_stack_end_to_end(pt_gpu, oracle_mod):
+    import torch
+    d = torch.empty((4, 256, 256), dtype=torch.uint16, device="cuda")
+    pt_gpu.synth_device(d, seed=21, emitters=30)
+    st = d.cpu().numpy()
+    assert 100 <= st.min() and st.mean() > 130
+    thr = oracle_mod.autothreshold(st[0])
+    got = pt_gpu.localization(st, thr, 0.0, 2, FIT_PARAMS_SYN)
+    exp = oracle_mod.localization(st, thr, 0.0, 2, FIT_PARAMS_SYN)
+    assert got.shape == exp.shape and got.shape[0] > 50
+    assert match_rows(got, exp, atol=1e-7, rtol=1e-7) == []
+    # Device-resident entry: same rows.
+    out = torch.empty(((256 * 256) // 5 + 1) * 4 * 18, dtype=torch.float64, device="cuda")
+    n = pt_gpu.localization_device(d, out, thr, 0.0, 2, FIT_PARAMS_SYN)
+    assert np.array_equal(out[: n * 18].view(-1, 18).cpu().numpy(), got)
