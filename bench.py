@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Fitted localizations/s of the detection + fitting path on 512x512 frames, one MI355X.
+
+Workload (the 512^2 headline configuration): 10,000 seeded frames of 512x512 u16, 200 emitters per
+frame (ADU = 100 + 2 * Poisson(20 + sum of pixel-integrated Gaussians, sigma 1.3 px, photons
+log-normal median 2000, sigma_log 0.5)), a trous detection with the threshold AutoThreshold returns
+on frame 0, watershed ratio 0, then the 7x7 Gaussian LM fit with free sigma (fit mode 2,
+fit_params [7, 1.3, 1.3, 0], at most 75 iterations). The stack is generated on the device and stays
+in HBM; one step is one Localization over the whole stack (detection, segmentation, fit, row
+assembly), rows written to HBM.
+
+Prints one JSON line. `roofline` is the dominant stage's kernel measured with HIP events on the
+library's stream; `cpu_baseline` times the compiled reference (oracle/_ref) or, without it, the CPU
+oracle, on a bounded sample of the same frames on this host's cores.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "palmtracer-cpp_amd"))
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+# Algorithmic work (per unit): a trous detection reads the u16 pixel and writes the 4-byte
+# thresholded W1 potential; the 7x7 / 5-parameter LM pass is 49 * 60 FLOP with the Jacobian and
+# alpha/beta accumulation, 49 * 15 FLOP for a trial model evaluation.
+ATROUS_BYTES_PER_PX = 6.0
+JAC_FLOP_7x7 = 2940.0
+TRIAL_FLOP_7x7 = 735.0
+
+
+def cpu_baseline(stack_host, thr, fit_params, budget_s):
+    """Reference (or oracle) Localization on the first frames of the stack, OpenMP over frames."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    R = oracle.ref_lib()
+    kind = "reference" if R is not None else "port"
+    fp = np.asarray(fit_params, np.float64)
+
+    def run(nf):
+        st = np.ascontiguousarray(stack_host[:nf])
+        cap = nf * 1000 * 18
+        out = np.empty(cap)
+        t = time.perf_counter()
+        if R is not None:
+            n = R.ref_localization(st.ctypes.data, out.ctypes.data, cap, st.shape[1], st.shape[2], nf, thr, 0.0, 2,
+                                   fp.ctypes.data)
+        else:
+            n = oracle.lib().or_localization(st.ctypes.data, out.ctypes.data, cap, st.shape[1], st.shape[2], nf, thr,
+                                             0.0, 2, fp.ctypes.data, 75)
+        return n // 18, time.perf_counter() - t
+
+    nf = min(len(stack_host), 4 * threads)
+    rows, dt = run(nf)
+    per = dt / nf
+    nf = int(min(len(stack_host), max(nf, budget_s / max(per, 1e-6))))
+    rows, dt = run(nf)
+    return {"value": rows / dt, "unit": "localizations/s", "cores": threads, "kind": kind,
+            "sample": f"first {nf} frames of the same stack ({rows} rows in {dt:.2f} s), OpenMP over frames"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--frames", type=int, default=10000)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--emitters", type=int, default=200)
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import palmtracer_amd as pt
+
+    F, S = args.frames, args.size
+    fit_params = (7.0, 1.3, 1.3, 0.0)
+    seed = 0x5EED0002
+    stack = torch.empty((F, S, S), dtype=torch.uint16, device="cuda")
+    pt.synth_device(stack, seed=seed, emitters=args.emitters)
+    torch.cuda.synchronize()
+    thr = pt.autothreshold_device(stack[0])
+    max_per_plane = 1000
+    out = torch.empty(F * max_per_plane * 18, dtype=torch.float64, device="cuda")
+    pt.set_option("fit_fp64", 1 if args.precision == "fp64" else 0)
+    pt.set_option("max_iter", 75)
+    pt.set_option("profile", 1)
+
+    for _ in range(args.warmup):
+        pt.localization_device(stack, out, thr, 0.0, 2, fit_params, max_per_plane)
+    torch.cuda.synchronize()
+    stages = {"atrous_ms": 0.0, "segment_ms": 0.0, "fit_ms": 0.0, "assemble_ms": 0.0, "chunks": 0.0,
+              "jacobian_passes": 0.0, "trial_passes": 0.0}
+    rows = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rows = pt.localization_device(stack, out, thr, 0.0, 2, fit_params, max_per_plane)
+        st = pt.stage_times()
+        for k in stages:
+            stages[k] += st[k]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = 1e3 * dt / args.steps
+    res = out[: rows * 18].view(-1, 18)
+    fitted = int((res[:, 13] > 0).sum().item())
+
+    # Roofline of the dominant stage's kernel (averages over all launches of the timed steps).
+    launches = max(stages["chunks"], 1.0)
+    atrous_ms = stages["atrous_ms"] / launches
+    fit_ms = stages["fit_ms"] / launches
+    px_per_launch = F * S * S / (stages["chunks"] / args.steps)
+    if rows == 0:
+        raise SystemExit("no localizations: the pipeline reported an error (see stderr)")
+    if stages["fit_ms"] >= stages["atrous_ms"]:
+        flops = (stages["jacobian_passes"] * JAC_FLOP_7x7 + stages["trial_passes"] * TRIAL_FLOP_7x7) / launches
+        peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
+        ach = flops / (fit_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": "fit_kernel (FP32 VALU LM, no MFMA)", "achieved": ach, "peak": peak,
+                "unit": "TFLOP/s", "frac": ach / peak, "traffic": None, "launch_ms": fit_ms,
+                "flop_per_launch": flops}
+    else:
+        bytes_l = ATROUS_BYTES_PER_PX * px_per_launch
+        ach = bytes_l / (atrous_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": "atrous_detect_kernel", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None, "launch_ms": atrous_ms,
+                "bytes_per_launch": bytes_l}
+    stage_ms = {k: v / args.steps for k, v in stages.items() if k.endswith("_ms")}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        host = stack[: min(F, 4000)].cpu().numpy()
+        cpu = cpu_baseline(host, thr, fit_params, args.cpu_seconds)
+
+    line = {
+        "metric": "fitted localizations/sec at 512^2 frames",
+        "value": rows / (ms_step * 1e-3),
+        "unit": "localizations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else "f64",
+        "data": "synthetic: seeded on-device generator (Philox), stack resident in HBM",
+        "config": {"workload": f"{F} frames {S}x{S} u16, {args.emitters} emitters/frame, a trous + watershed "
+                               f"(ratio 0) + 7x7 Gaussian LM fit mode 2 (max 75 it), threshold {thr:.6f} "
+                               f"(AutoThreshold of frame 0)",
+                   "frames": F, "height": S, "width": S, "emitters_per_frame": args.emitters, "fit": 2,
+                   "roi": 7, "threshold": thr, "rows": rows, "fitted_rows": fitted, "seed": seed,
+                   "stage_ms_per_step": stage_ms},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

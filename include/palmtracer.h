@@ -52,7 +52,17 @@ PT_API uint64_t PT_CALL Fitting(const uint16_t* input, const double* positions, 
 PT_API void PT_CALL Wavelett(const uint16_t* input, double* output, uint64_t height, uint64_t width, uint64_t planes,
                              uint64_t levelOutput);
 
+/* Replaces AutoThreshold (CPU/main.cpp:75-85 -> lib/Palm.cpp:122-172): threshold for one plane,
+ * from the standard deviation of the pixels outside the ROI squares of the current detections
+ * (watershed ratio 10, no fit), iterated at most 5 times. fit_params[0] is the ROI side.
+ * Returns NaN on failure. */
+PT_API double PT_CALL AutoThreshold(const uint16_t* input, uint64_t height, uint64_t width, const double* fit_params);
+
 /* ---- no counterpart in the reference ------------------------------------------------------ */
+
+/* AutoThreshold of one frame in device memory. */
+PT_API double PT_CALL pt_autothreshold_device(const uint16_t* d_input, uint64_t height, uint64_t width,
+                                              const double* fit_params);
 
 /* Localization over device memory: d_input (planes x height x width u16) and d_output
  * (outputCapDoubles doubles) are HBM pointers; maxPerPlane is the per-plane row capacity. */
@@ -65,18 +75,25 @@ PT_API uint64_t PT_CALL pt_localization_device(const uint16_t* d_input, double* 
 PT_API int PT_CALL pt_wavelet_device(const uint16_t* d_input, double* d_output, uint64_t height, uint64_t width,
                                      uint64_t planes, uint64_t levelOutput);
 
-/* Watershed label map of one host frame (watershedRatio 0): labels[i] = raster index + 1 of the
- * first local maximum of the pixel's label, 0 below threshold. Returns 1 on success. */
+/* Watershed label map of one host frame: labels[i] = raster index + 1 of the first local maximum
+ * of the pixel's label (its region's identity), 0 below threshold. With (int)watershedRatio >= 1
+ * the map is the one after basin fusion and the second watershed. Returns 1 on success. */
 PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint64_t width, double thresholdVal,
-                                   uint32_t* labels);
+                                   double watershedRatio, uint32_t* labels);
 
 /* Options: "fit_fp64" (1: FP64 model and sums, 0: FP32), "max_iter" (LM iteration cap, 75),
- * "chunk_pixels" (frames per device chunk x pixels). Returns 1 on success. */
+ * "chunk_pixels" (frames per device chunk x pixels), "profile" (1: stage timing),
+ * "force_sequential_watershed" (1: every frame through the exact sequential flood). Returns 1 on success. */
 PT_API int PT_CALL pt_set_option(const char* name, double value);
 
 /* Counters of the last call: frames, chunks, rows, frames re-flooded sequentially, multi-label
  * components, mask pixels, status. Writes min(n, 7) values, returns 7. */
 PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n);
+
+/* With the "profile" option, device milliseconds of the last Localization per stage (a trous,
+ * segmentation, fit, assembly; HIP events on the library's stream, summed over chunks), the chunk
+ * count and the LM passes executed (model + Jacobian, trial). Writes min(n, 7) values, returns 7. */
+PT_API int PT_CALL pt_stage_times(double* out, uint64_t n);
 
 /* Seeded synthetic stack on the device (benchmark / test input): d_out receives frames x height x
  * width u16. params (15 doubles): bg photons, PSF sigma (px), photon median, photon sigma_log, camera

@@ -22,12 +22,18 @@ struct DevBuf {
 struct RunStats {
     uint64_t frames = 0, chunks = 0, rois = 0, flaggedFrames = 0, multiComponents = 0, maskPixels = 0;
     uint32_t status = 0;
+    // With Options::profile: device time per stage (HIP events on the engine stream, summed over
+    // chunks) and the LM passes executed (model + Jacobian, trial model).
+    double msAtrous = 0, msSegment = 0, msFit = 0, msAssemble = 0;
+    uint64_t jacPasses = 0, trialPasses = 0;
 };
 
 struct Options {
     bool fitFp64 = true;           // FP64 model/Jacobian (reference arithmetic) or FP32
     int maxIter = kMaxIterDefault; // Fitting::N_ITER
     size_t chunkPixels = size_t(64) << 20;
+    bool profile = false;           // stage timing with HIP events + LM pass counters
+    bool forceSequential = false;   // every frame through the exact sequential watershed
 };
 
 class Engine {
@@ -47,17 +53,23 @@ public:
                            uint64_t fit, const double* fitParams, double* d_out);
     void wavelet(const uint16_t* d_in, uint64_t planes, int H, int W, int level, double* d_out);
     // Label map of one frame after the watershed (labels = first-maximum raster index + 1).
-    void debug_labels(const uint16_t* d_frame, int H, int W, double thr, uint32_t* d_labels);
+    void debug_labels(const uint16_t* d_frame, int H, int W, double thr, double ratio, uint32_t* d_labels);
+    // AutoThreshold of one device frame (lib/Palm.cpp:122-172).
+    double autothreshold(const uint16_t* d_frame, int H, int W, const double* fitParams);
 
     DevBuf in, out, rows, rois;
 
 private:
     Engine();
     DetectBuffers b_{};
-    DevBuf P_, lab_, fin_, aux_, mask_, seed_, perFrame_, regions_, multi_, scratch_, fb_, counters_, flag_;
+    DevBuf P_, lab_, fin_, aux_, mask_, seed_, perFrame_, regions_, fz_, order_, multi_, scratch_, fb_, counters_, flag_;
+    DevBuf atMask_, atPart_, atRows_, passes_;
+    std::vector<hipEvent_t> ev_;
+    hipEvent_t event(size_t i);
     void prepare(size_t F, size_t npx);
     void zero_chunk(size_t F);
-    uint32_t detect_chunk(const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio, uint64_t maxPoints);
+    uint32_t detect_chunk(const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio, uint64_t maxPoints,
+                          size_t evBase);
     FitParams fit_params(uint64_t fit, const double* fp) const;
 };
 

@@ -94,6 +94,8 @@ void Engine::prepare(size_t F, size_t npx)
     perFrame_.ensure((F + 1) * 4 * 9);
     const size_t regionCap = npx / 4 + 2;  // roots are pairwise non-adjacent: at most ceil(H/2)*ceil(W/2)
     regions_.ensure(F * regionCap * sizeof(Region));
+    fz_.ensure(F * regionCap * sizeof(Fz));
+    order_.ensure(F * regionCap * 4);
     const size_t multiCap = n / 64 + 1024;
     multi_.ensure(multiCap * 8);
     const size_t scratchWords = n / 8 + 65536;
@@ -118,6 +120,8 @@ void Engine::prepare(size_t F, size_t npx)
     b_.frameFlag = pf + 5 * s;
     b_.roiBase = pf + 6 * s;
     b_.regions = regions_.as<Region>();
+    b_.fz = fz_.as<Fz>();
+    b_.order = order_.as<uint32_t>();
     b_.multiList = multi_.as<uint64_t>();
     b_.scratch = scratch_.as<uint64_t>();
     b_.fbScratch = fb_.as<uint32_t>();
@@ -131,11 +135,12 @@ void Engine::prepare(size_t F, size_t npx)
     b_.fbSlots = fbSlots;
 }
 
-void Engine::zero_chunk(size_t F)
+void Engine::zero_chunk(size_t)
 {
-    // Per-frame counters (6 arrays of F+1 words) and the chunk counters; a fill kernel rather than a
-    // memset so that the sequence can be captured in a graph.
-    fill_u32_kernel<<<64, 256, 0, stream>>>(b_.maskCnt, 6 * (F + 1), 0u);
+    // Per-frame counters (6 arrays laid out with a stride of frames + 1 words, frames being the
+    // chunk capacity, whatever the number of frames in this chunk) and the chunk counters; a fill
+    // kernel rather than a memset so that the sequence can be captured in a graph.
+    fill_u32_kernel<<<64, 256, 0, stream>>>(b_.maskCnt, 6 * (b_.frames + 1), 0u);
     fill_u32_kernel<<<1, 64, 0, stream>>>(b_.counters, kNumCounters, 0u);
     PT_CHECK(hipGetLastError());
 }
@@ -158,14 +163,27 @@ FitParams Engine::fit_params(uint64_t fit, const double* fp) const
     return p;
 }
 
-uint32_t Engine::detect_chunk(const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio, uint64_t maxPoints)
+hipEvent_t Engine::event(size_t i)
+{
+    while (ev_.size() <= i) {
+        hipEvent_t e;
+        PT_CHECK(hipEventCreate(&e));
+        ev_.push_back(e);
+    }
+    return ev_[i];
+}
+
+uint32_t Engine::detect_chunk(const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio, uint64_t maxPoints,
+                              size_t evBase)
 {
     const size_t npx = (size_t)W * H;
     zero_chunk(F);
+    if (opt.profile) PT_CHECK(hipEventRecord(event(evBase + 0), stream));
     launch_atrous_detect(d_frames, b_, W, H, (int)F, threshold_to_pmin(thr), stream);
-    if ((int)ratio > 0) throw Error("watershed fusion (watershedRatio >= 1) is not available in this build");
+    if (opt.profile) PT_CHECK(hipEventRecord(event(evBase + 1), stream));
     rois.ensure(F * (npx / 4 + 2) * sizeof(Roi));
-    launch_segment(b_, W, H, (int)F, maxPoints, rois.as<Roi>(), stream);
+    launch_segment(b_, W, H, (int)F, maxPoints, ratio, rois.as<Roi>(), stream, opt.forceSequential);
+    if (opt.profile) PT_CHECK(hipEventRecord(event(evBase + 2), stream));
     uint32_t tail[2] = {0, 0};
     PT_CHECK(hipMemcpyAsync(&tail[0], b_.roiBase + F, 4, hipMemcpyDeviceToHost, stream));
     PT_CHECK(hipMemcpyAsync(&tail[1], b_.counters + kStatus, 4, hipMemcpyDeviceToHost, stream));
@@ -190,20 +208,46 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     prepare(F, npx);
     const FitParams fp = fit_params(fit, fitParams);
     uint64_t rowsDone = 0;
+    unsigned long long* passes = nullptr;
+    if (opt.profile) {
+        passes_.ensure(16);
+        passes = passes_.as<unsigned long long>();
+        PT_CHECK(hipMemsetAsync(passes, 0, 16, stream));
+    }
+    std::vector<uint8_t> fitted;
     for (uint64_t z0 = 0; z0 < planes; z0 += F) {
         const size_t f = std::min<uint64_t>(F, planes - z0);
         const uint16_t* frames = d_in + z0 * npx;
-        const uint32_t n = detect_chunk(frames, f, H, W, thr, ratio, maxPerPlane);
+        const size_t eb = 6 * last.chunks;
+        const uint32_t n = detect_chunk(frames, f, H, W, thr, ratio, maxPerPlane, eb);
         if (rowsDone + n > outCapRows) throw Error("output capacity exceeded");
         rows.ensure((size_t)std::max<uint32_t>(n, 1) * kCols * 8);
-        if (n) {
-            launch_fit(frames, npx, W, H, rois.as<Roi>(), nullptr, n, fp, rows.as<double>(), opt.fitFp64, stream);
-            launch_assemble(rows.as<double>(), b_.roiBase, (int)f, rowsDone, z0, d_out + rowsDone * kCols, stream);
-        }
+        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 3), stream));
+        if (n) launch_fit(frames, npx, W, H, rois.as<Roi>(), nullptr, n, fp, rows.as<double>(), opt.fitFp64, stream, passes);
+        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 4), stream));
+        if (n) launch_assemble(rows.as<double>(), b_.roiBase, (int)f, rowsDone, z0, d_out + rowsDone * kCols, stream);
+        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 5), stream));
         rowsDone += n;
         last.chunks++;
     }
     PT_CHECK(hipStreamSynchronize(stream));
+    if (opt.profile) {
+        for (size_t c = 0; c < last.chunks; ++c) {
+            float ms[4] = {0, 0, 0, 0};
+            PT_CHECK(hipEventElapsedTime(&ms[0], event(6 * c + 0), event(6 * c + 1)));
+            PT_CHECK(hipEventElapsedTime(&ms[1], event(6 * c + 1), event(6 * c + 2)));
+            PT_CHECK(hipEventElapsedTime(&ms[2], event(6 * c + 3), event(6 * c + 4)));
+            PT_CHECK(hipEventElapsedTime(&ms[3], event(6 * c + 4), event(6 * c + 5)));
+            last.msAtrous += ms[0];
+            last.msSegment += ms[1];
+            last.msFit += ms[2];
+            last.msAssemble += ms[3];
+        }
+        unsigned long long pc[2];
+        PT_CHECK(hipMemcpy(pc, passes, 16, hipMemcpyDeviceToHost));
+        last.jacPasses = pc[0];
+        last.trialPasses = pc[1];
+    }
     last.frames = planes;
     last.rois = rowsDone;
     return rowsDone * kCols;
@@ -255,14 +299,14 @@ void Engine::wavelet(const uint16_t* d_in, uint64_t planes, int H, int W, int le
     PT_CHECK(hipStreamSynchronize(stream));
 }
 
-void Engine::debug_labels(const uint16_t* d_frame, int H, int W, double thr, uint32_t* d_labels)
+void Engine::debug_labels(const uint16_t* d_frame, int H, int W, double thr, double ratio, uint32_t* d_labels)
 {
     const size_t npx = (size_t)W * H;
     prepare(1, npx);
     zero_chunk(1);
     launch_atrous_detect(d_frame, b_, W, H, 1, threshold_to_pmin(thr), stream);
     rois.ensure((npx / 4 + 2) * sizeof(Roi));
-    launch_segment(b_, W, H, 1, ~0ull, rois.as<Roi>(), stream);
+    launch_segment(b_, W, H, 1, ~0ull, ratio, rois.as<Roi>(), stream, opt.forceSequential);
     // Labels only where P > 0 (fin is not written elsewhere).
     masked_labels<<<256, 256, 0, stream>>>(b_.fin, b_.P, d_labels, npx);
     PT_CHECK(hipGetLastError());

@@ -5,6 +5,7 @@
 #include "engine.h"
 #include "../../include/palmtracer.h"
 
+#include <cmath>
 #include <cstdio>
 #include <dlfcn.h>
 
@@ -106,6 +107,27 @@ PT_API void PT_CALL Wavelett(const uint16_t* input, double* output, uint64_t hei
     }, 0);
 }
 
+PT_API double PT_CALL AutoThreshold(const uint16_t* input, uint64_t height, uint64_t width, const double* fit_params)
+{
+    return guarded("AutoThreshold", [&]() -> double {
+        Engine& e = Engine::get();
+        std::lock_guard<std::mutex> lk(e.mu);
+        if (height == 0 || width == 0) return std::nan("");
+        HostToDevice h2d(e.in, input, (size_t)height * width, e.stream);
+        return e.autothreshold(h2d.dptr, (int)height, (int)width, fit_params);
+    }, std::nan(""));
+}
+
+PT_API double PT_CALL pt_autothreshold_device(const uint16_t* d_input, uint64_t height, uint64_t width,
+                                              const double* fit_params)
+{
+    return guarded("pt_autothreshold_device", [&]() -> double {
+        Engine& e = Engine::get();
+        std::lock_guard<std::mutex> lk(e.mu);
+        return e.autothreshold(d_input, (int)height, (int)width, fit_params);
+    }, std::nan(""));
+}
+
 // ----- device-pointer entry points (no counterpart in the reference) -----
 
 PT_API uint64_t PT_CALL pt_localization_device(const uint16_t* d_input, double* d_output, uint64_t outputCapDoubles,
@@ -134,7 +156,7 @@ PT_API int PT_CALL pt_wavelet_device(const uint16_t* d_input, double* d_output, 
 }
 
 PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint64_t width, double thresholdVal,
-                                   uint32_t* labels)
+                                   double watershedRatio, uint32_t* labels)
 {
     return guarded("pt_debug_labels", [&]() -> int {
         Engine& e = Engine::get();
@@ -142,7 +164,7 @@ PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint6
         const size_t npx = (size_t)height * width;
         HostToDevice h2d(e.in, image, npx, e.stream);
         e.out.ensure(npx * 4);
-        e.debug_labels(h2d.dptr, (int)height, (int)width, thresholdVal, e.out.as<uint32_t>());
+        e.debug_labels(h2d.dptr, (int)height, (int)width, thresholdVal, watershedRatio, e.out.as<uint32_t>());
         PT_CHECK(hipMemcpyAsync(labels, e.out.p, npx * 4, hipMemcpyDeviceToHost, e.stream));
         PT_CHECK(hipStreamSynchronize(e.stream));
         return 1;
@@ -158,6 +180,8 @@ PT_API int PT_CALL pt_set_option(const char* name, double value)
         if (n == "fit_fp64") e.opt.fitFp64 = value != 0.0;
         else if (n == "max_iter") e.opt.maxIter = std::max(1, (int)value);
         else if (n == "chunk_pixels") e.opt.chunkPixels = (size_t)std::max(1.0, value);
+        else if (n == "profile") e.opt.profile = value != 0.0;
+        else if (n == "force_sequential_watershed") e.opt.forceSequential = value != 0.0;
         else throw pt::Error("unknown option '" + n + "'");
         return 1;
     }, 0);
@@ -168,6 +192,15 @@ PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n)
     Engine& e = Engine::get();
     const uint64_t v[7] = {e.last.frames, e.last.chunks, e.last.rois, e.last.flaggedFrames, e.last.multiComponents,
                            e.last.maskPixels, e.last.status};
+    for (uint64_t i = 0; i < n && i < 7; ++i) out[i] = v[i];
+    return 7;
+}
+
+PT_API int PT_CALL pt_stage_times(double* out, uint64_t n)
+{
+    Engine& e = Engine::get();
+    const double v[7] = {e.last.msAtrous, e.last.msSegment, e.last.msFit, e.last.msAssemble, (double)e.last.chunks,
+                         (double)e.last.jacPasses, (double)e.last.trialPasses};
     for (uint64_t i = 0; i < n && i < 7; ++i) out[i] = v[i];
     return 7;
 }

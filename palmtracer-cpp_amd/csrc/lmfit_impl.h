@@ -1,4 +1,4 @@
-// Per-ROI Gaussian Levenberg-Marquardt fit on gfx950.
+// Per-ROI Gaussian Levenberg-Marquardt fit on gfx950 (templates; instantiated by lmfit_f32.hip and lmfit_f64.hip).
 //
 // Reference: lib/fit/GaussianFit.cpp:48-295 (Fit, lm, gain, getError, computeR2),
 // lib/fit/GaussianModel.cpp:19-52 (Evaluate), lib/fit/Fitting.cpp:61-175 (fill, preCompute, initP),
@@ -11,6 +11,7 @@
 // (Gauss-Jordan with row equilibration and partial pivoting, as the reference), the gain ratio and
 // R^2 stay in FP64. The control flow (lambda schedule, clamps, stopping rules, sentinels) is the
 // reference's, step for step.
+#pragma once
 #include "pt_common.h"
 #include "pt_kernels.h"
 
@@ -263,7 +264,8 @@ __device__ inline double lm_r2(const Patch& pa, const double* p, double sigma, d
 }
 
 template <typename T, int MODE>
-__device__ void fit_one(const uint16_t* img, int W, int H, const Roi& r, const FitParams& fp, double* row, uint32_t* iters)
+__device__ void fit_one(const uint16_t* img, int W, int H, const Roi& r, const FitParams& fp, double* row, uint32_t& nJac,
+                        uint32_t& nTrial)
 {
     constexpr int NP = NPar<MODE>::v;
     const int roi = fp.roi, half = roi / 2;
@@ -297,6 +299,7 @@ __device__ void fit_one(const uint16_t* img, int W, int H, const Roi& r, const F
 
     T al[NP][NP], be[NP];
     double err = (double)lm_pass<T, MODE>(pa, p, fp.sigmaX, fp.theta, al, be);
+    ++nJac;
     bool bad = !isfinite(err);
 #pragma unroll
     for (int a = 0; a < NP; ++a) bad |= !isfinite((double)be[a]);
@@ -322,9 +325,7 @@ __device__ void fit_one(const uint16_t* img, int W, int H, const Roi& r, const F
     };
     double lambda = 1e-2;
     double m = max_abs_beta();
-    uint32_t nIt = 0;
     for (int it = (m < 1e-6) ? fp.maxIter : 0; it < fp.maxIter; ++it) {
-        ++nIt;
         double A[NP][NP], Ai[NP][NP];
 #pragma unroll
         for (int a = 0; a < NP; ++a)
@@ -335,7 +336,6 @@ __device__ void fit_one(const uint16_t* img, int W, int H, const Roi& r, const F
         if (!gauss_jordan<NP>(A, Ai)) {
             if (lambda < 1e7) { lambda = fmin(lambda * 11, 1e7); continue; }
             fill_row(MODE, row, r.x, r.y, zero, r.seg, r.surf, -1, -1, -1, -1, -1, -1, -1);
-            if (iters) *iters = nIt;
             return;
         }
         bool nonfinite = false;
@@ -351,7 +351,6 @@ __device__ void fit_one(const uint16_t* img, int W, int H, const Roi& r, const F
         }
         if (nonfinite) {
             fill_row(MODE, row, r.x, r.y, zero, r.seg, r.surf, -1, -1, -1, -1, -1, -1, -1);
-            if (iters) *iters = nIt;
             return;
         }
         bool tiny = true;
@@ -384,11 +383,13 @@ __device__ void fit_one(const uint16_t* img, int W, int H, const Roi& r, const F
 #pragma unroll
         for (int a = 0; a < NP; ++a) m = m + dP[a] * ((lambda * dP[a]) + (double)be[a]);
         const double errTry = (double)lm_error<T, MODE>(pa, pTry, fp.sigmaX, fp.theta);
+        ++nTrial;
         const double rho = (err - errTry) / (2.0 * m);
         if (rho > 1e-2) {
 #pragma unroll
             for (int a = 0; a < NP; ++a) p[a] = pTry[a];
             err = (double)lm_pass<T, MODE>(pa, p, fp.sigmaX, fp.theta, al, be);
+            ++nJac;
             lambda = fmax(lambda / 9, 1e-7);
         } else {
             if (lambda >= 1e7) break;
@@ -401,7 +402,6 @@ __device__ void fit_one(const uint16_t* img, int W, int H, const Roi& r, const F
             if (max_abs_beta() < 1e-6) break;
         }
     }
-    if (iters) *iters = nIt;
     const double off = p[0], inten = p[1], x = p[2], y = p[3];
     const double rss = lm_r2<T, MODE>(pa, p, fp.sigmaX, fp.theta);
     if (rss < kEpsilon) {
@@ -420,45 +420,50 @@ template <typename T, int MODE>
 __global__ void __launch_bounds__(64) fit_kernel(const uint16_t* __restrict__ frames, size_t npx, int W, int H,
                                                  const Roi* __restrict__ rois, const uint32_t* __restrict__ count,
                                                  uint32_t nHost, FitParams fp, double* __restrict__ rows,
-                                                 uint32_t* __restrict__ iters)
+                                                 unsigned long long* __restrict__ passes)
 {
     const uint32_t n = count ? *count : nHost;
+    uint32_t nJac = 0, nTrial = 0;
     for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i < n; i += gridDim.x * 64) {
         const Roi r = rois[i];
         double row[kCols];
         if constexpr (MODE == 0) {
             fill_row(0, row, r.x, r.y, 0.0, r.seg, r.surf, 0, 0, -1, -1, -1, -1, -1);
         } else {
-            fit_one<T, MODE == 0 ? 1 : MODE>(frames + (size_t)r.frame * npx, W, H, r, fp, row, iters ? iters + i : nullptr);
+            fit_one<T, MODE == 0 ? 1 : MODE>(frames + (size_t)r.frame * npx, W, H, r, fp, row, nJac, nTrial);
         }
         double* o = rows + (size_t)i * kCols;
 #pragma unroll
         for (int c = 0; c < kCols; ++c) o[c] = row[c];
     }
+    if (passes) {
+        // Passes over the ROI actually executed (model + Jacobian, and trial model), for the FLOP count.
+        unsigned long long a = nJac, b = nTrial;
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o);
+            b += __shfl_xor(b, o);
+        }
+        if (threadIdx.x == 0) {
+            atomicAdd(&passes[0], a);
+            atomicAdd(&passes[1], b);
+        }
+    }
 }
 
 template <typename T>
 void launch_fit_t(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, const uint32_t* d_count,
-                  uint32_t nHost, const FitParams& fp, double* d_rows, uint32_t* d_iters, hipStream_t s)
+                  uint32_t nHost, const FitParams& fp, double* d_rows, unsigned long long* d_passes, hipStream_t s)
 {
     const unsigned grid = d_count ? 4096u : std::max(1u, std::min(65535u, ceil_div(nHost, 64)));
     switch (fp.mode) {
-    case 1: fit_kernel<T, 1><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_iters); break;
-    case 2: fit_kernel<T, 2><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_iters); break;
-    case 3: fit_kernel<T, 3><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_iters); break;
-    case 4: fit_kernel<T, 4><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_iters); break;
-    default: fit_kernel<T, 0><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_iters); break;
+    case 1: fit_kernel<T, 1><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_passes); break;
+    case 2: fit_kernel<T, 2><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_passes); break;
+    case 3: fit_kernel<T, 3><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_passes); break;
+    case 4: fit_kernel<T, 4><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_passes); break;
+    default: fit_kernel<T, 0><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_passes); break;
     }
     PT_CHECK(hipGetLastError());
 }
 
 } // namespace
-
-void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, const uint32_t* d_count,
-                uint32_t nHost, const FitParams& fp, double* d_rows, bool fp64, hipStream_t s, uint32_t* d_iters)
-{
-    if (fp64) launch_fit_t<double>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_iters, s);
-    else launch_fit_t<float>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_iters, s);
-}
-
 } // namespace pt

@@ -20,6 +20,20 @@ struct Roi {
     uint32_t slot;     // output row position within the frame (before the (Y, X) sort)
 };
 
+// Per-region state of the basin fusion (Segmentation2D.cpp:181-291, WSFusion), by region slot.
+// Keys pack (P << 32) | (0xFFFFFFFF - pixel) so that an atomic maximum keeps the first pixel in
+// raster order among equal potentials, as the reference's strict '>' updates do.
+struct Fz {
+    uint64_t maxKey;   // maximum potential and its first pixel
+    uint64_t nbKey;    // maximum potential among pixels touching another region, and its first pixel
+    uint32_t minP;     // minimum potential
+    uint32_t target;   // slot this region merges into (itself if none)
+    uint32_t locked;
+    uint32_t alive;
+    uint32_t nbLabel;  // label of the first other-region neighbour of the nbKey pixel (0: none)
+    uint32_t pad;
+};
+
 // Per-pixel side record, used sparsely: union-find component data at component roots and the
 // region slot at root seeds.
 struct PixAux {
@@ -45,6 +59,8 @@ struct DetectBuffers {
     uint32_t* roiBase = nullptr;  // [F+1] exclusive scan of keep
     uint32_t* frameFlag = nullptr;// [F] frame needs the exact sequential watershed
     Region* regions = nullptr;    // F * regionCap
+    Fz* fz = nullptr;             // F * regionCap (basin fusion)
+    uint32_t* order = nullptr;    // F * regionCap: slots in label order (basin fusion)
     uint64_t* multiList = nullptr;// (frame << 32 | component root) of multi-label components
     uint64_t* scratch = nullptr;  // flood work space (u64 words)
     uint32_t* fbScratch = nullptr;// exact-fallback work space: fbSlots * fb_slot_words(npx) u32
@@ -67,8 +83,9 @@ int64_t threshold_to_pmin(double thr);
 
 // Segmentation (ratio <= 0: watershed without fusion). Produces ROIs for frames [0, F) at
 // d_rois + roiBase[f], with per-frame truncation to maxPoints in label order.
-void launch_segment(const DetectBuffers& b, int W, int H, int frames, uint64_t maxPoints, Roi* d_rois,
-                    hipStream_t s);
+// forceSequential sends every frame through the exact sequential watershed.
+void launch_segment(const DetectBuffers& b, int W, int H, int frames, uint64_t maxPoints, double ratio, Roi* d_rois,
+                    hipStream_t s, bool forceSequential = false);
 
 // Gaussian LM fit of nRoi ROIs (count read from device memory *d_count when non-null).
 struct FitParams {
@@ -79,7 +96,7 @@ struct FitParams {
 };
 void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, const uint32_t* d_count,
                 uint32_t nRoiHost, const FitParams& fp, double* d_rows, bool fp64, hipStream_t s,
-                uint32_t* d_iters = nullptr);
+                unsigned long long* d_passes = nullptr);
 
 // Per-frame (Y, X) sort and row assembly into the caller's layout.
 void launch_assemble(const double* d_rows, const uint32_t* d_roiBase, int frames, uint64_t idBase, uint64_t planeBase,

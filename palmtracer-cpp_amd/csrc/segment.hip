@@ -241,7 +241,8 @@ __global__ void __launch_bounds__(NT) label_assign_kernel(const uint32_t* __rest
                                                           const uint32_t* __restrict__ maskList, const uint32_t* __restrict__ maskCnt,
                                                           uint64_t* __restrict__ multiList, size_t multiCap,
                                                           uint32_t* __restrict__ counters, uint32_t* __restrict__ frameFlag,
-                                                          uint32_t* __restrict__ flagList, int W, int H, size_t npx)
+                                                          uint32_t* __restrict__ flagList, int W, int H, size_t npx,
+                                                          int pass2)
 {
     const int f = blockIdx.y;
     const uint32_t cnt = maskCnt[f];
@@ -252,7 +253,8 @@ __global__ void __launch_bounds__(NT) label_assign_kernel(const uint32_t* __rest
         const PixAux a = aux[(size_t)f * npx + c];
         if (a.cmin == kNone) { flag_frame(frameFlag, counters, flagList, f); continue; }  // cannot happen
         if (a.cmin == a.cmax) { fin[(size_t)f * npx + p] = a.cmin; continue; }
-        if (!is_local_max(Pf, W, H, p)) fin[(size_t)f * npx + p] = 0u;
+        // Several labels: keep the seeds' labels only (second pass: fin already holds seeds only).
+        if (!pass2 && !is_local_max(Pf, W, H, p)) fin[(size_t)f * npx + p] = 0u;
         if (p == c) {
             const uint32_t j = atomicAdd(&counters[kMultiCnt], 1u);
             if (j < multiCap) multiList[j] = ((uint64_t)f << 32) | c;
@@ -376,7 +378,9 @@ __global__ void __launch_bounds__(64) flood_kernel(const uint32_t* __restrict__ 
 __global__ void __launch_bounds__(NT) fallback_kernel(const uint32_t* __restrict__ P, uint32_t* __restrict__ fin,
                                                       const uint32_t* __restrict__ maskList, const uint32_t* __restrict__ maskCnt,
                                                       const uint32_t* __restrict__ flagList, uint32_t* __restrict__ counters,
-                                                      uint32_t* __restrict__ fbScratch, int W, int H, size_t npx)
+                                                      uint32_t* __restrict__ fbScratch, int W, int H, size_t npx,
+                                                      const Fz* __restrict__ fz, const PixAux* __restrict__ aux,
+                                                      size_t regionCap)
 {
     __shared__ uint32_t sFrame, sWave[NT / 64], sTotal;
     const size_t npx2 = (npx + 1) & ~size_t(1);
@@ -393,11 +397,24 @@ __global__ void __launch_bounds__(NT) fallback_kernel(const uint32_t* __restrict
         if (f == kNone) return;
         const uint32_t* Pf = P + (size_t)f * npx;
         uint32_t* fn = fin + (size_t)f * npx;
-        // 1. Labels back to getLocalMax's state: maxima keep theirs, other mask pixels 0.
+        // 1. Labels back to the seed state: local maxima (first pass) or the one maximum pixel per
+        //    surviving region (after fusion) keep theirs, every other mask pixel 0.
         const uint32_t cnt = maskCnt[f];
         for (uint32_t k = threadIdx.x; k < cnt; k += NT) {
             const uint32_t p = maskList[(size_t)f * npx + k];
-            if (!is_local_max(Pf, W, H, (int)p)) fn[p] = 0u;
+            bool seed;
+            if (!fz) {
+                seed = is_local_max(Pf, W, H, (int)p);
+            } else {
+                const uint32_t L = fn[p];
+                seed = false;
+                if (L != 0 && L <= npx) {
+                    const uint32_t sl = aux[(size_t)f * npx + (L - 1)].rslot;
+                    seed = sl < regionCap && fz[(size_t)f * regionCap + sl].alive &&
+                           0xFFFFFFFFu - (uint32_t)fz[(size_t)f * regionCap + sl].maxKey == p;
+                }
+            }
+            if (!seed) fn[p] = 0u;
         }
         __syncthreads();
         // 2. Seeds to push, in raster order (block-wide ordered compaction).
@@ -448,6 +465,189 @@ __global__ void __launch_bounds__(NT) fallback_kernel(const uint32_t* __restrict
             }
         }
         __syncthreads();
+    }
+}
+
+
+// ---------------------------------------------------------------------------------------------
+// Basin fusion (Segmentation2D.cpp:181-291), one workgroup per frame. Regions are identified by
+// their first local maximum; label order is the order of those pixels, as in the reference.
+__device__ inline uint64_t pkey(uint32_t v, uint32_t p) { return ((uint64_t)v << 32) | (0xFFFFFFFFu - p); }
+__device__ inline uint32_t kpix(uint64_t k) { return 0xFFFFFFFFu - (uint32_t)k; }
+
+__global__ void __launch_bounds__(NT) fusion_kernel(const uint32_t* __restrict__ P, uint32_t* __restrict__ fin,
+                                                    const PixAux* __restrict__ aux, const uint32_t* __restrict__ maskList,
+                                                    const uint32_t* __restrict__ maskCnt, const Region* __restrict__ regions,
+                                                    size_t regionCap, const uint32_t* __restrict__ rootCnt, Fz* __restrict__ fz,
+                                                    uint32_t* __restrict__ order, double ratio, int W, int H, size_t npx)
+{
+    __shared__ uint32_t sChange;
+    const int f = blockIdx.x;
+    const uint32_t n = (uint32_t)min((size_t)rootCnt[f], regionCap);
+    if (n == 0) return;
+    const size_t base = (size_t)f * regionCap;
+    const uint32_t* Pf = P + (size_t)f * npx;
+    uint32_t* fn = fin + (size_t)f * npx;
+    const PixAux* ax = aux + (size_t)f * npx;
+    Fz* z = fz + base;
+    const Region* rg = regions + base;
+    uint32_t* ord = order + base;
+    const uint32_t cnt = maskCnt[f];
+    const uint32_t* ml = maskList + (size_t)f * npx;
+    for (uint32_t s = threadIdx.x; s < n; s += NT) {
+        z[s].alive = 1u;
+        uint32_t r = 0;
+        const uint32_t root = rg[s].root;
+        for (uint32_t t = 0; t < n; ++t) r += rg[t].root < root;
+        ord[r] = s;
+    }
+    __syncthreads();
+    auto slot_of = [&](uint32_t L) -> uint32_t { return (L != 0 && L <= npx) ? ax[L - 1].rslot : kNone; };
+    while (true) {
+        for (uint32_t s = threadIdx.x; s < n; s += NT)
+            if (z[s].alive) {
+                z[s].maxKey = 0;
+                z[s].nbKey = 0;
+                z[s].minP = kNone;
+                z[s].target = s;
+                z[s].locked = 0;
+                z[s].nbLabel = 0;
+            }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < cnt; k += NT) {
+            const uint32_t p = ml[k];
+            const uint32_t L = fn[p], s = slot_of(L);
+            if (s >= n) continue;
+            const uint32_t v = Pf[p];
+            atomicMin(&z[s].minP, v);
+            atomicMax((unsigned long long*)&z[s].maxKey, (unsigned long long)pkey(v, p));
+            const int y = p / W, x = p - y * W;
+            bool bnd = false;
+            for (int dy = -1; dy <= 1 && !bnd; ++dy) {
+                const int yy = y + dy;
+                if (yy < 0 || yy >= H) continue;
+                for (int dx = -1; dx <= 1 && !bnd; ++dx) {
+                    const int xx = x + dx;
+                    if ((dx | dy) == 0 || xx < 0 || xx >= W) continue;
+                    const int q = yy * W + xx;
+                    bnd = Pf[q] != 0 && fn[q] != 0 && fn[q] != L;
+                }
+            }
+            if (bnd) atomicMax((unsigned long long*)&z[s].nbKey, (unsigned long long)pkey(v, p));
+        }
+        __syncthreads();
+        // Neighbour label: first other-region neighbour, in the reference's scan order, of the first
+        // pixel that carries the region's largest boundary potential.
+        for (uint32_t s = threadIdx.x; s < n; s += NT) {
+            if (!z[s].alive || z[s].nbKey == 0) continue;
+            const uint32_t p = kpix(z[s].nbKey), L = fn[p];
+            const int y = p / W, x = p - y * W;
+            uint32_t nb = 0;
+            for (int dy = -1; dy <= 1 && !nb; ++dy) {
+                const int yy = y + dy;
+                if (yy < 0 || yy >= H) continue;
+                for (int dx = -1; dx <= 1 && !nb; ++dx) {
+                    const int xx = x + dx;
+                    if ((dx | dy) == 0 || xx < 0 || xx >= W) continue;
+                    const int q = yy * W + xx;
+                    if (Pf[q] != 0 && fn[q] != 0 && fn[q] != L) nb = fn[q];
+                }
+            }
+            z[s].nbLabel = nb;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t change = 0;
+            for (uint32_t r = 0; r < n; ++r) {
+                const uint32_t s = ord[r];
+                if (!z[s].alive || z[s].nbLabel == 0) continue;
+                const uint32_t t = slot_of(z[s].nbLabel);
+                if (t >= n || z[s].locked || z[t].locked) continue;
+                const uint32_t A = (uint32_t)(z[s].maxKey >> 32), m = z[s].minP, B = (uint32_t)(z[s].nbKey >> 32);
+                if (A == m) continue;                            // |Delta| <= DBL_EPSILON
+                const double rr = (double)(A - B) / (double)(A - m);
+                if (rr < ratio && A < (uint32_t)(z[t].maxKey >> 32)) {
+                    z[s].target = t;
+                    z[t].locked = 1;
+                    z[s].locked = 1;
+                    change = 1;
+                }
+            }
+            sChange = change;
+        }
+        __syncthreads();
+        if (!sChange) break;
+        for (uint32_t k = threadIdx.x; k < cnt; k += NT) {
+            const uint32_t p = ml[k];
+            const uint32_t s = slot_of(fn[p]);
+            if (s >= n) continue;
+            const uint32_t t = z[s].target;
+            if (t != s) fn[p] = rg[t].root + 1;
+        }
+        __syncthreads();
+        for (uint32_t s = threadIdx.x; s < n; s += NT)
+            if (z[s].alive && z[s].target != s) z[s].alive = 0;
+        __syncthreads();
+    }
+    // One seed per surviving region, at its first maximum pixel, for the second watershed.
+    for (uint32_t k = threadIdx.x; k < cnt; k += NT) fn[ml[k]] = 0u;
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < n; s += NT)
+        if (z[s].alive && z[s].maxKey != 0) fn[kpix(z[s].maxKey)] = rg[s].root + 1;
+}
+
+// Second watershed: clean component roots (the flood marked visited pixels) and reset the
+// per-component label range, then register the surviving seeds.
+__global__ void __launch_bounds__(NT) pass2_reset_kernel(uint32_t* __restrict__ lab, PixAux* __restrict__ aux,
+                                                         const uint32_t* __restrict__ maskList,
+                                                         const uint32_t* __restrict__ maskCnt, size_t npx)
+{
+    const int f = blockIdx.y;
+    const uint32_t cnt = maskCnt[f];
+    for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < cnt; k += gridDim.x * NT) {
+        const uint32_t p = maskList[(size_t)f * npx + k];
+        lab[(size_t)f * npx + p] &= ~kVisited;
+        aux[(size_t)f * npx + p].cmin = kNone;
+        aux[(size_t)f * npx + p].cmax = 0u;
+    }
+}
+
+__global__ void __launch_bounds__(NT) pass2_seed_kernel(const uint32_t* __restrict__ lab, PixAux* __restrict__ aux,
+                                                        const Fz* __restrict__ fz, const Region* __restrict__ regions,
+                                                        const uint32_t* __restrict__ rootCnt, size_t regionCap, size_t npx)
+{
+    const int f = blockIdx.y;
+    const uint32_t n = (uint32_t)min((size_t)rootCnt[f], regionCap);
+    for (uint32_t s = blockIdx.x * NT + threadIdx.x; s < n; s += gridDim.x * NT) {
+        const Fz z = fz[(size_t)f * regionCap + s];
+        if (!z.alive || z.maxKey == 0) continue;
+        const uint32_t p = kpix(z.maxKey), L = regions[(size_t)f * regionCap + s].root + 1;
+        const uint32_t c = lab[(size_t)f * npx + p];
+        atomicMin(&aux[(size_t)f * npx + c].cmin, L);
+        atomicMax(&aux[(size_t)f * npx + c].cmax, L);
+    }
+}
+
+// Every frame through the exact sequential watershed (verification of that path).
+__global__ void force_flag_kernel(uint32_t* __restrict__ counters, uint32_t* __restrict__ frameFlag,
+                                  uint32_t* __restrict__ flagList, int F)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < F; i += gridDim.x * blockDim.x) {
+        frameFlag[i] = 1u;
+        flagList[i] = (uint32_t)i;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[kFlagCnt] = (uint32_t)F;
+}
+
+__global__ void reset_pass_kernel(uint32_t* __restrict__ counters, uint32_t* __restrict__ frameFlag, int F)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < F; i += gridDim.x * blockDim.x) frameFlag[i] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counters[kMultiCnt] = 0u;
+        counters[kScratchUsed] = 0u;
+        counters[kScratchUsed + 1] = 0u;
+        counters[kFlagCnt] = 0u;
+        counters[kFbNext] = 0u;
     }
 }
 
@@ -614,7 +814,8 @@ void launch_exclusive_scan(const uint32_t* d_in, uint32_t* d_out, int n, hipStre
     PT_CHECK(hipGetLastError());
 }
 
-void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoints, Roi* d_rois, hipStream_t s)
+void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoints, double ratio, Roi* d_rois,
+                    hipStream_t s, bool forceSequential)
 {
     const size_t npx = b.npx;
     const unsigned bpf = (unsigned)std::min<size_t>(64, std::max<size_t>(1, npx / 16384));
@@ -623,13 +824,25 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
     cc_flatten_kernel<<<g, NT, 0, s>>>(b.lab, b.maskList, b.maskCnt, b.aux, npx);
     seed_label_kernel<<<g, NT, 0, s>>>(b.P, b.lab, b.fin, b.aux, b.seedList, b.seedCnt, b.rootCnt, b.regions,
                                        b.regionCap, b.counters, W, H, npx);
-    label_assign_kernel<<<g, NT, 0, s>>>(b.P, b.lab, b.fin, b.aux, b.maskList, b.maskCnt, b.multiList, b.multiCap,
-                                         b.counters, b.frameFlag, b.flagList, W, H, npx);
-    flood_kernel<<<2048, 64, 0, s>>>(b.P, b.lab, b.fin, b.aux, b.maskCnt, b.multiList, b.multiCap,
-                                                         b.scratch, b.scratchWords, b.counters, b.frameFlag, b.flagList,
-                                                         W, H, npx);
-    fallback_kernel<<<(unsigned)b.fbSlots, NT, 0, s>>>(b.P, b.fin, b.maskList, b.maskCnt, b.flagList, b.counters,
-                                                        b.fbScratch, W, H, npx);
+    auto watershed = [&](int pass2) {
+        if (forceSequential) force_flag_kernel<<<ceil_div(F, 256), 256, 0, s>>>(b.counters, b.frameFlag, b.flagList, F);
+        label_assign_kernel<<<g, NT, 0, s>>>(b.P, b.lab, b.fin, b.aux, b.maskList, b.maskCnt, b.multiList, b.multiCap,
+                                             b.counters, b.frameFlag, b.flagList, W, H, npx, pass2);
+        flood_kernel<<<2048, 64, 0, s>>>(b.P, b.lab, b.fin, b.aux, b.maskCnt, b.multiList, b.multiCap, b.scratch,
+                                         b.scratchWords, b.counters, b.frameFlag, b.flagList, W, H, npx);
+        fallback_kernel<<<(unsigned)b.fbSlots, NT, 0, s>>>(b.P, b.fin, b.maskList, b.maskCnt, b.flagList, b.counters,
+                                                            b.fbScratch, W, H, npx, pass2 ? b.fz : nullptr, b.aux,
+                                                            b.regionCap);
+    };
+    watershed(0);
+    if ((int)ratio > 0) {   // Segmentation2D.cpp:82-84
+        fusion_kernel<<<F, NT, 0, s>>>(b.P, b.fin, b.aux, b.maskList, b.maskCnt, b.regions, b.regionCap, b.rootCnt,
+                                       b.fz, b.order, ratio, W, H, npx);
+        pass2_reset_kernel<<<g, NT, 0, s>>>(b.lab, b.aux, b.maskList, b.maskCnt, npx);
+        pass2_seed_kernel<<<g, NT, 0, s>>>(b.lab, b.aux, b.fz, b.regions, b.rootCnt, b.regionCap, npx);
+        reset_pass_kernel<<<ceil_div(F, 256), 256, 0, s>>>(b.counters, b.frameFlag, F);
+        watershed(1);
+    }
     region_stats_kernel<<<g, NT, 0, s>>>(b.P, b.fin, b.aux, b.maskList, b.maskCnt, b.regions, b.regionCap, b.counters,
                                          W, npx);
     region_count_kernel<<<F, NT, 0, s>>>(b.regions, b.regionCap, b.rootCnt, b.found, b.keep, maxPoints);

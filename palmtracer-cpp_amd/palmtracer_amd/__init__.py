@@ -43,9 +43,15 @@ def lib():
         L.pt_wavelet_device.restype = _i32
         L.pt_wavelet_device.argtypes = [_vp, _vp, _u64, _u64, _u64, _u64]
         L.pt_debug_labels.restype = _i32
-        L.pt_debug_labels.argtypes = [_vp, _u64, _u64, _f64, _vp]
+        L.pt_debug_labels.argtypes = [_vp, _u64, _u64, _f64, _f64, _vp]
+        L.AutoThreshold.restype = _f64
+        L.AutoThreshold.argtypes = [_vp, _u64, _u64, _vp]
+        L.pt_autothreshold_device.restype = _f64
+        L.pt_autothreshold_device.argtypes = [_vp, _u64, _u64, _vp]
         L.pt_set_option.restype = _i32
         L.pt_set_option.argtypes = [ctypes.c_char_p, _f64]
+        L.pt_stage_times.restype = _i32
+        L.pt_stage_times.argtypes = [_vp, _u64]
         L.pt_last_stats.restype = _i32
         L.pt_last_stats.argtypes = [_vp, _u64]
         L.pt_synth_device.restype = _i32
@@ -110,11 +116,25 @@ def wavelett(stack, level=2):
     return out
 
 
-def debug_labels(frame, threshold):
+def autothreshold(frame, fit_params=(7,)):
+    """AutoThreshold (CPU/main.cpp:75-85) of one plane."""
+    f = np.ascontiguousarray(frame, np.uint16)
+    h, w = f.shape
+    p = _params(fit_params)
+    return lib().AutoThreshold(f.ctypes.data, h, w, p.ctypes.data)
+
+
+def autothreshold_device(d_frame, fit_params=(7,)):
+    h, w = d_frame.shape[-2:]
+    p = _params(fit_params)
+    return lib().pt_autothreshold_device(d_frame.data_ptr(), h, w, p.ctypes.data)
+
+
+def debug_labels(frame, threshold, watershed_ratio=0.0):
     f = np.ascontiguousarray(frame, np.uint16)
     h, w = f.shape
     out = np.zeros((h, w), np.uint32)
-    if not lib().pt_debug_labels(f.ctypes.data, h, w, float(threshold), out.ctypes.data):
+    if not lib().pt_debug_labels(f.ctypes.data, h, w, float(threshold), float(watershed_ratio), out.ctypes.data):
         raise RuntimeError("pt_debug_labels failed")
     return out
 
@@ -128,6 +148,14 @@ def last_stats():
     v = (ctypes.c_uint64 * 7)()
     lib().pt_last_stats(v, 7)
     keys = ["frames", "chunks", "rows", "flagged_frames", "multi_components", "mask_pixels", "status"]
+    return dict(zip(keys, list(v)))
+
+
+def stage_times():
+    """Per-stage device milliseconds of the last Localization (needs set_option('profile', 1))."""
+    v = (ctypes.c_double * 7)()
+    lib().pt_stage_times(v, 7)
+    keys = ["atrous_ms", "segment_ms", "fit_ms", "assemble_ms", "chunks", "jacobian_passes", "trial_passes"]
     return dict(zip(keys, list(v)))
 
 

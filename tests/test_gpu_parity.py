@@ -188,3 +188,109 @@ def test_synthetic_stack_end_to_end(pt_gpu, oracle_mod):
     out = torch.empty(((256 * 256) // 5 + 1) * 4 * 18, dtype=torch.float64, device="cuda")
     n = pt_gpu.localization_device(d, out, thr, 0.0, 2, FIT_PARAMS_SYN)
     assert np.array_equal(out[: n * 18].view(-1, 18).cpu().numpy(), got)
+
+
+# ------------------------------------------------------------------ basin fusion (ratio >= 1) ----
+def _filter_small(labels):
+    """FilterData's surface filter (Segmentation2D.cpp:88-141) on an identity-labelled map."""
+    u, c = np.unique(labels[labels > 0], return_counts=True)
+    out = labels.copy()
+    out[np.isin(out, u[c <= 4])] = 0
+    return rank_labels(out)
+
+
+@pytest.mark.parametrize("name", ["stack", "small", "dense"])
+def test_fusion_labels_bit_exact(pt_gpu, oracle_mod, ref_inputs, ref_vectors, name):
+    st, thr = _stacks(ref_inputs, ref_vectors)[name]
+    for f in range(st.shape[0]):
+        w1 = oracle_mod.atrous(st[f])[1]
+        exp, _ = oracle_mod.segment(w1, thr, 10.0)
+        got = pt_gpu.debug_labels(st[f], thr, 10.0)
+        assert np.array_equal(_filter_small(got), exp), f"frame {f}"
+
+
+@pytest.mark.parametrize("fit", [0, 1, 2, 3, 4])
+def test_localization_without_watershed_matches_golden_csv(pt_gpu, ref_inputs, ref_csv, fit):
+    rows = pt_gpu.localization(ref_inputs["stack"], 103.6, 10.0, fit, FIT_PARAMS_CSV)
+    exp = ref_csv[f"stack-localizations-103.6_False_{fit}_1.0_0.0_7"]
+    assert match_rows(theta_to_degrees(rows, fit), exp, atol=1e-3) == []
+
+
+@pytest.mark.parametrize("name", ["small", "dense"])
+def test_localization_fusion_matches_oracle(pt_gpu, oracle_mod, ref_vectors, name):
+    st, thr = ref_vectors[name], float(ref_vectors[f"{name}_autothreshold"][0])
+    got = pt_gpu.localization(st, thr, 10.0, 0, FIT_PARAMS_SYN)
+    exp = oracle_mod.localization(st, thr, 10.0, 0, FIT_PARAMS_SYN)
+    assert np.array_equal(got, exp)
+    got = pt_gpu.localization(st, thr, 10.0, 2, FIT_PARAMS_SYN)
+    exp = oracle_mod.localization(st, thr, 10.0, 2, FIT_PARAMS_SYN)
+    assert match_rows(got, exp, atol=1e-7, rtol=1e-7) == []
+
+
+# ------------------------------------------------------------------ AutoThreshold ------------------
+def test_autothreshold_known_answers(pt_gpu, ref_inputs):
+    expected = [63.651749, 66.142910, 63.192296, 64.368855, 63.954150,
+                63.400043, 66.521994, 63.373237, 62.613244, 63.866017]  # Tests_CPU/test_Palm.cpp:307-310
+    st = ref_inputs["stack"]
+    for f in range(10):
+        assert pt_gpu.autothreshold(st[f], (7.0,)) == pytest.approx(expected[f], abs=1e-5)
+
+
+@pytest.mark.parametrize("name", ["small", "dense"])
+def test_autothreshold_matches_compiled_reference(pt_gpu, ref_vectors, name):
+    thr = float(ref_vectors[f"{name}_autothreshold"][0])
+    assert pt_gpu.autothreshold(ref_vectors[name][0], (7.0,)) == pytest.approx(thr, rel=1e-12)
+
+
+# ------------------------------------------------------------------ 512^2 frames, both flood paths --
+@pytest.fixture(scope="module")
+def frames512(pt_gpu, oracle_mod):
+    import torch
+    d = torch.empty((12, 512, 512), dtype=torch.uint16, device="cuda")
+    pt_gpu.synth_device(d, seed=0x5EED0002, emitters=200)
+    st = d.cpu().numpy()
+    return st, oracle_mod.autothreshold(st[0])
+
+
+@pytest.mark.parametrize("sequential", [0, 1])
+@pytest.mark.parametrize("ratio", [0.0, 10.0])
+def test_labels_512(pt_gpu, oracle_mod, frames512, sequential, ratio):
+    st, thr = frames512
+    pt_gpu.set_option("force_sequential_watershed", sequential)
+    try:
+        for f in range(st.shape[0]):
+            w1 = oracle_mod.atrous(st[f])[1]
+            got = pt_gpu.debug_labels(st[f], thr, ratio)
+            assert pt_gpu.last_stats()["status"] == 0
+            if ratio == 0.0:
+                exp = oracle_mod.watershed_labels(w1, thr)
+                ok = np.array_equal(rank_labels(got), exp)
+            else:
+                exp, _ = oracle_mod.segment(w1, thr, ratio)
+                ok = np.array_equal(_filter_small(got), exp)
+            if not ok:
+                np.savez_compressed(f"gpurun_out/labels_mismatch_{int(ratio)}_{sequential}_{f}.npz", frame=st[f],
+                                    got=got, exp=exp, thr=thr)
+            assert ok, f"frame {f}"
+    finally:
+        pt_gpu.set_option("force_sequential_watershed", 0)
+
+
+def test_localization_512_no_fit(pt_gpu, oracle_mod, frames512):
+    st, thr = frames512
+    got = pt_gpu.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN)
+    exp = oracle_mod.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN)
+    assert got.shape == exp.shape and np.array_equal(got, exp)
+
+
+def test_partial_last_chunk(pt_gpu, oracle_mod, frames512):
+    """A last chunk with fewer frames than the chunk capacity starts from clean per-frame state."""
+    st, thr = frames512
+    exp = oracle_mod.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN)
+    try:
+        for frames_per_chunk in (5, 7, 11):
+            pt_gpu.set_option("chunk_pixels", frames_per_chunk * 512 * 512)
+            got = pt_gpu.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN)
+            assert got.shape == exp.shape and np.array_equal(got, exp), frames_per_chunk
+    finally:
+        pt_gpu.set_option("chunk_pixels", 64 << 20)
