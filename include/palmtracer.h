@@ -83,11 +83,13 @@ PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint6
 
 /* Options: "fit_fp64" (1: FP64 model and sums, 0: FP32), "max_iter" (LM iteration cap, 75),
  * "chunk_pixels" (frames per device chunk x pixels), "profile" (1: stage timing),
- * "force_sequential_watershed" (1: every frame through the exact sequential flood). Returns 1 on success. */
+ * "force_sequential_watershed" (1: every frame through the exact frame flood), "force_heap_watershed"
+ * (1: that flood with the heap restatement of the reference). Returns 1 on success. */
 PT_API int PT_CALL pt_set_option(const char* name, double value);
 
-/* Counters of the last call: frames, chunks, rows, frames re-flooded sequentially, multi-label
- * components, mask pixels, status. Writes min(n, 7) values, returns 7. */
+/* Counters of the last call: frames, chunks, rows, frames flagged for the exact frame flood,
+ * multi-label components, frames deferred to the final pass, status. Writes min(n, 7) values,
+ * returns 7. */
 PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n);
 
 /* With the "profile" option, device milliseconds of the last Localization per stage (a trous,

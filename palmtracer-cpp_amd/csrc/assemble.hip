@@ -25,8 +25,10 @@ __device__ inline bool before(int64_t ya, int64_t xa, uint32_t ia, int64_t yb, i
     return ya != yb ? ya < yb : (xa != xb ? xa < xb : ia < ib);
 }
 
-__global__ void __launch_bounds__(NT) assemble_kernel(const double* __restrict__ rows, const uint32_t* __restrict__ roiBase,
-                                                      uint64_t idBase, uint64_t planeBase, double* __restrict__ out)
+// One workgroup per frame: rank every row of the frame by (Y, X) (ties by position, which the
+// reference's unstable sort leaves unspecified) and write it at its rank.
+__global__ void __launch_bounds__(NT) sort_frames_kernel(const double* __restrict__ rows, const uint32_t* __restrict__ roiBase,
+                                                         double* __restrict__ staged)
 {
     __shared__ int64_t ky[LDS_ROWS], kx[LDS_ROWS];
     const int f = blockIdx.x;
@@ -47,20 +49,46 @@ __global__ void __launch_bounds__(NT) assemble_kernel(const double* __restrict__
             const int64_t xj = lds ? kx[j] : weak_key(rows[(size_t)(b0 + j) * kCols + kX]);
             rank += before(yj, xj, j, yi, xi, i);
         }
-        double* o = out + (size_t)(b0 + rank) * kCols;
+        double* o = staged + (size_t)(b0 + rank) * kCols;
         for (int c = 0; c < kCols; ++c) o[c] = r[c];
-        o[kId] = (double)(idBase + b0 + rank + 1);
-        o[kPlane] = (double)(planeBase + f + 1);
         o[kIndex] = (double)(rank + 1);
+    }
+}
+
+// One workgroup per plane: rows to their final place with the global Id and the plane number
+// (lib/Palm.cpp:24-43).
+__global__ void __launch_bounds__(NT) scatter_kernel(const double* __restrict__ staged, const uint64_t* __restrict__ src,
+                                                     const uint64_t* __restrict__ dst, const uint32_t* __restrict__ count,
+                                                     double* __restrict__ out, uint64_t p0)
+{
+    const uint64_t p = p0 + blockIdx.x;
+    const uint32_t n = count[p];
+    const double* a = staged + src[p] * kCols;
+    double* b = out + dst[p] * kCols;
+    for (uint32_t i = threadIdx.x; i < n * kCols; i += NT) {
+        const uint32_t r = i / kCols, c = i - r * kCols;
+        double v = a[i];
+        if (c == kId) v = (double)(dst[p] + r + 1);
+        else if (c == kPlane) v = (double)(p + 1);
+        b[i] = v;
     }
 }
 } // namespace
 
-void launch_assemble(const double* d_rows, const uint32_t* d_roiBase, int frames, uint64_t idBase, uint64_t planeBase,
-                     double* d_out, hipStream_t s)
+void launch_sort_frames(const double* d_rows, const uint32_t* d_roiBase, int frames, double* d_staged, hipStream_t s)
 {
     if (frames <= 0) return;
-    assemble_kernel<<<frames, NT, 0, s>>>(d_rows, d_roiBase, idBase, planeBase, d_out);
+    sort_frames_kernel<<<frames, NT, 0, s>>>(d_rows, d_roiBase, d_staged);
+    PT_CHECK(hipGetLastError());
+}
+
+void launch_scatter(const double* d_staged, const uint64_t* d_src, const uint64_t* d_dst, const uint32_t* d_count,
+                    uint64_t planes, double* d_out, hipStream_t s)
+{
+    for (uint64_t p0 = 0; p0 < planes; p0 += 1u << 30) {
+        const uint64_t n = std::min<uint64_t>(planes - p0, 1u << 30);
+        scatter_kernel<<<(unsigned)n, NT, 0, s>>>(d_staged, d_src, d_dst, d_count, d_out, p0);
+    }
     PT_CHECK(hipGetLastError());
 }
 

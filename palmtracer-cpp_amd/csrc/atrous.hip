@@ -143,7 +143,8 @@ __global__ void __launch_bounds__(NT) atrous_plane_kernel(const uint16_t* __rest
 __global__ void __launch_bounds__(NT) atrous_detect_kernel(const uint16_t* __restrict__ in, uint32_t* __restrict__ P,
                                                            uint32_t* __restrict__ lab, uint32_t* __restrict__ maskList,
                                                            uint32_t* __restrict__ maskCnt, uint32_t* __restrict__ seedList,
-                                                           uint32_t* __restrict__ seedCnt, int W, int H, int64_t tmin)
+                                                           uint32_t* __restrict__ seedCnt, int W, int H, int64_t tmin,
+                                                           const uint32_t* __restrict__ frameMap)
 {
     __shared__ uint32_t l0[L0_SIZE];
     __shared__ uint32_t l1[L1_SIZE];
@@ -151,7 +152,7 @@ __global__ void __launch_bounds__(NT) atrous_detect_kernel(const uint16_t* __res
     __shared__ uint32_t sCnt[2], sBase[2];
     const size_t npx = (size_t)W * H;
     const int f = blockIdx.z;
-    const uint16_t* img = in + (size_t)f * npx;
+    const uint16_t* img = in + (size_t)(frameMap ? frameMap[f] : (uint32_t)f) * npx;
     const int tx0 = blockIdx.x * TILE_W, ty0 = blockIdx.y * TILE_H;
     if (threadIdx.x < 2) sCnt[threadIdx.x] = 0;
     Win O, R1;
@@ -229,12 +230,12 @@ void launch_atrous_plane(const uint16_t* d_in, double* d_out, int W, int H, int 
 }
 
 void launch_atrous_detect(const uint16_t* d_in, const DetectBuffers& b, int W, int H, int frames, int64_t tmin,
-                          hipStream_t s)
+                          hipStream_t s, const uint32_t* frameMap)
 {
     if (frames <= 0 || W <= 0 || H <= 0) return;
     if (frames > 65535) throw Error("launch_atrous_detect: at most 65535 frames per launch");
     const dim3 grid(ceil_div(W, TILE_W), ceil_div(H, TILE_H), frames);
-    atrous_detect_kernel<<<grid, NT, 0, s>>>(d_in, b.P, b.lab, b.maskList, b.maskCnt, b.seedList, b.seedCnt, W, H, tmin);
+    atrous_detect_kernel<<<grid, NT, 0, s>>>(d_in, b.P, b.lab, b.maskList, b.maskCnt, b.seedList, b.seedCnt, W, H, tmin, frameMap);
     PT_CHECK(hipGetLastError());
 }
 

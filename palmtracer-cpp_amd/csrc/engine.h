@@ -20,7 +20,7 @@ struct DevBuf {
 };
 
 struct RunStats {
-    uint64_t frames = 0, chunks = 0, rois = 0, flaggedFrames = 0, multiComponents = 0, maskPixels = 0;
+    uint64_t frames = 0, chunks = 0, rois = 0, flaggedFrames = 0, multiComponents = 0, maskPixels = 0, deferredFrames = 0;
     uint32_t status = 0;
     // With Options::profile: device time per stage (HIP events on the engine stream, summed over
     // chunks) and the LM passes executed (model + Jacobian, trial model).
@@ -33,7 +33,8 @@ struct Options {
     int maxIter = kMaxIterDefault; // Fitting::N_ITER
     size_t chunkPixels = size_t(64) << 20;
     bool profile = false;           // stage timing with HIP events + LM pass counters
-    bool forceSequential = false;   // every frame through the exact sequential watershed
+    bool forceSequential = false;   // every frame through the exact frame flood
+    bool forceHeap = false;         // that flood through the libstdc++-order heap restatement
 };
 
 class Engine {
@@ -57,19 +58,24 @@ public:
     // AutoThreshold of one device frame (lib/Palm.cpp:122-172).
     double autothreshold(const uint16_t* d_frame, int H, int W, const double* fitParams);
 
-    DevBuf in, out, rows, rois;
+    DevBuf in, out, rows, rois, staged, frameMap, planeInfo;
 
 private:
     Engine();
     DetectBuffers b_{};
-    DevBuf P_, lab_, fin_, aux_, mask_, seed_, perFrame_, regions_, fz_, order_, multi_, scratch_, fb_, counters_, flag_;
+    DevBuf P_, lab_, fin_, aux_, mask_, seed_, perFrame_, regions_, fz_, order_, multi_, fb_, counters_, flag_;
     DevBuf atMask_, atPart_, atRows_, passes_;
     std::vector<hipEvent_t> ev_;
     hipEvent_t event(size_t i);
     void prepare(size_t F, size_t npx);
     void zero_chunk(size_t F);
+    // Detection of F frames (chunk frame f = input frame map[f], or base + f without a map).
+    // Returns the ROI count; with resolve = false, frames that need the exact frame flood are listed
+    // in deferred (input frame indices) and produce no ROI.
     uint32_t detect_chunk(const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio, uint64_t maxPoints,
-                          size_t evBase);
+                          size_t evBase, const uint32_t* d_map, bool resolve, uint64_t firstFrame,
+                          std::vector<uint32_t>* deferred);
+    void grow_staging(size_t rows, size_t keepRows);
     FitParams fit_params(uint64_t fit, const double* fp) const;
 };
 

@@ -98,12 +98,10 @@ void Engine::prepare(size_t F, size_t npx)
     order_.ensure(F * regionCap * 4);
     const size_t multiCap = n / 64 + 1024;
     multi_.ensure(multiCap * 8);
-    const size_t scratchWords = n / 8 + 65536;
-    scratch_.ensure(scratchWords * 8);
     const size_t fbSlots = std::clamp<size_t>((size_t(1) << 26) / std::max<size_t>(npx, 1), 4, 64);
     fb_.ensure(fbSlots * fb_slot_words(npx) * 4);
     counters_.ensure(kNumCounters * 4);
-    flag_.ensure((F + 1) * 4);
+    flag_.ensure((F + 1) * 8);
     uint32_t* pf = perFrame_.as<uint32_t>();
     const size_t s = F + 1;
     b_.P = P_.as<uint32_t>();
@@ -123,14 +121,13 @@ void Engine::prepare(size_t F, size_t npx)
     b_.fz = fz_.as<Fz>();
     b_.order = order_.as<uint32_t>();
     b_.multiList = multi_.as<uint64_t>();
-    b_.scratch = scratch_.as<uint64_t>();
     b_.fbScratch = fb_.as<uint32_t>();
     b_.counters = counters_.as<uint32_t>();
     b_.flagList = flag_.as<uint32_t>();
+    b_.slowList = flag_.as<uint32_t>() + (F + 1);
     b_.frames = F;
     b_.npx = npx;
     b_.regionCap = regionCap;
-    b_.scratchWords = scratchWords;
     b_.multiCap = multiCap;
     b_.fbSlots = fbSlots;
 }
@@ -174,27 +171,46 @@ hipEvent_t Engine::event(size_t i)
 }
 
 uint32_t Engine::detect_chunk(const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio, uint64_t maxPoints,
-                              size_t evBase)
+                              size_t evBase, const uint32_t* d_map, bool resolve, uint64_t firstFrame,
+                              std::vector<uint32_t>* deferred)
 {
     const size_t npx = (size_t)W * H;
     zero_chunk(F);
     if (opt.profile) PT_CHECK(hipEventRecord(event(evBase + 0), stream));
-    launch_atrous_detect(d_frames, b_, W, H, (int)F, threshold_to_pmin(thr), stream);
+    launch_atrous_detect(d_frames, b_, W, H, (int)F, threshold_to_pmin(thr), stream, d_map);
     if (opt.profile) PT_CHECK(hipEventRecord(event(evBase + 1), stream));
     rois.ensure(F * (npx / 4 + 2) * sizeof(Roi));
-    launch_segment(b_, W, H, (int)F, maxPoints, ratio, rois.as<Roi>(), stream, opt.forceSequential);
+    // The pass over deferred frames (d_map set) sends every frame through the exact frame flood.
+    const bool forceSeq = opt.forceSequential || d_map != nullptr;
+    launch_segment(b_, W, H, (int)F, maxPoints, ratio, rois.as<Roi>(), stream, forceSeq, opt.forceHeap, resolve);
     if (opt.profile) PT_CHECK(hipEventRecord(event(evBase + 2), stream));
-    uint32_t tail[2] = {0, 0};
-    PT_CHECK(hipMemcpyAsync(&tail[0], b_.roiBase + F, 4, hipMemcpyDeviceToHost, stream));
-    PT_CHECK(hipMemcpyAsync(&tail[1], b_.counters + kStatus, 4, hipMemcpyDeviceToHost, stream));
     uint32_t ctr[kNumCounters];
+    uint32_t total = 0;
+    PT_CHECK(hipMemcpyAsync(&total, b_.roiBase + F, 4, hipMemcpyDeviceToHost, stream));
     PT_CHECK(hipMemcpyAsync(ctr, b_.counters, sizeof(ctr), hipMemcpyDeviceToHost, stream));
     PT_CHECK(hipStreamSynchronize(stream));
     last.flaggedFrames += ctr[kFlagCnt];
     last.multiComponents += ctr[kMultiCnt];
-    last.status |= tail[1];
-    if (tail[1]) throw Error("segmentation status " + std::to_string(tail[1]));
-    return tail[0];
+    last.status |= ctr[kStatus];
+    if (ctr[kStatus]) throw Error("segmentation status " + std::to_string(ctr[kStatus]));
+    if (!resolve && deferred && ctr[kFlagCnt]) {
+        std::vector<uint32_t> fl(ctr[kFlagCnt]);
+        PT_CHECK(hipMemcpy(fl.data(), b_.flagList, fl.size() * 4, hipMemcpyDeviceToHost));
+        for (uint32_t f : fl) deferred->push_back((uint32_t)(firstFrame + f));
+    }
+    return total;
+}
+
+void Engine::grow_staging(size_t rowsNeeded, size_t keepRows)
+{
+    const size_t need = std::max<size_t>(rowsNeeded, 1) * kCols * 8;
+    if (need <= staged.bytes) return;
+    DevBuf nb;
+    nb.ensure(std::max(need, staged.bytes * 3 / 2));
+    if (keepRows) PT_CHECK(hipMemcpyAsync(nb.p, staged.p, keepRows * kCols * 8, hipMemcpyDeviceToDevice, stream));
+    PT_CHECK(hipStreamSynchronize(stream));
+    staged.release();
+    staged = nb;
 }
 
 uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, double thr, double ratio, uint64_t fit,
@@ -207,29 +223,67 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     const size_t F = std::max<size_t>(1, std::min<size_t>(planes, std::min<size_t>(65535, opt.chunkPixels / npx)));
     prepare(F, npx);
     const FitParams fp = fit_params(fit, fitParams);
-    uint64_t rowsDone = 0;
     unsigned long long* passes = nullptr;
     if (opt.profile) {
         passes_.ensure(16);
         passes = passes_.as<unsigned long long>();
         PT_CHECK(hipMemsetAsync(passes, 0, 16, stream));
     }
-    std::vector<uint8_t> fitted;
+    // Per plane: staged row offset and row count (rows of a plane are staged contiguously).
+    std::vector<uint64_t> src(planes, 0);
+    std::vector<uint32_t> cnt(planes, 0);
+    std::vector<uint32_t> deferred;
+    std::vector<uint32_t> base(F + 1);
+    uint64_t stagedRows = 0;
+    // One chunk: detection, fit of its ROIs, per-frame (Y, X) sort into the staging rows.
+    auto run_chunk = [&](const uint16_t* frames, size_t f, const uint32_t* d_map, const uint32_t* hostMap,
+                         uint64_t z0, bool resolve) {
+        const size_t eb = 6 * last.chunks;
+        const uint32_t n = detect_chunk(frames, f, H, W, thr, ratio, maxPerPlane, eb, d_map, resolve, z0,
+                                        resolve ? nullptr : &deferred);
+        PT_CHECK(hipMemcpyAsync(base.data(), b_.roiBase, (f + 1) * 4, hipMemcpyDeviceToHost, stream));
+        rows.ensure((size_t)std::max<uint32_t>(n, 1) * kCols * 8);
+        grow_staging(stagedRows + n, stagedRows);
+        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 3), stream));
+        if (n) launch_fit(frames, npx, W, H, rois.as<Roi>(), nullptr, n, fp, rows.as<double>(), opt.fitFp64, stream, passes,
+                          d_map);
+        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 4), stream));
+        if (n) launch_sort_frames(rows.as<double>(), b_.roiBase, (int)f, staged.as<double>() + stagedRows * kCols, stream);
+        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 5), stream));
+        PT_CHECK(hipStreamSynchronize(stream));
+        for (size_t k = 0; k < f; ++k) {
+            const uint64_t plane = hostMap ? hostMap[k] : z0 + k;
+            src[plane] = stagedRows + base[k];
+            cnt[plane] = base[k + 1] - base[k];
+        }
+        stagedRows += n;
+        last.chunks++;
+    };
     for (uint64_t z0 = 0; z0 < planes; z0 += F) {
         const size_t f = std::min<uint64_t>(F, planes - z0);
-        const uint16_t* frames = d_in + z0 * npx;
-        const size_t eb = 6 * last.chunks;
-        const uint32_t n = detect_chunk(frames, f, H, W, thr, ratio, maxPerPlane, eb);
-        if (rowsDone + n > outCapRows) throw Error("output capacity exceeded");
-        rows.ensure((size_t)std::max<uint32_t>(n, 1) * kCols * 8);
-        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 3), stream));
-        if (n) launch_fit(frames, npx, W, H, rois.as<Roi>(), nullptr, n, fp, rows.as<double>(), opt.fitFp64, stream, passes);
-        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 4), stream));
-        if (n) launch_assemble(rows.as<double>(), b_.roiBase, (int)f, rowsDone, z0, d_out + rowsDone * kCols, stream);
-        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 5), stream));
-        rowsDone += n;
-        last.chunks++;
+        run_chunk(d_in + z0 * npx, f, nullptr, nullptr, z0, false);
     }
+    // Frames whose component floods could not be proven: all of them at once, exact frame flood.
+    last.deferredFrames = deferred.size();
+    for (size_t d0 = 0; d0 < deferred.size(); d0 += F) {
+        const size_t f = std::min(F, deferred.size() - d0);
+        frameMap.ensure(f * 4);
+        PT_CHECK(hipMemcpyAsync(frameMap.p, deferred.data() + d0, f * 4, hipMemcpyHostToDevice, stream));
+        run_chunk(d_in, f, frameMap.as<uint32_t>(), deferred.data() + d0, 0, true);
+    }
+    // Final placement in plane order.
+    std::vector<uint64_t> dst(planes);
+    uint64_t total = 0;
+    for (uint64_t p = 0; p < planes; ++p) { dst[p] = total; total += cnt[p]; }
+    if (total > outCapRows) throw Error("output capacity exceeded");
+    planeInfo.ensure(planes * 20 + 64);
+    uint64_t* dSrc = planeInfo.as<uint64_t>();
+    uint64_t* dDst = dSrc + planes;
+    uint32_t* dCnt = reinterpret_cast<uint32_t*>(dDst + planes);
+    PT_CHECK(hipMemcpyAsync(dSrc, src.data(), planes * 8, hipMemcpyHostToDevice, stream));
+    PT_CHECK(hipMemcpyAsync(dDst, dst.data(), planes * 8, hipMemcpyHostToDevice, stream));
+    PT_CHECK(hipMemcpyAsync(dCnt, cnt.data(), planes * 4, hipMemcpyHostToDevice, stream));
+    if (total) launch_scatter(staged.as<double>(), dSrc, dDst, dCnt, planes, d_out, stream);
     PT_CHECK(hipStreamSynchronize(stream));
     if (opt.profile) {
         for (size_t c = 0; c < last.chunks; ++c) {
@@ -249,8 +303,8 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         last.trialPasses = pc[1];
     }
     last.frames = planes;
-    last.rois = rowsDone;
-    return rowsDone * kCols;
+    last.rois = total;
+    return total * kCols;
 }
 
 uint64_t Engine::fit_positions(const uint16_t* d_in, uint64_t planes, int H, int W, const double* hostPos, uint64_t n,
@@ -306,7 +360,7 @@ void Engine::debug_labels(const uint16_t* d_frame, int H, int W, double thr, dou
     zero_chunk(1);
     launch_atrous_detect(d_frame, b_, W, H, 1, threshold_to_pmin(thr), stream);
     rois.ensure((npx / 4 + 2) * sizeof(Roi));
-    launch_segment(b_, W, H, 1, ~0ull, ratio, rois.as<Roi>(), stream, opt.forceSequential);
+    launch_segment(b_, W, H, 1, ~0ull, ratio, rois.as<Roi>(), stream, opt.forceSequential, opt.forceHeap);
     // Labels only where P > 0 (fin is not written elsewhere).
     masked_labels<<<256, 256, 0, stream>>>(b_.fin, b_.P, d_labels, npx);
     PT_CHECK(hipGetLastError());

@@ -182,6 +182,7 @@ PT_API int PT_CALL pt_set_option(const char* name, double value)
         else if (n == "chunk_pixels") e.opt.chunkPixels = (size_t)std::max(1.0, value);
         else if (n == "profile") e.opt.profile = value != 0.0;
         else if (n == "force_sequential_watershed") e.opt.forceSequential = value != 0.0;
+        else if (n == "force_heap_watershed") e.opt.forceHeap = value != 0.0;
         else throw pt::Error("unknown option '" + n + "'");
         return 1;
     }, 0);
@@ -191,7 +192,7 @@ PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n)
 {
     Engine& e = Engine::get();
     const uint64_t v[7] = {e.last.frames, e.last.chunks, e.last.rois, e.last.flaggedFrames, e.last.multiComponents,
-                           e.last.maskPixels, e.last.status};
+                           e.last.deferredFrames, e.last.status};
     for (uint64_t i = 0; i < n && i < 7; ++i) out[i] = v[i];
     return 7;
 }

@@ -420,7 +420,7 @@ template <typename T, int MODE>
 __global__ void __launch_bounds__(64) fit_kernel(const uint16_t* __restrict__ frames, size_t npx, int W, int H,
                                                  const Roi* __restrict__ rois, const uint32_t* __restrict__ count,
                                                  uint32_t nHost, FitParams fp, double* __restrict__ rows,
-                                                 unsigned long long* __restrict__ passes)
+                                                 unsigned long long* __restrict__ passes, const uint32_t* __restrict__ frameMap)
 {
     const uint32_t n = count ? *count : nHost;
     uint32_t nJac = 0, nTrial = 0;
@@ -430,7 +430,8 @@ __global__ void __launch_bounds__(64) fit_kernel(const uint16_t* __restrict__ fr
         if constexpr (MODE == 0) {
             fill_row(0, row, r.x, r.y, 0.0, r.seg, r.surf, 0, 0, -1, -1, -1, -1, -1);
         } else {
-            fit_one<T, MODE == 0 ? 1 : MODE>(frames + (size_t)r.frame * npx, W, H, r, fp, row, nJac, nTrial);
+            const size_t fr = frameMap ? frameMap[r.frame] : r.frame;
+            fit_one<T, MODE == 0 ? 1 : MODE>(frames + fr * npx, W, H, r, fp, row, nJac, nTrial);
         }
         double* o = rows + (size_t)i * kCols;
 #pragma unroll
@@ -452,15 +453,16 @@ __global__ void __launch_bounds__(64) fit_kernel(const uint16_t* __restrict__ fr
 
 template <typename T>
 void launch_fit_t(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, const uint32_t* d_count,
-                  uint32_t nHost, const FitParams& fp, double* d_rows, unsigned long long* d_passes, hipStream_t s)
+                  uint32_t nHost, const FitParams& fp, double* d_rows, unsigned long long* d_passes,
+                  const uint32_t* frameMap, hipStream_t s)
 {
     const unsigned grid = d_count ? 4096u : std::max(1u, std::min(65535u, ceil_div(nHost, 64)));
     switch (fp.mode) {
-    case 1: fit_kernel<T, 1><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_passes); break;
-    case 2: fit_kernel<T, 2><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_passes); break;
-    case 3: fit_kernel<T, 3><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_passes); break;
-    case 4: fit_kernel<T, 4><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_passes); break;
-    default: fit_kernel<T, 0><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_passes); break;
+    case 1: fit_kernel<T, 1><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_passes, frameMap); break;
+    case 2: fit_kernel<T, 2><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_passes, frameMap); break;
+    case 3: fit_kernel<T, 3><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_passes, frameMap); break;
+    case 4: fit_kernel<T, 4><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_passes, frameMap); break;
+    default: fit_kernel<T, 0><<<grid, 64, 0, s>>>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_passes, frameMap); break;
     }
     PT_CHECK(hipGetLastError());
 }

@@ -62,30 +62,34 @@ struct DetectBuffers {
     Fz* fz = nullptr;             // F * regionCap (basin fusion)
     uint32_t* order = nullptr;    // F * regionCap: slots in label order (basin fusion)
     uint64_t* multiList = nullptr;// (frame << 32 | component root) of multi-label components
-    uint64_t* scratch = nullptr;  // flood work space (u64 words)
     uint32_t* fbScratch = nullptr;// exact-fallback work space: fbSlots * fb_slot_words(npx) u32
     uint32_t* counters = nullptr; // [kNumCounters], see Counter
-    uint32_t* flagList = nullptr; // [F]
-    size_t frames = 0, npx = 0, regionCap = 0, scratchWords = 0, multiCap = 0, fbSlots = 0;
+    uint32_t* flagList = nullptr; // [F] frames for the exact frame flood
+    uint32_t* slowList = nullptr; // [F] frames for the heap restatement (equal keys met)
+    size_t frames = 0, npx = 0, regionCap = 0, multiCap = 0, fbSlots = 0;
 };
 
-// Device counters of one chunk (u32 words; kScratchUsed is a 64-bit counter on words 2-3).
-enum Counter : int { kMultiCnt = 0, kStatus = 1, kScratchUsed = 2, kFlagCnt = 4, kFbNext = 5, kNumCounters = 8 };
+// Device counters of one chunk (u32 words).
+enum Counter : int { kMultiCnt = 0, kStatus = 1, kSlowCnt = 2, kSlowNext = 3, kFlagCnt = 4, kFbNext = 5, kNumCounters = 8 };
 // kStatus bits: 1 = region slots exhausted, 2 = unlabelled mask pixel (internal error).
 
 __host__ __device__ inline size_t fb_slot_words(size_t npx) { return 5 * ((npx + 1) & ~size_t(1)); }
 
 // A trous
 void launch_atrous_plane(const uint16_t* d_in, double* d_out, int W, int H, int planes, int level, hipStream_t s);
+// frameMap (optional): chunk frame f reads input frame frameMap[f] (else f).
 void launch_atrous_detect(const uint16_t* d_in, const DetectBuffers& b, int W, int H, int frames, int64_t tmin,
-                          hipStream_t s);
+                          hipStream_t s, const uint32_t* frameMap = nullptr);
 int64_t threshold_to_pmin(double thr);
 
 // Segmentation (ratio <= 0: watershed without fusion). Produces ROIs for frames [0, F) at
 // d_rois + roiBase[f], with per-frame truncation to maxPoints in label order.
-// forceSequential sends every frame through the exact sequential watershed.
+// forceSequential sends every frame through the exact frame flood; forceHeap makes that flood use
+// the libstdc++-order heap restatement instead of the wave queue. With resolve = false, frames whose
+// component floods could not be proven are left flagged (frameFlag / flagList) without rows, for a
+// later pass over them with forceSequential.
 void launch_segment(const DetectBuffers& b, int W, int H, int frames, uint64_t maxPoints, double ratio, Roi* d_rois,
-                    hipStream_t s, bool forceSequential = false);
+                    hipStream_t s, bool forceSequential = false, bool forceHeap = false, bool resolve = true);
 
 // Gaussian LM fit of nRoi ROIs (count read from device memory *d_count when non-null).
 struct FitParams {
@@ -96,11 +100,15 @@ struct FitParams {
 };
 void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, const uint32_t* d_count,
                 uint32_t nRoiHost, const FitParams& fp, double* d_rows, bool fp64, hipStream_t s,
-                unsigned long long* d_passes = nullptr);
+                unsigned long long* d_passes = nullptr, const uint32_t* frameMap = nullptr);
 
-// Per-frame (Y, X) sort and row assembly into the caller's layout.
-void launch_assemble(const double* d_rows, const uint32_t* d_roiBase, int frames, uint64_t idBase, uint64_t planeBase,
-                     double* d_out, hipStream_t s);
+// Per-frame (Y, X) sort (rows of frame f at d_rows + roiBase[f]) into d_staged with the same layout;
+// Index is set, Id and Plane are left for launch_scatter.
+void launch_sort_frames(const double* d_rows, const uint32_t* d_roiBase, int frames, double* d_staged, hipStream_t s);
+// Final placement: for each plane p, count[p] rows from staged row src[p] to output row dst[p],
+// with Id = dst[p] + k + 1 and Plane = p + 1.
+void launch_scatter(const double* d_staged, const uint64_t* d_src, const uint64_t* d_dst, const uint32_t* d_count,
+                    uint64_t planes, double* d_out, hipStream_t s);
 
 void launch_exclusive_scan(const uint32_t* d_in, uint32_t* d_out, int n, hipStream_t s);
 

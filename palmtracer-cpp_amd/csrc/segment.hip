@@ -263,118 +263,285 @@ __global__ void __launch_bounds__(NT) label_assign_kernel(const uint32_t* __rest
     }
 }
 
-// One thread floods one multi-label component (see file header, step 4).
-__device__ void flood_one(const uint32_t* __restrict__ P, uint32_t* __restrict__ lab, uint32_t* __restrict__ fin,
-                          const PixAux* __restrict__ aux, const uint32_t* __restrict__ maskCnt, uint64_t e,
-                          uint64_t* __restrict__ scratch, size_t scratchWords, uint32_t* __restrict__ counters,
-                          uint32_t* __restrict__ frameFlag, uint32_t* __restrict__ flagList, int W, int H, size_t npx)
+// ---------------------------------------------------------------------------------------------
+// Wave-cooperative priority queue in LDS: an unsorted array of entries (pixel, label, potential,
+// age); the key -P/2^16 + age * 1e-7 is recomputed from (potential, age) exactly as at push time,
+// and a pop is a wave-wide minimum. Without equal keys every exact priority queue pops in the
+// same order as the reference's heap, so equal keys at a pop are reported (and handled by the
+// caller) instead of being ordered.
+struct WaveQ {
+    uint64_t* pa;      // (P << 32) | age
+    uint32_t* pix;
+    uint32_t* lbl;
+    uint32_t cap;
+};
+
+__device__ inline double entry_key(uint64_t pa) { return age_key((uint32_t)(pa >> 32), (uint32_t)pa); }
+
+// Neighbour j (0..7) of a pixel in the reference's scan order (rows dy, then columns dx).
+__device__ inline void nb_offset(int j, int& dy, int& dx)
 {
-    const int f = (int)(e >> 32);
-    const uint32_t c = (uint32_t)e;
-    if (__hip_atomic_load(&frameFlag[f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-    const uint32_t* Pf = P + (size_t)f * npx;
-    uint32_t* lb = lab + (size_t)f * npx;
-    uint32_t* fn = fin + (size_t)f * npx;
-    const uint32_t n = aux[(size_t)f * npx + c].csize;
-    const uint64_t words = (uint64_t)(n + 1) / 2 + 2ull * n;
-    const uint64_t off = atomicAdd((unsigned long long*)&counters[kScratchUsed], (unsigned long long)words);
-    if (off + words > scratchWords) { flag_frame(frameFlag, counters, flagList, f); return; }
-    uint32_t* pix = (uint32_t*)(scratch + off);
-    DHeap h{(double*)(scratch + off + (n + 1) / 2), scratch + off + (n + 1) / 2 + n, 0u};
-
-    // Collect the component (breadth first from its root); mark visited in lab's top bit.
-    uint32_t tail = 0, head = 0;
-    pix[tail++] = c;
-    lb[c] = c | kVisited;
-    while (head < tail) {
-        const uint32_t p = pix[head++];
-        const int y = p / W, x = p - y * W;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int yy = y + dy;
-            if (yy < 0 || yy >= H) continue;
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int xx = x + dx;
-                if ((dx | dy) == 0 || xx < 0 || xx >= W) continue;
-                const uint32_t q = (uint32_t)(yy * W + xx);
-                if (Pf[q] != 0 && lb[q] == c && tail < n) {
-                    lb[q] = c | kVisited;
-                    pix[tail++] = q;
-                }
-            }
-        }
-    }
-    if (tail != n) { flag_frame(frameFlag, counters, flagList, f); return; }
-
-    // Seeds (pixels already labelled: the component's local maxima) in raster order: gather them
-    // in the heap value array, insertion-sort by index, then push like the reference's seed scan.
-    uint32_t ns = 0;
-    for (uint32_t i = 0; i < n; ++i)
-        if (fn[pix[i]] != 0) h.v[ns++] = pix[i];
-    for (uint32_t i = 1; i < ns; ++i) {
-        const uint64_t x = h.v[i];
-        uint32_t j = i;
-        while (j > 0 && h.v[j - 1] > x) { h.v[j] = h.v[j - 1]; --j; }
-        h.v[j] = x;
-    }
-    // Move the sorted seed list to the pixel array (no longer needed) before pushing.
-    for (uint32_t i = 0; i < ns; ++i) pix[i] = (uint32_t)h.v[i];
-    for (uint32_t i = 0; i < ns; ++i) {
-        const uint32_t s = pix[i];
-        if (has_unlabelled_nbr(Pf, fn, W, H, (int)s)) h.push(age_key(Pf[s], 0), (uint64_t)s);
-    }
-
-    const double eps = kEpsilon, delta = 1e-9;
-    const double nmax = (double)maskCnt[f];
-    uint32_t it = 0;
-    bool ok = true;
-    while (h.n > 0 && ok) {
-        const uint64_t av = h.pop();
-        const uint32_t a = (uint32_t)av, ta = (uint32_t)(av >> 32);
-        const double oa = (double)Pf[a] * kPScale;
-        for (uint32_t i = 0; i < h.n; ++i) {
-            const uint64_t bv = h.v[i];
-            const uint32_t b = (uint32_t)bv, tb = (uint32_t)(bv >> 32);
-            const double ob = (double)Pf[b] * kPScale;
-            if (ta == tb) { if (oa == ob) { ok = false; break; } }
-            else if (ta < tb) { if (!(ob - oa < (double)(tb - ta) * eps - delta)) { ok = false; break; } }
-            else if (!(oa - ob > nmax * eps + delta)) { ok = false; break; }
-        }
-        if (!ok) break;
-        ++it;
-        const uint32_t la = fn[a];
-        const int y = a / W, x = a - y * W;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int yy = y + dy;
-            if (yy < 0 || yy >= H) continue;
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int xx = x + dx;
-                if ((dx | dy) == 0 || xx < 0 || xx >= W) continue;
-                const uint32_t q = (uint32_t)(yy * W + xx);
-                if (Pf[q] != 0 && fn[q] == 0) {
-                    if (h.n >= n) { ok = false; break; }   // cannot happen: each pixel is pushed once
-                    h.push(age_key(Pf[q], it), (uint64_t)q | ((uint64_t)it << 32));
-                    fn[q] = la;
-                }
-            }
-        }
-    }
-    if (!ok) flag_frame(frameFlag, counters, flagList, f);
+    const int k = j < 4 ? j : j + 1;  // skip the centre
+    dy = k / 3 - 1;
+    dx = k % 3 - 1;
 }
 
-__global__ void __launch_bounds__(64) flood_kernel(const uint32_t* __restrict__ P, uint32_t* __restrict__ lab,
-                                                   uint32_t* __restrict__ fin, const PixAux* __restrict__ aux,
-                                                   const uint32_t* __restrict__ maskCnt, const uint64_t* __restrict__ multiList,
-                                                   size_t multiCap, uint64_t* __restrict__ scratch, size_t scratchWords,
+__device__ inline void waveq_put(WaveQ& q, uint32_t pos, uint32_t pix, uint32_t lbl, uint32_t p, uint32_t age)
+{
+    q.pa[pos] = ((uint64_t)p << 32) | age;
+    q.pix[pos] = pix;
+    q.lbl[pos] = lbl;
+}
+
+// Pops the minimum. Returns false (and does not pop) when two entries share the minimal key.
+// With kCheck, check(pa_popped, pa_other) == false marks the pop unproven (result in `proven`).
+template <bool kCheck, typename Check>
+__device__ inline bool waveq_pop(WaveQ& q, uint32_t& n, uint32_t& outPix, uint32_t& outLbl, bool& proven, Check&& check)
+{
+    const int lane = threadIdx.x & 63;
+    double km = 1.79769313486231570815e308;
+    uint32_t pm = 0xFFFFFFFFu, cm = 0;
+    for (uint32_t i = lane; i < n; i += 64) {
+        const double k = entry_key(q.pa[i]);
+        if (k < km) { km = k; pm = i; cm = 1; }
+        else if (k == km) ++cm;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double k2 = __shfl_xor(km, o);
+        const uint32_t p2 = __shfl_xor(pm, o), c2 = __shfl_xor(cm, o);
+        if (k2 < km) { km = k2; pm = p2; cm = c2; }
+        else if (k2 == km) { cm += c2; pm = p2 < pm ? p2 : pm; }
+    }
+    if (cm > 1) return false;  // equal keys: the heap order would decide
+    const uint64_t apa = q.pa[pm];
+    const uint32_t a = q.pix[pm], la = q.lbl[pm];
+    proven = true;
+    if constexpr (kCheck) {
+        bool bad = false;
+        for (uint32_t i = lane; i < n; i += 64)
+            if (i != pm && !check(apa, q.pa[i])) bad = true;
+        proven = __ballot(bad) == 0;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        q.pa[pm] = q.pa[n - 1];
+        q.pix[pm] = q.pix[n - 1];
+        q.lbl[pm] = q.lbl[n - 1];
+    }
+    __syncthreads();
+    n -= 1;
+    outPix = a;
+    outLbl = la;
+    return true;
+}
+
+// Claims the unlabelled mask neighbours of `a` for label la and pushes them with age `it`
+// (Segmentation2D.cpp:368-383). Returns false on queue overflow.
+__device__ inline bool waveq_expand(WaveQ& q, uint32_t& n, const uint32_t* Pf, uint32_t* fn, int W, int H, uint32_t a,
+                                    uint32_t la, uint32_t it)
+{
+    const int lane = threadIdx.x & 63;
+    bool claim = false;
+    uint32_t qpix = 0, qp = 0;
+    if (lane < 8) {
+        int dy, dx;
+        nb_offset(lane, dy, dx);
+        const int y = (int)(a / W) + dy, x = (int)(a % W) + dx;
+        if (y >= 0 && y < H && x >= 0 && x < W) {
+            qpix = (uint32_t)(y * W + x);
+            qp = Pf[qpix];
+            const uint32_t ql = fn[qpix];  // loaded with the potential (one round trip)
+            claim = qp != 0 && ql == 0u;
+        }
+    }
+    const uint64_t m = __ballot(claim);
+    const uint32_t cnt = __popcll(m);
+    if (n + cnt > q.cap) return false;
+    if (claim) {
+        waveq_put(q, n + __popcll(m & ((1ull << lane) - 1ull)), qpix, la, qp, it);
+        fn[qpix] = la;
+    }
+    __syncthreads();
+    n += cnt;
+    return true;
+}
+
+constexpr uint32_t QCAP_FLOOD = 1024;
+
+// One wave floods one multi-label component with the component-local pop count as the age, and
+// proves at every pop that the frame-global order would pop the same entry (file header, step 4).
+// Seeds are the component's local maxima (first pass) or its surviving regions' maximum pixels
+// (second pass, fz != nullptr).
+__global__ void __launch_bounds__(64) flood_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ lab,
+                                                   uint32_t* __restrict__ fin, const uint32_t* __restrict__ seedList,
+                                                   const uint32_t* __restrict__ seedCnt, const uint32_t* __restrict__ maskCnt,
+                                                   const uint64_t* __restrict__ multiList, size_t multiCap,
                                                    uint32_t* __restrict__ counters, uint32_t* __restrict__ frameFlag,
-                                                   uint32_t* __restrict__ flagList, int W, int H, size_t npx)
+                                                   uint32_t* __restrict__ flagList, int W, int H, size_t npx,
+                                                   const Fz* __restrict__ fz, const uint32_t* __restrict__ rootCnt,
+                                                   size_t regionCap)
 {
-    const uint32_t nMulti = min((size_t)counters[kMultiCnt], multiCap);
-    for (uint32_t t = blockIdx.x * 64 + threadIdx.x; t < nMulti; t += gridDim.x * 64)
-        flood_one(P, lab, fin, aux, maskCnt, multiList[t], scratch, scratchWords, counters, frameFlag, flagList, W, H, npx);
+    __shared__ uint64_t sPa[QCAP_FLOOD];
+    __shared__ uint32_t sPix[QCAP_FLOOD], sLbl[QCAP_FLOOD];
+    WaveQ q{sPa, sPix, sLbl, QCAP_FLOOD};
+    const int lane = threadIdx.x;
+    const uint32_t nMulti = (uint32_t)min((size_t)counters[kMultiCnt], multiCap);
+    for (uint32_t t = blockIdx.x; t < nMulti; t += gridDim.x) {
+        const uint64_t e = multiList[t];
+        const int f = (int)(e >> 32);
+        const uint32_t c = (uint32_t)e;
+        if (__hip_atomic_load(&frameFlag[f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
+        const uint32_t* Pf = P + (size_t)f * npx;
+        const uint32_t* lb = lab + (size_t)f * npx;
+        uint32_t* fn = fin + (size_t)f * npx;
+        // Seeds of the component that the reference would push (labelled, next to an unlabelled pixel).
+        uint32_t n = 0;
+        bool ok = true;
+        const uint32_t ns = fz ? (uint32_t)min((size_t)rootCnt[f], regionCap) : seedCnt[f];
+        for (uint32_t b0 = 0; b0 < ns && ok; b0 += 64) {
+            const uint32_t k = b0 + lane;
+            bool sel = false;
+            uint32_t sp = 0;
+            if (k < ns) {
+                if (fz) {
+                    const Fz z = fz[(size_t)f * regionCap + k];
+                    if (z.alive && z.maxKey != 0) { sp = 0xFFFFFFFFu - (uint32_t)z.maxKey; sel = true; }
+                } else {
+                    sp = seedList[(size_t)f * npx + k];
+                    sel = true;
+                }
+                sel = sel && (lb[sp] & ~kVisited) == c && fn[sp] != 0u && has_unlabelled_nbr(Pf, fn, W, H, (int)sp);
+            }
+            const uint64_t m = __ballot(sel);
+            const uint32_t cnt = __popcll(m);
+            if (n + cnt > q.cap) { ok = false; break; }
+            if (sel) waveq_put(q, n + __popcll(m & ((1ull << lane) - 1ull)), sp, fn[sp], Pf[sp], 0u);
+            n += cnt;
+        }
+        __syncthreads();
+        const double eps = kEpsilon, delta = 1e-9;
+        const double nmax = (double)maskCnt[f];
+        uint32_t it = 0;
+        while (ok && n > 0) {
+            uint32_t a, la;
+            bool proven = true;
+            // b (potential ob, age tb) stays behind the popped a (oa, ta) in the frame-global order
+            // for every admissible global age: the global gap between two pushes is at least the
+            // local gap (ta < tb case) and at most the frame's mask size (ta > tb case).
+            auto check = [&](uint64_t ap, uint64_t bp) {
+                const double oa = (double)(uint32_t)(ap >> 32) * kPScale, ob = (double)(uint32_t)(bp >> 32) * kPScale;
+                const uint32_t ta = (uint32_t)ap, tb = (uint32_t)bp;
+                if (ta == tb) return oa != ob;
+                if (ta < tb) return ob - oa < (double)(tb - ta) * eps - delta;
+                return oa - ob > nmax * eps + delta;
+            };
+            if (!waveq_pop<true>(q, n, a, la, proven, check) || !proven) { ok = false; break; }
+            ++it;
+            if (!waveq_expand(q, n, Pf, fn, W, H, a, la, it)) { ok = false; break; }
+        }
+        if (!ok && lane == 0) flag_frame(frameFlag, counters, flagList, f);
+        __syncthreads();
+    }
 }
 
-// Exact sequential restatement of Segmentation2D::watershed for one flagged frame per workgroup
-// (frames taken from flagList through a shared cursor). Work space per slot: 5 * npx words.
+constexpr uint32_t QCAP_FB = 3840;   // 60 KB of LDS
+
+// Exact frame flood for flagged frames, one wave per frame: the reference's process (seeds in any
+// order, global pop count as the age) with the wave queue. A frame whose queue meets equal keys at
+// a pop, or overflows, goes to the heap restatement below (fallback_kernel) through slowList.
+__global__ void __launch_bounds__(64) fallback_fast_kernel(const uint32_t* __restrict__ P, uint32_t* __restrict__ fin,
+                                                           const uint32_t* __restrict__ maskList,
+                                                           const uint32_t* __restrict__ maskCnt,
+                                                           const uint32_t* __restrict__ seedList,
+                                                           const uint32_t* __restrict__ seedCnt,
+                                                           const uint32_t* __restrict__ flagList, uint32_t* __restrict__ counters,
+                                                           uint32_t* __restrict__ slowList, int W, int H, size_t npx,
+                                                           const Fz* __restrict__ fz, const Region* __restrict__ regions,
+                                                           const uint32_t* __restrict__ rootCnt, const PixAux* __restrict__ aux,
+                                                           size_t regionCap, int forceSlow)
+{
+    __shared__ uint64_t sPa[QCAP_FB];
+    __shared__ uint32_t sPix[QCAP_FB], sLbl[QCAP_FB];
+    WaveQ q{sPa, sPix, sLbl, QCAP_FB};
+    const int lane = threadIdx.x;
+    while (true) {
+        uint32_t i = 0;
+        if (lane == 0) i = atomicAdd(&counters[kFbNext], 1u);
+        i = __shfl(i, 0);
+        if (i >= counters[kFlagCnt]) return;
+        const uint32_t f = flagList[i];
+        if (forceSlow) {
+            if (lane == 0) slowList[atomicAdd(&counters[kSlowCnt], 1u)] = f;
+            continue;
+        }
+        const uint32_t* Pf = P + (size_t)f * npx;
+        uint32_t* fn = fin + (size_t)f * npx;
+        const uint32_t cnt = maskCnt[f];
+        // 1. Labels back to the seed state.
+        for (uint32_t k = lane; k < cnt; k += 64) {
+            const uint32_t p = maskList[(size_t)f * npx + k];
+            bool seed;
+            if (!fz) {
+                seed = is_local_max(Pf, W, H, (int)p);
+            } else {
+                const uint32_t L = fn[p];
+                seed = false;
+                if (L != 0 && L <= npx) {
+                    const uint32_t sl = aux[(size_t)f * npx + (L - 1)].rslot;
+                    seed = sl < regionCap && fz[(size_t)f * regionCap + sl].alive &&
+                           0xFFFFFFFFu - (uint32_t)fz[(size_t)f * regionCap + sl].maxKey == p;
+                }
+            }
+            if (!seed) fn[p] = 0u;
+        }
+        __threadfence_block();
+        __syncthreads();
+        // 2. Seeds next to an unlabelled pixel (the reference pushes them in raster order; order is
+        //    irrelevant here without equal keys).
+        uint32_t n = 0;
+        bool ok = true;
+        const uint32_t nsrc = fz ? (uint32_t)min((size_t)rootCnt[f], regionCap) : seedCnt[f];
+        for (uint32_t b0 = 0; b0 < nsrc && ok; b0 += 64) {
+            const uint32_t k = b0 + lane;
+            bool sel = false;
+            uint32_t sp = 0;
+            if (k < nsrc) {
+                if (fz) {
+                    const Fz z = fz[(size_t)f * regionCap + k];
+                    if (z.alive && z.maxKey != 0) { sp = 0xFFFFFFFFu - (uint32_t)z.maxKey; sel = true; }
+                } else {
+                    sp = seedList[(size_t)f * npx + k];
+                    sel = true;
+                }
+                sel = sel && fn[sp] != 0u && has_unlabelled_nbr(Pf, fn, W, H, (int)sp);
+            }
+            const uint64_t m = __ballot(sel);
+            const uint32_t c = __popcll(m);
+            if (n + c > q.cap) { ok = false; break; }
+            if (sel) waveq_put(q, n + __popcll(m & ((1ull << lane) - 1ull)), sp, fn[sp], Pf[sp], 0u);
+            n += c;
+        }
+        __syncthreads();
+        // 3. Flood in key order with the frame-global pop count.
+        uint32_t it = 0;
+        while (ok && n > 0) {
+            uint32_t a, la;
+            bool proven = true;
+            if (!waveq_pop<false>(q, n, a, la, proven, [](uint64_t, uint64_t) { return true; })) { ok = false; break; }
+            ++it;
+            if (!waveq_expand(q, n, Pf, fn, W, H, a, la, it)) { ok = false; break; }
+        }
+        if (!ok && lane == 0) {
+            const uint32_t j = atomicAdd(&counters[kSlowCnt], 1u);
+            slowList[j] = f;
+        }
+        __syncthreads();
+    }
+}
+
+// Exact sequential restatement of Segmentation2D::watershed with the libstdc++ heap order, one
+// frame per workgroup (frames with equal keys at a pop, taken from slowList through a shared
+// cursor). Work space per slot: fb_slot_words(npx) words.
 __global__ void __launch_bounds__(NT) fallback_kernel(const uint32_t* __restrict__ P, uint32_t* __restrict__ fin,
                                                       const uint32_t* __restrict__ maskList, const uint32_t* __restrict__ maskCnt,
                                                       const uint32_t* __restrict__ flagList, uint32_t* __restrict__ counters,
@@ -389,8 +556,8 @@ __global__ void __launch_bounds__(NT) fallback_kernel(const uint32_t* __restrict
     uint64_t* hv = (uint64_t*)(seeds + 3 * npx2);
     while (true) {
         if (threadIdx.x == 0) {
-            const uint32_t i = atomicAdd(&counters[kFbNext], 1u);
-            sFrame = i < counters[kFlagCnt] ? flagList[i] : kNone;
+            const uint32_t i = atomicAdd(&counters[kSlowNext], 1u);
+            sFrame = i < counters[kSlowCnt] ? flagList[i] : kNone;
         }
         __syncthreads();
         const uint32_t f = sFrame;
@@ -479,11 +646,12 @@ __global__ void __launch_bounds__(NT) fusion_kernel(const uint32_t* __restrict__
                                                     const PixAux* __restrict__ aux, const uint32_t* __restrict__ maskList,
                                                     const uint32_t* __restrict__ maskCnt, const Region* __restrict__ regions,
                                                     size_t regionCap, const uint32_t* __restrict__ rootCnt, Fz* __restrict__ fz,
-                                                    uint32_t* __restrict__ order, double ratio, int W, int H, size_t npx)
+                                                    uint32_t* __restrict__ order, double ratio, int W, int H, size_t npx,
+                                                    const uint32_t* __restrict__ skip)
 {
     __shared__ uint32_t sChange;
     const int f = blockIdx.x;
-    const uint32_t n = (uint32_t)min((size_t)rootCnt[f], regionCap);
+    const uint32_t n = (skip && skip[f]) ? 0u : (uint32_t)min((size_t)rootCnt[f], regionCap);
     if (n == 0) return;
     const size_t base = (size_t)f * regionCap;
     const uint32_t* Pf = P + (size_t)f * npx;
@@ -639,14 +807,14 @@ __global__ void force_flag_kernel(uint32_t* __restrict__ counters, uint32_t* __r
     if (blockIdx.x == 0 && threadIdx.x == 0) counters[kFlagCnt] = (uint32_t)F;
 }
 
-__global__ void reset_pass_kernel(uint32_t* __restrict__ counters, uint32_t* __restrict__ frameFlag, int F)
+// Between the two watershed passes: per-pass counters only. Frame flags stay set: a frame flagged in
+// either pass is resolved (or deferred) as a whole.
+__global__ void reset_pass_kernel(uint32_t* __restrict__ counters)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < F; i += gridDim.x * blockDim.x) frameFlag[i] = 0u;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         counters[kMultiCnt] = 0u;
-        counters[kScratchUsed] = 0u;
-        counters[kScratchUsed + 1] = 0u;
-        counters[kFlagCnt] = 0u;
+        counters[kSlowCnt] = 0u;
+        counters[kSlowNext] = 0u;
         counters[kFbNext] = 0u;
     }
 }
@@ -654,9 +822,11 @@ __global__ void reset_pass_kernel(uint32_t* __restrict__ counters, uint32_t* __r
 __global__ void __launch_bounds__(NT) region_stats_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ fin,
                                                           const PixAux* __restrict__ aux, const uint32_t* __restrict__ maskList,
                                                           const uint32_t* __restrict__ maskCnt, Region* __restrict__ regions,
-                                                          size_t regionCap, uint32_t* __restrict__ counters, int W, size_t npx)
+                                                          size_t regionCap, uint32_t* __restrict__ counters, int W, size_t npx,
+                                                          const uint32_t* __restrict__ skip)
 {
     const int f = blockIdx.y;
+    if (skip && skip[f]) return;   // deferred frame: its labels are resolved in a later pass
     const uint32_t cnt = maskCnt[f];
     for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < cnt; k += gridDim.x * NT) {
         const uint32_t p = maskList[(size_t)f * npx + k];
@@ -689,11 +859,12 @@ __device__ inline uint32_t block_sum(uint32_t v, uint32_t* sh)
 // FilterData (surface > VOLUME_MIN) and the per-frame capacity of Fitting::FitAll.
 __global__ void __launch_bounds__(NT) region_count_kernel(const Region* __restrict__ regions, size_t regionCap,
                                                           const uint32_t* __restrict__ rootCnt, uint32_t* __restrict__ found,
-                                                          uint32_t* __restrict__ keep, uint64_t maxPoints)
+                                                          uint32_t* __restrict__ keep, uint64_t maxPoints,
+                                                          const uint32_t* __restrict__ skip)
 {
     __shared__ uint32_t sh[NT / 64];
     const int f = blockIdx.x;
-    const uint32_t n = min((size_t)rootCnt[f], regionCap);
+    const uint32_t n = (skip && skip[f]) ? 0u : (uint32_t)min((size_t)rootCnt[f], regionCap);
     uint32_t c = 0;
     for (uint32_t i = threadIdx.x; i < n; i += NT) c += (double)regions[(size_t)f * regionCap + i].surf > kVolumeMin;
     const uint32_t t = block_sum(c, sh);
@@ -728,8 +899,8 @@ __global__ void __launch_bounds__(NT) roi_emit_kernel(const uint32_t* __restrict
     __shared__ uint32_t sWave[NT / 64];
     __shared__ uint32_t sBase;
     const int f = blockIdx.x;
-    const uint32_t n = min((size_t)rootCnt[f], regionCap);
     const uint32_t kf = keep[f];
+    const uint32_t n = kf == 0 ? 0u : (uint32_t)min((size_t)rootCnt[f], regionCap);
     const bool truncate = found[f] > kf;
     const Region* rg = regions + (size_t)f * regionCap;
     if (threadIdx.x == 0) sBase = 0;
@@ -815,11 +986,14 @@ void launch_exclusive_scan(const uint32_t* d_in, uint32_t* d_out, int n, hipStre
 }
 
 void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoints, double ratio, Roi* d_rois,
-                    hipStream_t s, bool forceSequential)
+                    hipStream_t s, bool forceSequential, bool forceHeap, bool resolve)
 {
     const size_t npx = b.npx;
     const unsigned bpf = (unsigned)std::min<size_t>(64, std::max<size_t>(1, npx / 16384));
     const dim3 g(bpf, F);
+    // Frames that failed the component-flood proof: resolved here (exact frame flood) or left to a
+    // later pass that re-runs them with forceSequential (their rows are then skipped here).
+    const uint32_t* skip = resolve ? nullptr : b.frameFlag;
     cc_union_kernel<<<g, NT, 0, s>>>(b.lab, b.P, b.maskList, b.maskCnt, b.aux, W, H, npx);
     cc_flatten_kernel<<<g, NT, 0, s>>>(b.lab, b.maskList, b.maskCnt, b.aux, npx);
     seed_label_kernel<<<g, NT, 0, s>>>(b.P, b.lab, b.fin, b.aux, b.seedList, b.seedCnt, b.rootCnt, b.regions,
@@ -828,24 +1002,29 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
         if (forceSequential) force_flag_kernel<<<ceil_div(F, 256), 256, 0, s>>>(b.counters, b.frameFlag, b.flagList, F);
         label_assign_kernel<<<g, NT, 0, s>>>(b.P, b.lab, b.fin, b.aux, b.maskList, b.maskCnt, b.multiList, b.multiCap,
                                              b.counters, b.frameFlag, b.flagList, W, H, npx, pass2);
-        flood_kernel<<<2048, 64, 0, s>>>(b.P, b.lab, b.fin, b.aux, b.maskCnt, b.multiList, b.multiCap, b.scratch,
-                                         b.scratchWords, b.counters, b.frameFlag, b.flagList, W, H, npx);
-        fallback_kernel<<<(unsigned)b.fbSlots, NT, 0, s>>>(b.P, b.fin, b.maskList, b.maskCnt, b.flagList, b.counters,
+        flood_kernel<<<4096, 64, 0, s>>>(b.P, b.lab, b.fin, b.seedList, b.seedCnt, b.maskCnt, b.multiList, b.multiCap,
+                                         b.counters, b.frameFlag, b.flagList, W, H, npx, pass2 ? b.fz : nullptr,
+                                         b.rootCnt, b.regionCap);
+        if (!resolve) return;
+        fallback_fast_kernel<<<(unsigned)std::max<size_t>(1, std::min<size_t>(F, 1024)), 64, 0, s>>>(
+            b.P, b.fin, b.maskList, b.maskCnt, b.seedList, b.seedCnt, b.flagList, b.counters, b.slowList, W, H, npx,
+            pass2 ? b.fz : nullptr, b.regions, b.rootCnt, b.aux, b.regionCap, forceHeap ? 1 : 0);
+        fallback_kernel<<<(unsigned)b.fbSlots, NT, 0, s>>>(b.P, b.fin, b.maskList, b.maskCnt, b.slowList, b.counters,
                                                             b.fbScratch, W, H, npx, pass2 ? b.fz : nullptr, b.aux,
                                                             b.regionCap);
     };
     watershed(0);
     if ((int)ratio > 0) {   // Segmentation2D.cpp:82-84
         fusion_kernel<<<F, NT, 0, s>>>(b.P, b.fin, b.aux, b.maskList, b.maskCnt, b.regions, b.regionCap, b.rootCnt,
-                                       b.fz, b.order, ratio, W, H, npx);
+                                       b.fz, b.order, ratio, W, H, npx, skip);
         pass2_reset_kernel<<<g, NT, 0, s>>>(b.lab, b.aux, b.maskList, b.maskCnt, npx);
         pass2_seed_kernel<<<g, NT, 0, s>>>(b.lab, b.aux, b.fz, b.regions, b.rootCnt, b.regionCap, npx);
-        reset_pass_kernel<<<ceil_div(F, 256), 256, 0, s>>>(b.counters, b.frameFlag, F);
+        reset_pass_kernel<<<1, 64, 0, s>>>(b.counters);
         watershed(1);
     }
     region_stats_kernel<<<g, NT, 0, s>>>(b.P, b.fin, b.aux, b.maskList, b.maskCnt, b.regions, b.regionCap, b.counters,
-                                         W, npx);
-    region_count_kernel<<<F, NT, 0, s>>>(b.regions, b.regionCap, b.rootCnt, b.found, b.keep, maxPoints);
+                                         W, npx, skip);
+    region_count_kernel<<<F, NT, 0, s>>>(b.regions, b.regionCap, b.rootCnt, b.found, b.keep, maxPoints, skip);
     exclusive_scan_kernel<<<1, 1024, 0, s>>>(b.keep, b.roiBase, F);
     roi_emit_kernel<<<F, NT, 0, s>>>(b.P, b.fin, b.regions, b.regionCap, b.rootCnt, b.found, b.keep, b.roiBase, d_rois,
                                      W, npx);

@@ -147,7 +147,7 @@ def set_option(name, value):
 def last_stats():
     v = (ctypes.c_uint64 * 7)()
     lib().pt_last_stats(v, 7)
-    keys = ["frames", "chunks", "rows", "flagged_frames", "multi_components", "mask_pixels", "status"]
+    keys = ["frames", "chunks", "rows", "flagged_frames", "multi_components", "deferred_frames", "status"]
     return dict(zip(keys, list(v)))
 
 

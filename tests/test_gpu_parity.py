@@ -252,11 +252,13 @@ def frames512(pt_gpu, oracle_mod):
     return st, oracle_mod.autothreshold(st[0])
 
 
-@pytest.mark.parametrize("sequential", [0, 1])
+@pytest.mark.parametrize("sequential", [0, 1, 2])
 @pytest.mark.parametrize("ratio", [0.0, 10.0])
 def test_labels_512(pt_gpu, oracle_mod, frames512, sequential, ratio):
+    """0: component floods; 1: every frame through the wave-queue frame flood; 2: through the heap."""
     st, thr = frames512
-    pt_gpu.set_option("force_sequential_watershed", sequential)
+    pt_gpu.set_option("force_sequential_watershed", sequential > 0)
+    pt_gpu.set_option("force_heap_watershed", sequential > 1)
     try:
         for f in range(st.shape[0]):
             w1 = oracle_mod.atrous(st[f])[1]
@@ -274,6 +276,7 @@ def test_labels_512(pt_gpu, oracle_mod, frames512, sequential, ratio):
             assert ok, f"frame {f}"
     finally:
         pt_gpu.set_option("force_sequential_watershed", 0)
+        pt_gpu.set_option("force_heap_watershed", 0)
 
 
 def test_localization_512_no_fit(pt_gpu, oracle_mod, frames512):
