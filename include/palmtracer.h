@@ -82,9 +82,10 @@ PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint6
                                    double watershedRatio, uint32_t* labels);
 
 /* Options: "fit_fp64" (1: FP64 model and sums, 0: FP32), "max_iter" (LM iteration cap, 75),
- * "chunk_pixels" (frames per device chunk x pixels), "profile" (1: stage timing),
+ * "chunk_pixels" (frames per device chunk x pixels, 0: default 2^28), "profile" (1: stage timing),
  * "force_sequential_watershed" (1: every frame through the exact frame flood), "force_heap_watershed"
- * (1: that flood with the heap restatement of the reference). Returns 1 on success. */
+ * (1: that flood with the heap restatement of the reference), "fit_waves" (resident waves of the
+ * persistent FP32 fit, 0: default). Returns 1 on success. */
 PT_API int PT_CALL pt_set_option(const char* name, double value);
 
 /* Counters of the last call: frames, chunks, rows, frames flagged for the exact frame flood,

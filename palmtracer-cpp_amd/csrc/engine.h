@@ -31,10 +31,12 @@ struct RunStats {
 struct Options {
     bool fitFp64 = true;           // FP64 model/Jacobian (reference arithmetic) or FP32
     int maxIter = kMaxIterDefault; // Fitting::N_ITER
-    size_t chunkPixels = size_t(64) << 20;
+    static constexpr size_t kDefaultChunkPixels = size_t(256) << 20;  // 1024 frames of 512 x 512
+    size_t chunkPixels = kDefaultChunkPixels;
     bool profile = false;           // stage timing with HIP events + LM pass counters
     bool forceSequential = false;   // every frame through the exact frame flood
     bool forceHeap = false;         // that flood through the libstdc++-order heap restatement
+    unsigned fitWaves = 0;          // persistent FP32 fit: resident waves (0: 16 per CU)
 };
 
 class Engine {
@@ -64,7 +66,12 @@ private:
     Engine();
     DetectBuffers b_{};
     DevBuf P_, lab_, fin_, aux_, mask_, seed_, perFrame_, regions_, fz_, order_, multi_, fb_, counters_, flag_;
-    DevBuf atMask_, atPart_, atRows_, passes_;
+    DevBuf atMask_, atPart_, atRows_, passes_, fitNext_;
+    unsigned int* fitNext()
+    {
+        fitNext_.ensure(256);
+        return fitNext_.as<unsigned int>();
+    }
     std::vector<hipEvent_t> ev_;
     hipEvent_t event(size_t i);
     void prepare(size_t F, size_t npx);

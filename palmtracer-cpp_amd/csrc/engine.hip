@@ -245,8 +245,8 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         rows.ensure((size_t)std::max<uint32_t>(n, 1) * kCols * 8);
         grow_staging(stagedRows + n, stagedRows);
         if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 3), stream));
-        if (n) launch_fit(frames, npx, W, H, rois.as<Roi>(), nullptr, n, fp, rows.as<double>(), opt.fitFp64, stream, passes,
-                          d_map);
+        if (n) launch_fit(frames, npx, W, H, rois.as<Roi>(), n, fp, rows.as<double>(), opt.fitFp64, fitNext(), stream,
+                          passes, d_map, opt.fitWaves);
         if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 4), stream));
         if (n) launch_sort_frames(rows.as<double>(), b_.roiBase, (int)f, staged.as<double>() + stagedRows * kCols, stream);
         if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 5), stream));
@@ -333,7 +333,8 @@ uint64_t Engine::fit_positions(const uint16_t* d_in, uint64_t planes, int H, int
     positions_to_rois<<<g, 256, 0, stream>>>(dpos.as<double>(), n, planes, rois.as<Roi>(), dbad.as<uint32_t>());
     PT_CHECK(hipGetLastError());
     const FitParams fp = fit_params(fit, fitParams);
-    launch_fit(d_in, (size_t)W * H, W, H, rois.as<Roi>(), nullptr, (uint32_t)n, fp, d_out, opt.fitFp64, stream);
+    launch_fit(d_in, (size_t)W * H, W, H, rois.as<Roi>(), (uint32_t)n, fp, d_out, opt.fitFp64, fitNext(), stream, nullptr,
+               nullptr, opt.fitWaves);
     positions_ids<<<g, 256, 0, stream>>>(dpos.as<double>(), dloc.as<uint32_t>(), n, d_out);
     PT_CHECK(hipGetLastError());
     uint32_t bad = 0;

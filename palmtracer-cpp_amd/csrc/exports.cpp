@@ -179,10 +179,11 @@ PT_API int PT_CALL pt_set_option(const char* name, double value)
         const std::string n(name ? name : "");
         if (n == "fit_fp64") e.opt.fitFp64 = value != 0.0;
         else if (n == "max_iter") e.opt.maxIter = std::max(1, (int)value);
-        else if (n == "chunk_pixels") e.opt.chunkPixels = (size_t)std::max(1.0, value);
+        else if (n == "chunk_pixels") e.opt.chunkPixels = value > 0.0 ? (size_t)value : pt::Options::kDefaultChunkPixels;
         else if (n == "profile") e.opt.profile = value != 0.0;
         else if (n == "force_sequential_watershed") e.opt.forceSequential = value != 0.0;
         else if (n == "force_heap_watershed") e.opt.forceHeap = value != 0.0;
+        else if (n == "fit_waves") e.opt.fitWaves = (unsigned)std::max(0.0, std::min(1048576.0, value));
         else throw pt::Error("unknown option '" + n + "'");
         return 1;
     }, 0);

@@ -4,10 +4,10 @@
 #include "lmfit_impl.h"
 
 namespace pt {
-void launch_fit_f64(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, const uint32_t* d_count,
-                    uint32_t nHost, const FitParams& fp, double* d_rows, hipStream_t s, unsigned long long* d_passes,
+void launch_fit_f64(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, uint32_t nRoi,
+                    const FitParams& fp, double* d_rows, hipStream_t s, unsigned long long* d_passes,
                     const uint32_t* frameMap)
 {
-    launch_fit_t<double>(d_frames, npx, W, H, d_rois, d_count, nHost, fp, d_rows, d_passes, frameMap, s);
+    launch_fit_t<double>(d_frames, npx, W, H, d_rois, nullptr, nRoi, fp, d_rows, d_passes, frameMap, s);
 }
 } // namespace pt

@@ -1,0 +1,504 @@
+// Persistent FP32 Levenberg-Marquardt fit: one ROI per lane, lanes refill from a work queue.
+//
+// Reference algorithm: lib/fit/GaussianFit.cpp:48-295 (same parameters, bounds, lambda schedule,
+// clamps, gain ratio, stopping rules and sentinels; lib/fit/Fitting.cpp:61-175 for initP / fill).
+//
+// Schedule. Each lane runs the reference's LM for its ROI as a state machine whose states are
+// separated by passes over the ROI's pixels: chi^2 with the Jacobian products (at the initial
+// parameters, or at a trial step), or the correlation sums of R^2. Every loop turn each lane
+// performs the pass its state asks for, then advances to its next pass (damped solves that fail or
+// give no gain run in between, as in the reference). A lane whose ROI is finished takes the next
+// ROI from a global counter (one atomic per wave), so lanes do not idle behind a slow neighbour
+// until the queue runs dry. A lane stages its ROI's pixels in LDS, laid out [pixel][lane], when it
+// takes the ROI; every pass then reads LDS instead of the frame.
+//
+// Precision. Model, Jacobian, alpha and beta in FP32 (exp through the hardware exp2), chi^2
+// accumulated in FP64 from FP32 residuals; parameters, the damped normal equations (LDL^T of the
+// symmetric matrix), the step, the gain ratio and R^2 in FP64.
+#include "pt_common.h"
+#include "pt_kernels.h"
+
+#include <type_traits>
+
+namespace pt {
+namespace {
+
+constexpr double kSqrt2d = 1.41421356237309504880;
+constexpr double kPid = 3.14159265358979323846;
+constexpr float kLog2e = 1.44269504088896340736f;
+
+template <int MODE> struct NP { static constexpr int v = MODE == 1 ? 4 : MODE == 2 ? 5 : MODE == 3 ? 6 : 7; };
+template <int N> struct Tri { static constexpr int v = N * (N + 1) / 2; };
+// Packed upper triangle of a symmetric N x N matrix: element (i, j) at tri(i, j) == tri(j, i).
+template <int N> __host__ __device__ constexpr int tri(int i, int j)
+{
+    return i <= j ? i * N - i * (i - 1) / 2 + (j - i) : j * N - j * (j - 1) / 2 + (i - j);
+}
+
+__device__ inline uint64_t to_size_f(double v)
+{
+    if (!(v == v) || v >= 9223372036854775808.0 || v <= -9223372036854775808.0) return 1ull << 63;
+    return (uint64_t)(long long)v;
+}
+
+// GaussianModel with reciprocal widths (FP32).
+template <int MODE>
+struct FModel {
+    float off, amp, isx2, isy2, isx, isy, c, s, cs, c2ms2, k, k2;  // k: 1/sx^3 (mode 2), k2: -log2(e)/(2 sx^2)
+    __device__ void set(const double* p, double sigma, double theta)
+    {
+        off = (float)p[0];
+        amp = (float)p[1];
+        if constexpr (MODE <= 2) {
+            const float sg = (float)(MODE == 1 ? sigma : p[MODE == 1 ? 0 : 4]);
+            isx2 = 1.0f / (sg * sg);
+            k = isx2 / sg;
+            k2 = -0.5f * kLog2e * isx2;
+        } else {
+            const double th = MODE == 3 ? theta : p[MODE == 3 ? 0 : 6];
+            const float sx = (float)p[4], sy = (float)p[5];
+            isx = 1.0f / sx;
+            isy = 1.0f / sy;
+            isx2 = isx * isx;
+            isy2 = isy * isy;
+            c = (float)cos(th);
+            s = (float)sin(th);
+            cs = c * s;
+            c2ms2 = c * c - s * s;
+        }
+    }
+    // Value; with g != nullptr also the gradient (same components as GaussianModel::Evaluate).
+    __device__ float eval(float x, float y, float* g) const
+    {
+        if constexpr (MODE <= 2) {
+            const float r2 = x * x + y * y;
+            const float E = __builtin_amdgcn_exp2f(r2 * k2);
+            const float P = amp * E;
+            if (g) {
+                g[0] = 1.0f;
+                g[1] = E;
+                g[2] = P * x * isx2;
+                g[3] = P * y * isx2;
+                if constexpr (MODE == 2) g[4] = P * r2 * k;
+            }
+            return P + off;
+        } else {
+            const float F1 = (x * c - y * s) * isx, F2 = (x * s + y * c) * isy;
+            const float E = __builtin_amdgcn_exp2f(-kLog2e * (F1 * F1 + F2 * F2));
+            const float P = amp * E;
+            if (g) {
+                g[0] = 1.0f;
+                g[1] = E;
+                g[2] = P * (2.0f * (c * c * x - y * cs) * isx2 + 2.0f * (s * s * x + y * cs) * isy2);
+                g[3] = P * (2.0f * (s * s * y - x * cs) * isx2 + 2.0f * (c * c * y + x * cs) * isy2);
+                g[4] = P * 2.0f * F1 * F1 * isx;
+                g[5] = P * 2.0f * F2 * F2 * isy;
+                if constexpr (MODE == 4)
+                    g[6] = P * 2.0f * (isx2 - isy2) * (cs * (x * x - y * y) + c2ms2 * x * y);
+            }
+            return P + off;
+        }
+    }
+};
+
+__device__ inline void row_fill(double* pt, int mode, double x, double y, double seg, double surf, double off, double inten,
+                                double sx, double sy, double th, double rss)
+{
+    pt[kId] = 0; pt[kPlane] = 0; pt[kIndex] = 0; pt[kChannel] = -1;
+    pt[kX] = x; pt[kY] = y; pt[kZ] = 0.0;
+    pt[kIntensity] = (inten > kEpsilon && sx > kEpsilon && sy > kEpsilon) ? 2.0 * kPid * inten * sx * sy : -1;
+    pt[kSigmaX] = sx; pt[kSigmaY] = sy; pt[kTheta] = th;
+    pt[kQualityXY] = rss; pt[kQualityZ] = -1;
+    pt[kAmplitude] = inten; pt[kOffset] = off;
+    pt[kSegIntensity] = seg; pt[kSurface] = surf;
+    pt[kCircularity] = (sx > kEpsilon && sy > kEpsilon) ? (sy < sx ? sy : sx) / (sx < sy ? sy : sx) : 1;
+    (void)mode;
+}
+
+__device__ inline void row_fail(double* pt, int mode, double x, double y, double seg, double surf)
+{
+    row_fill(pt, mode, x, y, seg, surf, -1, -1, -1, -1, -1, -1);
+}
+
+template <int B, int E, typename F>
+__device__ inline void static_for(F&& f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// Solve (alpha_R) dp = beta, alpha_R = alpha with diagonal (1 + lambda) * a_ii (1 where a_ii == 0),
+// by LDL^T in FP64. Returns 0 on success, 1 when the damped matrix is not numerically positive
+// definite (the reference's singular-matrix case), 2 when the step is not finite.
+template <int N>
+__device__ inline int ldl_solve(const float (&al)[Tri<N>::v], const float (&be)[N], double lambda, double (&dp)[N])
+{
+    double L[Tri<N>::v], D[N], z[N];
+    static_for<0, Tri<N>::v>([&](auto i) { L[i] = (double)al[i]; });
+    static_for<0, N>([&](auto i) {
+        constexpr int d = tri<N>(i, i);
+        L[d] = L[d] == 0.0 ? 1.0 : (1.0 + lambda) * L[d];
+    });
+    bool pd = true;
+    static_for<0, N>([&](auto j) {
+        const double ajj = L[tri<N>(j, j)];
+        double dj = ajj;
+        static_for<0, j>([&](auto k) { dj -= L[tri<N>(j, k)] * L[tri<N>(j, k)] * D[k]; });
+        pd &= (dj > 2.220446049250313e-16 * N * fabs(ajj)) && isfinite(dj);
+        D[j] = dj;
+        const double inv = 1.0 / dj;
+        static_for<j + 1, N>([&](auto i) {
+            double v = L[tri<N>(i, j)];
+            static_for<0, j>([&](auto k) { v -= L[tri<N>(i, k)] * L[tri<N>(j, k)] * D[k]; });
+            L[tri<N>(i, j)] = v * inv;
+        });
+    });
+    if (!pd) return 1;
+    static_for<0, N>([&](auto i) {
+        double v = (double)be[i];
+        static_for<0, i>([&](auto k) { v -= L[tri<N>(i, k)] * z[k]; });
+        z[i] = v;
+    });
+    static_for<0, N>([&](auto ii) {
+        constexpr int i = N - 1 - ii;
+        double v = z[i] / D[i];
+        static_for<i + 1, N>([&](auto k) { v -= L[tri<N>(k, i)] * dp[k]; });
+        dp[i] = v;
+    });
+    bool ok = true;
+    static_for<0, N>([&](auto i) { ok &= isfinite(dp[i]); });
+    return ok ? 0 : 2;
+}
+
+enum : int { kFetch = 0, kDone = 1, kPassInit = 2, kPassTrial = 3, kPassR2 = 4 };
+
+// Largest ROI side whose pixels are staged in LDS (15 x 15 x 64 lanes x 2 bytes = 28.8 KB per wave);
+// larger ROIs read the frame in every pass.
+constexpr int kMaxStagedRoi = 15;
+
+// Every Jacobian-type pass also accumulates the Jacobian products (alT, beT): the reference
+// evaluates the Jacobian at pTry right after accepting pTry (GaussianFit.cpp:147-151) and chi^2 at
+// pTry is the same sum either way, so an accepted trial pass already holds the next alpha, beta and
+// chi^2.
+template <int MODE, bool STAGED>
+__global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __restrict__ frames, size_t npx, int W, int H,
+                                                            const Roi* __restrict__ rois, uint32_t nRoi, FitParams fp,
+                                                            double* __restrict__ rows, unsigned int* __restrict__ next,
+                                                            unsigned long long* __restrict__ passes,
+                                                            const uint32_t* __restrict__ frameMap)
+{
+    constexpr int N = NP<MODE>::v;
+    constexpr int T = Tri<N>::v;
+    extern __shared__ uint16_t patch[];  // [pixel][lane] when STAGED
+    const int lane = threadIdx.x;
+    const int roi = fp.roi, half = roi / 2, nPixI = roi * roi;
+    const double nPix = (double)nPixI;
+
+    int state = kFetch, it = 0;
+    uint32_t idx = 0, ux = 0, uy = 0;
+    const uint16_t* img = nullptr;     // ROI's top-left pixel in the frame (unstaged reads)
+    double x0 = 0, y0 = 0;             // detection position (bounds, failure rows)
+    double p[N], pTry[N];
+    double relAcc = 0, relRej = 0;     // max |dP| / max(|p|, 1e-12) with p after acceptance / rejection
+    double lambda = 0, err = 0, mq = 0;
+    float al[T], be[N];
+    uint32_t nJac = 0, nTrial = 0;
+
+    auto pix = [&](int k, int i, int j) -> float {
+        if constexpr (STAGED) return (float)patch[k * 64 + lane];
+        else return (float)img[(size_t)i * (size_t)W + j];
+    };
+
+    while (true) {
+        // ---- lanes without a ROI take the next ones (one atomic per wave) ----
+        while (true) {
+            const uint64_t want = __ballot(state == kFetch);
+            if (!want) break;
+            const int leader = __ffsll((unsigned long long)want) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(next, (unsigned)__popcll(want));
+            base = __shfl(base, leader);
+            if (state != kFetch) continue;
+            idx = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+            if (idx >= nRoi) {
+                state = kDone;
+                continue;
+            }
+            const Roi r = rois[idx];
+            x0 = r.x;
+            y0 = r.y;
+            const uint64_t sx = to_size_f(r.x), sy = to_size_f(r.y), h64 = (uint64_t)half;
+            // Border check with the reference's unsigned arithmetic (GaussianFit.cpp:51-56).
+            if (h64 > sx || sx >= (uint64_t)W - h64 || h64 > sy || sy >= (uint64_t)H - h64) {
+                row_fail(rows + (size_t)idx * kCols, MODE, r.x, r.y, r.seg, r.surf);
+                continue;  // still kFetch
+            }
+            ux = (uint32_t)sx;
+            uy = (uint32_t)sy;
+            const size_t fr = frameMap ? frameMap[r.frame] : r.frame;
+            img = frames + fr * npx + (size_t)(uy - half) * W + (ux - half);
+            float mn = 3.0e38f, mx = -3.0e38f;
+            for (int i = 0, k = 0; i < roi; ++i)
+                for (int j = 0; j < roi; ++j, ++k) {
+                    const uint16_t u = img[(size_t)i * W + j];
+                    if constexpr (STAGED) patch[k * 64 + lane] = u;
+                    const float v = (float)u;
+                    mn = v < mn ? v : mn;
+                    mx = mx < v ? v : mx;
+                }
+            // initP (Fitting.cpp:126-175)
+            p[0] = mn; p[1] = (double)mx - (double)mn; p[2] = r.x; p[3] = r.y;
+            if constexpr (MODE == 2) p[4] = fp.sigmaX;
+            if constexpr (MODE >= 3) { p[4] = kSqrt2d * fp.sigmaX; p[5] = kSqrt2d * fp.sigmaY; }
+            if constexpr (MODE == 4) p[6] = fp.theta;
+            lambda = 1e-2;
+            state = kPassInit;
+        }
+        if (__ballot(state != kDone) == 0) break;
+        if (state == kDone) continue;
+
+        double* o = rows + (size_t)idx * kCols;
+        if (state == kPassR2) {
+            // computeR2 at p: squared Pearson correlation of data and model, NaN -> 0
+            // (GaussianFit.cpp:270-295). Data sums are exact integers.
+            FModel<MODE> m;
+            m.set(p, fp.sigmaX, fp.theta);
+            // Pixel offsets from the centre, formed in FP64 relative to the ROI corner.
+            const float bx = (float)((double)(ux - half) - p[2]), by = (float)((double)(uy - half) - p[3]);
+            uint32_t sdI = 0;
+            uint64_t sddI = 0;
+            double sm = 0, smm = 0, sdm = 0;
+            for (int i = 0, k = 0; i < roi; ++i) {
+                const float Y = by + (float)i;
+                for (int j = 0; j < roi; ++j, ++k) {
+                    const float X = bx + (float)j;
+                    const float d = pix(k, i, j);
+                    const double mm = (double)m.eval(X, Y, nullptr);
+                    const uint32_t di = (uint32_t)d;
+                    sdI += di;
+                    sddI += (uint64_t)di * di;
+                    sm += mm;
+                    smm += mm * mm;
+                    sdm += (double)d * mm;
+                }
+            }
+            const double sd = (double)sdI, sdd = (double)sddI;
+            const double vd = sdd - sd * sd / nPix, vm = smm - sm * sm / nPix, cv = sdm - sd * sm / nPix;
+            const double rr = cv / sqrt(vd * vm);
+            const double rss = isnan(rr) ? 0.0 : rr * rr;
+            const Roi& r = rois[idx];
+            if (rss < kEpsilon) {
+                row_fail(o, MODE, p[2], p[3], r.seg, r.surf);
+            } else {
+                double sx = fp.sigmaX, sy = fp.sigmaY, th = fp.theta;
+                if constexpr (MODE == 1) { sx = sy = fp.sigmaX; th = 0; }
+                if constexpr (MODE == 2) { sx = sy = p[4]; th = 0; }
+                if constexpr (MODE >= 3) { sx = p[4] / kSqrt2d; sy = p[5] / kSqrt2d; }
+                if constexpr (MODE == 4) th = p[6];
+                row_fill(o, MODE, p[2], p[3], r.seg, r.surf, p[0], p[1], sx, sy, th, rss);
+            }
+            state = kFetch;
+            continue;
+        }
+
+        // ---- chi^2 and Jacobian products at p (initial pass) or pTry (trial pass) ----
+        const bool trial = state == kPassTrial;
+        double e = 0.0;
+        float alT[T], beT[N];
+#pragma unroll
+        for (int a = 0; a < T; ++a) alT[a] = 0.0f;
+#pragma unroll
+        for (int a = 0; a < N; ++a) beT[a] = 0.0f;
+        {
+            double q[N];
+#pragma unroll
+            for (int a = 0; a < N; ++a) q[a] = trial ? pTry[a] : p[a];
+            FModel<MODE> m;
+            m.set(q, fp.sigmaX, fp.theta);
+            bool circ = false;
+            if constexpr (MODE == 4) circ = fabs(q[4] - q[5]) <= 2.220446049250313e-16;
+            const float bx = (float)((double)(ux - half) - q[2]), by = (float)((double)(uy - half) - q[3]);
+            for (int i = 0, k = 0; i < roi; ++i) {
+                const float Y = by + (float)i;
+                for (int j = 0; j < roi; ++j, ++k) {
+                    const float X = bx + (float)j;
+                    float g[N];
+                    const float res = pix(k, i, j) - m.eval(X, Y, g);
+                    e += (double)res * (double)res;
+                    if constexpr (MODE == 4)
+                        if (circ) g[6] = 0.0f;
+#pragma unroll
+                    for (int a = 0; a < N; ++a) {
+                        beT[a] += g[a] * res;
+#pragma unroll
+                        for (int b = a; b < N; ++b) alT[tri<N>(a, b)] += g[a] * g[b];
+                    }
+                }
+            }
+        }
+
+        // ---- advance the LM state machine to its next pass (GaussianFit.cpp:75-165) ----
+        auto take = [&]() {
+#pragma unroll
+            for (int a = 0; a < T; ++a) al[a] = alT[a];
+#pragma unroll
+            for (int a = 0; a < N; ++a) be[a] = beT[a];
+        };
+        auto max_abs_beta = [&]() {
+            double mb = fabs((double)be[0]);
+#pragma unroll
+            for (int a = 1; a < N; ++a) { const double v = fabs((double)be[a]); mb = mb < v ? v : mb; }
+            return mb;
+        };
+        // End-of-iteration stopping rules (GaussianFit.cpp:155-162); otherwise the next iteration.
+        auto stop_after_step = [&](double rel) {
+            if (it >= 2 && (rel < 1e-6 || err / nPix < 1e-5 || max_abs_beta() < 1e-6)) return true;
+            ++it;
+            return false;
+        };
+        bool finish = false, failed = false;
+        if (!trial) {
+            ++nJac;
+            take();
+            err = e;
+            bool bad = !isfinite(err);
+#pragma unroll
+            for (int a = 0; a < N; ++a) bad |= !isfinite(be[a]);
+            if (bad) failed = true;
+            else it = (max_abs_beta() < 1e-6) ? fp.maxIter : 0;
+        } else {
+            ++nTrial;
+            const double rho = (err - e) / (2.0 * mq);
+            if (rho > 1e-2) {
+                ++nJac;
+#pragma unroll
+                for (int a = 0; a < N; ++a) p[a] = pTry[a];
+                take();
+                err = e;
+                lambda = fmax(lambda / 9, 1e-7);
+                finish = stop_after_step(relAcc);
+            } else if (lambda >= 1e7) {
+                finish = true;
+            } else {
+                lambda = fmin(lambda * 11, 1e7);
+                finish = stop_after_step(relRej);
+            }
+        }
+        // Damped solves until a trial pass is needed (GaussianFit.cpp:83-144).
+        while (!finish && !failed) {
+            if (it >= fp.maxIter) { finish = true; break; }
+            double d[N];
+            const int rc = ldl_solve<N>(al, be, lambda, d);
+            if (rc == 1) {
+                if (lambda < 1e7) { lambda = fmin(lambda * 11, 1e7); ++it; continue; }
+                failed = true;
+                break;
+            }
+            if (rc == 2) { failed = true; break; }
+            bool tiny = true;
+            relAcc = 0.0;
+            relRej = 0.0;
+            double lin = 0.0;
+#pragma unroll
+            for (int a = 0; a < N; ++a) {
+                const double v = p[a] + d[a];
+                const double lo = a <= 1 ? 0.0 : (a == 2 ? x0 - 1 : (a == 3 ? y0 - 1 : (a == 6 ? -150.0 : 0.1)));
+                const double hi = a <= 1 ? 65535.0 : (a == 2 ? x0 + 1 : (a == 3 ? y0 + 1 : (a == 6 ? 150.0 : 20.0)));
+                pTry[a] = v < lo ? lo : (hi < v ? hi : v);
+                d[a] = pTry[a] - p[a];  // dP
+                tiny &= fabs(d[a]) < 1e-15;
+                const double ad = fabs(d[a]);
+                const double ra = ad / fmax(fabs(pTry[a]), 1e-12), rr = ad / fmax(fabs(p[a]), 1e-12);
+                relAcc = relAcc < ra ? ra : relAcc;
+                relRej = relRej < rr ? rr : relRej;
+                lin += d[a] * (double)be[a];
+            }
+            if (tiny) {
+                if (lambda >= 1e7) { finish = true; break; }
+                lambda = fmin(lambda * 11, 1e7);
+                ++it;
+                continue;
+            }
+            // gain = 2 dP.beta - dP.alpha.dP (GaussianFit.cpp:248-259)
+            double quad = 0.0;
+#pragma unroll
+            for (int a = 0; a < N; ++a) {
+                double ti = 0.0;
+#pragma unroll
+                for (int b = 0; b < N; ++b) ti += (double)al[tri<N>(a, b)] * d[b];
+                quad += d[a] * ti;
+            }
+            if ((2.0 * lin) - quad <= 0.0) {
+                if (lambda >= 1e7) { finish = true; break; }
+                lambda = fmin(lambda * 11, 1e7);
+                if (relRej < 1e-6 && it >= 2) { finish = true; break; }
+                ++it;
+                continue;
+            }
+            mq = 0.0;
+#pragma unroll
+            for (int a = 0; a < N; ++a) mq += d[a] * (lambda * d[a] + (double)be[a]);
+            state = kPassTrial;
+            break;
+        }
+        if (failed) {
+            const Roi& r = rois[idx];
+            row_fail(o, MODE, x0, y0, r.seg, r.surf);
+            state = kFetch;
+        } else if (finish) {
+            state = kPassR2;
+        }
+    }
+    if (passes) {
+        // Passes of the reference's schedule (Jacobian passes, trial passes) for the FLOP count.
+        unsigned long long a = nJac, b = nTrial;
+        for (int o2 = 32; o2 > 0; o2 >>= 1) {
+            a += __shfl_xor(a, o2);
+            b += __shfl_xor(b, o2);
+        }
+        if (lane == 0) {
+            atomicAdd(&passes[0], a);
+            atomicAdd(&passes[1], b);
+        }
+    }
+}
+
+__global__ void reset_next_kernel(unsigned int* next) { *next = 0u; }
+
+} // namespace
+
+void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, uint32_t nRoi, const FitParams& fp,
+                double* d_rows, bool fp64, unsigned int* d_next, hipStream_t s, unsigned long long* d_passes,
+                const uint32_t* frameMap, unsigned waves)
+{
+    if (nRoi == 0) return;
+    if (fp64 || fp.mode < 1 || fp.mode > 4) {
+        launch_fit_f64(d_frames, npx, W, H, d_rois, nRoi, fp, d_rows, s, d_passes, frameMap);
+        return;
+    }
+    if (!d_next) throw Error("launch_fit: the FP32 fit needs a queue word");
+    reset_next_kernel<<<1, 1, 0, s>>>(d_next);
+    // Persistent waves: by default 16 per CU of the 256-CU part, fewer when there are fewer ROIs.
+    const unsigned want = waves ? waves : 4096u;
+    const unsigned grid = std::max(1u, std::min(want, ceil_div(nRoi, 64)));
+    const bool staged = fp.roi <= kMaxStagedRoi;
+    const size_t lds = staged ? (size_t)fp.roi * fp.roi * 64 * sizeof(uint16_t) : 0;
+    auto go = [&](auto kern) {
+        kern<<<grid, 64, lds, s>>>(d_frames, npx, W, H, d_rois, nRoi, fp, d_rows, d_next, d_passes, frameMap);
+    };
+    switch (fp.mode * 2 + (staged ? 1 : 0)) {
+    case 2: go(fit_persistent_kernel<1, false>); break;
+    case 3: go(fit_persistent_kernel<1, true>); break;
+    case 4: go(fit_persistent_kernel<2, false>); break;
+    case 5: go(fit_persistent_kernel<2, true>); break;
+    case 6: go(fit_persistent_kernel<3, false>); break;
+    case 7: go(fit_persistent_kernel<3, true>); break;
+    case 8: go(fit_persistent_kernel<4, false>); break;
+    default: go(fit_persistent_kernel<4, true>); break;
+    }
+    PT_CHECK(hipGetLastError());
+}
+
+} // namespace pt

@@ -98,9 +98,16 @@ struct FitParams {
     int maxIter;     // Fitting::N_ITER (75 in the reference)
     double sigmaX, sigmaY, theta;
 };
-void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, const uint32_t* d_count,
-                uint32_t nRoiHost, const FitParams& fp, double* d_rows, bool fp64, hipStream_t s,
-                unsigned long long* d_passes = nullptr, const uint32_t* frameMap = nullptr);
+// LM fit of nRoi ROIs into d_rows (18 columns each). fp64: the reference's arithmetic, one ROI per
+// lane; otherwise the FP32 persistent work-queue kernel, whose queue head is d_next (one word).
+// d_passes (optional) accumulates {Jacobian passes, trial passes}; frameMap maps ROI frames to
+// stack frames.
+void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, uint32_t nRoi, const FitParams& fp,
+                double* d_rows, bool fp64, unsigned int* d_next, hipStream_t s, unsigned long long* d_passes = nullptr,
+                const uint32_t* frameMap = nullptr, unsigned waves = 0);
+void launch_fit_f64(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, uint32_t nRoi,
+                    const FitParams& fp, double* d_rows, hipStream_t s, unsigned long long* d_passes,
+                    const uint32_t* frameMap);
 
 // Per-frame (Y, X) sort (rows of frame f at d_rows + roiBase[f]) into d_staged with the same layout;
 // Index is set, Id and Plane are left for launch_scatter.

