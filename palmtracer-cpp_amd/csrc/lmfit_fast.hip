@@ -12,9 +12,12 @@
 // until the queue runs dry. A lane stages its ROI's pixels in LDS, laid out [pixel][lane], when it
 // takes the ROI; every pass then reads LDS instead of the frame.
 //
-// Precision. Model, Jacobian, alpha and beta in FP32 (exp through the hardware exp2), chi^2
-// accumulated in FP64 from FP32 residuals; parameters, the damped normal equations (LDL^T of the
-// symmetric matrix), the step, the gain ratio and R^2 in FP64.
+// Precision. Jacobian, alpha and beta in FP32. The model value, residual and chi^2 are FP64: the
+// gain ratio and the stopping rules compare chi^2 differences that fall below FP32 resolution near
+// convergence (an FP32 exp is good to ~1e-7 relative, i.e. ~5e-4 ADU at a 4000 ADU peak, which
+// makes late trial steps accept or reject at random). An axis-aligned Gaussian needs only
+// 2 * roi FP64 exps per pass (column factors kept in LDS, one row factor per row). Parameters, the
+// damped normal equations (LDL^T of the symmetric matrix), the step and R^2 are FP64.
 #include "pt_common.h"
 #include "pt_kernels.h"
 
@@ -25,7 +28,6 @@ namespace {
 
 constexpr double kSqrt2d = 1.41421356237309504880;
 constexpr double kPid = 3.14159265358979323846;
-constexpr float kLog2e = 1.44269504088896340736f;
 
 template <int MODE> struct NP { static constexpr int v = MODE == 1 ? 4 : MODE == 2 ? 5 : MODE == 3 ? 6 : 7; };
 template <int N> struct Tri { static constexpr int v = N * (N + 1) / 2; };
@@ -41,62 +43,63 @@ __device__ inline uint64_t to_size_f(double v)
     return (uint64_t)(long long)v;
 }
 
-// GaussianModel with reciprocal widths (FP32).
+// GaussianModel (lib/fit/GaussianModel.cpp:19-52) at one parameter vector. The value is FP64:
+// E = exp(-(ax X^2 + ay Y^2 + axy X Y)) with X, Y the pixel offsets from the centre, which for an
+// axis-aligned Gaussian (axy == 0) is the product of a column factor and a row factor. The
+// gradient is FP32, formed from the FP64 E.
 template <int MODE>
-struct FModel {
-    float off, amp, isx2, isy2, isx, isy, c, s, cs, c2ms2, k, k2;  // k: 1/sx^3 (mode 2), k2: -log2(e)/(2 sx^2)
+struct Model {
+    double off, amp, ax, ay, axy;
+    float ampf, isx2, isy2, isx, isy, c, s, cs, c2ms2, k;  // k: 1/sigma^3 (mode 2)
     __device__ void set(const double* p, double sigma, double theta)
     {
-        off = (float)p[0];
-        amp = (float)p[1];
+        off = p[0];
+        amp = p[1];
+        ampf = (float)amp;
         if constexpr (MODE <= 2) {
-            const float sg = (float)(MODE == 1 ? sigma : p[MODE == 1 ? 0 : 4]);
-            isx2 = 1.0f / (sg * sg);
-            k = isx2 / sg;
-            k2 = -0.5f * kLog2e * isx2;
+            const double sg = MODE == 1 ? sigma : p[MODE == 1 ? 0 : 4];
+            ax = ay = 1.0 / (2.0 * sg * sg);
+            axy = 0.0;
+            const float sf = (float)sg;
+            isx2 = 1.0f / (sf * sf);
+            k = isx2 / sf;
         } else {
             const double th = MODE == 3 ? theta : p[MODE == 3 ? 0 : 6];
-            const float sx = (float)p[4], sy = (float)p[5];
-            isx = 1.0f / sx;
-            isy = 1.0f / sy;
-            isx2 = isx * isx;
-            isy2 = isy * isy;
-            c = (float)cos(th);
-            s = (float)sin(th);
+            const double sx = p[4], sy = p[5];
+            const double cd = cos(th), sd = sin(th);
+            const double ix2 = 1.0 / (sx * sx), iy2 = 1.0 / (sy * sy);
+            ax = cd * cd * ix2 + sd * sd * iy2;
+            ay = sd * sd * ix2 + cd * cd * iy2;
+            axy = 2.0 * cd * sd * (iy2 - ix2);
+            isx = (float)(1.0 / sx);
+            isy = (float)(1.0 / sy);
+            isx2 = (float)ix2;
+            isy2 = (float)iy2;
+            c = (float)cd;
+            s = (float)sd;
             cs = c * s;
             c2ms2 = c * c - s * s;
         }
     }
-    // Value; with g != nullptr also the gradient (same components as GaussianModel::Evaluate).
-    __device__ float eval(float x, float y, float* g) const
+    __device__ bool separable() const { return axy == 0.0; }
+    __device__ double E(double X, double Y) const { return exp(-(ax * X * X + ay * Y * Y + axy * X * Y)); }
+    // Gradient at (x, y) given E (same components as GaussianModel::Evaluate).
+    __device__ void grad(float x, float y, float Ef, float* g) const
     {
+        const float P = ampf * Ef;
+        g[0] = 1.0f;
+        g[1] = Ef;
         if constexpr (MODE <= 2) {
-            const float r2 = x * x + y * y;
-            const float E = __builtin_amdgcn_exp2f(r2 * k2);
-            const float P = amp * E;
-            if (g) {
-                g[0] = 1.0f;
-                g[1] = E;
-                g[2] = P * x * isx2;
-                g[3] = P * y * isx2;
-                if constexpr (MODE == 2) g[4] = P * r2 * k;
-            }
-            return P + off;
+            g[2] = P * x * isx2;
+            g[3] = P * y * isx2;
+            if constexpr (MODE == 2) g[4] = P * (x * x + y * y) * k;
         } else {
             const float F1 = (x * c - y * s) * isx, F2 = (x * s + y * c) * isy;
-            const float E = __builtin_amdgcn_exp2f(-kLog2e * (F1 * F1 + F2 * F2));
-            const float P = amp * E;
-            if (g) {
-                g[0] = 1.0f;
-                g[1] = E;
-                g[2] = P * (2.0f * (c * c * x - y * cs) * isx2 + 2.0f * (s * s * x + y * cs) * isy2);
-                g[3] = P * (2.0f * (s * s * y - x * cs) * isx2 + 2.0f * (c * c * y + x * cs) * isy2);
-                g[4] = P * 2.0f * F1 * F1 * isx;
-                g[5] = P * 2.0f * F2 * F2 * isy;
-                if constexpr (MODE == 4)
-                    g[6] = P * 2.0f * (isx2 - isy2) * (cs * (x * x - y * y) + c2ms2 * x * y);
-            }
-            return P + off;
+            g[2] = P * (2.0f * (c * c * x - y * cs) * isx2 + 2.0f * (s * s * x + y * cs) * isy2);
+            g[3] = P * (2.0f * (s * s * y - x * cs) * isx2 + 2.0f * (c * c * y + x * cs) * isy2);
+            g[4] = P * 2.0f * F1 * F1 * isx;
+            g[5] = P * 2.0f * F2 * F2 * isy;
+            if constexpr (MODE == 4) g[6] = P * 2.0f * (isx2 - isy2) * (cs * (x * x - y * y) + c2ms2 * x * y);
         }
     }
 };
@@ -174,9 +177,15 @@ __device__ inline int ldl_solve(const float (&al)[Tri<N>::v], const float (&be)[
 
 enum : int { kFetch = 0, kDone = 1, kPassInit = 2, kPassTrial = 3, kPassR2 = 4 };
 
-// Largest ROI side whose pixels are staged in LDS (15 x 15 x 64 lanes x 2 bytes = 28.8 KB per wave);
-// larger ROIs read the frame in every pass.
+// Largest ROI side staged in LDS: pixels (15 x 15 x 64 lanes x 2 bytes = 28.8 KB per wave) and the
+// column factors of the separable exponential (15 x 64 x 8 bytes). Larger ROIs read the frame and
+// evaluate the exponential per pixel in every pass.
 constexpr int kMaxStagedRoi = 15;
+
+__host__ __device__ constexpr size_t staged_lds_bytes(int roi)
+{
+    return (size_t)roi * roi * 64 * sizeof(uint16_t) + (size_t)roi * 64 * sizeof(double);
+}
 
 // Every Jacobian-type pass also accumulates the Jacobian products (alT, beT): the reference
 // evaluates the Jacobian at pTry right after accepting pTry (GaussianFit.cpp:147-151) and chi^2 at
@@ -191,9 +200,12 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
 {
     constexpr int N = NP<MODE>::v;
     constexpr int T = Tri<N>::v;
-    extern __shared__ uint16_t patch[];  // [pixel][lane] when STAGED
+    extern __shared__ __align__(16) unsigned char lds[];
     const int lane = threadIdx.x;
     const int roi = fp.roi, half = roi / 2, nPixI = roi * roi;
+    // STAGED: column factors [column][lane] (FP64), then the ROI's pixels [pixel][lane].
+    double* colE = reinterpret_cast<double*>(lds);
+    uint16_t* patch = reinterpret_cast<uint16_t*>(lds + (size_t)roi * 64 * sizeof(double));
     const double nPix = (double)nPixI;
 
     int state = kFetch, it = 0;
@@ -209,6 +221,26 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
     auto pix = [&](int k, int i, int j) -> float {
         if constexpr (STAGED) return (float)patch[k * 64 + lane];
         else return (float)img[(size_t)i * (size_t)W + j];
+    };
+    // Visit the ROI's pixels in raster order with the model's FP64 exponential E at each:
+    // body(k, i, j, X, Y, E) with X, Y the FP64 offsets from the centre (cx, cy).
+    auto for_pixels = [&](const Model<MODE>& m, double cx, double cy, auto&& body) {
+        const double bx = (double)(ux - half) - cx, by = (double)(uy - half) - cy;
+        const bool sep = STAGED && m.separable();
+        if (sep)
+            for (int j = 0; j < roi; ++j) {
+                const double X = bx + j;
+                colE[j * 64 + lane] = exp(-(m.ax * X * X));
+            }
+        for (int i = 0, k = 0; i < roi; ++i) {
+            const double Y = by + i;
+            const double rowE = sep ? exp(-(m.ay * Y * Y)) : 0.0;
+            for (int j = 0; j < roi; ++j, ++k) {
+                const double X = bx + j;
+                const double E = sep ? colE[j * 64 + lane] * rowE : m.E(X, Y);
+                body(k, i, j, X, Y, E);
+            }
+        }
     };
 
     while (true) {
@@ -263,27 +295,21 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
         if (state == kPassR2) {
             // computeR2 at p: squared Pearson correlation of data and model, NaN -> 0
             // (GaussianFit.cpp:270-295). Data sums are exact integers.
-            FModel<MODE> m;
+            Model<MODE> m;
             m.set(p, fp.sigmaX, fp.theta);
-            // Pixel offsets from the centre, formed in FP64 relative to the ROI corner.
-            const float bx = (float)((double)(ux - half) - p[2]), by = (float)((double)(uy - half) - p[3]);
             uint32_t sdI = 0;
             uint64_t sddI = 0;
             double sm = 0, smm = 0, sdm = 0;
-            for (int i = 0, k = 0; i < roi; ++i) {
-                const float Y = by + (float)i;
-                for (int j = 0; j < roi; ++j, ++k) {
-                    const float X = bx + (float)j;
-                    const float d = pix(k, i, j);
-                    const double mm = (double)m.eval(X, Y, nullptr);
-                    const uint32_t di = (uint32_t)d;
-                    sdI += di;
-                    sddI += (uint64_t)di * di;
-                    sm += mm;
-                    smm += mm * mm;
-                    sdm += (double)d * mm;
-                }
-            }
+            for_pixels(m, p[2], p[3], [&](int k, int i, int j, double, double, double E) {
+                const float d = pix(k, i, j);
+                const double mm = m.off + m.amp * E;
+                const uint32_t di = (uint32_t)d;
+                sdI += di;
+                sddI += (uint64_t)di * di;
+                sm += mm;
+                smm += mm * mm;
+                sdm += (double)d * mm;
+            });
             const double sd = (double)sdI, sdd = (double)sddI;
             const double vd = sdd - sd * sd / nPix, vm = smm - sm * sm / nPix, cv = sdm - sd * sm / nPix;
             const double rr = cv / sqrt(vd * vm);
@@ -315,28 +341,25 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
             double q[N];
 #pragma unroll
             for (int a = 0; a < N; ++a) q[a] = trial ? pTry[a] : p[a];
-            FModel<MODE> m;
+            Model<MODE> m;
             m.set(q, fp.sigmaX, fp.theta);
             bool circ = false;
             if constexpr (MODE == 4) circ = fabs(q[4] - q[5]) <= 2.220446049250313e-16;
-            const float bx = (float)((double)(ux - half) - q[2]), by = (float)((double)(uy - half) - q[3]);
-            for (int i = 0, k = 0; i < roi; ++i) {
-                const float Y = by + (float)i;
-                for (int j = 0; j < roi; ++j, ++k) {
-                    const float X = bx + (float)j;
-                    float g[N];
-                    const float res = pix(k, i, j) - m.eval(X, Y, g);
-                    e += (double)res * (double)res;
-                    if constexpr (MODE == 4)
-                        if (circ) g[6] = 0.0f;
+            for_pixels(m, q[2], q[3], [&](int k, int i, int j, double X, double Y, double E) {
+                const double resd = (double)pix(k, i, j) - (m.off + m.amp * E);
+                e += resd * resd;
+                const float res = (float)resd;
+                float g[N];
+                m.grad((float)X, (float)Y, (float)E, g);
+                if constexpr (MODE == 4)
+                    if (circ) g[6] = 0.0f;
 #pragma unroll
-                    for (int a = 0; a < N; ++a) {
-                        beT[a] += g[a] * res;
+                for (int a = 0; a < N; ++a) {
+                    beT[a] += g[a] * res;
 #pragma unroll
-                        for (int b = a; b < N; ++b) alT[tri<N>(a, b)] += g[a] * g[b];
-                    }
+                    for (int b = a; b < N; ++b) alT[tri<N>(a, b)] += g[a] * g[b];
                 }
-            }
+            });
         }
 
         // ---- advance the LM state machine to its next pass (GaussianFit.cpp:75-165) ----
@@ -484,7 +507,7 @@ void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d
     const unsigned want = waves ? waves : 4096u;
     const unsigned grid = std::max(1u, std::min(want, ceil_div(nRoi, 64)));
     const bool staged = fp.roi <= kMaxStagedRoi;
-    const size_t lds = staged ? (size_t)fp.roi * fp.roi * 64 * sizeof(uint16_t) : 0;
+    const size_t lds = staged ? staged_lds_bytes(fp.roi) : 0;
     auto go = [&](auto kern) {
         kern<<<grid, 64, lds, s>>>(d_frames, npx, W, H, d_rois, nRoi, fp, d_rows, d_next, d_passes, frameMap);
     };

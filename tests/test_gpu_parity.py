@@ -121,7 +121,7 @@ def test_quadrant_matches_golden_csv(pt_gpu, ref_inputs, ref_csv):
 
 @pytest.mark.parametrize("fit", [1, 2, 3, 4])
 def test_localization_fp32_within_tolerance(pt_gpu, oracle_mod, ref_vectors, fit):
-    """FP32 model/Jacobian: positions within 1e-3 px, amplitude/offset/sigma within 1e-4 relative."""
+    """FP32 Jacobian, FP64 chi^2: positions within 1e-3 px, amplitude/offset/sigma within 1e-4 relative."""
     st, thr = ref_vectors["small"], float(ref_vectors["small_autothreshold"][0])
     pt_gpu.set_option("fit_fp64", 0)
     got = pt_gpu.localization(st, thr, 0.0, fit, FIT_PARAMS_SYN)
@@ -130,7 +130,8 @@ def test_localization_fp32_within_tolerance(pt_gpu, oracle_mod, ref_vectors, fit
     ok = (exp[:, AMP] > 0) & (got[:, AMP] > 0)
     assert ok.mean() > 0.9
     bad = fit_mismatches(got, exp)
-    assert len(bad) <= max(1, len(exp) // 200), (bad[:5], fit_errors(got, exp))
+    rows = {b[0] for b in bad}
+    assert len(rows) <= (0 if fit < 4 else max(1, len(exp) // 200)), (bad[:5], fit_errors(got, exp))
 
 
 def test_per_plane_capacity(pt_gpu, oracle_mod, ref_inputs):
@@ -320,7 +321,11 @@ def test_fp32_fit_512(pt_gpu, oracle_mod, frames512, fit):
     exp = oracle_mod.localization(st, thr, 0.0, fit, FIT_PARAMS_SYN)
     assert got.shape == exp.shape and len(exp) > 1000
     bad = fit_mismatches(got, exp)
-    assert len(bad) <= len(exp) // 200, (len(bad), bad[:5], fit_errors(got, exp))
+    rows = {b[0] for b in bad}
+    # Modes 1-3 agree to a few 1e-6 relative. Mode 4 leaves theta free: on nearly round spots
+    # (sigma_x ~ sigma_y) theta is undetermined and the FP32 gradient can steer a handful of fits
+    # to another point of that flat valley.
+    assert len(rows) <= (0 if fit < 4 else len(exp) // 200), (len(rows), bad[:5], fit_errors(got, exp))
 
 
 @pytest.mark.parametrize("waves", [1, 3, 64])
