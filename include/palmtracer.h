@@ -12,6 +12,9 @@
 #define PALMTRACER_AMD_H
 
 #include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
 
 #if defined(_WIN32) || defined(_WIN64)
 #define PT_EXPORT __declspec(dllexport)
@@ -58,6 +61,30 @@ PT_API void PT_CALL Wavelett(const uint16_t* input, double* output, uint64_t hei
  * Returns NaN on failure. */
 PT_API double PT_CALL AutoThreshold(const uint16_t* input, uint64_t height, uint64_t width, const double* fit_params);
 
+/* ---- reference host-side entry points (forwarded) ----------------------------------------- */
+/* Exported with the reference's signatures so that this library replaces the reference library
+ * as a whole. They are not part of the detection + fitting path and run on the host: each call is
+ * forwarded to the reference CPU library named by the environment variable PALMTRACER_HOST_LIBRARY.
+ * Without it the call prints an error on stderr and returns 0 / false, outputs untouched. */
+
+/* Alignment (CPU/main.cpp:47-48): drift correction of a stack; factors = 2 x 10 coefficients. */
+PT_API void PT_CALL Alignment(const uint16_t* input, uint16_t* output, uint64_t height, uint64_t width, uint64_t planes,
+                              const double* factors, uint64_t upsampling);
+/* Tracking (CPU/main.cpp:159): writes 2 * nPoints values. */
+PT_API void PT_CALL Tracking(const double* input, uint64_t* output, uint64_t nPoints, double maxDistance);
+/* BlinkingReconnection (CPU/main.cpp:185-186): returns the number of doubles written. */
+PT_API uint64_t PT_CALL BlinkingReconnection(const double* input, double* output, uint64_t nRow, double pixelSize,
+                                             uint64_t mode, uint64_t maxDuration, double maxSpeed);
+/* TrackAnalysis (CPU/main.cpp:231-233). */
+PT_API bool PT_CALL TrackAnalysis(const double* input, double* outputMSD, double* outputInd, double* outputFit, uint64_t nRow,
+                                  bool isMSD, bool isInd, bool is3D, double pixelSize, double exposureTime, uint64_t fitMode,
+                                  const double* fitParams);
+/* Astigmatism3DCalibration (CPU/main.cpp:293): 10 model parameters from (sigmaX, sigmaY, z) points. */
+PT_API void PT_CALL Astigmatism3DCalibration(const double* input, double* output, uint64_t size, double pixelSize, bool center);
+/* Astigmatism3DEstimation (CPU/main.cpp:324-325): (z nm, Q_Z) per (sigmaX, sigmaY) pair. */
+PT_API void PT_CALL Astigmatism3DEstimation(const double* input, double* output, uint64_t size, double pixelSize,
+                                            const double* model, double zMax);
+
 /* ---- no counterpart in the reference ------------------------------------------------------ */
 
 /* AutoThreshold of one frame in device memory. */
@@ -89,8 +116,8 @@ PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint6
 PT_API int PT_CALL pt_set_option(const char* name, double value);
 
 /* Counters of the last call: frames, chunks, rows, frames flagged for the exact frame flood,
- * multi-label components, frames deferred to the final pass, status. Writes min(n, 7) values,
- * returns 7. */
+ * multi-label components, frames deferred to the final pass, status, component floods given up on a
+ * full queue, component floods given up on an unproven order. Writes min(n, 9) values, returns 9. */
 PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n);
 
 /* With the "profile" option, device milliseconds of the last Localization per stage (a trous,

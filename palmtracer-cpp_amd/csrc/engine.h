@@ -21,6 +21,7 @@ struct DevBuf {
 
 struct RunStats {
     uint64_t frames = 0, chunks = 0, rois = 0, flaggedFrames = 0, multiComponents = 0, maskPixels = 0, deferredFrames = 0;
+    uint64_t flagOverflow = 0, flagUnproven = 0;  // component floods given up: full queue / unproven order
     uint32_t status = 0;
     // With Options::profile: device time per stage (HIP events on the engine stream, summed over
     // chunks) and the LM passes executed (model + Jacobian, trial model).

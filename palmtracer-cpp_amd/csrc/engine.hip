@@ -191,6 +191,8 @@ uint32_t Engine::detect_chunk(const uint16_t* d_frames, size_t F, int H, int W, 
     PT_CHECK(hipStreamSynchronize(stream));
     last.flaggedFrames += ctr[kFlagCnt];
     last.multiComponents += ctr[kMultiCnt];
+    last.flagOverflow += ctr[kFlagOverflow];
+    last.flagUnproven += ctr[kFlagUnproven];
     last.status |= ctr[kStatus];
     if (ctr[kStatus]) throw Error("segmentation status " + std::to_string(ctr[kStatus]));
     if (!resolve && deferred && ctr[kFlagCnt]) {

@@ -192,10 +192,10 @@ PT_API int PT_CALL pt_set_option(const char* name, double value)
 PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n)
 {
     Engine& e = Engine::get();
-    const uint64_t v[7] = {e.last.frames, e.last.chunks, e.last.rois, e.last.flaggedFrames, e.last.multiComponents,
-                           e.last.deferredFrames, e.last.status};
-    for (uint64_t i = 0; i < n && i < 7; ++i) out[i] = v[i];
-    return 7;
+    const uint64_t v[9] = {e.last.frames, e.last.chunks, e.last.rois, e.last.flaggedFrames, e.last.multiComponents,
+                           e.last.deferredFrames, e.last.status, e.last.flagOverflow, e.last.flagUnproven};
+    for (uint64_t i = 0; i < n && i < 9; ++i) out[i] = v[i];
+    return 9;
 }
 
 PT_API int PT_CALL pt_stage_times(double* out, uint64_t n)

@@ -70,7 +70,12 @@ struct DetectBuffers {
 };
 
 // Device counters of one chunk (u32 words).
-enum Counter : int { kMultiCnt = 0, kStatus = 1, kSlowCnt = 2, kSlowNext = 3, kFlagCnt = 4, kFbNext = 5, kNumCounters = 8 };
+enum Counter : int {
+    kMultiCnt = 0, kStatus = 1, kSlowCnt = 2, kSlowNext = 3, kFlagCnt = 4, kFbNext = 5,
+    kFlagOverflow = 6,   // component floods given up on a full queue
+    kFlagUnproven = 7,   // component floods given up on an unproven order or equal keys
+    kNumCounters = 8
+};
 // kStatus bits: 1 = region slots exhausted, 2 = unlabelled mask pixel (internal error).
 
 __host__ __device__ inline size_t fb_slot_words(size_t npx) { return 5 * ((npx + 1) & ~size_t(1)); }

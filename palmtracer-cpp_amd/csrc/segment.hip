@@ -396,7 +396,7 @@ __global__ void __launch_bounds__(64) flood_kernel(const uint32_t* __restrict__ 
         uint32_t* fn = fin + (size_t)f * npx;
         // Seeds of the component that the reference would push (labelled, next to an unlabelled pixel).
         uint32_t n = 0;
-        bool ok = true;
+        bool ok = true, overflow = false;
         const uint32_t ns = fz ? (uint32_t)min((size_t)rootCnt[f], regionCap) : seedCnt[f];
         for (uint32_t b0 = 0; b0 < ns && ok; b0 += 64) {
             const uint32_t k = b0 + lane;
@@ -414,7 +414,7 @@ __global__ void __launch_bounds__(64) flood_kernel(const uint32_t* __restrict__ 
             }
             const uint64_t m = __ballot(sel);
             const uint32_t cnt = __popcll(m);
-            if (n + cnt > q.cap) { ok = false; break; }
+            if (n + cnt > q.cap) { ok = false; overflow = true; break; }
             if (sel) waveq_put(q, n + __popcll(m & ((1ull << lane) - 1ull)), sp, fn[sp], Pf[sp], 0u);
             n += cnt;
         }
@@ -437,9 +437,12 @@ __global__ void __launch_bounds__(64) flood_kernel(const uint32_t* __restrict__ 
             };
             if (!waveq_pop<true>(q, n, a, la, proven, check) || !proven) { ok = false; break; }
             ++it;
-            if (!waveq_expand(q, n, Pf, fn, W, H, a, la, it)) { ok = false; break; }
+            if (!waveq_expand(q, n, Pf, fn, W, H, a, la, it)) { ok = false; overflow = true; break; }
         }
-        if (!ok && lane == 0) flag_frame(frameFlag, counters, flagList, f);
+        if (!ok && lane == 0) {
+            atomicAdd(&counters[overflow ? kFlagOverflow : kFlagUnproven], 1u);
+            flag_frame(frameFlag, counters, flagList, f);
+        }
         __syncthreads();
     }
 }

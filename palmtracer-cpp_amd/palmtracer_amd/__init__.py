@@ -145,9 +145,10 @@ def set_option(name, value):
 
 
 def last_stats():
-    v = (ctypes.c_uint64 * 7)()
-    lib().pt_last_stats(v, 7)
-    keys = ["frames", "chunks", "rows", "flagged_frames", "multi_components", "deferred_frames", "status"]
+    v = (ctypes.c_uint64 * 9)()
+    lib().pt_last_stats(v, 9)
+    keys = ["frames", "chunks", "rows", "flagged_frames", "multi_components", "deferred_frames", "status",
+            "flag_overflow", "flag_unproven"]
     return dict(zip(keys, list(v)))
 
 

@@ -44,3 +44,37 @@ def test_package_fails_loudly_without_library(tmp_path, monkeypatch):
     monkeypatch.setattr(mod, "_lib", None)
     with pytest.raises(RuntimeError):
         mod.lib()
+
+
+FORWARD_PROBE = r'''
+import ctypes, sys
+lib = ctypes.CDLL(sys.argv[1])
+lib.BlinkingReconnection.restype = ctypes.c_uint64
+lib.BlinkingReconnection.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64,
+                                     ctypes.c_uint64, ctypes.c_double]
+lib.TrackAnalysis.restype = ctypes.c_bool
+print(lib.BlinkingReconnection(None, None, 7, 0.1, 1, 3, 2.0))
+'''
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+@pytest.mark.parametrize("with_host", [False, True])
+def test_host_exports_forward(tmp_path, with_host):
+    """Host-side exports go to the library named by PALMTRACER_HOST_LIBRARY, else return 0."""
+    env = dict(os.environ)
+    env.pop("PALMTRACER_HOST_LIBRARY", None)
+    if with_host:
+        src = tmp_path / "host.c"
+        src.write_text("#include <stdint.h>\n"
+                       "uint64_t BlinkingReconnection(const double* i, double* o, uint64_t n, double p, uint64_t m,"
+                       " uint64_t d, double s) { return n * 6 + m + d; }\n")
+        so = tmp_path / "libhost.so"
+        subprocess.run(["gcc", "-shared", "-fPIC", str(src), "-o", str(so)], check=True)
+        env["PALMTRACER_HOST_LIBRARY"] = str(so)
+    probe = tmp_path / "probe.py"
+    probe.write_text(FORWARD_PROBE)
+    r = subprocess.run([os.sys.executable, str(probe), LIB], capture_output=True, text=True, env=env, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == ("46" if with_host else "0")
+    if not with_host:
+        assert "PALMTRACER_HOST_LIBRARY" in r.stderr
