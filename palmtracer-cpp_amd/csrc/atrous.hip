@@ -134,86 +134,232 @@ __global__ void __launch_bounds__(NT) atrous_plane_kernel(const uint16_t* __rest
                    });
 }
 
-// Detection front end: P = 65536 * W1 where W1 > threshold (and > 0), else 0; local maxima.
-//   P[frame*npx + i]           dense, every pixel
+// ---------------------------------------------------------------------------------------------
+// Detection front end, one wave per (column strip, row segment) of a frame.
+//
+// A strip is 64 lanes x SC columns (lane l holds columns 3l, 3l+1, 3l+2 of the strip); it covers
+// SOUT output columns plus a 7-column halo on each side (the dependency reach of the local-maximum
+// test through both scales). The wave walks its segment's rows top to bottom as a pipeline: input
+// row t gives A(t) (row pass, scale 0), V1s(t-2) (column pass from a 5-row ring of A), B(t-2) (row
+// pass, scale 1), V2s(t-6) (column pass from a 9-row ring of B), P(t-6) and the local-maximum test
+// of row t-7 (3-row ring of P). Horizontal neighbours come from the adjacent lanes through DPP
+// wave shifts; vertical neighbours from registers. Clamp-to-edge is applied per plane as in the
+// reference: input rows and columns are read clamped (so A is right everywhere), V1s columns
+// outside the frame are replaced by the edge column, and the B ring is filled with B(0) at the top
+// and repeats B(H-1) below the frame.
+//
+//   P[frame*npx + i]           dense, every pixel: 65536 * W1 if >= tmin, else 0
 //   lab[frame*npx + i] = i     for pixels with P > 0 (union-find initialisation)
 //   maskList / seedList        per-frame compact lists (capacity npx each) of pixel indices with
 //                              P > 0 / local maxima (Segmentation2D.cpp:294-333 test: no 8-neighbour
 //                              inside the frame strictly greater), counts in maskCnt / seedCnt.
-__global__ void __launch_bounds__(NT) atrous_detect_kernel(const uint16_t* __restrict__ in, uint32_t* __restrict__ P,
+constexpr int SC = 3;                   // columns per lane
+constexpr int SW = 64 * SC;             // strip width
+constexpr int SHALO = 7;                // halo columns on each side
+constexpr int SOUT = SW - 2 * SHALO;    // output columns per strip
+
+// Value of lane (l - 1) / (l + 1) of the wave (DPP wave shifts; edge lanes read 0).
+__device__ inline uint32_t from_left(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false); }
+__device__ inline uint32_t from_right(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false); }
+
+__device__ inline uint32_t b3(uint32_t m2, uint32_t m1, uint32_t c, uint32_t p1, uint32_t p2)
+{
+    return m2 + p2 + 4u * (m1 + p1) + 6u * c;
+}
+
+// Slot k of value-from-uniform-position: v[k] where k is wave-uniform.
+__device__ inline uint32_t pick(const uint32_t (&v)[SC], int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
+
+__global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __restrict__ in, uint32_t* __restrict__ P,
                                                            uint32_t* __restrict__ lab, uint32_t* __restrict__ maskList,
                                                            uint32_t* __restrict__ maskCnt, uint32_t* __restrict__ seedList,
                                                            uint32_t* __restrict__ seedCnt, int W, int H, int64_t tmin,
-                                                           const uint32_t* __restrict__ frameMap)
+                                                           int strips, int segH, const uint32_t* __restrict__ frameMap)
 {
-    __shared__ uint32_t l0[L0_SIZE];
-    __shared__ uint32_t l1[L1_SIZE];
-    __shared__ uint32_t lp[L2_SIZE];
-    __shared__ uint32_t sCnt[2], sBase[2];
+    const int lane = threadIdx.x;
+    const int f = blockIdx.y;
+    const int strip = blockIdx.x % strips, seg = blockIdx.x / strips;
     const size_t npx = (size_t)W * H;
-    const int f = blockIdx.z;
     const uint16_t* img = in + (size_t)(frameMap ? frameMap[f] : (uint32_t)f) * npx;
-    const int tx0 = blockIdx.x * TILE_W, ty0 = blockIdx.y * TILE_H;
-    if (threadIdx.x < 2) sCnt[threadIdx.x] = 0;
-    Win O, R1;
-    atrous_tile<RING>(img, W, H, tx0, ty0, l0, l1, O, R1, [&](int i, int, int, uint32_t v1s, uint32_t v2s) {
-        const int64_t w1s = (int64_t)v1s * 256 - (int64_t)v2s;
-        lp[i] = w1s >= tmin ? (uint32_t)w1s : 0u;
-    });
-    __syncthreads();
+    uint32_t* Pf = P + (size_t)f * npx;
+    uint32_t* lb = lab + (size_t)f * npx;
+    uint32_t* ml = maskList + (size_t)f * npx;
+    uint32_t* sl = seedList + (size_t)f * npx;
+    const int xs = strip * SOUT - SHALO;    // frame column of strip column 0
+    const int ys = seg * segH, ye = min(ys + segH, H);
 
-    // Each thread owns up to 8 tile pixels: a column x and rows r, r+4, ..., so rows stay coalesced.
-    const int ow = O.w();
-    const int lx = threadIdx.x % TILE_W, ly0 = threadIdx.x / TILE_W;
-    uint32_t myMask = 0, mySeed = 0;  // bit k: pixel k of this thread
-    uint32_t pv[TILE_H / 4];
-    const int x = tx0 + lx;
+    int xcol[SC], xcl[SC];
+    bool outCol[SC], inFrame[SC];
 #pragma unroll
-    for (int k = 0; k < TILE_H / 4; ++k) {
-        const int y = ty0 + ly0 + 4 * k;
-        pv[k] = 0;
-        if (x >= W || y >= H) continue;
-        const int li = (y - O.y0) * ow + (x - O.x0);
-        const uint32_t v = lp[li];
-        pv[k] = v;
-        if (v == 0) continue;
-        myMask |= 1u << k;
-        bool mx = true;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int yy = y + dy;
-            if (yy < 0 || yy >= H) continue;
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int xx = x + dx;
-                if ((dx | dy) == 0 || xx < 0 || xx >= W) continue;
-                if (v < lp[(yy - O.y0) * ow + (xx - O.x0)]) mx = false;
+    for (int k = 0; k < SC; ++k) {
+        const int o = SC * lane + k;
+        xcol[k] = xs + o;
+        xcl[k] = xcol[k] < 0 ? 0 : (xcol[k] > W - 1 ? W - 1 : xcol[k]);
+        inFrame[k] = xcol[k] >= 0 && xcol[k] < W;
+        outCol[k] = inFrame[k] && o >= SHALO && o < SW - SHALO;
+    }
+    // Strip positions of the frame's first / last column (wave-uniform) for the V1s edge fix-up.
+    const int oL = -xs, oR = W - 1 - xs;
+    const bool fixL = oL > 0, fixR = oR < SW - 1;
+
+    uint32_t rA[5][SC], rB[9][SC], rV[5][SC], rP[3][SC], rPl[3], rPr[3];
+#pragma unroll
+    for (int k = 0; k < SC; ++k) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) { rA[i][k] = 0; rV[i][k] = 0; }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) rB[i][k] = 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) rP[i][k] = 0;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { rPl[i] = 0; rPr[i] = 0; }
+
+    auto load_row = [&](int t, uint32_t (&v)[SC]) {
+        const int y = t < 0 ? 0 : (t > H - 1 ? H - 1 : t);
+        const uint16_t* r = img + (size_t)y * W;
+#pragma unroll
+        for (int k = 0; k < SC; ++k) v[k] = r[xcl[k]];
+    };
+
+    const int t0 = ys - 7, t1 = ye + 6;
+    uint32_t nxt[SC];
+    load_row(t0, nxt);
+    for (int t = t0; t <= t1; ++t) {
+        uint32_t v0[SC];
+#pragma unroll
+        for (int k = 0; k < SC; ++k) v0[k] = nxt[k];
+        if (t < t1) load_row(t + 1, nxt);
+
+        // A(t) = 16 * rowpass_0(V0): taps at columns c-2..c+2.
+        {
+            const uint32_t l1 = from_left(v0[1]), l2 = from_left(v0[2]);
+            const uint32_t r0 = from_right(v0[0]), r1 = from_right(v0[1]);
+            uint32_t a[SC];
+            a[0] = b3(l1, l2, v0[0], v0[1], v0[2]);
+            a[1] = b3(l2, v0[0], v0[1], v0[2], r0);
+            a[2] = b3(v0[0], v0[1], v0[2], r0, r1);
+#pragma unroll
+            for (int k = 0; k < SC; ++k) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) rA[i][k] = rA[i + 1][k];
+                rA[4][k] = a[k];
             }
         }
-        if (mx) mySeed |= 1u << k;
-    }
-    // Dense P for the tile.
+        // V1s(y1) = colpass_0(A), y1 = t - 2.
+        const int y1 = t - 2;
+        uint32_t u[SC];
 #pragma unroll
-    for (int k = 0; k < TILE_H / 4; ++k) {
-        const int y = ty0 + ly0 + 4 * k;
-        if (x < W && y < H) P[(size_t)f * npx + (size_t)y * W + x] = pv[k];
-    }
-    const uint32_t nm = __popc(myMask), ns = __popc(mySeed);
-    uint32_t om = 0, os = 0;
-    if (nm) om = atomicAdd(&sCnt[0], nm);
-    if (ns) os = atomicAdd(&sCnt[1], ns);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        sBase[0] = sCnt[0] ? atomicAdd(&maskCnt[f], sCnt[0]) : 0;
-        sBase[1] = sCnt[1] ? atomicAdd(&seedCnt[f], sCnt[1]) : 0;
-    }
-    __syncthreads();
-    uint32_t* ml = maskList + (size_t)f * npx + sBase[0] + om;
-    uint32_t* sl = seedList + (size_t)f * npx + sBase[1] + os;
-    uint32_t* lb = lab + (size_t)f * npx;
+        for (int k = 0; k < SC; ++k) u[k] = b3(rA[0][k], rA[1][k], rA[2][k], rA[3][k], rA[4][k]);
+        // Columns outside the frame hold the edge column's V1s (clamp of the scale-1 taps).
+        if (fixL) {
+            const uint32_t e = __builtin_amdgcn_readlane(pick(u, oL % SC), oL / SC);
 #pragma unroll
-    for (int k = 0; k < TILE_H / 4; ++k) {
-        const uint32_t idx = (uint32_t)((ty0 + ly0 + 4 * k) * W + x);
-        if (myMask & (1u << k)) { *ml++ = idx; lb[idx] = idx; }
-        if (mySeed & (1u << k)) *sl++ = idx;
+            for (int k = 0; k < SC; ++k) if (xcol[k] < 0) u[k] = e;
+        }
+        if (fixR) {
+            const uint32_t e = __builtin_amdgcn_readlane(pick(u, oR % SC), oR / SC);
+#pragma unroll
+            for (int k = 0; k < SC; ++k) if (xcol[k] > W - 1) u[k] = e;
+        }
+#pragma unroll
+        for (int k = 0; k < SC; ++k) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) rV[i][k] = rV[i + 1][k];
+            rV[4][k] = u[k];
+        }
+        // B(y1) = 16 * rowpass_1(V1s): taps at columns c-4, c-2, c, c+2, c+4.
+        if (y1 >= 0) {
+            uint32_t bb[SC];
+            if (y1 < H) {
+                const uint32_t L0 = from_left(u[0]), L1 = from_left(u[1]), L2 = from_left(u[2]);
+                const uint32_t R0 = from_right(u[0]), R1 = from_right(u[1]), R2 = from_right(u[2]);
+                const uint32_t LL2 = from_left(L2), RR0 = from_right(R0);
+                bb[0] = b3(LL2, L1, u[0], u[2], R1);
+                bb[1] = b3(L0, L2, u[1], R0, R2);
+                bb[2] = b3(L1, u[0], u[2], R1, RR0);
+            } else {
+#pragma unroll
+                for (int k = 0; k < SC; ++k) bb[k] = rB[8][k];  // below the frame: B(H-1)
+            }
+#pragma unroll
+            for (int k = 0; k < SC; ++k) {
+                if (y1 == 0) {
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) rB[i][k] = bb[k];  // above the frame: B(0)
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) rB[i][k] = rB[i + 1][k];
+                }
+                rB[8][k] = bb[k];
+            }
+        }
+        // P(y2), y2 = t - 6: V2s = colpass_1(B) from rows y2-4, y2-2, y2, y2+2, y2+4.
+        const int y2 = t - 6;
+        uint32_t pr[SC];
+#pragma unroll
+        for (int k = 0; k < SC; ++k) {
+            pr[k] = 0;
+            if (y2 >= 0 && y2 < H && inFrame[k]) {
+                const uint32_t v2s = b3(rB[0][k], rB[2][k], rB[4][k], rB[6][k], rB[8][k]);
+                const int64_t w1s = (int64_t)rV[0][k] * 256 - (int64_t)v2s;
+                pr[k] = w1s >= tmin ? (uint32_t)w1s : 0u;
+            }
+        }
+        {
+            const uint32_t pl = from_left(pr[2]), prr = from_right(pr[0]);
+#pragma unroll
+            for (int k = 0; k < SC; ++k) {
+                rP[0][k] = rP[1][k];
+                rP[1][k] = rP[2][k];
+                rP[2][k] = pr[k];
+            }
+            rPl[0] = rPl[1]; rPl[1] = rPl[2]; rPl[2] = pl;
+            rPr[0] = rPr[1]; rPr[1] = rPr[2]; rPr[2] = prr;
+        }
+        // Output row y3 = t - 7: dense P, mask and local maxima.
+        const int y3 = t - 7;
+        if (y3 < ys) continue;
+        uint32_t mbits = 0, sbits = 0;
+#pragma unroll
+        for (int k = 0; k < SC; ++k) {
+            const uint32_t v = rP[1][k];
+            if (outCol[k]) Pf[(size_t)y3 * W + xcol[k]] = v;
+            if (!outCol[k] || v == 0) continue;
+            mbits |= 1u << k;
+            bool mx = true;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const uint32_t left = k == 0 ? rPl[r] : rP[r][k - 1];
+                const uint32_t right = k == SC - 1 ? rPr[r] : rP[r][k + 1];
+                if (v < left || v < right || (r != 1 && v < rP[r][k])) mx = false;
+            }
+            if (mx) sbits |= 1u << k;
+        }
+        const uint32_t nm = __popc(mbits), ns = __popc(sbits);
+        uint32_t tm = nm, tsd = ns;
+        // Wave prefix sums of the per-lane counts.
+        uint32_t pm = nm, ps = ns;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t qm = __shfl_up(pm, o), qs = __shfl_up(ps, o);
+            if (lane >= o) { pm += qm; ps += qs; }
+        }
+        tm = __shfl(pm, 63);
+        tsd = __shfl(ps, 63);
+        uint32_t bm = 0, bs = 0;
+        if (lane == 0) {
+            if (tm) bm = atomicAdd(&maskCnt[f], tm);
+            if (tsd) bs = atomicAdd(&seedCnt[f], tsd);
+        }
+        bm = __shfl(bm, 0) + pm - nm;
+        bs = __shfl(bs, 0) + ps - ns;
+#pragma unroll
+        for (int k = 0; k < SC; ++k) {
+            const uint32_t idx = (uint32_t)(y3 * W + xcol[k]);
+            if (mbits & (1u << k)) { ml[bm++] = idx; lb[idx] = idx; }
+            if (sbits & (1u << k)) sl[bs++] = idx;
+        }
     }
 }
 } // namespace
@@ -234,8 +380,16 @@ void launch_atrous_detect(const uint16_t* d_in, const DetectBuffers& b, int W, i
 {
     if (frames <= 0 || W <= 0 || H <= 0) return;
     if (frames > 65535) throw Error("launch_atrous_detect: at most 65535 frames per launch");
-    const dim3 grid(ceil_div(W, TILE_W), ceil_div(H, TILE_H), frames);
-    atrous_detect_kernel<<<grid, NT, 0, s>>>(d_in, b.P, b.lab, b.maskList, b.maskCnt, b.seedList, b.seedCnt, W, H, tmin, frameMap);
+    const int strips = (int)ceil_div(W, SOUT);
+    // Row segments: long enough to amortise the 13-row pipeline warm-up, short enough to give the
+    // chip a few waves per SIMD when the chunk is small.
+    int segH = 128;
+    while (segH > 32 && (size_t)frames * strips * ceil_div(H, segH) < 8192) segH /= 2;
+    const int segs = (int)ceil_div(H, segH);
+    if ((size_t)strips * segs > 0x7FFFFFFF) throw Error("launch_atrous_detect: frame too large");
+    const dim3 grid((unsigned)(strips * segs), (unsigned)frames);
+    atrous_detect_kernel<<<grid, 64, 0, s>>>(d_in, b.P, b.lab, b.maskList, b.maskCnt, b.seedList, b.seedCnt, W, H, tmin,
+                                             strips, segH, frameMap);
     PT_CHECK(hipGetLastError());
 }
 
