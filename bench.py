@@ -103,7 +103,7 @@ def main():
         pt.localization_device(stack, out, thr, 0.0, 2, fit_params, max_per_plane)
     torch.cuda.synchronize()
     stages = {"atrous_ms": 0.0, "segment_ms": 0.0, "fit_ms": 0.0, "assemble_ms": 0.0, "chunks": 0.0,
-              "jacobian_passes": 0.0, "trial_passes": 0.0}
+              "jacobian_passes": 0.0, "trial_passes": 0.0, "deferred_ms": 0.0}
     rows = 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()

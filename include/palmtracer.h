@@ -121,8 +121,10 @@ PT_API int PT_CALL pt_set_option(const char* name, double value);
 PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n);
 
 /* With the "profile" option, device milliseconds of the last Localization per stage (a trous,
- * segmentation, fit, assembly; HIP events on the library's stream, summed over chunks), the chunk
- * count and the LM passes executed (model + Jacobian, trial). Writes min(n, 7) values, returns 7. */
+ * segmentation, fit, assembly; HIP events on the library's main stream, summed over chunks), the
+ * chunk count, the LM passes executed (model + Jacobian, trial) and the device milliseconds of the
+ * deferred-frame batches on the side stream (overlapped with the main stream). Writes min(n, 8)
+ * values, returns 8. */
 PT_API int PT_CALL pt_stage_times(double* out, uint64_t n);
 
 /* Seeded synthetic stack on the device (benchmark / test input): d_out receives frames x height x

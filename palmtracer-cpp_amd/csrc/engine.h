@@ -24,8 +24,9 @@ struct RunStats {
     uint64_t flagOverflow = 0, flagUnproven = 0;  // component floods given up: full queue / unproven order
     uint32_t status = 0;
     // With Options::profile: device time per stage (HIP events on the engine stream, summed over
-    // chunks) and the LM passes executed (model + Jacobian, trial model).
-    double msAtrous = 0, msSegment = 0, msFit = 0, msAssemble = 0;
+    // chunks), the device time of the deferred-frame batches on the side stream, and the LM passes
+    // executed (model + Jacobian, trial model).
+    double msAtrous = 0, msSegment = 0, msFit = 0, msAssemble = 0, msDeferred = 0;
     uint64_t jacPasses = 0, trialPasses = 0;
 };
 
@@ -40,13 +41,27 @@ struct Options {
     unsigned fitWaves = 0;          // persistent FP32 fit: resident waves (0: 16 per CU)
 };
 
+// One stream with its own detection buffers. The engine runs the chunks on the main lane and the
+// frames deferred to the exact frame flood on the side lane, concurrently.
+struct Lane {
+    hipStream_t stream = nullptr;
+    DetectBuffers b{};
+    DevBuf P, lab, fin, aux, mask, seed, perFrame, regions, fz, order, multi, fb, counters, flag;
+    DevBuf rois, rows, staged, frameMap, fitNext;
+    uint32_t* pinned = nullptr;  // host-visible: frame map, roiBase and counters of an asynchronous batch
+    size_t pinnedWords = 0;
+    void prepare(size_t F, size_t npx);
+    void zero();
+    uint32_t* host(size_t words);
+};
+
 class Engine {
 public:
     static Engine& get();
     std::mutex mu;
     Options opt;
     RunStats last;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // the main lane's stream
 
     // Localization over a device-resident stack; rows written to d_out (capacity outCapRows).
     // Returns the number of doubles written (a multiple of 18).
@@ -61,29 +76,23 @@ public:
     // AutoThreshold of one device frame (lib/Palm.cpp:122-172).
     double autothreshold(const uint16_t* d_frame, int H, int W, const double* fitParams);
 
-    DevBuf in, out, rows, rois, staged, frameMap, planeInfo;
+    DevBuf in, out, planeInfo;
 
 private:
     Engine();
-    DetectBuffers b_{};
-    DevBuf P_, lab_, fin_, aux_, mask_, seed_, perFrame_, regions_, fz_, order_, multi_, fb_, counters_, flag_;
-    DevBuf atMask_, atPart_, atRows_, passes_, fitNext_;
-    unsigned int* fitNext()
-    {
-        fitNext_.ensure(256);
-        return fitNext_.as<unsigned int>();
-    }
+    Lane main_, side_;
+    hipEvent_t sideEv_[2] = {nullptr, nullptr};
+    DevBuf atMask_, atPart_, atRows_, passes_;
     std::vector<hipEvent_t> ev_;
     hipEvent_t event(size_t i);
-    void prepare(size_t F, size_t npx);
-    void zero_chunk(size_t F);
-    // Detection of F frames (chunk frame f = input frame map[f], or base + f without a map).
-    // Returns the ROI count; with resolve = false, frames that need the exact frame flood are listed
-    // in deferred (input frame indices) and produce no ROI.
-    uint32_t detect_chunk(const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio, uint64_t maxPoints,
-                          size_t evBase, const uint32_t* d_map, bool resolve, uint64_t firstFrame,
-                          std::vector<uint32_t>* deferred);
-    void grow_staging(size_t rows, size_t keepRows);
+    // Detection of F frames on lane L (chunk frame f = input frame map[f], or base + f without a
+    // map). sync: wait and return the ROI count (frames needing the exact frame flood are appended
+    // to deferred as input frame indices, with resolve = false); otherwise everything stays queued
+    // on the lane's stream and 0 is returned.
+    uint32_t detect_chunk(Lane& L, const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio,
+                          uint64_t maxPoints, size_t evBase, const uint32_t* d_map, bool resolve, uint64_t firstFrame,
+                          std::vector<uint32_t>* deferred, bool sync);
+    void grow_staging(Lane& L, size_t rows, size_t keepRows);
     FitParams fit_params(uint64_t fit, const double* fp) const;
 };
 

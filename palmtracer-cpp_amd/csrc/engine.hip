@@ -76,70 +76,87 @@ Engine& Engine::get()
 
 Engine::Engine()
 {
-    PT_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    PT_CHECK(hipStreamCreateWithFlags(&main_.stream, hipStreamNonBlocking));
+    PT_CHECK(hipStreamCreateWithFlags(&side_.stream, hipStreamNonBlocking));
+    stream = main_.stream;
+    PT_CHECK(hipEventCreate(&sideEv_[0]));
+    PT_CHECK(hipEventCreate(&sideEv_[1]));
     if (const char* v = std::getenv("PALMTRACER_FIT_PRECISION")) opt.fitFp64 = std::strcmp(v, "fp32") != 0;
     if (const char* v = std::getenv("PALMTRACER_MAX_ITER")) opt.maxIter = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("PALMTRACER_CHUNK_PIXELS")) opt.chunkPixels = std::max<size_t>(1, std::strtoull(v, nullptr, 10));
 }
 
-void Engine::prepare(size_t F, size_t npx)
+void Lane::prepare(size_t F, size_t npx)
 {
     const size_t n = F * npx;
-    P_.ensure(n * 4);
-    lab_.ensure(n * 4);
-    fin_.ensure(n * 4);
-    aux_.ensure(n * sizeof(PixAux));
-    mask_.ensure(n * 4);
-    seed_.ensure(n * 4);
-    perFrame_.ensure((F + 1) * 4 * 9);
+    P.ensure(n * 4);
+    lab.ensure(n * 4);
+    fin.ensure(n * 4);
+    aux.ensure(n * sizeof(PixAux));
+    mask.ensure(n * 4);
+    seed.ensure(n * 4);
+    perFrame.ensure((F + 1) * 4 * 9);
     const size_t regionCap = npx / 4 + 2;  // roots are pairwise non-adjacent: at most ceil(H/2)*ceil(W/2)
-    regions_.ensure(F * regionCap * sizeof(Region));
-    fz_.ensure(F * regionCap * sizeof(Fz));
-    order_.ensure(F * regionCap * 4);
+    regions.ensure(F * regionCap * sizeof(Region));
+    fz.ensure(F * regionCap * sizeof(Fz));
+    order.ensure(F * regionCap * 4);
     const size_t multiCap = n / 64 + 1024;
-    multi_.ensure(multiCap * 8);
+    multi.ensure(multiCap * 8);
     const size_t fbSlots = std::clamp<size_t>((size_t(1) << 26) / std::max<size_t>(npx, 1), 4, 64);
-    fb_.ensure(fbSlots * fb_slot_words(npx) * 4);
-    counters_.ensure(kNumCounters * 4);
-    flag_.ensure((F + 1) * 8);
-    uint32_t* pf = perFrame_.as<uint32_t>();
+    fb.ensure(fbSlots * fb_slot_words(npx) * 4);
+    counters.ensure(kNumCounters * 4);
+    flag.ensure((F + 1) * 8);
+    fitNext.ensure(256);
+    uint32_t* pf = perFrame.as<uint32_t>();
     const size_t s = F + 1;
-    b_.P = P_.as<uint32_t>();
-    b_.lab = lab_.as<uint32_t>();
-    b_.fin = fin_.as<uint32_t>();
-    b_.aux = aux_.as<PixAux>();
-    b_.maskList = mask_.as<uint32_t>();
-    b_.seedList = seed_.as<uint32_t>();
-    b_.maskCnt = pf;
-    b_.seedCnt = pf + s;
-    b_.rootCnt = pf + 2 * s;
-    b_.keep = pf + 3 * s;
-    b_.found = pf + 4 * s;
-    b_.frameFlag = pf + 5 * s;
-    b_.roiBase = pf + 6 * s;
-    b_.regions = regions_.as<Region>();
-    b_.fz = fz_.as<Fz>();
-    b_.order = order_.as<uint32_t>();
-    b_.multiList = multi_.as<uint64_t>();
-    b_.fbScratch = fb_.as<uint32_t>();
-    b_.counters = counters_.as<uint32_t>();
-    b_.flagList = flag_.as<uint32_t>();
-    b_.slowList = flag_.as<uint32_t>() + (F + 1);
-    b_.frames = F;
-    b_.npx = npx;
-    b_.regionCap = regionCap;
-    b_.multiCap = multiCap;
-    b_.fbSlots = fbSlots;
+    b.P = P.as<uint32_t>();
+    b.lab = lab.as<uint32_t>();
+    b.fin = fin.as<uint32_t>();
+    b.aux = aux.as<PixAux>();
+    b.maskList = mask.as<uint32_t>();
+    b.seedList = seed.as<uint32_t>();
+    b.maskCnt = pf;
+    b.seedCnt = pf + s;
+    b.rootCnt = pf + 2 * s;
+    b.keep = pf + 3 * s;
+    b.found = pf + 4 * s;
+    b.frameFlag = pf + 5 * s;
+    b.roiBase = pf + 6 * s;
+    b.regions = regions.as<Region>();
+    b.fz = fz.as<Fz>();
+    b.order = order.as<uint32_t>();
+    b.multiList = multi.as<uint64_t>();
+    b.fbScratch = fb.as<uint32_t>();
+    b.counters = counters.as<uint32_t>();
+    b.flagList = flag.as<uint32_t>();
+    b.slowList = flag.as<uint32_t>() + (F + 1);
+    b.frames = F;
+    b.npx = npx;
+    b.regionCap = regionCap;
+    b.multiCap = multiCap;
+    b.fbSlots = fbSlots;
 }
 
-void Engine::zero_chunk(size_t)
+void Lane::zero()
 {
     // Per-frame counters (6 arrays laid out with a stride of frames + 1 words, frames being the
     // chunk capacity, whatever the number of frames in this chunk) and the chunk counters; a fill
     // kernel rather than a memset so that the sequence can be captured in a graph.
-    fill_u32_kernel<<<64, 256, 0, stream>>>(b_.maskCnt, 6 * (b_.frames + 1), 0u);
-    fill_u32_kernel<<<1, 64, 0, stream>>>(b_.counters, kNumCounters, 0u);
+    fill_u32_kernel<<<64, 256, 0, stream>>>(b.maskCnt, 6 * (b.frames + 1), 0u);
+    fill_u32_kernel<<<1, 64, 0, stream>>>(b.counters, kNumCounters, 0u);
     PT_CHECK(hipGetLastError());
+}
+
+uint32_t* Lane::host(size_t words)
+{
+    if (words > pinnedWords) {
+        if (pinned) PT_CHECK(hipHostFree(pinned));
+        pinned = nullptr;
+        pinnedWords = 0;
+        PT_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pinned), words * 4, hipHostMallocDefault));
+        pinnedWords = words;
+    }
+    return pinned;
 }
 
 FitParams Engine::fit_params(uint64_t fit, const double* fp) const
@@ -170,25 +187,28 @@ hipEvent_t Engine::event(size_t i)
     return ev_[i];
 }
 
-uint32_t Engine::detect_chunk(const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio, uint64_t maxPoints,
-                              size_t evBase, const uint32_t* d_map, bool resolve, uint64_t firstFrame,
-                              std::vector<uint32_t>* deferred)
+uint32_t Engine::detect_chunk(Lane& L, const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio,
+                              uint64_t maxPoints, size_t evBase, const uint32_t* d_map, bool resolve,
+                              uint64_t firstFrame, std::vector<uint32_t>* deferred, bool sync)
 {
     const size_t npx = (size_t)W * H;
-    zero_chunk(F);
-    if (opt.profile) PT_CHECK(hipEventRecord(event(evBase + 0), stream));
-    launch_atrous_detect(d_frames, b_, W, H, (int)F, threshold_to_pmin(thr), stream, d_map);
-    if (opt.profile) PT_CHECK(hipEventRecord(event(evBase + 1), stream));
-    rois.ensure(F * (npx / 4 + 2) * sizeof(Roi));
-    // The pass over deferred frames (d_map set) sends every frame through the exact frame flood.
+    const hipStream_t st = L.stream;
+    L.zero();
+    const bool prof = opt.profile && sync;
+    if (prof) PT_CHECK(hipEventRecord(event(evBase + 0), st));
+    launch_atrous_detect(d_frames, L.b, W, H, (int)F, threshold_to_pmin(thr), st, d_map);
+    if (prof) PT_CHECK(hipEventRecord(event(evBase + 1), st));
+    L.rois.ensure(F * (npx / 4 + 2) * sizeof(Roi));
+    // Deferred frames (d_map set) go through the exact frame flood.
     const bool forceSeq = opt.forceSequential || d_map != nullptr;
-    launch_segment(b_, W, H, (int)F, maxPoints, ratio, rois.as<Roi>(), stream, forceSeq, opt.forceHeap, resolve);
-    if (opt.profile) PT_CHECK(hipEventRecord(event(evBase + 2), stream));
+    launch_segment(L.b, W, H, (int)F, maxPoints, ratio, L.rois.as<Roi>(), st, forceSeq, opt.forceHeap, resolve);
+    if (prof) PT_CHECK(hipEventRecord(event(evBase + 2), st));
+    if (!sync) return 0;
     uint32_t ctr[kNumCounters];
     uint32_t total = 0;
-    PT_CHECK(hipMemcpyAsync(&total, b_.roiBase + F, 4, hipMemcpyDeviceToHost, stream));
-    PT_CHECK(hipMemcpyAsync(ctr, b_.counters, sizeof(ctr), hipMemcpyDeviceToHost, stream));
-    PT_CHECK(hipStreamSynchronize(stream));
+    PT_CHECK(hipMemcpyAsync(&total, L.b.roiBase + F, 4, hipMemcpyDeviceToHost, st));
+    PT_CHECK(hipMemcpyAsync(ctr, L.b.counters, sizeof(ctr), hipMemcpyDeviceToHost, st));
+    PT_CHECK(hipStreamSynchronize(st));
     last.flaggedFrames += ctr[kFlagCnt];
     last.multiComponents += ctr[kMultiCnt];
     last.flagOverflow += ctr[kFlagOverflow];
@@ -197,22 +217,22 @@ uint32_t Engine::detect_chunk(const uint16_t* d_frames, size_t F, int H, int W, 
     if (ctr[kStatus]) throw Error("segmentation status " + std::to_string(ctr[kStatus]));
     if (!resolve && deferred && ctr[kFlagCnt]) {
         std::vector<uint32_t> fl(ctr[kFlagCnt]);
-        PT_CHECK(hipMemcpy(fl.data(), b_.flagList, fl.size() * 4, hipMemcpyDeviceToHost));
+        PT_CHECK(hipMemcpy(fl.data(), L.b.flagList, fl.size() * 4, hipMemcpyDeviceToHost));
         for (uint32_t f : fl) deferred->push_back((uint32_t)(firstFrame + f));
     }
     return total;
 }
 
-void Engine::grow_staging(size_t rowsNeeded, size_t keepRows)
+void Engine::grow_staging(Lane& L, size_t rowsNeeded, size_t keepRows)
 {
     const size_t need = std::max<size_t>(rowsNeeded, 1) * kCols * 8;
-    if (need <= staged.bytes) return;
+    if (need <= L.staged.bytes) return;
     DevBuf nb;
-    nb.ensure(std::max(need, staged.bytes * 3 / 2));
-    if (keepRows) PT_CHECK(hipMemcpyAsync(nb.p, staged.p, keepRows * kCols * 8, hipMemcpyDeviceToDevice, stream));
-    PT_CHECK(hipStreamSynchronize(stream));
-    staged.release();
-    staged = nb;
+    nb.ensure(std::max(need, L.staged.bytes * 3 / 2));
+    if (keepRows) PT_CHECK(hipMemcpyAsync(nb.p, L.staged.p, keepRows * kCols * 8, hipMemcpyDeviceToDevice, L.stream));
+    PT_CHECK(hipStreamSynchronize(L.stream));
+    L.staged.release();
+    L.staged = nb;
 }
 
 uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, double thr, double ratio, uint64_t fit,
@@ -223,13 +243,14 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     if (fit == 5) throw Error("fit mode 5 (spline PSF) is not part of this library");
     const size_t npx = (size_t)W * H;
     const size_t F = std::max<size_t>(1, std::min<size_t>(planes, std::min<size_t>(65535, opt.chunkPixels / npx)));
-    prepare(F, npx);
+    main_.prepare(F, npx);
     const FitParams fp = fit_params(fit, fitParams);
     unsigned long long* passes = nullptr;
     if (opt.profile) {
         passes_.ensure(16);
         passes = passes_.as<unsigned long long>();
         PT_CHECK(hipMemsetAsync(passes, 0, 16, stream));
+        PT_CHECK(hipStreamSynchronize(stream));
     }
     // Per plane: staged row offset and row count (rows of a plane are staged contiguously).
     std::vector<uint64_t> src(planes, 0);
@@ -237,42 +258,112 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     std::vector<uint32_t> deferred;
     std::vector<uint32_t> base(F + 1);
     uint64_t stagedRows = 0;
-    // One chunk: detection, fit of its ROIs, per-frame (Y, X) sort into the staging rows.
-    auto run_chunk = [&](const uint16_t* frames, size_t f, const uint32_t* d_map, const uint32_t* hostMap,
-                         uint64_t z0, bool resolve) {
+
+    // Side lane: frames whose component floods could not be proven are re-detected with the exact
+    // frame flood, fitted and sorted on the side stream while the main lane goes on. A batch is
+    // queued without host synchronisation and collected at a later chunk boundary.
+    const size_t SF = std::max<size_t>(1, std::min<size_t>(F, std::max<size_t>(64, (size_t(1) << 24) / npx)));
+    const uint64_t sideRowCap = SF * std::min<uint64_t>(maxPerPlane, npx / 4 + 2);
+    size_t deferredNext = 0;            // next entry of `deferred` not yet sent
+    size_t batchFirst = 0, batchCount = 0;
+    hipEvent_t sideStart = nullptr, sideDone = nullptr;
+    auto side_launch = [&]() {
+        batchCount = std::min(SF, deferred.size() - deferredNext);
+        batchFirst = deferredNext;
+        deferredNext += batchCount;
+        side_.prepare(SF, npx);
+        side_.rows.ensure(std::max<uint64_t>(sideRowCap, 1) * kCols * 8);
+        side_.staged.ensure(std::max<uint64_t>(sideRowCap, 1) * kCols * 8);
+        uint32_t* h = side_.host(SF + (SF + 1) + kNumCounters);
+        std::memcpy(h, deferred.data() + batchFirst, batchCount * 4);
+        side_.frameMap.ensure(SF * 4);
+        sideStart = sideEv_[0];
+        sideDone = sideEv_[1];
+        if (opt.profile) PT_CHECK(hipEventRecord(sideStart, side_.stream));
+        PT_CHECK(hipMemcpyAsync(side_.frameMap.p, h, batchCount * 4, hipMemcpyHostToDevice, side_.stream));
+        const uint32_t* dmap = side_.frameMap.as<uint32_t>();
+        detect_chunk(side_, d_in, batchCount, H, W, thr, ratio, maxPerPlane, 0, dmap, true, 0, nullptr, false);
+        launch_fit(d_in, npx, W, H, side_.rois.as<Roi>(), (uint32_t)std::min<uint64_t>(sideRowCap, 0xFFFFFFFFull), fp,
+                   side_.rows.as<double>(), opt.fitFp64, side_.fitNext.as<unsigned int>(), side_.stream, passes, dmap,
+                   opt.fitWaves, side_.b.roiBase + batchCount);
+        launch_sort_frames(side_.rows.as<double>(), side_.b.roiBase, (int)batchCount, side_.staged.as<double>(),
+                           side_.stream);
+        PT_CHECK(hipMemcpyAsync(h + SF, side_.b.roiBase, (batchCount + 1) * 4, hipMemcpyDeviceToHost, side_.stream));
+        PT_CHECK(hipMemcpyAsync(h + SF + SF + 1, side_.b.counters, kNumCounters * 4, hipMemcpyDeviceToHost, side_.stream));
+        PT_CHECK(hipEventRecord(sideDone, side_.stream));
+    };
+    // Waits for the batch in flight and appends its rows to the main staging buffer.
+    auto side_collect = [&]() {
+        if (!batchCount) return;
+        PT_CHECK(hipEventSynchronize(sideDone));
+        const uint32_t* h = side_.pinned;
+        const uint32_t* sbase = h + SF;
+        const uint32_t* ctr = h + SF + SF + 1;
+        last.status |= ctr[kStatus];
+        if (ctr[kStatus]) throw Error("segmentation status " + std::to_string(ctr[kStatus]));
+        if (opt.profile) {
+            float ms = 0;
+            PT_CHECK(hipEventElapsedTime(&ms, sideStart, sideDone));
+            last.msDeferred += ms;
+        }
+        const uint32_t n = sbase[batchCount];
+        grow_staging(main_, stagedRows + n, stagedRows);
+        if (n)
+            PT_CHECK(hipMemcpyAsync(main_.staged.as<double>() + stagedRows * kCols, side_.staged.p, (size_t)n * kCols * 8,
+                                    hipMemcpyDeviceToDevice, main_.stream));
+        for (size_t k = 0; k < batchCount; ++k) {
+            const uint64_t plane = deferred[batchFirst + k];
+            src[plane] = stagedRows + sbase[k];
+            cnt[plane] = sbase[k + 1] - sbase[k];
+        }
+        stagedRows += n;
+        batchCount = 0;
+        // The copy reads the side lane's staging buffer, which the next batch overwrites.
+        PT_CHECK(hipStreamSynchronize(main_.stream));
+    };
+    // A batch is sent when a full one is pending or no main chunk is left to wait for: one exact
+    // frame flood costs the same latency for one frame as for a full batch.
+    auto side_poll = [&](bool wait, bool last) {
+        while (true) {
+            if (batchCount) {
+                if (!wait && hipEventQuery(sideDone) == hipErrorNotReady) return;
+                side_collect();
+            }
+            const size_t pending = deferred.size() - deferredNext;
+            if (!pending || (!wait && !last && pending < SF)) return;
+            side_launch();
+            if (!wait) return;
+        }
+    };
+
+    for (uint64_t z0 = 0; z0 < planes; z0 += F) {
+        const size_t f = std::min<uint64_t>(F, planes - z0);
+        const uint16_t* frames = d_in + z0 * npx;
         const size_t eb = 6 * last.chunks;
-        const uint32_t n = detect_chunk(frames, f, H, W, thr, ratio, maxPerPlane, eb, d_map, resolve, z0,
-                                        resolve ? nullptr : &deferred);
-        PT_CHECK(hipMemcpyAsync(base.data(), b_.roiBase, (f + 1) * 4, hipMemcpyDeviceToHost, stream));
-        rows.ensure((size_t)std::max<uint32_t>(n, 1) * kCols * 8);
-        grow_staging(stagedRows + n, stagedRows);
-        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 3), stream));
-        if (n) launch_fit(frames, npx, W, H, rois.as<Roi>(), n, fp, rows.as<double>(), opt.fitFp64, fitNext(), stream,
-                          passes, d_map, opt.fitWaves);
-        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 4), stream));
-        if (n) launch_sort_frames(rows.as<double>(), b_.roiBase, (int)f, staged.as<double>() + stagedRows * kCols, stream);
-        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 5), stream));
-        PT_CHECK(hipStreamSynchronize(stream));
+        const uint32_t n = detect_chunk(main_, frames, f, H, W, thr, ratio, maxPerPlane, eb, nullptr, false, z0,
+                                        &deferred, true);
+        side_poll(false, z0 + f >= planes);
+        PT_CHECK(hipMemcpyAsync(base.data(), main_.b.roiBase, (f + 1) * 4, hipMemcpyDeviceToHost, main_.stream));
+        main_.rows.ensure((size_t)std::max<uint32_t>(n, 1) * kCols * 8);
+        grow_staging(main_, stagedRows + n, stagedRows);
+        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 3), main_.stream));
+        if (n) launch_fit(frames, npx, W, H, main_.rois.as<Roi>(), n, fp, main_.rows.as<double>(), opt.fitFp64,
+                          main_.fitNext.as<unsigned int>(), main_.stream, passes, nullptr, opt.fitWaves);
+        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 4), main_.stream));
+        if (n) launch_sort_frames(main_.rows.as<double>(), main_.b.roiBase, (int)f,
+                                  main_.staged.as<double>() + stagedRows * kCols, main_.stream);
+        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 5), main_.stream));
+        PT_CHECK(hipStreamSynchronize(main_.stream));
         for (size_t k = 0; k < f; ++k) {
-            const uint64_t plane = hostMap ? hostMap[k] : z0 + k;
-            src[plane] = stagedRows + base[k];
-            cnt[plane] = base[k + 1] - base[k];
+            src[z0 + k] = stagedRows + base[k];
+            cnt[z0 + k] = base[k + 1] - base[k];
         }
         stagedRows += n;
         last.chunks++;
-    };
-    for (uint64_t z0 = 0; z0 < planes; z0 += F) {
-        const size_t f = std::min<uint64_t>(F, planes - z0);
-        run_chunk(d_in + z0 * npx, f, nullptr, nullptr, z0, false);
     }
-    // Frames whose component floods could not be proven: all of them at once, exact frame flood.
+    side_poll(true, true);
     last.deferredFrames = deferred.size();
-    for (size_t d0 = 0; d0 < deferred.size(); d0 += F) {
-        const size_t f = std::min(F, deferred.size() - d0);
-        frameMap.ensure(f * 4);
-        PT_CHECK(hipMemcpyAsync(frameMap.p, deferred.data() + d0, f * 4, hipMemcpyHostToDevice, stream));
-        run_chunk(d_in, f, frameMap.as<uint32_t>(), deferred.data() + d0, 0, true);
-    }
+
     // Final placement in plane order.
     std::vector<uint64_t> dst(planes);
     uint64_t total = 0;
@@ -285,7 +376,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     PT_CHECK(hipMemcpyAsync(dSrc, src.data(), planes * 8, hipMemcpyHostToDevice, stream));
     PT_CHECK(hipMemcpyAsync(dDst, dst.data(), planes * 8, hipMemcpyHostToDevice, stream));
     PT_CHECK(hipMemcpyAsync(dCnt, cnt.data(), planes * 4, hipMemcpyHostToDevice, stream));
-    if (total) launch_scatter(staged.as<double>(), dSrc, dDst, dCnt, planes, d_out, stream);
+    if (total) launch_scatter(main_.staged.as<double>(), dSrc, dDst, dCnt, planes, d_out, stream);
     PT_CHECK(hipStreamSynchronize(stream));
     if (opt.profile) {
         for (size_t c = 0; c < last.chunks; ++c) {
@@ -327,15 +418,15 @@ uint64_t Engine::fit_positions(const uint16_t* d_in, uint64_t planes, int H, int
     dpos.ensure(n * kInCols * 8);
     dloc.ensure(n * 4);
     dbad.ensure(4);
-    rois.ensure(n * sizeof(Roi));
+    main_.rois.ensure(n * sizeof(Roi));
     PT_CHECK(hipMemcpyAsync(dpos.p, hostPos, n * kInCols * 8, hipMemcpyHostToDevice, stream));
     PT_CHECK(hipMemcpyAsync(dloc.p, local.data(), n * 4, hipMemcpyHostToDevice, stream));
     fill_u32_kernel<<<1, 64, 0, stream>>>(dbad.as<uint32_t>(), 1, 0u);
     const unsigned g = std::min<uint64_t>(4096, (n + 255) / 256);
-    positions_to_rois<<<g, 256, 0, stream>>>(dpos.as<double>(), n, planes, rois.as<Roi>(), dbad.as<uint32_t>());
+    positions_to_rois<<<g, 256, 0, stream>>>(dpos.as<double>(), n, planes, main_.rois.as<Roi>(), dbad.as<uint32_t>());
     PT_CHECK(hipGetLastError());
     const FitParams fp = fit_params(fit, fitParams);
-    launch_fit(d_in, (size_t)W * H, W, H, rois.as<Roi>(), (uint32_t)n, fp, d_out, opt.fitFp64, fitNext(), stream, nullptr,
+    launch_fit(d_in, (size_t)W * H, W, H, main_.rois.as<Roi>(), (uint32_t)n, fp, d_out, opt.fitFp64, main_.fitNext.as<unsigned int>(), stream, nullptr,
                nullptr, opt.fitWaves);
     positions_ids<<<g, 256, 0, stream>>>(dpos.as<double>(), dloc.as<uint32_t>(), n, d_out);
     PT_CHECK(hipGetLastError());
@@ -359,16 +450,16 @@ void Engine::wavelet(const uint16_t* d_in, uint64_t planes, int H, int W, int le
 void Engine::debug_labels(const uint16_t* d_frame, int H, int W, double thr, double ratio, uint32_t* d_labels)
 {
     const size_t npx = (size_t)W * H;
-    prepare(1, npx);
-    zero_chunk(1);
-    launch_atrous_detect(d_frame, b_, W, H, 1, threshold_to_pmin(thr), stream);
-    rois.ensure((npx / 4 + 2) * sizeof(Roi));
-    launch_segment(b_, W, H, 1, ~0ull, ratio, rois.as<Roi>(), stream, opt.forceSequential, opt.forceHeap);
+    main_.prepare(1, npx);
+    main_.zero();
+    launch_atrous_detect(d_frame, main_.b, W, H, 1, threshold_to_pmin(thr), stream);
+    main_.rois.ensure((npx / 4 + 2) * sizeof(Roi));
+    launch_segment(main_.b, W, H, 1, ~0ull, ratio, main_.rois.as<Roi>(), stream, opt.forceSequential, opt.forceHeap);
     // Labels only where P > 0 (fin is not written elsewhere).
-    masked_labels<<<256, 256, 0, stream>>>(b_.fin, b_.P, d_labels, npx);
+    masked_labels<<<256, 256, 0, stream>>>(main_.b.fin, main_.b.P, d_labels, npx);
     PT_CHECK(hipGetLastError());
     uint32_t ctr[kNumCounters];
-    PT_CHECK(hipMemcpyAsync(ctr, b_.counters, sizeof(ctr), hipMemcpyDeviceToHost, stream));
+    PT_CHECK(hipMemcpyAsync(ctr, main_.b.counters, sizeof(ctr), hipMemcpyDeviceToHost, stream));
     PT_CHECK(hipStreamSynchronize(stream));
     last = RunStats{};
     last.flaggedFrames = ctr[kFlagCnt];

@@ -201,10 +201,10 @@ PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n)
 PT_API int PT_CALL pt_stage_times(double* out, uint64_t n)
 {
     Engine& e = Engine::get();
-    const double v[7] = {e.last.msAtrous, e.last.msSegment, e.last.msFit, e.last.msAssemble, (double)e.last.chunks,
-                         (double)e.last.jacPasses, (double)e.last.trialPasses};
-    for (uint64_t i = 0; i < n && i < 7; ++i) out[i] = v[i];
-    return 7;
+    const double v[8] = {e.last.msAtrous, e.last.msSegment, e.last.msFit, e.last.msAssemble, (double)e.last.chunks,
+                         (double)e.last.jacPasses, (double)e.last.trialPasses, e.last.msDeferred};
+    for (uint64_t i = 0; i < n && i < 8; ++i) out[i] = v[i];
+    return 8;
 }
 
 } // extern "C"

@@ -193,11 +193,13 @@ __host__ __device__ constexpr size_t staged_lds_bytes(int roi)
 // chi^2.
 template <int MODE, bool STAGED>
 __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __restrict__ frames, size_t npx, int W, int H,
-                                                            const Roi* __restrict__ rois, uint32_t nRoi, FitParams fp,
+                                                            const Roi* __restrict__ rois, uint32_t nRoiMax, FitParams fp,
                                                             double* __restrict__ rows, unsigned int* __restrict__ next,
                                                             unsigned long long* __restrict__ passes,
-                                                            const uint32_t* __restrict__ frameMap)
+                                                            const uint32_t* __restrict__ frameMap,
+                                                            const uint32_t* __restrict__ count)
 {
+    const uint32_t nRoi = count ? *count : nRoiMax;
     constexpr int N = NP<MODE>::v;
     constexpr int T = Tri<N>::v;
     extern __shared__ __align__(16) unsigned char lds[];
@@ -494,11 +496,11 @@ __global__ void reset_next_kernel(unsigned int* next) { *next = 0u; }
 
 void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, uint32_t nRoi, const FitParams& fp,
                 double* d_rows, bool fp64, unsigned int* d_next, hipStream_t s, unsigned long long* d_passes,
-                const uint32_t* frameMap, unsigned waves)
+                const uint32_t* frameMap, unsigned waves, const uint32_t* d_count)
 {
     if (nRoi == 0) return;
     if (fp64 || fp.mode < 1 || fp.mode > 4) {
-        launch_fit_f64(d_frames, npx, W, H, d_rois, nRoi, fp, d_rows, s, d_passes, frameMap);
+        launch_fit_f64(d_frames, npx, W, H, d_rois, nRoi, fp, d_rows, s, d_passes, frameMap, d_count);
         return;
     }
     if (!d_next) throw Error("launch_fit: the FP32 fit needs a queue word");
@@ -509,7 +511,7 @@ void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d
     const bool staged = fp.roi <= kMaxStagedRoi;
     const size_t lds = staged ? staged_lds_bytes(fp.roi) : 0;
     auto go = [&](auto kern) {
-        kern<<<grid, 64, lds, s>>>(d_frames, npx, W, H, d_rois, nRoi, fp, d_rows, d_next, d_passes, frameMap);
+        kern<<<grid, 64, lds, s>>>(d_frames, npx, W, H, d_rois, nRoi, fp, d_rows, d_next, d_passes, frameMap, d_count);
     };
     switch (fp.mode * 2 + (staged ? 1 : 0)) {
     case 2: go(fit_persistent_kernel<1, false>); break;

@@ -107,12 +107,13 @@ struct FitParams {
 // lane; otherwise the FP32 persistent work-queue kernel, whose queue head is d_next (one word).
 // d_passes (optional) accumulates {Jacobian passes, trial passes}; frameMap maps ROI frames to
 // stack frames.
+// With d_count set, the ROI count is read on the device (nRoi is then an upper bound).
 void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, uint32_t nRoi, const FitParams& fp,
                 double* d_rows, bool fp64, unsigned int* d_next, hipStream_t s, unsigned long long* d_passes = nullptr,
-                const uint32_t* frameMap = nullptr, unsigned waves = 0);
+                const uint32_t* frameMap = nullptr, unsigned waves = 0, const uint32_t* d_count = nullptr);
 void launch_fit_f64(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, uint32_t nRoi,
                     const FitParams& fp, double* d_rows, hipStream_t s, unsigned long long* d_passes,
-                    const uint32_t* frameMap);
+                    const uint32_t* frameMap, const uint32_t* d_count = nullptr);
 
 // Per-frame (Y, X) sort (rows of frame f at d_rows + roiBase[f]) into d_staged with the same layout;
 // Index is set, Id and Plane are left for launch_scatter.

@@ -30,6 +30,7 @@ namespace {
 constexpr uint32_t kVisited = 0x80000000u;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int NT = 256;
+constexpr uint32_t RUN = 8;   // consecutive list entries per thread in the per-pixel reduction kernels
 
 __device__ inline uint32_t ld_relaxed(const uint32_t* p)
 {
@@ -187,11 +188,23 @@ __global__ void __launch_bounds__(NT) cc_flatten_kernel(uint32_t* __restrict__ l
     const int f = blockIdx.y;
     const uint32_t cnt = maskCnt[f];
     uint32_t* lb = lab + (size_t)f * npx;
-    for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < cnt; k += gridDim.x * NT) {
-        const uint32_t p = maskList[(size_t)f * npx + k];
-        const uint32_t r = uf_find(lb, p);
-        lb[p] = r;
-        atomicAdd(&aux[(size_t)f * npx + r].csize, 1u);
+    // Each thread takes RUN consecutive list entries (row runs of one component, mostly) and adds
+    // their count to the root once per run.
+    for (uint32_t k0 = (blockIdx.x * NT + threadIdx.x) * RUN; k0 < cnt; k0 += gridDim.x * NT * RUN) {
+        const uint32_t k1 = min(k0 + RUN, cnt);
+        uint32_t cur = kNone, n = 0;
+        for (uint32_t k = k0; k < k1; ++k) {
+            const uint32_t p = maskList[(size_t)f * npx + k];
+            const uint32_t r = uf_find(lb, p);
+            lb[p] = r;
+            if (r != cur) {
+                if (n) atomicAdd(&aux[(size_t)f * npx + cur].csize, n);
+                cur = r;
+                n = 0;
+            }
+            ++n;
+        }
+        if (n) atomicAdd(&aux[(size_t)f * npx + cur].csize, n);
     }
 }
 
@@ -447,7 +460,7 @@ __global__ void __launch_bounds__(64) flood_kernel(const uint32_t* __restrict__ 
     }
 }
 
-constexpr uint32_t QCAP_FB = 3840;   // 60 KB of LDS
+constexpr uint32_t QCAP_FB = 9216;   // 144 KB of LDS (one wave per CU)
 
 // Exact frame flood for flagged frames, one wave per frame: the reference's process (seeds in any
 // order, global pop count as the age) with the wave queue. A frame whose queue meets equal keys at
@@ -831,19 +844,43 @@ __global__ void __launch_bounds__(NT) region_stats_kernel(const uint32_t* __rest
     const int f = blockIdx.y;
     if (skip && skip[f]) return;   // deferred frame: its labels are resolved in a later pass
     const uint32_t cnt = maskCnt[f];
-    for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < cnt; k += gridDim.x * NT) {
-        const uint32_t p = maskList[(size_t)f * npx + k];
-        const uint32_t L = fin[(size_t)f * npx + p];
-        if (L == 0 || L > npx) { atomicOr(&counters[kStatus], 2u); continue; }
-        const uint32_t slot = aux[(size_t)f * npx + (L - 1)].rslot;
-        if (slot >= regionCap) { atomicOr(&counters[kStatus], 2u); continue; }
-        Region* r = regions + (size_t)f * regionCap + slot;
-        const uint64_t v = P[(size_t)f * npx + p];
-        const uint32_t y = p / W, x = p - y * W;
-        atomicAdd(&r->surf, 1u);
-        atomicAdd((unsigned long long*)&r->sP, (unsigned long long)v);
-        atomicAdd((unsigned long long*)&r->sPx, (unsigned long long)(v * x));
-        atomicAdd((unsigned long long*)&r->sPy, (unsigned long long)(v * y));
+    // Each thread takes RUN consecutive list entries (row runs of one region, mostly) and adds their
+    // sums to the region once per run (integer sums: the order of additions does not matter).
+    for (uint32_t k0 = (blockIdx.x * NT + threadIdx.x) * RUN; k0 < cnt; k0 += gridDim.x * NT * RUN) {
+        const uint32_t k1 = min(k0 + RUN, cnt);
+        uint32_t curL = 0, slot = kNone, n = 0;
+        uint64_t sP = 0, sPx = 0, sPy = 0;
+        auto flush = [&]() {
+            if (!n) return;
+            Region* r = regions + (size_t)f * regionCap + slot;
+            atomicAdd(&r->surf, n);
+            atomicAdd((unsigned long long*)&r->sP, (unsigned long long)sP);
+            atomicAdd((unsigned long long*)&r->sPx, (unsigned long long)sPx);
+            atomicAdd((unsigned long long*)&r->sPy, (unsigned long long)sPy);
+            n = 0;
+            sP = sPx = sPy = 0;
+        };
+        for (uint32_t k = k0; k < k1; ++k) {
+            const uint32_t p = maskList[(size_t)f * npx + k];
+            const uint32_t L = fin[(size_t)f * npx + p];
+            if (L == 0 || L > npx) { atomicOr(&counters[kStatus], 2u); continue; }
+            if (L != curL) {
+                const uint32_t s2 = aux[(size_t)f * npx + (L - 1)].rslot;
+                if (s2 >= regionCap) { atomicOr(&counters[kStatus], 2u); continue; }
+                curL = L;
+                if (s2 != slot) {
+                    flush();
+                    slot = s2;
+                }
+            }
+            const uint64_t v = P[(size_t)f * npx + p];
+            const uint32_t y = p / W, x = p - y * W;
+            ++n;
+            sP += v;
+            sPx += v * x;
+            sPy += v * y;
+        }
+        flush();
     }
 }
 

@@ -154,9 +154,10 @@ def last_stats():
 
 def stage_times():
     """Per-stage device milliseconds of the last Localization (needs set_option('profile', 1))."""
-    v = (ctypes.c_double * 7)()
-    lib().pt_stage_times(v, 7)
-    keys = ["atrous_ms", "segment_ms", "fit_ms", "assemble_ms", "chunks", "jacobian_passes", "trial_passes"]
+    v = (ctypes.c_double * 8)()
+    lib().pt_stage_times(v, 8)
+    keys = ["atrous_ms", "segment_ms", "fit_ms", "assemble_ms", "chunks", "jacobian_passes", "trial_passes",
+            "deferred_ms"]
     return dict(zip(keys, list(v)))
 
 
