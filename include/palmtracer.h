@@ -110,14 +110,16 @@ PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint6
 
 /* Options: "fit_fp64" (1: FP64 model and sums, 0: FP32), "max_iter" (LM iteration cap, 75),
  * "chunk_pixels" (frames per device chunk x pixels, 0: default 2^28), "profile" (1: stage timing),
- * "force_sequential_watershed" (1: every frame through the exact frame flood), "force_heap_watershed"
- * (1: that flood with the heap restatement of the reference), "fit_waves" (resident waves of the
+ * "force_sequential_watershed" (1: every frame through the exact frame flood), "force_wave_flood"
+ * (1: that flood without the rank flood), "force_heap_watershed" (1: that flood with the heap
+ * restatement of the reference), "fit_waves" (resident waves of the
  * persistent FP32 fit, 0: default). Returns 1 on success. */
 PT_API int PT_CALL pt_set_option(const char* name, double value);
 
 /* Counters of the last call: frames, chunks, rows, frames flagged for the exact frame flood,
  * multi-label components, frames deferred to the final pass, status, component floods given up on a
- * full queue, component floods given up on an unproven order. Writes min(n, 9) values, returns 9. */
+ * full queue, component floods given up on an unproven order, pops of the exact frame floods, their
+ * largest queue. Writes min(n, 11) values, returns 11. */
 PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n);
 
 /* With the "profile" option, device milliseconds of the last Localization per stage (a trous,

@@ -22,6 +22,7 @@ struct DevBuf {
 struct RunStats {
     uint64_t frames = 0, chunks = 0, rois = 0, flaggedFrames = 0, multiComponents = 0, maskPixels = 0, deferredFrames = 0;
     uint64_t flagOverflow = 0, flagUnproven = 0;  // component floods given up: full queue / unproven order
+    uint64_t floodPops = 0, floodMaxQueue = 0;    // exact frame flood: pops, largest queue
     uint32_t status = 0;
     // With Options::profile: device time per stage (HIP events on the engine stream, summed over
     // chunks), the device time of the deferred-frame batches on the side stream, and the LM passes
@@ -37,7 +38,7 @@ struct Options {
     size_t chunkPixels = kDefaultChunkPixels;
     bool profile = false;           // stage timing with HIP events + LM pass counters
     bool forceSequential = false;   // every frame through the exact frame flood
-    bool forceHeap = false;         // that flood through the libstdc++-order heap restatement
+    int exactPath = 0;              // exact frame flood from: 0 rank flood, 1 wave-queue flood, 2 heap
     unsigned fitWaves = 0;          // persistent FP32 fit: resident waves (0: 16 per CU)
 };
 
@@ -46,7 +47,7 @@ struct Options {
 struct Lane {
     hipStream_t stream = nullptr;
     DetectBuffers b{};
-    DevBuf P, lab, fin, aux, mask, seed, perFrame, regions, fz, order, multi, fb, counters, flag;
+    DevBuf P, lab, fin, aux, mask, seed, perFrame, regions, fz, order, multi, fb, counters, flag, rf;
     DevBuf rois, rows, staged, frameMap, fitNext;
     uint32_t* pinned = nullptr;  // host-visible: frame map, roiBase and counters of an asynchronous batch
     size_t pinnedWords = 0;

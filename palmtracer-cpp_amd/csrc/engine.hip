@@ -105,7 +105,9 @@ void Lane::prepare(size_t F, size_t npx)
     const size_t fbSlots = std::clamp<size_t>((size_t(1) << 26) / std::max<size_t>(npx, 1), 4, 64);
     fb.ensure(fbSlots * fb_slot_words(npx) * 4);
     counters.ensure(kNumCounters * 4);
-    flag.ensure((F + 1) * 8);
+    flag.ensure((F + 1) * 12);
+    const size_t rfSlots = std::clamp<size_t>(F, 1, 128);
+    rf.ensure(rfSlots * rf_slot_words(npx) * 4);
     fitNext.ensure(256);
     uint32_t* pf = perFrame.as<uint32_t>();
     const size_t s = F + 1;
@@ -130,6 +132,9 @@ void Lane::prepare(size_t F, size_t npx)
     b.counters = counters.as<uint32_t>();
     b.flagList = flag.as<uint32_t>();
     b.slowList = flag.as<uint32_t>() + (F + 1);
+    b.waveList = flag.as<uint32_t>() + 2 * (F + 1);
+    b.rfScratch = rf.as<uint32_t>();
+    b.rfSlots = rfSlots;
     b.frames = F;
     b.npx = npx;
     b.regionCap = regionCap;
@@ -201,7 +206,7 @@ uint32_t Engine::detect_chunk(Lane& L, const uint16_t* d_frames, size_t F, int H
     L.rois.ensure(F * (npx / 4 + 2) * sizeof(Roi));
     // Deferred frames (d_map set) go through the exact frame flood.
     const bool forceSeq = opt.forceSequential || d_map != nullptr;
-    launch_segment(L.b, W, H, (int)F, maxPoints, ratio, L.rois.as<Roi>(), st, forceSeq, opt.forceHeap, resolve);
+    launch_segment(L.b, W, H, (int)F, maxPoints, ratio, L.rois.as<Roi>(), st, forceSeq, opt.exactPath, resolve);
     if (prof) PT_CHECK(hipEventRecord(event(evBase + 2), st));
     if (!sync) return 0;
     uint32_t ctr[kNumCounters];
@@ -300,6 +305,8 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         const uint32_t* sbase = h + SF;
         const uint32_t* ctr = h + SF + SF + 1;
         last.status |= ctr[kStatus];
+        last.floodPops += ctr[kFloodPops];
+        last.floodMaxQueue = std::max<uint64_t>(last.floodMaxQueue, ctr[kFloodMaxQueue]);
         if (ctr[kStatus]) throw Error("segmentation status " + std::to_string(ctr[kStatus]));
         if (opt.profile) {
             float ms = 0;
@@ -454,7 +461,7 @@ void Engine::debug_labels(const uint16_t* d_frame, int H, int W, double thr, dou
     main_.zero();
     launch_atrous_detect(d_frame, main_.b, W, H, 1, threshold_to_pmin(thr), stream);
     main_.rois.ensure((npx / 4 + 2) * sizeof(Roi));
-    launch_segment(main_.b, W, H, 1, ~0ull, ratio, main_.rois.as<Roi>(), stream, opt.forceSequential, opt.forceHeap);
+    launch_segment(main_.b, W, H, 1, ~0ull, ratio, main_.rois.as<Roi>(), stream, opt.forceSequential, opt.exactPath);
     // Labels only where P > 0 (fin is not written elsewhere).
     masked_labels<<<256, 256, 0, stream>>>(main_.b.fin, main_.b.P, d_labels, npx);
     PT_CHECK(hipGetLastError());

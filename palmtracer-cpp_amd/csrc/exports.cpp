@@ -182,7 +182,8 @@ PT_API int PT_CALL pt_set_option(const char* name, double value)
         else if (n == "chunk_pixels") e.opt.chunkPixels = value > 0.0 ? (size_t)value : pt::Options::kDefaultChunkPixels;
         else if (n == "profile") e.opt.profile = value != 0.0;
         else if (n == "force_sequential_watershed") e.opt.forceSequential = value != 0.0;
-        else if (n == "force_heap_watershed") e.opt.forceHeap = value != 0.0;
+        else if (n == "force_heap_watershed") e.opt.exactPath = value != 0.0 ? 2 : 0;
+        else if (n == "force_wave_flood") e.opt.exactPath = value != 0.0 ? 1 : 0;
         else if (n == "fit_waves") e.opt.fitWaves = (unsigned)std::max(0.0, std::min(1048576.0, value));
         else throw pt::Error("unknown option '" + n + "'");
         return 1;
@@ -192,10 +193,11 @@ PT_API int PT_CALL pt_set_option(const char* name, double value)
 PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n)
 {
     Engine& e = Engine::get();
-    const uint64_t v[9] = {e.last.frames, e.last.chunks, e.last.rois, e.last.flaggedFrames, e.last.multiComponents,
-                           e.last.deferredFrames, e.last.status, e.last.flagOverflow, e.last.flagUnproven};
-    for (uint64_t i = 0; i < n && i < 9; ++i) out[i] = v[i];
-    return 9;
+    const uint64_t v[11] = {e.last.frames, e.last.chunks, e.last.rois, e.last.flaggedFrames, e.last.multiComponents,
+                            e.last.deferredFrames, e.last.status, e.last.flagOverflow, e.last.flagUnproven,
+                            e.last.floodPops, e.last.floodMaxQueue};
+    for (uint64_t i = 0; i < n && i < 11; ++i) out[i] = v[i];
+    return 11;
 }
 
 PT_API int PT_CALL pt_stage_times(double* out, uint64_t n)

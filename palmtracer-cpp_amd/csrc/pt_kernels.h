@@ -66,7 +66,9 @@ struct DetectBuffers {
     uint32_t* counters = nullptr; // [kNumCounters], see Counter
     uint32_t* flagList = nullptr; // [F] frames for the exact frame flood
     uint32_t* slowList = nullptr; // [F] frames for the heap restatement (equal keys met)
-    size_t frames = 0, npx = 0, regionCap = 0, multiCap = 0, fbSlots = 0;
+    uint32_t* waveList = nullptr; // [F] frames the rank flood could not settle (wave-queue flood)
+    uint32_t* rfScratch = nullptr;// rank-flood work space: rfSlots * rf_slot_words(npx) u32
+    size_t frames = 0, npx = 0, regionCap = 0, multiCap = 0, fbSlots = 0, rfSlots = 0;
 };
 
 // Device counters of one chunk (u32 words).
@@ -74,11 +76,19 @@ enum Counter : int {
     kMultiCnt = 0, kStatus = 1, kSlowCnt = 2, kSlowNext = 3, kFlagCnt = 4, kFbNext = 5,
     kFlagOverflow = 6,   // component floods given up on a full queue
     kFlagUnproven = 7,   // component floods given up on an unproven order or equal keys
-    kNumCounters = 8
+    kFloodPops = 8,      // pops of the exact frame flood (all frames of the launch)
+    kFloodMaxQueue = 9,  // largest queue of the exact frame flood
+    kRfNext = 10,        // rank flood: next flagged frame
+    kWaveCnt = 11,       // frames handed from the rank flood to the wave-queue flood
+    kNumCounters = 12
 };
 // kStatus bits: 1 = region slots exhausted, 2 = unlabelled mask pixel (internal error).
 
 __host__ __device__ inline size_t fb_slot_words(size_t npx) { return 5 * ((npx + 1) & ~size_t(1)); }
+// Rank flood: frames with at most kRankMax pixels in the flood (mask pixels but the seeds that are
+// never pushed). Work space per slot: pixel -> entry map, then 20 words per entry.
+constexpr uint32_t kRankMax = 16384;
+__host__ __device__ inline size_t rf_slot_words(size_t npx) { return ((npx + 1) & ~size_t(1)) + 20 * (size_t)kRankMax; }
 
 // A trous
 void launch_atrous_plane(const uint16_t* d_in, double* d_out, int W, int H, int planes, int level, hipStream_t s);
@@ -89,12 +99,13 @@ int64_t threshold_to_pmin(double thr);
 
 // Segmentation (ratio <= 0: watershed without fusion). Produces ROIs for frames [0, F) at
 // d_rois + roiBase[f], with per-frame truncation to maxPoints in label order.
-// forceSequential sends every frame through the exact frame flood; forceHeap makes that flood use
-// the libstdc++-order heap restatement instead of the wave queue. With resolve = false, frames whose
-// component floods could not be proven are left flagged (frameFlag / flagList) without rows, for a
-// later pass over them with forceSequential.
+// forceSequential sends every frame through the exact frame flood. That flood is the rank flood,
+// then, for frames it cannot settle, the wave-queue flood, then, for frames with equal keys at a
+// pop, the libstdc++-order heap restatement; exactPath = 1 starts at the wave-queue flood, 2 at the
+// heap restatement. With resolve = false, frames whose component floods could not be proven are
+// left flagged (frameFlag / flagList) without rows, for a later pass over them with forceSequential.
 void launch_segment(const DetectBuffers& b, int W, int H, int frames, uint64_t maxPoints, double ratio, Roi* d_rois,
-                    hipStream_t s, bool forceSequential = false, bool forceHeap = false, bool resolve = true);
+                    hipStream_t s, bool forceSequential = false, int exactPath = 0, bool resolve = true);
 
 // Gaussian LM fit of nRoi ROIs (count read from device memory *d_count when non-null).
 struct FitParams {

@@ -188,23 +188,11 @@ __global__ void __launch_bounds__(NT) cc_flatten_kernel(uint32_t* __restrict__ l
     const int f = blockIdx.y;
     const uint32_t cnt = maskCnt[f];
     uint32_t* lb = lab + (size_t)f * npx;
-    // Each thread takes RUN consecutive list entries (row runs of one component, mostly) and adds
-    // their count to the root once per run.
-    for (uint32_t k0 = (blockIdx.x * NT + threadIdx.x) * RUN; k0 < cnt; k0 += gridDim.x * NT * RUN) {
-        const uint32_t k1 = min(k0 + RUN, cnt);
-        uint32_t cur = kNone, n = 0;
-        for (uint32_t k = k0; k < k1; ++k) {
-            const uint32_t p = maskList[(size_t)f * npx + k];
-            const uint32_t r = uf_find(lb, p);
-            lb[p] = r;
-            if (r != cur) {
-                if (n) atomicAdd(&aux[(size_t)f * npx + cur].csize, n);
-                cur = r;
-                n = 0;
-            }
-            ++n;
-        }
-        if (n) atomicAdd(&aux[(size_t)f * npx + cur].csize, n);
+    for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < cnt; k += gridDim.x * NT) {
+        const uint32_t p = maskList[(size_t)f * npx + k];
+        const uint32_t r = uf_find(lb, p);
+        lb[p] = r;
+        atomicAdd(&aux[(size_t)f * npx + r].csize, 1u);
     }
 }
 
@@ -284,12 +272,21 @@ __global__ void __launch_bounds__(NT) label_assign_kernel(const uint32_t* __rest
 // caller) instead of being ordered.
 struct WaveQ {
     uint64_t* pa;      // (P << 32) | age
+    double* key;       // reference key of the entry, computed once at push
     uint32_t* pix;
     uint32_t* lbl;
     uint32_t cap;
 };
 
-__device__ inline double entry_key(uint64_t pa) { return age_key((uint32_t)(pa >> 32), (uint32_t)pa); }
+
+// Orders the LDS accesses of a one-wave workgroup. A wave's LDS instructions execute in order, so no
+// hardware wait is needed (a workgroup barrier would also wait for every outstanding global store);
+// this only keeps the compiler from moving memory accesses across the point.
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_wave_barrier();
+    __asm__ __volatile__("" ::: "memory");
+}
 
 // Neighbour j (0..7) of a pixel in the reference's scan order (rows dy, then columns dx).
 __device__ inline void nb_offset(int j, int& dy, int& dx)
@@ -302,6 +299,7 @@ __device__ inline void nb_offset(int j, int& dy, int& dx)
 __device__ inline void waveq_put(WaveQ& q, uint32_t pos, uint32_t pix, uint32_t lbl, uint32_t p, uint32_t age)
 {
     q.pa[pos] = ((uint64_t)p << 32) | age;
+    q.key[pos] = age_key(p, age);
     q.pix[pos] = pix;
     q.lbl[pos] = lbl;
 }
@@ -314,10 +312,23 @@ __device__ inline bool waveq_pop(WaveQ& q, uint32_t& n, uint32_t& outPix, uint32
     const int lane = threadIdx.x & 63;
     double km = 1.79769313486231570815e308;
     uint32_t pm = 0xFFFFFFFFu, cm = 0;
-    for (uint32_t i = lane; i < n; i += 64) {
-        const double k = entry_key(q.pa[i]);
-        if (k < km) { km = k; pm = i; cm = 1; }
-        else if (k == km) ++cm;
+    // One wave alone on its CU: issue a batch of LDS reads before comparing, so that their latency
+    // is paid once per batch rather than once per entry.
+    constexpr uint32_t U = 8;
+    for (uint32_t b = 0; b < n; b += 64 * U) {
+        double kk[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t i = b + u * 64 + lane;
+            kk[u] = i < n ? q.key[i] : 1.79769313486231570815e308;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t i = b + u * 64 + lane;
+            if (i >= n) continue;
+            if (kk[u] < km) { km = kk[u]; pm = i; cm = 1; }
+            else if (kk[u] == km) ++cm;
+        }
     }
     for (int o = 32; o > 0; o >>= 1) {
         const double k2 = __shfl_xor(km, o);
@@ -331,17 +342,29 @@ __device__ inline bool waveq_pop(WaveQ& q, uint32_t& n, uint32_t& outPix, uint32
     proven = true;
     if constexpr (kCheck) {
         bool bad = false;
-        for (uint32_t i = lane; i < n; i += 64)
-            if (i != pm && !check(apa, q.pa[i])) bad = true;
+        for (uint32_t b = 0; b < n; b += 64 * U) {
+            uint64_t pp[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint32_t i = b + u * 64 + lane;
+                pp[u] = i < n ? q.pa[i] : 0ull;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint32_t i = b + u * 64 + lane;
+                if (i < n && i != pm && !check(apa, pp[u])) bad = true;
+            }
+        }
         proven = __ballot(bad) == 0;
     }
-    __syncthreads();
+    wave_sync();
     if (lane == 0) {
         q.pa[pm] = q.pa[n - 1];
+        q.key[pm] = q.key[n - 1];
         q.pix[pm] = q.pix[n - 1];
         q.lbl[pm] = q.lbl[n - 1];
     }
-    __syncthreads();
+    wave_sync();
     n -= 1;
     outPix = a;
     outLbl = la;
@@ -363,7 +386,8 @@ __device__ inline bool waveq_expand(WaveQ& q, uint32_t& n, const uint32_t* Pf, u
         if (y >= 0 && y < H && x >= 0 && x < W) {
             qpix = (uint32_t)(y * W + x);
             qp = Pf[qpix];
-            const uint32_t ql = fn[qpix];  // loaded with the potential (one round trip)
+            // Labels written by this wave in earlier pops: read at L2 (coherent), with the potential.
+            const uint32_t ql = ld_relaxed(fn + qpix);
             claim = qp != 0 && ql == 0u;
         }
     }
@@ -372,9 +396,9 @@ __device__ inline bool waveq_expand(WaveQ& q, uint32_t& n, const uint32_t* Pf, u
     if (n + cnt > q.cap) return false;
     if (claim) {
         waveq_put(q, n + __popcll(m & ((1ull << lane) - 1ull)), qpix, la, qp, it);
-        fn[qpix] = la;
+        __hip_atomic_store(fn + qpix, la, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __syncthreads();
+    wave_sync();
     n += cnt;
     return true;
 }
@@ -395,8 +419,9 @@ __global__ void __launch_bounds__(64) flood_kernel(const uint32_t* __restrict__ 
                                                    size_t regionCap)
 {
     __shared__ uint64_t sPa[QCAP_FLOOD];
+    __shared__ double sKey[QCAP_FLOOD];
     __shared__ uint32_t sPix[QCAP_FLOOD], sLbl[QCAP_FLOOD];
-    WaveQ q{sPa, sPix, sLbl, QCAP_FLOOD};
+    WaveQ q{sPa, sKey, sPix, sLbl, QCAP_FLOOD};
     const int lane = threadIdx.x;
     const uint32_t nMulti = (uint32_t)min((size_t)counters[kMultiCnt], multiCap);
     for (uint32_t t = blockIdx.x; t < nMulti; t += gridDim.x) {
@@ -460,42 +485,119 @@ __global__ void __launch_bounds__(64) flood_kernel(const uint32_t* __restrict__ 
     }
 }
 
-constexpr uint32_t QCAP_FB = 9216;   // 144 KB of LDS (one wave per CU)
+constexpr uint32_t FB_LDS_BYTES = 163840;   // all of a CU's LDS: one wave per CU, one frame per wave
 
-// Exact frame flood for flagged frames, one wave per frame: the reference's process (seeds in any
-// order, global pop count as the age) with the wave queue. A frame whose queue meets equal keys at
-// a pop, or overflows, goes to the heap restatement below (fallback_kernel) through slowList.
-__global__ void __launch_bounds__(64) fallback_fast_kernel(const uint32_t* __restrict__ P, uint32_t* __restrict__ fin,
+// Frame state held in LDS by the exact frame flood: the mask as a bitmap with per-word prefix
+// counts (rank of a mask pixel = its index among the frame's mask pixels in raster order), the
+// potentials of the mask pixels by rank, and the labelled state by rank. Every neighbour test of
+// the flood is then an LDS access.
+struct FrameLds {
+    uint32_t* mbits;   // npx bits
+    uint32_t* pre;     // exclusive prefix of popc(mbits[w])
+    uint32_t* pc;      // P by rank
+    uint32_t* lbits;   // labelled, by rank
+    __device__ bool in_mask(uint32_t q) const { return (mbits[q >> 5] >> (q & 31)) & 1u; }
+    __device__ uint32_t rank(uint32_t q) const
+    {
+        const uint32_t w = mbits[q >> 5];
+        return pre[q >> 5] + __popc(w & ((1u << (q & 31)) - 1u));
+    }
+    __device__ bool labelled(uint32_t r) const { return (lbits[r >> 5] >> (r & 31)) & 1u; }
+};
+
+// waveq_expand on the LDS frame state: claims read and set the labelled bits, labels are only
+// written out to fin.
+__device__ inline bool waveq_expand_lds(WaveQ& q, uint32_t& n, const FrameLds& fr, uint32_t* fn, int W, int H, uint32_t a,
+                                        uint32_t la, uint32_t it)
+{
+    const int lane = threadIdx.x & 63;
+    bool claim = false;
+    uint32_t qpix = 0, qp = 0, r = 0;
+    if (lane < 8) {
+        int dy, dx;
+        nb_offset(lane, dy, dx);
+        const int y = (int)(a / W) + dy, x = (int)(a % W) + dx;
+        if (y >= 0 && y < H && x >= 0 && x < W) {
+            qpix = (uint32_t)(y * W + x);
+            if (fr.in_mask(qpix)) {
+                r = fr.rank(qpix);
+                qp = fr.pc[r];
+                claim = !fr.labelled(r);
+            }
+        }
+    }
+    const uint64_t m = __ballot(claim);
+    const uint32_t cnt = __popcll(m);
+    if (n + cnt > q.cap) return false;
+    if (claim) {
+        waveq_put(q, n + __popcll(m & ((1ull << lane) - 1ull)), qpix, la, qp, it);
+        atomicOr(&fr.lbits[r >> 5], 1u << (r & 31));
+        fn[qpix] = la;
+    }
+    wave_sync();
+    n += cnt;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Rank flood: the exact frame flood as a parallel fixed point.
+//
+// Number the pops of the reference's flood 1, 2, ... and let G(x) be the pop number of entry x.
+// A seed with an unlabelled neighbour is pushed at age T = 0; any other mask pixel is claimed,
+// labelled and pushed by its first popped neighbour, at age T(x) = min over its flooded neighbours
+// of G. Its key is K(x) = -W1(x) + T(x) * 1e-7. If the keys are pairwise distinct and every entry is
+// pushed before the pop at its key rank (T(x) < G(x) for every non-seed), the pop sequence is the
+// keys in increasing order, i.e. G(x) = 1 + #{y : K(y) < K(x)} (induction on the pop number: the
+// entry of rank m is present and is the smallest key present). Conversely the reference's G satisfies
+// both equations. So the flood is the fixed point of T = min-neighbour(G), G = rank(K(T)), found by
+// iterating from G = rank of W1 alone; keys only move by at most n * 1e-7 (n entries), so only
+// entries whose potentials lie within that distance ("the window", contiguous in potential order)
+// can swap ranks, and the rank of x is (entries with a potential above the window) + (window entries
+// with a smaller key). The labels follow the first-popped-neighbour links. A frame whose iteration
+// does not settle, meets two equal keys, or breaks T < G goes to the wave-queue flood.
+constexpr int RF_NT = 1024;
+constexpr uint32_t RF_NONE = 0xFFFFFFFFu;
+constexpr int RF_MAX_ITER = 64;
+
+__global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __restrict__ P, uint32_t* __restrict__ fin,
                                                            const uint32_t* __restrict__ maskList,
                                                            const uint32_t* __restrict__ maskCnt,
-                                                           const uint32_t* __restrict__ seedList,
-                                                           const uint32_t* __restrict__ seedCnt,
                                                            const uint32_t* __restrict__ flagList, uint32_t* __restrict__ counters,
-                                                           uint32_t* __restrict__ slowList, int W, int H, size_t npx,
-                                                           const Fz* __restrict__ fz, const Region* __restrict__ regions,
-                                                           const uint32_t* __restrict__ rootCnt, const PixAux* __restrict__ aux,
-                                                           size_t regionCap, int forceSlow)
+                                                           uint32_t* __restrict__ waveList, uint32_t* __restrict__ rfScratch,
+                                                           int W, int H, size_t npx, const Fz* __restrict__ fz,
+                                                           const PixAux* __restrict__ aux, size_t regionCap)
 {
-    __shared__ uint64_t sPa[QCAP_FB];
-    __shared__ uint32_t sPix[QCAP_FB], sLbl[QCAP_FB];
-    WaveQ q{sPa, sPix, sLbl, QCAP_FB};
-    const int lane = threadIdx.x;
+    __shared__ uint64_t sk[kRankMax];  // (~P << 32) | pixel, sorted ascending = potential descending
+    __shared__ uint32_t sN, sFrame;
+    const int tid = threadIdx.x;
+    uint32_t* idxOf = rfScratch + (size_t)blockIdx.x * rf_slot_words(npx);
+    uint32_t* sp = idxOf + ((npx + 1) & ~size_t(1));   // P by entry
+    uint32_t* spix = sp + kRankMax;                      // pixel by entry
+    uint32_t* lo = spix + kRankMax;                      // window [lo, hi) of the entry
+    uint32_t* hi = lo + kRankMax;
+    uint32_t* Ga = hi + kRankMax;                        // pop ranks (two buffers)
+    uint32_t* Gb = Ga + kRankMax;
+    uint32_t* T = Gb + kRankMax;                         // push ages
+    uint32_t* par = T + kRankMax;                        // first popped neighbour (labels)
+    uint32_t* lbl = par + kRankMax;
+    uint32_t* seedf = lbl + kRankMax;                    // entry is a seed
+    double* K = reinterpret_cast<double*>(seedf + kRankMax);
+    uint32_t* nbr = reinterpret_cast<uint32_t*>(K + kRankMax);  // [entry][8] flooded neighbours
     while (true) {
-        uint32_t i = 0;
-        if (lane == 0) i = atomicAdd(&counters[kFbNext], 1u);
-        i = __shfl(i, 0);
-        if (i >= counters[kFlagCnt]) return;
-        const uint32_t f = flagList[i];
-        if (forceSlow) {
-            if (lane == 0) slowList[atomicAdd(&counters[kSlowCnt], 1u)] = f;
-            continue;
-        }
+        __syncthreads();
+        if (tid == 0) sFrame = atomicAdd(&counters[kRfNext], 1u);
+        __syncthreads();
+        const uint32_t fi = sFrame;
+        if (fi >= counters[kFlagCnt]) return;
+        const uint32_t f = flagList[fi];
         const uint32_t* Pf = P + (size_t)f * npx;
         uint32_t* fn = fin + (size_t)f * npx;
+        const uint32_t* ml = maskList + (size_t)f * npx;
         const uint32_t cnt = maskCnt[f];
-        // 1. Labels back to the seed state.
-        for (uint32_t k = lane; k < cnt; k += 64) {
-            const uint32_t p = maskList[(size_t)f * npx + k];
+
+        // 1. Labels back to the seed state; seed marks in idxOf for now.
+        for (uint32_t k = tid; k < cnt; k += RF_NT) {
+            const uint32_t p = ml[k];
             bool seed;
             if (!fz) {
                 seed = is_local_max(Pf, W, H, (int)p);
@@ -509,6 +611,276 @@ __global__ void __launch_bounds__(64) fallback_fast_kernel(const uint32_t* __res
                 }
             }
             if (!seed) fn[p] = 0u;
+            idxOf[p] = seed ? 1u : 0u;
+        }
+        if (tid == 0) sN = 0;
+        __threadfence_block();
+        __syncthreads();
+        // 2. Flood entries: non-seeds, and seeds next to an unlabelled pixel.
+        for (uint32_t k = tid; k < cnt; k += RF_NT) {
+            const uint32_t p = ml[k];
+            const bool seed = idxOf[p] != 0u;
+            if (!seed || has_unlabelled_nbr(Pf, fn, W, H, (int)p)) {
+                const uint32_t j = atomicAdd(&sN, 1u);
+                if (j < kRankMax) sk[j] = ((uint64_t)(~Pf[p]) << 32) | p;
+            }
+        }
+        __syncthreads();
+        const uint32_t n = sN;
+        bool ok = n <= kRankMax;
+        if (ok && n > 0) {
+            // 3. Sort by potential (bitonic, in LDS).
+            uint32_t n2 = 2;
+            while (n2 < n) n2 <<= 1;
+            for (uint32_t i = n + tid; i < n2; i += RF_NT) sk[i] = ~0ull;
+            __syncthreads();
+            for (uint32_t kk = 2; kk <= n2; kk <<= 1) {
+                for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+                    for (uint32_t i = tid; i < n2; i += RF_NT) {
+                        const uint32_t ixj = i ^ j;
+                        if (ixj > i) {
+                            const uint64_t a = sk[i], b = sk[ixj];
+                            if ((a > b) == ((i & kk) == 0)) { sk[i] = b; sk[ixj] = a; }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            // 4. Entries in potential order; pixel -> entry map; windows; flooded neighbours.
+            for (uint32_t i = tid; i < n; i += RF_NT) {
+                const uint32_t p = (uint32_t)sk[i];
+                sp[i] = ~(uint32_t)(sk[i] >> 32);
+                spix[i] = p;
+                seedf[i] = idxOf[p];
+            }
+            __syncthreads();
+            for (uint32_t k = tid; k < cnt; k += RF_NT) idxOf[ml[k]] = RF_NONE;  // unpushed seeds stay out
+            __syncthreads();
+            for (uint32_t i = tid; i < n; i += RF_NT) idxOf[spix[i]] = i;
+            __syncthreads();
+            // Potentials further apart than D quanta keep their order for every age <= n.
+            const uint64_t D = (uint64_t)floor(((double)n * kEpsilon + 1e-9) * 65536.0) + 1;
+            for (uint32_t i = tid; i < n; i += RF_NT) {
+                const uint64_t pi = sp[i];
+                // lo: first entry with P <= pi + D; hi: first entry with P < pi - D (P descending).
+                uint32_t a = 0, b = n;
+                while (a < b) {
+                    const uint32_t m = (a + b) >> 1;
+                    if ((uint64_t)(~(uint32_t)(sk[m] >> 32)) > pi + D) a = m + 1; else b = m;
+                }
+                lo[i] = a;
+                a = 0; b = n;
+                while (a < b) {
+                    const uint32_t m = (a + b) >> 1;
+                    if ((uint64_t)(~(uint32_t)(sk[m] >> 32)) + D >= pi) a = m + 1; else b = m;
+                }
+                hi[i] = a;
+                const uint32_t p = spix[i];
+                const int y = (int)(p / W), x = (int)(p % W);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    int dy, dx;
+                    nb_offset(j, dy, dx);
+                    const int yy = y + dy, xx = x + dx;
+                    uint32_t e = RF_NONE;
+                    if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                        const uint32_t q = (uint32_t)(yy * W + xx);
+                        if (Pf[q] != 0u) e = idxOf[q];
+                    }
+                    nbr[(size_t)i * 8 + j] = e;
+                }
+                Ga[i] = i + 1;
+            }
+            __syncthreads();
+            // 5. Fixed point.
+            uint32_t* G = Ga;
+            uint32_t* Gn = Gb;
+            bool settled = false, tie = false;
+            for (int iter = 0; iter < RF_MAX_ITER; ++iter) {
+                for (uint32_t i = tid; i < n; i += RF_NT) {
+                    uint32_t t = 0;
+                    if (!seedf[i]) {
+                        t = RF_NONE;
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const uint32_t e = nbr[(size_t)i * 8 + j];
+                            if (e != RF_NONE) t = min(t, G[e]);
+                        }
+                    }
+                    T[i] = t;
+                    K[i] = t == RF_NONE ? 1.79769313486231570815e308 : age_key(sp[i], t);
+                }
+                __syncthreads();
+                bool changed = false, tieHere = false;
+                for (uint32_t i = tid; i < n; i += RF_NT) {
+                    const double ki = K[i];
+                    uint32_t c = 0;
+                    for (uint32_t j = lo[i]; j < hi[i]; ++j) {
+                        if (j == i) continue;
+                        const double kj = K[j];
+                        c += kj < ki;
+                        tieHere |= kj == ki;
+                    }
+                    const uint32_t g = 1 + lo[i] + c;
+                    Gn[i] = g;
+                    changed |= g != G[i];
+                }
+                const bool anyChange = __syncthreads_or(changed);
+                tie = __syncthreads_or(tieHere);
+                uint32_t* t2 = G; G = Gn; Gn = t2;
+                if (!anyChange) { settled = true; break; }
+            }
+            // 6. Checks: settled, distinct keys, every entry pushed before its pop.
+            bool bad = !settled || tie;
+            if (!bad) {
+                bool viol = false;
+                for (uint32_t i = tid; i < n; i += RF_NT)
+                    if (!seedf[i] && !(T[i] < G[i])) viol = true;
+                bad = __syncthreads_or(viol);
+            }
+            ok = !bad;
+            if (ok) {
+                // 7. Labels through the first-popped-neighbour links (pointer jumping).
+                for (uint32_t i = tid; i < n; i += RF_NT) {
+                    if (seedf[i]) {
+                        par[i] = i;
+                        lbl[i] = fn[spix[i]];
+                    } else {
+                        uint32_t best = RF_NONE, bg = RF_NONE;
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const uint32_t e = nbr[(size_t)i * 8 + j];
+                            if (e != RF_NONE && G[e] < bg) { bg = G[e]; best = e; }
+                        }
+                        par[i] = best;
+                        lbl[i] = RF_NONE;
+                    }
+                }
+                __syncthreads();
+                for (int round = 0; round < 48; ++round) {
+                    bool pending = false;
+                    for (uint32_t i = tid; i < n; i += RF_NT) {
+                        if (lbl[i] != RF_NONE) continue;
+                        const uint32_t pp = par[i];
+                        const uint32_t lp = lbl[pp];
+                        if (lp != RF_NONE) lbl[i] = lp;
+                        else { par[i] = par[pp]; pending = true; }
+                    }
+                    if (!__syncthreads_or(pending)) break;
+                }
+                bool missing = false;
+                for (uint32_t i = tid; i < n; i += RF_NT) {
+                    const uint32_t L = lbl[i];
+                    if (L == RF_NONE || L == 0u) missing = true;
+                    else if (!seedf[i]) fn[spix[i]] = L;
+                }
+                ok = !__syncthreads_or(missing);
+            }
+        }
+        if (!ok && tid == 0) waveList[atomicAdd(&counters[kWaveCnt], 1u)] = f;
+    }
+}
+
+__device__ inline size_t align8(size_t v) { return (v + 7) & ~size_t(7); }
+
+// Exact frame flood for flagged frames, one wave per frame: the reference's process (seeds in any
+// order, global pop count as the age) with the wave queue. A frame whose queue meets equal keys at
+// a pop, or overflows, goes to the heap restatement below (fallback_kernel) through slowList.
+// The frame's mask, potentials and labelled state live in LDS when they fit next to a queue of
+// useful size; otherwise they are read from global memory.
+__global__ void __launch_bounds__(64) fallback_fast_kernel(const uint32_t* __restrict__ P, uint32_t* __restrict__ fin,
+                                                           const uint32_t* __restrict__ maskList,
+                                                           const uint32_t* __restrict__ maskCnt,
+                                                           const uint32_t* __restrict__ seedList,
+                                                           const uint32_t* __restrict__ seedCnt,
+                                                           const uint32_t* __restrict__ flagList, int countSlot,
+                                                           uint32_t* __restrict__ counters,
+                                                           uint32_t* __restrict__ slowList, int W, int H, size_t npx,
+                                                           const Fz* __restrict__ fz, const Region* __restrict__ regions,
+                                                           const uint32_t* __restrict__ rootCnt, const PixAux* __restrict__ aux,
+                                                           size_t regionCap, int forceSlow)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    const uint32_t nw = (uint32_t)((npx + 31) / 32);
+    while (true) {
+        uint32_t i = 0;
+        if (lane == 0) i = atomicAdd(&counters[kFbNext], 1u);
+        i = __shfl(i, 0);
+        if (i >= counters[countSlot]) return;
+        const uint32_t f = flagList[i];
+        if (forceSlow) {
+            if (lane == 0) slowList[atomicAdd(&counters[kSlowCnt], 1u)] = f;
+            continue;
+        }
+        const uint32_t* Pf = P + (size_t)f * npx;
+        uint32_t* fn = fin + (size_t)f * npx;
+        const uint32_t* ml = maskList + (size_t)f * npx;
+        const uint32_t cnt = maskCnt[f];
+        // LDS layout: frame state (when it fits beside a queue of at least 1024 entries), then the queue.
+        const size_t frameBytes = align8((size_t)nw * 8 + (size_t)cnt * 4 + ((size_t)cnt + 31) / 32 * 4);
+        const bool inLds = frameBytes + 1024 * 24 <= FB_LDS_BYTES;
+        FrameLds fr{};
+        size_t off = 0;
+        if (inLds) {
+            fr.mbits = reinterpret_cast<uint32_t*>(smem);
+            fr.pre = fr.mbits + nw;
+            fr.pc = fr.pre + nw;
+            fr.lbits = fr.pc + cnt;
+            off = frameBytes;
+        }
+        const uint32_t cap = (uint32_t)((FB_LDS_BYTES - off) / 24);
+        WaveQ q{reinterpret_cast<uint64_t*>(smem + off), reinterpret_cast<double*>(smem + off + (size_t)cap * 8),
+                reinterpret_cast<uint32_t*>(smem + off + (size_t)cap * 16),
+                reinterpret_cast<uint32_t*>(smem + off + (size_t)cap * 20), cap};
+
+        if (inLds) {
+            // Mask bitmap, prefix counts, labelled bits cleared.
+            for (uint32_t w = lane; w < nw; w += 64) fr.mbits[w] = 0u;
+            for (uint32_t w = lane; w < (cnt + 31) / 32; w += 64) fr.lbits[w] = 0u;
+            __syncthreads();
+            for (uint32_t k = lane; k < cnt; k += 64) {
+                const uint32_t p = ml[k];
+                atomicOr(&fr.mbits[p >> 5], 1u << (p & 31));
+            }
+            __syncthreads();
+            // Exclusive scan of the word popcounts: each lane a contiguous block, then across lanes.
+            const uint32_t per = (nw + 63) / 64, w0 = lane * per, w1 = min(nw, w0 + per);
+            uint32_t s = 0;
+            for (uint32_t w = w0; w < w1; ++w) s += __popc(fr.mbits[w]);
+            uint32_t incl = s;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t t = __shfl_up(incl, o);
+                if (lane >= o) incl += t;
+            }
+            uint32_t run = incl - s;
+            for (uint32_t w = w0; w < w1; ++w) {
+                fr.pre[w] = run;
+                run += __popc(fr.mbits[w]);
+            }
+            __syncthreads();
+        }
+        // 1. Labels back to the seed state (seeds marked labelled); potentials by rank.
+        for (uint32_t k = lane; k < cnt; k += 64) {
+            const uint32_t p = ml[k];
+            bool seed;
+            if (!fz) {
+                seed = is_local_max(Pf, W, H, (int)p);
+            } else {
+                const uint32_t L = fn[p];
+                seed = false;
+                if (L != 0 && L <= npx) {
+                    const uint32_t sl = aux[(size_t)f * npx + (L - 1)].rslot;
+                    seed = sl < regionCap && fz[(size_t)f * regionCap + sl].alive &&
+                           0xFFFFFFFFu - (uint32_t)fz[(size_t)f * regionCap + sl].maxKey == p;
+                }
+            }
+            if (!seed) fn[p] = 0u;
+            if (inLds) {
+                const uint32_t r = fr.rank(p);
+                fr.pc[r] = Pf[p];
+                if (seed) atomicOr(&fr.lbits[r >> 5], 1u << (r & 31));
+            }
         }
         __threadfence_block();
         __syncthreads();
@@ -539,13 +911,20 @@ __global__ void __launch_bounds__(64) fallback_fast_kernel(const uint32_t* __res
         }
         __syncthreads();
         // 3. Flood in key order with the frame-global pop count.
-        uint32_t it = 0;
+        uint32_t it = 0, maxN = n;
         while (ok && n > 0) {
+            maxN = n > maxN ? n : maxN;
             uint32_t a, la;
             bool proven = true;
             if (!waveq_pop<false>(q, n, a, la, proven, [](uint64_t, uint64_t) { return true; })) { ok = false; break; }
             ++it;
-            if (!waveq_expand(q, n, Pf, fn, W, H, a, la, it)) { ok = false; break; }
+            const bool pushed = inLds ? waveq_expand_lds(q, n, fr, fn, W, H, a, la, it)
+                                      : waveq_expand(q, n, Pf, fn, W, H, a, la, it);
+            if (!pushed) { ok = false; break; }
+        }
+        if (lane == 0) {
+            atomicAdd(&counters[kFloodPops], it);
+            atomicMax(&counters[kFloodMaxQueue], maxN);
         }
         if (!ok && lane == 0) {
             const uint32_t j = atomicAdd(&counters[kSlowCnt], 1u);
@@ -832,6 +1211,8 @@ __global__ void reset_pass_kernel(uint32_t* __restrict__ counters)
         counters[kSlowCnt] = 0u;
         counters[kSlowNext] = 0u;
         counters[kFbNext] = 0u;
+        counters[kRfNext] = 0u;
+        counters[kWaveCnt] = 0u;
     }
 }
 
@@ -1026,7 +1407,7 @@ void launch_exclusive_scan(const uint32_t* d_in, uint32_t* d_out, int n, hipStre
 }
 
 void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoints, double ratio, Roi* d_rois,
-                    hipStream_t s, bool forceSequential, bool forceHeap, bool resolve)
+                    hipStream_t s, bool forceSequential, int exactPath, bool resolve)
 {
     const size_t npx = b.npx;
     const unsigned bpf = (unsigned)std::min<size_t>(64, std::max<size_t>(1, npx / 16384));
@@ -1046,9 +1427,17 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
                                          b.counters, b.frameFlag, b.flagList, W, H, npx, pass2 ? b.fz : nullptr,
                                          b.rootCnt, b.regionCap);
         if (!resolve) return;
-        fallback_fast_kernel<<<(unsigned)std::max<size_t>(1, std::min<size_t>(F, 1024)), 64, 0, s>>>(
-            b.P, b.fin, b.maskList, b.maskCnt, b.seedList, b.seedCnt, b.flagList, b.counters, b.slowList, W, H, npx,
-            pass2 ? b.fz : nullptr, b.regions, b.rootCnt, b.aux, b.regionCap, forceHeap ? 1 : 0);
+        // Exact frame flood of the flagged frames: rank flood, then the wave-queue flood for the frames
+        // it hands over, then the heap restatement for frames with equal keys at a pop.
+        const bool rank = exactPath == 0;
+        if (rank)
+            rank_flood_kernel<<<(unsigned)b.rfSlots, RF_NT, 0, s>>>(b.P, b.fin, b.maskList, b.maskCnt, b.flagList, b.counters,
+                                                                   b.waveList, b.rfScratch, W, H, npx,
+                                                                   pass2 ? b.fz : nullptr, b.aux, b.regionCap);
+        fallback_fast_kernel<<<(unsigned)std::max<size_t>(1, std::min<size_t>(F, 1024)), 64, FB_LDS_BYTES, s>>>(
+            b.P, b.fin, b.maskList, b.maskCnt, b.seedList, b.seedCnt, rank ? b.waveList : b.flagList,
+            rank ? kWaveCnt : kFlagCnt, b.counters, b.slowList, W, H, npx, pass2 ? b.fz : nullptr, b.regions, b.rootCnt,
+            b.aux, b.regionCap, exactPath == 2 ? 1 : 0);
         fallback_kernel<<<(unsigned)b.fbSlots, NT, 0, s>>>(b.P, b.fin, b.maskList, b.maskCnt, b.slowList, b.counters,
                                                             b.fbScratch, W, H, npx, pass2 ? b.fz : nullptr, b.aux,
                                                             b.regionCap);

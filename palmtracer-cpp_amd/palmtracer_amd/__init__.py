@@ -145,10 +145,10 @@ def set_option(name, value):
 
 
 def last_stats():
-    v = (ctypes.c_uint64 * 9)()
-    lib().pt_last_stats(v, 9)
+    v = (ctypes.c_uint64 * 11)()
+    lib().pt_last_stats(v, 11)
     keys = ["frames", "chunks", "rows", "flagged_frames", "multi_components", "deferred_frames", "status",
-            "flag_overflow", "flag_unproven"]
+            "flag_overflow", "flag_unproven", "exact_flood_pops", "exact_flood_max_queue"]
     return dict(zip(keys, list(v)))
 
 

@@ -267,13 +267,15 @@ def frames512(pt_gpu, oracle_mod):
     return st, oracle_mod.autothreshold(st[0])
 
 
-@pytest.mark.parametrize("sequential", [0, 1, 2])
+@pytest.mark.parametrize("sequential", [0, 1, 2, 3])
 @pytest.mark.parametrize("ratio", [0.0, 10.0])
 def test_labels_512(pt_gpu, oracle_mod, frames512, sequential, ratio):
-    """0: component floods; 1: every frame through the wave-queue frame flood; 2: through the heap."""
+    """0: component floods; every frame through the exact frame flood: 1 rank flood (then the
+    wave-queue flood for frames it hands over), 2 wave-queue flood, 3 heap restatement."""
     st, thr = frames512
     pt_gpu.set_option("force_sequential_watershed", sequential > 0)
-    pt_gpu.set_option("force_heap_watershed", sequential > 1)
+    pt_gpu.set_option("force_wave_flood", sequential == 2)
+    pt_gpu.set_option("force_heap_watershed", sequential == 3)
     try:
         for f in range(st.shape[0]):
             w1 = oracle_mod.atrous(st[f])[1]
@@ -290,6 +292,7 @@ def test_labels_512(pt_gpu, oracle_mod, frames512, sequential, ratio):
     finally:
         pt_gpu.set_option("force_sequential_watershed", 0)
         pt_gpu.set_option("force_heap_watershed", 0)
+        pt_gpu.set_option("force_wave_flood", 0)
 
 
 def test_localization_512_no_fit(pt_gpu, oracle_mod, frames512):
@@ -340,3 +343,37 @@ def test_fp32_fit_queue_widths(pt_gpu, frames512, waves):
     finally:
         pt_gpu.set_option("fit_waves", 0)
     assert np.array_equal(got, base)
+
+
+@pytest.mark.parametrize("name", ["stack", "small", "dense"])
+@pytest.mark.parametrize("exact", [1, 2, 3])
+def test_exact_floods_bit_exact(pt_gpu, oracle_mod, ref_inputs, ref_vectors, name, exact):
+    """Every frame through each exact frame flood (rank, wave queue, heap): labels equal the oracle's."""
+    st, thr = _stacks(ref_inputs, ref_vectors)[name]
+    pt_gpu.set_option("force_sequential_watershed", 1)
+    pt_gpu.set_option("force_wave_flood", exact == 2)
+    pt_gpu.set_option("force_heap_watershed", exact == 3)
+    try:
+        for f in range(st.shape[0]):
+            w1 = oracle_mod.atrous(st[f])[1]
+            exp = oracle_mod.watershed_labels(w1, thr)
+            got = pt_gpu.debug_labels(st[f], thr, 0.0)
+            assert np.array_equal(rank_labels(got), exp), f"frame {f}"
+    finally:
+        pt_gpu.set_option("force_sequential_watershed", 0)
+        pt_gpu.set_option("force_heap_watershed", 0)
+        pt_gpu.set_option("force_wave_flood", 0)
+
+
+def test_localization_with_all_frames_exact(pt_gpu, oracle_mod, frames512):
+    """Localization with every frame deferred to the side stream's exact floods: rows unchanged."""
+    st, thr = frames512
+    exp = oracle_mod.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN)
+    pt_gpu.set_option("force_sequential_watershed", 1)
+    try:
+        got = pt_gpu.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN)
+        stats = pt_gpu.last_stats()
+    finally:
+        pt_gpu.set_option("force_sequential_watershed", 0)
+    assert stats["deferred_frames"] == st.shape[0]
+    assert got.shape == exp.shape and np.array_equal(got, exp)
