@@ -118,8 +118,10 @@ PT_API int PT_CALL pt_set_option(const char* name, double value);
 
 /* Counters of the last call: frames, chunks, rows, frames flagged for the exact frame flood,
  * multi-label components, frames deferred to the final pass, status, component floods given up on a
- * full queue, component floods given up on an unproven order, pops of the exact frame floods, their
- * largest queue. Writes min(n, 11) values, returns 11. */
+ * full queue, component floods given up on an unproven order, pops of the wave-queue frame floods,
+ * their largest queue, frames the rank flood handed over (too many entries, equal keys, not
+ * settled), wave-queue overflows, frames of the heap restatement. Writes min(n, 16) values,
+ * returns 16. */
 PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n);
 
 /* With the "profile" option, device milliseconds of the last Localization per stage (a trous,

@@ -22,7 +22,10 @@ struct DevBuf {
 struct RunStats {
     uint64_t frames = 0, chunks = 0, rois = 0, flaggedFrames = 0, multiComponents = 0, maskPixels = 0, deferredFrames = 0;
     uint64_t flagOverflow = 0, flagUnproven = 0;  // component floods given up: full queue / unproven order
-    uint64_t floodPops = 0, floodMaxQueue = 0;    // exact frame flood: pops, largest queue
+    uint64_t floodPops = 0, floodMaxQueue = 0;    // wave-queue frame flood: pops, largest queue
+    // Exact frame flood paths: frames handed by the rank flood to the wave-queue flood (too many
+    // entries / equal keys / not settled), wave-queue overflows, frames for the heap restatement.
+    uint64_t rfTooBig = 0, rfTie = 0, rfUnsettled = 0, fbOverflow = 0, heapFrames = 0;
     uint32_t status = 0;
     // With Options::profile: device time per stage (HIP events on the engine stream, summed over
     // chunks), the device time of the deferred-frame batches on the side stream, and the LM passes

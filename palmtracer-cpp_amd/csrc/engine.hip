@@ -306,6 +306,11 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         const uint32_t* ctr = h + SF + SF + 1;
         last.status |= ctr[kStatus];
         last.floodPops += ctr[kFloodPops];
+        last.rfTooBig += ctr[kRfTooBig];
+        last.rfTie += ctr[kRfTie];
+        last.rfUnsettled += ctr[kRfUnsettled];
+        last.fbOverflow += ctr[kFbOverflow];
+        last.heapFrames += ctr[kSlowCnt];
         last.floodMaxQueue = std::max<uint64_t>(last.floodMaxQueue, ctr[kFloodMaxQueue]);
         if (ctr[kStatus]) throw Error("segmentation status " + std::to_string(ctr[kStatus]));
         if (opt.profile) {
@@ -328,16 +333,15 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         // The copy reads the side lane's staging buffer, which the next batch overwrites.
         PT_CHECK(hipStreamSynchronize(main_.stream));
     };
-    // A batch is sent when a full one is pending or no main chunk is left to wait for: one exact
-    // frame flood costs the same latency for one frame as for a full batch.
-    auto side_poll = [&](bool wait, bool last) {
+    // A batch is sent whenever the side lane is idle, so that the deferred frames of a chunk are
+    // resolved while the next chunks run.
+    auto side_poll = [&](bool wait) {
         while (true) {
             if (batchCount) {
                 if (!wait && hipEventQuery(sideDone) == hipErrorNotReady) return;
                 side_collect();
             }
-            const size_t pending = deferred.size() - deferredNext;
-            if (!pending || (!wait && !last && pending < SF)) return;
+            if (deferred.size() == deferredNext) return;
             side_launch();
             if (!wait) return;
         }
@@ -349,7 +353,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         const size_t eb = 6 * last.chunks;
         const uint32_t n = detect_chunk(main_, frames, f, H, W, thr, ratio, maxPerPlane, eb, nullptr, false, z0,
                                         &deferred, true);
-        side_poll(false, z0 + f >= planes);
+        side_poll(false);
         PT_CHECK(hipMemcpyAsync(base.data(), main_.b.roiBase, (f + 1) * 4, hipMemcpyDeviceToHost, main_.stream));
         main_.rows.ensure((size_t)std::max<uint32_t>(n, 1) * kCols * 8);
         grow_staging(main_, stagedRows + n, stagedRows);
@@ -368,7 +372,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         stagedRows += n;
         last.chunks++;
     }
-    side_poll(true, true);
+    side_poll(true);
     last.deferredFrames = deferred.size();
 
     // Final placement in plane order.

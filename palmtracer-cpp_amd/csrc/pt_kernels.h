@@ -80,15 +80,19 @@ enum Counter : int {
     kFloodMaxQueue = 9,  // largest queue of the exact frame flood
     kRfNext = 10,        // rank flood: next flagged frame
     kWaveCnt = 11,       // frames handed from the rank flood to the wave-queue flood
-    kNumCounters = 12
+    kRfTooBig = 12,      // rank flood: frames with more than kRankMax flood entries
+    kRfTie = 13,         // rank flood: frames with two equal keys
+    kRfUnsettled = 14,   // rank flood: frames whose iteration did not settle (or broke T < G)
+    kFbOverflow = 15,    // wave-queue flood: frames whose queue overflowed
+    kNumCounters = 16
 };
 // kStatus bits: 1 = region slots exhausted, 2 = unlabelled mask pixel (internal error).
 
 __host__ __device__ inline size_t fb_slot_words(size_t npx) { return 5 * ((npx + 1) & ~size_t(1)); }
 // Rank flood: frames with at most kRankMax pixels in the flood (mask pixels but the seeds that are
-// never pushed). Work space per slot: pixel -> entry map, then 20 words per entry.
+// never pushed). Work space per slot: pixel -> entry map, then 21 words per entry.
 constexpr uint32_t kRankMax = 16384;
-__host__ __device__ inline size_t rf_slot_words(size_t npx) { return ((npx + 1) & ~size_t(1)) + 20 * (size_t)kRankMax; }
+__host__ __device__ inline size_t rf_slot_words(size_t npx) { return ((npx + 1) & ~size_t(1)) + 21 * (size_t)kRankMax; }
 
 // A trous
 void launch_atrous_plane(const uint16_t* d_in, double* d_out, int W, int H, int planes, int level, hipStream_t s);

@@ -568,7 +568,7 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
                                                            const PixAux* __restrict__ aux, size_t regionCap)
 {
     __shared__ uint64_t sk[kRankMax];  // (~P << 32) | pixel, sorted ascending = potential descending
-    __shared__ uint32_t sN, sFrame;
+    __shared__ uint32_t sN, sFrame, sPairs, sPairI, sPairJ;
     const int tid = threadIdx.x;
     uint32_t* idxOf = rfScratch + (size_t)blockIdx.x * rf_slot_words(npx);
     uint32_t* sp = idxOf + ((npx + 1) & ~size_t(1));   // P by entry
@@ -583,6 +583,7 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
     uint32_t* seedf = lbl + kRankMax;                    // entry is a seed
     double* K = reinterpret_cast<double*>(seedf + kRankMax);
     uint32_t* nbr = reinterpret_cast<uint32_t*>(K + kRankMax);  // [entry][8] flooded neighbours
+    uint32_t* lblA = nbr + (size_t)8 * kRankMax;                 // labels of the first tie order
     while (true) {
         __syncthreads();
         if (tid == 0) sFrame = atomicAdd(&counters[kRfNext], 1u);
@@ -628,6 +629,7 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
         __syncthreads();
         const uint32_t n = sN;
         bool ok = n <= kRankMax;
+        if (!ok && tid == 0) atomicAdd(&counters[kRfTooBig], 1u);
         if (ok && n > 0) {
             // 3. Sort by potential (bitonic, in LDS).
             uint32_t n2 = 2;
@@ -692,55 +694,69 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
                 Ga[i] = i + 1;
             }
             __syncthreads();
-            // 5. Fixed point.
-            uint32_t* G = Ga;
-            uint32_t* Gn = Gb;
-            bool settled = false, tie = false;
-            for (int iter = 0; iter < RF_MAX_ITER; ++iter) {
-                for (uint32_t i = tid; i < n; i += RF_NT) {
-                    uint32_t t = 0;
-                    if (!seedf[i]) {
-                        t = RF_NONE;
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            const uint32_t e = nbr[(size_t)i * 8 + j];
-                            if (e != RF_NONE) t = min(t, G[e]);
-                        }
-                    }
-                    T[i] = t;
-                    K[i] = t == RF_NONE ? 1.79769313486231570815e308 : age_key(sp[i], t);
-                }
+            // 5.-7. Fixed point, checks and labels. Two equal keys belong to siblings pushed by one
+            // pop with one potential; the heap orders them by its internal layout. With one such pair,
+            // the flood is solved for both orders of the pair; if both solutions keep that single
+            // pair and give the same labels, the heap's choice does not matter.
+            uint32_t pairI = RF_NONE, pairJ = RF_NONE;
+            bool done = false;
+            for (int orient = 0; orient < 2 && ok && !done; ++orient) {
+                for (uint32_t i = tid; i < n; i += RF_NT) Ga[i] = i + 1;
+                if (tid == 0) sPairs = 0;
                 __syncthreads();
-                bool changed = false, tieHere = false;
-                for (uint32_t i = tid; i < n; i += RF_NT) {
-                    const double ki = K[i];
-                    uint32_t c = 0;
-                    for (uint32_t j = lo[i]; j < hi[i]; ++j) {
-                        if (j == i) continue;
-                        const double kj = K[j];
-                        c += kj < ki;
-                        tieHere |= kj == ki;
+                uint32_t* G = Ga;
+                uint32_t* Gn = Gb;
+                bool settled = false;
+                for (int iter = 0; iter < RF_MAX_ITER; ++iter) {
+                    for (uint32_t i = tid; i < n; i += RF_NT) {
+                        uint32_t t = 0;
+                        if (!seedf[i]) {
+                            t = RF_NONE;
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) {
+                                const uint32_t e = nbr[(size_t)i * 8 + j];
+                                if (e != RF_NONE) t = min(t, G[e]);
+                            }
+                        }
+                        T[i] = t;
+                        K[i] = t == RF_NONE ? 1.79769313486231570815e308 : age_key(sp[i], t);
                     }
-                    const uint32_t g = 1 + lo[i] + c;
-                    Gn[i] = g;
-                    changed |= g != G[i];
+                    __syncthreads();
+                    bool changed = false;
+                    for (uint32_t i = tid; i < n; i += RF_NT) {
+                        const double ki = K[i];
+                        uint32_t c = 0;
+                        for (uint32_t j = lo[i]; j < hi[i]; ++j) {
+                            if (j == i) continue;
+                            const double kj = K[j];
+                            c += kj < ki || (kj == ki && (orient == 0 ? j < i : j > i));
+                        }
+                        const uint32_t g = 1 + lo[i] + c;
+                        Gn[i] = g;
+                        changed |= g != G[i];
+                    }
+                    const bool anyChange = __syncthreads_or(changed);
+                    uint32_t* t2 = G; G = Gn; Gn = t2;
+                    if (!anyChange) { settled = true; break; }
                 }
-                const bool anyChange = __syncthreads_or(changed);
-                tie = __syncthreads_or(tieHere);
-                uint32_t* t2 = G; G = Gn; Gn = t2;
-                if (!anyChange) { settled = true; break; }
-            }
-            // 6. Checks: settled, distinct keys, every entry pushed before its pop.
-            bool bad = !settled || tie;
-            if (!bad) {
-                bool viol = false;
-                for (uint32_t i = tid; i < n; i += RF_NT)
+                // Checks: settled, every entry pushed before its pop, equal-key pairs.
+                bool viol = !settled;
+                for (uint32_t i = tid; i < n && settled; i += RF_NT) {
                     if (!seedf[i] && !(T[i] < G[i])) viol = true;
-                bad = __syncthreads_or(viol);
-            }
-            ok = !bad;
-            if (ok) {
-                // 7. Labels through the first-popped-neighbour links (pointer jumping).
+                    const double ki = K[i];
+                    for (uint32_t j = i + 1; j < hi[i]; ++j)
+                        if (K[j] == ki) {
+                            if (atomicAdd(&sPairs, 1u) == 0u) { sPairI = i; sPairJ = j; }
+                        }
+                }
+                const bool bad = __syncthreads_or(viol);
+                const uint32_t pairs = sPairs;
+                if (bad || pairs > 1 || (orient == 1 && (pairs != 1 || sPairI != pairI || sPairJ != pairJ))) {
+                    if (tid == 0) atomicAdd(&counters[bad ? kRfUnsettled : kRfTie], 1u);
+                    ok = false;
+                    break;
+                }
+                // Labels through the first-popped-neighbour links (pointer jumping).
                 for (uint32_t i = tid; i < n; i += RF_NT) {
                     if (seedf[i]) {
                         par[i] = i;
@@ -768,14 +784,28 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
                     }
                     if (!__syncthreads_or(pending)) break;
                 }
-                bool missing = false;
+                bool missing = false, differ = false;
                 for (uint32_t i = tid; i < n; i += RF_NT) {
                     const uint32_t L = lbl[i];
                     if (L == RF_NONE || L == 0u) missing = true;
-                    else if (!seedf[i]) fn[spix[i]] = L;
+                    if (orient == 0) lblA[i] = L;
+                    else differ |= L != lblA[i];
                 }
-                ok = !__syncthreads_or(missing);
+                if (__syncthreads_or(missing) || __syncthreads_or(differ)) {
+                    if (tid == 0) atomicAdd(&counters[missing ? kRfUnsettled : kRfTie], 1u);
+                    ok = false;
+                    break;
+                }
+                if (orient == 0 && pairs == 1) {
+                    pairI = sPairI;
+                    pairJ = sPairJ;
+                    continue;  // solve again with the pair in the other order
+                }
+                for (uint32_t i = tid; i < n; i += RF_NT)
+                    if (!seedf[i]) fn[spix[i]] = lbl[i];
+                done = true;
             }
+            __syncthreads();
         }
         if (!ok && tid == 0) waveList[atomicAdd(&counters[kWaveCnt], 1u)] = f;
     }
@@ -920,7 +950,11 @@ __global__ void __launch_bounds__(64) fallback_fast_kernel(const uint32_t* __res
             ++it;
             const bool pushed = inLds ? waveq_expand_lds(q, n, fr, fn, W, H, a, la, it)
                                       : waveq_expand(q, n, Pf, fn, W, H, a, la, it);
-            if (!pushed) { ok = false; break; }
+            if (!pushed) {
+                ok = false;
+                if (lane == 0) atomicAdd(&counters[kFbOverflow], 1u);
+                break;
+            }
         }
         if (lane == 0) {
             atomicAdd(&counters[kFloodPops], it);
