@@ -120,8 +120,8 @@ PT_API int PT_CALL pt_set_option(const char* name, double value);
  * multi-label components, frames deferred to the final pass, status, component floods given up on a
  * full queue, component floods given up on an unproven order, pops of the wave-queue frame floods,
  * their largest queue, frames the rank flood handed over (too many entries, equal keys, not
- * settled), wave-queue overflows, frames of the heap restatement. Writes min(n, 16) values,
- * returns 16. */
+ * settled), wave-queue overflows, frames of the heap restatement, frames the rank flood settled by
+ * solving both orders of an equal-key pair. Writes min(n, 17) values, returns 17. */
 PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n);
 
 /* With the "profile" option, device milliseconds of the last Localization per stage (a trous,

@@ -26,6 +26,7 @@ struct RunStats {
     // Exact frame flood paths: frames handed by the rank flood to the wave-queue flood (too many
     // entries / equal keys / not settled), wave-queue overflows, frames for the heap restatement.
     uint64_t rfTooBig = 0, rfTie = 0, rfUnsettled = 0, fbOverflow = 0, heapFrames = 0;
+    uint64_t rfTwoOrders = 0;  // frames the rank flood settled by solving both orders of an equal-key pair
     uint32_t status = 0;
     // With Options::profile: device time per stage (HIP events on the engine stream, summed over
     // chunks), the device time of the deferred-frame batches on the side stream, and the LM passes

@@ -311,6 +311,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         last.rfUnsettled += ctr[kRfUnsettled];
         last.fbOverflow += ctr[kFbOverflow];
         last.heapFrames += ctr[kSlowCnt];
+        last.rfTwoOrders += ctr[kRfTwoOrders];
         last.floodMaxQueue = std::max<uint64_t>(last.floodMaxQueue, ctr[kFloodMaxQueue]);
         if (ctr[kStatus]) throw Error("segmentation status " + std::to_string(ctr[kStatus]));
         if (opt.profile) {
@@ -475,6 +476,12 @@ void Engine::debug_labels(const uint16_t* d_frame, int H, int W, double thr, dou
     last = RunStats{};
     last.flaggedFrames = ctr[kFlagCnt];
     last.multiComponents = ctr[kMultiCnt];
+    last.rfTooBig = ctr[kRfTooBig];
+    last.rfTie = ctr[kRfTie];
+    last.rfUnsettled = ctr[kRfUnsettled];
+    last.fbOverflow = ctr[kFbOverflow];
+    last.heapFrames = ctr[kSlowCnt];
+    last.rfTwoOrders = ctr[kRfTwoOrders];
     last.status = ctr[kStatus];
 }
 

@@ -84,7 +84,8 @@ enum Counter : int {
     kRfTie = 13,         // rank flood: frames with two equal keys
     kRfUnsettled = 14,   // rank flood: frames whose iteration did not settle (or broke T < G)
     kFbOverflow = 15,    // wave-queue flood: frames whose queue overflowed
-    kNumCounters = 16
+    kRfTwoOrders = 16,   // rank flood: frames settled by solving both orders of an equal-key pair
+    kNumCounters = 20
 };
 // kStatus bits: 1 = region slots exhausted, 2 = unlabelled mask pixel (internal error).
 

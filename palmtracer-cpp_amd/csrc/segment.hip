@@ -803,6 +803,7 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
                 }
                 for (uint32_t i = tid; i < n; i += RF_NT)
                     if (!seedf[i]) fn[spix[i]] = lbl[i];
+                if (tid == 0 && orient == 1) atomicAdd(&counters[kRfTwoOrders], 1u);
                 done = true;
             }
             __syncthreads();

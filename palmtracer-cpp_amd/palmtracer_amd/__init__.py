@@ -145,11 +145,11 @@ def set_option(name, value):
 
 
 def last_stats():
-    v = (ctypes.c_uint64 * 16)()
-    lib().pt_last_stats(v, 16)
+    v = (ctypes.c_uint64 * 17)()
+    lib().pt_last_stats(v, 17)
     keys = ["frames", "chunks", "rows", "flagged_frames", "multi_components", "deferred_frames", "status",
             "flag_overflow", "flag_unproven", "wave_flood_pops", "wave_flood_max_queue", "rank_too_big",
-            "rank_equal_keys", "rank_unsettled", "wave_queue_overflow", "heap_frames"]
+            "rank_equal_keys", "rank_unsettled", "wave_queue_overflow", "heap_frames", "rank_two_orders"]
     return dict(zip(keys, list(v)))
 
 

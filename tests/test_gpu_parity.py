@@ -377,3 +377,28 @@ def test_localization_with_all_frames_exact(pt_gpu, oracle_mod, frames512):
         pt_gpu.set_option("force_sequential_watershed", 0)
     assert stats["deferred_frames"] == st.shape[0]
     assert got.shape == exp.shape and np.array_equal(got, exp)
+
+
+# Seeded 256x256 frames (tests/golden/synth_np.py, 60 emitters) whose flood has equal keys (siblings
+# with one potential): one pair (the rank flood solves both orders) or several (wave-queue flood,
+# then the heap restatement). Found by solving the flood on the CPU.
+TIE_FRAMES = [(101, 5.0, 1), (102, 4.0, 1), (104, 4.0, 1), (101, 4.0, 2), (103, 4.0, 3)]
+
+
+@pytest.mark.parametrize("seed,thr,pairs", TIE_FRAMES)
+def test_equal_keys_in_the_flood(pt_gpu, oracle_mod, seed, thr, pairs):
+    import synth_np
+    frame = synth_np.frames(1, 256, 256, 60, seed=seed)[0]
+    w1 = oracle_mod.atrous(frame)[1]
+    exp = oracle_mod.watershed_labels(w1, thr)
+    pt_gpu.set_option("force_sequential_watershed", 1)
+    try:
+        got = pt_gpu.debug_labels(frame, thr, 0.0)
+        stats = pt_gpu.last_stats()
+    finally:
+        pt_gpu.set_option("force_sequential_watershed", 0)
+    assert np.array_equal(rank_labels(got), exp)
+    if pairs == 1:
+        assert stats["rank_two_orders"] == 1, stats
+    else:
+        assert stats["rank_equal_keys"] == 1 and stats["heap_frames"] == 1, stats
