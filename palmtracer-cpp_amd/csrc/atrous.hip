@@ -153,6 +153,7 @@ __global__ void __launch_bounds__(NT) atrous_plane_kernel(const uint16_t* __rest
 //   maskList / seedList        per-frame compact lists (capacity npx each) of pixel indices with
 //                              P > 0 / local maxima (Segmentation2D.cpp:294-333 test: no 8-neighbour
 //                              inside the frame strictly greater), counts in maskCnt / seedCnt.
+//   seedBits                   per-frame bitmap of the local maxima (zeroed by the caller).
 constexpr int SC = 3;                   // columns per lane
 constexpr int SW = 64 * SC;             // strip width
 constexpr int SHALO = 7;                // halo columns on each side
@@ -173,8 +174,9 @@ __device__ inline uint32_t pick(const uint32_t (&v)[SC], int k) { return k == 0 
 __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __restrict__ in, uint32_t* __restrict__ P,
                                                            uint32_t* __restrict__ lab, uint32_t* __restrict__ maskList,
                                                            uint32_t* __restrict__ maskCnt, uint32_t* __restrict__ seedList,
-                                                           uint32_t* __restrict__ seedCnt, int W, int H, int64_t tmin,
-                                                           int strips, int segH, const uint32_t* __restrict__ frameMap)
+                                                           uint32_t* __restrict__ seedCnt, uint32_t* __restrict__ seedBits,
+                                                           int W, int H, int64_t tmin, int strips, int segH,
+                                                           const uint32_t* __restrict__ frameMap)
 {
     const int lane = threadIdx.x;
     const int f = blockIdx.y;
@@ -185,6 +187,7 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
     uint32_t* lb = lab + (size_t)f * npx;
     uint32_t* ml = maskList + (size_t)f * npx;
     uint32_t* sl = seedList + (size_t)f * npx;
+    uint32_t* sbf = seedBits + (size_t)f * seed_words(npx);
     const int xs = strip * SOUT - SHALO;    // frame column of strip column 0
     const int ys = seg * segH, ye = min(ys + segH, H);
 
@@ -358,7 +361,10 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
         for (int k = 0; k < SC; ++k) {
             const uint32_t idx = (uint32_t)(y3 * W + xcol[k]);
             if (mbits & (1u << k)) { ml[bm++] = idx; lb[idx] = idx; }
-            if (sbits & (1u << k)) sl[bs++] = idx;
+            if (sbits & (1u << k)) {
+                sl[bs++] = idx;
+                atomicOr(&sbf[idx >> 5], 1u << (idx & 31));
+            }
         }
     }
 }
@@ -388,8 +394,8 @@ void launch_atrous_detect(const uint16_t* d_in, const DetectBuffers& b, int W, i
     const int segs = (int)ceil_div(H, segH);
     if ((size_t)strips * segs > 0x7FFFFFFF) throw Error("launch_atrous_detect: frame too large");
     const dim3 grid((unsigned)(strips * segs), (unsigned)frames);
-    atrous_detect_kernel<<<grid, 64, 0, s>>>(d_in, b.P, b.lab, b.maskList, b.maskCnt, b.seedList, b.seedCnt, W, H, tmin,
-                                             strips, segH, frameMap);
+    atrous_detect_kernel<<<grid, 64, 0, s>>>(d_in, b.P, b.lab, b.maskList, b.maskCnt, b.seedList, b.seedCnt, b.seedBits,
+                                             W, H, tmin, strips, segH, frameMap);
     PT_CHECK(hipGetLastError());
 }
 

@@ -108,6 +108,7 @@ void Lane::prepare(size_t F, size_t npx)
     flag.ensure((F + 1) * 12);
     const size_t rfSlots = std::clamp<size_t>(F, 1, 128);
     rf.ensure(rfSlots * rf_slot_words(npx) * 4);
+    sbits.ensure(F * seed_words(npx) * 4);
     fitNext.ensure(256);
     uint32_t* pf = perFrame.as<uint32_t>();
     const size_t s = F + 1;
@@ -135,6 +136,7 @@ void Lane::prepare(size_t F, size_t npx)
     b.waveList = flag.as<uint32_t>() + 2 * (F + 1);
     b.rfScratch = rf.as<uint32_t>();
     b.rfSlots = rfSlots;
+    b.seedBits = sbits.as<uint32_t>();
     b.frames = F;
     b.npx = npx;
     b.regionCap = regionCap;
@@ -149,6 +151,7 @@ void Lane::zero()
     // kernel rather than a memset so that the sequence can be captured in a graph.
     fill_u32_kernel<<<64, 256, 0, stream>>>(b.maskCnt, 6 * (b.frames + 1), 0u);
     fill_u32_kernel<<<1, 64, 0, stream>>>(b.counters, kNumCounters, 0u);
+    fill_u32_kernel<<<1024, 256, 0, stream>>>(b.seedBits, b.frames * seed_words(b.npx), 0u);
     PT_CHECK(hipGetLastError());
 }
 

@@ -68,6 +68,7 @@ struct DetectBuffers {
     uint32_t* slowList = nullptr; // [F] frames for the heap restatement (equal keys met)
     uint32_t* waveList = nullptr; // [F] frames the rank flood could not settle (wave-queue flood)
     uint32_t* rfScratch = nullptr;// rank-flood work space: rfSlots * rf_slot_words(npx) u32
+    uint32_t* seedBits = nullptr; // per frame: bitmap of the local maxima, seed_words(npx) u32
     size_t frames = 0, npx = 0, regionCap = 0, multiCap = 0, fbSlots = 0, rfSlots = 0;
 };
 
@@ -90,6 +91,7 @@ enum Counter : int {
 // kStatus bits: 1 = region slots exhausted, 2 = unlabelled mask pixel (internal error).
 
 __host__ __device__ inline size_t fb_slot_words(size_t npx) { return 5 * ((npx + 1) & ~size_t(1)); }
+__host__ __device__ inline size_t seed_words(size_t npx) { return (npx + 31) / 32; }
 // Rank flood: frames with at most kRankMax pixels in the flood (mask pixels but the seeds that are
 // never pushed). Work space per slot: pixel -> entry map, then 21 words per entry.
 constexpr uint32_t kRankMax = 16384;

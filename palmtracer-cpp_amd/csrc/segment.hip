@@ -30,29 +30,15 @@ namespace {
 constexpr uint32_t kVisited = 0x80000000u;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int NT = 256;
-constexpr uint32_t RUN = 8;   // consecutive list entries per thread in the per-pixel reduction kernels
 
 __device__ inline uint32_t ld_relaxed(const uint32_t* p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ inline bool is_local_max(const uint32_t* Pf, int W, int H, int idx)
-{
-    const uint32_t v = Pf[idx];
-    if (v == 0) return false;
-    const int y = idx / W, x = idx - y * W;
-    for (int dy = -1; dy <= 1; ++dy) {
-        const int yy = y + dy;
-        if (yy < 0 || yy >= H) continue;
-        for (int dx = -1; dx <= 1; ++dx) {
-            const int xx = x + dx;
-            if ((dx | dy) == 0 || xx < 0 || xx >= W) continue;
-            if (v < Pf[yy * W + xx]) return false;
-        }
-    }
-    return true;
-}
+// Local maximum of getLocalMax (Segmentation2D.cpp:294-333: P > 0 and no 8-neighbour inside the
+// frame strictly greater), from the bitmap the detection kernel wrote.
+__device__ inline bool seed_bit(const uint32_t* sbf, uint32_t idx) { return (sbf[idx >> 5] >> (idx & 31)) & 1u; }
 
 // A pixel that the watershed would seed: labelled with some 8-neighbour (inside the frame) that is
 // unlabelled (Segmentation2D.cpp:340-359). Unlabelled = below threshold or label 0.
@@ -198,7 +184,7 @@ __global__ void __launch_bounds__(NT) cc_flatten_kernel(uint32_t* __restrict__ l
 
 // getLocalMax labels (Segmentation2D.cpp:309-327): a maximum inherits the label of the first
 // earlier maximum among (y-1,x-1), (y-1,x), (y-1,x+1), (y,x-1); otherwise it opens a new label.
-__global__ void __launch_bounds__(NT) seed_label_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ lab,
+__global__ void __launch_bounds__(NT) seed_label_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ seedBits, const uint32_t* __restrict__ lab,
                                                         uint32_t* __restrict__ fin, PixAux* __restrict__ aux,
                                                         const uint32_t* __restrict__ seedList, const uint32_t* __restrict__ seedCnt,
                                                         uint32_t* __restrict__ rootCnt, Region* __restrict__ regions,
@@ -207,7 +193,7 @@ __global__ void __launch_bounds__(NT) seed_label_kernel(const uint32_t* __restri
 {
     const int f = blockIdx.y;
     const uint32_t cnt = seedCnt[f];
-    const uint32_t* Pf = P + (size_t)f * npx;
+    const uint32_t* sbf = seedBits + (size_t)f * seed_words(npx);
     for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < cnt; k += gridDim.x * NT) {
         const uint32_t s = seedList[(size_t)f * npx + k];
         uint32_t r = s;
@@ -219,7 +205,7 @@ __global__ void __launch_bounds__(NT) seed_label_kernel(const uint32_t* __restri
                 const int yy = y + dys[j], xx = x + dxs[j];
                 if (yy < 0 || xx < 0 || xx >= W) continue;
                 const int q = yy * W + xx;
-                if (is_local_max(Pf, W, H, q)) { r = (uint32_t)q; moved = true; }
+                if (seed_bit(sbf, (uint32_t)q)) { r = (uint32_t)q; moved = true; }
             }
             if (!moved) break;
         }
@@ -237,7 +223,7 @@ __global__ void __launch_bounds__(NT) seed_label_kernel(const uint32_t* __restri
     }
 }
 
-__global__ void __launch_bounds__(NT) label_assign_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ lab,
+__global__ void __launch_bounds__(NT) label_assign_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ seedBits, const uint32_t* __restrict__ lab,
                                                           uint32_t* __restrict__ fin, const PixAux* __restrict__ aux,
                                                           const uint32_t* __restrict__ maskList, const uint32_t* __restrict__ maskCnt,
                                                           uint64_t* __restrict__ multiList, size_t multiCap,
@@ -255,7 +241,7 @@ __global__ void __launch_bounds__(NT) label_assign_kernel(const uint32_t* __rest
         if (a.cmin == kNone) { flag_frame(frameFlag, counters, flagList, f); continue; }  // cannot happen
         if (a.cmin == a.cmax) { fin[(size_t)f * npx + p] = a.cmin; continue; }
         // Several labels: keep the seeds' labels only (second pass: fin already holds seeds only).
-        if (!pass2 && !is_local_max(Pf, W, H, p)) fin[(size_t)f * npx + p] = 0u;
+        if (!pass2 && !seed_bit(seedBits + (size_t)f * seed_words(npx), p)) fin[(size_t)f * npx + p] = 0u;
         if (p == c) {
             const uint32_t j = atomicAdd(&counters[kMultiCnt], 1u);
             if (j < multiCap) multiList[j] = ((uint64_t)f << 32) | c;
@@ -559,7 +545,7 @@ constexpr int RF_NT = 1024;
 constexpr uint32_t RF_NONE = 0xFFFFFFFFu;
 constexpr int RF_MAX_ITER = 64;
 
-__global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __restrict__ P, uint32_t* __restrict__ fin,
+__global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ seedBits, uint32_t* __restrict__ fin,
                                                            const uint32_t* __restrict__ maskList,
                                                            const uint32_t* __restrict__ maskCnt,
                                                            const uint32_t* __restrict__ flagList, uint32_t* __restrict__ counters,
@@ -601,7 +587,7 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
             const uint32_t p = ml[k];
             bool seed;
             if (!fz) {
-                seed = is_local_max(Pf, W, H, (int)p);
+                seed = seed_bit(seedBits + (size_t)f * seed_words(npx), p);
             } else {
                 const uint32_t L = fn[p];
                 seed = false;
@@ -819,7 +805,7 @@ __device__ inline size_t align8(size_t v) { return (v + 7) & ~size_t(7); }
 // a pop, or overflows, goes to the heap restatement below (fallback_kernel) through slowList.
 // The frame's mask, potentials and labelled state live in LDS when they fit next to a queue of
 // useful size; otherwise they are read from global memory.
-__global__ void __launch_bounds__(64) fallback_fast_kernel(const uint32_t* __restrict__ P, uint32_t* __restrict__ fin,
+__global__ void __launch_bounds__(64) fallback_fast_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ seedBits, uint32_t* __restrict__ fin,
                                                            const uint32_t* __restrict__ maskList,
                                                            const uint32_t* __restrict__ maskCnt,
                                                            const uint32_t* __restrict__ seedList,
@@ -896,7 +882,7 @@ __global__ void __launch_bounds__(64) fallback_fast_kernel(const uint32_t* __res
             const uint32_t p = ml[k];
             bool seed;
             if (!fz) {
-                seed = is_local_max(Pf, W, H, (int)p);
+                seed = seed_bit(seedBits + (size_t)f * seed_words(npx), p);
             } else {
                 const uint32_t L = fn[p];
                 seed = false;
@@ -972,7 +958,7 @@ __global__ void __launch_bounds__(64) fallback_fast_kernel(const uint32_t* __res
 // Exact sequential restatement of Segmentation2D::watershed with the libstdc++ heap order, one
 // frame per workgroup (frames with equal keys at a pop, taken from slowList through a shared
 // cursor). Work space per slot: fb_slot_words(npx) words.
-__global__ void __launch_bounds__(NT) fallback_kernel(const uint32_t* __restrict__ P, uint32_t* __restrict__ fin,
+__global__ void __launch_bounds__(NT) fallback_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ seedBits, uint32_t* __restrict__ fin,
                                                       const uint32_t* __restrict__ maskList, const uint32_t* __restrict__ maskCnt,
                                                       const uint32_t* __restrict__ flagList, uint32_t* __restrict__ counters,
                                                       uint32_t* __restrict__ fbScratch, int W, int H, size_t npx,
@@ -1001,7 +987,7 @@ __global__ void __launch_bounds__(NT) fallback_kernel(const uint32_t* __restrict
             const uint32_t p = maskList[(size_t)f * npx + k];
             bool seed;
             if (!fz) {
-                seed = is_local_max(Pf, W, H, (int)p);
+                seed = seed_bit(seedBits + (size_t)f * seed_words(npx), p);
             } else {
                 const uint32_t L = fn[p];
                 seed = false;
@@ -1251,52 +1237,60 @@ __global__ void reset_pass_kernel(uint32_t* __restrict__ counters)
     }
 }
 
-__global__ void __launch_bounds__(NT) region_stats_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ fin,
-                                                          const PixAux* __restrict__ aux, const uint32_t* __restrict__ maskList,
-                                                          const uint32_t* __restrict__ maskCnt, Region* __restrict__ regions,
-                                                          size_t regionCap, uint32_t* __restrict__ counters, int W, size_t npx,
-                                                          const uint32_t* __restrict__ skip)
+// ComputeStatistics sums (Segmentation2D.cpp:144-175) as exact integers: surface, sum P, sum P*x,
+// sum P*y per region. One workgroup per frame; the frame's regions (slots [0, rootCnt) in the order
+// seed_label_kernel opened them) are accumulated in LDS when they fit, else with global atomics.
+constexpr int RS_NT = 1024;
+constexpr uint32_t RS_LDS_REGIONS = 2048;
+
+__global__ void __launch_bounds__(RS_NT) region_stats_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ fin,
+                                                             const PixAux* __restrict__ aux, const uint32_t* __restrict__ maskList,
+                                                             const uint32_t* __restrict__ maskCnt, const uint32_t* __restrict__ rootCnt,
+                                                             Region* __restrict__ regions, size_t regionCap,
+                                                             uint32_t* __restrict__ counters, int W, size_t npx,
+                                                             const uint32_t* __restrict__ skip)
 {
-    const int f = blockIdx.y;
+    __shared__ unsigned long long sP[RS_LDS_REGIONS], sPx[RS_LDS_REGIONS], sPy[RS_LDS_REGIONS];
+    __shared__ uint32_t sSurf[RS_LDS_REGIONS];
+    const int f = blockIdx.x;
     if (skip && skip[f]) return;   // deferred frame: its labels are resolved in a later pass
     const uint32_t cnt = maskCnt[f];
-    // Each thread takes RUN consecutive list entries (row runs of one region, mostly) and adds their
-    // sums to the region once per run (integer sums: the order of additions does not matter).
-    for (uint32_t k0 = (blockIdx.x * NT + threadIdx.x) * RUN; k0 < cnt; k0 += gridDim.x * NT * RUN) {
-        const uint32_t k1 = min(k0 + RUN, cnt);
-        uint32_t curL = 0, slot = kNone, n = 0;
-        uint64_t sP = 0, sPx = 0, sPy = 0;
-        auto flush = [&]() {
-            if (!n) return;
-            Region* r = regions + (size_t)f * regionCap + slot;
-            atomicAdd(&r->surf, n);
-            atomicAdd((unsigned long long*)&r->sP, (unsigned long long)sP);
-            atomicAdd((unsigned long long*)&r->sPx, (unsigned long long)sPx);
-            atomicAdd((unsigned long long*)&r->sPy, (unsigned long long)sPy);
-            n = 0;
-            sP = sPx = sPy = 0;
-        };
-        for (uint32_t k = k0; k < k1; ++k) {
-            const uint32_t p = maskList[(size_t)f * npx + k];
-            const uint32_t L = fin[(size_t)f * npx + p];
-            if (L == 0 || L > npx) { atomicOr(&counters[kStatus], 2u); continue; }
-            if (L != curL) {
-                const uint32_t s2 = aux[(size_t)f * npx + (L - 1)].rslot;
-                if (s2 >= regionCap) { atomicOr(&counters[kStatus], 2u); continue; }
-                curL = L;
-                if (s2 != slot) {
-                    flush();
-                    slot = s2;
-                }
-            }
-            const uint64_t v = P[(size_t)f * npx + p];
-            const uint32_t y = p / W, x = p - y * W;
-            ++n;
-            sP += v;
-            sPx += v * x;
-            sPy += v * y;
+    const uint32_t nReg = (uint32_t)min((size_t)rootCnt[f], regionCap);
+    const bool inLds = nReg <= RS_LDS_REGIONS;
+    Region* rg = regions + (size_t)f * regionCap;
+    if (inLds) {
+        for (uint32_t r = threadIdx.x; r < nReg; r += RS_NT) { sP[r] = 0; sPx[r] = 0; sPy[r] = 0; sSurf[r] = 0; }
+        __syncthreads();
+    }
+    for (uint32_t k = threadIdx.x; k < cnt; k += RS_NT) {
+        const uint32_t p = maskList[(size_t)f * npx + k];
+        const uint32_t L = fin[(size_t)f * npx + p];
+        if (L == 0 || L > npx) { atomicOr(&counters[kStatus], 2u); continue; }
+        const uint32_t slot = aux[(size_t)f * npx + (L - 1)].rslot;
+        if (slot >= nReg) { atomicOr(&counters[kStatus], 2u); continue; }
+        const uint64_t v = P[(size_t)f * npx + p];
+        const uint32_t y = p / W, x = p - y * W;
+        if (inLds) {
+            atomicAdd(&sSurf[slot], 1u);
+            atomicAdd(&sP[slot], (unsigned long long)v);
+            atomicAdd(&sPx[slot], (unsigned long long)(v * x));
+            atomicAdd(&sPy[slot], (unsigned long long)(v * y));
+        } else {
+            Region* r = rg + slot;
+            atomicAdd(&r->surf, 1u);
+            atomicAdd((unsigned long long*)&r->sP, (unsigned long long)v);
+            atomicAdd((unsigned long long*)&r->sPx, (unsigned long long)(v * x));
+            atomicAdd((unsigned long long*)&r->sPy, (unsigned long long)(v * y));
         }
-        flush();
+    }
+    if (inLds) {
+        __syncthreads();
+        for (uint32_t r = threadIdx.x; r < nReg; r += RS_NT) {
+            rg[r].surf = sSurf[r];
+            rg[r].sP = sP[r];
+            rg[r].sPx = sPx[r];
+            rg[r].sPy = sPy[r];
+        }
     }
 }
 
@@ -1452,11 +1446,11 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
     const uint32_t* skip = resolve ? nullptr : b.frameFlag;
     cc_union_kernel<<<g, NT, 0, s>>>(b.lab, b.P, b.maskList, b.maskCnt, b.aux, W, H, npx);
     cc_flatten_kernel<<<g, NT, 0, s>>>(b.lab, b.maskList, b.maskCnt, b.aux, npx);
-    seed_label_kernel<<<g, NT, 0, s>>>(b.P, b.lab, b.fin, b.aux, b.seedList, b.seedCnt, b.rootCnt, b.regions,
+    seed_label_kernel<<<g, NT, 0, s>>>(b.P, b.seedBits, b.lab, b.fin, b.aux, b.seedList, b.seedCnt, b.rootCnt, b.regions,
                                        b.regionCap, b.counters, W, H, npx);
     auto watershed = [&](int pass2) {
         if (forceSequential) force_flag_kernel<<<ceil_div(F, 256), 256, 0, s>>>(b.counters, b.frameFlag, b.flagList, F);
-        label_assign_kernel<<<g, NT, 0, s>>>(b.P, b.lab, b.fin, b.aux, b.maskList, b.maskCnt, b.multiList, b.multiCap,
+        label_assign_kernel<<<g, NT, 0, s>>>(b.P, b.seedBits, b.lab, b.fin, b.aux, b.maskList, b.maskCnt, b.multiList, b.multiCap,
                                              b.counters, b.frameFlag, b.flagList, W, H, npx, pass2);
         flood_kernel<<<4096, 64, 0, s>>>(b.P, b.lab, b.fin, b.seedList, b.seedCnt, b.maskCnt, b.multiList, b.multiCap,
                                          b.counters, b.frameFlag, b.flagList, W, H, npx, pass2 ? b.fz : nullptr,
@@ -1466,14 +1460,14 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
         // it hands over, then the heap restatement for frames with equal keys at a pop.
         const bool rank = exactPath == 0;
         if (rank)
-            rank_flood_kernel<<<(unsigned)b.rfSlots, RF_NT, 0, s>>>(b.P, b.fin, b.maskList, b.maskCnt, b.flagList, b.counters,
+            rank_flood_kernel<<<(unsigned)b.rfSlots, RF_NT, 0, s>>>(b.P, b.seedBits, b.fin, b.maskList, b.maskCnt, b.flagList, b.counters,
                                                                    b.waveList, b.rfScratch, W, H, npx,
                                                                    pass2 ? b.fz : nullptr, b.aux, b.regionCap);
         fallback_fast_kernel<<<(unsigned)std::max<size_t>(1, std::min<size_t>(F, 1024)), 64, FB_LDS_BYTES, s>>>(
-            b.P, b.fin, b.maskList, b.maskCnt, b.seedList, b.seedCnt, rank ? b.waveList : b.flagList,
+            b.P, b.seedBits, b.fin, b.maskList, b.maskCnt, b.seedList, b.seedCnt, rank ? b.waveList : b.flagList,
             rank ? kWaveCnt : kFlagCnt, b.counters, b.slowList, W, H, npx, pass2 ? b.fz : nullptr, b.regions, b.rootCnt,
             b.aux, b.regionCap, exactPath == 2 ? 1 : 0);
-        fallback_kernel<<<(unsigned)b.fbSlots, NT, 0, s>>>(b.P, b.fin, b.maskList, b.maskCnt, b.slowList, b.counters,
+        fallback_kernel<<<(unsigned)b.fbSlots, NT, 0, s>>>(b.P, b.seedBits, b.fin, b.maskList, b.maskCnt, b.slowList, b.counters,
                                                             b.fbScratch, W, H, npx, pass2 ? b.fz : nullptr, b.aux,
                                                             b.regionCap);
     };
@@ -1486,8 +1480,8 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
         reset_pass_kernel<<<1, 64, 0, s>>>(b.counters);
         watershed(1);
     }
-    region_stats_kernel<<<g, NT, 0, s>>>(b.P, b.fin, b.aux, b.maskList, b.maskCnt, b.regions, b.regionCap, b.counters,
-                                         W, npx, skip);
+    region_stats_kernel<<<F, RS_NT, 0, s>>>(b.P, b.fin, b.aux, b.maskList, b.maskCnt, b.rootCnt, b.regions, b.regionCap,
+                                            b.counters, W, npx, skip);
     region_count_kernel<<<F, NT, 0, s>>>(b.regions, b.regionCap, b.rootCnt, b.found, b.keep, maxPoints, skip);
     exclusive_scan_kernel<<<1, 1024, 0, s>>>(b.keep, b.roiBase, F);
     roi_emit_kernel<<<F, NT, 0, s>>>(b.P, b.fin, b.regions, b.regionCap, b.rootCnt, b.found, b.keep, b.roiBase, d_rois,
