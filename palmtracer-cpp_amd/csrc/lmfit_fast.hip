@@ -191,7 +191,9 @@ __host__ __device__ constexpr size_t staged_lds_bytes(int roi)
 // evaluates the Jacobian at pTry right after accepting pTry (GaussianFit.cpp:147-151) and chi^2 at
 // pTry is the same sum either way, so an accepted trial pass already holds the next alpha, beta and
 // chi^2.
-template <int MODE, bool STAGED>
+// R > 0: the ROI side is the compile-time constant R (pixel loops unrolled, column factors in
+// registers); R = 0: the side is read from fp.roi.
+template <int MODE, bool STAGED, int R>
 __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __restrict__ frames, size_t npx, int W, int H,
                                                             const Roi* __restrict__ rois, uint32_t nRoiMax, FitParams fp,
                                                             double* __restrict__ rows, unsigned int* __restrict__ next,
@@ -204,7 +206,7 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
     constexpr int T = Tri<N>::v;
     extern __shared__ __align__(16) unsigned char lds[];
     const int lane = threadIdx.x;
-    const int roi = fp.roi, half = roi / 2, nPixI = roi * roi;
+    const int roi = R > 0 ? R : fp.roi, half = roi / 2, nPixI = roi * roi;
     // STAGED: column factors [column][lane] (FP64), then the ROI's pixels [pixel][lane].
     double* colE = reinterpret_cast<double*>(lds);
     uint16_t* patch = reinterpret_cast<uint16_t*>(lds + (size_t)roi * 64 * sizeof(double));
@@ -228,19 +230,40 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
     // body(k, i, j, X, Y, E) with X, Y the FP64 offsets from the centre (cx, cy).
     auto for_pixels = [&](const Model<MODE>& m, double cx, double cy, auto&& body) {
         const double bx = (double)(ux - half) - cx, by = (double)(uy - half) - cy;
-        const bool sep = STAGED && m.separable();
-        if (sep)
-            for (int j = 0; j < roi; ++j) {
+        if constexpr (R > 0) {
+            const bool sep = m.separable();
+            double ce[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
                 const double X = bx + j;
-                colE[j * 64 + lane] = exp(-(m.ax * X * X));
+                ce[j] = sep ? exp(-(m.ax * X * X)) : 0.0;
             }
-        for (int i = 0, k = 0; i < roi; ++i) {
-            const double Y = by + i;
-            const double rowE = sep ? exp(-(m.ay * Y * Y)) : 0.0;
-            for (int j = 0; j < roi; ++j, ++k) {
-                const double X = bx + j;
-                const double E = sep ? colE[j * 64 + lane] * rowE : m.E(X, Y);
-                body(k, i, j, X, Y, E);
+#pragma unroll 1
+            for (int i = 0; i < R; ++i) {
+                const double Y = by + i;
+                const double rowE = sep ? exp(-(m.ay * Y * Y)) : 0.0;
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    const double X = bx + j;
+                    const double E = sep ? ce[j] * rowE : m.E(X, Y);
+                    body(i * R + j, i, j, X, Y, E);
+                }
+            }
+        } else {
+            const bool sep = STAGED && m.separable();
+            if (sep)
+                for (int j = 0; j < roi; ++j) {
+                    const double X = bx + j;
+                    colE[j * 64 + lane] = exp(-(m.ax * X * X));
+                }
+            for (int i = 0, k = 0; i < roi; ++i) {
+                const double Y = by + i;
+                const double rowE = sep ? exp(-(m.ay * Y * Y)) : 0.0;
+                for (int j = 0; j < roi; ++j, ++k) {
+                    const double X = bx + j;
+                    const double E = sep ? colE[j * 64 + lane] * rowE : m.E(X, Y);
+                    body(k, i, j, X, Y, E);
+                }
             }
         }
     };
@@ -513,15 +536,32 @@ void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d
     auto go = [&](auto kern) {
         kern<<<grid, 64, lds, s>>>(d_frames, npx, W, H, d_rois, nRoi, fp, d_rows, d_next, d_passes, frameMap, d_count);
     };
-    switch (fp.mode * 2 + (staged ? 1 : 0)) {
-    case 2: go(fit_persistent_kernel<1, false>); break;
-    case 3: go(fit_persistent_kernel<1, true>); break;
-    case 4: go(fit_persistent_kernel<2, false>); break;
-    case 5: go(fit_persistent_kernel<2, true>); break;
-    case 6: go(fit_persistent_kernel<3, false>); break;
-    case 7: go(fit_persistent_kernel<3, true>); break;
-    case 8: go(fit_persistent_kernel<4, false>); break;
-    default: go(fit_persistent_kernel<4, true>); break;
+    // ROI sides 7 (the reference default) and 9 (astigmatic ROIs) are compiled out.
+    if (fp.roi == 7) {
+        switch (fp.mode) {
+        case 1: go(fit_persistent_kernel<1, true, 7>); break;
+        case 2: go(fit_persistent_kernel<2, true, 7>); break;
+        case 3: go(fit_persistent_kernel<3, true, 7>); break;
+        default: go(fit_persistent_kernel<4, true, 7>); break;
+        }
+    } else if (fp.roi == 9) {
+        switch (fp.mode) {
+        case 1: go(fit_persistent_kernel<1, true, 9>); break;
+        case 2: go(fit_persistent_kernel<2, true, 9>); break;
+        case 3: go(fit_persistent_kernel<3, true, 9>); break;
+        default: go(fit_persistent_kernel<4, true, 9>); break;
+        }
+    } else {
+        switch (fp.mode * 2 + (staged ? 1 : 0)) {
+        case 2: go(fit_persistent_kernel<1, false, 0>); break;
+        case 3: go(fit_persistent_kernel<1, true, 0>); break;
+        case 4: go(fit_persistent_kernel<2, false, 0>); break;
+        case 5: go(fit_persistent_kernel<2, true, 0>); break;
+        case 6: go(fit_persistent_kernel<3, false, 0>); break;
+        case 7: go(fit_persistent_kernel<3, true, 0>); break;
+        case 8: go(fit_persistent_kernel<4, false, 0>); break;
+        default: go(fit_persistent_kernel<4, true, 0>); break;
+        }
     }
     PT_CHECK(hipGetLastError());
 }

@@ -1463,7 +1463,10 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
             rank_flood_kernel<<<(unsigned)b.rfSlots, RF_NT, 0, s>>>(b.P, b.seedBits, b.fin, b.maskList, b.maskCnt, b.flagList, b.counters,
                                                                    b.waveList, b.rfScratch, W, H, npx,
                                                                    pass2 ? b.fz : nullptr, b.aux, b.regionCap);
-        fallback_fast_kernel<<<(unsigned)std::max<size_t>(1, std::min<size_t>(F, 1024)), 64, FB_LDS_BYTES, s>>>(
+        // Few frames reach the wave-queue flood when the rank flood runs first: a small persistent grid
+        // (each workgroup takes a whole CU's LDS).
+        const size_t fbGrid = std::max<size_t>(1, std::min<size_t>(F, rank ? 16 : 1024));
+        fallback_fast_kernel<<<(unsigned)fbGrid, 64, FB_LDS_BYTES, s>>>(
             b.P, b.seedBits, b.fin, b.maskList, b.maskCnt, b.seedList, b.seedCnt, rank ? b.waveList : b.flagList,
             rank ? kWaveCnt : kFlagCnt, b.counters, b.slowList, W, H, npx, pass2 ? b.fz : nullptr, b.regions, b.rootCnt,
             b.aux, b.regionCap, exactPath == 2 ? 1 : 0);
