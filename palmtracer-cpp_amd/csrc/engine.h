@@ -86,7 +86,7 @@ public:
 private:
     Engine();
     Lane main_, side_;
-    hipEvent_t sideEv_[2] = {nullptr, nullptr};
+    hipEvent_t sideEv_[3] = {nullptr, nullptr, nullptr};  // side batch start / done, its rows copied
     DevBuf atMask_, atPart_, atRows_, passes_;
     std::vector<hipEvent_t> ev_;
     hipEvent_t event(size_t i);

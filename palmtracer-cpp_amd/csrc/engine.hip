@@ -79,8 +79,7 @@ Engine::Engine()
     PT_CHECK(hipStreamCreateWithFlags(&main_.stream, hipStreamNonBlocking));
     PT_CHECK(hipStreamCreateWithFlags(&side_.stream, hipStreamNonBlocking));
     stream = main_.stream;
-    PT_CHECK(hipEventCreate(&sideEv_[0]));
-    PT_CHECK(hipEventCreate(&sideEv_[1]));
+    for (hipEvent_t& e : sideEv_) PT_CHECK(hipEventCreate(&e));
     if (const char* v = std::getenv("PALMTRACER_FIT_PRECISION")) opt.fitFp64 = std::strcmp(v, "fp32") != 0;
     if (const char* v = std::getenv("PALMTRACER_MAX_ITER")) opt.maxIter = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("PALMTRACER_CHUNK_PIXELS")) opt.chunkPixels = std::max<size_t>(1, std::strtoull(v, nullptr, 10));
@@ -273,6 +272,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     const size_t SF = std::max<size_t>(1, std::min<size_t>(F, std::max<size_t>(64, (size_t(1) << 24) / npx)));
     const uint64_t sideRowCap = SF * std::min<uint64_t>(maxPerPlane, npx / 4 + 2);
     size_t deferredNext = 0;            // next entry of `deferred` not yet sent
+    bool sideCopyPending = false;       // the main stream still copies the last batch's rows
     size_t batchFirst = 0, batchCount = 0;
     hipEvent_t sideStart = nullptr, sideDone = nullptr;
     auto side_launch = [&]() {
@@ -287,6 +287,8 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         side_.frameMap.ensure(SF * 4);
         sideStart = sideEv_[0];
         sideDone = sideEv_[1];
+        if (sideCopyPending) PT_CHECK(hipStreamWaitEvent(side_.stream, sideEv_[2], 0));
+        sideCopyPending = false;
         if (opt.profile) PT_CHECK(hipEventRecord(sideStart, side_.stream));
         PT_CHECK(hipMemcpyAsync(side_.frameMap.p, h, batchCount * 4, hipMemcpyHostToDevice, side_.stream));
         const uint32_t* dmap = side_.frameMap.as<uint32_t>();
@@ -334,8 +336,9 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         }
         stagedRows += n;
         batchCount = 0;
-        // The copy reads the side lane's staging buffer, which the next batch overwrites.
-        PT_CHECK(hipStreamSynchronize(main_.stream));
+        // The copy reads the side lane's staging buffer: the next batch waits for it on the device.
+        PT_CHECK(hipEventRecord(sideEv_[2], main_.stream));
+        sideCopyPending = true;
     };
     // A batch is sent whenever the side lane is idle, so that the deferred frames of a chunk are
     // resolved while the next chunks run.
@@ -357,23 +360,27 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         const size_t eb = 6 * last.chunks;
         const uint32_t n = detect_chunk(main_, frames, f, H, W, thr, ratio, maxPerPlane, eb, nullptr, false, z0,
                                         &deferred, true);
-        side_poll(false);
-        PT_CHECK(hipMemcpyAsync(base.data(), main_.b.roiBase, (f + 1) * 4, hipMemcpyDeviceToHost, main_.stream));
         main_.rows.ensure((size_t)std::max<uint32_t>(n, 1) * kCols * 8);
         grow_staging(main_, stagedRows + n, stagedRows);
+        // The chunk's staging rows are reserved before the side lane may append its own.
+        const uint64_t chunkBase = stagedRows;
+        stagedRows += n;
         if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 3), main_.stream));
         if (n) launch_fit(frames, npx, W, H, main_.rois.as<Roi>(), n, fp, main_.rows.as<double>(), opt.fitFp64,
                           main_.fitNext.as<unsigned int>(), main_.stream, passes, nullptr, opt.fitWaves);
         if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 4), main_.stream));
         if (n) launch_sort_frames(main_.rows.as<double>(), main_.b.roiBase, (int)f,
-                                  main_.staged.as<double>() + stagedRows * kCols, main_.stream);
+                                  main_.staged.as<double>() + chunkBase * kCols, main_.stream);
         if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 5), main_.stream));
+        PT_CHECK(hipMemcpyAsync(base.data(), main_.b.roiBase, (f + 1) * 4, hipMemcpyDeviceToHost, main_.stream));
+        // Host work for the side lane while the fit runs (work it queues on the main stream, such as
+        // appending side rows to the staging buffer, follows the chunk's sort in stream order).
+        side_poll(false);
         PT_CHECK(hipStreamSynchronize(main_.stream));
         for (size_t k = 0; k < f; ++k) {
-            src[z0 + k] = stagedRows + base[k];
+            src[z0 + k] = chunkBase + base[k];
             cnt[z0 + k] = base[k + 1] - base[k];
         }
-        stagedRows += n;
         last.chunks++;
     }
     side_poll(true);
