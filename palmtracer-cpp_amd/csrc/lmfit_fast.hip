@@ -175,7 +175,7 @@ __device__ inline int ldl_solve(const float (&al)[Tri<N>::v], const float (&be)[
     return ok ? 0 : 2;
 }
 
-enum : int { kFetch = 0, kDone = 1, kPassInit = 2, kPassTrial = 3, kPassR2 = 4 };
+enum : int { kFetch = 0, kDone = 1, kPassInit = 2, kPassTrial = 3 };
 
 // Largest ROI side staged in LDS: pixels (15 x 15 x 64 lanes x 2 bytes = 28.8 KB per wave) and the
 // column factors of the separable exponential (15 x 64 x 8 bytes). Larger ROIs read the frame and
@@ -219,6 +219,8 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
     double p[N], pTry[N];
     double relAcc = 0, relRej = 0;     // max |dP| / max(|p|, 1e-12) with p after acceptance / rejection
     double lambda = 0, err = 0, mq = 0;
+    double sd = 0, sdd = 0;            // data sums of the ROI (exact integers)
+    double cm = 0, cmm = 0, cdm = 0;   // model sums at p: sum m, sum m^2, sum d*m (R^2)
     float al[T], be[N];
     uint32_t nJac = 0, nTrial = 0;
 
@@ -297,6 +299,8 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
             const size_t fr = frameMap ? frameMap[r.frame] : r.frame;
             img = frames + fr * npx + (size_t)(uy - half) * W + (ux - half);
             float mn = 3.0e38f, mx = -3.0e38f;
+            uint32_t s1 = 0;
+            uint64_t s2 = 0;
             for (int i = 0, k = 0; i < roi; ++i)
                 for (int j = 0; j < roi; ++j, ++k) {
                     const uint16_t u = img[(size_t)i * W + j];
@@ -304,7 +308,11 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
                     const float v = (float)u;
                     mn = v < mn ? v : mn;
                     mx = mx < v ? v : mx;
+                    s1 += u;
+                    s2 += (uint64_t)u * u;
                 }
+            sd = (double)s1;
+            sdd = (double)s2;
             // initP (Fitting.cpp:126-175)
             p[0] = mn; p[1] = (double)mx - (double)mn; p[2] = r.x; p[3] = r.y;
             if constexpr (MODE == 2) p[4] = fp.sigmaX;
@@ -317,46 +325,9 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
         if (state == kDone) continue;
 
         double* o = rows + (size_t)idx * kCols;
-        if (state == kPassR2) {
-            // computeR2 at p: squared Pearson correlation of data and model, NaN -> 0
-            // (GaussianFit.cpp:270-295). Data sums are exact integers.
-            Model<MODE> m;
-            m.set(p, fp.sigmaX, fp.theta);
-            uint32_t sdI = 0;
-            uint64_t sddI = 0;
-            double sm = 0, smm = 0, sdm = 0;
-            for_pixels(m, p[2], p[3], [&](int k, int i, int j, double, double, double E) {
-                const float d = pix(k, i, j);
-                const double mm = m.off + m.amp * E;
-                const uint32_t di = (uint32_t)d;
-                sdI += di;
-                sddI += (uint64_t)di * di;
-                sm += mm;
-                smm += mm * mm;
-                sdm += (double)d * mm;
-            });
-            const double sd = (double)sdI, sdd = (double)sddI;
-            const double vd = sdd - sd * sd / nPix, vm = smm - sm * sm / nPix, cv = sdm - sd * sm / nPix;
-            const double rr = cv / sqrt(vd * vm);
-            const double rss = isnan(rr) ? 0.0 : rr * rr;
-            const Roi& r = rois[idx];
-            if (rss < kEpsilon) {
-                row_fail(o, MODE, p[2], p[3], r.seg, r.surf);
-            } else {
-                double sx = fp.sigmaX, sy = fp.sigmaY, th = fp.theta;
-                if constexpr (MODE == 1) { sx = sy = fp.sigmaX; th = 0; }
-                if constexpr (MODE == 2) { sx = sy = p[4]; th = 0; }
-                if constexpr (MODE >= 3) { sx = p[4] / kSqrt2d; sy = p[5] / kSqrt2d; }
-                if constexpr (MODE == 4) th = p[6];
-                row_fill(o, MODE, p[2], p[3], r.seg, r.surf, p[0], p[1], sx, sy, th, rss);
-            }
-            state = kFetch;
-            continue;
-        }
-
         // ---- chi^2 and Jacobian products at p (initial pass) or pTry (trial pass) ----
         const bool trial = state == kPassTrial;
-        double e = 0.0;
+        double e = 0.0, tm = 0.0, tmm = 0.0, tdm = 0.0;
         float alT[T], beT[N];
 #pragma unroll
         for (int a = 0; a < T; ++a) alT[a] = 0.0f;
@@ -371,8 +342,12 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
             bool circ = false;
             if constexpr (MODE == 4) circ = fabs(q[4] - q[5]) <= 2.220446049250313e-16;
             for_pixels(m, q[2], q[3], [&](int k, int i, int j, double X, double Y, double E) {
-                const double resd = (double)pix(k, i, j) - (m.off + m.amp * E);
+                const double d = (double)pix(k, i, j), mm = m.off + m.amp * E;
+                const double resd = d - mm;
                 e += resd * resd;
+                tm += mm;
+                tmm += mm * mm;
+                tdm += d * mm;
                 const float res = (float)resd;
                 float g[N];
                 m.grad((float)X, (float)Y, (float)E, g);
@@ -407,9 +382,11 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
             return false;
         };
         bool finish = false, failed = false;
+        auto take_r2 = [&]() { cm = tm; cmm = tmm; cdm = tdm; };
         if (!trial) {
             ++nJac;
             take();
+            take_r2();
             err = e;
             bool bad = !isfinite(err);
 #pragma unroll
@@ -424,6 +401,7 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
 #pragma unroll
                 for (int a = 0; a < N; ++a) p[a] = pTry[a];
                 take();
+                take_r2();
                 err = e;
                 lambda = fmax(lambda / 9, 1e-7);
                 finish = stop_after_step(relAcc);
@@ -496,7 +474,23 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
             row_fail(o, MODE, x0, y0, r.seg, r.surf);
             state = kFetch;
         } else if (finish) {
-            state = kPassR2;
+            // computeR2 at p (GaussianFit.cpp:270-295): squared Pearson correlation of data and model,
+            // NaN -> 0, from the sums of the pass that evaluated p.
+            const double vd = sdd - sd * sd / nPix, vm = cmm - cm * cm / nPix, cv = cdm - sd * cm / nPix;
+            const double rr = cv / sqrt(vd * vm);
+            const double rss = isnan(rr) ? 0.0 : rr * rr;
+            const Roi& r = rois[idx];
+            if (rss < kEpsilon) {
+                row_fail(o, MODE, p[2], p[3], r.seg, r.surf);
+            } else {
+                double sx = fp.sigmaX, sy = fp.sigmaY, th = fp.theta;
+                if constexpr (MODE == 1) { sx = sy = fp.sigmaX; th = 0; }
+                if constexpr (MODE == 2) { sx = sy = p[4]; th = 0; }
+                if constexpr (MODE >= 3) { sx = p[4] / kSqrt2d; sy = p[5] / kSqrt2d; }
+                if constexpr (MODE == 4) th = p[6];
+                row_fill(o, MODE, p[2], p[3], r.seg, r.surf, p[0], p[1], sx, sy, th, rss);
+            }
+            state = kFetch;
         }
     }
     if (passes) {
