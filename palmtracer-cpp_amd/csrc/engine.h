@@ -11,6 +11,11 @@
 
 namespace pt {
 
+struct DevBuf;
+void gather_rois(const Roi* src, uint32_t n, uint32_t z0, Roi* dst, const uint32_t* roiBase, uint32_t frames,
+                 uint32_t off, uint32_t* dBase, hipStream_t s);
+void set_u32(uint32_t* p, uint32_t v, hipStream_t s);
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -86,8 +91,9 @@ public:
 private:
     Engine();
     Lane main_, side_;
-    hipEvent_t sideEv_[3] = {nullptr, nullptr, nullptr};  // side batch start / done, its rows copied
-    DevBuf atMask_, atPart_, atRows_, passes_;
+    hipEvent_t sideEv_[2] = {nullptr, nullptr};  // side batch start / done
+    DevBuf atMask_, atPart_, atRows_, passes_, allRois_, allBase_, sideRows_;
+    hipEvent_t fitEv_[3] = {nullptr, nullptr, nullptr};
     std::vector<hipEvent_t> ev_;
     hipEvent_t event(size_t i);
     // Detection of F frames on lane L (chunk frame f = input frame map[f], or base + f without a
@@ -97,7 +103,7 @@ private:
     uint32_t detect_chunk(Lane& L, const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio,
                           uint64_t maxPoints, size_t evBase, const uint32_t* d_map, bool resolve, uint64_t firstFrame,
                           std::vector<uint32_t>* deferred, bool sync);
-    void grow_staging(Lane& L, size_t rows, size_t keepRows);
+    void grow(DevBuf& buf, size_t need, size_t keep, hipStream_t s);
     FitParams fit_params(uint64_t fit, const double* fp) const;
 };
 

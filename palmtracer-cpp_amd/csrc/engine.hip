@@ -34,6 +34,21 @@ __global__ void positions_to_rois(const double* __restrict__ pos, uint64_t n, ui
     }
 }
 
+// ROIs of a chunk appended to the call's ROI store with absolute frame numbers, and the chunk's
+// per-frame ROI offsets shifted to the store.
+__global__ void gather_rois_kernel(const Roi* __restrict__ src, uint32_t n, uint32_t z0, Roi* __restrict__ dst,
+                                   const uint32_t* __restrict__ roiBase, uint32_t frames, uint32_t off,
+                                   uint32_t* __restrict__ dBase)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    for (uint32_t i = t; i < n; i += stride) {
+        Roi r = src[i];
+        r.frame += z0;
+        dst[i] = r;
+    }
+    for (uint32_t k = t; k < frames; k += stride) dBase[k] = off + roiBase[k];
+}
+
 __global__ void masked_labels(const uint32_t* __restrict__ fin, const uint32_t* __restrict__ P, uint32_t* __restrict__ out,
                               size_t n)
 {
@@ -51,6 +66,20 @@ __global__ void positions_ids(const double* __restrict__ pos, const uint32_t* __
     }
 }
 } // namespace
+
+void gather_rois(const Roi* src, uint32_t n, uint32_t z0, Roi* dst, const uint32_t* roiBase, uint32_t frames,
+                 uint32_t off, uint32_t* dBase, hipStream_t s)
+{
+    const unsigned g = (unsigned)std::min<size_t>(4096, (std::max<size_t>(n, frames) + 255) / 256 + 1);
+    gather_rois_kernel<<<g, 256, 0, s>>>(src, n, z0, dst, roiBase, frames, off, dBase);
+    PT_CHECK(hipGetLastError());
+}
+
+void set_u32(uint32_t* p, uint32_t v, hipStream_t s)
+{
+    fill_u32_kernel<<<1, 64, 0, s>>>(p, 1, v);
+    PT_CHECK(hipGetLastError());
+}
 
 void DevBuf::ensure(size_t n)
 {
@@ -80,6 +109,7 @@ Engine::Engine()
     PT_CHECK(hipStreamCreateWithFlags(&side_.stream, hipStreamNonBlocking));
     stream = main_.stream;
     for (hipEvent_t& e : sideEv_) PT_CHECK(hipEventCreate(&e));
+    for (hipEvent_t& e : fitEv_) PT_CHECK(hipEventCreate(&e));
     if (const char* v = std::getenv("PALMTRACER_FIT_PRECISION")) opt.fitFp64 = std::strcmp(v, "fp32") != 0;
     if (const char* v = std::getenv("PALMTRACER_MAX_ITER")) opt.maxIter = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("PALMTRACER_CHUNK_PIXELS")) opt.chunkPixels = std::max<size_t>(1, std::strtoull(v, nullptr, 10));
@@ -230,16 +260,16 @@ uint32_t Engine::detect_chunk(Lane& L, const uint16_t* d_frames, size_t F, int H
     return total;
 }
 
-void Engine::grow_staging(Lane& L, size_t rowsNeeded, size_t keepRows)
+// Grows buf to hold at least `need` bytes, keeping its first `keep` bytes (copied on stream s).
+void Engine::grow(DevBuf& buf, size_t need, size_t keep, hipStream_t s)
 {
-    const size_t need = std::max<size_t>(rowsNeeded, 1) * kCols * 8;
-    if (need <= L.staged.bytes) return;
+    if (need <= buf.bytes) return;
     DevBuf nb;
-    nb.ensure(std::max(need, L.staged.bytes * 3 / 2));
-    if (keepRows) PT_CHECK(hipMemcpyAsync(nb.p, L.staged.p, keepRows * kCols * 8, hipMemcpyDeviceToDevice, L.stream));
-    PT_CHECK(hipStreamSynchronize(L.stream));
-    L.staged.release();
-    L.staged = nb;
+    nb.ensure(std::max(need, buf.bytes * 3 / 2));
+    if (keep) PT_CHECK(hipMemcpyAsync(nb.p, buf.p, keep, hipMemcpyDeviceToDevice, s));
+    PT_CHECK(hipStreamSynchronize(s));
+    buf.release();
+    buf = nb;
 }
 
 uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, double thr, double ratio, uint64_t fit,
@@ -248,6 +278,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     last = RunStats{};
     if (planes == 0 || maxPerPlane == 0 || H <= 0 || W <= 0) return 0;
     if (fit == 5) throw Error("fit mode 5 (spline PSF) is not part of this library");
+    if (planes >= 0xFFFFFFFFull) throw Error("too many planes");
     const size_t npx = (size_t)W * H;
     const size_t F = std::max<size_t>(1, std::min<size_t>(planes, std::min<size_t>(65535, opt.chunkPixels / npx)));
     main_.prepare(F, npx);
@@ -259,22 +290,26 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         PT_CHECK(hipMemsetAsync(passes, 0, 16, stream));
         PT_CHECK(hipStreamSynchronize(stream));
     }
-    // Per plane: staged row offset and row count (rows of a plane are staged contiguously).
-    std::vector<uint64_t> src(planes, 0);
-    std::vector<uint32_t> cnt(planes, 0);
+    // The main lane detects chunk after chunk and gathers the ROIs of all chunks; they are fitted in
+    // one launch at the end, which gives every lane of the persistent fit many ROIs, so that the
+    // uneven Levenberg-Marquardt iteration counts even out. ROIs are in plane order, each plane's
+    // in label order; allBase[p] is the first ROI of plane p.
     std::vector<uint32_t> deferred;
-    std::vector<uint32_t> base(F + 1);
-    uint64_t stagedRows = 0;
+    uint64_t totalRois = 0;
+    allBase_.ensure((planes + 1) * 4);
+    uint32_t* dBase = allBase_.as<uint32_t>();
 
     // Side lane: frames whose component floods could not be proven are re-detected with the exact
-    // frame flood, fitted and sorted on the side stream while the main lane goes on. A batch is
-    // queued without host synchronisation and collected at a later chunk boundary.
+    // frame flood, fitted and sorted on the side stream while the main lane goes on; each batch's
+    // sorted rows are appended to sideRows_ on that stream.
     const size_t SF = std::max<size_t>(1, std::min<size_t>(F, std::max<size_t>(64, (size_t(1) << 24) / npx)));
     const uint64_t sideRowCap = SF * std::min<uint64_t>(maxPerPlane, npx / 4 + 2);
+    std::vector<uint64_t> sideSrc;      // per deferred entry: first row in sideRows_
+    std::vector<uint32_t> sideCnt;
+    uint64_t sideRowsTotal = 0;
     size_t deferredNext = 0;            // next entry of `deferred` not yet sent
-    bool sideCopyPending = false;       // the main stream still copies the last batch's rows
     size_t batchFirst = 0, batchCount = 0;
-    hipEvent_t sideStart = nullptr, sideDone = nullptr;
+    hipEvent_t sideStart = sideEv_[0], sideDone = sideEv_[1];
     auto side_launch = [&]() {
         batchCount = std::min(SF, deferred.size() - deferredNext);
         batchFirst = deferredNext;
@@ -285,10 +320,6 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         uint32_t* h = side_.host(SF + (SF + 1) + kNumCounters);
         std::memcpy(h, deferred.data() + batchFirst, batchCount * 4);
         side_.frameMap.ensure(SF * 4);
-        sideStart = sideEv_[0];
-        sideDone = sideEv_[1];
-        if (sideCopyPending) PT_CHECK(hipStreamWaitEvent(side_.stream, sideEv_[2], 0));
-        sideCopyPending = false;
         if (opt.profile) PT_CHECK(hipEventRecord(sideStart, side_.stream));
         PT_CHECK(hipMemcpyAsync(side_.frameMap.p, h, batchCount * 4, hipMemcpyHostToDevice, side_.stream));
         const uint32_t* dmap = side_.frameMap.as<uint32_t>();
@@ -302,7 +333,8 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         PT_CHECK(hipMemcpyAsync(h + SF + SF + 1, side_.b.counters, kNumCounters * 4, hipMemcpyDeviceToHost, side_.stream));
         PT_CHECK(hipEventRecord(sideDone, side_.stream));
     };
-    // Waits for the batch in flight and appends its rows to the main staging buffer.
+    // Waits for the batch in flight and appends its rows to sideRows_ (on the side stream, before
+    // the next batch reuses the side lane's buffers).
     auto side_collect = [&]() {
         if (!batchCount) return;
         PT_CHECK(hipEventSynchronize(sideDone));
@@ -325,20 +357,16 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
             last.msDeferred += ms;
         }
         const uint32_t n = sbase[batchCount];
-        grow_staging(main_, stagedRows + n, stagedRows);
+        grow(sideRows_, (sideRowsTotal + n) * kCols * 8, sideRowsTotal * kCols * 8, side_.stream);
         if (n)
-            PT_CHECK(hipMemcpyAsync(main_.staged.as<double>() + stagedRows * kCols, side_.staged.p, (size_t)n * kCols * 8,
-                                    hipMemcpyDeviceToDevice, main_.stream));
+            PT_CHECK(hipMemcpyAsync(sideRows_.as<double>() + sideRowsTotal * kCols, side_.staged.p, (size_t)n * kCols * 8,
+                                    hipMemcpyDeviceToDevice, side_.stream));
         for (size_t k = 0; k < batchCount; ++k) {
-            const uint64_t plane = deferred[batchFirst + k];
-            src[plane] = stagedRows + sbase[k];
-            cnt[plane] = sbase[k + 1] - sbase[k];
+            sideSrc[batchFirst + k] = sideRowsTotal + sbase[k];
+            sideCnt[batchFirst + k] = sbase[k + 1] - sbase[k];
         }
-        stagedRows += n;
+        sideRowsTotal += n;
         batchCount = 0;
-        // The copy reads the side lane's staging buffer: the next batch waits for it on the device.
-        PT_CHECK(hipEventRecord(sideEv_[2], main_.stream));
-        sideCopyPending = true;
     };
     // A batch is sent whenever the side lane is idle, so that the deferred frames of a chunk are
     // resolved while the next chunks run.
@@ -349,6 +377,8 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
                 side_collect();
             }
             if (deferred.size() == deferredNext) return;
+            sideSrc.resize(deferred.size());
+            sideCnt.resize(deferred.size());
             side_launch();
             if (!wait) return;
         }
@@ -357,35 +387,52 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     for (uint64_t z0 = 0; z0 < planes; z0 += F) {
         const size_t f = std::min<uint64_t>(F, planes - z0);
         const uint16_t* frames = d_in + z0 * npx;
-        const size_t eb = 6 * last.chunks;
+        const size_t eb = 3 * last.chunks;
         const uint32_t n = detect_chunk(main_, frames, f, H, W, thr, ratio, maxPerPlane, eb, nullptr, false, z0,
                                         &deferred, true);
-        main_.rows.ensure((size_t)std::max<uint32_t>(n, 1) * kCols * 8);
-        grow_staging(main_, stagedRows + n, stagedRows);
-        // The chunk's staging rows are reserved before the side lane may append its own.
-        const uint64_t chunkBase = stagedRows;
-        stagedRows += n;
-        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 3), main_.stream));
-        if (n) launch_fit(frames, npx, W, H, main_.rois.as<Roi>(), n, fp, main_.rows.as<double>(), opt.fitFp64,
-                          main_.fitNext.as<unsigned int>(), main_.stream, passes, nullptr, opt.fitWaves);
-        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 4), main_.stream));
-        if (n) launch_sort_frames(main_.rows.as<double>(), main_.b.roiBase, (int)f,
-                                  main_.staged.as<double>() + chunkBase * kCols, main_.stream);
-        if (opt.profile) PT_CHECK(hipEventRecord(event(eb + 5), main_.stream));
-        PT_CHECK(hipMemcpyAsync(base.data(), main_.b.roiBase, (f + 1) * 4, hipMemcpyDeviceToHost, main_.stream));
-        // Host work for the side lane while the fit runs (work it queues on the main stream, such as
-        // appending side rows to the staging buffer, follows the chunk's sort in stream order).
+        if (totalRois + n > 0xFFFFFFF0ull) throw Error("too many ROIs in one call");
+        grow(allRois_, (totalRois + n) * sizeof(Roi), totalRois * sizeof(Roi), main_.stream);
+        gather_rois(main_.rois.as<Roi>(), n, (uint32_t)z0, allRois_.as<Roi>() + totalRois, main_.b.roiBase, (uint32_t)f,
+                    (uint32_t)totalRois, dBase + z0, main_.stream);
+        totalRois += n;
         side_poll(false);
-        PT_CHECK(hipStreamSynchronize(main_.stream));
-        for (size_t k = 0; k < f; ++k) {
-            src[z0 + k] = chunkBase + base[k];
-            cnt[z0 + k] = base[k + 1] - base[k];
-        }
         last.chunks++;
     }
+    set_u32(dBase + planes, (uint32_t)totalRois, main_.stream);
+    // Fit of every ROI, then each plane's rows in (Y, X) order.
+    main_.rows.ensure(std::max<uint64_t>(totalRois, 1) * kCols * 8);
+    grow(main_.staged, std::max<uint64_t>(totalRois, 1) * kCols * 8, 0, main_.stream);
+    if (opt.profile) PT_CHECK(hipEventRecord(fitEv_[0], main_.stream));
+    if (totalRois)
+        launch_fit(d_in, npx, W, H, allRois_.as<Roi>(), (uint32_t)totalRois, fp, main_.rows.as<double>(), opt.fitFp64,
+                   main_.fitNext.as<unsigned int>(), main_.stream, passes, nullptr, opt.fitWaves);
+    if (opt.profile) PT_CHECK(hipEventRecord(fitEv_[1], main_.stream));
+    if (totalRois) launch_sort_frames(main_.rows.as<double>(), dBase, (int)planes, main_.staged.as<double>(), main_.stream);
+    if (opt.profile) PT_CHECK(hipEventRecord(fitEv_[2], main_.stream));
     side_poll(true);
+    PT_CHECK(hipStreamSynchronize(side_.stream));
     last.deferredFrames = deferred.size();
 
+    // Per plane: staged row offset and row count; deferred planes take the side lane's rows, placed
+    // after the main lane's.
+    std::vector<uint32_t> hb(planes + 1);
+    PT_CHECK(hipMemcpyAsync(hb.data(), dBase, (planes + 1) * 4, hipMemcpyDeviceToHost, main_.stream));
+    PT_CHECK(hipStreamSynchronize(main_.stream));
+    std::vector<uint64_t> src(planes);
+    std::vector<uint32_t> cnt(planes);
+    for (uint64_t p = 0; p < planes; ++p) {
+        src[p] = hb[p];
+        cnt[p] = hb[p + 1] - hb[p];
+    }
+    for (size_t k = 0; k < deferred.size(); ++k) {
+        src[deferred[k]] = totalRois + sideSrc[k];
+        cnt[deferred[k]] = sideCnt[k];
+    }
+    if (sideRowsTotal) {
+        grow(main_.staged, (totalRois + sideRowsTotal) * kCols * 8, totalRois * kCols * 8, main_.stream);
+        PT_CHECK(hipMemcpyAsync(main_.staged.as<double>() + totalRois * kCols, sideRows_.p, sideRowsTotal * kCols * 8,
+                                hipMemcpyDeviceToDevice, main_.stream));
+    }
     // Final placement in plane order.
     std::vector<uint64_t> dst(planes);
     uint64_t total = 0;
@@ -402,16 +449,17 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     PT_CHECK(hipStreamSynchronize(stream));
     if (opt.profile) {
         for (size_t c = 0; c < last.chunks; ++c) {
-            float ms[4] = {0, 0, 0, 0};
-            PT_CHECK(hipEventElapsedTime(&ms[0], event(6 * c + 0), event(6 * c + 1)));
-            PT_CHECK(hipEventElapsedTime(&ms[1], event(6 * c + 1), event(6 * c + 2)));
-            PT_CHECK(hipEventElapsedTime(&ms[2], event(6 * c + 3), event(6 * c + 4)));
-            PT_CHECK(hipEventElapsedTime(&ms[3], event(6 * c + 4), event(6 * c + 5)));
+            float ms[2] = {0, 0};
+            PT_CHECK(hipEventElapsedTime(&ms[0], event(3 * c + 0), event(3 * c + 1)));
+            PT_CHECK(hipEventElapsedTime(&ms[1], event(3 * c + 1), event(3 * c + 2)));
             last.msAtrous += ms[0];
             last.msSegment += ms[1];
-            last.msFit += ms[2];
-            last.msAssemble += ms[3];
         }
+        float ms[2] = {0, 0};
+        PT_CHECK(hipEventElapsedTime(&ms[0], fitEv_[0], fitEv_[1]));
+        PT_CHECK(hipEventElapsedTime(&ms[1], fitEv_[1], fitEv_[2]));
+        last.msFit = ms[0];
+        last.msAssemble = ms[1];
         unsigned long long pc[2];
         PT_CHECK(hipMemcpy(pc, passes, 16, hipMemcpyDeviceToHost));
         last.jacPasses = pc[0];
