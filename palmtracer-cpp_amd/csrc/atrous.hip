@@ -19,6 +19,8 @@
 #include "pt_common.h"
 #include "pt_kernels.h"
 
+#include <type_traits>
+
 namespace pt {
 
 namespace {
@@ -138,38 +140,66 @@ __global__ void __launch_bounds__(NT) atrous_plane_kernel(const uint16_t* __rest
 // Detection front end, one wave per (column strip, row segment) of a frame.
 //
 // A strip is 64 lanes x SC columns (lane l holds columns 3l, 3l+1, 3l+2 of the strip); it covers
-// SOUT output columns plus a 7-column halo on each side (the dependency reach of the local-maximum
-// test through both scales). The wave walks its segment's rows top to bottom as a pipeline: input
-// row t gives A(t) (row pass, scale 0), V1s(t-2) (column pass from a 5-row ring of A), B(t-2) (row
-// pass, scale 1), V2s(t-6) (column pass from a 9-row ring of B), P(t-6) and the local-maximum test
-// of row t-7 (3-row ring of P). Horizontal neighbours come from the adjacent lanes through DPP
-// wave shifts; vertical neighbours from registers. Clamp-to-edge is applied per plane as in the
-// reference: input rows and columns are read clamped (so A is right everywhere), V1s columns
-// outside the frame are replaced by the edge column, and the B ring is filled with B(0) at the top
-// and repeats B(H-1) below the frame.
+// SOUT output columns (lanes 3..60) plus 9 halo columns (3 lanes) on each side, more than the
+// 7-column dependency reach of the local-maximum test through both scales. The wave walks its
+// segment's rows top to bottom as a pipeline: input row t gives A(t) (row pass, scale 0), V1s(t-2)
+// (column pass from a 5-row ring of A), B(t-2) (row pass, scale 1), V2s(t-6) (column pass from a
+// 9-row ring of B), P(t-6) and the local-maximum test of row t-7 (3-row ring of P). Horizontal
+// neighbours come from the adjacent lanes through DPP wave shifts; vertical neighbours from
+// registers. Clamp-to-edge is applied per plane as in the reference: input rows and columns are
+// read clamped (so A is right everywhere), V1s columns outside the frame are replaced by the edge
+// column, and the B ring is filled with B(0) at the top and repeats B(H-1) below the frame.
+// Rows whose every stage lies inside the frame and the segment run in an unrolled loop without
+// those edge cases, so that the rings rotate by register renaming instead of copies.
 //
 //   P[frame*npx + i]           dense, every pixel: 65536 * W1 if >= tmin, else 0
 //   lab[frame*npx + i] = i     for pixels with P > 0 (union-find initialisation)
-//   maskList / seedList        per-frame compact lists (capacity npx each) of pixel indices with
-//                              P > 0 / local maxima (Segmentation2D.cpp:294-333 test: no 8-neighbour
-//                              inside the frame strictly greater), counts in maskCnt / seedCnt.
+//   maskList / seedList        per-frame compact lists (capacity npx each, any order) of pixel
+//                              indices with P > 0 / local maxima (Segmentation2D.cpp:294-333
+//                              test: no 8-neighbour inside the frame strictly greater), counts in
+//                              maskCnt / seedCnt. Entries are staged per wave in LDS and appended
+//                              with one atomic per batch.
 //   seedBits                   per-frame bitmap of the local maxima (zeroed by the caller).
 constexpr int SC = 3;                   // columns per lane
 constexpr int SW = 64 * SC;             // strip width
-constexpr int SHALO = 7;                // halo columns on each side
+constexpr int SHALO = 9;                // halo columns on each side (3 lanes)
 constexpr int SOUT = SW - 2 * SHALO;    // output columns per strip
+constexpr int MCAP = 1024;              // staged mask entries per wave
+constexpr int SCAP = 256;               // staged seed entries per wave
+constexpr int UNROLL = 4;               // rows per iteration of the interior loop
+constexpr uint64_t kOutLanes = ((1ull << (64 - SHALO / SC)) - 1ull) & ~((1ull << (SHALO / SC)) - 1ull);
+constexpr int kIcmpNe = 33, kIcmpUge = 35;  // LLVM integer predicates of __builtin_amdgcn_uicmp
 
 // Value of lane (l - 1) / (l + 1) of the wave (DPP wave shifts; edge lanes read 0).
-__device__ inline uint32_t from_left(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false); }
-__device__ inline uint32_t from_right(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false); }
+__device__ inline uint32_t from_left(uint32_t v) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x138, 0xf, 0xf, true); }
+__device__ inline uint32_t from_right(uint32_t v) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xf, 0xf, true); }
 
+// {1, 4, 6, 4, 1} taps with shifts and adds only (no 32-bit multiply on the VALU).
 __device__ inline uint32_t b3(uint32_t m2, uint32_t m1, uint32_t c, uint32_t p1, uint32_t p2)
 {
-    return m2 + p2 + 4u * (m1 + p1) + 6u * c;
+    return ((m1 + p1 + c) << 2) + (c << 1) + m2 + p2;
 }
 
-// Slot k of value-from-uniform-position: v[k] where k is wave-uniform.
-__device__ inline uint32_t pick(const uint32_t (&v)[SC], int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
+__device__ inline uint32_t max3u(uint32_t a, uint32_t b, uint32_t c) { return max(max(a, b), c); }
+
+// Number of set bits of m in the lanes below this one.
+__device__ inline uint32_t lanes_below(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ inline void wave_fence()
+{
+    __builtin_amdgcn_wave_barrier();
+    __asm__ __volatile__("" ::: "memory");
+}
+
+// Buffer resource over `bytes` bytes at p (raw buffer, dword 3 of gfx9). Loads and stores through
+// it take a per-lane 32-bit offset and a wave-uniform scalar offset, so no 64-bit address math.
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
 
 __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __restrict__ in, uint32_t* __restrict__ P,
                                                            uint32_t* __restrict__ lab, uint32_t* __restrict__ maskList,
@@ -178,34 +208,46 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
                                                            int W, int H, int64_t tmin, int strips, int segH,
                                                            const uint32_t* __restrict__ frameMap)
 {
+    __shared__ uint32_t sMask[MCAP];
+    __shared__ uint32_t sSeed[SCAP];
     const int lane = threadIdx.x;
     const int f = blockIdx.y;
     const int strip = blockIdx.x % strips, seg = blockIdx.x / strips;
-    const size_t npx = (size_t)W * H;
-    const uint16_t* img = in + (size_t)(frameMap ? frameMap[f] : (uint32_t)f) * npx;
-    uint32_t* Pf = P + (size_t)f * npx;
-    uint32_t* lb = lab + (size_t)f * npx;
+    const uint32_t npx = (uint32_t)W * (uint32_t)H;     // < 2^30 (checked by the launcher)
+    const char* img = reinterpret_cast<const char*>(in + (size_t)(frameMap ? frameMap[f] : (uint32_t)f) * npx);
+    const __amdgpu_buffer_rsrc_t rsP = make_rsrc(P + (size_t)f * npx, npx * 4u);
+    const __amdgpu_buffer_rsrc_t rsLab = make_rsrc(lab + (size_t)f * npx, npx * 4u);
     uint32_t* ml = maskList + (size_t)f * npx;
     uint32_t* sl = seedList + (size_t)f * npx;
     uint32_t* sbf = seedBits + (size_t)f * seed_words(npx);
     const int xs = strip * SOUT - SHALO;    // frame column of strip column 0
     const int ys = seg * segH, ye = min(ys + segH, H);
+    // P < 2^32 - 1 (256 * V1s <= 2^32 - 2^16), so a saturated threshold passes nothing.
+    const uint32_t tm = tmin > (int64_t)0xFFFFFFFFu ? 0xFFFFFFFFu : (uint32_t)tmin;
 
-    int xcol[SC], xcl[SC];
-    bool outCol[SC], inFrame[SC];
+    int xcol[SC];
+    uint32_t inOff[SC], outOff[SC];     // byte offsets of the lane's columns in a row of u16 / u32
+    uint32_t tmk[SC];                   // threshold per column; columns outside the frame never pass
+    bool outCol[SC];
+    const bool outLane = lane >= SHALO / SC && lane < 64 - SHALO / SC;
 #pragma unroll
     for (int k = 0; k < SC; ++k) {
-        const int o = SC * lane + k;
-        xcol[k] = xs + o;
-        xcl[k] = xcol[k] < 0 ? 0 : (xcol[k] > W - 1 ? W - 1 : xcol[k]);
-        inFrame[k] = xcol[k] >= 0 && xcol[k] < W;
-        outCol[k] = inFrame[k] && o >= SHALO && o < SW - SHALO;
+        xcol[k] = xs + SC * lane + k;
+        const int xc = xcol[k] < 0 ? 0 : (xcol[k] > W - 1 ? W - 1 : xcol[k]);
+        inOff[k] = (uint32_t)xc * 2u;
+        outOff[k] = (uint32_t)xc * 4u;
+        const bool inFrame = xcol[k] >= 0 && xcol[k] < W;
+        tmk[k] = inFrame ? tm : 0xFFFFFFFFu;
+        outCol[k] = outLane && inFrame;
     }
-    // Strip positions of the frame's first / last column (wave-uniform) for the V1s edge fix-up.
-    const int oL = -xs, oR = W - 1 - xs;
-    const bool fixL = oL > 0, fixR = oR < SW - 1;
+    // The frame's first column is strip column SHALO of strip 0 (lane SHALO / SC, slot 0); its last
+    // column is strip column oR of the last strip.
+    const bool fixL = xs < 0;
+    const int oR = W - 1 - xs;
+    const bool fixR = oR < SW - 1;
+    const int lR = oR / SC, kR = oR % SC;
 
-    uint32_t rA[5][SC], rB[9][SC], rV[5][SC], rP[3][SC], rPl[3], rPr[3];
+    uint32_t rA[5][SC], rB[9][SC], rV[5][SC], rP[3][SC];
 #pragma unroll
     for (int k = 0; k < SC; ++k) {
 #pragma unroll
@@ -215,33 +257,89 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
 #pragma unroll
         for (int i = 0; i < 3; ++i) rP[i][k] = 0;
     }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) { rPl[i] = 0; rPr[i] = 0; }
+    uint32_t mPend = 0, sPend = 0;      // staged list entries (wave-uniform)
 
     auto load_row = [&](int t, uint32_t (&v)[SC]) {
         const int y = t < 0 ? 0 : (t > H - 1 ? H - 1 : t);
-        const uint16_t* r = img + (size_t)y * W;
+        const char* row = img + (size_t)y * W * 2;
 #pragma unroll
-        for (int k = 0; k < SC; ++k) v[k] = r[xcl[k]];
+        for (int k = 0; k < SC; ++k) v[k] = *reinterpret_cast<const uint16_t*>(row + inOff[k]);
+    };
+    auto flush = [&](const uint32_t* stage, uint32_t n, uint32_t* cnt, uint32_t* list) {
+        wave_fence();
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(cnt, n);
+        base = __builtin_amdgcn_readfirstlane(base);
+        for (uint32_t i = lane; i < n; i += 64) list[base + i] = stage[i];
+        wave_fence();
+    };
+    // Dense P, mask and local maxima of output row y3 from the P ring.
+    auto emit = [&](int y3) {
+        const uint32_t rowBase = (uint32_t)y3 * (uint32_t)W;
+        const int rowOff = (int)(rowBase * 4u);
+        uint32_t v[SC], cm[SC];
+#pragma unroll
+        for (int k = 0; k < SC; ++k) {
+            v[k] = rP[1][k];
+            cm[k] = max3u(rP[0][k], rP[1][k], rP[2][k]);
+        }
+#pragma unroll
+        for (int k = 0; k < SC; ++k)
+            if (outCol[k]) __builtin_amdgcn_raw_buffer_store_b32(v[k], rsP, (int)outOff[k], rowOff, 0);
+        // Pixels with P > 0 (columns outside the frame have P = 0) and the local maxima among them.
+        const uint32_t cL = from_left(cm[2]), cR = from_right(cm[0]);
+        const bool m0 = outLane && v[0] != 0u, m1 = outLane && v[1] != 0u, m2 = outLane && v[2] != 0u;
+        const uint32_t M0 = max3u(cL, cm[0], cm[1]), M1 = max3u(cm[0], cm[1], cm[2]), M2 = max3u(cm[1], cm[2], cR);
+        const bool s0 = m0 && v[0] >= M0, s1 = m1 && v[1] >= M1, s2 = m2 && v[2] >= M2;
+        // The same predicates as wave masks (compare results, no round trip through a VGPR).
+        const uint64_t mb0 = __builtin_amdgcn_uicmp(v[0], 0u, kIcmpNe) & kOutLanes;
+        const uint64_t mb1 = __builtin_amdgcn_uicmp(v[1], 0u, kIcmpNe) & kOutLanes;
+        const uint64_t mb2 = __builtin_amdgcn_uicmp(v[2], 0u, kIcmpNe) & kOutLanes;
+        const uint64_t sb0 = __builtin_amdgcn_uicmp(v[0], M0, kIcmpUge) & mb0;
+        const uint64_t sb1 = __builtin_amdgcn_uicmp(v[1], M1, kIcmpUge) & mb1;
+        const uint64_t sb2 = __builtin_amdgcn_uicmp(v[2], M2, kIcmpUge) & mb2;
+        const uint32_t mc0 = (uint32_t)__popcll(mb0), mc1 = (uint32_t)__popcll(mb1);
+        const uint32_t mRow = mc0 + mc1 + (uint32_t)__popcll(mb2);
+        if (mRow == 0) return;
+        const uint32_t i0 = rowBase + (uint32_t)xcol[0], i1 = i0 + 1u, i2 = i0 + 2u;
+        if (m0) __builtin_amdgcn_raw_buffer_store_b32(i0, rsLab, (int)outOff[0], rowOff, 0);
+        if (m1) __builtin_amdgcn_raw_buffer_store_b32(i1, rsLab, (int)outOff[1], rowOff, 0);
+        if (m2) __builtin_amdgcn_raw_buffer_store_b32(i2, rsLab, (int)outOff[2], rowOff, 0);
+        if (mPend + mRow > (uint32_t)MCAP) { flush(sMask, mPend, &maskCnt[f], ml); mPend = 0; }
+        if (m0) sMask[mPend + lanes_below(mb0)] = i0;
+        if (m1) sMask[mPend + mc0 + lanes_below(mb1)] = i1;
+        if (m2) sMask[mPend + mc0 + mc1 + lanes_below(mb2)] = i2;
+        mPend += mRow;
+
+        const uint32_t sc0 = (uint32_t)__popcll(sb0), sc1 = (uint32_t)__popcll(sb1);
+        const uint32_t sRow = sc0 + sc1 + (uint32_t)__popcll(sb2);
+        if (sRow == 0) return;
+        if (s0) atomicOr(&sbf[i0 >> 5], 1u << (i0 & 31));
+        if (s1) atomicOr(&sbf[i1 >> 5], 1u << (i1 & 31));
+        if (s2) atomicOr(&sbf[i2 >> 5], 1u << (i2 & 31));
+        if (sPend + sRow > (uint32_t)SCAP) { flush(sSeed, sPend, &seedCnt[f], sl); sPend = 0; }
+        if (s0) sSeed[sPend + lanes_below(sb0)] = i0;
+        if (s1) sSeed[sPend + sc0 + lanes_below(sb1)] = i1;
+        if (s2) sSeed[sPend + sc0 + sc1 + lanes_below(sb2)] = i2;
+        sPend += sRow;
     };
 
-    const int t0 = ys - 7, t1 = ye + 6;
     uint32_t nxt[SC];
-    load_row(t0, nxt);
-    for (int t = t0; t <= t1; ++t) {
+    // One pipeline step for input row t. Interior steps (every stage inside the frame, output row
+    // inside the segment) skip the edge handling.
+    auto step = [&](int t, auto interior) {
+        constexpr bool kInterior = decltype(interior)::value;
         uint32_t v0[SC];
 #pragma unroll
         for (int k = 0; k < SC; ++k) v0[k] = nxt[k];
-        if (t < t1) load_row(t + 1, nxt);
+        load_row(t + 1, nxt);               // clamped: the row after the last one is still in the frame
 
         // A(t) = 16 * rowpass_0(V0): taps at columns c-2..c+2.
         {
             const uint32_t l1 = from_left(v0[1]), l2 = from_left(v0[2]);
             const uint32_t r0 = from_right(v0[0]), r1 = from_right(v0[1]);
-            uint32_t a[SC];
-            a[0] = b3(l1, l2, v0[0], v0[1], v0[2]);
-            a[1] = b3(l2, v0[0], v0[1], v0[2], r0);
-            a[2] = b3(v0[0], v0[1], v0[2], r0, r1);
+            const uint32_t a[SC] = {b3(l1, l2, v0[0], v0[1], v0[2]), b3(l2, v0[0], v0[1], v0[2], r0),
+                                    b3(v0[0], v0[1], v0[2], r0, r1)};
 #pragma unroll
             for (int k = 0; k < SC; ++k) {
 #pragma unroll
@@ -256,14 +354,14 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
         for (int k = 0; k < SC; ++k) u[k] = b3(rA[0][k], rA[1][k], rA[2][k], rA[3][k], rA[4][k]);
         // Columns outside the frame hold the edge column's V1s (clamp of the scale-1 taps).
         if (fixL) {
-            const uint32_t e = __builtin_amdgcn_readlane(pick(u, oL % SC), oL / SC);
+            const uint32_t e = __builtin_amdgcn_readlane(u[0], SHALO / SC);
 #pragma unroll
-            for (int k = 0; k < SC; ++k) if (xcol[k] < 0) u[k] = e;
+            for (int k = 0; k < SC; ++k) u[k] = lane < SHALO / SC ? e : u[k];
         }
         if (fixR) {
-            const uint32_t e = __builtin_amdgcn_readlane(pick(u, oR % SC), oR / SC);
+            const uint32_t e = __builtin_amdgcn_readlane(kR == 0 ? u[0] : (kR == 1 ? u[1] : u[2]), lR);
 #pragma unroll
-            for (int k = 0; k < SC; ++k) if (xcol[k] > W - 1) u[k] = e;
+            for (int k = 0; k < SC; ++k) u[k] = xcol[k] > W - 1 ? e : u[k];
         }
 #pragma unroll
         for (int k = 0; k < SC; ++k) {
@@ -272,15 +370,27 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
             rV[4][k] = u[k];
         }
         // B(y1) = 16 * rowpass_1(V1s): taps at columns c-4, c-2, c, c+2, c+4.
-        if (y1 >= 0) {
+        auto row_b = [&](uint32_t (&bb)[SC]) {
+            const uint32_t L0 = from_left(u[0]), L1 = from_left(u[1]), L2 = from_left(u[2]);
+            const uint32_t R0 = from_right(u[0]), R1 = from_right(u[1]), R2 = from_right(u[2]);
+            const uint32_t LL2 = from_left(L2), RR0 = from_right(R0);
+            bb[0] = b3(LL2, L1, u[0], u[2], R1);
+            bb[1] = b3(L0, L2, u[1], R0, R2);
+            bb[2] = b3(L1, u[0], u[2], R1, RR0);
+        };
+        if constexpr (kInterior) {
+            uint32_t bb[SC];
+            row_b(bb);
+#pragma unroll
+            for (int k = 0; k < SC; ++k) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) rB[i][k] = rB[i + 1][k];
+                rB[8][k] = bb[k];
+            }
+        } else if (y1 >= 0) {
             uint32_t bb[SC];
             if (y1 < H) {
-                const uint32_t L0 = from_left(u[0]), L1 = from_left(u[1]), L2 = from_left(u[2]);
-                const uint32_t R0 = from_right(u[0]), R1 = from_right(u[1]), R2 = from_right(u[2]);
-                const uint32_t LL2 = from_left(L2), RR0 = from_right(R0);
-                bb[0] = b3(LL2, L1, u[0], u[2], R1);
-                bb[1] = b3(L0, L2, u[1], R0, R2);
-                bb[2] = b3(L1, u[0], u[2], R1, RR0);
+                row_b(bb);
             } else {
 #pragma unroll
                 for (int k = 0; k < SC; ++k) bb[k] = rB[8][k];  // below the frame: B(H-1)
@@ -297,76 +407,45 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
                 rB[8][k] = bb[k];
             }
         }
-        // P(y2), y2 = t - 6: V2s = colpass_1(B) from rows y2-4, y2-2, y2, y2+2, y2+4.
+        // P(y2), y2 = t - 6: V2s = colpass_1(B) from rows y2-4, y2-2, y2, y2+2, y2+4;
+        // 65536 * W1 = 256 * V1s - V2s, kept when >= tmin (u32: both terms are below 2^32).
         const int y2 = t - 6;
+        const bool rowIn = kInterior || (y2 >= 0 && y2 < H);
         uint32_t pr[SC];
 #pragma unroll
         for (int k = 0; k < SC; ++k) {
-            pr[k] = 0;
-            if (y2 >= 0 && y2 < H && inFrame[k]) {
-                const uint32_t v2s = b3(rB[0][k], rB[2][k], rB[4][k], rB[6][k], rB[8][k]);
-                const int64_t w1s = (int64_t)rV[0][k] * 256 - (int64_t)v2s;
-                pr[k] = w1s >= tmin ? (uint32_t)w1s : 0u;
-            }
+            const uint32_t v2s = b3(rB[0][k], rB[2][k], rB[4][k], rB[6][k], rB[8][k]);
+            const uint32_t a = rV[0][k] << 8;
+            const uint32_t d = a - v2s;
+            pr[k] = (rowIn && a >= v2s && d >= tmk[k]) ? d : 0u;
         }
-        {
-            const uint32_t pl = from_left(pr[2]), prr = from_right(pr[0]);
 #pragma unroll
-            for (int k = 0; k < SC; ++k) {
-                rP[0][k] = rP[1][k];
-                rP[1][k] = rP[2][k];
-                rP[2][k] = pr[k];
-            }
-            rPl[0] = rPl[1]; rPl[1] = rPl[2]; rPl[2] = pl;
-            rPr[0] = rPr[1]; rPr[1] = rPr[2]; rPr[2] = prr;
+        for (int k = 0; k < SC; ++k) {
+            rP[0][k] = rP[1][k];
+            rP[1][k] = rP[2][k];
+            rP[2][k] = pr[k];
         }
-        // Output row y3 = t - 7: dense P, mask and local maxima.
+        // Output row y3 = t - 7.
         const int y3 = t - 7;
-        if (y3 < ys) continue;
-        uint32_t mbits = 0, sbits = 0;
+        if (kInterior || y3 >= ys) emit(y3);
+    };
+
+    const int t0 = ys - 7, t1 = ye + 6;
+    // Interior steps: output row inside the segment (t >= ys + 7) and V1s row inside the frame
+    // (t - 2 <= H - 1); the P row t - 6 is then inside the frame too.
+    const int tI0 = ys + 7, tI1 = min(t1, H + 1);
+    load_row(t0, nxt);
+    int t = t0;
+    for (; t < tI0; ++t) step(t, std::false_type{});
+    for (; t + UNROLL - 1 <= tI1; t += UNROLL) {
 #pragma unroll
-        for (int k = 0; k < SC; ++k) {
-            const uint32_t v = rP[1][k];
-            if (outCol[k]) Pf[(size_t)y3 * W + xcol[k]] = v;
-            if (!outCol[k] || v == 0) continue;
-            mbits |= 1u << k;
-            bool mx = true;
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                const uint32_t left = k == 0 ? rPl[r] : rP[r][k - 1];
-                const uint32_t right = k == SC - 1 ? rPr[r] : rP[r][k + 1];
-                if (v < left || v < right || (r != 1 && v < rP[r][k])) mx = false;
-            }
-            if (mx) sbits |= 1u << k;
-        }
-        const uint32_t nm = __popc(mbits), ns = __popc(sbits);
-        uint32_t tm = nm, tsd = ns;
-        // Wave prefix sums of the per-lane counts.
-        uint32_t pm = nm, ps = ns;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t qm = __shfl_up(pm, o), qs = __shfl_up(ps, o);
-            if (lane >= o) { pm += qm; ps += qs; }
-        }
-        tm = __shfl(pm, 63);
-        tsd = __shfl(ps, 63);
-        uint32_t bm = 0, bs = 0;
-        if (lane == 0) {
-            if (tm) bm = atomicAdd(&maskCnt[f], tm);
-            if (tsd) bs = atomicAdd(&seedCnt[f], tsd);
-        }
-        bm = __shfl(bm, 0) + pm - nm;
-        bs = __shfl(bs, 0) + ps - ns;
-#pragma unroll
-        for (int k = 0; k < SC; ++k) {
-            const uint32_t idx = (uint32_t)(y3 * W + xcol[k]);
-            if (mbits & (1u << k)) { ml[bm++] = idx; lb[idx] = idx; }
-            if (sbits & (1u << k)) {
-                sl[bs++] = idx;
-                atomicOr(&sbf[idx >> 5], 1u << (idx & 31));
-            }
-        }
+        for (int j = 0; j < UNROLL; ++j) step(t + j, std::true_type{});
     }
+    for (; t <= tI1; ++t) step(t, std::true_type{});
+    for (; t <= t1; ++t) step(t, std::false_type{});
+
+    if (mPend) flush(sMask, mPend, &maskCnt[f], ml);
+    if (sPend) flush(sSeed, sPend, &seedCnt[f], sl);
 }
 } // namespace
 
@@ -386,6 +465,7 @@ void launch_atrous_detect(const uint16_t* d_in, const DetectBuffers& b, int W, i
 {
     if (frames <= 0 || W <= 0 || H <= 0) return;
     if (frames > 65535) throw Error("launch_atrous_detect: at most 65535 frames per launch");
+    if ((size_t)W * H >= (size_t(1) << 30)) throw Error("launch_atrous_detect: frames of 2^30 pixels or more are not supported");
     const int strips = (int)ceil_div(W, SOUT);
     // Row segments: long enough to amortise the 13-row pipeline warm-up, short enough to give the
     // chip a few waves per SIMD when the chunk is small.
