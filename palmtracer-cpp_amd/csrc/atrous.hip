@@ -153,7 +153,6 @@ __global__ void __launch_bounds__(NT) atrous_plane_kernel(const uint16_t* __rest
 // those edge cases, so that the rings rotate by register renaming instead of copies.
 //
 //   P[frame*npx + i]           dense, every pixel: 65536 * W1 if >= tmin, else 0
-//   lab[frame*npx + i] = i     for pixels with P > 0 (union-find initialisation)
 //   maskList / seedList        per-frame compact lists (capacity npx each, any order) of pixel
 //                              indices with P > 0 / local maxima (Segmentation2D.cpp:294-333
 //                              test: no 8-neighbour inside the frame strictly greater), counts in
@@ -202,7 +201,7 @@ __device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes
 }
 
 __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __restrict__ in, uint32_t* __restrict__ P,
-                                                           uint32_t* __restrict__ lab, uint32_t* __restrict__ maskList,
+                                                           uint32_t* __restrict__ maskList,
                                                            uint32_t* __restrict__ maskCnt, uint32_t* __restrict__ seedList,
                                                            uint32_t* __restrict__ seedCnt, uint32_t* __restrict__ seedBits,
                                                            int W, int H, int64_t tmin, int strips, int segH,
@@ -216,7 +215,6 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
     const uint32_t npx = (uint32_t)W * (uint32_t)H;     // < 2^30 (checked by the launcher)
     const char* img = reinterpret_cast<const char*>(in + (size_t)(frameMap ? frameMap[f] : (uint32_t)f) * npx);
     const __amdgpu_buffer_rsrc_t rsP = make_rsrc(P + (size_t)f * npx, npx * 4u);
-    const __amdgpu_buffer_rsrc_t rsLab = make_rsrc(lab + (size_t)f * npx, npx * 4u);
     uint32_t* ml = maskList + (size_t)f * npx;
     uint32_t* sl = seedList + (size_t)f * npx;
     uint32_t* sbf = seedBits + (size_t)f * seed_words(npx);
@@ -302,9 +300,6 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
         const uint32_t mRow = mc0 + mc1 + (uint32_t)__popcll(mb2);
         if (mRow == 0) return;
         const uint32_t i0 = rowBase + (uint32_t)xcol[0], i1 = i0 + 1u, i2 = i0 + 2u;
-        if (m0) __builtin_amdgcn_raw_buffer_store_b32(i0, rsLab, (int)outOff[0], rowOff, 0);
-        if (m1) __builtin_amdgcn_raw_buffer_store_b32(i1, rsLab, (int)outOff[1], rowOff, 0);
-        if (m2) __builtin_amdgcn_raw_buffer_store_b32(i2, rsLab, (int)outOff[2], rowOff, 0);
         if (mPend + mRow > (uint32_t)MCAP) { flush(sMask, mPend, &maskCnt[f], ml); mPend = 0; }
         if (m0) sMask[mPend + lanes_below(mb0)] = i0;
         if (m1) sMask[mPend + mc0 + lanes_below(mb1)] = i1;
@@ -474,7 +469,7 @@ void launch_atrous_detect(const uint16_t* d_in, const DetectBuffers& b, int W, i
     const int segs = (int)ceil_div(H, segH);
     if ((size_t)strips * segs > 0x7FFFFFFF) throw Error("launch_atrous_detect: frame too large");
     const dim3 grid((unsigned)(strips * segs), (unsigned)frames);
-    atrous_detect_kernel<<<grid, 64, 0, s>>>(d_in, b.P, b.lab, b.maskList, b.maskCnt, b.seedList, b.seedCnt, b.seedBits,
+    atrous_detect_kernel<<<grid, 64, 0, s>>>(d_in, b.P, b.maskList, b.maskCnt, b.seedList, b.seedCnt, b.seedBits,
                                              W, H, tmin, strips, segH, frameMap);
     PT_CHECK(hipGetLastError());
 }

@@ -27,7 +27,7 @@
 namespace pt {
 namespace {
 
-constexpr uint32_t kVisited = 0x80000000u;
+constexpr uint32_t kCompRoot = 0x80000000u;   // lab bit of a component's first pixel
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int NT = 256;
 
@@ -134,115 +134,25 @@ __device__ inline void flag_frame(uint32_t* frameFlag, uint32_t* counters, uint3
 }
 
 // ---------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(NT) cc_union_kernel(uint32_t* __restrict__ lab, const uint32_t* __restrict__ P,
-                                                      const uint32_t* __restrict__ maskList, const uint32_t* __restrict__ maskCnt,
-                                                      PixAux* __restrict__ aux, int W, int H, size_t npx)
+// Second-pass labelling of the components: aux[component] holds the smallest and largest label of
+// the component's surviving maxima.
+__global__ void __launch_bounds__(NT) label_assign_kernel(const uint32_t* __restrict__ lab, uint32_t* __restrict__ fin,
+                                                          const PixAux* __restrict__ aux, const uint32_t* __restrict__ maskList,
+                                                          const uint32_t* __restrict__ maskCnt, uint64_t* __restrict__ multiList,
+                                                          size_t multiCap, uint32_t* __restrict__ counters,
+                                                          uint32_t* __restrict__ frameFlag, uint32_t* __restrict__ flagList,
+                                                          size_t npx)
 {
     const int f = blockIdx.y;
     const uint32_t cnt = maskCnt[f];
-    uint32_t* lb = lab + (size_t)f * npx;
-    const uint32_t* Pf = P + (size_t)f * npx;
     for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < cnt; k += gridDim.x * NT) {
         const uint32_t p = maskList[(size_t)f * npx + k];
-        aux[(size_t)f * npx + p] = PixAux{kNone, 0u, 0u, kNone};
-        const int y = p / W, x = p - y * W;
-        // Earlier 8-neighbours: W, NW, N, NE.
-        const int dys[4] = {0, -1, -1, -1}, dxs[4] = {-1, -1, 0, 1};
-        for (int j = 0; j < 4; ++j) {
-            const int yy = y + dys[j], xx = x + dxs[j];
-            if (yy < 0 || xx < 0 || xx >= W) continue;
-            const uint32_t q = (uint32_t)(yy * W + xx);
-            if (Pf[q] == 0) continue;
-            uint32_t a = p, b = q;
-            while (true) {
-                a = uf_find(lb, a);
-                b = uf_find(lb, b);
-                if (a == b) break;
-                if (a > b) { const uint32_t t = a; a = b; b = t; }
-                const uint32_t old = atomicMin(&lb[b], a);
-                if (old == b) break;
-                b = old;
-            }
-        }
-    }
-}
-
-__global__ void __launch_bounds__(NT) cc_flatten_kernel(uint32_t* __restrict__ lab, const uint32_t* __restrict__ maskList,
-                                                        const uint32_t* __restrict__ maskCnt, PixAux* __restrict__ aux,
-                                                        size_t npx)
-{
-    const int f = blockIdx.y;
-    const uint32_t cnt = maskCnt[f];
-    uint32_t* lb = lab + (size_t)f * npx;
-    for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < cnt; k += gridDim.x * NT) {
-        const uint32_t p = maskList[(size_t)f * npx + k];
-        const uint32_t r = uf_find(lb, p);
-        lb[p] = r;
-        atomicAdd(&aux[(size_t)f * npx + r].csize, 1u);
-    }
-}
-
-// getLocalMax labels (Segmentation2D.cpp:309-327): a maximum inherits the label of the first
-// earlier maximum among (y-1,x-1), (y-1,x), (y-1,x+1), (y,x-1); otherwise it opens a new label.
-__global__ void __launch_bounds__(NT) seed_label_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ seedBits, const uint32_t* __restrict__ lab,
-                                                        uint32_t* __restrict__ fin, PixAux* __restrict__ aux,
-                                                        const uint32_t* __restrict__ seedList, const uint32_t* __restrict__ seedCnt,
-                                                        uint32_t* __restrict__ rootCnt, Region* __restrict__ regions,
-                                                        size_t regionCap, uint32_t* __restrict__ counters, int W, int H,
-                                                        size_t npx)
-{
-    const int f = blockIdx.y;
-    const uint32_t cnt = seedCnt[f];
-    const uint32_t* sbf = seedBits + (size_t)f * seed_words(npx);
-    for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < cnt; k += gridDim.x * NT) {
-        const uint32_t s = seedList[(size_t)f * npx + k];
-        uint32_t r = s;
-        while (true) {
-            const int y = r / W, x = r - y * W;
-            const int dys[4] = {-1, -1, -1, 0}, dxs[4] = {-1, 0, 1, -1};
-            bool moved = false;
-            for (int j = 0; j < 4 && !moved; ++j) {
-                const int yy = y + dys[j], xx = x + dxs[j];
-                if (yy < 0 || xx < 0 || xx >= W) continue;
-                const int q = yy * W + xx;
-                if (seed_bit(sbf, (uint32_t)q)) { r = (uint32_t)q; moved = true; }
-            }
-            if (!moved) break;
-        }
-        const uint32_t L = r + 1;
-        fin[(size_t)f * npx + s] = L;
-        const uint32_t c = lab[(size_t)f * npx + s];
-        atomicMin(&aux[(size_t)f * npx + c].cmin, L);
-        atomicMax(&aux[(size_t)f * npx + c].cmax, L);
-        if (r == s) {
-            const uint32_t slot = atomicAdd(&rootCnt[f], 1u);
-            aux[(size_t)f * npx + s].rslot = slot;
-            if (slot < regionCap) regions[(size_t)f * regionCap + slot] = Region{s, 0u, 0ull, 0ull, 0ull};
-            else atomicOr(&counters[kStatus], 1u);
-        }
-    }
-}
-
-__global__ void __launch_bounds__(NT) label_assign_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ seedBits, const uint32_t* __restrict__ lab,
-                                                          uint32_t* __restrict__ fin, const PixAux* __restrict__ aux,
-                                                          const uint32_t* __restrict__ maskList, const uint32_t* __restrict__ maskCnt,
-                                                          uint64_t* __restrict__ multiList, size_t multiCap,
-                                                          uint32_t* __restrict__ counters, uint32_t* __restrict__ frameFlag,
-                                                          uint32_t* __restrict__ flagList, int W, int H, size_t npx,
-                                                          int pass2)
-{
-    const int f = blockIdx.y;
-    const uint32_t cnt = maskCnt[f];
-    const uint32_t* Pf = P + (size_t)f * npx;
-    for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < cnt; k += gridDim.x * NT) {
-        const uint32_t p = maskList[(size_t)f * npx + k];
-        const uint32_t c = lab[(size_t)f * npx + p];
+        const uint32_t e = lab[(size_t)f * npx + p], c = e & ~kCompRoot;
         const PixAux a = aux[(size_t)f * npx + c];
         if (a.cmin == kNone) { flag_frame(frameFlag, counters, flagList, f); continue; }  // cannot happen
         if (a.cmin == a.cmax) { fin[(size_t)f * npx + p] = a.cmin; continue; }
-        // Several labels: keep the seeds' labels only (second pass: fin already holds seeds only).
-        if (!pass2 && !seed_bit(seedBits + (size_t)f * seed_words(npx), p)) fin[(size_t)f * npx + p] = 0u;
-        if (p == c) {
+        // Several labels: fin holds the surviving maxima only; list the component once.
+        if (e & kCompRoot) {
             const uint32_t j = atomicAdd(&counters[kMultiCnt], 1u);
             if (j < multiCap) multiList[j] = ((uint64_t)f << 32) | c;
             else flag_frame(frameFlag, counters, flagList, f);
@@ -250,6 +160,295 @@ __global__ void __launch_bounds__(NT) label_assign_kernel(const uint32_t* __rest
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Component labels, getLocalMax labels and the wholesale labelling of single-label components for
+// one frame per workgroup (steps 1-3 of the file header).
+//
+// Component identities: in frames whose mask fits in LDS, the union-find runs over the ranks of the
+// mask pixels (raster-order index among the frame's mask pixels, from a mask bitmap with per-word
+// prefix counts), so a component is identified by the rank of its first pixel; otherwise it runs
+// over pixel indices in global memory and a component is its first pixel. Either way
+// lab[p] = component | (kCompRoot when p is that first pixel) for every mask pixel p, and the
+// multi-label components are listed once, as (frame << 32) | component.
+//
+// getLocalMax labels (Segmentation2D.cpp:309-327): a maximum inherits the label of the first earlier
+// maximum among (y-1,x-1), (y-1,x), (y-1,x+1), (y,x-1); otherwise it opens a new label. Here the
+// label of a maximum is the raster index of the first maximum of its chain, plus one, which orders
+// labels as the reference numbers them. A component's label state is 0 (no maximum yet), its single
+// label, or kNone once two maxima of different labels were seen.
+constexpr int FL_NT = 1024;
+constexpr int FL_PER = 8;                       // mask entries per thread in the LDS path
+constexpr uint32_t FL_MAX_LDS_N = FL_NT * FL_PER;
+
+__device__ inline uint32_t chain_label(const uint32_t* sbf, uint32_t s, int W)
+{
+    uint32_t r = s;
+    while (true) {
+        const int y = (int)(r / (uint32_t)W), x = (int)r - y * W;
+        uint32_t q = kNone;
+        if (y > 0) {
+            const uint32_t up = r - (uint32_t)W;
+            if (x > 0 && seed_bit(sbf, up - 1)) q = up - 1;
+            else if (seed_bit(sbf, up)) q = up;
+            else if (x < W - 1 && seed_bit(sbf, up + 1)) q = up + 1;
+        }
+        if (q == kNone && x > 0 && seed_bit(sbf, r - 1)) q = r - 1;
+        if (q == kNone) return r + 1;
+        r = q;
+    }
+}
+
+// Adds label L to a component's label state (0 -> L -> kNone on a second, different label).
+template <typename Cas, typename Xchg>
+__device__ inline void add_label(uint32_t* st, uint32_t L, Cas cas, Xchg xchg)
+{
+    const uint32_t old = cas(st, 0u, L);
+    if (old != 0u && old != L && old != kNone) xchg(st, kNone);
+}
+
+__global__ void __launch_bounds__(FL_NT) frame_label_kernel(
+    const uint32_t* __restrict__ P, const uint32_t* __restrict__ seedBits, uint32_t* __restrict__ lab,
+    uint32_t* __restrict__ fin, PixAux* __restrict__ aux, const uint32_t* __restrict__ maskList,
+    const uint32_t* __restrict__ maskCnt, const uint32_t* __restrict__ seedList, const uint32_t* __restrict__ seedCnt,
+    uint32_t* __restrict__ rootCnt, Region* __restrict__ regions, size_t regionCap, uint64_t* __restrict__ multiList,
+    size_t multiCap, uint32_t* __restrict__ counters, uint32_t* __restrict__ frameFlag, uint32_t* __restrict__ flagList,
+    int W, int H, size_t npx, uint32_t ldsCap, uint32_t parentOff)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ uint32_t waveSum[FL_NT / 64];
+    __shared__ uint32_t nRoots;
+    const int f = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t n = maskCnt[f], ns = seedCnt[f];
+    const uint32_t* ml = maskList + (size_t)f * npx;
+    const uint32_t* sl = seedList + (size_t)f * npx;
+    const uint32_t* sbf = seedBits + (size_t)f * seed_words(npx);
+    const uint32_t* Pf = P + (size_t)f * npx;
+    uint32_t* lb = lab + (size_t)f * npx;
+    uint32_t* fn = fin + (size_t)f * npx;
+    PixAux* ax = aux + (size_t)f * npx;
+    if (tid == 0) nRoots = 0u;
+
+    // A maximum s with chain label L: its label, and its region when it opens one.
+    auto label_seed = [&](uint32_t s) -> uint32_t {
+        const uint32_t L = chain_label(sbf, s, W);
+        fn[s] = L;
+        if (L - 1u == s) {
+            const uint32_t slot = atomicAdd(&nRoots, 1u);
+            ax[s].rslot = slot;
+            if (slot < regionCap) regions[(size_t)f * regionCap + slot] = Region{s, 0u, 0ull, 0ull, 0ull};
+            else atomicOr(&counters[kStatus], 1u);
+        }
+        return L;
+    };
+    // Final labels of mask pixel p of component c with label state st.
+    auto assign = [&](uint32_t p, uint32_t c, bool first, uint32_t st) {
+        lb[p] = c | (first ? kCompRoot : 0u);
+        if (st == 0u) { flag_frame(frameFlag, counters, flagList, f); return; }   // cannot happen
+        if (st != kNone) { fn[p] = st; return; }
+        if (!seed_bit(sbf, p)) fn[p] = 0u;     // several labels: the flood starts from the maxima
+        if (first) {
+            const uint32_t j = atomicAdd(&counters[kMultiCnt], 1u);
+            if (j < multiCap) multiList[j] = ((uint64_t)f << 32) | c;
+            else flag_frame(frameFlag, counters, flagList, f);
+        }
+    };
+
+    if (ldsCap > 0 && n <= ldsCap) {
+        const uint32_t nw = (uint32_t)((npx + 31) / 32);
+        uint32_t* mbits = reinterpret_cast<uint32_t*>(smem);
+        uint16_t* pre = reinterpret_cast<uint16_t*>(mbits + nw);
+        uint32_t* parent = mbits + parentOff;
+        uint32_t* state = mbits;               // reuses the bitmap once every rank is known
+        auto rank = [&](uint32_t q) { return (uint32_t)pre[q >> 5] + __popc(mbits[q >> 5] & ((1u << (q & 31)) - 1u)); };
+        auto in_mask = [&](uint32_t q) { return (mbits[q >> 5] >> (q & 31)) & 1u; };
+        auto ld = [&](const uint32_t* a) { return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); };
+        auto find = [&](uint32_t x) {
+            uint32_t p = ld(parent + x);
+            while (p != x) { x = p; p = ld(parent + x); }
+            return x;
+        };
+
+        // 1. Mask bitmap and prefix counts.
+        for (uint32_t w = tid; w < nw; w += FL_NT) mbits[w] = 0u;
+        __syncthreads();
+        uint32_t pp[FL_PER], rr[FL_PER];
+#pragma unroll
+        for (int j = 0; j < FL_PER; ++j) {
+            const uint32_t k = tid + j * FL_NT;
+            pp[j] = k < n ? ml[k] : 0u;
+            if (k < n) atomicOr(&mbits[pp[j] >> 5], 1u << (pp[j] & 31));
+        }
+        __syncthreads();
+        {
+            const uint32_t per = (nw + FL_NT - 1) / FL_NT, w0 = min(nw, tid * per), w1 = min(nw, w0 + per);
+            uint32_t s = 0;
+            for (uint32_t w = w0; w < w1; ++w) s += __popc(mbits[w]);
+            uint32_t incl = s;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t t = __shfl_up(incl, o);
+                if (lane >= o) incl += t;
+            }
+            if (lane == 63) waveSum[wave] = incl;
+            __syncthreads();
+            uint32_t base = 0;
+            for (int i = 0; i < wave; ++i) base += waveSum[i];
+            uint32_t run = base + incl - s;
+            for (uint32_t w = w0; w < w1; ++w) {
+                pre[w] = (uint16_t)run;
+                run += __popc(mbits[w]);
+            }
+        }
+        __syncthreads();
+        // 2. Union-find over ranks: every pixel with its earlier 8-neighbours W, NW, N, NE.
+#pragma unroll
+        for (int j = 0; j < FL_PER; ++j) {
+            if (tid + j * FL_NT < n) {
+                rr[j] = rank(pp[j]);
+                parent[rr[j]] = rr[j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < FL_PER; ++j) {
+            if (tid + j * FL_NT >= n) continue;
+            const uint32_t p = pp[j];
+            const int y = (int)(p / (uint32_t)W), x = (int)p - y * W;
+            auto unite = [&](uint32_t q) {
+                uint32_t a = rr[j], b = rank(q);
+                while (true) {
+                    a = find(a);
+                    b = find(b);
+                    if (a == b) return;
+                    if (a > b) { const uint32_t t = a; a = b; b = t; }
+                    const uint32_t old = atomicMin(&parent[b], a);
+                    if (old == b) return;
+                    b = old;
+                }
+            };
+            if (x > 0 && in_mask(p - 1)) unite(p - 1);
+            if (y > 0) {
+                const uint32_t up = p - (uint32_t)W;
+                if (x > 0 && in_mask(up - 1)) unite(up - 1);
+                if (in_mask(up)) unite(up);
+                if (x < W - 1 && in_mask(up + 1)) unite(up + 1);
+            }
+        }
+        __syncthreads();
+        // 3. Flatten: every rank points at its component (the component's smallest rank).
+        uint32_t firstBits = 0;
+#pragma unroll
+        for (int j = 0; j < FL_PER; ++j) {
+            if (tid + j * FL_NT >= n) continue;
+            const uint32_t c = find(rr[j]);
+            if (c == rr[j]) firstBits |= 1u << j;
+            else __hip_atomic_store(&parent[rr[j]], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            rr[j] = c;
+        }
+        __syncthreads();
+        // 4. Maxima: labels, regions, and their components (ranks still readable).
+        uint32_t sc[FL_PER], sL[FL_PER];
+#pragma unroll
+        for (int j = 0; j < FL_PER; ++j) {
+            const uint32_t k = tid + j * FL_NT;
+            sc[j] = kNone;
+            if (k < ns) {
+                const uint32_t s = sl[k];
+                sc[j] = parent[rank(s)];
+                sL[j] = label_seed(s);
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < n; i += FL_NT) state[i] = 0u;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < FL_PER; ++j)
+            if (sc[j] != kNone)
+                add_label(&state[sc[j]], sL[j], [](uint32_t* a, uint32_t c, uint32_t v) { return atomicCAS(a, c, v); },
+                          [](uint32_t* a, uint32_t v) { atomicExch(a, v); });
+        __syncthreads();
+        // 5. Labels of every mask pixel.
+#pragma unroll
+        for (int j = 0; j < FL_PER; ++j)
+            if (tid + j * FL_NT < n) assign(pp[j], rr[j], (firstBits >> j) & 1u, state[rr[j]]);
+    } else {
+        // Same steps in global memory, over pixel indices (label state in aux[component].cmin).
+        for (uint32_t k = tid; k < n; k += FL_NT) lb[ml[k]] = ml[k];
+        __syncthreads();
+        for (uint32_t k = tid; k < n; k += FL_NT) {
+            const uint32_t p = ml[k];
+            const int y = (int)(p / (uint32_t)W), x = (int)p - y * W;
+            const int dys[4] = {0, -1, -1, -1}, dxs[4] = {-1, -1, 0, 1};
+            for (int j = 0; j < 4; ++j) {
+                const int yy = y + dys[j], xx = x + dxs[j];
+                if (yy < 0 || xx < 0 || xx >= W) continue;
+                const uint32_t q = (uint32_t)(yy * W + xx);
+                if (Pf[q] == 0) continue;
+                uint32_t a = p, b = q;
+                while (true) {
+                    a = uf_find(lb, a);
+                    b = uf_find(lb, b);
+                    if (a == b) break;
+                    if (a > b) { const uint32_t t = a; a = b; b = t; }
+                    const uint32_t old = atomicMin(&lb[b], a);
+                    if (old == b) break;
+                    b = old;
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t k = tid; k < n; k += FL_NT) {
+            const uint32_t p = ml[k];
+            const uint32_t c = uf_find(lb, p);
+            if (c == p) ax[p].cmin = 0u;
+            else __hip_atomic_store(&lb[p], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        for (uint32_t k = tid; k < ns; k += FL_NT) {
+            const uint32_t s = sl[k];
+            const uint32_t c = ld_relaxed(&lb[s]);
+            const uint32_t L = label_seed(s);
+            add_label(&ax[c].cmin, L, [](uint32_t* a, uint32_t c, uint32_t v) { return atomicCAS(a, c, v); },
+                      [](uint32_t* a, uint32_t v) { atomicExch(a, v); });
+        }
+        __syncthreads();
+        for (uint32_t k = tid; k < n; k += FL_NT) {
+            const uint32_t p = ml[k];
+            const uint32_t c = ld_relaxed(&lb[p]);
+            assign(p, c, c == p, ld_relaxed(&ax[c].cmin));
+        }
+    }
+    __syncthreads();
+    if (tid == 0) rootCnt[f] = nRoots;
+}
+
+// LDS of frame_label_kernel for frames of npx pixels: the bitmap and prefix counts (later the
+// component label states), then the union-find parents. Sets the largest mask handled in LDS and
+// the offset of the parents in words.
+size_t frame_label_lds(size_t npx, uint32_t& ldsCap, uint32_t& parentOff)
+{
+    const size_t nw = (npx + 31) / 32;
+    const size_t base = nw * 4 + (nw + 1) / 2 * 4;        // bitmap + u16 prefix counts
+    const size_t staticBytes = 256;
+    auto total = [&](size_t cap) { return std::max(base, cap * 4) + cap * 4; };
+    auto fit = [&](size_t budget) {
+        size_t cap = FL_MAX_LDS_N;
+        while (cap > 0 && total(cap) > budget) cap -= 64;
+        return cap;
+    };
+    // Two workgroups per CU when that leaves room for a typical mask, else one.
+    size_t budget = 81920 - staticBytes;
+    size_t cap = fit(budget);
+    if (cap < 2048) {
+        budget = 163840 - staticBytes;
+        cap = fit(budget);
+    }
+    if (cap < 256) { ldsCap = 0; parentOff = 0; return 0; }
+    ldsCap = (uint32_t)cap;
+    parentOff = (uint32_t)(std::max(base, cap * 4) / 4);
+    return total(cap);
+}
 // ---------------------------------------------------------------------------------------------
 // Wave-cooperative priority queue in LDS: an unsorted array of entries (pixel, label, potential,
 // age); the key -P/2^16 + age * 1e-7 is recomputed from (potential, age) exactly as at push time,
@@ -434,7 +633,7 @@ __global__ void __launch_bounds__(64) flood_kernel(const uint32_t* __restrict__ 
                     sp = seedList[(size_t)f * npx + k];
                     sel = true;
                 }
-                sel = sel && (lb[sp] & ~kVisited) == c && fn[sp] != 0u && has_unlabelled_nbr(Pf, fn, W, H, (int)sp);
+                sel = sel && (lb[sp] & ~kCompRoot) == c && fn[sp] != 0u && has_unlabelled_nbr(Pf, fn, W, H, (int)sp);
             }
             const uint64_t m = __ballot(sel);
             const uint32_t cnt = __popcll(m);
@@ -1180,8 +1379,7 @@ __global__ void __launch_bounds__(NT) fusion_kernel(const uint32_t* __restrict__
         if (z[s].alive && z[s].maxKey != 0) fn[kpix(z[s].maxKey)] = rg[s].root + 1;
 }
 
-// Second watershed: clean component roots (the flood marked visited pixels) and reset the
-// per-component label range, then register the surviving seeds.
+// Second watershed: reset the per-component label range, then register the surviving seeds.
 __global__ void __launch_bounds__(NT) pass2_reset_kernel(uint32_t* __restrict__ lab, PixAux* __restrict__ aux,
                                                          const uint32_t* __restrict__ maskList,
                                                          const uint32_t* __restrict__ maskCnt, size_t npx)
@@ -1190,9 +1388,9 @@ __global__ void __launch_bounds__(NT) pass2_reset_kernel(uint32_t* __restrict__ 
     const uint32_t cnt = maskCnt[f];
     for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < cnt; k += gridDim.x * NT) {
         const uint32_t p = maskList[(size_t)f * npx + k];
-        lab[(size_t)f * npx + p] &= ~kVisited;
-        aux[(size_t)f * npx + p].cmin = kNone;
-        aux[(size_t)f * npx + p].cmax = 0u;
+        const uint32_t c = lab[(size_t)f * npx + p] & ~kCompRoot;
+        aux[(size_t)f * npx + c].cmin = kNone;
+        aux[(size_t)f * npx + c].cmax = 0u;
     }
 }
 
@@ -1206,7 +1404,7 @@ __global__ void __launch_bounds__(NT) pass2_seed_kernel(const uint32_t* __restri
         const Fz z = fz[(size_t)f * regionCap + s];
         if (!z.alive || z.maxKey == 0) continue;
         const uint32_t p = kpix(z.maxKey), L = regions[(size_t)f * regionCap + s].root + 1;
-        const uint32_t c = lab[(size_t)f * npx + p];
+        const uint32_t c = lab[(size_t)f * npx + p] & ~kCompRoot;
         atomicMin(&aux[(size_t)f * npx + c].cmin, L);
         atomicMax(&aux[(size_t)f * npx + c].cmax, L);
     }
@@ -1444,14 +1642,16 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
     // Frames that failed the component-flood proof: resolved here (exact frame flood) or left to a
     // later pass that re-runs them with forceSequential (their rows are then skipped here).
     const uint32_t* skip = resolve ? nullptr : b.frameFlag;
-    cc_union_kernel<<<g, NT, 0, s>>>(b.lab, b.P, b.maskList, b.maskCnt, b.aux, W, H, npx);
-    cc_flatten_kernel<<<g, NT, 0, s>>>(b.lab, b.maskList, b.maskCnt, b.aux, npx);
-    seed_label_kernel<<<g, NT, 0, s>>>(b.P, b.seedBits, b.lab, b.fin, b.aux, b.seedList, b.seedCnt, b.rootCnt, b.regions,
-                                       b.regionCap, b.counters, W, H, npx);
+    uint32_t ldsCap = 0, parentOff = 0;
+    const size_t flLds = frame_label_lds(npx, ldsCap, parentOff);
+    frame_label_kernel<<<F, FL_NT, flLds, s>>>(b.P, b.seedBits, b.lab, b.fin, b.aux, b.maskList, b.maskCnt, b.seedList,
+                                               b.seedCnt, b.rootCnt, b.regions, b.regionCap, b.multiList, b.multiCap,
+                                               b.counters, b.frameFlag, b.flagList, W, H, npx, ldsCap, parentOff);
     auto watershed = [&](int pass2) {
         if (forceSequential) force_flag_kernel<<<ceil_div(F, 256), 256, 0, s>>>(b.counters, b.frameFlag, b.flagList, F);
-        label_assign_kernel<<<g, NT, 0, s>>>(b.P, b.seedBits, b.lab, b.fin, b.aux, b.maskList, b.maskCnt, b.multiList, b.multiCap,
-                                             b.counters, b.frameFlag, b.flagList, W, H, npx, pass2);
+        if (pass2)
+            label_assign_kernel<<<g, NT, 0, s>>>(b.lab, b.fin, b.aux, b.maskList, b.maskCnt, b.multiList, b.multiCap,
+                                                 b.counters, b.frameFlag, b.flagList, npx);
         flood_kernel<<<4096, 64, 0, s>>>(b.P, b.lab, b.fin, b.seedList, b.seedCnt, b.maskCnt, b.multiList, b.multiCap,
                                          b.counters, b.frameFlag, b.flagList, W, H, npx, pass2 ? b.fz : nullptr,
                                          b.rootCnt, b.regionCap);

@@ -302,6 +302,28 @@ def test_localization_512_no_fit(pt_gpu, oracle_mod, frames512):
     assert got.shape == exp.shape and np.array_equal(got, exp)
 
 
+@pytest.mark.parametrize("ratio", [0.0, 10.0])
+def test_dense_and_sparse_frames_in_one_chunk(pt_gpu, oracle_mod, frames512, ratio):
+    """Frames whose masks exceed the per-frame LDS capacity (about 8K pixels at 512^2) take the
+    global-memory component labelling; sparse frames of the same launch stay in LDS."""
+    st, thr = frames512
+    rng = np.random.default_rng(7)
+    dense = np.clip(st[1:3].astype(np.int64) + rng.integers(0, 3000, size=st[1:3].shape), 0, 65535).astype(np.uint16)
+    stack = np.ascontiguousarray(np.stack([st[0], dense[0], st[3], dense[1]]))
+    sizes = [int((oracle_mod.atrous(stack[f])[1] > thr).sum()) for f in range(stack.shape[0])]
+    assert sizes[1] > 20000 and sizes[3] > 20000 and sizes[0] < 8000 and sizes[2] < 8000, sizes
+    got = pt_gpu.localization(stack, thr, ratio, 0, FIT_PARAMS_SYN)
+    exp = oracle_mod.localization(stack, thr, ratio, 0, FIT_PARAMS_SYN)
+    assert got.shape == exp.shape and np.array_equal(got, exp)
+    for f in (1, 3):
+        w1 = oracle_mod.atrous(stack[f])[1]
+        lab = pt_gpu.debug_labels(stack[f], thr, ratio)
+        if ratio == 0.0:
+            assert np.array_equal(rank_labels(lab), oracle_mod.watershed_labels(w1, thr)), f
+        else:
+            assert np.array_equal(_filter_small(lab), oracle_mod.segment(w1, thr, ratio)[0]), f
+
+
 def test_partial_last_chunk(pt_gpu, oracle_mod, frames512):
     """A last chunk with fewer frames than the chunk capacity starts from clean per-frame state."""
     st, thr = frames512
