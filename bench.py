@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run one device-to-device copy of this many bytes (calibrates the HBM byte "
+                         "counters of rocprofv3 --pmc against a transfer of known size) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -84,6 +87,14 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_bytes > 0:
+        src = torch.empty(args.copy_bytes, dtype=torch.uint8, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": args.copy_bytes}))
+        return
     import palmtracer_amd as pt
 
     F, S = args.frames, args.size
@@ -126,7 +137,9 @@ def main():
     if rows == 0:
         raise SystemExit("no localizations: the pipeline reported an error (see stderr)")
     if stages["fit_ms"] >= stages["atrous_ms"]:
-        flops = (stages["jacobian_passes"] * JAC_FLOP_7x7 + stages["trial_passes"] * TRIAL_FLOP_7x7) / launches
+        # One fit launch per Localization call (all ROIs of the call in one persistent launch).
+        fit_ms = stages["fit_ms"] / args.steps
+        flops = (stages["jacobian_passes"] * JAC_FLOP_7x7 + stages["trial_passes"] * TRIAL_FLOP_7x7) / args.steps
         peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
         ach = flops / (fit_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "kernel": "fit_kernel (FP32 VALU LM, no MFMA)", "achieved": ach, "peak": peak,
