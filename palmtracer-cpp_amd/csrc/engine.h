@@ -12,8 +12,8 @@
 namespace pt {
 
 struct DevBuf;
-void gather_rois(const Roi* src, uint32_t n, uint32_t z0, Roi* dst, const uint32_t* roiBase, uint32_t frames,
-                 uint32_t off, uint32_t* dBase, hipStream_t s);
+void gather_rois(const Roi* src, uint32_t z0, Roi* dst, const uint32_t* roiBase, uint32_t frames, const uint32_t* run,
+                 uint32_t cap, uint32_t* dBase, uint32_t* counters, hipStream_t s);
 void set_u32(uint32_t* p, uint32_t v, hipStream_t s);
 
 struct DevBuf {
@@ -94,15 +94,16 @@ private:
     hipEvent_t sideEv_[2] = {nullptr, nullptr};  // side batch start / done
     DevBuf atMask_, atPart_, atRows_, passes_, allRois_, allBase_, sideRows_;
     hipEvent_t fitEv_[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t chunkEv_[2] = {nullptr, nullptr};  // a main-lane chunk's counters are on the host
+    DevBuf run_;                                    // ROI running total of the call (device)
     std::vector<hipEvent_t> ev_;
     hipEvent_t event(size_t i);
-    // Detection of F frames on lane L (chunk frame f = input frame map[f], or base + f without a
-    // map). sync: wait and return the ROI count (frames needing the exact frame flood are appended
-    // to deferred as input frame indices, with resolve = false); otherwise everything stays queued
-    // on the lane's stream and 0 is returned.
-    uint32_t detect_chunk(Lane& L, const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio,
-                          uint64_t maxPoints, size_t evBase, const uint32_t* d_map, bool resolve, uint64_t firstFrame,
-                          std::vector<uint32_t>* deferred, bool sync);
+    // Detection and segmentation of F frames queued on lane L's stream (chunk frame f = input frame
+    // map[f], or the chunk's own frame f without a map); per-stage profiling events evBase..+2
+    // unless evBase is kNoEvents.
+    static constexpr size_t kNoEvents = ~size_t(0);
+    void detect_chunk(Lane& L, const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio,
+                      uint64_t maxPoints, size_t evBase, const uint32_t* d_map, bool resolve);
     void grow(DevBuf& buf, size_t need, size_t keep, hipStream_t s);
     FitParams fit_params(uint64_t fit, const double* fp) const;
 };

@@ -14,6 +14,17 @@ __global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v)
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// Zeroes three word ranges in one launch.
+__global__ void zero3_kernel(uint32_t* a, size_t na, uint32_t* b, size_t nb, uint32_t* c, size_t nc)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < na + nb + nc; i += stride) {
+        if (i < na) a[i] = 0u;
+        else if (i < na + nb) b[i - na] = 0u;
+        else c[i - na - nb] = 0u;
+    }
+}
+
 // Fit jobs from caller rows [Id, Plane, X, Y, SegIntensity, Surface] (lib/Palm.cpp:105-109).
 __global__ void positions_to_rois(const double* __restrict__ pos, uint64_t n, uint64_t planes, Roi* __restrict__ rois,
                                   uint32_t* __restrict__ bad)
@@ -35,16 +46,20 @@ __global__ void positions_to_rois(const double* __restrict__ pos, uint64_t n, ui
 }
 
 // ROIs of a chunk appended to the call's ROI store with absolute frame numbers, and the chunk's
-// per-frame ROI offsets shifted to the store.
-__global__ void gather_rois_kernel(const Roi* __restrict__ src, uint32_t n, uint32_t z0, Roi* __restrict__ dst,
-                                   const uint32_t* __restrict__ roiBase, uint32_t frames, uint32_t off,
-                                   uint32_t* __restrict__ dBase)
+// per-frame ROI offsets shifted to the store. run[1] is the chunk's first ROI in the store (set by
+// the chunk's ROI scan); ROIs beyond the store's capacity set status bit 4.
+__global__ void gather_rois_kernel(const Roi* __restrict__ src, uint32_t z0, Roi* __restrict__ dst,
+                                   const uint32_t* __restrict__ roiBase, uint32_t frames, const uint32_t* __restrict__ run,
+                                   uint32_t cap, uint32_t* __restrict__ dBase, uint32_t* __restrict__ counters)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-    for (uint32_t i = t; i < n; i += stride) {
+    const uint32_t off = run[1], n = roiBase[frames];
+    if (t == 0 && (uint64_t)off + n > cap) atomicOr(&counters[kStatus], 4u);
+    const uint32_t m = off >= cap ? 0u : min(n, cap - off);
+    for (uint32_t i = t; i < m; i += stride) {
         Roi r = src[i];
         r.frame += z0;
-        dst[i] = r;
+        dst[off + i] = r;
     }
     for (uint32_t k = t; k < frames; k += stride) dBase[k] = off + roiBase[k];
 }
@@ -67,11 +82,10 @@ __global__ void positions_ids(const double* __restrict__ pos, const uint32_t* __
 }
 } // namespace
 
-void gather_rois(const Roi* src, uint32_t n, uint32_t z0, Roi* dst, const uint32_t* roiBase, uint32_t frames,
-                 uint32_t off, uint32_t* dBase, hipStream_t s)
+void gather_rois(const Roi* src, uint32_t z0, Roi* dst, const uint32_t* roiBase, uint32_t frames, const uint32_t* run,
+                 uint32_t cap, uint32_t* dBase, uint32_t* counters, hipStream_t s)
 {
-    const unsigned g = (unsigned)std::min<size_t>(4096, (std::max<size_t>(n, frames) + 255) / 256 + 1);
-    gather_rois_kernel<<<g, 256, 0, s>>>(src, n, z0, dst, roiBase, frames, off, dBase);
+    gather_rois_kernel<<<1024, 256, 0, s>>>(src, z0, dst, roiBase, frames, run, cap, dBase, counters);
     PT_CHECK(hipGetLastError());
 }
 
@@ -109,6 +123,7 @@ Engine::Engine()
     PT_CHECK(hipStreamCreateWithFlags(&side_.stream, hipStreamNonBlocking));
     stream = main_.stream;
     for (hipEvent_t& e : sideEv_) PT_CHECK(hipEventCreate(&e));
+    for (hipEvent_t& e : chunkEv_) PT_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (hipEvent_t& e : fitEv_) PT_CHECK(hipEventCreate(&e));
     if (const char* v = std::getenv("PALMTRACER_FIT_PRECISION")) opt.fitFp64 = std::strcmp(v, "fp32") != 0;
     if (const char* v = std::getenv("PALMTRACER_MAX_ITER")) opt.maxIter = std::max(1, std::atoi(v));
@@ -178,9 +193,8 @@ void Lane::zero()
     // Per-frame counters (6 arrays laid out with a stride of frames + 1 words, frames being the
     // chunk capacity, whatever the number of frames in this chunk) and the chunk counters; a fill
     // kernel rather than a memset so that the sequence can be captured in a graph.
-    fill_u32_kernel<<<64, 256, 0, stream>>>(b.maskCnt, 6 * (b.frames + 1), 0u);
-    fill_u32_kernel<<<1, 64, 0, stream>>>(b.counters, kNumCounters, 0u);
-    fill_u32_kernel<<<1024, 256, 0, stream>>>(b.seedBits, b.frames * seed_words(b.npx), 0u);
+    zero3_kernel<<<1024, 256, 0, stream>>>(b.maskCnt, 6 * (b.frames + 1), b.counters, kNumCounters, b.seedBits,
+                                           b.frames * seed_words(b.npx));
     PT_CHECK(hipGetLastError());
 }
 
@@ -224,14 +238,13 @@ hipEvent_t Engine::event(size_t i)
     return ev_[i];
 }
 
-uint32_t Engine::detect_chunk(Lane& L, const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio,
-                              uint64_t maxPoints, size_t evBase, const uint32_t* d_map, bool resolve,
-                              uint64_t firstFrame, std::vector<uint32_t>* deferred, bool sync)
+void Engine::detect_chunk(Lane& L, const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio,
+                          uint64_t maxPoints, size_t evBase, const uint32_t* d_map, bool resolve)
 {
     const size_t npx = (size_t)W * H;
     const hipStream_t st = L.stream;
     L.zero();
-    const bool prof = opt.profile && sync;
+    const bool prof = opt.profile && evBase != kNoEvents;
     if (prof) PT_CHECK(hipEventRecord(event(evBase + 0), st));
     launch_atrous_detect(d_frames, L.b, W, H, (int)F, threshold_to_pmin(thr), st, d_map);
     if (prof) PT_CHECK(hipEventRecord(event(evBase + 1), st));
@@ -240,24 +253,6 @@ uint32_t Engine::detect_chunk(Lane& L, const uint16_t* d_frames, size_t F, int H
     const bool forceSeq = opt.forceSequential || d_map != nullptr;
     launch_segment(L.b, W, H, (int)F, maxPoints, ratio, L.rois.as<Roi>(), st, forceSeq, opt.exactPath, resolve);
     if (prof) PT_CHECK(hipEventRecord(event(evBase + 2), st));
-    if (!sync) return 0;
-    uint32_t ctr[kNumCounters];
-    uint32_t total = 0;
-    PT_CHECK(hipMemcpyAsync(&total, L.b.roiBase + F, 4, hipMemcpyDeviceToHost, st));
-    PT_CHECK(hipMemcpyAsync(ctr, L.b.counters, sizeof(ctr), hipMemcpyDeviceToHost, st));
-    PT_CHECK(hipStreamSynchronize(st));
-    last.flaggedFrames += ctr[kFlagCnt];
-    last.multiComponents += ctr[kMultiCnt];
-    last.flagOverflow += ctr[kFlagOverflow];
-    last.flagUnproven += ctr[kFlagUnproven];
-    last.status |= ctr[kStatus];
-    if (ctr[kStatus]) throw Error("segmentation status " + std::to_string(ctr[kStatus]));
-    if (!resolve && deferred && ctr[kFlagCnt]) {
-        std::vector<uint32_t> fl(ctr[kFlagCnt]);
-        PT_CHECK(hipMemcpy(fl.data(), L.b.flagList, fl.size() * 4, hipMemcpyDeviceToHost));
-        for (uint32_t f : fl) deferred->push_back((uint32_t)(firstFrame + f));
-    }
-    return total;
 }
 
 // Grows buf to hold at least `need` bytes, keeping its first `keep` bytes (copied on stream s).
@@ -323,7 +318,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         if (opt.profile) PT_CHECK(hipEventRecord(sideStart, side_.stream));
         PT_CHECK(hipMemcpyAsync(side_.frameMap.p, h, batchCount * 4, hipMemcpyHostToDevice, side_.stream));
         const uint32_t* dmap = side_.frameMap.as<uint32_t>();
-        detect_chunk(side_, d_in, batchCount, H, W, thr, ratio, maxPerPlane, 0, dmap, true, 0, nullptr, false);
+        detect_chunk(side_, d_in, batchCount, H, W, thr, ratio, maxPerPlane, kNoEvents, dmap, true);
         launch_fit(d_in, npx, W, H, side_.rois.as<Roi>(), (uint32_t)std::min<uint64_t>(sideRowCap, 0xFFFFFFFFull), fp,
                    side_.rows.as<double>(), opt.fitFp64, side_.fitNext.as<unsigned int>(), side_.stream, passes, dmap,
                    opt.fitWaves, side_.b.roiBase + batchCount);
@@ -384,20 +379,53 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         }
     };
 
-    for (uint64_t z0 = 0; z0 < planes; z0 += F) {
+    // Chunks are queued back to back; the host reads a chunk's counters and deferred frames (copied
+    // to pinned memory behind the chunk) once the next chunk is queued, so the main stream does not
+    // wait for the host between chunks. ROI offsets in the store are carried on the device.
+    const size_t metaWords = kNumCounters + 1 + F;      // counters, ROIs of the call so far, flag list
+    uint32_t* meta = main_.host(2 * metaWords);
+    size_t chunkNo = 0;
+    std::vector<uint64_t> chunkZ0;
+    auto read_meta = [&](size_t c) {
+        PT_CHECK(hipEventSynchronize(chunkEv_[c & 1]));
+        const uint32_t* m = meta + (c & 1) * metaWords;
+        const uint32_t* ctr = m;
+        last.flaggedFrames += ctr[kFlagCnt];
+        last.multiComponents += ctr[kMultiCnt];
+        last.flagOverflow += ctr[kFlagOverflow];
+        last.flagUnproven += ctr[kFlagUnproven];
+        last.status |= ctr[kStatus];
+        if (ctr[kStatus] & 4u) throw Error("output capacity exceeded");
+        if (ctr[kStatus]) throw Error("segmentation status " + std::to_string(ctr[kStatus]));
+        totalRois = m[kNumCounters];
+        const uint32_t* fl = m + kNumCounters + 1;
+        for (uint32_t k = 0; k < std::min<uint32_t>(ctr[kFlagCnt], (uint32_t)F); ++k)
+            deferred.push_back((uint32_t)(chunkZ0[c] + fl[k]));
+    };
+    run_.ensure(16);
+    uint32_t* dRun = run_.as<uint32_t>();
+    fill_u32_kernel<<<1, 64, 0, main_.stream>>>(dRun, 2, 0u);
+    main_.b.runTotal = dRun;
+    const uint64_t roiCap = std::min<uint64_t>(std::min<uint64_t>(outCapRows, planes * std::min<uint64_t>(maxPerPlane, npx / 4 + 2)),
+                                               0xFFFFFFF0ull);
+    allRois_.ensure(std::max<uint64_t>(roiCap, 1) * sizeof(Roi));
+    for (uint64_t z0 = 0; z0 < planes; z0 += F, ++chunkNo) {
         const size_t f = std::min<uint64_t>(F, planes - z0);
-        const uint16_t* frames = d_in + z0 * npx;
-        const size_t eb = 3 * last.chunks;
-        const uint32_t n = detect_chunk(main_, frames, f, H, W, thr, ratio, maxPerPlane, eb, nullptr, false, z0,
-                                        &deferred, true);
-        if (totalRois + n > 0xFFFFFFF0ull) throw Error("too many ROIs in one call");
-        grow(allRois_, (totalRois + n) * sizeof(Roi), totalRois * sizeof(Roi), main_.stream);
-        gather_rois(main_.rois.as<Roi>(), n, (uint32_t)z0, allRois_.as<Roi>() + totalRois, main_.b.roiBase, (uint32_t)f,
-                    (uint32_t)totalRois, dBase + z0, main_.stream);
-        totalRois += n;
+        chunkZ0.push_back(z0);
+        detect_chunk(main_, d_in + z0 * npx, f, H, W, thr, ratio, maxPerPlane, 3 * chunkNo, nullptr, false);
+        gather_rois(main_.rois.as<Roi>(), (uint32_t)z0, allRois_.as<Roi>(), main_.b.roiBase, (uint32_t)f, dRun,
+                    (uint32_t)roiCap, dBase + z0, main_.b.counters, main_.stream);
+        uint32_t* m = meta + (chunkNo & 1) * metaWords;
+        PT_CHECK(hipMemcpyAsync(m, main_.b.counters, kNumCounters * 4, hipMemcpyDeviceToHost, main_.stream));
+        PT_CHECK(hipMemcpyAsync(m + kNumCounters, dRun, 4, hipMemcpyDeviceToHost, main_.stream));
+        PT_CHECK(hipMemcpyAsync(m + kNumCounters + 1, main_.b.flagList, f * 4, hipMemcpyDeviceToHost, main_.stream));
+        PT_CHECK(hipEventRecord(chunkEv_[chunkNo & 1], main_.stream));
+        if (chunkNo > 0) read_meta(chunkNo - 1);
         side_poll(false);
         last.chunks++;
     }
+    read_meta(chunkNo - 1);
+    main_.b.runTotal = nullptr;
     set_u32(dBase + planes, (uint32_t)totalRois, main_.stream);
     // Fit of every ROI, then each plane's rows in (Y, X) order.
     main_.rows.ensure(std::max<uint64_t>(totalRois, 1) * kCols * 8);

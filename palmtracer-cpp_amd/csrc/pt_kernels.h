@@ -69,6 +69,7 @@ struct DetectBuffers {
     uint32_t* waveList = nullptr; // [F] frames the rank flood could not settle (wave-queue flood)
     uint32_t* rfScratch = nullptr;// rank-flood work space: rfSlots * rf_slot_words(npx) u32
     uint32_t* seedBits = nullptr; // per frame: bitmap of the local maxima, seed_words(npx) u32
+    uint32_t* runTotal = nullptr; // optional: [0] ROIs of the call so far, [1] first ROI of this chunk
     size_t frames = 0, npx = 0, regionCap = 0, multiCap = 0, fbSlots = 0, rfSlots = 0;
 };
 

@@ -1605,7 +1605,10 @@ __global__ void __launch_bounds__(NT) roi_emit_kernel(const uint32_t* __restrict
     }
 }
 
-__global__ void __launch_bounds__(1024) exclusive_scan_kernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int n)
+// Exclusive scan of n counts into out[0..n]; with run set, also run[1] = run[0] (the first ROI of
+// this chunk in the call) and run[0] += total.
+__global__ void __launch_bounds__(1024) exclusive_scan_kernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int n,
+                                                              uint32_t* __restrict__ run)
 {
     // Single workgroup: each thread scans a contiguous segment, then the segment totals.
     __shared__ uint32_t seg[1024];
@@ -1619,6 +1622,10 @@ __global__ void __launch_bounds__(1024) exclusive_scan_kernel(const uint32_t* __
         uint32_t acc = 0;
         for (int i = 0; i < 1024; ++i) { const uint32_t t = seg[i]; seg[i] = acc; acc += t; }
         out[n] = acc;
+        if (run) {
+            run[1] = run[0];
+            run[0] += acc;
+        }
     }
     __syncthreads();
     uint32_t acc = seg[threadIdx.x];
@@ -1629,7 +1636,7 @@ __global__ void __launch_bounds__(1024) exclusive_scan_kernel(const uint32_t* __
 
 void launch_exclusive_scan(const uint32_t* d_in, uint32_t* d_out, int n, hipStream_t s)
 {
-    exclusive_scan_kernel<<<1, 1024, 0, s>>>(d_in, d_out, n);
+    exclusive_scan_kernel<<<1, 1024, 0, s>>>(d_in, d_out, n, nullptr);
     PT_CHECK(hipGetLastError());
 }
 
@@ -1686,7 +1693,7 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
     region_stats_kernel<<<F, RS_NT, 0, s>>>(b.P, b.fin, b.aux, b.maskList, b.maskCnt, b.rootCnt, b.regions, b.regionCap,
                                             b.counters, W, npx, skip);
     region_count_kernel<<<F, NT, 0, s>>>(b.regions, b.regionCap, b.rootCnt, b.found, b.keep, maxPoints, skip);
-    exclusive_scan_kernel<<<1, 1024, 0, s>>>(b.keep, b.roiBase, F);
+    exclusive_scan_kernel<<<1, 1024, 0, s>>>(b.keep, b.roiBase, F, b.runTotal);
     roi_emit_kernel<<<F, NT, 0, s>>>(b.P, b.fin, b.regions, b.regionCap, b.rootCnt, b.found, b.keep, b.roiBase, d_rois,
                                      W, npx);
     PT_CHECK(hipGetLastError());
