@@ -61,6 +61,13 @@ PT_API void PT_CALL Wavelett(const uint16_t* input, double* output, uint64_t hei
  * Returns NaN on failure. */
 PT_API double PT_CALL AutoThreshold(const uint16_t* input, uint64_t height, uint64_t width, const double* fit_params);
 
+/* Astigmatism3DEstimation (CPU/main.cpp:324-353, lib/fit/AstigFit.cpp:36-72): for each of the size
+ * (sigmaX, sigmaY) pairs in pixels, the z (nm) of the model's closest sigmas on [-zMax, zMax]
+ * (201-point grid, then Brent's method) and the axial score Q_Z; output holds 2 * size doubles.
+ * model = 2 axes x [z0, w, c3, c4, A] as Astigmatism3DCalibration writes it. Runs on the GPU. */
+PT_API void PT_CALL Astigmatism3DEstimation(const double* input, double* output, uint64_t size, double pixelSize,
+                                            const double* model, double zMax);
+
 /* ---- reference host-side entry points (forwarded) ----------------------------------------- */
 /* Exported with the reference's signatures so that this library replaces the reference library
  * as a whole. They are not part of the detection + fitting path and run on the host: each call is
@@ -81,11 +88,13 @@ PT_API bool PT_CALL TrackAnalysis(const double* input, double* outputMSD, double
                                   const double* fitParams);
 /* Astigmatism3DCalibration (CPU/main.cpp:293): 10 model parameters from (sigmaX, sigmaY, z) points. */
 PT_API void PT_CALL Astigmatism3DCalibration(const double* input, double* output, uint64_t size, double pixelSize, bool center);
-/* Astigmatism3DEstimation (CPU/main.cpp:324-325): (z nm, Q_Z) per (sigmaX, sigmaY) pair. */
-PT_API void PT_CALL Astigmatism3DEstimation(const double* input, double* output, uint64_t size, double pixelSize,
-                                            const double* model, double zMax);
 
 /* ---- no counterpart in the reference ------------------------------------------------------ */
+
+/* Astigmatism3DEstimation over device memory (d_input: 2 * size doubles, d_output: 2 * size doubles);
+ * model (10 doubles) in host memory. Returns 1 on success, 0 on failure. */
+PT_API int PT_CALL pt_astig_estimate_device(const double* d_input, double* d_output, uint64_t size, double pixelSize,
+                                            const double* model, double zMax);
 
 /* AutoThreshold of one frame in device memory. */
 PT_API double PT_CALL pt_autothreshold_device(const uint16_t* d_input, uint64_t height, uint64_t width,

@@ -22,6 +22,9 @@
  *   or_autothreshold     the 10 known answers of Tests_CPU/test_Palm.cpp:307-310.
  *   or_localization / or_fitting / or_wavelett   golden CSVs + FitLocalizations == Localization
  *                        (Tests_CPU/test_Palm.cpp:269-295) + per-plane capacity semantics.
+ *   or_astig_estimate    the axial cases of Tests_CPU/test_fit_AstigFit.cpp:41-54, 241-345 (bounds,
+ *                        between-grid refinement, score formula and scale invariance, invalid widths)
+ *                        + bit equality with the compiled reference on random and degenerate pairs.
  *
  * Floating-point: built with -ffp-contract=off so that every product and sum rounds as in
  * the reference's x86-64 build (which has no FMA at its default target).
@@ -933,4 +936,133 @@ double or_autothreshold(const uint16_t* input, uint64_t height, uint64_t width, 
     free(w1);
     free(rows);
     return thr;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Astigmatic z estimation (lib/fit/AstigFit.cpp:36-72 Estimate, :74-108 model / cost / score,   */
+/* :340-409 brent; CPU/main.cpp:324-353 Astigmatism3DEstimation).                               */
+/* ------------------------------------------------------------------------------------------ */
+
+#define OR_AST_ZSTEP 201          /* AstigFit.hpp:83 default zStep */
+#define OR_AST_BRENT_ITER 80      /* AstigFit.hpp:110 */
+#define OR_AST_BRENT_TOL 1e-3     /* AstigFit.hpp:111 */
+#define OR_AST_Z_TOL 1e-9         /* AstigFit.hpp:112 */
+#define OR_AST_QUALITY_SCALE 0.2  /* AstigFit.hpp:114 */
+
+/* std::min / std::max argument order (ties and NaN keep the first argument). */
+static inline double std_min(double a, double b) { return b < a ? b : a; }
+static inline double std_max(double a, double b) { return a < b ? b : a; }
+
+/* getSigma (AstigFit.cpp:76-88): sigma in pixels of one axis at z; sampling is 1. */
+static double ast_sigma(const double* c, double z, double pixelSize)
+{
+    const double u = (z - c[0]) / c[1];
+    const double u2 = u * u, u3 = u2 * u, u4 = u3 * u;
+    const double poly = 1.0 + u2 + (c[2] * u3) + (c[3] * u4);
+    const double sigma = c[4] * sqrt(std_max(0.0, poly));
+    return 1.0 / pixelSize * sigma;
+}
+
+/* getError (AstigFit.cpp:91-96): squared distance of the model sigmas at z to the observed pair. */
+static double ast_error(const double* m, double z, double sx, double sy, double pixelSize)
+{
+    const double dx = ast_sigma(m, z, pixelSize) - sx, dy = ast_sigma(m + 5, z, pixelSize) - sy;
+    return (dx * dx) + (dy * dy);
+}
+
+/* getQuality (AstigFit.cpp:99-108). */
+static double ast_quality(const double* m, double z, double sx, double sy, double pixelSize)
+{
+    const double px = ast_sigma(m, z, pixelSize), py = ast_sigma(m + 5, z, pixelSize);
+    if (!isfinite(px) || !isfinite(py) || px <= 0 || py <= 0 || !isfinite(sx) || !isfinite(sy) || sx <= 0 || sy <= 0) return 0;
+    const double rx = (sx - px) / px / OR_AST_QUALITY_SCALE, ry = (sy - py) / py / OR_AST_QUALITY_SCALE;
+    return 1.0 / (1.0 + (0.5 * ((rx * rx) + (ry * ry))));
+}
+
+/* brent (AstigFit.cpp:340-409): Brent's minimisation on [a, b]; returns x, writes f(x). */
+static double ast_brent(double a, double b, const double* m, double sx, double sy, double pixelSize, double* fxOut)
+{
+    if (b < a) { const double t = a; a = b; b = t; }
+    const double invPhi2 = 1.0 - 0.6180339887498948482;
+    double x = a + (invPhi2 * (b - a));
+    double w = x, v = x;
+    double fx = ast_error(m, x, sx, sy, pixelSize);
+    double fw = fx, fv = fx;
+    double d = 0.0, e = 0.0;
+    for (int iter = 0; iter < OR_AST_BRENT_ITER; ++iter) {
+        const double mid = 0.5 * (a + b);
+        const double tol1 = (OR_AST_BRENT_TOL * fabs(x)) + 1e-12, tol2 = 2.0 * tol1;
+        if (fabs(x - mid) <= tol2 - (0.5 * (b - a))) break;
+        int parabolic = 0;
+        if (fabs(e) > tol1) {
+            const double r = (x - w) * (fx - fv);
+            double q = (x - v) * (fx - fw);
+            double p = ((x - v) * q) - ((x - w) * r);
+            q = 2.0 * (q - r);
+            if (q > 0.0) p = -p;
+            q = fabs(q);
+            const double etmp = e;
+            e = d;
+            if (fabs(p) < fabs(0.5 * q * etmp) && p > q * (a - x) && p < q * (b - x)) {
+                parabolic = 1;
+                d = p / q;
+                const double u = x + d;
+                if (u - a < tol2 || b - u < tol2) d = x < mid ? tol1 : -tol1;
+            }
+        }
+        if (!parabolic) {
+            e = x < mid ? b - x : a - x;
+            d = invPhi2 * e;
+        }
+        const double minStep = d > 0.0 ? tol1 : -tol1;
+        const double u = x + (fabs(d) >= tol1 ? d : minStep);
+        const double fu = ast_error(m, u, sx, sy, pixelSize);
+        if (fu <= fx) {
+            if (u < x) b = x;
+            else a = x;
+            v = w; fv = fw;
+            w = x; fw = fx;
+            x = u; fx = fu;
+        } else {
+            if (u < x) a = u;
+            else b = u;
+            if (fu <= fw || fabs(x - w) <= OR_AST_Z_TOL) {
+                v = w; fv = fw;
+                w = u; fw = fu;
+            } else if (fu <= fv || fabs(v - x) <= OR_AST_Z_TOL || fabs(v - w) <= OR_AST_Z_TOL) {
+                v = u; fv = fu;
+            }
+        }
+    }
+    *fxOut = fx;
+    return x;
+}
+
+void or_astig_estimate(const double* input, double* output, uint64_t size, double pixelSize, const double* model,
+                       double zMax)
+{
+    const double z0 = -zMax, z1 = zMax, deltaZ = z1 - z0;
+    for (uint64_t n = 0; n < size; ++n) {
+        const double sx = input[2 * n], sy = input[2 * n + 1];
+        double bestZ = z0, bestE = INFINITY;
+        for (int i = 0; i < OR_AST_ZSTEP; ++i) {
+            const double t = (double)i / (double)(OR_AST_ZSTEP - 1);
+            const double z = z0 + (t * deltaZ);
+            const double e = ast_error(model, z, sx, sy, pixelSize);
+            if (e < bestE) { bestE = e; bestZ = z; }
+        }
+        const double dz = deltaZ / (double)(OR_AST_ZSTEP - 1);
+        const double a = std_max(z0, std_min(bestZ - (2.0 * dz), z1));
+        const double b = std_max(z0, std_min(bestZ + (2.0 * dz), z1));
+        double fr = 0.0;
+        const double zr = ast_brent(a, b, model, sx, sy, pixelSize, &fr);
+        const double z = fr < bestE ? zr : bestZ;
+        output[2 * n] = z;
+        output[2 * n + 1] = ast_quality(model, z, sx, sy, pixelSize);
+    }
+}
+
+double or_astig_quality(const double* model, double z, double sx, double sy, double pixelSize)
+{
+    return ast_quality(model, z, sx, sy, pixelSize);
 }

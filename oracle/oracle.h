@@ -57,6 +57,14 @@ void or_wavelett(const uint16_t* input, double* output, uint64_t height, uint64_
                  uint64_t levelOutput);
 double or_autothreshold(const uint16_t* input, uint64_t height, uint64_t width, const double* fitParams);
 
+/* AstigFit::Estimate (lib/fit/AstigFit.cpp:36-72) over size (sigma_x, sigma_y) pairs in pixels:
+ * output (z in nm, axial score) pairs; model = 2 x [z0, w, c3, c4, A]; search on [-zMax, zMax]. */
+void or_astig_estimate(const double* input, double* output, uint64_t size, double pixelSize, const double* model,
+                       double zMax);
+
+/* AstigFit::getQuality (lib/fit/AstigFit.cpp:99-108) of one observed pair at z. */
+double or_astig_quality(const double* model, double z, double sx, double sy, double pixelSize);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,9 @@ def lib():
         L.or_wavelett.argtypes = [_vp, _vp, _u64, _u64, _u64, _u64]
         L.or_autothreshold.restype = _f64
         L.or_autothreshold.argtypes = [_vp, _u64, _u64, _vp]
+        L.or_astig_estimate.argtypes = [_vp, _vp, _u64, _f64, _vp, _f64]
+        L.or_astig_quality.restype = _f64
+        L.or_astig_quality.argtypes = [_vp, _f64, _f64, _f64, _f64]
         _or = L
     return _or
 
@@ -63,6 +66,7 @@ def ref_lib():
         L.ref_segment.restype = _u64
         L.ref_segment.argtypes = [_vp, _u64, _u64, _f64, _f64, _vp, _vp, _u64]
         L.ref_atrous.argtypes = [_vp, _u64, _u64, _vp, _vp, _vp]
+        L.ref_astig_estimate.argtypes = [_vp, _vp, _u64, _f64, _vp, _f64]
         _ref = L
     return _ref
 
@@ -163,3 +167,31 @@ def fit_region(mode, frame, fit_params, x, y, seg=0.0, surf=0.0, max_iter=75):
     p = _params(fit_params)
     lib().or_fit_region(int(mode), f.ctypes.data, w, h, p.ctypes.data, int(max_iter), info.ctypes.data, row.ctypes.data)
     return row
+
+
+def astig_estimate(sigmas, pixel_size, model, z_max):
+    """AstigFit::Estimate restated: (n, 2) sigma pairs in pixels -> (n, 2) [z nm, axial score]."""
+    s = np.ascontiguousarray(sigmas, np.float64).reshape(-1, 2)
+    m = np.ascontiguousarray(model, np.float64).ravel()
+    assert m.size == 10
+    out = np.empty_like(s)
+    lib().or_astig_estimate(s.ctypes.data, out.ctypes.data, s.shape[0], float(pixel_size), m.ctypes.data, float(z_max))
+    return out
+
+
+def astig_quality(model, z, sx, sy, pixel_size):
+    """AstigFit::getQuality restated."""
+    m = np.ascontiguousarray(model, np.float64).ravel()
+    return lib().or_astig_quality(m.ctypes.data, float(z), float(sx), float(sy), float(pixel_size))
+
+
+def ref_astig_estimate(sigmas, pixel_size, model, z_max):
+    """The compiled reference's AstigFit::Estimate (None without oracle/_ref)."""
+    R = ref_lib()
+    if R is None:
+        return None
+    s = np.ascontiguousarray(sigmas, np.float64).reshape(-1, 2)
+    m = np.ascontiguousarray(model, np.float64).ravel()
+    out = np.empty_like(s)
+    R.ref_astig_estimate(s.ctypes.data, out.ctypes.data, s.shape[0], float(pixel_size), m.ctypes.data, float(z_max))
+    return out

@@ -118,6 +118,35 @@ PT_API double PT_CALL AutoThreshold(const uint16_t* input, uint64_t height, uint
     }, std::nan(""));
 }
 
+PT_API void PT_CALL Astigmatism3DEstimation(const double* input, double* output, uint64_t size, double pixelSize,
+                                            const double* model, double zMax)
+{
+    guarded("Astigmatism3DEstimation", [&]() -> int {
+        if (size == 0) return 0;
+        Engine& e = Engine::get();
+        std::lock_guard<std::mutex> lk(e.mu);
+        e.in.ensure(size * 16);
+        e.out.ensure(size * 16);
+        PT_CHECK(hipMemcpyAsync(e.in.p, input, size * 16, hipMemcpyHostToDevice, e.stream));
+        pt::launch_astig_estimate(e.in.as<double>(), e.out.as<double>(), size, pixelSize, model, zMax, e.stream);
+        PT_CHECK(hipMemcpyAsync(output, e.out.p, size * 16, hipMemcpyDeviceToHost, e.stream));
+        PT_CHECK(hipStreamSynchronize(e.stream));
+        return 0;
+    }, 0);
+}
+
+PT_API int PT_CALL pt_astig_estimate_device(const double* d_input, double* d_output, uint64_t size, double pixelSize,
+                                            const double* model, double zMax)
+{
+    return guarded("pt_astig_estimate_device", [&]() -> int {
+        Engine& e = Engine::get();
+        std::lock_guard<std::mutex> lk(e.mu);
+        pt::launch_astig_estimate(d_input, d_output, size, pixelSize, model, zMax, e.stream);
+        PT_CHECK(hipStreamSynchronize(e.stream));
+        return 1;
+    }, 0);
+}
+
 PT_API double PT_CALL pt_autothreshold_device(const uint16_t* d_input, uint64_t height, uint64_t width,
                                               const double* fit_params)
 {
@@ -185,6 +214,7 @@ PT_API int PT_CALL pt_set_option(const char* name, double value)
         else if (n == "force_heap_watershed") e.opt.exactPath = value != 0.0 ? 2 : 0;
         else if (n == "force_wave_flood") e.opt.exactPath = value != 0.0 ? 1 : 0;
         else if (n == "fit_waves") e.opt.fitWaves = (unsigned)std::max(0.0, std::min(1048576.0, value));
+        else if (n == "fit_refill") e.opt.fitRefill = (int)std::max(1.0, std::min(64.0, value));
         else throw pt::Error("unknown option '" + n + "'");
         return 1;
     }, 0);

@@ -1,6 +1,6 @@
 // Host-side exports of the reference library that stay on the CPU: drift alignment, tracking,
-// blinking reconnection, track analysis and the astigmatism calibration / z estimation
-// (CPU/main.cpp:47-48, :159, :185-186, :231-233, :293, :324-325). They are not part of the
+// blinking reconnection, track analysis and the astigmatism calibration
+// (CPU/main.cpp:47-48, :159, :185-186, :231-233, :293). They are not part of the
 // detection + fitting path; this library exports them with the reference's signatures so that it
 // can replace the reference library as a whole, and forwards each call to the reference CPU library
 // named by the environment variable PALMTRACER_HOST_LIBRARY (loaded once, on first use). Without
@@ -76,10 +76,4 @@ PT_API bool PT_CALL TrackAnalysis(const double* input, double* outputMSD, double
 PT_API void PT_CALL Astigmatism3DCalibration(const double* input, double* output, uint64_t size, double pixelSize, bool center)
 {
     if (auto f = host_symbol("Astigmatism3DCalibration", &Astigmatism3DCalibration)) f(input, output, size, pixelSize, center);
-}
-
-PT_API void PT_CALL Astigmatism3DEstimation(const double* input, double* output, uint64_t size, double pixelSize,
-                                            const double* model, double zMax)
-{
-    if (auto f = host_symbol("Astigmatism3DEstimation", &Astigmatism3DEstimation)) f(input, output, size, pixelSize, model, zMax);
 }

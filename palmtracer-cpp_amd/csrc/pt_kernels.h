@@ -121,6 +121,7 @@ struct FitParams {
     int roi;         // odd ROI side
     int maxIter;     // Fitting::N_ITER (75 in the reference)
     double sigmaX, sigmaY, theta;
+    int refill = 16; // FP32 fit: free lanes of a wave that trigger a refill round while others work
 };
 // LM fit of nRoi ROIs into d_rows (18 columns each). fp64: the reference's arithmetic, one ROI per
 // lane; otherwise the FP32 persistent work-queue kernel, whose queue head is d_next (one word).
@@ -143,5 +144,9 @@ void launch_scatter(const double* d_staged, const uint64_t* d_src, const uint64_
                     uint64_t planes, double* d_out, hipStream_t s);
 
 void launch_exclusive_scan(const uint32_t* d_in, uint32_t* d_out, int n, hipStream_t s);
+// AstigFit::Estimate of n (sigma_x, sigma_y) pairs (d_in, 2n doubles) into (z, score) pairs (d_out);
+// model: 10 host doubles.
+void launch_astig_estimate(const double* d_in, double* d_out, uint64_t n, double pixelSize, const double* model,
+                           double zMax, hipStream_t s);
 
 } // namespace pt

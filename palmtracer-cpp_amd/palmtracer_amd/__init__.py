@@ -58,6 +58,10 @@ def lib():
         L.pt_synth_device.argtypes = [_vp, _u64, _i32, _i32, _i32, _i32, _vp]
         L.pt_synth_emitters_device.restype = _i32
         L.pt_synth_emitters_device.argtypes = [_vp, _u64, _i32, _i32, _i32, _i32, _vp]
+        L.Astigmatism3DEstimation.restype = None
+        L.Astigmatism3DEstimation.argtypes = [_vp, _vp, _u64, _f64, _vp, _f64]
+        L.pt_astig_estimate_device.restype = _i32
+        L.pt_astig_estimate_device.argtypes = [_vp, _vp, _u64, _f64, _vp, _f64]
         _lib = L
     return _lib
 
@@ -198,3 +202,27 @@ def localization_device(d_stack, d_out, threshold, watershed_ratio=0.0, fit=2, f
     n = lib().pt_localization_device(d_stack.data_ptr(), d_out.data_ptr(), cap, h, w, z, float(threshold),
                                      float(watershed_ratio), int(fit), p.ctypes.data, mpp)
     return n // NCOL
+
+
+def astig_estimate(sigmas, pixel_size, model, z_max):
+    """Astigmatism3DEstimation: (n, 2) observed (sigma_x, sigma_y) in pixels -> (n, 2) [z nm, Q_Z]."""
+    s = np.ascontiguousarray(sigmas, np.float64).reshape(-1, 2)
+    m = np.ascontiguousarray(model, np.float64).ravel()
+    if m.size != 10:
+        raise ValueError("the astigmatic model has 10 parameters (2 axes x [z0, w, c3, c4, A])")
+    out = np.zeros_like(s)
+    lib().Astigmatism3DEstimation(s.ctypes.data, out.ctypes.data, s.shape[0], float(pixel_size), m.ctypes.data,
+                                  float(z_max))
+    return out
+
+
+def astig_estimate_device(d_sigmas, d_out, pixel_size, model, z_max):
+    """Astigmatism3DEstimation over HBM-resident float64 torch tensors of shape (n, 2)."""
+    m = np.ascontiguousarray(model, np.float64).ravel()
+    if m.size != 10:
+        raise ValueError("the astigmatic model has 10 parameters (2 axes x [z0, w, c3, c4, A])")
+    n = d_sigmas.numel() // 2
+    if not lib().pt_astig_estimate_device(d_sigmas.data_ptr(), d_out.data_ptr(), n, float(pixel_size), m.ctypes.data,
+                                          float(z_max)):
+        raise RuntimeError("pt_astig_estimate_device failed")
+    return d_out

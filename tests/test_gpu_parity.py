@@ -424,3 +424,57 @@ def test_equal_keys_in_the_flood(pt_gpu, oracle_mod, seed, thr, pairs):
         assert stats["rank_two_orders"] == 1, stats
     else:
         assert stats["rank_equal_keys"] == 1 and stats["heap_frames"] == 1, stats
+
+
+# ------------------------------------------------------------------ astigmatic z estimation --------
+AST_PIXEL = 160.0
+AST_REF_MODEL = [-60.0, 300.0, 0.06, 0.02, 140.0, 60.0, 300.0, -0.06, 0.02, 140.0]
+AST_EST_MODEL = [-1000.0, 1000.0, 0.0, 0.0, 160.0, 1000.0, 1000.0, 0.0, 0.0, 160.0]
+
+
+def _ast_pairs(model, z):
+    m = np.asarray(model, np.float64)
+    out = []
+    for c in (m[:5], m[5:]):
+        u = (np.asarray(z, np.float64) - c[0]) / c[1]
+        poly = 1.0 + u * u + c[2] * u ** 3 + c[3] * u ** 4
+        out.append((1.0 / AST_PIXEL) * c[4] * np.sqrt(np.maximum(0.0, poly)))
+    return np.stack(out, axis=-1)
+
+
+def _ast_inputs(n, seed):
+    rng = np.random.default_rng(seed)
+    z = rng.uniform(-700.0, 700.0, n)
+    sig = _ast_pairs(AST_REF_MODEL, z) * rng.normal(1.0, 0.05, (n, 2))
+    sig[:50] = rng.normal(0.0, 1.0, (50, 2))                      # random pairs (EstimateBad)
+    sig[50:54] = [[0.0, 1.0], [np.nan, 1.0], [np.inf, 2.0], [1.0, -1.0]]
+    cases = [-600.0, -500.0, -499.0, -123.4, 0.0, 123.4, 499.0, 500.0, 600.0]
+    sig[54:63] = _ast_pairs(AST_EST_MODEL, cases)                 # the reference's axial cases
+    return sig
+
+
+@pytest.mark.parametrize("model,zmax", [(AST_REF_MODEL, 600.0), (AST_EST_MODEL, 500.0), (AST_REF_MODEL, 50.0)])
+def test_astig_estimate_bit_exact(pt_gpu, oracle_mod, model, zmax):
+    """Astigmatism3DEstimation on the GPU equals the oracle (itself equal to the compiled reference)
+    bit for bit, degenerate pairs included."""
+    sig = _ast_inputs(20000, 5)
+    got = pt_gpu.astig_estimate(sig, AST_PIXEL, model, zmax)
+    exp = oracle_mod.astig_estimate(sig, AST_PIXEL, model, zmax)
+    assert np.array_equal(got, exp, equal_nan=True), np.argwhere(~((got == exp) | (np.isnan(got) & np.isnan(exp))))[:3]
+
+
+def test_astig_estimate_device_entry(pt_gpu, oracle_mod):
+    """Device-pointer entry over 1M pairs: equal to the host entry, z inside the limits, scores in [0, 1]."""
+    import torch
+    n = 1 << 20
+    rng = np.random.default_rng(9)
+    z = rng.uniform(-600.0, 600.0, n)
+    sig = _ast_pairs(AST_REF_MODEL, z) + rng.normal(0.0, 0.02, (n, 2))
+    d_in = torch.from_numpy(sig).cuda()
+    d_out = torch.empty_like(d_in)
+    pt_gpu.astig_estimate_device(d_in, d_out, AST_PIXEL, AST_REF_MODEL, 600.0)
+    got = d_out.cpu().numpy()
+    assert np.all(np.abs(got[:, 0]) <= 600.0) and np.all((got[:, 1] >= 0) & (got[:, 1] <= 1))
+    k = rng.choice(n, 4096, replace=False)
+    assert np.array_equal(got[k], oracle_mod.astig_estimate(sig[k], AST_PIXEL, AST_REF_MODEL, 600.0))
+    assert np.array_equal(got[:1000], pt_gpu.astig_estimate(sig[:1000], AST_PIXEL, AST_REF_MODEL, 600.0))

@@ -286,3 +286,99 @@ def test_per_plane_capacity(oracle_mod, ref_inputs):
 def test_empty_and_black(oracle_mod):
     assert oracle_mod.localization(np.zeros((3, 12, 16), np.uint16), 0.0, 0.0, 0).shape == (0, 18)
     assert oracle_mod.localization(np.zeros((2, 16, 16), np.uint16), 0.0, 0.0, 0, output_size=0).shape == (0, 18)
+
+
+# ---------------------------------------------------------------- astigmatic z ------------------
+# Models and cases of Tests_CPU/test_fit_AstigFit.cpp:20-54 (pixel size 160 nm, sampling 1).
+AST_PIXEL = 160.0
+AST_REF_MODEL = [-60.0, 300.0, 0.06, 0.02, 140.0, 60.0, 300.0, -0.06, 0.02, 140.0]
+AST_EST_MODEL = [-1000.0, 1000.0, 0.0, 0.0, 160.0, 1000.0, 1000.0, 0.0, 0.0, 160.0]
+AST_CASES = [(-600.0, -500.0), (-500.0, -500.0), (-499.0, -499.0), (-123.4, -123.4), (0.0, 0.0),
+             (123.4, 123.4), (499.0, 499.0), (500.0, 500.0), (600.0, 500.0)]
+
+
+def _ast_sigma(c, z, pixel=AST_PIXEL):
+    """SigmaModel of Tests_CPU/test_fit_AstigFit.cpp:93-101, numpy."""
+    u = (np.asarray(z, np.float64) - c[0]) / c[1]
+    poly = 1.0 + u * u + c[2] * u ** 3 + c[3] * u ** 4
+    return (1.0 / pixel) * c[4] * np.sqrt(np.maximum(0.0, poly))
+
+
+def _ast_pairs(model, z):
+    return np.stack([_ast_sigma(model[:5], z), _ast_sigma(model[5:], z)], axis=-1)
+
+
+def test_astig_estimate_axial_limits(oracle_mod):
+    """EstimateWithinLimits (test_fit_AstigFit.cpp:241-272): bounds are exact, interior points within
+    1 nm, the score is the closed form at the returned z, and Brent improves on the 5 nm grid."""
+    obs = np.array([c[0] for c in AST_CASES])
+    sig = _ast_pairs(AST_EST_MODEL, obs)
+    res = oracle_mod.astig_estimate(sig, AST_PIXEL, AST_EST_MODEL, 500.0)
+    for (o, e), (z, q), s in zip(AST_CASES, res, sig):
+        if abs(e) == 500.0:
+            assert z == e
+        else:
+            assert abs(z - e) <= 1.0 and -500.0 < z < 500.0
+        p = _ast_pairs(AST_EST_MODEL, z)
+        r = (p - s) / p
+        assert abs(q - 1.0 / (1.0 + (r[0] ** 2 + r[1] ** 2) / (2 * 0.2 * 0.2))) <= 1e-12
+        assert 0.0 <= q <= 1.0
+        if abs(o) > 500.0:
+            assert q < 1.0
+        if abs(o) == 123.4:
+            g = np.round(o / 5.0) * 5.0
+            dg = _ast_pairs(AST_EST_MODEL, g) - s
+            assert ((p - s) ** 2).sum() < (dg ** 2).sum()
+
+
+@pytest.mark.parametrize("rel,score", [(0.0, 1.0), (0.01, 400.0 / 401.0), (0.05, 16.0 / 17.0), (0.10, 0.8),
+                                       (0.20, 0.5), (1.0, 1.0 / 26.0), (-0.20, 0.5)])
+def test_astig_quality_scale(oracle_mod, rel, score):
+    """RelativeSigmaScore (test_fit_AstigFit.cpp:299-319): both observed sigmas off by the same
+    relative error at z = 123.4."""
+    z = 123.4
+    obs = _ast_pairs(AST_EST_MODEL, z) * (1 + rel)
+    assert abs(oracle_mod.astig_quality(AST_EST_MODEL, z, obs[0], obs[1], AST_PIXEL) - score) <= 1e-14
+
+
+def test_astig_scale_invariance_and_invalid_widths(oracle_mod):
+    """QualityScaleInvariant and InvalidWidthQuality (test_fit_AstigFit.cpp:322-346)."""
+    s = _ast_pairs(AST_EST_MODEL, 123.4) * np.array([1.05, 0.98])
+    a = oracle_mod.astig_estimate(s[None], AST_PIXEL, AST_EST_MODEL, 500.0)[0]
+    b = oracle_mod.astig_estimate(2 * s[None], AST_PIXEL / 2, AST_EST_MODEL, 500.0)[0]
+    assert a[0] == b[0] and a[1] == b[1]
+    r = s / _ast_pairs(AST_EST_MODEL, a[0]) - 1
+    assert abs(a[1] - 1.0 / (1.0 + (r ** 2).sum() / 0.08)) <= 1e-14
+    for sx, sy in ((0.0, 1.0), (1.0, -1.0), (np.nan, 1.0)):
+        assert oracle_mod.astig_quality(AST_EST_MODEL, 0.0, sx, sy, AST_PIXEL) == 0.0
+    bad = np.array([[0.0, 1.0], [1.0, -1.0], [np.nan, 1.0]])
+    assert np.all(oracle_mod.astig_estimate(bad, AST_PIXEL, AST_EST_MODEL, 500.0)[:, 1] == 0.0)
+    inv = list(AST_EST_MODEL)
+    inv[4] = 0.0
+    assert oracle_mod.astig_quality(inv, 0.0, 1.0, 1.0, AST_PIXEL) == 0.0
+
+
+def test_astig_estimate_recovers_z(oracle_mod):
+    """Estimate (test_fit_AstigFit.cpp:223-238): noisy pairs from the calibrated model within 40 nm,
+    score above 0.95."""
+    rng = np.random.default_rng(42)
+    z = np.linspace(-200.0, 200.0, 10)
+    sig = _ast_pairs(AST_REF_MODEL, z) + rng.normal(0.0, 0.02, (10, 2))
+    res = oracle_mod.astig_estimate(sig, AST_PIXEL, AST_REF_MODEL, 800.0)
+    assert np.all(np.abs(res[:, 0] - z) <= 40.0)
+    assert np.all((res[:, 1] > 0.95) & (res[:, 1] <= 1.0))
+
+
+def test_astig_estimate_bit_exact_vs_compiled_reference(oracle_mod):
+    rng = np.random.default_rng(11)
+    n = 4000
+    z = rng.uniform(-700.0, 700.0, n)
+    sig = _ast_pairs(AST_REF_MODEL, z) * rng.normal(1.0, 0.05, (n, 2))
+    sig[:50] = rng.normal(0.0, 1.0, (50, 2))           # EstimateBad: random pairs
+    sig[50:53] = [[0.0, 1.0], [np.nan, 1.0], [np.inf, 2.0]]
+    for model, zmax in ((AST_REF_MODEL, 600.0), (AST_EST_MODEL, 500.0), (AST_REF_MODEL, 50.0)):
+        exp = oracle_mod.ref_astig_estimate(sig, AST_PIXEL, model, zmax)
+        if exp is None:
+            pytest.skip("compiled reference not built (oracle/_ref)")
+        got = oracle_mod.astig_estimate(sig, AST_PIXEL, model, zmax)
+        assert np.array_equal(got, exp, equal_nan=True)
