@@ -34,6 +34,18 @@ JAC_FLOP_7x7 = 2940.0
 TRIAL_FLOP_7x7 = 735.0
 
 
+def measured_traffic_per_px(size):
+    """HBM bytes per pixel of the detection kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (measured on 512x512 frames); None for other frame sizes."""
+    path = os.path.join(ROOT, "profiles", "atrous_detect_kernel_512x512_traffic.md")
+    if size != 512 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_pixel:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(stack_host, thr, fit_params, budget_s):
     """Reference (or oracle) Localization on the first frames of the stack, OpenMP over frames."""
     import numpy as np
@@ -148,8 +160,10 @@ def main():
     else:
         bytes_l = ATROUS_BYTES_PER_PX * px_per_launch
         ach = bytes_l / (atrous_ms * 1e-3) / 1e9
+        tpp = measured_traffic_per_px(S)
         roof = {"bound": "hbm", "kernel": "atrous_detect_kernel", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None, "launch_ms": atrous_ms,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                "traffic": None if tpp is None else tpp * px_per_launch, "launch_ms": atrous_ms,
                 "bytes_per_launch": bytes_l}
     stage_ms = {k: v / args.steps for k, v in stages.items() if k.endswith("_ms")}
 

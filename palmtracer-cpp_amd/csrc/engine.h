@@ -49,6 +49,7 @@ struct Options {
     bool forceSequential = false;   // every frame through the exact frame flood
     int exactPath = 0;              // exact frame flood from: 0 rank flood, 1 wave-queue flood, 2 heap
     unsigned fitWaves = 0;          // persistent FP32 fit: resident waves (0: 16 per CU)
+    int fitRefill = 16;             // persistent FP32 fit: free lanes that trigger a refill round
 };
 
 // One stream with its own detection buffers. The engine runs the chunks on the main lane and the

@@ -225,6 +225,7 @@ FitParams Engine::fit_params(uint64_t fit, const double* fp) const
     if (roi % 2 == 0) roi--;                                // Fitting.cpp:37
     p.roi = (int)std::max<long>(roi, 1);
     p.maxIter = opt.maxIter;
+    p.refill = opt.fitRefill;
     return p;
 }
 

@@ -270,24 +270,60 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
         }
     };
 
+    // Job queue. The wave claims blocks of 64 ROI indices (one atomic per block, the next block
+    // claimed one block ahead), and every lane holds its next job prefetched (index and Roi record,
+    // loaded one refill ahead), so a refill round waits only for the new ROIs' pixels.
+    uint32_t poolBase = 0, poolUsed = 64, claim = 0;
+    if (lane == 0) claim = atomicAdd(next, 64u);
+    auto next_block = [&]() {
+        const uint32_t b = __builtin_amdgcn_readfirstlane(claim);
+        claim = 0;
+        if (lane == 0) claim = atomicAdd(next, 64u);
+        return b;
+    };
+    uint32_t nIdx = 0;
+    Roi nR{};
+    double seg = 0, surf = 0;
+    {
+        poolBase = next_block();
+        poolUsed = 64;
+        nIdx = poolBase + (uint32_t)lane;
+        if (nIdx < nRoi) nR = rois[nIdx];
+    }
+
     while (true) {
-        // ---- lanes without a ROI take the next ones (one atomic per wave) ----
+        // ---- lanes without a ROI take their prefetched jobs ----
         while (true) {
             const uint64_t want = __ballot(state == kFetch);
             if (!want) break;
-            const int leader = __ffsll((unsigned long long)want) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(next, (unsigned)__popcll(want));
-            base = __shfl(base, leader);
+            // Refills are batched: a refill round costs the whole wave a memory round trip, so
+            // lanes wait for one until enough of them are free or none can run a pass.
+            if (__popcll(want) < fp.refill && __ballot(state >= kPassInit) != 0) break;
+            // Next prefetch indices from the block pool, in lane order.
+            const uint32_t c = (uint32_t)__popcll(want), rank = (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+            uint32_t newIdx;
+            if (poolUsed + c <= 64u) {
+                newIdx = poolBase + poolUsed + rank;
+                poolUsed += c;
+            } else {
+                const uint32_t k0 = 64u - poolUsed, nb = next_block();
+                newIdx = rank < k0 ? poolBase + poolUsed + rank : nb + (rank - k0);
+                poolBase = nb;
+                poolUsed = c - k0;
+            }
             if (state != kFetch) continue;
-            idx = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+            idx = nIdx;
+            const Roi r = nR;
+            nIdx = newIdx;
+            if (nIdx < nRoi) nR = rois[nIdx];
             if (idx >= nRoi) {
                 state = kDone;
                 continue;
             }
-            const Roi r = rois[idx];
             x0 = r.x;
             y0 = r.y;
+            seg = r.seg;
+            surf = r.surf;
             const uint64_t sx = to_size_f(r.x), sy = to_size_f(r.y), h64 = (uint64_t)half;
             // Border check with the reference's unsigned arithmetic (GaussianFit.cpp:51-56).
             if (h64 > sx || sx >= (uint64_t)W - h64 || h64 > sy || sy >= (uint64_t)H - h64) {
@@ -322,7 +358,7 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
             state = kPassInit;
         }
         if (__ballot(state != kDone) == 0) break;
-        if (state == kDone) continue;
+        if (state < kPassInit) continue;
 
         double* o = rows + (size_t)idx * kCols;
         // ---- chi^2 and Jacobian products at p (initial pass) or pTry (trial pass) ----
@@ -470,8 +506,7 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
             break;
         }
         if (failed) {
-            const Roi& r = rois[idx];
-            row_fail(o, MODE, x0, y0, r.seg, r.surf);
+            row_fail(o, MODE, x0, y0, seg, surf);
             state = kFetch;
         } else if (finish) {
             // computeR2 at p (GaussianFit.cpp:270-295): squared Pearson correlation of data and model,
@@ -479,16 +514,15 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
             const double vd = sdd - sd * sd / nPix, vm = cmm - cm * cm / nPix, cv = cdm - sd * cm / nPix;
             const double rr = cv / sqrt(vd * vm);
             const double rss = isnan(rr) ? 0.0 : rr * rr;
-            const Roi& r = rois[idx];
             if (rss < kEpsilon) {
-                row_fail(o, MODE, p[2], p[3], r.seg, r.surf);
+                row_fail(o, MODE, p[2], p[3], seg, surf);
             } else {
                 double sx = fp.sigmaX, sy = fp.sigmaY, th = fp.theta;
                 if constexpr (MODE == 1) { sx = sy = fp.sigmaX; th = 0; }
                 if constexpr (MODE == 2) { sx = sy = p[4]; th = 0; }
                 if constexpr (MODE >= 3) { sx = p[4] / kSqrt2d; sy = p[5] / kSqrt2d; }
                 if constexpr (MODE == 4) th = p[6];
-                row_fill(o, MODE, p[2], p[3], r.seg, r.surf, p[0], p[1], sx, sy, th, rss);
+                row_fill(o, MODE, p[2], p[3], seg, surf, p[0], p[1], sx, sy, th, rss);
             }
             state = kFetch;
         }
