@@ -46,14 +46,14 @@ __global__ void positions_to_rois(const double* __restrict__ pos, uint64_t n, ui
 }
 
 // ROIs of a chunk appended to the call's ROI store with absolute frame numbers, and the chunk's
-// per-frame ROI offsets shifted to the store. run[1] is the chunk's first ROI in the store (set by
+// per-frame ROI offsets shifted to the store. *base is the chunk's first ROI in the store (set by
 // the chunk's ROI scan); ROIs beyond the store's capacity set status bit 4.
 __global__ void gather_rois_kernel(const Roi* __restrict__ src, uint32_t z0, Roi* __restrict__ dst,
-                                   const uint32_t* __restrict__ roiBase, uint32_t frames, const uint32_t* __restrict__ run,
+                                   const uint32_t* __restrict__ roiBase, uint32_t frames, const uint32_t* __restrict__ base,
                                    uint32_t cap, uint32_t* __restrict__ dBase, uint32_t* __restrict__ counters)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-    const uint32_t off = run[1], n = roiBase[frames];
+    const uint32_t off = *base, n = roiBase[frames];
     if (t == 0 && (uint64_t)off + n > cap) atomicOr(&counters[kStatus], 4u);
     const uint32_t m = off >= cap ? 0u : min(n, cap - off);
     for (uint32_t i = t; i < m; i += stride) {
@@ -82,10 +82,10 @@ __global__ void positions_ids(const double* __restrict__ pos, const uint32_t* __
 }
 } // namespace
 
-void gather_rois(const Roi* src, uint32_t z0, Roi* dst, const uint32_t* roiBase, uint32_t frames, const uint32_t* run,
+void gather_rois(const Roi* src, uint32_t z0, Roi* dst, const uint32_t* roiBase, uint32_t frames, const uint32_t* base,
                  uint32_t cap, uint32_t* dBase, uint32_t* counters, hipStream_t s)
 {
-    gather_rois_kernel<<<1024, 256, 0, s>>>(src, z0, dst, roiBase, frames, run, cap, dBase, counters);
+    gather_rois_kernel<<<1024, 256, 0, s>>>(src, z0, dst, roiBase, frames, base, cap, dBase, counters);
     PT_CHECK(hipGetLastError());
 }
 
@@ -120,6 +120,8 @@ Engine& Engine::get()
 Engine::Engine()
 {
     PT_CHECK(hipStreamCreateWithFlags(&main_.stream, hipStreamNonBlocking));
+    PT_CHECK(hipStreamCreateWithFlags(&mainB_.stream, hipStreamNonBlocking));
+    for (hipEvent_t& e : scanEv_) PT_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     PT_CHECK(hipStreamCreateWithFlags(&side_.stream, hipStreamNonBlocking));
     stream = main_.stream;
     for (hipEvent_t& e : sideEv_) PT_CHECK(hipEventCreate(&e));
@@ -169,6 +171,7 @@ void Lane::prepare(size_t F, size_t npx)
     b.found = pf + 4 * s;
     b.frameFlag = pf + 5 * s;
     b.roiBase = pf + 6 * s;
+    b.chunkBase = pf + 7 * s;
     b.regions = regions.as<Region>();
     b.fz = fz.as<Fz>();
     b.order = order.as<uint32_t>();
@@ -278,6 +281,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     const size_t npx = (size_t)W * H;
     const size_t F = std::max<size_t>(1, std::min<size_t>(planes, std::min<size_t>(65535, opt.chunkPixels / npx)));
     main_.prepare(F, npx);
+    if (planes > F) mainB_.prepare(F, npx);
     const FitParams fp = fit_params(fit, fitParams);
     unsigned long long* passes = nullptr;
     if (opt.profile) {
@@ -406,27 +410,35 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     run_.ensure(16);
     uint32_t* dRun = run_.as<uint32_t>();
     fill_u32_kernel<<<1, 64, 0, main_.stream>>>(dRun, 2, 0u);
-    main_.b.runTotal = dRun;
+    PT_CHECK(hipEventRecord(scanEv_[1], main_.stream));   // the first chunk's scan waits for the reset
     const uint64_t roiCap = std::min<uint64_t>(std::min<uint64_t>(outCapRows, planes * std::min<uint64_t>(maxPerPlane, npx / 4 + 2)),
                                                0xFFFFFFF0ull);
     allRois_.ensure(std::max<uint64_t>(roiCap, 1) * sizeof(Roi));
+    // Consecutive chunks alternate between two lanes (buffers and streams), so that the latency-bound
+    // end of one chunk (component floods, statistics, ROI list) overlaps the next chunk's detection.
+    // Only the ROI scans are ordered across the lanes (the running ROI total).
     for (uint64_t z0 = 0; z0 < planes; z0 += F, ++chunkNo) {
+        Lane& L = (chunkNo & 1) ? mainB_ : main_;
         const size_t f = std::min<uint64_t>(F, planes - z0);
         chunkZ0.push_back(z0);
-        detect_chunk(main_, d_in + z0 * npx, f, H, W, thr, ratio, maxPerPlane, 3 * chunkNo, nullptr, false);
-        gather_rois(main_.rois.as<Roi>(), (uint32_t)z0, allRois_.as<Roi>(), main_.b.roiBase, (uint32_t)f, dRun,
-                    (uint32_t)roiCap, dBase + z0, main_.b.counters, main_.stream);
+        L.b.runTotal = dRun;
+        L.b.scanWait = scanEv_[(chunkNo + 1) & 1];
+        L.b.scanDone = scanEv_[chunkNo & 1];
+        detect_chunk(L, d_in + z0 * npx, f, H, W, thr, ratio, maxPerPlane, 3 * chunkNo, nullptr, false);
+        L.b.runTotal = nullptr;
+        L.b.scanWait = L.b.scanDone = nullptr;
+        gather_rois(L.rois.as<Roi>(), (uint32_t)z0, allRois_.as<Roi>(), L.b.roiBase, (uint32_t)f, L.b.chunkBase,
+                    (uint32_t)roiCap, dBase + z0, L.b.counters, L.stream);
         uint32_t* m = meta + (chunkNo & 1) * metaWords;
-        PT_CHECK(hipMemcpyAsync(m, main_.b.counters, kNumCounters * 4, hipMemcpyDeviceToHost, main_.stream));
-        PT_CHECK(hipMemcpyAsync(m + kNumCounters, dRun, 4, hipMemcpyDeviceToHost, main_.stream));
-        PT_CHECK(hipMemcpyAsync(m + kNumCounters + 1, main_.b.flagList, f * 4, hipMemcpyDeviceToHost, main_.stream));
-        PT_CHECK(hipEventRecord(chunkEv_[chunkNo & 1], main_.stream));
+        PT_CHECK(hipMemcpyAsync(m, L.b.counters, kNumCounters * 4, hipMemcpyDeviceToHost, L.stream));
+        PT_CHECK(hipMemcpyAsync(m + kNumCounters, dRun, 4, hipMemcpyDeviceToHost, L.stream));
+        PT_CHECK(hipMemcpyAsync(m + kNumCounters + 1, L.b.flagList, f * 4, hipMemcpyDeviceToHost, L.stream));
+        PT_CHECK(hipEventRecord(chunkEv_[chunkNo & 1], L.stream));
         if (chunkNo > 0) read_meta(chunkNo - 1);
         side_poll(false);
         last.chunks++;
     }
     read_meta(chunkNo - 1);
-    main_.b.runTotal = nullptr;
     set_u32(dBase + planes, (uint32_t)totalRois, main_.stream);
     // Fit of every ROI, then each plane's rows in (Y, X) order.
     main_.rows.ensure(std::max<uint64_t>(totalRois, 1) * kCols * 8);

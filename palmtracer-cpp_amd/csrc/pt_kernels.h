@@ -69,7 +69,10 @@ struct DetectBuffers {
     uint32_t* waveList = nullptr; // [F] frames the rank flood could not settle (wave-queue flood)
     uint32_t* rfScratch = nullptr;// rank-flood work space: rfSlots * rf_slot_words(npx) u32
     uint32_t* seedBits = nullptr; // per frame: bitmap of the local maxima, seed_words(npx) u32
-    uint32_t* runTotal = nullptr; // optional: [0] ROIs of the call so far, [1] first ROI of this chunk
+    uint32_t* runTotal = nullptr; // optional: ROIs of the call so far (shared by the lanes of a call)
+    uint32_t* chunkBase = nullptr;// with runTotal: receives the first ROI of this chunk in the call
+    hipEvent_t scanWait = nullptr;// with runTotal: the ROI scan waits for it (previous chunk's scan)
+    hipEvent_t scanDone = nullptr;// with runTotal: recorded after this chunk's ROI scan
     size_t frames = 0, npx = 0, regionCap = 0, multiCap = 0, fbSlots = 0, rfSlots = 0;
 };
 

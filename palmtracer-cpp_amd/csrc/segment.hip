@@ -179,6 +179,7 @@ __global__ void __launch_bounds__(NT) label_assign_kernel(const uint32_t* __rest
 constexpr int FL_NT = 1024;
 constexpr int FL_PER = 8;                       // mask entries per thread in the LDS path
 constexpr uint32_t FL_MAX_LDS_N = FL_NT * FL_PER;
+constexpr uint32_t FL_SLOTS = 16;               // multi-label components flooded inside the workgroup
 
 __device__ inline uint32_t chain_label(const uint32_t* sbf, uint32_t s, int W)
 {
@@ -206,6 +207,70 @@ __device__ inline void add_label(uint32_t* st, uint32_t L, Cas cas, Xchg xchg)
     if (old != 0u && old != L && old != kNone) xchg(st, kNone);
 }
 
+// Watershed of one multi-label component of at most 64 pixels by one wave, one pixel per lane, the
+// queue held in registers: the same flood and the same certainty proof as flood_kernel (step 3 of
+// the file header), with the component's adjacency as lane masks. list holds the component's
+// pixels; seeds carry their labels in fin and every other pixel 0. Returns false when a pop meets
+// equal keys or cannot be proven.
+__device__ inline bool flood_small(const uint32_t* list, uint32_t cnt, const uint32_t* Pf, uint32_t* fn, int W, int H,
+                                   double nmax)
+{
+    const int lane = threadIdx.x & 63;
+    const bool act = (uint32_t)lane < cnt;
+    const uint32_t p = act ? list[lane] : 0u;
+    const int y = (int)(p / (uint32_t)W), x = (int)p - y * W;
+    const uint32_t P = act ? Pf[p] : 0u;
+    uint32_t lbl = act ? ld_relaxed(fn + p) : 0u;
+    uint64_t adj = 0;
+    for (uint32_t j = 0; j < cnt; ++j) {
+        const int qy = __shfl(y, (int)j), qx = __shfl(x, (int)j);
+        const int dy = qy - y, dx = qx - x;
+        if (act && (uint32_t)lane != j && dy >= -1 && dy <= 1 && dx >= -1 && dx <= 1) adj |= 1ull << j;
+    }
+    // Neighbours inside the frame; those outside the component are below the threshold.
+    const int inFrame = (1 + (y > 0) + (y < H - 1)) * (1 + (x > 0) + (x < W - 1)) - 1;
+    uint64_t unl = __ballot(act && lbl == 0u);
+    // Seeds next to an unlabelled pixel are pushed at age 0 (Segmentation2D.cpp:340-359).
+    uint64_t qm = __ballot(act && lbl != 0u && (__popcll(adj) < inFrame || (adj & unl) != 0));
+    uint32_t age = 0;
+    double key = age_key(P, 0);
+    uint32_t it = 0;
+    const double eps = kEpsilon, delta = 1e-9;
+    while (qm) {
+        const bool queued = (qm >> lane) & 1ull;
+        double km = queued ? key : 1.79769313486231570815e308;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double k2 = __shfl_xor(km, o);
+            km = k2 < km ? k2 : km;
+        }
+        const uint64_t at = __ballot(queued && key == km);
+        if (__popcll(at) > 1) return false;           // equal keys: the heap order would decide
+        const int a = __ffsll((unsigned long long)at) - 1;
+        const uint32_t pa = __shfl(P, a), ta = __shfl(age, a), la = __shfl(lbl, a);
+        bool good = true;
+        if (queued && lane != a) {
+            const double oa = (double)pa * kPScale, ob = (double)P * kPScale;
+            const uint32_t tb = age;
+            if (ta == tb) good = oa != ob;
+            else if (ta < tb) good = ob - oa < (double)(tb - ta) * eps - delta;
+            else good = oa - ob > nmax * eps + delta;
+        }
+        if (__ballot(!good)) return false;
+        ++it;
+        const uint64_t claims = __shfl(adj, a) & unl;
+        if ((claims >> lane) & 1ull) {
+            lbl = la;
+            age = it;
+            key = age_key(P, it);
+            fn[p] = la;
+        }
+        unl &= ~claims;
+        qm = (qm & ~(1ull << a)) | claims;
+    }
+    return true;
+}
+
 __global__ void __launch_bounds__(FL_NT) frame_label_kernel(
     const uint32_t* __restrict__ P, const uint32_t* __restrict__ seedBits, uint32_t* __restrict__ lab,
     uint32_t* __restrict__ fin, PixAux* __restrict__ aux, const uint32_t* __restrict__ maskList,
@@ -216,7 +281,7 @@ __global__ void __launch_bounds__(FL_NT) frame_label_kernel(
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ uint32_t waveSum[FL_NT / 64];
-    __shared__ uint32_t nRoots;
+    __shared__ uint32_t nRoots, nSlots;
     const int f = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t n = maskCnt[f], ns = seedCnt[f];
@@ -227,7 +292,7 @@ __global__ void __launch_bounds__(FL_NT) frame_label_kernel(
     uint32_t* lb = lab + (size_t)f * npx;
     uint32_t* fn = fin + (size_t)f * npx;
     PixAux* ax = aux + (size_t)f * npx;
-    if (tid == 0) nRoots = 0u;
+    if (tid == 0) { nRoots = 0u; nSlots = 0u; }
 
     // A maximum s with chain label L: its label, and its region when it opens one.
     auto label_seed = [&](uint32_t s) -> uint32_t {
@@ -368,10 +433,67 @@ __global__ void __launch_bounds__(FL_NT) frame_label_kernel(
                 add_label(&state[sc[j]], sL[j], [](uint32_t* a, uint32_t c, uint32_t v) { return atomicCAS(a, c, v); },
                           [](uint32_t* a, uint32_t v) { atomicExch(a, v); });
         __syncthreads();
-        // 5. Labels of every mask pixel.
+        // 5. Labels of every mask pixel. Multi-label components take a slot (their pixel list, in
+        //    the parents' space, which is free now) for the flood below; the rest of them are listed
+        //    for flood_kernel.
+        uint32_t* slotComp = parent;
+        uint32_t* slotCnt = parent + FL_SLOTS;
+        uint32_t* slotPix = parent + 2 * FL_SLOTS;
+        const uint32_t slotMax = ldsCap >= 2 * FL_SLOTS + 64 ? min((uint32_t)FL_SLOTS, (ldsCap - 2 * FL_SLOTS) / 64) : 0u;
+        if (tid < FL_SLOTS) slotCnt[tid] = 0u;
+        uint32_t multiBits = 0;
 #pragma unroll
-        for (int j = 0; j < FL_PER; ++j)
-            if (tid + j * FL_NT < n) assign(pp[j], rr[j], (firstBits >> j) & 1u, state[rr[j]]);
+        for (int j = 0; j < FL_PER; ++j) {
+            if (tid + j * FL_NT >= n) continue;
+            const uint32_t st = state[rr[j]];
+            const bool first = (firstBits >> j) & 1u;
+            if (st != kNone || slotMax == 0) { assign(pp[j], rr[j], first, st); continue; }
+            lb[pp[j]] = rr[j] | (first ? kCompRoot : 0u);
+            if (!seed_bit(sbf, pp[j])) fn[pp[j]] = 0u;
+            multiBits |= 1u << j;
+            if (first) {
+                const uint32_t sl = atomicAdd(&nSlots, 1u);
+                if (sl < slotMax) {
+                    slotComp[sl] = rr[j];
+                } else {
+                    const uint32_t k = atomicAdd(&counters[kMultiCnt], 1u);
+                    if (k < multiCap) multiList[k] = ((uint64_t)f << 32) | rr[j];
+                    else flag_frame(frameFlag, counters, flagList, f);
+                }
+            }
+        }
+        __syncthreads();
+        // 6. Pixel lists of the slotted components, then one wave per component: the register
+        //    flood for components of up to 64 pixels, flood_kernel for larger ones.
+        const uint32_t nSl = min(nSlots, slotMax);
+#pragma unroll
+        for (int j = 0; j < FL_PER; ++j) {
+            if (!((multiBits >> j) & 1u)) continue;
+            for (uint32_t sl = 0; sl < nSl; ++sl) {
+                if (slotComp[sl] != rr[j]) continue;
+                const uint32_t pos = atomicAdd(&slotCnt[sl], 1u);
+                if (pos < 64u) slotPix[sl * 64 + pos] = pp[j];
+                break;
+            }
+        }
+        __syncthreads();
+        for (uint32_t sl = (uint32_t)wave; sl < nSl; sl += FL_NT / 64) {
+            const uint32_t cnt = slotCnt[sl];
+            bool ok = true;
+            if (cnt > 64u) {
+                if (lane == 0) {
+                    const uint32_t k = atomicAdd(&counters[kMultiCnt], 1u);
+                    if (k < multiCap) multiList[k] = ((uint64_t)f << 32) | slotComp[sl];
+                    else flag_frame(frameFlag, counters, flagList, f);
+                }
+            } else {
+                ok = flood_small(slotPix + sl * 64, cnt, Pf, fn, W, H, (double)n);
+            }
+            if (!ok && lane == 0) {
+                atomicAdd(&counters[kFlagUnproven], 1u);
+                flag_frame(frameFlag, counters, flagList, f);
+            }
+        }
     } else {
         // Same steps in global memory, over pixel indices (label state in aux[component].cmin).
         for (uint32_t k = tid; k < n; k += FL_NT) lb[ml[k]] = ml[k];
@@ -1605,10 +1727,10 @@ __global__ void __launch_bounds__(NT) roi_emit_kernel(const uint32_t* __restrict
     }
 }
 
-// Exclusive scan of n counts into out[0..n]; with run set, also run[1] = run[0] (the first ROI of
-// this chunk in the call) and run[0] += total.
+// Exclusive scan of n counts into out[0..n]; with run set, also *base = *run (the first ROI of this
+// chunk in the call) and *run += total.
 __global__ void __launch_bounds__(1024) exclusive_scan_kernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int n,
-                                                              uint32_t* __restrict__ run)
+                                                              uint32_t* __restrict__ run, uint32_t* __restrict__ base)
 {
     // Single workgroup: each thread scans a contiguous segment, then the segment totals.
     __shared__ uint32_t seg[1024];
@@ -1623,8 +1745,8 @@ __global__ void __launch_bounds__(1024) exclusive_scan_kernel(const uint32_t* __
         for (int i = 0; i < 1024; ++i) { const uint32_t t = seg[i]; seg[i] = acc; acc += t; }
         out[n] = acc;
         if (run) {
-            run[1] = run[0];
-            run[0] += acc;
+            *base = *run;
+            *run += acc;
         }
     }
     __syncthreads();
@@ -1636,7 +1758,7 @@ __global__ void __launch_bounds__(1024) exclusive_scan_kernel(const uint32_t* __
 
 void launch_exclusive_scan(const uint32_t* d_in, uint32_t* d_out, int n, hipStream_t s)
 {
-    exclusive_scan_kernel<<<1, 1024, 0, s>>>(d_in, d_out, n, nullptr);
+    exclusive_scan_kernel<<<1, 1024, 0, s>>>(d_in, d_out, n, nullptr, nullptr);
     PT_CHECK(hipGetLastError());
 }
 
@@ -1693,7 +1815,9 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
     region_stats_kernel<<<F, RS_NT, 0, s>>>(b.P, b.fin, b.aux, b.maskList, b.maskCnt, b.rootCnt, b.regions, b.regionCap,
                                             b.counters, W, npx, skip);
     region_count_kernel<<<F, NT, 0, s>>>(b.regions, b.regionCap, b.rootCnt, b.found, b.keep, maxPoints, skip);
-    exclusive_scan_kernel<<<1, 1024, 0, s>>>(b.keep, b.roiBase, F, b.runTotal);
+    if (b.runTotal && b.scanWait) PT_CHECK(hipStreamWaitEvent(s, b.scanWait, 0));
+    exclusive_scan_kernel<<<1, 1024, 0, s>>>(b.keep, b.roiBase, F, b.runTotal, b.chunkBase);
+    if (b.runTotal && b.scanDone) PT_CHECK(hipEventRecord(b.scanDone, s));
     roi_emit_kernel<<<F, NT, 0, s>>>(b.P, b.fin, b.regions, b.regionCap, b.rootCnt, b.found, b.keep, b.roiBase, d_rois,
                                      W, npx);
     PT_CHECK(hipGetLastError());

@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(HERE), "libpalmtracer_amd.so")
+# The in-tree library; PALMTRACER_AMD_LIBRARY names another build of it.
+LIB_PATH = os.environ.get("PALMTRACER_AMD_LIBRARY") or os.path.join(os.path.dirname(HERE), "libpalmtracer_amd.so")
 COLUMNS = ["Id", "Plane", "Index", "Channel", "X", "Y", "Z", "Integrated Intensity", "Sigma X", "Sigma Y", "Theta",
            "MSE XY", "MSE Z", "Intensity 0", "Intensity Offset", "Intensity", "Surface", "Circularity"]
 NCOL = 18
