@@ -135,6 +135,7 @@ def main():
         st = pt.stage_times()
         for k in stages:
             stages[k] += st[k]
+    last = pt.last_stats()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ms_step = 1e3 * dt / args.steps
@@ -190,7 +191,11 @@ def main():
                                f"(AutoThreshold of frame 0)",
                    "frames": F, "height": S, "width": S, "emitters_per_frame": args.emitters, "fit": 2,
                    "roi": 7, "threshold": thr, "rows": rows, "fitted_rows": fitted, "seed": seed,
-                   "stage_ms_per_step": stage_ms},
+                   "stage_ms_per_step": stage_ms,
+                   "watershed_frames": {k: last[k] for k in ("flagged_frames", "deferred_frames", "flag_unproven",
+                                                             "flag_overflow", "rank_too_big", "rank_equal_keys",
+                                                             "rank_unsettled", "rank_two_orders", "heap_frames",
+                                                             "multi_components")}},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
