@@ -526,6 +526,7 @@ static void fill_row(int mode, const or_region* info, double* pt, double x, doub
     pt[0] = 0; pt[1] = 0; pt[2] = 0; pt[3] = -1;
     pt[4] = x; pt[5] = y; pt[6] = z;
     if (mode == 0) pt[7] = info->intensity;
+    else if (mode == 5) pt[7] = intensity;                              /* Fitting.cpp:86-88 */
     else pt[7] = (intensity > OR_EPSILON && sx > OR_EPSILON && sy > OR_EPSILON) ? 2.0 * OR_PI * intensity * sx * sy : -1;
     pt[8] = sx; pt[9] = sy; pt[10] = theta; pt[11] = rssXY; pt[12] = rssZ;
     pt[13] = intensity; pt[14] = offset; pt[15] = info->intensity; pt[16] = info->surf;
@@ -639,10 +640,339 @@ static double rel_step(const lm_ws* ws)
     return m;
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* Spline PSF fit, EMCCD Poisson likelihood (lib/fit/SplineFit.cpp:36-318, Fitting.cpp:60-175,  */
+/* Matrix.cpp:245-296, :358-411). fp = [roi, spline x, y, z sizes, deltaZ, 64 x size coefficients]. */
+/* ------------------------------------------------------------------------------------------ */
+
+#define OR_SPL_N 5
+#define OR_SPL_EPS2 1e-10        /* SplineFit.hpp: EPSILON_2 */
+
+typedef struct {
+    size_t roi, half, n, sx, sy, sz, sxy, ssize, off;
+    double initZ, deltaZ, ratio, roiParam;
+    const double* c;
+} spl_ws;
+
+/* parseParams (SplineFit.cpp:36-49), then the odd ROI side of Fitting::initialize (:35-39). */
+static void spl_parse(const double* fp, spl_ws* s)
+{
+    s->roiParam = fp[0];
+    s->roi = to_size(fp[0]);
+    s->sx = to_size(fp[1]); s->sy = to_size(fp[2]); s->sz = to_size(fp[3]);
+    s->sxy = s->sx * s->sy;
+    s->ssize = s->sx * s->sy * s->sz;
+    s->initZ = (double)s->sz / 2.0;
+    s->deltaZ = fp[4];
+    s->c = fp + 5;
+    const double center = s->c[(32 * s->ssize) + (s->sz / 2 * s->sxy) + (s->sy / 2 * s->sx) + (s->sx / 2)];
+    s->ratio = fabs(center) < DBL_EPSILON ? 1.0 : 4.0 / center;
+    s->off = to_size(floor(((double)s->sx + 1.0 - (double)s->roi) / 2));
+    if (s->roi % 2 == 0) s->roi--;
+    s->half = s->roi / 2;
+    s->n = s->roi * s->roi;
+}
+
+/* delta3D (SplineFit.cpp:296-317): monomials x^k y^j z^i of the 4x4x4 patch and their derivatives. */
+static void spl_delta3d(double x, double y, double z, double* f, double* dfx, double* dfy, double* dfz)
+{
+    for (int i = 0; i < 64; ++i) { f[i] = 0.0; dfx[i] = 0.0; dfy[i] = 0.0; dfz[i] = 0.0; }
+    double cz = 1.0;
+    for (int i = 0; i < 4; ++i) {
+        double cy = 1.0;
+        for (int j = 0; j < 4; ++j) {
+            double cx = 1.0;
+            for (int k = 0; k < 4; ++k) {
+                const int id = (i * 16) + (j * 4) + k;
+                const double coef = cz * cy * cx;
+                f[id] = coef;
+                if (k + 1 < 4) dfx[id + 1] = (double)(k + 1) * coef;
+                if (j + 1 < 4) dfy[id + 4] = (double)(j + 1) * coef;
+                if (i + 1 < 4) dfz[id + 16] = (double)(i + 1) * coef;
+                cx *= x;
+            }
+            cy *= y;
+        }
+        cz *= z;
+    }
+}
+
+static inline size_t clamp_sz(size_t v, size_t hi) { return hi < v ? hi : v; }   /* std::clamp<size_t>(v, 0, hi) */
+static inline double clamp_d(double v, double lo, double hi) { return v < lo ? lo : (hi < v ? hi : v); }
+
+/* derivativeSpline (SplineFit.cpp:219-240): pos clamped in place (unsigned: negative indices wrap
+ * and clamp to the far edge). */
+static double spl_deriv(const spl_ws* s, size_t* px, size_t* py, size_t* pz, const double* f, const double* dfx,
+                        const double* dfy, const double* dfz, const double* p, double* dudt)
+{
+    *px = clamp_sz(*px, s->sx - 1); *py = clamp_sz(*py, s->sy - 1); *pz = clamp_sz(*pz, s->sz - 1);
+    const size_t pos = (((*pz * s->sy) + *py) * s->sx) + *px;
+    for (int i = 0; i < OR_SPL_N; ++i) dudt[i] = 0.0;
+    for (size_t i = 0; i < 64; ++i) {
+        const size_t id = (i * s->ssize) + pos;
+        dudt[1] += f[i] * s->c[id];
+        dudt[2] += dfx[i] * s->c[id];
+        dudt[3] += dfy[i] * s->c[id];
+        dudt[4] += dfz[i] * s->c[id];
+    }
+    dudt[0] = 1;
+    dudt[2] *= -1 * p[1];
+    dudt[3] *= -1 * p[1];
+    dudt[4] *= p[1];
+    return p[0] + (p[1] * dudt[1]);
+}
+
+/* Cholesky (Matrix.cpp:272-294). */
+static int spl_cholesky(const double* a, double* l, double* u)
+{
+    const int n = OR_SPL_N;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i + 1; ++j) {
+            double sum = 0;
+            for (int k = 0; k < j; ++k) sum += u[(i * n) + k] * u[(j * n) + k];
+            if (i == j) {
+                if (a[(i * n) + i] - sum >= 0) {
+                    u[(i * n) + j] = sqrt(a[(i * n) + i] - sum);
+                    l[(j * n) + i] = u[(i * n) + j];
+                } else {
+                    return 0;
+                }
+            } else {
+                u[(i * n) + j] = 1.0 / u[(j * n) + j] * (a[(i * n) + j] - sum);
+                l[(j * n) + i] = u[(i * n) + j];
+            }
+        }
+    return 1;
+}
+
+/* luEvaluate (SplineFit.cpp:270-293). */
+static void spl_lu_eval(const double* l, const double* u, const double* b, double* x)
+{
+    const int n = OR_SPL_N;
+    double y[OR_SPL_N];
+    for (int i = 0; i < n; ++i) {
+        y[i] = b[i];
+        for (int j = 0; j < i; ++j) y[i] -= l[(j * n) + i] * y[j];
+        y[i] /= l[(i * n) + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        x[i] = y[i];
+        for (int j = i + 1; j < n; j++) x[i] -= u[(j * n) + i] * x[j];
+        x[i] /= u[(i * n) + i];
+    }
+}
+
+/* InverseLU (Matrix.cpp:358-411), in place on in. */
+static void spl_inverse_lu(double* in, double* out)
+{
+    const size_t n = OR_SPL_N;
+    double tmp1 = 0, y[OR_SPL_N * OR_SPL_N];
+    for (size_t j = 0; j < n; ++j) {
+        for (size_t i = 0; i <= j; ++i) {
+            if (i > 0) {
+                for (size_t k = 0; k <= i - 1; ++k) tmp1 += in[i + (k * n)] * in[k + (j * n)];
+                in[i + (j * n)] -= tmp1;
+                tmp1 = 0;
+            }
+        }
+        for (size_t i = j + 1; i < n; ++i) {
+            if (j > 0) {
+                for (size_t k = 0; k <= j - 1; ++k) tmp1 += in[i + (k * n)] * in[k + (j * n)];
+                in[i + (j * n)] = (1 / in[j + (j * n)]) * (in[i + (j * n)] - tmp1);
+                tmp1 = 0;
+            } else {
+                in[i + (j * n)] = (1 / in[j + (j * n)]) * in[i + (j * n)];
+            }
+        }
+    }
+    for (size_t num = 0; num < n; ++num) {
+        y[0] = num == 0 ? 1 : 0;
+        for (size_t i = 1; i < n; ++i) {
+            const size_t b = i == num ? 1 : 0;
+            for (size_t j = 0; j <= i - 1; ++j) tmp1 += in[i + (j * n)] * y[j];
+            y[i] = (double)b - tmp1;
+            tmp1 = 0;
+        }
+        out[n - 1 + (num * n)] = y[n - 1] / in[(n - 1) + ((n - 1) * n)];
+        for (int i = (int)n - 2; i >= 0; --i) {
+            for (size_t j = (size_t)i + 1; j < n; ++j) tmp1 += in[(size_t)i + (j * n)] * out[j + (num * n)];
+            out[(size_t)i + (num * n)] = (1 / in[(size_t)i + ((size_t)i * n)]) * (y[i] - tmp1);
+            tmp1 = 0;
+        }
+    }
+}
+
+/* SqrtReal (SplineFit.cpp:30-34): real part of the complex square root of x + 0i (glibc csqrt:
+ * sqrt(x) for x > 0, +0 for x <= 0 and x = -inf, NaN for NaN). */
+static double spl_sqrt_real(double x)
+{
+    if (x != x) return x;
+    return x > 0 ? sqrt(x) : 0.0;
+}
+
+/* updateMuLambda (SplineFit.cpp:257-267). */
+static void spl_mu_lambda(double* mu, double* lambda, int up)
+{
+    if (up) {
+        const double a = (1 + (*lambda * 10)) / (1 + *lambda);
+        *mu = a < 1.3 ? 1.3 : a;
+        *lambda = 10 * *lambda;
+    } else {
+        *lambda = 0.1 * *lambda;
+        *mu = 1 + *lambda;
+    }
+}
+
+/* One pass of the model over the ROI at p: derivatives, and the callback per pixel. */
+static void spl_pass(const spl_ws* s, const double* p, const double* gv, double* err, double* jac, double* hes,
+                     double* fisher)
+{
+    double f[64], dfx[64], dfy[64], dfz[64], dudt[OR_SPL_N];
+    const double xr = -1 * (p[2] - (double)s->half + 0.5), yr = -1 * (p[3] - (double)s->half + 0.5);
+    const int xi = (int)floor(xr), yi = (int)floor(yr), zi = (int)floor(p[4]);
+    const double xc = xr - xi, yc = yr - yi, zc = p[4] - zi;
+    spl_delta3d(xc, yc, zc, f, dfx, dfy, dfz);
+    size_t px = 0, py = 0, pz = (size_t)zi;
+    for (size_t i = 0, idx = 0; i < s->roi; ++i) {
+        px = (size_t)xi + i + s->off;
+        for (size_t j = 0; j < s->roi; ++j, ++idx) {
+            py = (size_t)yi + j + s->off;
+            const double model = spl_deriv(s, &px, &py, &pz, f, dfx, dfy, dfz, p, dudt);
+            if (fisher) {
+                for (int i2 = 0; i2 < OR_SPL_N; ++i2)
+                    for (int j2 = i2; j2 < OR_SPL_N; ++j2) {
+                        fisher[(i2 * OR_SPL_N) + j2] += dudt[j2] * dudt[i2] / model;
+                        fisher[(j2 * OR_SPL_N) + i2] = fisher[(i2 * OR_SPL_N) + j2];
+                    }
+                continue;
+            }
+            /* updateJacobianHessian (SplineFit.cpp:243-254) */
+            const double data = gv[idx];
+            if (data > 0) *err = *err + (2 * (model - data - (data * log(model / data))));
+            else *err = *err + (2 * model);
+            const double t1 = 1 - (data / model);
+            for (int a = 0; a < OR_SPL_N; ++a) jac[a] += t1 * dudt[a];
+            const double t2 = data / (model * model);
+            for (int a = 0; a < OR_SPL_N; ++a)
+                for (int b = a; b < OR_SPL_N; ++b) {
+                    hes[(a * OR_SPL_N) + b] += t2 * dudt[a] * dudt[b];
+                    hes[(b * OR_SPL_N) + a] = hes[(a * OR_SPL_N) + b];
+                }
+        }
+    }
+}
+
+/* SplineFit::Fit (SplineFit.cpp:52-216). */
+static void spline_fit_region(const uint16_t* img, size_t w, size_t h, const double* fp, const or_region* info,
+                              double* pt)
+{
+    spl_ws s;
+    spl_parse(fp, &s);
+    const size_t ux = to_size(info->x), uy = to_size(info->y);
+    if (s.half > ux || ux >= w - s.half || s.half > uy || uy >= h - s.half) {
+        fill_row(5, info, pt, info->x, info->y, info->z, -1, -1, -1, -1, -1, -1, -1);
+        return;
+    }
+    double z = s.initZ;
+    double lambda = 0.1, err = DBL_MAX, mu = 1 + lambda;
+    double hes[25], l[25], u[25], jac[OR_SPL_N], jump[OR_SPL_N], upd[OR_SPL_N] = {0, 0, 0, 0, 0};
+    double gvbuf[225];
+    double* gv = s.n <= 225 ? gvbuf : (double*)malloc(sizeof(double) * s.n);
+    /* preCompute (Fitting.cpp:109-123) */
+    double minI = DBL_MAX, maxI = -DBL_MAX;
+    const long half = (long)s.half;
+    for (long i = -half; i <= half; ++i)
+        for (long j = -half; j <= half; ++j) {
+            const size_t k = (size_t)(half + ((i + half) * (long)s.roi) + j);
+            gv[k] = (double)img[ux + j + ((uy + i) * w)];
+            if (gv[k] < minI) minI = gv[k];
+            if (maxI < gv[k]) maxI = gv[k];
+        }
+    /* CenterOfMass (Matrix.cpp:245-269): ROI-local centre */
+    double x = 0.0, y = 0.0, sum = 0.0;
+    for (size_t i = 0, idx = 0; i < s.roi; ++i)
+        for (size_t j = 0; j < s.roi; ++j, ++idx) {
+            const double v = gv[idx];
+            y += v * (double)i;
+            x += v * (double)j;
+            sum += v;
+        }
+    if (sum > 0) { x /= sum; y /= sum; }
+    else { x = (double)s.roi / 2.0; y = (double)s.roi / 2.0; }
+    /* initP (Fitting.cpp:159-171) and the spline adjustments (SplineFit.cpp:73-77) */
+    const double roiC = (s.roiParam - 1) / 2.0, roiD = s.roiParam / 4.0;
+    const double dmin = roiC - roiD, dmax = roiC + roiD;
+    const double lo[OR_SPL_N] = {-0.01, -1, dmin, dmin, 0};
+    double hi[OR_SPL_N] = {0.01, 1, dmax, dmax, 1};
+    double p[OR_SPL_N] = {minI, maxI - minI, x, y, z};
+    p[1] *= s.ratio;
+    hi[4] = (double)s.sz;
+    for (int i = 0; i < OR_SPL_N; ++i) p[i] = clamp_d(p[i], lo[i], hi[i]);
+    jump[0] = 20.0 < p[1] ? p[1] : 20.0;
+    jump[1] = 100.0 < p[1] ? p[1] : 100.0;
+    jump[2] = 1.0;
+    jump[3] = 1.0;
+    jump[4] = (double)s.sz / 3.0 < 2.0 ? 2.0 : (double)s.sz / 3.0;
+
+    for (size_t iteration = 0;;) {
+        iteration++;
+        const double oldErr = err, oldLambda = lambda;
+        double oldP[OR_SPL_N], oldUpd[OR_SPL_N];
+        memcpy(oldP, p, sizeof(p));
+        memcpy(oldUpd, upd, sizeof(upd));
+        for (int i = 0; i < OR_SPL_N; ++i) { jac[i] = 0.0; upd[i] = 0.0; }
+        for (int i = 0; i < 25; ++i) { hes[i] = 0.0; l[i] = 0.0; u[i] = 0.0; }
+        spl_pass(&s, p, gv, &err, jac, hes, NULL);
+        for (int i = 0; i < OR_SPL_N; ++i) hes[(i * OR_SPL_N) + i] = hes[(i * OR_SPL_N) + i] * mu;
+        if (spl_cholesky(hes, l, u)) {
+            spl_lu_eval(l, u, jac, upd);
+            for (int i = 0; i < OR_SPL_N; ++i) {
+                if (fabs(oldUpd[i]) > OR_EPSILON && upd[i] / oldUpd[i] < -0.5) jump[i] *= 0.5;
+                const double scale = upd[i] / (1.0 + (fabs(upd[i]) / jump[i]));
+                p[i] -= scale;
+                p[i] = clamp_d(p[i], lo[i], hi[i]);
+                upd[i] = scale;
+            }
+            if (err > oldErr) {
+                err = oldErr;
+                memcpy(p, oldP, sizeof(p));
+                memcpy(upd, oldUpd, sizeof(upd));
+                lambda = oldLambda;
+                spl_mu_lambda(&mu, &lambda, 1);
+            } else {
+                spl_mu_lambda(&mu, &lambda, 0);
+            }
+        } else {
+            err = oldErr;
+            memcpy(p, oldP, sizeof(p));
+            memcpy(upd, oldUpd, sizeof(upd));
+            lambda = oldLambda;
+            spl_mu_lambda(&mu, &lambda, 1);
+        }
+        if (iteration >= 75) break;                                       /* Fitting.hpp: N_ITER */
+        if (iteration > 2 && (err < 1e-6 || fabs(err - oldErr) / err < 1e-6)) break;
+    }
+    /* CRLB: inverse of the Fisher information at p (SplineFit.cpp:166-203) */
+    double m[25], mInv[25], crlb[OR_SPL_N];
+    for (int i = 0; i < 25; ++i) { m[i] = 0.0; mInv[i] = 0.0; }
+    spl_pass(&s, p, gv, NULL, NULL, NULL, m);
+    for (int i = 0; i < OR_SPL_N; ++i)
+        if (fabs(m[(i * OR_SPL_N) + i]) < DBL_EPSILON) m[(i * OR_SPL_N) + i] = OR_SPL_EPS2;
+    spl_inverse_lu(m, mInv);
+    for (int i = 0; i < OR_SPL_N; ++i) crlb[i] = spl_sqrt_real(mInv[(i * OR_SPL_N) + i]);
+    const double rssXY = (crlb[2] + crlb[3]) / 2, rssZ = s.deltaZ * crlb[4];
+    const double fx = p[2] - (double)s.half + info->x;
+    const double fy = p[3] - (double)s.half + info->y;
+    const double fz = s.deltaZ * (p[4] - s.initZ);
+    fill_row(5, info, pt, fx, fy, fz, crlb[0], crlb[1], 0, 0, 0, rssXY, rssZ);
+    if (gv != gvbuf) free(gv);
+}
+
 void or_fit_region(int mode, const uint16_t* img, size_t w, size_t h, const double* fp, int maxIter,
                    const or_region* info, double* pt)
 {
     if (mode == 0) { fill_row(0, info, pt, info->x, info->y, info->z, 0, 0, -1, -1, -1, -1, -1); return; }
+    if (mode == 5) { spline_fit_region(img, w, h, fp, info, pt); return; }
     lm_ws ws;
     ws.mode = mode;
     ws.np = mode == 1 ? 4 : mode == 2 ? 5 : mode == 3 ? 6 : 7;      /* Fitting.cpp:14-40 */
@@ -792,7 +1122,7 @@ static int row_cmp(const void* pa, const void* pb)
     return c != 0 ? c : weak_cmp(a[4], b[4]);
 }
 
-static int fit_mode_of(uint64_t fit) { return (fit >= 1 && fit <= 4) ? (int)fit : 0; }
+static int fit_mode_of(uint64_t fit) { return (fit >= 1 && fit <= 5) ? (int)fit : 0; }
 
 /* FrameProcessor::Process for one plane: rows (capacity-truncated in label order, then sorted). */
 static size_t process_frame(const uint16_t* img, size_t w, size_t h, double thr, double ratio, int mode,
@@ -813,7 +1143,6 @@ uint64_t or_localization(const uint16_t* input, double* output, uint64_t outputS
 {
     const size_t maxPts = planes == 0 ? 0 : (outputSize / OR_COLS) / planes;   /* Palm.cpp:51 */
     if (maxPts == 0 || width == 0 || height == 0) return 0;                   /* Palm.cpp:57 */
-    if (fit == 5) { fprintf(stderr, "oracle: spline fit (mode 5) is not restated\n"); return 0; }
     const size_t npx = (size_t)width * height;
     const int mode = fit_mode_of(fit);
     size_t* counts = (size_t*)calloc(planes, sizeof(size_t));
@@ -852,7 +1181,7 @@ uint64_t or_fitting(const uint16_t* input, const double* pos, uint64_t nPoints, 
 {
     (void)outputSize;                                                           /* Palm.cpp:74 */
     if (nPoints == 0) return 0;
-    if (fit < 1 || fit > 4) { fprintf(stderr, "oracle: Fitting supports modes 1-4\n"); return 0; }
+    if (fit < 1 || fit > 5) { fprintf(stderr, "oracle: Fitting supports modes 1-5\n"); return 0; }
     const size_t npx = (size_t)width * height;
     size_t* local = (size_t*)malloc(sizeof(size_t) * nPoints);
     size_t* pc = (size_t*)calloc(planes + 1, sizeof(size_t));

@@ -77,9 +77,11 @@ def _stack(s):
 
 
 def _params(fp):
-    p = np.zeros(4)
+    """fit_params as the exports read them: at least 4 values (zero-padded); longer arrays (the spline
+    fit's [roi, sizes, deltaZ, coefficients]) are passed whole."""
     a = np.asarray(fp, np.float64).ravel()
-    p[:min(4, a.size)] = a[:4]
+    p = np.zeros(max(4, a.size))
+    p[:a.size] = a
     return p
 
 
@@ -195,3 +197,33 @@ def ref_astig_estimate(sigmas, pixel_size, model, z_max):
     out = np.empty_like(s)
     R.ref_astig_estimate(s.ctypes.data, out.ctypes.data, s.shape[0], float(pixel_size), m.ctypes.data, float(z_max))
     return out
+
+
+def ref_localization(stack, threshold, ratio=0.0, fit=2, fit_params=(7, 1.3, 1.3, 0), output_size=None):
+    """The compiled reference's Localization (None without oracle/_ref)."""
+    R = ref_lib()
+    if R is None:
+        return None
+    s = _stack(stack)
+    z, h, w = s.shape
+    cap = default_capacity(z, h, w) if output_size is None else int(output_size)
+    out = np.empty(max(cap, 1))
+    p = _params(fit_params)
+    n = R.ref_localization(s.ctypes.data, out.ctypes.data, cap, h, w, z, float(threshold), float(ratio), int(fit),
+                           p.ctypes.data)
+    return out[:n].reshape(-1, NCOL).copy()
+
+
+def ref_fitting(stack, positions, fit=2, fit_params=(7, 1.3, 1.3, 0)):
+    """The compiled reference's Fitting (None without oracle/_ref)."""
+    R = ref_lib()
+    if R is None:
+        return None
+    s = _stack(stack)
+    z, h, w = s.shape
+    pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 6)
+    out = np.empty(max(pos.shape[0] * NCOL, 1))
+    p = _params(fit_params)
+    n = R.ref_fitting(s.ctypes.data, pos.ctypes.data, pos.shape[0], out.ctypes.data, out.size, h, w, z, int(fit),
+                      p.ctypes.data)
+    return out[:n].reshape(-1, NCOL).copy()

@@ -107,7 +107,10 @@ private:
     void detect_chunk(Lane& L, const uint16_t* d_frames, size_t F, int H, int W, double thr, double ratio,
                       uint64_t maxPoints, size_t evBase, const uint32_t* d_map, bool resolve);
     void grow(DevBuf& buf, size_t need, size_t keep, hipStream_t s);
-    FitParams fit_params(uint64_t fit, const double* fp) const;
+    // Fit parameters of a call; for mode 5 it parses the spline model and uploads its coefficients.
+    FitParams fit_params(uint64_t fit, const double* fp);
+    SplineParams spline_;
+    DevBuf splineRaw_, splineCv_;
 };
 
 } // namespace pt

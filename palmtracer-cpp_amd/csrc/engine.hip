@@ -213,9 +213,56 @@ uint32_t* Lane::host(size_t words)
     return pinned;
 }
 
-FitParams Engine::fit_params(uint64_t fit, const double* fp) const
+namespace {
+// Double to size_t as the reference's x86-64 build converts (truncation; wraps for negatives;
+// 2^63 for NaN and out-of-range values).
+uint64_t host_to_size(double v)
+{
+    if (!(v == v) || v >= 9223372036854775808.0 || v <= -9223372036854775808.0) return 1ull << 63;
+    return (uint64_t)(long long)v;
+}
+} // namespace
+
+FitParams Engine::fit_params(uint64_t fit, const double* fp)
 {
     FitParams p{};
+    if (fit == 5) {
+        // SplineFit::parseParams (SplineFit.cpp:36-49), then the odd ROI side (Fitting.cpp:35-38).
+        if (!fp) throw Error("fit mode 5 needs the spline parameters");
+        SplineParams& sp = spline_;
+        sp = SplineParams{};
+        sp.roiParam = fp[0];
+        const uint64_t roiRaw = host_to_size(fp[0]);
+        sp.sx = host_to_size(fp[1]);
+        sp.sy = host_to_size(fp[2]);
+        sp.sz = host_to_size(fp[3]);
+        if (roiRaw < 1 || roiRaw > 4095) throw Error("fit mode 5: the ROI side must be in [1, 4095]");
+        const uint64_t kMaxSide = 1u << 16;
+        if (sp.sx < 1 || sp.sy < 1 || sp.sz < 1 || sp.sx > kMaxSide || sp.sy > kMaxSide || sp.sz > kMaxSide ||
+            sp.sx * sp.sy > (1ull << 26) / sp.sz)
+            throw Error("fit mode 5: spline sizes must be positive and hold at most 2^26 voxels");
+        const uint64_t ssize = sp.sx * sp.sy * sp.sz, sxy = sp.sx * sp.sy;
+        sp.initZ = (double)sp.sz / 2.0;
+        sp.deltaZ = fp[4];
+        const double* c = fp + 5;
+        const double center = c[(32 * ssize) + (sp.sz / 2 * sxy) + (sp.sy / 2 * sp.sx) + (sp.sx / 2)];
+        sp.ratio = std::fabs(center) < 2.220446049250313e-16 ? 1.0 : 4.0 / center;
+        sp.off = host_to_size(std::floor(((double)sp.sx + 1.0 - (double)roiRaw) / 2));
+        sp.roi = roiRaw % 2 == 0 ? roiRaw - 1 : roiRaw;
+        sp.half = sp.roi / 2;
+        const size_t bytes = (size_t)ssize * 64 * 8;
+        splineRaw_.ensure(bytes);
+        splineCv_.ensure(bytes);
+        PT_CHECK(hipMemcpyAsync(splineRaw_.p, c, bytes, hipMemcpyHostToDevice, stream));
+        launch_spline_transpose(splineRaw_.as<double>(), splineCv_.as<double>(), ssize, stream);
+        PT_CHECK(hipStreamSynchronize(stream));
+        sp.cv = splineCv_.as<double>();
+        p.mode = 5;
+        p.roi = (int)sp.roi;
+        p.maxIter = opt.maxIter;
+        p.spline = &spline_;
+        return p;
+    }
     p.mode = (fit >= 1 && fit <= 4) ? (int)fit : 0;
     long roi = 7;
     if (fp) {
@@ -276,7 +323,6 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
 {
     last = RunStats{};
     if (planes == 0 || maxPerPlane == 0 || H <= 0 || W <= 0) return 0;
-    if (fit == 5) throw Error("fit mode 5 (spline PSF) is not part of this library");
     if (planes >= 0xFFFFFFFFull) throw Error("too many planes");
     const size_t npx = (size_t)W * H;
     const size_t F = std::max<size_t>(1, std::min<size_t>(planes, std::min<size_t>(65535, opt.chunkPixels / npx)));
@@ -519,7 +565,7 @@ uint64_t Engine::fit_positions(const uint16_t* d_in, uint64_t planes, int H, int
                                uint64_t fit, const double* fitParams, double* d_out)
 {
     if (n == 0) return 0;
-    if (fit < 1 || fit > 4) throw Error("Fitting supports the Gaussian modes 1-4");
+    if (fit < 1 || fit > 5) throw Error("Fitting supports the Gaussian modes 1-4 and the spline mode 5");
     if (n > 0xFFFFFFFFull) throw Error("too many positions");
     // Per-plane local index in input order (lib/Palm.cpp:79-84).
     std::vector<uint32_t> local(n);

@@ -326,7 +326,10 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
             surf = r.surf;
             const uint64_t sx = to_size_f(r.x), sy = to_size_f(r.y), h64 = (uint64_t)half;
             // Border check with the reference's unsigned arithmetic (GaussianFit.cpp:51-56).
-            if (h64 > sx || sx >= (uint64_t)W - h64 || h64 > sy || sy >= (uint64_t)H - h64) {
+            // A half side beyond the frame (where the reference's unsigned bound wraps and it reads out
+            // of the image) gets the border row.
+            if (h64 > (uint64_t)W || h64 > (uint64_t)H || h64 > sx || sx >= (uint64_t)W - h64 || h64 > sy ||
+                sy >= (uint64_t)H - h64) {
                 row_fail(rows + (size_t)idx * kCols, MODE, r.x, r.y, r.seg, r.surf);
                 continue;  // still kFetch
             }
@@ -550,6 +553,11 @@ void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d
                 const uint32_t* frameMap, unsigned waves, const uint32_t* d_count)
 {
     if (nRoi == 0) return;
+    if (fp.mode == 5) {
+        if (!fp.spline) throw Error("launch_fit: fit mode 5 needs the spline model");
+        launch_spline_fit(d_frames, npx, W, H, d_rois, nRoi, *fp.spline, d_rows, s, frameMap, d_count);
+        return;
+    }
     if (fp64 || fp.mode < 1 || fp.mode > 4) {
         launch_fit_f64(d_frames, npx, W, H, d_rois, nRoi, fp, d_rows, s, d_passes, frameMap, d_count);
         return;

@@ -273,7 +273,10 @@ __device__ void fit_one(const uint16_t* img, int W, int H, const Roi& r, const F
     const uint64_t ux = to_size(r.x), uy = to_size(r.y);
     const uint64_t h64 = (uint64_t)half;
     // Border check (GaussianFit.cpp:51-56), with the reference's unsigned arithmetic.
-    if (h64 > ux || ux >= (uint64_t)W - h64 || h64 > uy || uy >= (uint64_t)H - h64) {
+    // A half side beyond the frame (where the reference's unsigned bound wraps and it reads out of the
+    // image) gets the border row.
+    if (h64 > (uint64_t)W || h64 > (uint64_t)H || h64 > ux || ux >= (uint64_t)W - h64 || h64 > uy ||
+        uy >= (uint64_t)H - h64) {
         fill_row(MODE, row, r.x, r.y, zero, r.seg, r.surf, -1, -1, -1, -1, -1, -1, -1);
         return;
     }

@@ -118,13 +118,23 @@ int64_t threshold_to_pmin(double thr);
 void launch_segment(const DetectBuffers& b, int W, int H, int frames, uint64_t maxPoints, double ratio, Roi* d_rois,
                     hipStream_t s, bool forceSequential = false, int exactPath = 0, bool resolve = true);
 
+// Spline PSF model of fit mode 5 (SplineFit.cpp:36-49): sizes, offsets and the device copy of the
+// coefficients in voxel-major order (cv[voxel * 64 + monomial]; the reference's array is
+// monomial-major).
+struct SplineParams {
+    uint64_t roi = 0, half = 0, off = 0, sx = 0, sy = 0, sz = 0;
+    double initZ = 0, deltaZ = 0, ratio = 1, roiParam = 0;
+    const double* cv = nullptr;
+};
+
 // Gaussian LM fit of nRoi ROIs (count read from device memory *d_count when non-null).
 struct FitParams {
-    int mode;        // 1..4 (0: no fit, rows carry the segmentation only)
+    int mode;        // 1..4 Gaussian, 5 spline (0: no fit, rows carry the segmentation only)
     int roi;         // odd ROI side
     int maxIter;     // Fitting::N_ITER (75 in the reference)
     double sigmaX, sigmaY, theta;
     int refill = 16; // FP32 fit: free lanes of a wave that trigger a refill round while others work
+    const SplineParams* spline = nullptr;  // mode 5 (host struct, read by the launcher)
 };
 // LM fit of nRoi ROIs into d_rows (18 columns each). fp64: the reference's arithmetic, one ROI per
 // lane; otherwise the FP32 persistent work-queue kernel, whose queue head is d_next (one word).
@@ -137,6 +147,13 @@ void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d
 void launch_fit_f64(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, uint32_t nRoi,
                     const FitParams& fp, double* d_rows, hipStream_t s, unsigned long long* d_passes,
                     const uint32_t* frameMap, const uint32_t* d_count = nullptr);
+
+// Spline fit (mode 5) of nRoi ROIs, one per lane in FP64 (count read on the device when d_count is set).
+void launch_spline_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, uint32_t nRoi,
+                       const SplineParams& sp, double* d_rows, hipStream_t s, const uint32_t* frameMap,
+                       const uint32_t* d_count);
+// Voxel-major copy of the reference's coefficient array (64 monomials x ssize voxels).
+void launch_spline_transpose(const double* d_raw, double* d_cv, uint64_t ssize, hipStream_t s);
 
 // Per-frame (Y, X) sort (rows of frame f at d_rows + roiBase[f]) into d_staged with the same layout;
 // Index is set, Id and Plane are left for launch_scatter.

@@ -77,9 +77,11 @@ def _stack(stack):
 
 
 def _params(fit_params):
-    p = np.zeros(4, np.float64)
+    """fit_params as a contiguous float64 array of at least 4 values (the Gaussian modes read 4; the
+    spline mode 5 reads [roi, sx, sy, sz, deltaZ] and 64 * sx * sy * sz coefficients)."""
     fp = np.asarray(fit_params, np.float64).ravel()
-    p[:min(4, fp.size)] = fp[:4]
+    p = np.zeros(max(4, fp.size), np.float64)
+    p[:fp.size] = fp
     return p
 
 

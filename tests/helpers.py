@@ -114,3 +114,26 @@ def rank_labels(labels):
     lut = np.zeros(int(labels.max()) + 1 if labels.size else 1, np.uint32)
     lut[u] = np.arange(1, len(u) + 1, dtype=np.uint32)
     return lut[labels]
+
+
+def spline_params(roi=7, size=(11, 11, 21), delta_z=50.0, sigma=1.3, seed=0, noise=1e-3):
+    """fit_params of the spline fit (fit mode 5): [roi, sx, sy, sz, deltaZ, coefficients], the
+    coefficients of a separable Gaussian PSF interpolated per axis by cubic Hermite polynomials
+    (lib/fit/SplineFit.cpp:36-49 layout: coefficient q of voxel v at q * sx*sy*sz + v, monomial
+    x^(q % 4) y^((q / 4) % 4) z^(q / 16), v = (z * sy + y) * sx + x), with a small seeded
+    perturbation so that no two coefficients are tied."""
+    sx, sy, sz = size
+
+    def axis(n, c, s):
+        t = np.arange(n + 1, dtype=np.float64)
+        g = np.exp(-(t - c) ** 2 / (2 * s * s))
+        d = -(t - c) / (s * s) * g
+        g0, g1, d0, d1 = g[:-1], g[1:], d[:-1], d[1:]
+        return np.stack([g0, d0, 3 * (g1 - g0) - 2 * d0 - d1, 2 * (g0 - g1) + d0 + d1], axis=1)
+
+    px = axis(sx, (sx - 1) / 2, sigma)
+    py = axis(sy, (sy - 1) / 2, sigma)
+    pz = axis(sz, (sz - 1) / 2, sz / 3)
+    coef = np.einsum("za,yb,xc->abczyx", pz, py, px).reshape(64, sz * sy * sx)
+    coef = coef * (1.0 + noise * np.random.default_rng(seed).standard_normal(coef.shape))
+    return np.concatenate([[roi, sx, sy, sz, delta_z], coef.ravel()])

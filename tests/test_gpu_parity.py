@@ -483,3 +483,57 @@ def test_astig_estimate_device_entry(pt_gpu, oracle_mod):
     k = rng.choice(n, 4096, replace=False)
     assert np.array_equal(got[k], oracle_mod.astig_estimate(sig[k], AST_PIXEL, AST_REF_MODEL, 600.0))
     assert np.array_equal(got[:1000], pt_gpu.astig_estimate(sig[:1000], AST_PIXEL, AST_REF_MODEL, 600.0))
+
+
+# ------------------------------------------------------------------ spline fit (mode 5) ----------
+# The spline fit runs in FP64 with the reference's operations in its order (no contraction, correctly
+# rounded sqrt and division), so its rows equal the oracle's (itself bit-exact to the compiled
+# reference); the assertion allows the fit tolerance on the fitted columns for a device log ulp.
+def assert_spline_rows(got, exp):
+    assert got.shape == exp.shape and got.shape[0] > 0
+    bad = fit_mismatches(got, exp)
+    assert bad == [], bad[:5]
+    same = np.all((got == exp) | (np.isnan(got) & np.isnan(exp)), axis=1)
+    assert same.mean() >= 0.99, (same.mean(), fit_errors(got, exp))
+
+
+@pytest.mark.parametrize("size,roi", [((11, 11, 21), 7), ((13, 13, 9), 9), ((9, 9, 5), 7), ((11, 11, 21), 8)])
+def test_spline_localization_matches_oracle(pt_gpu, oracle_mod, size, roi):
+    import synth_np
+    from helpers import spline_params
+    st = synth_np.frames(3, 96, 128, 14, seed=21)
+    fp = spline_params(roi=roi, size=size, seed=5)
+    thr = oracle_mod.autothreshold(st[0])
+    got = pt_gpu.localization(st, thr, 0.0, 5, fp)
+    exp = oracle_mod.localization(st, thr, 0.0, 5, fp)
+    assert_spline_rows(got, exp)
+
+
+def test_spline_fitting_matches_oracle(pt_gpu, oracle_mod, ref_vectors):
+    from helpers import spline_params
+    st = ref_vectors["dense"]
+    pos = ref_vectors["dense_positions"]
+    fp = spline_params()
+    got = pt_gpu.fitting(st, pos, 5, fp)
+    exp = oracle_mod.fitting(st, pos, 5, fp)
+    assert np.array_equal(got[:, [ID, PLANE, INDEX]], exp[:, [ID, PLANE, INDEX]])
+    assert_spline_rows(got, exp)
+
+
+def test_spline_localization_512(pt_gpu, oracle_mod, frames512):
+    """512 x 512 frames with the fusion: the spline rows of a chunked, multi-lane call."""
+    from helpers import spline_params
+    st, thr = frames512[0][:6], frames512[1]
+    fp = spline_params(seed=3)
+    got = pt_gpu.localization(st, thr, 10.0, 5, fp)
+    exp = oracle_mod.localization(st, thr, 10.0, 5, fp)
+    assert_spline_rows(got, exp)
+
+
+def test_spline_bad_sizes_are_rejected(pt_gpu, capfd):
+    """A spline size of 0 (where the reference reads out of bounds) returns no rows and says why."""
+    st = np.full((1, 32, 32), 100, np.uint16)
+    st[0, 16, 16] = 5000
+    got = pt_gpu.localization(st, 10.0, 0.0, 5, [7.0, 0.0, 11.0, 11.0, 50.0])
+    assert got.shape[0] == 0
+    assert "spline sizes" in capfd.readouterr().err

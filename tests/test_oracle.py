@@ -382,3 +382,44 @@ def test_astig_estimate_bit_exact_vs_compiled_reference(oracle_mod):
             pytest.skip("compiled reference not built (oracle/_ref)")
         got = oracle_mod.astig_estimate(sig, AST_PIXEL, model, zmax)
         assert np.array_equal(got, exp, equal_nan=True)
+
+
+# ---------------------------------------------------------------- spline fit (mode 5) -----------
+def test_spline_fit_bit_exact_vs_compiled_reference(oracle_mod):
+    """SplineFit (lib/fit/SplineFit.cpp) restated: Localization and Fitting with fit mode 5 equal the
+    compiled reference's rows bit for bit (NaN where the reference produces NaN)."""
+    import synth_np
+    from helpers import spline_params
+    st = synth_np.frames(3, 96, 128, 14, seed=21)
+    fp = spline_params()
+    thr = oracle_mod.autothreshold(st[0])
+    exp = oracle_mod.ref_localization(st, thr, 0.0, 5, fp)
+    if exp is None:
+        pytest.skip("compiled reference not built (oracle/_ref)")
+    got = oracle_mod.localization(st, thr, 0.0, 5, fp)
+    assert got.shape == exp.shape and got.shape[0] > 20
+    assert np.array_equal(got, exp, equal_nan=True)
+    pos = np.stack([got[:, ID], got[:, PLANE], got[:, X] + 0.3, got[:, Y] - 0.2, got[:, 15], got[:, 16]], axis=1)
+    exp2 = oracle_mod.ref_fitting(st, pos, 5, fp)
+    got2 = oracle_mod.fitting(st, pos, 5, fp)
+    assert np.array_equal(got2, exp2, equal_nan=True)
+
+
+@pytest.mark.parametrize("size,roi", [((11, 11, 21), 7), ((13, 13, 9), 9), ((9, 9, 5), 7), ((11, 11, 21), 8),
+                                      ((7, 7, 4), 9)])
+def test_spline_fit_shapes_and_crlb(oracle_mod, size, roi):
+    """Other spline and ROI sizes (the offset between them, the clamped spline indices) against the
+    compiled reference; the CRLB columns are square roots (>= 0) where finite."""
+    import synth_np
+    from helpers import spline_params
+    st = synth_np.frames(2, 64, 64, 8, seed=33)
+    fp = spline_params(roi=roi, size=size, seed=5)
+    thr = oracle_mod.autothreshold(st[0])
+    got = oracle_mod.localization(st, thr, 0.0, 5, fp)
+    exp = oracle_mod.ref_localization(st, thr, 0.0, 5, fp)
+    if exp is not None:
+        assert np.array_equal(got, exp, equal_nan=True)
+    fitted = got[got[:, SX] == 0]                   # rows of the fit (border rows carry -1)
+    for c in (11, 13, 14):                         # CRLB columns: QualityXY, Amplitude, Offset
+        v = fitted[:, c]
+        assert np.all(v[np.isfinite(v)] >= 0)
