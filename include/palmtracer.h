@@ -36,7 +36,9 @@
  *  Amplitude, Offset, SegIntensity, Surface, Circularity] (lib/Constants.hpp:31-51), plane-major,
  * each plane sorted by (Y, X). At most (outputSize / 18) / planes rows per plane, kept in label
  * order (lib/fit/Fitting.cpp:44-58). fit: 0 none, 1 sigma fixed, 2 sigma free, 3 sigmaX/sigmaY,
- * 4 sigmaX/sigmaY/theta; fit_params = [roi, sigmaX, sigmaY, theta]. watershedRatio <= 0 keeps the
+ * 4 sigmaX/sigmaY/theta, 5 spline PSF with the EMCCD Poisson likelihood (lib/fit/SplineFit.cpp:36-216);
+ * fit_params = [roi, sigmaX, sigmaY, theta] for 1-4, [roi, spline sizes x, y, z, deltaZ (nm), then
+ * 64 * x * y * z coefficients, monomial-major] for 5. watershedRatio <= 0 keeps the
  * maxima separate (reference value 0); (int)ratio >= 1 enables basin fusion.
  * Returns the number of doubles written. */
 PT_API uint64_t PT_CALL Localization(const uint16_t* input, double* output, uint64_t outputSize, uint64_t height,
@@ -45,7 +47,8 @@ PT_API uint64_t PT_CALL Localization(const uint16_t* input, double* output, uint
 
 /* Replaces Fitting (CPU/main.cpp:141-148 -> lib/Palm.cpp:73-119). positions: nPoints rows
  * [Id, Plane(1-based), X, Y, SegIntensity, Surface] (lib/Constants.hpp:56-64). Rows are written in
- * input order; outputSize is ignored as in the reference. fit in 1..4. Returns 18 * nPoints. */
+ * input order; outputSize is ignored as in the reference. fit in 1..5, fit_params as for
+ * Localization. Returns 18 * nPoints. */
 PT_API uint64_t PT_CALL Fitting(const uint16_t* input, const double* positions, uint64_t nPoints, double* output,
                                 uint64_t outputSize, uint64_t height, uint64_t width, uint64_t planes, uint64_t fit,
                                 const double* fit_params);
@@ -122,7 +125,8 @@ PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint6
  * "force_sequential_watershed" (1: every frame through the exact frame flood), "force_wave_flood"
  * (1: that flood without the rank flood), "force_heap_watershed" (1: that flood with the heap
  * restatement of the reference), "fit_waves" (resident waves of the
- * persistent FP32 fit, 0: default). Returns 1 on success. */
+ * persistent FP32 fit, 0: default), "fit_refill" (free lanes of an FP32 fit wave that start a refill
+ * round, 1-64, default 16). Returns 1 on success. */
 PT_API int PT_CALL pt_set_option(const char* name, double value);
 
 /* Counters of the last call: frames, chunks, rows, frames flagged for the exact frame flood,
