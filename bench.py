@@ -78,6 +78,119 @@ def cpu_baseline(stack_host, thr, fit_params, budget_s):
             "sample": f"first {nf} frames of the same stack ({rows} rows in {dt:.2f} s), OpenMP over frames"}
 
 
+def serve_latency(pt, torch, seconds, slots, n_cams, fit_params):
+    """Streamed load: n_cams simulated cameras, each with a fixed frame size (256^2, 512^2, 1024^2
+    in turn), a frame rate uniform in [100, 1000] fps and chunks of 10-100 frames (seeded), the
+    headline workload's emitter density (200 per 512^2 frame, scaled by area). Chunks arrive as Poisson
+    processes (mean rate = fps / chunk frames) and are submitted to a pt_server with `slots` slots; a
+    chunk's latency is the time from its arrival to its rows in HBM, counted once per frame."""
+    import random
+    import threading
+    import numpy as np
+    rng = random.Random(0x5EED0005)
+    cams = []
+    for c in range(n_cams):
+        size = (256, 512, 1024)[c % 3]
+        fps = rng.uniform(100.0, 1000.0)
+        chunk = rng.randint(10, 100)
+        emitters = size * size * 200 // (512 * 512)
+        pool = torch.empty((128, size, size), dtype=torch.uint16, device="cuda")
+        pt.synth_device(pool, seed=0x5EED1000 + c, emitters=emitters)
+        thr = pt.autothreshold_device(pool[0])
+        mpp = 2 * emitters + 64
+        outs = [torch.empty(chunk * mpp * 18, dtype=torch.float64, device="cuda") for _ in range(8)]
+        cams.append(dict(size=size, fps=fps, chunk=chunk, pool=pool, thr=thr, mpp=mpp, outs=outs, off=0, k=0,
+                         busy=[None] * len(outs)))
+    torch.cuda.synchronize()
+    budget = 32 << 30
+    srv = pt.Server(slots=slots, budget_bytes=budget, chunk_pixels=1 << 25)
+
+    done = set()
+
+    def take(cam):
+        """The camera's next chunk of frames and a free output buffer (one whose last request is done)."""
+        n = cam["chunk"]
+        if cam["off"] + n > cam["pool"].shape[0]:
+            cam["off"] = 0
+        d = cam["pool"][cam["off"]: cam["off"] + n]
+        cam["off"] += n
+        b = cam["k"] % len(cam["outs"])
+        cam["k"] += 1
+        while cam["busy"][b] is not None and cam["busy"][b] not in done:
+            time.sleep(0.0002)
+        return d, cam["outs"][b], b
+
+    def submit(cam):
+        d, o, b = take(cam)
+        rid = srv.submit(d, o, cam["thr"], 0.0, 2, fit_params, cam["mpp"])
+        cam["busy"][b] = rid
+        return rid
+
+    # Warm-up: every slot sees the largest chunks, so that no buffer grows during the timed phase.
+    warm = []
+    for cam in sorted(cams, key=lambda c: -c["size"] * c["size"] * c["chunk"])[: 2 * slots]:
+        warm.append(submit(cam))
+    for r in warm:
+        srv.wait(r)
+        done.add(r)
+
+    arrivals = []
+    for ci, cam in enumerate(cams):
+        t, lam = 0.0, cam["fps"] / cam["chunk"]
+        while True:
+            t += rng.expovariate(lam)
+            if t >= seconds:
+                break
+            arrivals.append((t, ci))
+    arrivals.sort()
+    submitted = []
+    base = srv.now() + 50.0
+    t_base = time.perf_counter() + 0.05
+
+    def scheduler():
+        for t, ci in arrivals:
+            delay = t_base + t - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            rid = submit(cams[ci])
+            submitted.append((rid, base + 1e3 * t, cams[ci]["chunk"]))
+
+    th = threading.Thread(target=scheduler)
+    th.start()
+    lat, weights, rows, lags, i = [], [], 0, [], 0
+    while i < len(arrivals):
+        if i >= len(submitted):
+            time.sleep(0.0002)
+            continue
+        rid, planned, n = submitted[i]
+        r, (t_sub, t_start, t_done, _) = srv.wait(rid)
+        done.add(rid)
+        lat.append(t_done - planned)
+        lags.append(t_sub - planned)
+        weights.append(n)
+        rows += r
+        i += 1
+    th.join()
+    t_end = srv.now()
+    srv.close()
+    lat, weights = np.array(lat), np.array(weights, dtype=np.float64)
+    order = np.argsort(lat)
+    cw = np.cumsum(weights[order]) / weights.sum()
+    pct = lambda q: float(lat[order][min(int(np.searchsorted(cw, q)), len(lat) - 1)])
+    frames = int(weights.sum())
+    px = sum(cams[ci]["chunk"] * cams[ci]["size"] ** 2 for _, ci in arrivals)
+    return {
+        "p50_ms": pct(0.50), "p99_ms": pct(0.99), "max_ms": float(lat.max()),
+        "latency": "chunk arrival to its rows in HBM, per frame",
+        "requests": len(arrivals), "frames": frames, "rows": rows,
+        "offered_frames_per_s": sum(c["fps"] for c in cams), "offered_gpx_per_s": px / seconds / 1e9,
+        "localizations_per_s": rows / ((t_end - base) * 1e-3),
+        "cameras": n_cams, "frame_sizes": [c["size"] for c in cams], "emitters_per_512x512": 200,
+        "slots": slots, "budget_gb": budget >> 30,
+        "seconds": seconds, "max_submit_lag_ms": float(max(lags)),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +202,12 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=4.0,
+                    help="length of the streamed-load phase (0: skip it)")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-cameras", type=int, default=16)
+    ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
+                    help="library option (pt_set_option) applied before the run, e.g. detect_rows=103")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (calibrates the HBM byte "
                          "counters of rocprofv3 --pmc against a transfer of known size) and exit")
@@ -121,6 +240,11 @@ def main():
     pt.set_option("fit_fp64", 1 if args.precision == "fp64" else 0)
     pt.set_option("max_iter", 75)
     pt.set_option("profile", 1)
+    options = {}
+    for kv in args.set:
+        name, value = kv.split("=", 1)
+        pt.set_option(name, float(value))
+        options[name] = float(value)
 
     for _ in range(args.warmup):
         pt.localization_device(stack, out, thr, 0.0, 2, fit_params, max_per_plane)
@@ -168,6 +292,10 @@ def main():
                 "bytes_per_launch": bytes_l}
     stage_ms = {k: v / args.steps for k, v in stages.items() if k.endswith("_ms")}
 
+    serving = None
+    if args.serve_seconds > 0:
+        serving = serve_latency(pt, torch, args.serve_seconds, args.serve_slots, args.serve_cameras, fit_params)
+
     cpu = None
     if not args.no_cpu_baseline:
         host = stack[: min(F, 4000)].cpu().numpy()
@@ -191,12 +319,13 @@ def main():
                                f"(AutoThreshold of frame 0)",
                    "frames": F, "height": S, "width": S, "emitters_per_frame": args.emitters, "fit": 2,
                    "roi": 7, "threshold": thr, "rows": rows, "fitted_rows": fitted, "seed": seed,
-                   "stage_ms_per_step": stage_ms,
+                   "stage_ms_per_step": stage_ms, "options": options,
                    "watershed_frames": {k: last[k] for k in ("flagged_frames", "deferred_frames", "flag_unproven",
                                                              "flag_overflow", "rank_too_big", "rank_equal_keys",
                                                              "rank_unsettled", "rank_two_orders", "heap_frames",
                                                              "multi_components")}},
         "roofline": roof,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

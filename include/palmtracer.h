@@ -99,6 +99,30 @@ PT_API void PT_CALL Astigmatism3DCalibration(const double* input, double* output
 PT_API int PT_CALL pt_astig_estimate_device(const double* d_input, double* d_output, uint64_t size, double pixelSize,
                                             const double* model, double zMax);
 
+/* Serving loop for several streams of frame chunks (cameras). `slots` independent engines (own
+ * streams and chunk buffers; chunkPixels frames x pixels per device chunk, 0: the default) each run
+ * one request at a time, so requests on different slots overlap on the GPU. Requests leave the queue
+ * in arrival order when a slot is free and the HBM bytes of the requests in flight (input frames +
+ * output capacity) plus their own fit in budgetBytes. A request is Localization over device memory
+ * (as pt_localization_device; nFitParams doubles of fit_params are copied at submission, so the
+ * caller may reuse them); its result equals that call's whatever else is in flight. Options
+ * (pt_set_option) in force at creation apply to every slot. Returns NULL on failure. */
+PT_API void* PT_CALL pt_server_create(uint64_t slots, uint64_t budgetBytes, uint64_t chunkPixels);
+/* Queues a request; d_input and d_output must stay valid until pt_server_wait returns for it.
+ * Returns the request id (>= 0) or -1 (bad arguments, request larger than the budget). */
+PT_API int64_t PT_CALL pt_server_submit(void* server, const uint16_t* d_input, uint64_t planes, uint64_t height,
+                                        uint64_t width, double thresholdVal, double watershedRatio, uint64_t fit,
+                                        const double* fit_params, uint64_t nFitParams, double* d_output,
+                                        uint64_t outputCapDoubles, uint64_t maxPerPlane);
+/* Blocks until the request is done. doublesWritten: doubles of rows in d_output; times (4 doubles):
+ * submission, start and completion in ms on the server's clock, and the slot that ran it. Returns 1
+ * on success, 0 if the request failed (message on stderr). Each id is waited for once. */
+PT_API int PT_CALL pt_server_wait(void* server, int64_t request, uint64_t* doublesWritten, double* times);
+/* The server's clock (ms since creation). */
+PT_API double PT_CALL pt_server_now(void* server);
+/* Finishes the queued requests, then releases the slots. */
+PT_API void PT_CALL pt_server_destroy(void* server);
+
 /* AutoThreshold of one frame in device memory. */
 PT_API double PT_CALL pt_autothreshold_device(const uint16_t* d_input, uint64_t height, uint64_t width,
                                               const double* fit_params);

@@ -456,7 +456,7 @@ void launch_atrous_plane(const uint16_t* d_in, double* d_out, int W, int H, int 
 }
 
 void launch_atrous_detect(const uint16_t* d_in, const DetectBuffers& b, int W, int H, int frames, int64_t tmin,
-                          hipStream_t s, const uint32_t* frameMap)
+                          hipStream_t s, const uint32_t* frameMap, int segRows)
 {
     if (frames <= 0 || W <= 0 || H <= 0) return;
     if (frames > 65535) throw Error("launch_atrous_detect: at most 65535 frames per launch");
@@ -466,6 +466,7 @@ void launch_atrous_detect(const uint16_t* d_in, const DetectBuffers& b, int W, i
     // chip a few waves per SIMD when the chunk is small.
     int segH = 128;
     while (segH > 32 && (size_t)frames * strips * ceil_div(H, segH) < 8192) segH /= 2;
+    if (segRows > 0) segH = std::min(segRows, 1 << 20);
     const int segs = (int)ceil_div(H, segH);
     if ((size_t)strips * segs > 0x7FFFFFFF) throw Error("launch_atrous_detect: frame too large");
     const dim3 grid((unsigned)(strips * segs), (unsigned)frames);

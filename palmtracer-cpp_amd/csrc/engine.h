@@ -50,6 +50,7 @@ struct Options {
     int exactPath = 0;              // exact frame flood from: 0 rank flood, 1 wave-queue flood, 2 heap
     unsigned fitWaves = 0;          // persistent FP32 fit: resident waves (0: 16 per CU)
     int fitRefill = 16;             // persistent FP32 fit: free lanes that trigger a refill round
+    int detectRows = 0;             // rows per detection wave (0: chosen from the chunk size)
 };
 
 // One stream with its own detection buffers. The engine runs the chunks on the main lane and the
@@ -64,11 +65,18 @@ struct Lane {
     void prepare(size_t F, size_t npx);
     void zero();
     uint32_t* host(size_t words);
+    void release();   // device and pinned memory and the stream
 };
 
 class Engine {
 public:
+    // The process-wide engine of the C entry points (lives until the process ends).
     static Engine& get();
+    // An independent engine (own streams and buffers), e.g. one per serving slot.
+    Engine();
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
     std::mutex mu;
     Options opt;
     RunStats last;
@@ -90,7 +98,6 @@ public:
     DevBuf in, out, planeInfo;
 
 private:
-    Engine();
     Lane main_, mainB_, side_;   // two alternating lanes for the chunks of a call, one for deferred frames
     hipEvent_t sideEv_[2] = {nullptr, nullptr};  // side batch start / done
     DevBuf atMask_, atPart_, atRows_, passes_, allRois_, allBase_, sideRows_;

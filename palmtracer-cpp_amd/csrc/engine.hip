@@ -132,6 +132,33 @@ Engine::Engine()
     if (const char* v = std::getenv("PALMTRACER_CHUNK_PIXELS")) opt.chunkPixels = std::max<size_t>(1, std::strtoull(v, nullptr, 10));
 }
 
+Engine::~Engine()
+{
+    for (Lane* L : {&main_, &mainB_, &side_})
+        if (L->stream) (void)hipStreamSynchronize(L->stream);
+    for (Lane* L : {&main_, &mainB_, &side_}) L->release();
+    for (DevBuf* b : {&in, &out, &planeInfo, &atMask_, &atPart_, &atRows_, &passes_, &allRois_, &allBase_, &sideRows_,
+                      &run_, &splineRaw_, &splineCv_})
+        b->release();
+    for (hipEvent_t e : sideEv_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : fitEv_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : chunkEv_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : scanEv_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
+}
+
+void Lane::release()
+{
+    for (DevBuf* d : {&P, &lab, &fin, &aux, &mask, &seed, &perFrame, &regions, &fz, &order, &multi, &fb, &counters, &flag,
+                      &rf, &sbits, &rois, &rows, &staged, &frameMap, &fitNext})
+        d->release();
+    if (pinned) (void)hipHostFree(pinned);
+    pinned = nullptr;
+    pinnedWords = 0;
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+}
+
 void Lane::prepare(size_t F, size_t npx)
 {
     const size_t n = F * npx;
@@ -297,7 +324,7 @@ void Engine::detect_chunk(Lane& L, const uint16_t* d_frames, size_t F, int H, in
     L.zero();
     const bool prof = opt.profile && evBase != kNoEvents;
     if (prof) PT_CHECK(hipEventRecord(event(evBase + 0), st));
-    launch_atrous_detect(d_frames, L.b, W, H, (int)F, threshold_to_pmin(thr), st, d_map);
+    launch_atrous_detect(d_frames, L.b, W, H, (int)F, threshold_to_pmin(thr), st, d_map, opt.detectRows);
     if (prof) PT_CHECK(hipEventRecord(event(evBase + 1), st));
     L.rois.ensure(F * (npx / 4 + 2) * sizeof(Roi));
     // Deferred frames (d_map set) go through the exact frame flood.

@@ -97,15 +97,18 @@ enum Counter : int {
 __host__ __device__ inline size_t fb_slot_words(size_t npx) { return 5 * ((npx + 1) & ~size_t(1)); }
 __host__ __device__ inline size_t seed_words(size_t npx) { return (npx + 31) / 32; }
 // Rank flood: frames with at most kRankMax pixels in the flood (mask pixels but the seeds that are
-// never pushed). Work space per slot: pixel -> entry map, then 21 words per entry.
-constexpr uint32_t kRankMax = 16384;
-__host__ __device__ inline size_t rf_slot_words(size_t npx) { return ((npx + 1) & ~size_t(1)) + 21 * (size_t)kRankMax; }
+// never pushed); their sort keys are sorted in LDS up to kRankLds entries, in the work space beyond.
+// Work space per slot: pixel -> entry map, then 23 words per entry.
+constexpr uint32_t kRankMax = 65536;
+constexpr uint32_t kRankLds = 16384;
+__host__ __device__ inline size_t rf_slot_words(size_t npx) { return ((npx + 1) & ~size_t(1)) + 23 * (size_t)kRankMax; }
 
 // A trous
 void launch_atrous_plane(const uint16_t* d_in, double* d_out, int W, int H, int planes, int level, hipStream_t s);
-// frameMap (optional): chunk frame f reads input frame frameMap[f] (else f).
+// frameMap (optional): chunk frame f reads input frame frameMap[f] (else f). segRows: rows per wave
+// (0: chosen from the number of frames).
 void launch_atrous_detect(const uint16_t* d_in, const DetectBuffers& b, int W, int H, int frames, int64_t tmin,
-                          hipStream_t s, const uint32_t* frameMap = nullptr);
+                          hipStream_t s, const uint32_t* frameMap = nullptr, int segRows = 0);
 int64_t threshold_to_pmin(double thr);
 
 // Segmentation (ratio <= 0: watershed without fusion). Produces ROIs for frames [0, F) at

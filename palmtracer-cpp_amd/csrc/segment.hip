@@ -874,7 +874,7 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
                                                            int W, int H, size_t npx, const Fz* __restrict__ fz,
                                                            const PixAux* __restrict__ aux, size_t regionCap)
 {
-    __shared__ uint64_t sk[kRankMax];  // (~P << 32) | pixel, sorted ascending = potential descending
+    __shared__ uint64_t sk[kRankLds];  // (~P << 32) | pixel, sorted ascending = potential descending
     __shared__ uint32_t sN, sFrame, sPairs, sPairI, sPairJ;
     const int tid = threadIdx.x;
     uint32_t* idxOf = rfScratch + (size_t)blockIdx.x * rf_slot_words(npx);
@@ -891,6 +891,7 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
     double* K = reinterpret_cast<double*>(seedf + kRankMax);
     uint32_t* nbr = reinterpret_cast<uint32_t*>(K + kRankMax);  // [entry][8] flooded neighbours
     uint32_t* lblA = nbr + (size_t)8 * kRankMax;                 // labels of the first tie order
+    uint64_t* skg = reinterpret_cast<uint64_t*>(lblA + kRankMax); // sort keys beyond kRankLds entries
     while (true) {
         __syncthreads();
         if (tid == 0) sFrame = atomicAdd(&counters[kRfNext], 1u);
@@ -930,7 +931,7 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
             const bool seed = idxOf[p] != 0u;
             if (!seed || has_unlabelled_nbr(Pf, fn, W, H, (int)p)) {
                 const uint32_t j = atomicAdd(&sN, 1u);
-                if (j < kRankMax) sk[j] = ((uint64_t)(~Pf[p]) << 32) | p;
+                if (j < kRankMax) skg[j] = ((uint64_t)(~Pf[p]) << 32) | p;
             }
         }
         __syncthreads();
@@ -938,31 +939,42 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
         bool ok = n <= kRankMax;
         if (!ok && tid == 0) atomicAdd(&counters[kRfTooBig], 1u);
         if (ok && n > 0) {
-            // 3. Sort by potential (bitonic, in LDS).
+            // 3. Sort by potential (bitonic; in LDS up to kRankLds entries, else in the work space),
+            //    then the entries in potential order.
             uint32_t n2 = 2;
             while (n2 < n) n2 <<= 1;
-            for (uint32_t i = n + tid; i < n2; i += RF_NT) sk[i] = ~0ull;
-            __syncthreads();
-            for (uint32_t kk = 2; kk <= n2; kk <<= 1) {
-                for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-                    for (uint32_t i = tid; i < n2; i += RF_NT) {
-                        const uint32_t ixj = i ^ j;
-                        if (ixj > i) {
-                            const uint64_t a = sk[i], b = sk[ixj];
-                            if ((a > b) == ((i & kk) == 0)) { sk[i] = b; sk[ixj] = a; }
+            auto sort_entries = [&](uint64_t* s) {
+                for (uint32_t i = n + tid; i < n2; i += RF_NT) s[i] = ~0ull;
+                __syncthreads();
+                for (uint32_t kk = 2; kk <= n2; kk <<= 1) {
+                    for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+                        for (uint32_t i = tid; i < n2; i += RF_NT) {
+                            const uint32_t ixj = i ^ j;
+                            if (ixj > i) {
+                                const uint64_t a = s[i], b = s[ixj];
+                                if ((a > b) == ((i & kk) == 0)) { s[i] = b; s[ixj] = a; }
+                            }
                         }
+                        __syncthreads();
                     }
-                    __syncthreads();
                 }
-            }
-            // 4. Entries in potential order; pixel -> entry map; windows; flooded neighbours.
-            for (uint32_t i = tid; i < n; i += RF_NT) {
-                const uint32_t p = (uint32_t)sk[i];
-                sp[i] = ~(uint32_t)(sk[i] >> 32);
-                spix[i] = p;
-                seedf[i] = idxOf[p];
+                for (uint32_t i = tid; i < n; i += RF_NT) {
+                    const uint64_t v = s[i];
+                    const uint32_t p = (uint32_t)v;
+                    sp[i] = ~(uint32_t)(v >> 32);
+                    spix[i] = p;
+                    seedf[i] = idxOf[p];
+                }
+            };
+            if (n <= kRankLds) {
+                for (uint32_t i = tid; i < n; i += RF_NT) sk[i] = skg[i];
+                __syncthreads();
+                sort_entries(sk);
+            } else {
+                sort_entries(skg);
             }
             __syncthreads();
+            // 4. Pixel -> entry map; windows; flooded neighbours.
             for (uint32_t k = tid; k < cnt; k += RF_NT) idxOf[ml[k]] = RF_NONE;  // unpushed seeds stay out
             __syncthreads();
             for (uint32_t i = tid; i < n; i += RF_NT) idxOf[spix[i]] = i;
@@ -975,13 +987,13 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
                 uint32_t a = 0, b = n;
                 while (a < b) {
                     const uint32_t m = (a + b) >> 1;
-                    if ((uint64_t)(~(uint32_t)(sk[m] >> 32)) > pi + D) a = m + 1; else b = m;
+                    if ((uint64_t)sp[m] > pi + D) a = m + 1; else b = m;
                 }
                 lo[i] = a;
                 a = 0; b = n;
                 while (a < b) {
                     const uint32_t m = (a + b) >> 1;
-                    if ((uint64_t)(~(uint32_t)(sk[m] >> 32)) + D >= pi) a = m + 1; else b = m;
+                    if ((uint64_t)sp[m] + D >= pi) a = m + 1; else b = m;
                 }
                 hi[i] = a;
                 const uint32_t p = spix[i];

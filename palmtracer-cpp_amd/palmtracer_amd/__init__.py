@@ -63,6 +63,16 @@ def lib():
         L.Astigmatism3DEstimation.argtypes = [_vp, _vp, _u64, _f64, _vp, _f64]
         L.pt_astig_estimate_device.restype = _i32
         L.pt_astig_estimate_device.argtypes = [_vp, _vp, _u64, _f64, _vp, _f64]
+        L.pt_server_create.restype = _vp
+        L.pt_server_create.argtypes = [_u64, _u64, _u64]
+        L.pt_server_submit.restype = ctypes.c_int64
+        L.pt_server_submit.argtypes = [_vp, _vp, _u64, _u64, _u64, _f64, _f64, _u64, _vp, _u64, _vp, _u64, _u64]
+        L.pt_server_wait.restype = _i32
+        L.pt_server_wait.argtypes = [_vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)]
+        L.pt_server_now.restype = _f64
+        L.pt_server_now.argtypes = [_vp]
+        L.pt_server_destroy.restype = None
+        L.pt_server_destroy.argtypes = [_vp]
         _lib = L
     return _lib
 
@@ -229,3 +239,54 @@ def astig_estimate_device(d_sigmas, d_out, pixel_size, model, z_max):
                                           float(z_max)):
         raise RuntimeError("pt_astig_estimate_device failed")
     return d_out
+
+
+class Server:
+    """Serving loop over device-resident frame chunks (pt_server_*): `slots` engines run requests
+    side by side; requests are admitted in arrival order as slots and the HBM budget allow."""
+
+    def __init__(self, slots=4, budget_bytes=32 << 30, chunk_pixels=0):
+        self._h = lib().pt_server_create(int(slots), int(budget_bytes), int(chunk_pixels))
+        if not self._h:
+            raise RuntimeError("pt_server_create failed")
+
+    def submit(self, d_stack, d_out, threshold, watershed_ratio=0.0, fit=2, fit_params=(7, 1.3, 1.3, 0),
+               max_per_plane=None):
+        """Queues Localization of d_stack (planes, h, w) u16 into d_out (float64); returns the id."""
+        z, h, w = d_stack.shape
+        cap = d_out.numel()
+        mpp = (cap // NCOL) // max(z, 1) if max_per_plane is None else int(max_per_plane)
+        p = _params(fit_params)
+        rid = lib().pt_server_submit(self._h, d_stack.data_ptr(), z, h, w, float(threshold), float(watershed_ratio),
+                                     int(fit), p.ctypes.data, p.size, d_out.data_ptr(), cap, mpp)
+        if rid < 0:
+            raise RuntimeError("pt_server_submit failed")
+        return rid
+
+    def wait(self, rid):
+        """(rows, times) of a request; times = (submitted, started, done) ms on the server clock, slot."""
+        n = ctypes.c_uint64(0)
+        t = (ctypes.c_double * 4)()
+        if not lib().pt_server_wait(self._h, int(rid), ctypes.byref(n), t):
+            raise RuntimeError(f"request {rid} failed")
+        return n.value // NCOL, (t[0], t[1], t[2], int(t[3]))
+
+    def now(self):
+        return lib().pt_server_now(self._h)
+
+    def close(self):
+        if self._h:
+            lib().pt_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -537,3 +537,45 @@ def test_spline_bad_sizes_are_rejected(pt_gpu, capfd):
     got = pt_gpu.localization(st, 10.0, 0.0, 5, [7.0, 0.0, 11.0, 11.0, 50.0])
     assert got.shape[0] == 0
     assert "spline sizes" in capfd.readouterr().err
+
+
+# ------------------------------------------------------------------ large and dense frames ---------
+@pytest.fixture(scope="module")
+def large_frames(pt_gpu, oracle_mod):
+    """A 1024^2 frame at the headline density (800 emitters) and a dense 512^2 frame (1311 emitters,
+    0.5 per um^2 at 100 nm pixels): their exact frame floods hold more than 16384 entries."""
+    import torch
+    out = []
+    for size, em, seed in [(1024, 800, 0x5EED2001), (512, 1311, 0x5EED2002)]:
+        d = torch.empty((1, size, size), dtype=torch.uint16, device="cuda")
+        pt_gpu.synth_device(d, seed=seed, emitters=em)
+        st = d.cpu().numpy()
+        out.append((st, oracle_mod.autothreshold(st[0])))
+    return out
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_large_frames_exact_flood(pt_gpu, oracle_mod, large_frames, k):
+    """Every frame through the exact frame flood: labels equal the oracle's; the rank flood takes
+    frames beyond its LDS sort (global-memory sort up to 65536 entries)."""
+    st, thr = large_frames[k]
+    w1 = oracle_mod.atrous(st[0])[1]
+    exp = oracle_mod.watershed_labels(w1, thr)
+    for forced in (0, 1):
+        pt_gpu.set_option("force_sequential_watershed", forced)
+        try:
+            got = pt_gpu.debug_labels(st[0], thr, 0.0)
+            stats = pt_gpu.last_stats()
+        finally:
+            pt_gpu.set_option("force_sequential_watershed", 0)
+        assert stats["status"] == 0
+        assert np.array_equal(rank_labels(got), exp), (forced, stats)
+        if forced and k == 0:
+            assert stats["rank_too_big"] == 0, stats
+
+
+def test_large_frames_localization(pt_gpu, oracle_mod, large_frames):
+    for st, thr in large_frames:
+        got = pt_gpu.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN)
+        exp = oracle_mod.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN)
+        assert np.array_equal(got, exp)

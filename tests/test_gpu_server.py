@@ -1,0 +1,81 @@
+"""Serving loop (pt_server_*): requests from several simulated cameras, admitted onto engine slots
+under an HBM budget, give the same rows as the same Localization run alone."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+FIT_PARAMS = (7.0, 1.3, 1.3, 0.0)
+
+
+@pytest.fixture
+def cameras(pt_gpu):
+    """Six chunks: 256^2 and 512^2 frames, 3 to 40 frames, different densities and seeds."""
+    import torch
+    specs = [(256, 12, 60, 1), (512, 5, 200, 2), (256, 40, 30, 3), (512, 3, 400, 4), (256, 7, 120, 5), (512, 9, 90, 6)]
+    cams = []
+    for size, frames, emitters, seed in specs:
+        d = torch.empty((frames, size, size), dtype=torch.uint16, device="cuda")
+        pt_gpu.synth_device(d, seed=0x5E0000 + seed, emitters=emitters)
+        thr = pt_gpu.autothreshold_device(d[0])
+        cams.append((d, thr, 4 * emitters + 64))
+    torch.cuda.synchronize()
+    return cams
+
+
+def _alone(pt_gpu, cams, ratio, fit):
+    import torch
+    out = []
+    for d, thr, mpp in cams:
+        o = torch.zeros(d.shape[0] * mpp * 18, dtype=torch.float64, device="cuda")
+        n = pt_gpu.localization_device(d, o, thr, ratio, fit, FIT_PARAMS, mpp)
+        out.append(o[: n * 18].view(-1, 18).cpu().numpy())
+    return out
+
+
+@pytest.mark.parametrize("fp64,ratio,fit", [(1, 0.0, 2), (0, 0.0, 2), (0, 10.0, 4)])
+def test_server_results_equal_alone(pt_gpu, cameras, fp64, ratio, fit):
+    import torch
+    pt_gpu.set_option("fit_fp64", fp64)
+    try:
+        exp = _alone(pt_gpu, cameras, ratio, fit)
+        with pt_gpu.Server(slots=3, budget_bytes=8 << 30, chunk_pixels=1 << 22) as srv:
+            outs, ids = [], []
+            for rep in range(2):                       # every chunk twice, all queued at once
+                for d, thr, mpp in cameras:
+                    o = torch.full((d.shape[0] * mpp * 18,), -7.0, dtype=torch.float64, device="cuda")
+                    outs.append(o)
+                    ids.append(srv.submit(d, o, thr, ratio, fit, FIT_PARAMS, mpp))
+            slots = set()
+            for k, rid in enumerate(ids):
+                n, t = srv.wait(rid)
+                slots.add(t[3])
+                assert t[0] <= t[1] <= t[2]
+                got = outs[k][: n * 18].view(-1, 18).cpu().numpy()
+                e = exp[k % len(cameras)]
+                assert got.shape == e.shape and got.shape[0] > 0
+                assert np.array_equal(got, e, equal_nan=True), f"request {k}"
+            assert len(slots) > 1                      # the requests were spread over the slots
+    finally:
+        pt_gpu.set_option("fit_fp64", 1)
+
+
+def test_server_budget_admits_one_at_a_time(pt_gpu, cameras):
+    """With a budget below two requests' bytes, no two requests are in flight together; a request
+    above the budget is refused."""
+    import torch
+    d, thr, mpp = cameras[1]
+    cap = d.shape[0] * mpp * 18
+    req_bytes = d.numel() * 2 + cap * 8
+    with pt_gpu.Server(slots=3, budget_bytes=req_bytes + req_bytes // 2, chunk_pixels=1 << 22) as srv:
+        outs = [torch.zeros(cap, dtype=torch.float64, device="cuda") for _ in range(4)]
+        ids = [srv.submit(d, o, thr, 0.0, 2, FIT_PARAMS, mpp) for o in outs]
+        spans = [srv.wait(r)[1] for r in ids]
+        spans.sort(key=lambda t: t[1])
+        for a, b in zip(spans, spans[1:]):
+            assert b[1] >= a[2], (a, b)
+        big = torch.zeros(cap * 20, dtype=torch.float64, device="cuda")
+        with pytest.raises(RuntimeError):
+            srv.submit(d, big, thr, 0.0, 2, FIT_PARAMS, mpp)
+        first = outs[0].cpu().numpy()
+        for o in outs[1:]:
+            assert np.array_equal(o.cpu().numpy(), first)
