@@ -78,9 +78,9 @@ def cpu_baseline(stack_host, thr, fit_params, budget_s):
             "sample": f"first {nf} frames of the same stack ({rows} rows in {dt:.2f} s), OpenMP over frames"}
 
 
-def serve_latency(pt, torch, seconds, slots, n_cams, fit_params):
-    """Streamed load: n_cams simulated cameras, each with a fixed frame size (256^2, 512^2, 1024^2
-    in turn), a frame rate uniform in [100, 1000] fps and chunks of 10-100 frames (seeded), the
+def serve_latency(pt, torch, seconds, slots, n_cams, sizes, fit_params):
+    """Streamed load: n_cams simulated cameras, each with a fixed frame size (`sizes` in turn), a frame
+    rate uniform in [100, 1000] fps and chunks of 10-100 frames (seeded), the
     headline workload's emitter density (200 per 512^2 frame, scaled by area). Chunks arrive as Poisson
     processes (mean rate = fps / chunk frames) and are submitted to a pt_server with `slots` slots; a
     chunk's latency is the time from its arrival to its rows in HBM, counted once per frame."""
@@ -90,7 +90,7 @@ def serve_latency(pt, torch, seconds, slots, n_cams, fit_params):
     rng = random.Random(0x5EED0005)
     cams = []
     for c in range(n_cams):
-        size = (256, 512, 1024)[c % 3]
+        size = sizes[c % len(sizes)]
         fps = rng.uniform(100.0, 1000.0)
         chunk = rng.randint(10, 100)
         emitters = size * size * 200 // (512 * 512)
@@ -103,7 +103,7 @@ def serve_latency(pt, torch, seconds, slots, n_cams, fit_params):
                          busy=[None] * len(outs)))
     torch.cuda.synchronize()
     budget = 32 << 30
-    srv = pt.Server(slots=slots, budget_bytes=budget, chunk_pixels=1 << 25)
+    srv = pt.Server(slots=slots, budget_bytes=budget, chunk_pixels=1 << 26)
 
     done = set()
 
@@ -206,6 +206,8 @@ def main():
                     help="length of the streamed-load phase (0: skip it)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-cameras", type=int, default=16)
+    ap.add_argument("--serve-sizes", default="256,512,1024",
+                    help="frame sides of the cameras, assigned in turn (e.g. 256,512,1024,2048)")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
                     help="library option (pt_set_option) applied before the run, e.g. detect_rows=103")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -294,7 +296,8 @@ def main():
 
     serving = None
     if args.serve_seconds > 0:
-        serving = serve_latency(pt, torch, args.serve_seconds, args.serve_slots, args.serve_cameras, fit_params)
+        sizes = [int(v) for v in args.serve_sizes.split(",")]
+        serving = serve_latency(pt, torch, args.serve_seconds, args.serve_slots, args.serve_cameras, sizes, fit_params)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -179,7 +179,8 @@ void Lane::prepare(size_t F, size_t npx)
     fb.ensure(fbSlots * fb_slot_words(npx) * 4);
     counters.ensure(kNumCounters * 4);
     flag.ensure((F + 1) * 12);
-    const size_t rfSlots = std::clamp<size_t>(F, 1, 128);
+    // Rank-flood slots: up to 128 frames at a time, within 1 GiB of work space.
+    const size_t rfSlots = std::clamp<size_t>(std::min<size_t>(F, (size_t(1) << 30) / (rf_slot_words(npx) * 4)), 1, 128);
     rf.ensure(rfSlots * rf_slot_words(npx) * 4);
     sbits.ensure(F * seed_words(npx) * 4);
     fitNext.ensure(256);

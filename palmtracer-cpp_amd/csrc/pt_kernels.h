@@ -85,7 +85,7 @@ enum Counter : int {
     kFloodMaxQueue = 9,  // largest queue of the exact frame flood
     kRfNext = 10,        // rank flood: next flagged frame
     kWaveCnt = 11,       // frames handed from the rank flood to the wave-queue flood
-    kRfTooBig = 12,      // rank flood: frames with more than kRankMax flood entries
+    kRfTooBig = 12,      // rank flood: frames with more flood entries than rf_stride(npx)
     kRfTie = 13,         // rank flood: frames with two equal keys
     kRfUnsettled = 14,   // rank flood: frames whose iteration did not settle (or broke T < G)
     kFbOverflow = 15,    // wave-queue flood: frames whose queue overflowed
@@ -96,12 +96,19 @@ enum Counter : int {
 
 __host__ __device__ inline size_t fb_slot_words(size_t npx) { return 5 * ((npx + 1) & ~size_t(1)); }
 __host__ __device__ inline size_t seed_words(size_t npx) { return (npx + 31) / 32; }
-// Rank flood: frames with at most kRankMax pixels in the flood (mask pixels but the seeds that are
-// never pushed); their sort keys are sorted in LDS up to kRankLds entries, in the work space beyond.
-// Work space per slot: pixel -> entry map, then 23 words per entry.
-constexpr uint32_t kRankMax = 65536;
+// Rank flood: frames with at most rf_stride(npx) pixels in the flood (mask pixels but the seeds that
+// are never pushed; at most kRankMax); their sort keys are sorted in LDS up to kRankLds entries, in
+// the work space beyond. Work space per slot: pixel -> entry map, then 23 words per entry. The stride
+// is a power of two, so the padded sort of any admitted frame fits.
+constexpr uint32_t kRankMax = 262144;
 constexpr uint32_t kRankLds = 16384;
-__host__ __device__ inline size_t rf_slot_words(size_t npx) { return ((npx + 1) & ~size_t(1)) + 23 * (size_t)kRankMax; }
+__host__ __device__ inline size_t rf_stride(size_t npx)
+{
+    size_t r = kRankLds;
+    while (r < npx && r < kRankMax) r <<= 1;
+    return r;
+}
+__host__ __device__ inline size_t rf_slot_words(size_t npx) { return ((npx + 1) & ~size_t(1)) + 23 * rf_stride(npx); }
 
 // A trous
 void launch_atrous_plane(const uint16_t* d_in, double* d_out, int W, int H, int planes, int level, hipStream_t s);

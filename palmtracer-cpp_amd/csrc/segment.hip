@@ -877,21 +877,22 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
     __shared__ uint64_t sk[kRankLds];  // (~P << 32) | pixel, sorted ascending = potential descending
     __shared__ uint32_t sN, sFrame, sPairs, sPairI, sPairJ;
     const int tid = threadIdx.x;
+    const size_t R = rf_stride(npx);                     // entry capacity (power of two)
     uint32_t* idxOf = rfScratch + (size_t)blockIdx.x * rf_slot_words(npx);
     uint32_t* sp = idxOf + ((npx + 1) & ~size_t(1));   // P by entry
-    uint32_t* spix = sp + kRankMax;                      // pixel by entry
-    uint32_t* lo = spix + kRankMax;                      // window [lo, hi) of the entry
-    uint32_t* hi = lo + kRankMax;
-    uint32_t* Ga = hi + kRankMax;                        // pop ranks (two buffers)
-    uint32_t* Gb = Ga + kRankMax;
-    uint32_t* T = Gb + kRankMax;                         // push ages
-    uint32_t* par = T + kRankMax;                        // first popped neighbour (labels)
-    uint32_t* lbl = par + kRankMax;
-    uint32_t* seedf = lbl + kRankMax;                    // entry is a seed
-    double* K = reinterpret_cast<double*>(seedf + kRankMax);
-    uint32_t* nbr = reinterpret_cast<uint32_t*>(K + kRankMax);  // [entry][8] flooded neighbours
-    uint32_t* lblA = nbr + (size_t)8 * kRankMax;                 // labels of the first tie order
-    uint64_t* skg = reinterpret_cast<uint64_t*>(lblA + kRankMax); // sort keys beyond kRankLds entries
+    uint32_t* spix = sp + R;                           // pixel by entry
+    uint32_t* lo = spix + R;                           // window [lo, hi) of the entry
+    uint32_t* hi = lo + R;
+    uint32_t* Ga = hi + R;                             // pop ranks (two buffers)
+    uint32_t* Gb = Ga + R;
+    uint32_t* T = Gb + R;                              // push ages
+    uint32_t* par = T + R;                             // first popped neighbour (labels)
+    uint32_t* lbl = par + R;
+    uint32_t* seedf = lbl + R;                         // entry is a seed
+    double* K = reinterpret_cast<double*>(seedf + R);
+    uint32_t* nbr = reinterpret_cast<uint32_t*>(K + R);      // [entry][8] flooded neighbours
+    uint32_t* lblA = nbr + (size_t)8 * R;                    // labels of the first tie order
+    uint64_t* skg = reinterpret_cast<uint64_t*>(lblA + R);   // sort keys beyond kRankLds entries
     while (true) {
         __syncthreads();
         if (tid == 0) sFrame = atomicAdd(&counters[kRfNext], 1u);
@@ -931,12 +932,12 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
             const bool seed = idxOf[p] != 0u;
             if (!seed || has_unlabelled_nbr(Pf, fn, W, H, (int)p)) {
                 const uint32_t j = atomicAdd(&sN, 1u);
-                if (j < kRankMax) skg[j] = ((uint64_t)(~Pf[p]) << 32) | p;
+                if (j < R) skg[j] = ((uint64_t)(~Pf[p]) << 32) | p;
             }
         }
         __syncthreads();
         const uint32_t n = sN;
-        bool ok = n <= kRankMax;
+        bool ok = n <= R;
         if (!ok && tid == 0) atomicAdd(&counters[kRfTooBig], 1u);
         if (ok && n > 0) {
             // 3. Sort by potential (bitonic; in LDS up to kRankLds entries, else in the work space),

@@ -542,11 +542,12 @@ def test_spline_bad_sizes_are_rejected(pt_gpu, capfd):
 # ------------------------------------------------------------------ large and dense frames ---------
 @pytest.fixture(scope="module")
 def large_frames(pt_gpu, oracle_mod):
-    """A 1024^2 frame at the headline density (800 emitters) and a dense 512^2 frame (1311 emitters,
-    0.5 per um^2 at 100 nm pixels): their exact frame floods hold more than 16384 entries."""
+    """1024^2 and 2048^2 frames at the headline density (200 emitters per 512^2) and a dense 512^2
+    frame (1311 emitters, 0.5 per um^2 at 100 nm pixels): their exact frame floods hold more than
+    16384 entries (the 2048^2 one more than 65536)."""
     import torch
     out = []
-    for size, em, seed in [(1024, 800, 0x5EED2001), (512, 1311, 0x5EED2002)]:
+    for size, em, seed in [(1024, 800, 0x5EED2001), (512, 1311, 0x5EED2002), (2048, 3200, 0x5EED2003)]:
         d = torch.empty((1, size, size), dtype=torch.uint16, device="cuda")
         pt_gpu.synth_device(d, seed=seed, emitters=em)
         st = d.cpu().numpy()
@@ -554,7 +555,7 @@ def large_frames(pt_gpu, oracle_mod):
     return out
 
 
-@pytest.mark.parametrize("k", [0, 1])
+@pytest.mark.parametrize("k", [0, 1, 2])
 def test_large_frames_exact_flood(pt_gpu, oracle_mod, large_frames, k):
     """Every frame through the exact frame flood: labels equal the oracle's; the rank flood takes
     frames beyond its LDS sort (global-memory sort up to 65536 entries)."""
@@ -570,7 +571,7 @@ def test_large_frames_exact_flood(pt_gpu, oracle_mod, large_frames, k):
             pt_gpu.set_option("force_sequential_watershed", 0)
         assert stats["status"] == 0
         assert np.array_equal(rank_labels(got), exp), (forced, stats)
-        if forced and k == 0:
+        if forced and k != 1:
             assert stats["rank_too_big"] == 0, stats
 
 
