@@ -58,7 +58,7 @@ struct Options {
 struct Lane {
     hipStream_t stream = nullptr;
     DetectBuffers b{};
-    DevBuf P, lab, fin, aux, mask, seed, perFrame, regions, fz, order, multi, fb, counters, flag, rf, sbits;
+    DevBuf P, lab, fin, aux, mask, seed, perFrame, regions, fz, order, multi, fb, counters, flag, rf, sbits, hist;
     DevBuf rois, rows, staged, frameMap, fitNext;
     uint32_t* pinned = nullptr;  // host-visible: frame map, roiBase and counters of an asynchronous batch
     size_t pinnedWords = 0;

@@ -150,7 +150,7 @@ Engine::~Engine()
 void Lane::release()
 {
     for (DevBuf* d : {&P, &lab, &fin, &aux, &mask, &seed, &perFrame, &regions, &fz, &order, &multi, &fb, &counters, &flag,
-                      &rf, &sbits, &rois, &rows, &staged, &frameMap, &fitNext})
+                      &rf, &sbits, &hist, &rois, &rows, &staged, &frameMap, &fitNext})
         d->release();
     if (pinned) (void)hipHostFree(pinned);
     pinned = nullptr;
@@ -183,6 +183,7 @@ void Lane::prepare(size_t F, size_t npx)
     const size_t rfSlots = std::clamp<size_t>(std::min<size_t>(F, (size_t(1) << 30) / (rf_slot_words(npx) * 4)), 1, 128);
     rf.ensure(rfSlots * rf_slot_words(npx) * 4);
     sbits.ensure(F * seed_words(npx) * 4);
+    hist.ensure(F * kHistBins * 4);
     fitNext.ensure(256);
     uint32_t* pf = perFrame.as<uint32_t>();
     const size_t s = F + 1;
@@ -212,6 +213,7 @@ void Lane::prepare(size_t F, size_t npx)
     b.rfScratch = rf.as<uint32_t>();
     b.rfSlots = rfSlots;
     b.seedBits = sbits.as<uint32_t>();
+    b.hist = hist.as<uint32_t>();
     b.frames = F;
     b.npx = npx;
     b.regionCap = regionCap;

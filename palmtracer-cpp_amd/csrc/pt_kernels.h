@@ -69,6 +69,7 @@ struct DetectBuffers {
     uint32_t* waveList = nullptr; // [F] frames the rank flood could not settle (wave-queue flood)
     uint32_t* rfScratch = nullptr;// rank-flood work space: rfSlots * rf_slot_words(npx) u32
     uint32_t* seedBits = nullptr; // per frame: bitmap of the local maxima, seed_words(npx) u32
+    uint32_t* hist = nullptr;     // per frame: kHistBins counts of mask pixels at or above each level
     uint32_t* runTotal = nullptr; // optional: ROIs of the call so far (shared by the lanes of a call)
     uint32_t* chunkBase = nullptr;// with runTotal: receives the first ROI of this chunk in the call
     hipEvent_t scanWait = nullptr;// with runTotal: the ROI scan waits for it (previous chunk's scan)
@@ -89,16 +90,24 @@ enum Counter : int {
     kRfTie = 13,         // rank flood: frames with two equal keys
     kRfUnsettled = 14,   // rank flood: frames whose iteration did not settle (or broke T < G)
     kFbOverflow = 15,    // wave-queue flood: frames whose queue overflowed
-    kRfTwoOrders = 16,   // rank flood: frames settled by solving both orders of an equal-key pair
+    kRfTwoOrders = 16,   // rank flood: frames settled by solving their equal-key pairs in both orders
     kNumCounters = 20
 };
 // kStatus bits: 1 = region slots exhausted, 2 = unlabelled mask pixel (internal error).
 
+// Levels of the per-frame potential histogram: P < 16 one bin each, then 16 bins per octave.
+constexpr uint32_t kHistBins = 512;
+__host__ __device__ inline uint32_t hist_bin(uint32_t v)
+{
+    if (v < 16u) return v;
+    const uint32_t msb = 31u - (uint32_t)__builtin_clz(v);
+    return 16u * (msb - 3u) + ((v >> (msb - 4u)) & 15u);
+}
 __host__ __device__ inline size_t fb_slot_words(size_t npx) { return 5 * ((npx + 1) & ~size_t(1)); }
 __host__ __device__ inline size_t seed_words(size_t npx) { return (npx + 31) / 32; }
 // Rank flood: frames with at most rf_stride(npx) pixels in the flood (mask pixels but the seeds that
 // are never pushed; at most kRankMax); their sort keys are sorted in LDS up to kRankLds entries, in
-// the work space beyond. Work space per slot: pixel -> entry map, then 23 words per entry. The stride
+// the work space beyond. Work space per slot: pixel -> entry map, then 25 words per entry. The stride
 // is a power of two, so the padded sort of any admitted frame fits.
 constexpr uint32_t kRankMax = 262144;
 constexpr uint32_t kRankLds = 16384;
@@ -108,7 +117,7 @@ __host__ __device__ inline size_t rf_stride(size_t npx)
     while (r < npx && r < kRankMax) r <<= 1;
     return r;
 }
-__host__ __device__ inline size_t rf_slot_words(size_t npx) { return ((npx + 1) & ~size_t(1)) + 23 * rf_stride(npx); }
+__host__ __device__ inline size_t rf_slot_words(size_t npx) { return ((npx + 1) & ~size_t(1)) + 25 * rf_stride(npx); }
 
 // A trous
 void launch_atrous_plane(const uint16_t* d_in, double* d_out, int W, int H, int planes, int level, hipStream_t s);

@@ -10,14 +10,17 @@
 //   2. every local maximum gets its getLocalMax label, identified here by the raster index of the
 //      first maximum of its plateau chain (+1), which orders labels exactly as the reference does;
 //   3. a component whose maxima all share one label is labelled wholesale;
-//   4. a component with several labels is flooded by one thread with a private heap that follows
-//      libstdc++'s push/pop sequence, using the component-local pop count as the age. At every
-//      pop the thread proves that the popped entry precedes every other entry of the component in
-//      the frame-global order too, whatever the other components' pops did to the counter
-//      (the global gap between two pushes is at least the local gap and at most the frame's mask
-//      size). If the proof fails (near-equal potentials, exact ties), the frame is flagged;
-//   5. flagged frames are re-flooded by an exact sequential restatement of the reference
-//      (raster-order seeds, one frame-wide heap, global counter).
+//   4. a component with several labels is flooded by one wave with an exact priority queue, using
+//      the component-local pop count as the age. At every pop the wave proves that the popped entry
+//      precedes every other queued entry of the component in the frame-global order too, whatever
+//      the other components' pops did to the counter: the global gap between two pushes is at least
+//      the local gap, and at most the number of mask pixels whose potential is at least the
+//      earlier-pushed entry's minus n * 1e-7 (n the frame's mask size), since every entry popped
+//      while that entry waited had a key no larger than its key. If the proof fails (near-equal
+//      potentials, exact ties), the frame is flagged;
+//   5. flagged frames are re-flooded exactly as a whole (rank flood, wave-queue flood, or a
+//      sequential restatement of the reference's heap: raster-order seeds, one frame-wide heap,
+//      global counter).
 // Statistics are exact integer sums of P = 65536 * W1, so they equal the reference's FP64
 // raster-order sums whenever those are exact (sums below 2^53 ulps); larger sums are recomputed in
 // raster order in FP64.
@@ -207,19 +210,61 @@ __device__ inline void add_label(uint32_t* st, uint32_t L, Cas cas, Xchg xchg)
     if (old != 0u && old != L && old != kNone) xchg(st, kNone);
 }
 
+// Upper bound of the global pops between the push of an entry of potential Pb and any later push
+// while it waits: every entry popped meanwhile had a key no larger than its key, hence a potential
+// above Pb - D (D quanta cover n * 1e-7 and the rounding of the keys, n the frame's mask size);
+// cum[k] counts the frame's mask pixels at levels >= k.
+__device__ inline double gap_bound(uint32_t Pb, const uint32_t* cum, uint32_t D, double nmax)
+{
+    return Pb > D ? (double)cum[hist_bin(Pb - D)] : nmax;
+}
+
+// Quanta of potential that the age term of n pops (and the FP64 rounding of the keys) can cover.
+__host__ __device__ inline uint32_t age_quanta(uint32_t n)
+{
+    return (uint32_t)floor(((double)n * kEpsilon + 1e-9) * 65536.0) + 1u;
+}
+
+// Per-frame histogram of the mask potentials by level (hist_bin), as counts at or above each level.
+constexpr int MH_NT = 512;
+static_assert(MH_NT == (int)kHistBins, "one thread per level");
+__global__ void __launch_bounds__(MH_NT) mask_hist_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ maskList,
+                                                          const uint32_t* __restrict__ maskCnt, uint32_t* __restrict__ hist,
+                                                          size_t npx)
+{
+    __shared__ uint32_t h[kHistBins];
+    const int f = blockIdx.x, t = threadIdx.x;
+    h[t] = 0u;
+    __syncthreads();
+    const uint32_t n = maskCnt[f];
+    const uint32_t* ml = maskList + (size_t)f * npx;
+    const uint32_t* Pf = P + (size_t)f * npx;
+    for (uint32_t k = t; k < n; k += MH_NT) atomicAdd(&h[hist_bin(Pf[ml[k]])], 1u);
+    __syncthreads();
+    // Suffix sums (Hillis-Steele).
+    for (int o = 1; o < MH_NT; o <<= 1) {
+        const uint32_t v = t + o < MH_NT ? h[t + o] : 0u;
+        __syncthreads();
+        h[t] += v;
+        __syncthreads();
+    }
+    hist[(size_t)f * kHistBins + t] = h[t];
+}
+
 // Watershed of one multi-label component of at most 64 pixels by one wave, one pixel per lane, the
 // queue held in registers: the same flood and the same certainty proof as flood_kernel (step 3 of
 // the file header), with the component's adjacency as lane masks. list holds the component's
 // pixels; seeds carry their labels in fin and every other pixel 0. Returns false when a pop meets
 // equal keys or cannot be proven.
 __device__ inline bool flood_small(const uint32_t* list, uint32_t cnt, const uint32_t* Pf, uint32_t* fn, int W, int H,
-                                   double nmax)
+                                   double nmax, const uint32_t* cum, uint32_t D)
 {
     const int lane = threadIdx.x & 63;
     const bool act = (uint32_t)lane < cnt;
     const uint32_t p = act ? list[lane] : 0u;
     const int y = (int)(p / (uint32_t)W), x = (int)p - y * W;
     const uint32_t P = act ? Pf[p] : 0u;
+    const double gapB = gap_bound(P, cum, D, nmax);     // global pops while this pixel waits, at most
     uint32_t lbl = act ? ld_relaxed(fn + p) : 0u;
     uint64_t adj = 0;
     for (uint32_t j = 0; j < cnt; ++j) {
@@ -254,7 +299,7 @@ __device__ inline bool flood_small(const uint32_t* list, uint32_t cnt, const uin
             const uint32_t tb = age;
             if (ta == tb) good = oa != ob;
             else if (ta < tb) good = ob - oa < (double)(tb - ta) * eps - delta;
-            else good = oa - ob > nmax * eps + delta;
+            else good = oa - ob > gapB * eps + delta;
         }
         if (__ballot(!good)) return false;
         ++it;
@@ -277,7 +322,7 @@ __global__ void __launch_bounds__(FL_NT) frame_label_kernel(
     const uint32_t* __restrict__ maskCnt, const uint32_t* __restrict__ seedList, const uint32_t* __restrict__ seedCnt,
     uint32_t* __restrict__ rootCnt, Region* __restrict__ regions, size_t regionCap, uint64_t* __restrict__ multiList,
     size_t multiCap, uint32_t* __restrict__ counters, uint32_t* __restrict__ frameFlag, uint32_t* __restrict__ flagList,
-    int W, int H, size_t npx, uint32_t ldsCap, uint32_t parentOff)
+    int W, int H, size_t npx, uint32_t ldsCap, uint32_t parentOff, const uint32_t* __restrict__ hist)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ uint32_t waveSum[FL_NT / 64];
@@ -487,7 +532,8 @@ __global__ void __launch_bounds__(FL_NT) frame_label_kernel(
                     else flag_frame(frameFlag, counters, flagList, f);
                 }
             } else {
-                ok = flood_small(slotPix + sl * 64, cnt, Pf, fn, W, H, (double)n);
+                ok = flood_small(slotPix + sl * 64, cnt, Pf, fn, W, H, (double)n, hist + (size_t)f * kHistBins,
+                                 age_quanta(n));
             }
             if (!ok && lane == 0) {
                 atomicAdd(&counters[kFlagUnproven], 1u);
@@ -723,7 +769,7 @@ __global__ void __launch_bounds__(64) flood_kernel(const uint32_t* __restrict__ 
                                                    uint32_t* __restrict__ counters, uint32_t* __restrict__ frameFlag,
                                                    uint32_t* __restrict__ flagList, int W, int H, size_t npx,
                                                    const Fz* __restrict__ fz, const uint32_t* __restrict__ rootCnt,
-                                                   size_t regionCap)
+                                                   size_t regionCap, const uint32_t* __restrict__ hist)
 {
     __shared__ uint64_t sPa[QCAP_FLOOD];
     __shared__ double sKey[QCAP_FLOOD];
@@ -766,19 +812,21 @@ __global__ void __launch_bounds__(64) flood_kernel(const uint32_t* __restrict__ 
         __syncthreads();
         const double eps = kEpsilon, delta = 1e-9;
         const double nmax = (double)maskCnt[f];
+        const uint32_t D = age_quanta(maskCnt[f]);
+        const uint32_t* cum = hist + (size_t)f * kHistBins;
         uint32_t it = 0;
         while (ok && n > 0) {
             uint32_t a, la;
             bool proven = true;
             // b (potential ob, age tb) stays behind the popped a (oa, ta) in the frame-global order
             // for every admissible global age: the global gap between two pushes is at least the
-            // local gap (ta < tb case) and at most the frame's mask size (ta > tb case).
+            // local gap (ta < tb case) and at most gap_bound of b (ta > tb case).
             auto check = [&](uint64_t ap, uint64_t bp) {
                 const double oa = (double)(uint32_t)(ap >> 32) * kPScale, ob = (double)(uint32_t)(bp >> 32) * kPScale;
                 const uint32_t ta = (uint32_t)ap, tb = (uint32_t)bp;
                 if (ta == tb) return oa != ob;
                 if (ta < tb) return ob - oa < (double)(tb - ta) * eps - delta;
-                return oa - ob > nmax * eps + delta;
+                return oa - ob > gap_bound((uint32_t)(bp >> 32), cum, D, nmax) * eps + delta;
             };
             if (!waveq_pop<true>(q, n, a, la, proven, check) || !proven) { ok = false; break; }
             ++it;
@@ -865,6 +913,7 @@ __device__ inline bool waveq_expand_lds(WaveQ& q, uint32_t& n, const FrameLds& f
 constexpr int RF_NT = 1024;
 constexpr uint32_t RF_NONE = 0xFFFFFFFFu;
 constexpr int RF_MAX_ITER = 64;
+constexpr uint32_t RF_PAIRS = 6;              // equal-key pairs per frame (2^6 combinations of orders)
 
 __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ seedBits, uint32_t* __restrict__ fin,
                                                            const uint32_t* __restrict__ maskList,
@@ -875,7 +924,7 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
                                                            const PixAux* __restrict__ aux, size_t regionCap)
 {
     __shared__ uint64_t sk[kRankLds];  // (~P << 32) | pixel, sorted ascending = potential descending
-    __shared__ uint32_t sN, sFrame, sPairs, sPairI, sPairJ;
+    __shared__ uint32_t sN, sFrame, sPairs, sPairI[RF_PAIRS], sPairJ[RF_PAIRS];
     const int tid = threadIdx.x;
     const size_t R = rf_stride(npx);                     // entry capacity (power of two)
     uint32_t* idxOf = rfScratch + (size_t)blockIdx.x * rf_slot_words(npx);
@@ -891,8 +940,10 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
     uint32_t* seedf = lbl + R;                         // entry is a seed
     double* K = reinterpret_cast<double*>(seedf + R);
     uint32_t* nbr = reinterpret_cast<uint32_t*>(K + R);      // [entry][8] flooded neighbours
-    uint32_t* lblA = nbr + (size_t)8 * R;                    // labels of the first tie order
-    uint64_t* skg = reinterpret_cast<uint64_t*>(lblA + R);   // sort keys beyond kRankLds entries
+    uint32_t* lblA = nbr + (size_t)8 * R;                    // labels of the base solution
+    uint32_t* G0 = lblA + R;                                 // ranks and ages of the base solution
+    uint32_t* T0 = G0 + R;
+    uint64_t* skg = reinterpret_cast<uint64_t*>(T0 + R);     // sort keys beyond kRankLds entries
     while (true) {
         __syncthreads();
         if (tid == 0) sFrame = atomicAdd(&counters[kRfNext], 1u);
@@ -1015,13 +1066,32 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
             }
             __syncthreads();
             // 5.-7. Fixed point, checks and labels. Two equal keys belong to siblings pushed by one
-            // pop with one potential; the heap orders them by its internal layout. With one such pair,
-            // the flood is solved for both orders of the pair; if both solutions keep that single
-            // pair and give the same labels, the heap's choice does not matter.
-            uint32_t pairI = RF_NONE, pairJ = RF_NONE;
-            bool done = false;
-            for (int orient = 0; orient < 2 && ok && !done; ++orient) {
-                for (uint32_t i = tid; i < n; i += RF_NT) Ga[i] = i + 1;
+            // pop with one potential; the heap orders them by its internal layout. The flood is solved
+            // with every pair in index order (the base), then for every other combination of the
+            // pairs' orders (up to RF_PAIRS pairs), each started from the base with the flipped pairs'
+            // ranks exchanged. The heap's choice is one of the combinations; when every combination
+            // settles, keeps the base's pairs and gives the base's labels, the labels are the
+            // reference's whatever the heap chose. Otherwise the frame goes to the wave-queue flood.
+            uint32_t nPairs = 0;
+            auto flipped = [&](uint32_t mask, uint32_t a, uint32_t b) {
+                for (uint32_t p = 0; p < nPairs; ++p)
+                    if (((mask >> p) & 1u) && ((a == sPairI[p] && b == sPairJ[p]) || (a == sPairJ[p] && b == sPairI[p])))
+                        return true;
+                return false;
+            };
+            for (uint32_t trial = 0; ok && trial < (1u << nPairs); ++trial) {
+                for (uint32_t i = tid; i < n; i += RF_NT) {
+                    uint32_t g = i + 1;
+                    if (trial) {
+                        g = G0[i];
+                        for (uint32_t p = 0; p < nPairs; ++p)
+                            if ((trial >> p) & 1u) {
+                                if (i == sPairI[p]) g = G0[sPairJ[p]];
+                                else if (i == sPairJ[p]) g = G0[sPairI[p]];
+                            }
+                    }
+                    Ga[i] = g;
+                }
                 if (tid == 0) sPairs = 0;
                 __syncthreads();
                 uint32_t* G = Ga;
@@ -1049,7 +1119,7 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
                         for (uint32_t j = lo[i]; j < hi[i]; ++j) {
                             if (j == i) continue;
                             const double kj = K[j];
-                            c += kj < ki || (kj == ki && (orient == 0 ? j < i : j > i));
+                            c += kj < ki || (kj == ki && ((j < i) != (trial != 0 && flipped(trial, i, j))));
                         }
                         const uint32_t g = 1 + lo[i] + c;
                         Gn[i] = g;
@@ -1059,25 +1129,47 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
                     uint32_t* t2 = G; G = Gn; Gn = t2;
                     if (!anyChange) { settled = true; break; }
                 }
-                // Checks: settled, every entry pushed before its pop, equal-key pairs.
-                bool viol = !settled;
+                // Checks: settled, every entry pushed before its pop, the equal-key pairs (listed by the
+                // base; the same pairs in every other combination).
+                bool viol = !settled, dep = false;
                 for (uint32_t i = tid; i < n && settled; i += RF_NT) {
                     if (!seedf[i] && !(T[i] < G[i])) viol = true;
                     const double ki = K[i];
                     for (uint32_t j = i + 1; j < hi[i]; ++j)
                         if (K[j] == ki) {
-                            if (atomicAdd(&sPairs, 1u) == 0u) { sPairI = i; sPairJ = j; }
+                            const uint32_t k = atomicAdd(&sPairs, 1u);
+                            if (trial == 0) {
+                                if (k < RF_PAIRS) { sPairI[k] = i; sPairJ[k] = j; }
+                            } else if (!flipped(~0u, i, j)) {
+                                dep = true;
+                            }
                         }
                 }
                 const bool bad = __syncthreads_or(viol);
-                const uint32_t pairs = sPairs;
-                if (bad || pairs > 1 || (orient == 1 && (pairs != 1 || sPairI != pairI || sPairJ != pairJ))) {
+                bool tie = __syncthreads_or(dep) || (trial != 0 && sPairs != nPairs);
+                if (!bad && trial == 0) {
+                    nPairs = sPairs;
+                    tie = nPairs > RF_PAIRS;
+                    // Each entry in at most one pair (no three equal keys).
+                    for (uint32_t a = tid; a < nPairs * nPairs && !tie; a += RF_NT) {
+                        const uint32_t p = a / nPairs, q = a % nPairs;
+                        if (p < q && (sPairI[p] == sPairI[q] || sPairI[p] == sPairJ[q] || sPairJ[p] == sPairI[q] ||
+                                      sPairJ[p] == sPairJ[q]))
+                            tie = true;
+                    }
+                    tie = __syncthreads_or(tie);
+                }
+                if (bad || tie) {
                     if (tid == 0) atomicAdd(&counters[bad ? kRfUnsettled : kRfTie], 1u);
                     ok = false;
                     break;
                 }
                 // Labels through the first-popped-neighbour links (pointer jumping).
                 for (uint32_t i = tid; i < n; i += RF_NT) {
+                    if (trial == 0) {
+                        G0[i] = G[i];
+                        T0[i] = T[i];
+                    }
                     if (seedf[i]) {
                         par[i] = i;
                         lbl[i] = fn[spix[i]];
@@ -1108,7 +1200,7 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
                 for (uint32_t i = tid; i < n; i += RF_NT) {
                     const uint32_t L = lbl[i];
                     if (L == RF_NONE || L == 0u) missing = true;
-                    if (orient == 0) lblA[i] = L;
+                    if (trial == 0) lblA[i] = L;
                     else differ |= L != lblA[i];
                 }
                 if (__syncthreads_or(missing) || __syncthreads_or(differ)) {
@@ -1116,15 +1208,11 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
                     ok = false;
                     break;
                 }
-                if (orient == 0 && pairs == 1) {
-                    pairI = sPairI;
-                    pairJ = sPairJ;
-                    continue;  // solve again with the pair in the other order
-                }
+            }
+            if (ok) {
                 for (uint32_t i = tid; i < n; i += RF_NT)
-                    if (!seedf[i]) fn[spix[i]] = lbl[i];
-                if (tid == 0 && orient == 1) atomicAdd(&counters[kRfTwoOrders], 1u);
-                done = true;
+                    if (!seedf[i]) fn[spix[i]] = lblA[i];
+                if (tid == 0 && nPairs > 0) atomicAdd(&counters[kRfTwoOrders], 1u);
             }
             __syncthreads();
         }
@@ -1786,9 +1874,10 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
     const uint32_t* skip = resolve ? nullptr : b.frameFlag;
     uint32_t ldsCap = 0, parentOff = 0;
     const size_t flLds = frame_label_lds(npx, ldsCap, parentOff);
+    mask_hist_kernel<<<F, MH_NT, 0, s>>>(b.P, b.maskList, b.maskCnt, b.hist, npx);
     frame_label_kernel<<<F, FL_NT, flLds, s>>>(b.P, b.seedBits, b.lab, b.fin, b.aux, b.maskList, b.maskCnt, b.seedList,
                                                b.seedCnt, b.rootCnt, b.regions, b.regionCap, b.multiList, b.multiCap,
-                                               b.counters, b.frameFlag, b.flagList, W, H, npx, ldsCap, parentOff);
+                                               b.counters, b.frameFlag, b.flagList, W, H, npx, ldsCap, parentOff, b.hist);
     auto watershed = [&](int pass2) {
         if (forceSequential) force_flag_kernel<<<ceil_div(F, 256), 256, 0, s>>>(b.counters, b.frameFlag, b.flagList, F);
         if (pass2)
@@ -1796,7 +1885,7 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
                                                  b.counters, b.frameFlag, b.flagList, npx);
         flood_kernel<<<4096, 64, 0, s>>>(b.P, b.lab, b.fin, b.seedList, b.seedCnt, b.maskCnt, b.multiList, b.multiCap,
                                          b.counters, b.frameFlag, b.flagList, W, H, npx, pass2 ? b.fz : nullptr,
-                                         b.rootCnt, b.regionCap);
+                                         b.rootCnt, b.regionCap, b.hist);
         if (!resolve) return;
         // Exact frame flood of the flagged frames: rank flood, then the wave-queue flood for the frames
         // it hands over, then the heap restatement for frames with equal keys at a pop.

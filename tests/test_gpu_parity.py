@@ -407,8 +407,9 @@ def test_localization_with_all_frames_exact(pt_gpu, oracle_mod, frames512):
 
 
 # Seeded 256x256 frames (tests/golden/synth_np.py, 60 emitters) whose flood has equal keys (siblings
-# with one potential): one pair (the rank flood solves both orders) or several (wave-queue flood,
-# then the heap restatement). Found by solving the flood on the CPU.
+# with one potential): one pair or several. The rank flood settles a frame whose pairs are independent
+# (each pair's other order changes nothing but its two ranks); other frames go on to the wave-queue
+# flood, then the heap restatement. Found by solving the flood on the CPU.
 TIE_FRAMES = [(101, 5.0, 1), (102, 4.0, 1), (104, 4.0, 1), (101, 4.0, 2), (103, 4.0, 3)]
 
 
@@ -428,7 +429,7 @@ def test_equal_keys_in_the_flood(pt_gpu, oracle_mod, seed, thr, pairs):
     if pairs == 1:
         assert stats["rank_two_orders"] == 1, stats
     else:
-        assert stats["rank_equal_keys"] == 1 and stats["heap_frames"] == 1, stats
+        assert stats["rank_two_orders"] == 1 or (stats["rank_equal_keys"] == 1 and stats["heap_frames"] == 1), stats
 
 
 # ------------------------------------------------------------------ astigmatic z estimation --------
