@@ -55,6 +55,72 @@ __global__ void __launch_bounds__(NT) sort_frames_kernel(const double* __restric
     }
 }
 
+// Frames with up to SORT_MAX rows, one workgroup per frame: (Y, X, position) keys in LDS, a bitonic
+// sort of the row positions, rows written in sorted order. For callers whose frames can hold more
+// rows than the ranking kernel above keeps in LDS.
+constexpr int SORT_MAX = 8192;
+constexpr size_t SORT_LDS = (size_t)SORT_MAX * (8 + 8 + 2);
+__global__ void __launch_bounds__(NT) sort_frames_big_kernel(const double* __restrict__ rows,
+                                                             const uint32_t* __restrict__ roiBase,
+                                                             double* __restrict__ staged)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    int64_t* ky = reinterpret_cast<int64_t*>(smem);
+    int64_t* kx = ky + SORT_MAX;
+    uint16_t* id = reinterpret_cast<uint16_t*>(kx + SORT_MAX);
+    constexpr uint16_t kPad = 0xFFFF;
+    const int f = blockIdx.x;
+    const uint32_t b0 = roiBase[f], n = roiBase[f + 1] - b0;
+    if (n > (uint32_t)SORT_MAX) {
+        // Beyond the LDS: rank every row against all others (global memory).
+        for (uint32_t i = threadIdx.x; i < n; i += NT) {
+            const double* r = rows + (size_t)(b0 + i) * kCols;
+            const int64_t yi = weak_key(r[kY]), xi = weak_key(r[kX]);
+            uint32_t rank = 0;
+            for (uint32_t j = 0; j < n; ++j)
+                rank += before(weak_key(rows[(size_t)(b0 + j) * kCols + kY]), weak_key(rows[(size_t)(b0 + j) * kCols + kX]),
+                               j, yi, xi, i);
+            double* o = staged + (size_t)(b0 + rank) * kCols;
+            for (int c = 0; c < kCols; ++c) o[c] = r[c];
+            o[kIndex] = (double)(rank + 1);
+        }
+        return;
+    }
+    uint32_t n2 = 1;
+    while (n2 < n) n2 <<= 1;
+    for (uint32_t i = threadIdx.x; i < n2; i += NT) {
+        if (i < n) {
+            ky[i] = weak_key(rows[(size_t)(b0 + i) * kCols + kY]);
+            kx[i] = weak_key(rows[(size_t)(b0 + i) * kCols + kX]);
+        }
+        id[i] = i < n ? (uint16_t)i : kPad;
+    }
+    __syncthreads();
+    auto less = [&](uint16_t a, uint16_t b) {
+        if (a == kPad) return false;
+        if (b == kPad) return true;
+        return before(ky[a], kx[a], a, ky[b], kx[b], b);
+    };
+    for (uint32_t k = 2; k <= n2; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n2; i += NT) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const uint16_t a = id[i], b = id[ixj];
+                    if ((i & k) == 0 ? less(b, a) : less(a, b)) { id[i] = b; id[ixj] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t r = threadIdx.x; r < n; r += NT) {
+        const double* src = rows + (size_t)(b0 + id[r]) * kCols;
+        double* o = staged + (size_t)(b0 + r) * kCols;
+        for (int c = 0; c < kCols; ++c) o[c] = src[c];
+        o[kIndex] = (double)(r + 1);
+    }
+}
+
 // One workgroup per plane: rows to their final place with the global Id and the plane number
 // (lib/Palm.cpp:24-43).
 __global__ void __launch_bounds__(NT) scatter_kernel(const double* __restrict__ staged, const uint64_t* __restrict__ src,
@@ -75,10 +141,14 @@ __global__ void __launch_bounds__(NT) scatter_kernel(const double* __restrict__ 
 }
 } // namespace
 
-void launch_sort_frames(const double* d_rows, const uint32_t* d_roiBase, int frames, double* d_staged, hipStream_t s)
+void launch_sort_frames(const double* d_rows, const uint32_t* d_roiBase, int frames, double* d_staged, hipStream_t s,
+                        uint64_t maxRows)
 {
     if (frames <= 0) return;
-    sort_frames_kernel<<<frames, NT, 0, s>>>(d_rows, d_roiBase, d_staged);
+    if (maxRows <= (uint64_t)LDS_ROWS)
+        sort_frames_kernel<<<frames, NT, 0, s>>>(d_rows, d_roiBase, d_staged);
+    else
+        sort_frames_big_kernel<<<frames, NT, SORT_LDS, s>>>(d_rows, d_roiBase, d_staged);
     PT_CHECK(hipGetLastError());
 }
 

@@ -359,6 +359,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     main_.prepare(F, npx);
     if (planes > F) mainB_.prepare(F, npx);
     const FitParams fp = fit_params(fit, fitParams);
+    const uint64_t frameRows = std::min<uint64_t>(maxPerPlane, npx / 4 + 2);   // rows a plane can hold
     unsigned long long* passes = nullptr;
     if (opt.profile) {
         passes_.ensure(16);
@@ -404,7 +405,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
                    side_.rows.as<double>(), opt.fitFp64, side_.fitNext.as<unsigned int>(), side_.stream, passes, dmap,
                    opt.fitWaves, side_.b.roiBase + batchCount);
         launch_sort_frames(side_.rows.as<double>(), side_.b.roiBase, (int)batchCount, side_.staged.as<double>(),
-                           side_.stream);
+                           side_.stream, frameRows);
         PT_CHECK(hipMemcpyAsync(h + SF, side_.b.roiBase, (batchCount + 1) * 4, hipMemcpyDeviceToHost, side_.stream));
         PT_CHECK(hipMemcpyAsync(h + SF + SF + 1, side_.b.counters, kNumCounters * 4, hipMemcpyDeviceToHost, side_.stream));
         PT_CHECK(hipEventRecord(sideDone, side_.stream));
@@ -528,7 +529,8 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         launch_fit(d_in, npx, W, H, allRois_.as<Roi>(), (uint32_t)totalRois, fp, main_.rows.as<double>(), opt.fitFp64,
                    main_.fitNext.as<unsigned int>(), main_.stream, passes, nullptr, opt.fitWaves);
     if (opt.profile) PT_CHECK(hipEventRecord(fitEv_[1], main_.stream));
-    if (totalRois) launch_sort_frames(main_.rows.as<double>(), dBase, (int)planes, main_.staged.as<double>(), main_.stream);
+    if (totalRois)
+        launch_sort_frames(main_.rows.as<double>(), dBase, (int)planes, main_.staged.as<double>(), main_.stream, frameRows);
     if (opt.profile) PT_CHECK(hipEventRecord(fitEv_[2], main_.stream));
     side_poll(true);
     PT_CHECK(hipStreamSynchronize(side_.stream));

@@ -175,8 +175,9 @@ void launch_spline_fit(const uint16_t* d_frames, size_t npx, int W, int H, const
 void launch_spline_transpose(const double* d_raw, double* d_cv, uint64_t ssize, hipStream_t s);
 
 // Per-frame (Y, X) sort (rows of frame f at d_rows + roiBase[f]) into d_staged with the same layout;
-// Index is set, Id and Plane are left for launch_scatter.
-void launch_sort_frames(const double* d_rows, const uint32_t* d_roiBase, int frames, double* d_staged, hipStream_t s);
+// Index is set, Id and Plane are left for launch_scatter. maxRows: the most rows a frame can hold.
+void launch_sort_frames(const double* d_rows, const uint32_t* d_roiBase, int frames, double* d_staged, hipStream_t s,
+                        uint64_t maxRows);
 // Final placement: for each plane p, count[p] rows from staged row src[p] to output row dst[p],
 // with Id = dst[p] + k + 1 and Plane = p + 1.
 void launch_scatter(const double* d_staged, const uint64_t* d_src, const uint64_t* d_dst, const uint32_t* d_count,
