@@ -144,13 +144,15 @@ __global__ void __launch_bounds__(NT) atrous_plane_kernel(const uint16_t* __rest
 // 7-column dependency reach of the local-maximum test through both scales. The wave walks its
 // segment's rows top to bottom as a pipeline: input row t gives A(t) (row pass, scale 0), V1s(t-2)
 // (column pass from a 5-row ring of A), B(t-2) (row pass, scale 1), V2s(t-6) (column pass from a
-// 9-row ring of B), P(t-6) and the local-maximum test of row t-7 (3-row ring of P). Horizontal
+// 10-row ring of B), P(t-6) and the local-maximum test of row t-7 (running column maxima of P). Horizontal
 // neighbours come from the adjacent lanes through DPP wave shifts; vertical neighbours from
 // registers. Clamp-to-edge is applied per plane as in the reference: input rows and columns are
 // read clamped (so A is right everywhere), V1s columns outside the frame are replaced by the edge
 // column, and the B ring is filled with B(0) at the top and repeats B(H-1) below the frame.
-// Rows whose every stage lies inside the frame and the segment run in an unrolled loop without
-// those edge cases, so that the rings rotate by register renaming instead of copies.
+// Rows whose every stage lies inside the frame and the segment run in a loop unrolled by a multiple
+// of the ring sizes without those edge cases: within an iteration the rings are addressed circularly
+// with compile-time slots, and after it they are back in their shifted layout, so rows never move
+// between registers.
 //
 //   P[frame*npx + i]           dense, every pixel: 65536 * W1 if >= tmin, else 0
 //   maskList / seedList        per-frame compact lists (capacity npx each, any order) of pixel
@@ -165,7 +167,13 @@ constexpr int SHALO = 9;                // halo columns on each side (3 lanes)
 constexpr int SOUT = SW - 2 * SHALO;    // output columns per strip
 constexpr int MCAP = 1024;              // staged mask entries per wave
 constexpr int SCAP = 256;               // staged seed entries per wave
-constexpr int UNROLL = 4;               // rows per iteration of the interior loop
+constexpr int UNROLL = 10;              // rows per iteration of the interior loop (a multiple of the ring sizes)
+constexpr int NA = 5, NV = 5, NB = 10;  // ring sizes: A rows, V1s rows, B rows (9 used)
+#ifndef PT_ROW_PREFETCH
+#define PT_ROW_PREFETCH 1
+#endif
+constexpr int PF = PT_ROW_PREFETCH;     // input rows loaded ahead of the row in use (divides UNROLL)
+static_assert(UNROLL % NA == 0 && UNROLL % NV == 0 && UNROLL % NB == 0 && UNROLL % PF == 0, "ring sizes");
 constexpr uint64_t kOutLanes = ((1ull << (64 - SHALO / SC)) - 1ull) & ~((1ull << (SHALO / SC)) - 1ull);
 constexpr int kIcmpNe = 33, kIcmpUge = 35;  // LLVM integer predicates of __builtin_amdgcn_uicmp
 
@@ -213,7 +221,7 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
     const int f = blockIdx.y;
     const int strip = blockIdx.x % strips, seg = blockIdx.x / strips;
     const uint32_t npx = (uint32_t)W * (uint32_t)H;     // < 2^30 (checked by the launcher)
-    const char* img = reinterpret_cast<const char*>(in + (size_t)(frameMap ? frameMap[f] : (uint32_t)f) * npx);
+    const __amdgpu_buffer_rsrc_t rsI = make_rsrc(in + (size_t)(frameMap ? frameMap[f] : (uint32_t)f) * npx, npx * 2u);
     const __amdgpu_buffer_rsrc_t rsP = make_rsrc(P + (size_t)f * npx, npx * 4u);
     uint32_t* ml = maskList + (size_t)f * npx;
     uint32_t* sl = seedList + (size_t)f * npx;
@@ -245,23 +253,27 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
     const bool fixR = oR < SW - 1;
     const int lR = oR / SC, kR = oR % SC;
 
-    uint32_t rA[5][SC], rB[9][SC], rV[5][SC], rP[3][SC];
+    // Rings in shifted layout between steps (slot 0 oldest); Pp = P of the last row, Mp = maximum of
+    // the last two rows of P, per column.
+    uint32_t rA[NA][SC], rB[NB][SC], rV[NV][SC], Pp[SC], Mp[SC];
 #pragma unroll
     for (int k = 0; k < SC; ++k) {
 #pragma unroll
-        for (int i = 0; i < 5; ++i) { rA[i][k] = 0; rV[i][k] = 0; }
+        for (int i = 0; i < NA; ++i) rA[i][k] = 0;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) rB[i][k] = 0;
+        for (int i = 0; i < NV; ++i) rV[i][k] = 0;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) rP[i][k] = 0;
+        for (int i = 0; i < NB; ++i) rB[i][k] = 0;
+        Pp[k] = 0;
+        Mp[k] = 0;
     }
     uint32_t mPend = 0, sPend = 0;      // staged list entries (wave-uniform)
 
     auto load_row = [&](int t, uint32_t (&v)[SC]) {
         const int y = t < 0 ? 0 : (t > H - 1 ? H - 1 : t);
-        const char* row = img + (size_t)y * W * 2;
+        const int rowOff = (int)((uint32_t)y * (uint32_t)W * 2u);
 #pragma unroll
-        for (int k = 0; k < SC; ++k) v[k] = *reinterpret_cast<const uint16_t*>(row + inOff[k]);
+        for (int k = 0; k < SC; ++k) v[k] = __builtin_amdgcn_raw_buffer_load_b16(rsI, (int)inOff[k], rowOff, 0);
     };
     auto flush = [&](const uint32_t* stage, uint32_t n, uint32_t* cnt, uint32_t* list) {
         wave_fence();
@@ -271,16 +283,11 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
         for (uint32_t i = lane; i < n; i += 64) list[base + i] = stage[i];
         wave_fence();
     };
-    // Dense P, mask and local maxima of output row y3 from the P ring.
-    auto emit = [&](int y3) {
+    // Dense P, mask and local maxima of output row y3: v = its P, cm = the column maxima of rows
+    // y3 - 1 .. y3 + 1.
+    auto emit = [&](int y3, const uint32_t (&v)[SC], const uint32_t (&cm)[SC]) {
         const uint32_t rowBase = (uint32_t)y3 * (uint32_t)W;
         const int rowOff = (int)(rowBase * 4u);
-        uint32_t v[SC], cm[SC];
-#pragma unroll
-        for (int k = 0; k < SC; ++k) {
-            v[k] = rP[1][k];
-            cm[k] = max3u(rP[0][k], rP[1][k], rP[2][k]);
-        }
 #pragma unroll
         for (int k = 0; k < SC; ++k)
             if (outCol[k]) __builtin_amdgcn_raw_buffer_store_b32(v[k], rsP, (int)outOff[k], rowOff, 0);
@@ -319,35 +326,27 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
         sPend += sRow;
     };
 
-    uint32_t nxt[SC];
-    // One pipeline step for input row t. Interior steps (every stage inside the frame, output row
-    // inside the segment) skip the edge handling.
-    auto step = [&](int t, auto interior) {
-        constexpr bool kInterior = decltype(interior)::value;
-        uint32_t v0[SC];
+    uint32_t pre[PF][SC];                // input rows t + 1 .. t + PF
+    // Next input row: the oldest prefetched one; the row PF ahead is loaded in its place.
+    auto next_row = [&](int t, uint32_t (&v0)[SC]) {
 #pragma unroll
-        for (int k = 0; k < SC; ++k) v0[k] = nxt[k];
-        load_row(t + 1, nxt);               // clamped: the row after the last one is still in the frame
-
-        // A(t) = 16 * rowpass_0(V0): taps at columns c-2..c+2.
-        {
-            const uint32_t l1 = from_left(v0[1]), l2 = from_left(v0[2]);
-            const uint32_t r0 = from_right(v0[0]), r1 = from_right(v0[1]);
-            const uint32_t a[SC] = {b3(l1, l2, v0[0], v0[1], v0[2]), b3(l2, v0[0], v0[1], v0[2], r0),
-                                    b3(v0[0], v0[1], v0[2], r0, r1)};
+        for (int k = 0; k < SC; ++k) v0[k] = pre[0][k];
 #pragma unroll
-            for (int k = 0; k < SC; ++k) {
+        for (int i = 0; i < PF - 1; ++i)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) rA[i][k] = rA[i + 1][k];
-                rA[4][k] = a[k];
-            }
-        }
-        // V1s(y1) = colpass_0(A), y1 = t - 2.
-        const int y1 = t - 2;
-        uint32_t u[SC];
-#pragma unroll
-        for (int k = 0; k < SC; ++k) u[k] = b3(rA[0][k], rA[1][k], rA[2][k], rA[3][k], rA[4][k]);
-        // Columns outside the frame hold the edge column's V1s (clamp of the scale-1 taps).
+            for (int k = 0; k < SC; ++k) pre[i][k] = pre[i + 1][k];
+        load_row(t + PF, pre[PF - 1]);      // clamped: rows past the last one are the last one
+    };
+    // A(t) = 16 * rowpass_0(V0): taps at columns c-2..c+2.
+    auto row_a = [&](const uint32_t (&v0)[SC], uint32_t (&a)[SC]) {
+        const uint32_t l1 = from_left(v0[1]), l2 = from_left(v0[2]);
+        const uint32_t r0 = from_right(v0[0]), r1 = from_right(v0[1]);
+        a[0] = b3(l1, l2, v0[0], v0[1], v0[2]);
+        a[1] = b3(l2, v0[0], v0[1], v0[2], r0);
+        a[2] = b3(v0[0], v0[1], v0[2], r0, r1);
+    };
+    // Columns outside the frame hold the edge column's V1s (clamp of the scale-1 taps).
+    auto fix_edges = [&](uint32_t (&u)[SC]) {
         if (fixL) {
             const uint32_t e = __builtin_amdgcn_readlane(u[0], SHALO / SC);
 #pragma unroll
@@ -358,83 +357,155 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
 #pragma unroll
             for (int k = 0; k < SC; ++k) u[k] = xcol[k] > W - 1 ? e : u[k];
         }
+    };
+    // B(y1) = 16 * rowpass_1(V1s): taps at columns c-4, c-2, c, c+2, c+4.
+    auto row_b = [&](const uint32_t (&u)[SC], uint32_t (&bb)[SC]) {
+        const uint32_t L0 = from_left(u[0]), L1 = from_left(u[1]), L2 = from_left(u[2]);
+        const uint32_t R0 = from_right(u[0]), R1 = from_right(u[1]), R2 = from_right(u[2]);
+        const uint32_t LL2 = from_left(L2), RR0 = from_right(R0);
+        bb[0] = b3(LL2, L1, u[0], u[2], R1);
+        bb[1] = b3(L0, L2, u[1], R0, R2);
+        bb[2] = b3(L1, u[0], u[2], R1, RR0);
+    };
+    // 65536 * W1 = 256 * V1s - V2s, kept when >= tmin (u32: both terms are below 2^32); then the
+    // running column maxima and the output row.
+    auto finish = [&](int y3, bool rowIn, const uint32_t (&v1)[SC], const uint32_t (&v2s)[SC], bool emitRow) {
+        uint32_t v[SC], cm[SC];
+#pragma unroll
+        for (int k = 0; k < SC; ++k) {
+            const uint32_t a = v1[k] << 8;
+            const uint32_t d = a - v2s[k];
+            const uint32_t pr = (rowIn && a >= v2s[k] && d >= tmk[k]) ? d : 0u;
+            v[k] = Pp[k];
+            cm[k] = max(Mp[k], pr);
+            Mp[k] = max(Pp[k], pr);
+            Pp[k] = pr;
+        }
+        if (emitRow) emit(y3, v, cm);
+    };
+
+    // One pipeline step for input row t, rings in shifted layout. Interior steps (every stage inside
+    // the frame, output row inside the segment) skip the edge handling.
+    auto step = [&](int t, auto interior) {
+        constexpr bool kInterior = decltype(interior)::value;
+        uint32_t v0[SC];
+        next_row(t, v0);
+        uint32_t a[SC];
+        row_a(v0, a);
 #pragma unroll
         for (int k = 0; k < SC; ++k) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) rV[i][k] = rV[i + 1][k];
-            rV[4][k] = u[k];
+            for (int i = 0; i < NA - 1; ++i) rA[i][k] = rA[i + 1][k];
+            rA[NA - 1][k] = a[k];
         }
-        // B(y1) = 16 * rowpass_1(V1s): taps at columns c-4, c-2, c, c+2, c+4.
-        auto row_b = [&](uint32_t (&bb)[SC]) {
-            const uint32_t L0 = from_left(u[0]), L1 = from_left(u[1]), L2 = from_left(u[2]);
-            const uint32_t R0 = from_right(u[0]), R1 = from_right(u[1]), R2 = from_right(u[2]);
-            const uint32_t LL2 = from_left(L2), RR0 = from_right(R0);
-            bb[0] = b3(LL2, L1, u[0], u[2], R1);
-            bb[1] = b3(L0, L2, u[1], R0, R2);
-            bb[2] = b3(L1, u[0], u[2], R1, RR0);
-        };
+        // V1s(y1) = colpass_0(A), y1 = t - 2.
+        const int y1 = t - 2;
+        uint32_t u[SC];
+#pragma unroll
+        for (int k = 0; k < SC; ++k) u[k] = b3(rA[0][k], rA[1][k], rA[2][k], rA[3][k], rA[4][k]);
+        fix_edges(u);
+#pragma unroll
+        for (int k = 0; k < SC; ++k) {
+#pragma unroll
+            for (int i = 0; i < NV - 1; ++i) rV[i][k] = rV[i + 1][k];
+            rV[NV - 1][k] = u[k];
+        }
         if constexpr (kInterior) {
             uint32_t bb[SC];
-            row_b(bb);
+            row_b(u, bb);
 #pragma unroll
             for (int k = 0; k < SC; ++k) {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) rB[i][k] = rB[i + 1][k];
-                rB[8][k] = bb[k];
+                for (int i = 0; i < NB - 1; ++i) rB[i][k] = rB[i + 1][k];
+                rB[NB - 1][k] = bb[k];
             }
         } else if (y1 >= 0) {
             uint32_t bb[SC];
             if (y1 < H) {
-                row_b(bb);
+                row_b(u, bb);
             } else {
 #pragma unroll
-                for (int k = 0; k < SC; ++k) bb[k] = rB[8][k];  // below the frame: B(H-1)
+                for (int k = 0; k < SC; ++k) bb[k] = rB[NB - 1][k];  // below the frame: B(H-1)
             }
 #pragma unroll
             for (int k = 0; k < SC; ++k) {
                 if (y1 == 0) {
 #pragma unroll
-                    for (int i = 0; i < 8; ++i) rB[i][k] = bb[k];  // above the frame: B(0)
+                    for (int i = 0; i < NB - 1; ++i) rB[i][k] = bb[k];  // above the frame: B(0)
                 } else {
 #pragma unroll
-                    for (int i = 0; i < 8; ++i) rB[i][k] = rB[i + 1][k];
+                    for (int i = 0; i < NB - 1; ++i) rB[i][k] = rB[i + 1][k];
                 }
-                rB[8][k] = bb[k];
+                rB[NB - 1][k] = bb[k];
             }
         }
-        // P(y2), y2 = t - 6: V2s = colpass_1(B) from rows y2-4, y2-2, y2, y2+2, y2+4;
-        // 65536 * W1 = 256 * V1s - V2s, kept when >= tmin (u32: both terms are below 2^32).
+        // P(y2), y2 = t - 6: V2s = colpass_1(B) from rows y2-4, y2-2, y2, y2+2, y2+4.
         const int y2 = t - 6;
-        const bool rowIn = kInterior || (y2 >= 0 && y2 < H);
-        uint32_t pr[SC];
+        uint32_t v2s[SC], v1[SC];
 #pragma unroll
         for (int k = 0; k < SC; ++k) {
-            const uint32_t v2s = b3(rB[0][k], rB[2][k], rB[4][k], rB[6][k], rB[8][k]);
-            const uint32_t a = rV[0][k] << 8;
-            const uint32_t d = a - v2s;
-            pr[k] = (rowIn && a >= v2s && d >= tmk[k]) ? d : 0u;
-        }
-#pragma unroll
-        for (int k = 0; k < SC; ++k) {
-            rP[0][k] = rP[1][k];
-            rP[1][k] = rP[2][k];
-            rP[2][k] = pr[k];
+            v2s[k] = b3(rB[1][k], rB[3][k], rB[5][k], rB[7][k], rB[9][k]);
+            v1[k] = rV[0][k];
         }
         // Output row y3 = t - 7.
         const int y3 = t - 7;
-        if (kInterior || y3 >= ys) emit(y3);
+        finish(y3, kInterior || (y2 >= 0 && y2 < H), v1, v2s, kInterior || y3 >= ys);
+    };
+
+    // Interior step j (0..UNROLL-1) of an unrolled iteration: the rings are addressed circularly (the
+    // newest row of a ring of N at slot j % N, the row of age d at slot (j - d) mod N); the iteration
+    // starts and ends with them in shifted layout because UNROLL is a multiple of every N.
+    auto step_c = [&](int t, auto J) {
+        constexpr int j = decltype(J)::value;
+        uint32_t v0[SC];
+        next_row(t, v0);
+        uint32_t a[SC];
+        row_a(v0, a);
+        constexpr int sA = j % NA;
+#pragma unroll
+        for (int k = 0; k < SC; ++k) rA[sA][k] = a[k];
+        uint32_t u[SC];
+#pragma unroll
+        for (int k = 0; k < SC; ++k)
+            u[k] = b3(rA[(j + 1) % NA][k], rA[(j + 2) % NA][k], rA[(j + 3) % NA][k], rA[(j + 4) % NA][k], rA[sA][k]);
+        fix_edges(u);
+        uint32_t v1[SC];
+#pragma unroll
+        for (int k = 0; k < SC; ++k) {
+            v1[k] = rV[(j + 1) % NV][k];          // V1s(t - 6), age 4
+            rV[j % NV][k] = u[k];
+        }
+        uint32_t bb[SC];
+        row_b(u, bb);
+        constexpr int sB = j % NB;
+#pragma unroll
+        for (int k = 0; k < SC; ++k) rB[sB][k] = bb[k];
+        uint32_t v2s[SC];
+#pragma unroll
+        for (int k = 0; k < SC; ++k)
+            v2s[k] = b3(rB[(j + 2) % NB][k], rB[(j + 4) % NB][k], rB[(j + 6) % NB][k], rB[(j + 8) % NB][k], rB[sB][k]);
+        finish(t - 7, true, v1, v2s, true);
     };
 
     const int t0 = ys - 7, t1 = ye + 6;
     // Interior steps: output row inside the segment (t >= ys + 7) and V1s row inside the frame
     // (t - 2 <= H - 1); the P row t - 6 is then inside the frame too.
     const int tI0 = ys + 7, tI1 = min(t1, H + 1);
-    load_row(t0, nxt);
+#pragma unroll
+    for (int i = 0; i < PF; ++i) load_row(t0 + i, pre[i]);
     int t = t0;
     for (; t < tI0; ++t) step(t, std::false_type{});
     for (; t + UNROLL - 1 <= tI1; t += UNROLL) {
-#pragma unroll
-        for (int j = 0; j < UNROLL; ++j) step(t + j, std::true_type{});
+        step_c(t + 0, std::integral_constant<int, 0>{});
+        step_c(t + 1, std::integral_constant<int, 1>{});
+        step_c(t + 2, std::integral_constant<int, 2>{});
+        step_c(t + 3, std::integral_constant<int, 3>{});
+        step_c(t + 4, std::integral_constant<int, 4>{});
+        step_c(t + 5, std::integral_constant<int, 5>{});
+        step_c(t + 6, std::integral_constant<int, 6>{});
+        step_c(t + 7, std::integral_constant<int, 7>{});
+        step_c(t + 8, std::integral_constant<int, 8>{});
+        step_c(t + 9, std::integral_constant<int, 9>{});
     }
     for (; t <= tI1; ++t) step(t, std::true_type{});
     for (; t <= t1; ++t) step(t, std::false_type{});

@@ -51,6 +51,7 @@ struct Options {
     unsigned fitWaves = 0;          // persistent FP32 fit: resident waves (0: 16 per CU)
     int fitRefill = 16;             // persistent FP32 fit: free lanes that trigger a refill round
     int detectRows = 0;             // rows per detection wave (0: chosen from the chunk size)
+    int lanes = 2;                  // main lanes the chunks alternate between (1: one stream, for timing)
 };
 
 // One stream with its own detection buffers. The engine runs the chunks on the main lane and the

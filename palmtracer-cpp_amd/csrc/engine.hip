@@ -357,7 +357,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     const size_t npx = (size_t)W * H;
     const size_t F = std::max<size_t>(1, std::min<size_t>(planes, std::min<size_t>(65535, opt.chunkPixels / npx)));
     main_.prepare(F, npx);
-    if (planes > F) mainB_.prepare(F, npx);
+    if (planes > F && opt.lanes > 1) mainB_.prepare(F, npx);
     const FitParams fp = fit_params(fit, fitParams);
     const uint64_t frameRows = std::min<uint64_t>(maxPerPlane, npx / 4 + 2);   // rows a plane can hold
     unsigned long long* passes = nullptr;
@@ -496,7 +496,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     // end of one chunk (component floods, statistics, ROI list) overlaps the next chunk's detection.
     // Only the ROI scans are ordered across the lanes (the running ROI total).
     for (uint64_t z0 = 0; z0 < planes; z0 += F, ++chunkNo) {
-        Lane& L = (chunkNo & 1) ? mainB_ : main_;
+        Lane& L = (opt.lanes > 1 && (chunkNo & 1)) ? mainB_ : main_;
         const size_t f = std::min<uint64_t>(F, planes - z0);
         chunkZ0.push_back(z0);
         L.b.runTotal = dRun;
