@@ -231,21 +231,23 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
     // P < 2^32 - 1 (256 * V1s <= 2^32 - 2^16), so a saturated threshold passes nothing.
     const uint32_t tm = tmin > (int64_t)0xFFFFFFFFu ? 0xFFFFFFFFu : (uint32_t)tmin;
 
-    int xcol[SC];
-    uint32_t inOff[SC], outOff[SC];     // byte offsets of the lane's columns in a row of u16 / u32
-    uint32_t tmk[SC];                   // threshold per column; columns outside the frame never pass
-    bool outCol[SC];
+    // Per column k of the lane: its frame column x0 + k, the byte offset of its (clamped) input
+    // column, whether it lies in the frame (columns outside never pass the threshold) and whether
+    // the lane writes it. An output column is in the frame, so its P offset is 4 * (x0 + k).
+    const int x0 = xs + SC * lane;
+    uint32_t inOff[SC];
+    bool inCol[SC], outCol[SC], pastR[SC];
     const bool outLane = lane >= SHALO / SC && lane < 64 - SHALO / SC;
 #pragma unroll
     for (int k = 0; k < SC; ++k) {
-        xcol[k] = xs + SC * lane + k;
-        const int xc = xcol[k] < 0 ? 0 : (xcol[k] > W - 1 ? W - 1 : xcol[k]);
+        const int x = x0 + k;
+        const int xc = x < 0 ? 0 : (x > W - 1 ? W - 1 : x);
         inOff[k] = (uint32_t)xc * 2u;
-        outOff[k] = (uint32_t)xc * 4u;
-        const bool inFrame = xcol[k] >= 0 && xcol[k] < W;
-        tmk[k] = inFrame ? tm : 0xFFFFFFFFu;
-        outCol[k] = outLane && inFrame;
+        inCol[k] = x >= 0 && x < W;
+        outCol[k] = outLane && inCol[k];
+        pastR[k] = x > W - 1;
     }
+    const uint32_t outOff0 = (uint32_t)x0 * 4u;   // used only for columns in the frame
     // The frame's first column is strip column SHALO of strip 0 (lane SHALO / SC, slot 0); its last
     // column is strip column oR of the last strip.
     const bool fixL = xs < 0;
@@ -290,7 +292,7 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
         const int rowOff = (int)(rowBase * 4u);
 #pragma unroll
         for (int k = 0; k < SC; ++k)
-            if (outCol[k]) __builtin_amdgcn_raw_buffer_store_b32(v[k], rsP, (int)outOff[k], rowOff, 0);
+            if (outCol[k]) __builtin_amdgcn_raw_buffer_store_b32(v[k], rsP, (int)(outOff0 + 4u * k), rowOff, 0);
         // Pixels with P > 0 (columns outside the frame have P = 0) and the local maxima among them.
         const uint32_t cL = from_left(cm[2]), cR = from_right(cm[0]);
         const bool m0 = outLane && v[0] != 0u, m1 = outLane && v[1] != 0u, m2 = outLane && v[2] != 0u;
@@ -306,7 +308,7 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
         const uint32_t mc0 = (uint32_t)__popcll(mb0), mc1 = (uint32_t)__popcll(mb1);
         const uint32_t mRow = mc0 + mc1 + (uint32_t)__popcll(mb2);
         if (mRow == 0) return;
-        const uint32_t i0 = rowBase + (uint32_t)xcol[0], i1 = i0 + 1u, i2 = i0 + 2u;
+        const uint32_t i0 = rowBase + (uint32_t)x0, i1 = i0 + 1u, i2 = i0 + 2u;
         if (mPend + mRow > (uint32_t)MCAP) { flush(sMask, mPend, &maskCnt[f], ml); mPend = 0; }
         if (m0) sMask[mPend + lanes_below(mb0)] = i0;
         if (m1) sMask[mPend + mc0 + lanes_below(mb1)] = i1;
@@ -355,7 +357,7 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
         if (fixR) {
             const uint32_t e = __builtin_amdgcn_readlane(kR == 0 ? u[0] : (kR == 1 ? u[1] : u[2]), lR);
 #pragma unroll
-            for (int k = 0; k < SC; ++k) u[k] = xcol[k] > W - 1 ? e : u[k];
+            for (int k = 0; k < SC; ++k) u[k] = pastR[k] ? e : u[k];
         }
     };
     // B(y1) = 16 * rowpass_1(V1s): taps at columns c-4, c-2, c, c+2, c+4.
@@ -375,7 +377,7 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
         for (int k = 0; k < SC; ++k) {
             const uint32_t a = v1[k] << 8;
             const uint32_t d = a - v2s[k];
-            const uint32_t pr = (rowIn && a >= v2s[k] && d >= tmk[k]) ? d : 0u;
+            const uint32_t pr = (rowIn && inCol[k] && a >= v2s[k] && d >= tm) ? d : 0u;
             v[k] = Pp[k];
             cm[k] = max(Mp[k], pr);
             Mp[k] = max(Pp[k], pr);
