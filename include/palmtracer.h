@@ -150,8 +150,8 @@ PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint6
  * (1: that flood without the rank flood), "force_heap_watershed" (1: that flood with the heap
  * restatement of the reference), "fit_waves" (resident waves of the
  * persistent FP32 fit, 0: default), "fit_refill" (free lanes of an FP32 fit wave that start a refill
- * round, 1-64, default 16), "detect_rows" (rows per detection wave, 0: chosen from the chunk).
- * Returns 1 on success. */
+ * round, 1-64, default 16), "detect_rows" (rows per detection wave, 0: chosen from the chunk),
+ * "lanes" (2: consecutive chunks alternate between two streams; 1: one stream). Returns 1 on success. */
 PT_API int PT_CALL pt_set_option(const char* name, double value);
 
 /* Counters of the last call: frames, chunks, rows, frames flagged for the exact frame flood,
