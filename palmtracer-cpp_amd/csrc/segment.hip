@@ -1018,12 +1018,59 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
                     seedf[i] = idxOf[p];
                 }
             };
+            // Beyond the LDS: the bitonic network with every pass of stride below kRankLds run on
+            // kRankLds-key chunks in LDS (each pair of such a pass lies in one aligned chunk; the
+            // direction comes from the global index), the wider passes in the work space.
+            auto sort_large = [&]() {
+                constexpr uint32_t C = kRankLds;
+                for (uint32_t i = n + tid; i < n2; i += RF_NT) skg[i] = ~0ull;
+                __syncthreads();
+                auto chunk_passes = [&](uint32_t c0, uint32_t kk0, uint32_t kk1, uint32_t j0) {
+                    for (uint32_t i = tid; i < C; i += RF_NT) sk[i] = skg[c0 + i];
+                    __syncthreads();
+                    for (uint32_t kk = kk0; kk <= kk1; kk <<= 1) {
+                        for (uint32_t j = (kk == kk0 ? j0 : kk >> 1); j > 0; j >>= 1) {
+                            for (uint32_t i = tid; i < C; i += RF_NT) {
+                                const uint32_t ixj = i ^ j;
+                                if (ixj > i) {
+                                    const uint64_t a = sk[i], b = sk[ixj];
+                                    if ((a > b) == (((c0 + i) & kk) == 0)) { sk[i] = b; sk[ixj] = a; }
+                                }
+                            }
+                            __syncthreads();
+                        }
+                    }
+                    for (uint32_t i = tid; i < C; i += RF_NT) skg[c0 + i] = sk[i];
+                    __syncthreads();
+                };
+                for (uint32_t c0 = 0; c0 < n2; c0 += C) chunk_passes(c0, 2, C, 1);
+                for (uint32_t kk = 2 * C; kk <= n2; kk <<= 1) {
+                    for (uint32_t j = kk >> 1; j >= C; j >>= 1) {
+                        for (uint32_t i = tid; i < n2; i += RF_NT) {
+                            const uint32_t ixj = i ^ j;
+                            if (ixj > i) {
+                                const uint64_t a = skg[i], b = skg[ixj];
+                                if ((a > b) == ((i & kk) == 0)) { skg[i] = b; skg[ixj] = a; }
+                            }
+                        }
+                        __syncthreads();
+                    }
+                    for (uint32_t c0 = 0; c0 < n2; c0 += C) chunk_passes(c0, kk, kk, C >> 1);
+                }
+                for (uint32_t i = tid; i < n; i += RF_NT) {
+                    const uint64_t v = skg[i];
+                    const uint32_t p = (uint32_t)v;
+                    sp[i] = ~(uint32_t)(v >> 32);
+                    spix[i] = p;
+                    seedf[i] = idxOf[p];
+                }
+            };
             if (n <= kRankLds) {
                 for (uint32_t i = tid; i < n; i += RF_NT) sk[i] = skg[i];
                 __syncthreads();
                 sort_entries(sk);
             } else {
-                sort_entries(skg);
+                sort_large();
             }
             __syncthreads();
             // 4. Pixel -> entry map; windows; flooded neighbours.
@@ -1033,21 +1080,35 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
             __syncthreads();
             // Potentials further apart than D quanta keep their order for every age <= n.
             const uint64_t D = (uint64_t)floor(((double)n * kEpsilon + 1e-9) * 65536.0) + 1;
+            // lo[i]: first entry with P <= P_i + D; hi[i]: first entry with P < P_i - D. Both only grow
+            // with i (P descending), so each thread binary-searches them for the first entry of a
+            // contiguous block and sweeps forward over the rest of it.
+            {
+                const uint32_t per = (n + RF_NT - 1) / RF_NT, i0 = min(n, tid * per), i1 = min(n, i0 + per);
+                uint32_t a = 0, b = n, lw = 0, hw = 0;
+                if (i0 < i1) {
+                    const uint64_t p0 = sp[i0];
+                    while (a < b) {
+                        const uint32_t m = (a + b) >> 1;
+                        if ((uint64_t)sp[m] > p0 + D) a = m + 1; else b = m;
+                    }
+                    lw = a;
+                    a = 0; b = n;
+                    while (a < b) {
+                        const uint32_t m = (a + b) >> 1;
+                        if ((uint64_t)sp[m] + D >= p0) a = m + 1; else b = m;
+                    }
+                    hw = a;
+                }
+                for (uint32_t i = i0; i < i1; ++i) {
+                    const uint64_t pi = sp[i];
+                    while (lw < n && (uint64_t)sp[lw] > pi + D) ++lw;
+                    while (hw < n && (uint64_t)sp[hw] + D >= pi) ++hw;
+                    lo[i] = lw;
+                    hi[i] = hw;
+                }
+            }
             for (uint32_t i = tid; i < n; i += RF_NT) {
-                const uint64_t pi = sp[i];
-                // lo: first entry with P <= pi + D; hi: first entry with P < pi - D (P descending).
-                uint32_t a = 0, b = n;
-                while (a < b) {
-                    const uint32_t m = (a + b) >> 1;
-                    if ((uint64_t)sp[m] > pi + D) a = m + 1; else b = m;
-                }
-                lo[i] = a;
-                a = 0; b = n;
-                while (a < b) {
-                    const uint32_t m = (a + b) >> 1;
-                    if ((uint64_t)sp[m] + D >= pi) a = m + 1; else b = m;
-                }
-                hi[i] = a;
                 const uint32_t p = spix[i];
                 const int y = (int)(p / W), x = (int)(p % W);
 #pragma unroll
