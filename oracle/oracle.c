@@ -1,7 +1,8 @@
 /* CPU oracle: FP64 restatement of the PALMTracer detection + fitting path.
  *
  * TEST INFRASTRUCTURE ONLY. Imported by tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg, as the checker. The product library never links or calls it.
+ * cpu_baseline leg, as the reference against which results are compared. The product library
+ * never links or calls it.
  *
  * Reference: tmonseigne/PALMTracer-cpp (paths relative to its root).
  *

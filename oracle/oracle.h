@@ -2,7 +2,7 @@
  *
  * TEST INFRASTRUCTURE ONLY: an FP64 restatement in plain C of the reference algorithm
  * (tmonseigne/PALMTracer-cpp, lib/), used by tests/, the smoke check and the bench's CPU
- * baseline leg as the checker. The product library never links, loads or calls it.
+ * baseline leg, to compare results against. The product library never links, loads or calls it.
  *
  * Every function names the reference file:line it restates. Pinning (how each part is
  * checked independently of this restatement) is listed in oracle.c's header.

@@ -1,7 +1,7 @@
 """ctypes access to the CPU oracle (liboracle.so) and, where built, the compiled reference (_ref/libref.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
-leg as the checker; the product never imports it.
+leg, to compare results against; the product never imports it.
 """
 import ctypes
 import os
