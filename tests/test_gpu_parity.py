@@ -581,3 +581,21 @@ def test_large_frames_localization(pt_gpu, oracle_mod, large_frames):
         got = pt_gpu.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN)
         exp = oracle_mod.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN)
         assert np.array_equal(got, exp)
+
+
+def test_tuning_options_do_not_change_rows(pt_gpu, frames512):
+    """Rows are identical with one or two main lanes and for any detection row-segment height."""
+    st, thr = frames512
+    base = pt_gpu.localization(st, thr, 10.0, 2, FIT_PARAMS_SYN)
+    assert base.shape[0] > 0
+    try:
+        pt_gpu.set_option("chunk_pixels", 5 * 512 * 512)      # several chunks, so both lanes run
+        for name, value in [("lanes", 1), ("lanes", 2), ("detect_rows", 37), ("detect_rows", 103),
+                            ("detect_rows", 512)]:
+            pt_gpu.set_option(name, value)
+            got = pt_gpu.localization(st, thr, 10.0, 2, FIT_PARAMS_SYN)
+            assert np.array_equal(got, base), (name, value)
+    finally:
+        pt_gpu.set_option("lanes", 2)
+        pt_gpu.set_option("detect_rows", 0)
+        pt_gpu.set_option("chunk_pixels", 0)
