@@ -78,6 +78,43 @@ def cpu_baseline(stack_host, thr, fit_params, budget_s):
             "sample": f"first {nf} frames of the same stack ({rows} rows in {dt:.2f} s), OpenMP over frames"}
 
 
+def fit_precision(pt, torch, stack, thr, fit_params, frames):
+    """FP32 fit (the bench's) against the FP64 fit (the reference's arithmetic, equal to the CPU oracle
+    to ~1e-14 in the parity tests) on the first frames of the stack: rows paired within each plane by
+    position, largest errors over rows fitted on both sides."""
+    import numpy as np
+    sub = stack[:frames]
+    out = torch.empty(frames * 1000 * 18, dtype=torch.float64, device="cuda")
+    res = {}
+    for prec in (0, 1):
+        pt.set_option("fit_fp64", prec)
+        n = pt.localization_device(sub, out, thr, 0.0, 2, fit_params, 1000)
+        res[prec] = out[: n * 18].view(-1, 18).cpu().numpy().copy()
+    pt.set_option("fit_fp64", 0)
+    a, e = res[0], res[1]
+    X, Y, INT, SX, AMP, OFF = 4, 5, 7, 8, 13, 14
+    pairs = []
+    for p in np.unique(e[:, 1]):
+        A, E = a[a[:, 1] == p], e[e[:, 1] == p]
+        d = ((A[:, None, X:Y + 1] - E[None, :, X:Y + 1]) ** 2).sum(-1)
+        j = d.argmin(1)
+        pairs.append((A, E[j]))
+    A = np.concatenate([q[0] for q in pairs])
+    E = np.concatenate([q[1] for q in pairs])
+    both = (A[:, AMP] > 0) & (E[:, AMP] > 0)
+    one = int(((A[:, AMP] > 0) != (E[:, AMP] > 0)).sum())
+    A, E = A[both], E[both]
+    rel = lambda c: np.abs(A[:, c] - E[:, c]) / np.maximum(np.abs(E[:, c]), 1e-300)
+    within = ((np.abs(A[:, X] - E[:, X]) <= 1e-3) & (np.abs(A[:, Y] - E[:, Y]) <= 1e-3) & (rel(INT) <= 1e-4) &
+              (rel(SX) <= 1e-4) & (rel(AMP) <= 1e-4) & (rel(OFF) <= 1e-4))
+    return {"frames": frames, "rows": int(len(e)), "rows_fp32": int(len(a)), "fitted_on_one_side_only": one,
+            "max_abs_dx_px": float(np.abs(A[:, X] - E[:, X]).max()), "max_abs_dy_px": float(np.abs(A[:, Y] - E[:, Y]).max()),
+            "max_rel_intensity": float(rel(INT).max()), "max_rel_amplitude": float(rel(AMP).max()),
+            "max_rel_sigma": float(rel(SX).max()), "max_rel_offset": float(rel(OFF).max()),
+            "fraction_within_tolerance": float(within.mean()),
+            "tolerance": "x, y 1e-3 px; intensity, amplitude, sigma, offset 1e-4 relative"}
+
+
 def serve_latency(pt, torch, seconds, slots, n_cams, sizes, fit_params):
     """Streamed load: n_cams simulated cameras, each with a fixed frame size (`sizes` in turn), a frame
     rate uniform in [100, 1000] fps and chunks of 10-100 frames (seeded), the
@@ -294,6 +331,8 @@ def main():
                 "bytes_per_launch": bytes_l}
     stage_ms = {k: v / args.steps for k, v in stages.items() if k.endswith("_ms")}
 
+    precision = fit_precision(pt, torch, stack, thr, fit_params, min(F, 200)) if args.precision == "fp32" else None
+
     serving = None
     if args.serve_seconds > 0:
         sizes = [int(v) for v in args.serve_sizes.split(",")]
@@ -328,6 +367,7 @@ def main():
                                                              "rank_unsettled", "rank_two_orders", "heap_frames",
                                                              "multi_components")}},
         "roofline": roof,
+        "fit_precision": precision,
         "serving": serving,
         "cpu_baseline": cpu,
     }
