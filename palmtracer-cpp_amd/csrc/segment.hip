@@ -316,7 +316,10 @@ __device__ inline bool flood_small(const uint32_t* list, uint32_t cnt, const uin
     return true;
 }
 
-__global__ void __launch_bounds__(FL_NT) frame_label_kernel(
+// At most 80 SGPRs: MI355X then holds two of these 1024-thread workgroups per CU (8 waves per
+// SIMD); from about 86 SGPRs on it holds one (measured with a residency probe), which halves the
+// overlap of the workgroups' barrier-separated phases.
+__global__ void __launch_bounds__(FL_NT) __attribute__((amdgpu_num_sgpr(80))) frame_label_kernel(
     const uint32_t* __restrict__ P, const uint32_t* __restrict__ seedBits, uint32_t* __restrict__ lab,
     uint32_t* __restrict__ fin, PixAux* __restrict__ aux, const uint32_t* __restrict__ maskList,
     const uint32_t* __restrict__ maskCnt, const uint32_t* __restrict__ seedList, const uint32_t* __restrict__ seedCnt,
@@ -365,12 +368,15 @@ __global__ void __launch_bounds__(FL_NT) frame_label_kernel(
     };
 
     if (ldsCap > 0 && n <= ldsCap) {
-        const uint32_t nw = (uint32_t)((npx + 31) / 32);
+        const uint32_t nd = (uint32_t)((npx + 63) / 64), nw = 2 * nd;   // bitmap in 64-bit and 32-bit words
         uint32_t* mbits = reinterpret_cast<uint32_t*>(smem);
-        uint16_t* pre = reinterpret_cast<uint16_t*>(mbits + nw);
+        const uint64_t* mb64 = reinterpret_cast<const uint64_t*>(smem);
+        uint16_t* pre = reinterpret_cast<uint16_t*>(mbits + nw);      // mask pixels before each 64-bit word
         uint32_t* parent = mbits + parentOff;
         uint32_t* state = mbits;               // reuses the bitmap once every rank is known
-        auto rank = [&](uint32_t q) { return (uint32_t)pre[q >> 5] + __popc(mbits[q >> 5] & ((1u << (q & 31)) - 1u)); };
+        auto rank = [&](uint32_t q) {
+            return (uint32_t)pre[q >> 6] + (uint32_t)__popcll(mb64[q >> 6] & ((1ull << (q & 63)) - 1ull));
+        };
         auto in_mask = [&](uint32_t q) { return (mbits[q >> 5] >> (q & 31)) & 1u; };
         auto ld = [&](const uint32_t* a) { return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); };
         auto find = [&](uint32_t x) {
@@ -391,9 +397,9 @@ __global__ void __launch_bounds__(FL_NT) frame_label_kernel(
         }
         __syncthreads();
         {
-            const uint32_t per = (nw + FL_NT - 1) / FL_NT, w0 = min(nw, tid * per), w1 = min(nw, w0 + per);
+            const uint32_t per = (nd + FL_NT - 1) / FL_NT, w0 = min(nd, tid * per), w1 = min(nd, w0 + per);
             uint32_t s = 0;
-            for (uint32_t w = w0; w < w1; ++w) s += __popc(mbits[w]);
+            for (uint32_t w = w0; w < w1; ++w) s += (uint32_t)__popcll(mb64[w]);
             uint32_t incl = s;
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
@@ -407,16 +413,30 @@ __global__ void __launch_bounds__(FL_NT) frame_label_kernel(
             uint32_t run = base + incl - s;
             for (uint32_t w = w0; w < w1; ++w) {
                 pre[w] = (uint16_t)run;
-                run += __popc(mbits[w]);
+                run += (uint32_t)__popcll(mb64[w]);
             }
         }
         __syncthreads();
-        // 2. Union-find over ranks: every pixel with its earlier 8-neighbours W, NW, N, NE.
+        // 2. Union-find over ranks. A run of mask pixels in a row is one set from the start: every
+        //    pixel's parent is the run's first pixel (the ranks of a run are consecutive). Run R =
+        //    [a, b] of row y touches (8-connectivity) each run U = [c, d] of row y-1 with c <= b+1
+        //    and d >= a-1, and is united with it once, from one pixel: from a with NW when c < a,
+        //    from a with N when c == a, from c-1 with NE when c > a (there N is not in the mask).
 #pragma unroll
         for (int j = 0; j < FL_PER; ++j) {
             if (tid + j * FL_NT < n) {
-                rr[j] = rank(pp[j]);
-                parent[rr[j]] = rr[j];
+                const uint32_t p = pp[j];
+                const uint32_t rowStart = p - p % (uint32_t)W;
+                uint32_t q = p;                       // [q, p] is in the mask
+                while (q > rowStart) {
+                    const uint32_t w = q >> 5, zeros = ~mbits[w] & ((1u << (q & 31)) - 1u);
+                    if (zeros) { q = max(rowStart, (w << 5) + 32u - (uint32_t)__clz(zeros)); break; }
+                    q = max(rowStart, w << 5);
+                    if (q == rowStart || !in_mask(q - 1)) break;
+                    --q;
+                }
+                rr[j] = rank(p);
+                parent[rr[j]] = rr[j] - (p - q);
             }
         }
         __syncthreads();
@@ -425,6 +445,7 @@ __global__ void __launch_bounds__(FL_NT) frame_label_kernel(
             if (tid + j * FL_NT >= n) continue;
             const uint32_t p = pp[j];
             const int y = (int)(p / (uint32_t)W), x = (int)p - y * W;
+            if (y == 0) continue;
             auto unite = [&](uint32_t q) {
                 uint32_t a = rr[j], b = rank(q);
                 while (true) {
@@ -437,13 +458,13 @@ __global__ void __launch_bounds__(FL_NT) frame_label_kernel(
                     b = old;
                 }
             };
-            if (x > 0 && in_mask(p - 1)) unite(p - 1);
-            if (y > 0) {
-                const uint32_t up = p - (uint32_t)W;
+            const uint32_t up = p - (uint32_t)W;
+            const bool nIn = in_mask(up);
+            if (x == 0 || !in_mask(p - 1)) {          // p starts its run
                 if (x > 0 && in_mask(up - 1)) unite(up - 1);
-                if (in_mask(up)) unite(up);
-                if (x < W - 1 && in_mask(up + 1)) unite(up + 1);
+                else if (nIn) unite(up);
             }
+            if (!nIn && x < W - 1 && in_mask(up + 1)) unite(up + 1);
         }
         __syncthreads();
         // 3. Flatten: every rank points at its component (the component's smallest rank).
@@ -596,8 +617,8 @@ __global__ void __launch_bounds__(FL_NT) frame_label_kernel(
 // the offset of the parents in words.
 size_t frame_label_lds(size_t npx, uint32_t& ldsCap, uint32_t& parentOff)
 {
-    const size_t nw = (npx + 31) / 32;
-    const size_t base = nw * 4 + (nw + 1) / 2 * 4;        // bitmap + u16 prefix counts
+    const size_t nd = (npx + 63) / 64;
+    const size_t base = nd * 8 + (nd + 1) / 2 * 4;        // bitmap + u16 prefix counts per 64-bit word
     const size_t staticBytes = 256;
     auto total = [&](size_t cap) { return std::max(base, cap * 4) + cap * 4; };
     auto fit = [&](size_t budget) {
