@@ -186,6 +186,16 @@ __device__ inline uint32_t b3(uint32_t m2, uint32_t m1, uint32_t c, uint32_t p1,
 {
     return ((m1 + p1 + c) << 2) + (c << 1) + m2 + p2;
 }
+// The same sum when the centre tap is below 2^24 (the input, A and V1s; not B): 6 * c added by one
+// 24-bit multiply-add, three VALU operations instead of four (the compiler splits c * 6 + t into a
+// multiply and an add, so the multiply-add is written out).
+__device__ inline uint32_t b3s(uint32_t m2, uint32_t m1, uint32_t c, uint32_t p1, uint32_t p2)
+{
+    const uint32_t t = m2 + p2 + ((m1 + p1) << 2);
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, 6, %2" : "=v"(r) : "v"(c), "v"(t));
+    return r;
+}
 
 __device__ inline uint32_t max3u(uint32_t a, uint32_t b, uint32_t c) { return max(max(a, b), c); }
 
@@ -208,7 +218,9 @@ __device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
 }
 
-__global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __restrict__ in, uint32_t* __restrict__ P,
+// Six waves per SIMD: the kernel fits in 80 VGPRs without spills, and at five (82 VGPRs unconstrained)
+// the SIMDs idle on its dependency chains; six cut the launch by ~10 %.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) atrous_detect_kernel(const uint16_t* __restrict__ in, uint32_t* __restrict__ P,
                                                            uint32_t* __restrict__ maskList,
                                                            uint32_t* __restrict__ maskCnt, uint32_t* __restrict__ seedList,
                                                            uint32_t* __restrict__ seedCnt, uint32_t* __restrict__ seedBits,
@@ -343,9 +355,9 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
     auto row_a = [&](const uint32_t (&v0)[SC], uint32_t (&a)[SC]) {
         const uint32_t l1 = from_left(v0[1]), l2 = from_left(v0[2]);
         const uint32_t r0 = from_right(v0[0]), r1 = from_right(v0[1]);
-        a[0] = b3(l1, l2, v0[0], v0[1], v0[2]);
-        a[1] = b3(l2, v0[0], v0[1], v0[2], r0);
-        a[2] = b3(v0[0], v0[1], v0[2], r0, r1);
+        a[0] = b3s(l1, l2, v0[0], v0[1], v0[2]);
+        a[1] = b3s(l2, v0[0], v0[1], v0[2], r0);
+        a[2] = b3s(v0[0], v0[1], v0[2], r0, r1);
     };
     // Columns outside the frame hold the edge column's V1s (clamp of the scale-1 taps).
     auto fix_edges = [&](uint32_t (&u)[SC]) {
@@ -365,9 +377,9 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
         const uint32_t L0 = from_left(u[0]), L1 = from_left(u[1]), L2 = from_left(u[2]);
         const uint32_t R0 = from_right(u[0]), R1 = from_right(u[1]), R2 = from_right(u[2]);
         const uint32_t LL2 = from_left(L2), RR0 = from_right(R0);
-        bb[0] = b3(LL2, L1, u[0], u[2], R1);
-        bb[1] = b3(L0, L2, u[1], R0, R2);
-        bb[2] = b3(L1, u[0], u[2], R1, RR0);
+        bb[0] = b3s(LL2, L1, u[0], u[2], R1);
+        bb[1] = b3s(L0, L2, u[1], R0, R2);
+        bb[2] = b3s(L1, u[0], u[2], R1, RR0);
     };
     // 65536 * W1 = 256 * V1s - V2s, kept when >= tmin (u32: both terms are below 2^32); then the
     // running column maxima and the output row.
@@ -404,7 +416,7 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
         const int y1 = t - 2;
         uint32_t u[SC];
 #pragma unroll
-        for (int k = 0; k < SC; ++k) u[k] = b3(rA[0][k], rA[1][k], rA[2][k], rA[3][k], rA[4][k]);
+        for (int k = 0; k < SC; ++k) u[k] = b3s(rA[0][k], rA[1][k], rA[2][k], rA[3][k], rA[4][k]);
         fix_edges(u);
 #pragma unroll
         for (int k = 0; k < SC; ++k) {
@@ -469,7 +481,7 @@ __global__ void __launch_bounds__(64) atrous_detect_kernel(const uint16_t* __res
         uint32_t u[SC];
 #pragma unroll
         for (int k = 0; k < SC; ++k)
-            u[k] = b3(rA[(j + 1) % NA][k], rA[(j + 2) % NA][k], rA[(j + 3) % NA][k], rA[(j + 4) % NA][k], rA[sA][k]);
+            u[k] = b3s(rA[(j + 1) % NA][k], rA[(j + 2) % NA][k], rA[(j + 3) % NA][k], rA[(j + 4) % NA][k], rA[sA][k]);
         fix_edges(u);
         uint32_t v1[SC];
 #pragma unroll
