@@ -87,6 +87,37 @@ def test_labels_threshold_edges(pt_gpu, oracle_mod, ref_vectors):
         assert np.array_equal(rank_labels(got), exp)
 
 
+def _odd_frame(h, w, seed):
+    """Spots of several widths, one broad blob (long mask runs, several maxima) and shot noise."""
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    img = np.full((h, w), 100.0)
+    for _ in range(max(1, h * w // 150)):
+        cy, cx = rng.uniform(-2, h + 1), rng.uniform(-2, w + 1)
+        sg, amp = rng.uniform(1.0, 2.5), rng.uniform(300, 3000)
+        img += amp * np.exp(-((yy - cy) ** 2 + (xx - cx) ** 2) / (2 * sg * sg))
+    img += 1500 * np.exp(-((yy - h / 2) ** 2 + (xx - w / 3) ** 2) / (2 * 6.0 ** 2))
+    return np.clip(rng.poisson(img), 0, 65535).astype(np.uint16)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 70), (70, 1), (23, 37), (33, 65), (61, 97), (7, 300), (90, 91)])
+def test_labels_odd_shapes(pt_gpu, oracle_mod, shape):
+    """Frame widths and pixel counts that are not multiples of the bitmap words: mask runs that start
+    inside a word and cross word and row ends, at sparse, dense and full masks."""
+    h, w = shape
+    fr = _odd_frame(h, w, 1000 + h * 7 + w)
+    w1 = oracle_mod.atrous(fr)[1]
+    for thr in (float(np.percentile(w1, 90)), float(np.percentile(w1, 50)), -1e9):
+        exp = oracle_mod.watershed_labels(w1, thr)
+        got = pt_gpu.debug_labels(fr, thr)
+        assert np.array_equal(got > 0, exp > 0), f"mask differs at {shape} thr {thr}"
+        assert np.array_equal(rank_labels(got), exp), f"labels differ at {shape} thr {thr}"
+    st = fr[None]
+    thr = float(np.percentile(w1, 80))
+    assert np.array_equal(pt_gpu.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN),
+                          oracle_mod.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN))
+
+
 # ------------------------------------------------------------------ localization ------------------
 @pytest.mark.parametrize("name", ["stack", "small", "dense"])
 def test_localization_no_fit_bit_exact(pt_gpu, oracle_mod, ref_inputs, ref_vectors, name):
