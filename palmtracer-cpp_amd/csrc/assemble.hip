@@ -121,20 +121,66 @@ __global__ void __launch_bounds__(NT) sort_frames_big_kernel(const double* __res
     }
 }
 
+// Placement plan of a call, one workgroup: plane p's rows are the main rows [base[p], base[p + 1])
+// of the staged array, unless p is one of the nDef deferred planes listed in def (planes, then
+// their first side row, then their row counts), whose rows start at staged row sideOff + that side
+// row. dst is the exclusive prefix of the counts in plane order (lib/Palm.cpp:24-43: planes one
+// after another), *total their sum.
+constexpr int PL_NT = 1024;
+__global__ void __launch_bounds__(PL_NT) plan_kernel(const uint32_t* __restrict__ base, uint64_t planes,
+                                                     const uint64_t* __restrict__ def, uint32_t nDef, uint64_t sideOff,
+                                                     uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
+                                                     uint32_t* __restrict__ cnt, uint64_t* __restrict__ total)
+{
+    __shared__ uint64_t waveSum[PL_NT / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (uint64_t p = t; p < planes; p += PL_NT) {
+        cnt[p] = base[p + 1] - base[p];
+        src[p] = base[p];
+    }
+    __syncthreads();
+    for (uint32_t k = t; k < nDef; k += PL_NT) {
+        const uint64_t p = def[k];
+        src[p] = sideOff + def[nDef + k];
+        cnt[p] = (uint32_t)def[2 * (uint64_t)nDef + k];
+    }
+    __syncthreads();
+    const uint64_t per = (planes + PL_NT - 1) / PL_NT, p0 = min(planes, t * per), p1 = min(planes, p0 + per);
+    uint64_t sum = 0;
+    for (uint64_t p = p0; p < p1; ++p) sum += cnt[p];
+    uint64_t incl = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t u = __shfl_up(incl, o);
+        if (lane >= o) incl += u;
+    }
+    if (lane == 63) waveSum[wave] = incl;
+    __syncthreads();
+    uint64_t run = incl - sum;
+    for (int w = 0; w < wave; ++w) run += waveSum[w];
+    for (uint64_t p = p0; p < p1; ++p) {
+        dst[p] = run;
+        run += cnt[p];
+    }
+    if (t == PL_NT - 1) *total = run;
+}
+
 // One workgroup per plane: rows to their final place with the global Id and the plane number
-// (lib/Palm.cpp:24-43).
+// (lib/Palm.cpp:24-43); output rows at or past cap are not written.
 __global__ void __launch_bounds__(NT) scatter_kernel(const double* __restrict__ staged, const uint64_t* __restrict__ src,
                                                      const uint64_t* __restrict__ dst, const uint32_t* __restrict__ count,
-                                                     double* __restrict__ out, uint64_t p0)
+                                                     double* __restrict__ out, uint64_t p0, uint64_t cap)
 {
     const uint64_t p = p0 + blockIdx.x;
-    const uint32_t n = count[p];
+    const uint64_t d = dst[p];
+    if (d >= cap) return;
+    const uint32_t n = (uint32_t)min<uint64_t>(count[p], cap - d);
     const double* a = staged + src[p] * kCols;
-    double* b = out + dst[p] * kCols;
+    double* b = out + d * kCols;
     for (uint32_t i = threadIdx.x; i < n * kCols; i += NT) {
         const uint32_t r = i / kCols, c = i - r * kCols;
         double v = a[i];
-        if (c == kId) v = (double)(dst[p] + r + 1);
+        if (c == kId) v = (double)(d + r + 1);
         else if (c == kPlane) v = (double)(p + 1);
         b[i] = v;
     }
@@ -152,12 +198,19 @@ void launch_sort_frames(const double* d_rows, const uint32_t* d_roiBase, int fra
     PT_CHECK(hipGetLastError());
 }
 
+void launch_plan(const uint32_t* d_base, uint64_t planes, const uint64_t* d_def, uint32_t nDef, uint64_t sideOff,
+                 uint64_t* d_src, uint64_t* d_dst, uint32_t* d_count, uint64_t* d_total, hipStream_t s)
+{
+    plan_kernel<<<1, PL_NT, 0, s>>>(d_base, planes, d_def, nDef, sideOff, d_src, d_dst, d_count, d_total);
+    PT_CHECK(hipGetLastError());
+}
+
 void launch_scatter(const double* d_staged, const uint64_t* d_src, const uint64_t* d_dst, const uint32_t* d_count,
-                    uint64_t planes, double* d_out, hipStream_t s)
+                    uint64_t planes, double* d_out, uint64_t cap, hipStream_t s)
 {
     for (uint64_t p0 = 0; p0 < planes; p0 += 1u << 30) {
         const uint64_t n = std::min<uint64_t>(planes - p0, 1u << 30);
-        scatter_kernel<<<(unsigned)n, NT, 0, s>>>(d_staged, d_src, d_dst, d_count, d_out, p0);
+        scatter_kernel<<<(unsigned)n, NT, 0, s>>>(d_staged, d_src, d_dst, d_count, d_out, p0, cap);
     }
     PT_CHECK(hipGetLastError());
 }

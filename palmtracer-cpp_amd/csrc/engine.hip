@@ -536,40 +536,34 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     PT_CHECK(hipStreamSynchronize(side_.stream));
     last.deferredFrames = deferred.size();
 
-    // Per plane: staged row offset and row count; deferred planes take the side lane's rows, placed
-    // after the main lane's.
-    std::vector<uint32_t> hb(planes + 1);
-    PT_CHECK(hipMemcpyAsync(hb.data(), dBase, (planes + 1) * 4, hipMemcpyDeviceToHost, main_.stream));
-    PT_CHECK(hipStreamSynchronize(main_.stream));
-    std::vector<uint64_t> src(planes);
-    std::vector<uint32_t> cnt(planes);
-    for (uint64_t p = 0; p < planes; ++p) {
-        src[p] = hb[p];
-        cnt[p] = hb[p + 1] - hb[p];
-    }
-    for (size_t k = 0; k < deferred.size(); ++k) {
-        src[deferred[k]] = totalRois + sideSrc[k];
-        cnt[deferred[k]] = sideCnt[k];
+    // Placement in plane order, planned on the device from the plane bases of the main rows; the
+    // deferred planes take the side lane's rows, placed after the main lane's in the staged array,
+    // and are the only host input. The host reads the row total once, after the last copy.
+    const uint32_t nDef = (uint32_t)deferred.size();
+    uint64_t* hp = reinterpret_cast<uint64_t*>(main_.host(2 * (3 * (size_t)nDef + 1)));
+    for (uint32_t k = 0; k < nDef; ++k) {
+        hp[k] = deferred[k];
+        hp[nDef + k] = sideSrc[k];
+        hp[2 * (size_t)nDef + k] = sideCnt[k];
     }
     if (sideRowsTotal) {
         grow(main_.staged, (totalRois + sideRowsTotal) * kCols * 8, totalRois * kCols * 8, main_.stream);
         PT_CHECK(hipMemcpyAsync(main_.staged.as<double>() + totalRois * kCols, sideRows_.p, sideRowsTotal * kCols * 8,
                                 hipMemcpyDeviceToDevice, main_.stream));
     }
-    // Final placement in plane order.
-    std::vector<uint64_t> dst(planes);
-    uint64_t total = 0;
-    for (uint64_t p = 0; p < planes; ++p) { dst[p] = total; total += cnt[p]; }
-    if (total > outCapRows) throw Error("output capacity exceeded");
-    planeInfo.ensure(planes * 20 + 64);
+    planeInfo.ensure(planes * 20 + (3 * (uint64_t)nDef + 1) * 8 + 64);
     uint64_t* dSrc = planeInfo.as<uint64_t>();
     uint64_t* dDst = dSrc + planes;
     uint32_t* dCnt = reinterpret_cast<uint32_t*>(dDst + planes);
-    PT_CHECK(hipMemcpyAsync(dSrc, src.data(), planes * 8, hipMemcpyHostToDevice, stream));
-    PT_CHECK(hipMemcpyAsync(dDst, dst.data(), planes * 8, hipMemcpyHostToDevice, stream));
-    PT_CHECK(hipMemcpyAsync(dCnt, cnt.data(), planes * 4, hipMemcpyHostToDevice, stream));
-    if (total) launch_scatter(main_.staged.as<double>(), dSrc, dDst, dCnt, planes, d_out, stream);
-    PT_CHECK(hipStreamSynchronize(stream));
+    uint64_t* dDef = dDst + planes + (planes + 1) / 2;      // after the counts, 8-byte aligned
+    uint64_t* dTotal = dDef + 3 * (uint64_t)nDef;
+    if (nDef) PT_CHECK(hipMemcpyAsync(dDef, hp, 3 * (size_t)nDef * 8, hipMemcpyHostToDevice, main_.stream));
+    launch_plan(dBase, planes, dDef, nDef, totalRois, dSrc, dDst, dCnt, dTotal, main_.stream);
+    launch_scatter(main_.staged.as<double>(), dSrc, dDst, dCnt, planes, d_out, outCapRows, main_.stream);
+    PT_CHECK(hipMemcpyAsync(hp + 3 * (size_t)nDef, dTotal, 8, hipMemcpyDeviceToHost, main_.stream));
+    PT_CHECK(hipStreamSynchronize(main_.stream));
+    const uint64_t total = hp[3 * (size_t)nDef];
+    if (total > outCapRows) throw Error("output capacity exceeded");
     if (opt.profile) {
         for (size_t c = 0; c < last.chunks; ++c) {
             float ms[2] = {0, 0};

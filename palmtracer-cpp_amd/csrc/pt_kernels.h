@@ -178,10 +178,16 @@ void launch_spline_transpose(const double* d_raw, double* d_cv, uint64_t ssize, 
 // Index is set, Id and Plane are left for launch_scatter. maxRows: the most rows a frame can hold.
 void launch_sort_frames(const double* d_rows, const uint32_t* d_roiBase, int frames, double* d_staged, hipStream_t s,
                         uint64_t maxRows);
+// Placement plan on the device: per plane p, its staged first row src[p] (main rows from the plane
+// bases base[0..planes]; the nDef deferred planes from d_def = {planes, first side rows, counts},
+// offset by sideOff), its row count and its output row dst[p] (prefix of the counts); *d_total the
+// number of rows.
+void launch_plan(const uint32_t* d_base, uint64_t planes, const uint64_t* d_def, uint32_t nDef, uint64_t sideOff,
+                 uint64_t* d_src, uint64_t* d_dst, uint32_t* d_count, uint64_t* d_total, hipStream_t s);
 // Final placement: for each plane p, count[p] rows from staged row src[p] to output row dst[p],
-// with Id = dst[p] + k + 1 and Plane = p + 1.
+// with Id = dst[p] + k + 1 and Plane = p + 1; output rows from cap on are not written.
 void launch_scatter(const double* d_staged, const uint64_t* d_src, const uint64_t* d_dst, const uint32_t* d_count,
-                    uint64_t planes, double* d_out, hipStream_t s);
+                    uint64_t planes, double* d_out, uint64_t cap, hipStream_t s);
 
 void launch_exclusive_scan(const uint32_t* d_in, uint32_t* d_out, int n, hipStream_t s);
 // AstigFit::Estimate of n (sigma_x, sigma_y) pairs (d_in, 2n doubles) into (z, score) pairs (d_out);

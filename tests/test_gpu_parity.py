@@ -180,6 +180,22 @@ def test_empty_inputs(pt_gpu):
     assert pt_gpu.fitting(np.zeros((1, 16, 16), np.uint16), np.zeros((0, 6)), 2).shape == (0, 18)
 
 
+def test_output_capacity_exceeded(pt_gpu, frames512):
+    """An output smaller than the rows of the call is an error (no rows returned) and nothing is
+    written past it; with room for every row the device entry returns them all."""
+    import torch
+    st, thr = frames512
+    d = torch.from_numpy(st).cuda()
+    full = torch.zeros(d.shape[0] * 400 * 18, dtype=torch.float64, device="cuda")
+    n = pt_gpu.localization_device(d, full, thr, 0.0, 2, FIT_PARAMS_SYN, 400)
+    assert n > 10
+    buf = torch.full(((n + 8) * 18,), -7.0, dtype=torch.float64, device="cuda")
+    assert pt_gpu.localization_device(d, buf[: (n - 5) * 18], thr, 0.0, 2, FIT_PARAMS_SYN, 400) == 0
+    assert bool((buf[(n - 5) * 18:] == -7.0).all())
+    assert pt_gpu.localization_device(d, buf[: n * 18], thr, 0.0, 2, FIT_PARAMS_SYN, 400) == n
+    assert torch.equal(buf[: n * 18], full[: n * 18]) and bool((buf[n * 18:] == -7.0).all())
+
+
 def test_frames_are_independent(pt_gpu, ref_inputs):
     """Empty frames between useful ones change nothing (Tests_CPU/test_Palm.cpp:160-215)."""
     st = ref_inputs["stack"]
