@@ -364,8 +364,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     if (opt.profile) {
         passes_.ensure(16);
         passes = passes_.as<unsigned long long>();
-        PT_CHECK(hipMemsetAsync(passes, 0, 16, stream));
-        PT_CHECK(hipStreamSynchronize(stream));
+        fill_u32_kernel<<<1, 64, 0, main_.stream>>>(reinterpret_cast<uint32_t*>(passes), 4, 0u);   // stream-ordered
     }
     // The main lane detects chunk after chunk and gathers the ROIs of all chunks; they are fitted in
     // one launch at the end, which gives every lane of the persistent fit many ROIs, so that the
@@ -540,7 +539,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     // deferred planes take the side lane's rows, placed after the main lane's in the staged array,
     // and are the only host input. The host reads the row total once, after the last copy.
     const uint32_t nDef = (uint32_t)deferred.size();
-    uint64_t* hp = reinterpret_cast<uint64_t*>(main_.host(2 * (3 * (size_t)nDef + 1)));
+    uint64_t* hp = reinterpret_cast<uint64_t*>(main_.host(2 * (3 * (size_t)nDef + 3)));   // + total, LM passes
     for (uint32_t k = 0; k < nDef; ++k) {
         hp[k] = deferred[k];
         hp[nDef + k] = sideSrc[k];
@@ -561,6 +560,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     launch_plan(dBase, planes, dDef, nDef, totalRois, dSrc, dDst, dCnt, dTotal, main_.stream);
     launch_scatter(main_.staged.as<double>(), dSrc, dDst, dCnt, planes, d_out, outCapRows, main_.stream);
     PT_CHECK(hipMemcpyAsync(hp + 3 * (size_t)nDef, dTotal, 8, hipMemcpyDeviceToHost, main_.stream));
+    if (passes) PT_CHECK(hipMemcpyAsync(hp + 3 * (size_t)nDef + 1, passes, 16, hipMemcpyDeviceToHost, main_.stream));
     PT_CHECK(hipStreamSynchronize(main_.stream));
     const uint64_t total = hp[3 * (size_t)nDef];
     if (total > outCapRows) throw Error("output capacity exceeded");
@@ -577,10 +577,8 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         PT_CHECK(hipEventElapsedTime(&ms[1], fitEv_[1], fitEv_[2]));
         last.msFit = ms[0];
         last.msAssemble = ms[1];
-        unsigned long long pc[2];
-        PT_CHECK(hipMemcpy(pc, passes, 16, hipMemcpyDeviceToHost));
-        last.jacPasses = pc[0];
-        last.trialPasses = pc[1];
+        last.jacPasses = hp[3 * (size_t)nDef + 1];
+        last.trialPasses = hp[3 * (size_t)nDef + 2];
     }
     last.frames = planes;
     last.rois = total;
