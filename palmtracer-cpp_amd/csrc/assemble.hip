@@ -31,15 +31,29 @@ __global__ void __launch_bounds__(NT) sort_frames_kernel(const double* __restric
                                                          double* __restrict__ staged)
 {
     __shared__ int64_t ky[LDS_ROWS], kx[LDS_ROWS];
+    __shared__ uint16_t perm[LDS_ROWS];          // perm[rank] = the frame's row of that rank
     const int f = blockIdx.x;
     const uint32_t b0 = roiBase[f], n = roiBase[f + 1] - b0;
     const bool lds = n <= LDS_ROWS;
-    if (lds)
+    if (lds) {
         for (uint32_t i = threadIdx.x; i < n; i += NT) {
             ky[i] = weak_key(rows[(size_t)(b0 + i) * kCols + kY]);
             kx[i] = weak_key(rows[(size_t)(b0 + i) * kCols + kX]);
         }
-    __syncthreads();
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += NT) {
+            uint32_t rank = 0;
+            for (uint32_t j = 0; j < n; ++j) rank += before(ky[j], kx[j], j, ky[i], kx[i], i);
+            perm[rank] = (uint16_t)i;
+        }
+        __syncthreads();
+        // Rows to their sorted place, one double per thread (consecutive threads, consecutive doubles).
+        for (uint32_t k = threadIdx.x; k < n * kCols; k += NT) {
+            const uint32_t r = k / kCols, c = k - r * kCols;
+            staged[(size_t)b0 * kCols + k] = c == kIndex ? (double)(r + 1) : rows[(size_t)(b0 + perm[r]) * kCols + c];
+        }
+        return;
+    }
     for (uint32_t i = threadIdx.x; i < n; i += NT) {
         const double* r = rows + (size_t)(b0 + i) * kCols;
         const int64_t yi = lds ? ky[i] : weak_key(r[kY]), xi = lds ? kx[i] : weak_key(r[kX]);
