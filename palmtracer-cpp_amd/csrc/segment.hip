@@ -251,6 +251,41 @@ __global__ void __launch_bounds__(MH_NT) mask_hist_kernel(const uint32_t* __rest
     hist[(size_t)f * kHistBins + t] = h[t];
 }
 
+// Value of lane l (wave-uniform l) in every lane, without a round trip through LDS.
+__device__ inline uint32_t from_lane(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
+__device__ inline int from_lane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ inline uint64_t from_lane(uint64_t v, int l)
+{
+    return ((uint64_t)from_lane((uint32_t)(v >> 32), l) << 32) | from_lane((uint32_t)v, l);
+}
+
+// One step of wave_min_f64: the minimum of x and x as moved by the DPP control CTRL (lanes of rows
+// outside ROWS keep x).
+template <int CTRL, int ROWS>
+__device__ inline double dpp_min_step(double x)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    const int lo = (int)(uint32_t)b, hi = (int)(uint32_t)(b >> 32);
+    const uint32_t lo2 = (uint32_t)__builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWS, 0xf, false);
+    const uint32_t hi2 = (uint32_t)__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWS, 0xf, false);
+    const double y = __longlong_as_double((long long)(((uint64_t)hi2 << 32) | lo2));
+    return y < x ? y : x;
+}
+
+// Minimum of x over the 64 lanes of a fully active wave, in every lane: DPP steps within quads,
+// half rows and rows, then the row broadcasts, and lane 63 read back. Exact (a minimum of doubles
+// does not depend on the order).
+__device__ inline double wave_min_f64(double x)
+{
+    x = dpp_min_step<0xB1, 0xf>(x);    // quad_perm [1, 0, 3, 2]
+    x = dpp_min_step<0x4E, 0xf>(x);    // quad_perm [2, 3, 0, 1]
+    x = dpp_min_step<0x141, 0xf>(x);   // row_half_mirror
+    x = dpp_min_step<0x140, 0xf>(x);   // row_mirror: every lane holds its row's minimum
+    x = dpp_min_step<0x142, 0xa>(x);   // row_bcast15 into rows 1 and 3
+    x = dpp_min_step<0x143, 0xc>(x);   // row_bcast31 into rows 2 and 3: lane 63 holds all
+    return __longlong_as_double((long long)from_lane((uint64_t)__double_as_longlong(x), 63));
+}
+
 // Watershed of one multi-label component of at most 64 pixels by one wave, one pixel per lane, the
 // queue held in registers: the same flood and the same certainty proof as flood_kernel (step 3 of
 // the file header), with the component's adjacency as lane masks. list holds the component's
@@ -268,7 +303,7 @@ __device__ inline bool flood_small(const uint32_t* list, uint32_t cnt, const uin
     uint32_t lbl = act ? ld_relaxed(fn + p) : 0u;
     uint64_t adj = 0;
     for (uint32_t j = 0; j < cnt; ++j) {
-        const int qy = __shfl(y, (int)j), qx = __shfl(x, (int)j);
+        const int qy = from_lane(y, (int)j), qx = from_lane(x, (int)j);
         const int dy = qy - y, dx = qx - x;
         if (act && (uint32_t)lane != j && dy >= -1 && dy <= 1 && dx >= -1 && dx <= 1) adj |= 1ull << j;
     }
@@ -283,16 +318,11 @@ __device__ inline bool flood_small(const uint32_t* list, uint32_t cnt, const uin
     const double eps = kEpsilon, delta = 1e-9;
     while (qm) {
         const bool queued = (qm >> lane) & 1ull;
-        double km = queued ? key : 1.79769313486231570815e308;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const double k2 = __shfl_xor(km, o);
-            km = k2 < km ? k2 : km;
-        }
+        const double km = wave_min_f64(queued ? key : 1.79769313486231570815e308);
         const uint64_t at = __ballot(queued && key == km);
-        if (__popcll(at) > 1) return false;           // equal keys: the heap order would decide
+        if (__popcll(at) != 1) return false;          // equal keys: the heap order would decide
         const int a = __ffsll((unsigned long long)at) - 1;
-        const uint32_t pa = __shfl(P, a), ta = __shfl(age, a), la = __shfl(lbl, a);
+        const uint32_t pa = from_lane(P, a), ta = from_lane(age, a), la = from_lane(lbl, a);
         bool good = true;
         if (queued && lane != a) {
             const double oa = (double)pa * kPScale, ob = (double)P * kPScale;
@@ -303,7 +333,7 @@ __device__ inline bool flood_small(const uint32_t* list, uint32_t cnt, const uin
         }
         if (__ballot(!good)) return false;
         ++it;
-        const uint64_t claims = __shfl(adj, a) & unl;
+        const uint64_t claims = from_lane(adj, a) & unl;
         if ((claims >> lane) & 1ull) {
             lbl = la;
             age = it;
