@@ -144,7 +144,12 @@ PT_API int PT_CALL pt_wavelet_device(const uint16_t* d_input, double* d_output, 
 PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint64_t width, double thresholdVal,
                                    double watershedRatio, uint32_t* labels);
 
-/* Options: "fit_fp64" (1: FP64 model and sums, 0: FP32), "max_iter" (LM iteration cap, 75),
+/* Options: "fit_fp64" (1, the default: FP64 fit in the reference's operation order, one ROI per lane;
+ * 0: the fast fit, an FP32 persistent kernel whose ROIs still iterating after "fit_handover"
+ * iterations, or meeting a decision FP32 cannot take, are fitted again in FP64 by a cooperative
+ * kernel of eight lanes per ROI; fit mode 4 runs in that FP64 kernel whole), "fit_handover" (FP32
+ * iterations of the fast fit, 16; 0: no hand-over, every ROI and mode in FP32, errors as stated for
+ * that precision), "max_iter" (LM iteration cap, 75),
  * "chunk_pixels" (frames per device chunk x pixels, 0: default 2^28), "profile" (1: stage timing),
  * "force_sequential_watershed" (1: every frame through the exact frame flood), "force_wave_flood"
  * (1: that flood without the rank flood), "force_heap_watershed" (1: that flood with the heap
@@ -159,14 +164,16 @@ PT_API int PT_CALL pt_set_option(const char* name, double value);
  * full queue, component floods given up on an unproven order, pops of the wave-queue frame floods,
  * their largest queue, frames the rank flood handed over (too many entries, equal keys, not
  * settled), wave-queue overflows, frames of the heap restatement, frames the rank flood settled by
- * solving both orders of an equal-key pair. Writes min(n, 17) values, returns 17. */
+ * solving both orders of an equal-key pair, ROIs of the fast fit fitted by the FP64 group kernel
+ * (main lanes). Writes min(n, 18) values, returns 18. */
 PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n);
 
 /* With the "profile" option, device milliseconds of the last Localization per stage (a trous,
  * segmentation, fit, assembly; HIP events on the library's main stream, summed over chunks), the
- * chunk count, the LM passes executed (model + Jacobian, trial) and the device milliseconds of the
- * deferred-frame batches on the side stream (overlapped with the main stream). Writes min(n, 8)
- * values, returns 8. */
+ * chunk count, the LM passes of the reference's schedule (model + Jacobian, trial) of the per-lane
+ * fit kernel, the device milliseconds of the deferred-frame batches on the side stream (overlapped
+ * with the main stream), and the LM passes (model + Jacobian, trial) of the FP64 group kernel.
+ * Writes min(n, 10) values, returns 10. */
 PT_API int PT_CALL pt_stage_times(double* out, uint64_t n);
 
 /* Seeded synthetic stack on the device (benchmark / test input): d_out receives frames x height x

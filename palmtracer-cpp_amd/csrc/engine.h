@@ -37,7 +37,9 @@ struct RunStats {
     // chunks), the device time of the deferred-frame batches on the side stream, and the LM passes
     // executed (model + Jacobian, trial model).
     double msAtrous = 0, msSegment = 0, msFit = 0, msAssemble = 0, msDeferred = 0;
-    uint64_t jacPasses = 0, trialPasses = 0;
+    uint64_t jacPasses = 0, trialPasses = 0;            // per-lane fit kernel (FP32 or FP64)
+    uint64_t groupJacPasses = 0, groupTrialPasses = 0;  // cooperative FP64 group kernel
+    uint64_t fitHanded = 0;   // fast fit of the main lanes: ROIs handed from the FP32 kernel to the group kernel
 };
 
 struct Options {
@@ -50,6 +52,7 @@ struct Options {
     int exactPath = 0;              // exact frame flood from: 0 rank flood, 1 wave-queue flood, 2 heap
     unsigned fitWaves = 0;          // persistent FP32 fit: resident waves (0: 16 per CU)
     int fitRefill = 16;             // persistent FP32 fit: free lanes that trigger a refill round
+    int fitHandover = 16;           // fast fit: FP32 iterations before a ROI moves to the FP64 group kernel (0: never)
     int detectRows = 0;             // rows per detection wave (0: chosen from the chunk size)
     int lanes = 2;                  // main lanes the chunks alternate between (1: one stream, for timing)
 };
@@ -60,11 +63,13 @@ struct Lane {
     hipStream_t stream = nullptr;
     DetectBuffers b{};
     DevBuf P, lab, fin, aux, mask, seed, perFrame, regions, fz, order, multi, fb, counters, flag, rf, sbits, hist;
-    DevBuf rois, rows, staged, frameMap, fitNext;
+    DevBuf rois, rows, staged, frameMap, fitNext, fitHand;
     uint32_t* pinned = nullptr;  // host-visible: frame map, roiBase and counters of an asynchronous batch
     size_t pinnedWords = 0;
     void prepare(size_t F, size_t npx);
     void zero();
+    // Work space of a fast fit of up to nRoi ROIs on this lane (queue words and hand-over list).
+    FitScratch fit_scratch(size_t nRoi);
     uint32_t* host(size_t words);
     void release();   // device and pinned memory and the stream
 };

@@ -150,7 +150,7 @@ Engine::~Engine()
 void Lane::release()
 {
     for (DevBuf* d : {&P, &lab, &fin, &aux, &mask, &seed, &perFrame, &regions, &fz, &order, &multi, &fb, &counters, &flag,
-                      &rf, &sbits, &hist, &rois, &rows, &staged, &frameMap, &fitNext})
+                      &rf, &sbits, &hist, &rois, &rows, &staged, &frameMap, &fitNext, &fitHand})
         d->release();
     if (pinned) (void)hipHostFree(pinned);
     pinned = nullptr;
@@ -184,7 +184,6 @@ void Lane::prepare(size_t F, size_t npx)
     rf.ensure(rfSlots * rf_slot_words(npx) * 4);
     sbits.ensure(F * seed_words(npx) * 4);
     hist.ensure(F * kHistBins * 4);
-    fitNext.ensure(256);
     uint32_t* pf = perFrame.as<uint32_t>();
     const size_t s = F + 1;
     b.P = P.as<uint32_t>();
@@ -229,6 +228,13 @@ void Lane::zero()
     zero3_kernel<<<1024, 256, 0, stream>>>(b.maskCnt, 6 * (b.frames + 1), b.counters, kNumCounters, b.seedBits,
                                            b.frames * seed_words(b.npx));
     PT_CHECK(hipGetLastError());
+}
+
+FitScratch Lane::fit_scratch(size_t nRoi)
+{
+    fitNext.ensure(kFitWords * 4);
+    fitHand.ensure(std::max<size_t>(nRoi, 1) * 4);
+    return FitScratch{fitNext.as<unsigned int>(), fitHand.as<uint32_t>()};
 }
 
 uint32_t* Lane::host(size_t words)
@@ -306,6 +312,7 @@ FitParams Engine::fit_params(uint64_t fit, const double* fp)
     p.roi = (int)std::max<long>(roi, 1);
     p.maxIter = opt.maxIter;
     p.refill = opt.fitRefill;
+    p.handover = opt.fitHandover;
     return p;
 }
 
@@ -362,9 +369,9 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     const uint64_t frameRows = std::min<uint64_t>(maxPerPlane, npx / 4 + 2);   // rows a plane can hold
     unsigned long long* passes = nullptr;
     if (opt.profile) {
-        passes_.ensure(16);
+        passes_.ensure(32);
         passes = passes_.as<unsigned long long>();
-        fill_u32_kernel<<<1, 64, 0, main_.stream>>>(reinterpret_cast<uint32_t*>(passes), 4, 0u);   // stream-ordered
+        fill_u32_kernel<<<1, 64, 0, main_.stream>>>(reinterpret_cast<uint32_t*>(passes), 8, 0u);   // stream-ordered
     }
     // The main lane detects chunk after chunk and gathers the ROIs of all chunks; they are fitted in
     // one launch at the end, which gives every lane of the persistent fit many ROIs, so that the
@@ -400,9 +407,9 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         PT_CHECK(hipMemcpyAsync(side_.frameMap.p, h, batchCount * 4, hipMemcpyHostToDevice, side_.stream));
         const uint32_t* dmap = side_.frameMap.as<uint32_t>();
         detect_chunk(side_, d_in, batchCount, H, W, thr, ratio, maxPerPlane, kNoEvents, dmap, true);
-        launch_fit(d_in, npx, W, H, side_.rois.as<Roi>(), (uint32_t)std::min<uint64_t>(sideRowCap, 0xFFFFFFFFull), fp,
-                   side_.rows.as<double>(), opt.fitFp64, side_.fitNext.as<unsigned int>(), side_.stream, passes, dmap,
-                   opt.fitWaves, side_.b.roiBase + batchCount);
+        const uint32_t sideFitCap = (uint32_t)std::min<uint64_t>(sideRowCap, 0xFFFFFFFFull);
+        launch_fit(d_in, npx, W, H, side_.rois.as<Roi>(), sideFitCap, fp, side_.rows.as<double>(), opt.fitFp64,
+                   side_.fit_scratch(sideFitCap), side_.stream, passes, dmap, opt.fitWaves, side_.b.roiBase + batchCount);
         launch_sort_frames(side_.rows.as<double>(), side_.b.roiBase, (int)batchCount, side_.staged.as<double>(),
                            side_.stream, frameRows);
         PT_CHECK(hipMemcpyAsync(h + SF, side_.b.roiBase, (batchCount + 1) * 4, hipMemcpyDeviceToHost, side_.stream));
@@ -526,7 +533,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     if (opt.profile) PT_CHECK(hipEventRecord(fitEv_[0], main_.stream));
     if (totalRois)
         launch_fit(d_in, npx, W, H, allRois_.as<Roi>(), (uint32_t)totalRois, fp, main_.rows.as<double>(), opt.fitFp64,
-                   main_.fitNext.as<unsigned int>(), main_.stream, passes, nullptr, opt.fitWaves);
+                   main_.fit_scratch(totalRois), main_.stream, passes, nullptr, opt.fitWaves);
     if (opt.profile) PT_CHECK(hipEventRecord(fitEv_[1], main_.stream));
     if (totalRois)
         launch_sort_frames(main_.rows.as<double>(), dBase, (int)planes, main_.staged.as<double>(), main_.stream, frameRows);
@@ -539,7 +546,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     // deferred planes take the side lane's rows, placed after the main lane's in the staged array,
     // and are the only host input. The host reads the row total once, after the last copy.
     const uint32_t nDef = (uint32_t)deferred.size();
-    uint64_t* hp = reinterpret_cast<uint64_t*>(main_.host(2 * (3 * (size_t)nDef + 3)));   // + total, LM passes
+    uint64_t* hp = reinterpret_cast<uint64_t*>(main_.host(2 * (3 * (size_t)nDef + 6)));   // + total, LM passes, hand-over
     for (uint32_t k = 0; k < nDef; ++k) {
         hp[k] = deferred[k];
         hp[nDef + k] = sideSrc[k];
@@ -560,7 +567,12 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     launch_plan(dBase, planes, dDef, nDef, totalRois, dSrc, dDst, dCnt, dTotal, main_.stream);
     launch_scatter(main_.staged.as<double>(), dSrc, dDst, dCnt, planes, d_out, outCapRows, main_.stream);
     PT_CHECK(hipMemcpyAsync(hp + 3 * (size_t)nDef, dTotal, 8, hipMemcpyDeviceToHost, main_.stream));
-    if (passes) PT_CHECK(hipMemcpyAsync(hp + 3 * (size_t)nDef + 1, passes, 16, hipMemcpyDeviceToHost, main_.stream));
+    if (passes) PT_CHECK(hipMemcpyAsync(hp + 3 * (size_t)nDef + 1, passes, 32, hipMemcpyDeviceToHost, main_.stream));
+    hp[3 * (size_t)nDef + 5] = 0;
+    const bool fastFit = totalRois && !opt.fitFp64 && fp.mode >= 1 && fp.mode <= 4;
+    if (fastFit)
+        PT_CHECK(hipMemcpyAsync(hp + 3 * (size_t)nDef + 5, main_.fitNext.as<unsigned int>() + 1, 4, hipMemcpyDeviceToHost,
+                                main_.stream));
     PT_CHECK(hipStreamSynchronize(main_.stream));
     const uint64_t total = hp[3 * (size_t)nDef];
     if (total > outCapRows) throw Error("output capacity exceeded");
@@ -579,7 +591,10 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         last.msAssemble = ms[1];
         last.jacPasses = hp[3 * (size_t)nDef + 1];
         last.trialPasses = hp[3 * (size_t)nDef + 2];
+        last.groupJacPasses = hp[3 * (size_t)nDef + 3];
+        last.groupTrialPasses = hp[3 * (size_t)nDef + 4];
     }
+    last.fitHanded = fastFit ? (fp.mode == 4 && fp.handover > 0 ? totalRois : hp[3 * (size_t)nDef + 5]) : 0;
     last.frames = planes;
     last.rois = total;
     return total * kCols;
@@ -611,8 +626,8 @@ uint64_t Engine::fit_positions(const uint16_t* d_in, uint64_t planes, int H, int
     positions_to_rois<<<g, 256, 0, stream>>>(dpos.as<double>(), n, planes, main_.rois.as<Roi>(), dbad.as<uint32_t>());
     PT_CHECK(hipGetLastError());
     const FitParams fp = fit_params(fit, fitParams);
-    launch_fit(d_in, (size_t)W * H, W, H, main_.rois.as<Roi>(), (uint32_t)n, fp, d_out, opt.fitFp64, main_.fitNext.as<unsigned int>(), stream, nullptr,
-               nullptr, opt.fitWaves);
+    launch_fit(d_in, (size_t)W * H, W, H, main_.rois.as<Roi>(), (uint32_t)n, fp, d_out, opt.fitFp64, main_.fit_scratch(n),
+               stream, nullptr, nullptr, opt.fitWaves);
     positions_ids<<<g, 256, 0, stream>>>(dpos.as<double>(), dloc.as<uint32_t>(), n, d_out);
     PT_CHECK(hipGetLastError());
     uint32_t bad = 0;

@@ -215,6 +215,7 @@ PT_API int PT_CALL pt_set_option(const char* name, double value)
         else if (n == "force_wave_flood") e.opt.exactPath = value != 0.0 ? 1 : 0;
         else if (n == "fit_waves") e.opt.fitWaves = (unsigned)std::max(0.0, std::min(1048576.0, value));
         else if (n == "fit_refill") e.opt.fitRefill = (int)std::max(1.0, std::min(64.0, value));
+        else if (n == "fit_handover") e.opt.fitHandover = (int)std::max(0.0, std::min(1e6, value));
         else if (n == "detect_rows") e.opt.detectRows = (int)std::max(0.0, std::min(1048576.0, value));
         else if (n == "lanes") e.opt.lanes = value >= 2.0 ? 2 : 1;
         else throw pt::Error("unknown option '" + n + "'");
@@ -225,21 +226,23 @@ PT_API int PT_CALL pt_set_option(const char* name, double value)
 PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n)
 {
     Engine& e = Engine::get();
-    const uint64_t v[17] = {e.last.frames, e.last.chunks, e.last.rois, e.last.flaggedFrames, e.last.multiComponents,
+    const uint64_t v[18] = {e.last.frames, e.last.chunks, e.last.rois, e.last.flaggedFrames, e.last.multiComponents,
                             e.last.deferredFrames, e.last.status, e.last.flagOverflow, e.last.flagUnproven,
                             e.last.floodPops, e.last.floodMaxQueue, e.last.rfTooBig, e.last.rfTie,
-                            e.last.rfUnsettled, e.last.fbOverflow, e.last.heapFrames, e.last.rfTwoOrders};
-    for (uint64_t i = 0; i < n && i < 17; ++i) out[i] = v[i];
-    return 17;
+                            e.last.rfUnsettled, e.last.fbOverflow, e.last.heapFrames, e.last.rfTwoOrders,
+                            e.last.fitHanded};
+    for (uint64_t i = 0; i < n && i < 18; ++i) out[i] = v[i];
+    return 18;
 }
 
 PT_API int PT_CALL pt_stage_times(double* out, uint64_t n)
 {
     Engine& e = Engine::get();
-    const double v[8] = {e.last.msAtrous, e.last.msSegment, e.last.msFit, e.last.msAssemble, (double)e.last.chunks,
-                         (double)e.last.jacPasses, (double)e.last.trialPasses, e.last.msDeferred};
-    for (uint64_t i = 0; i < n && i < 8; ++i) out[i] = v[i];
-    return 8;
+    const double v[10] = {e.last.msAtrous, e.last.msSegment, e.last.msFit, e.last.msAssemble, (double)e.last.chunks,
+                          (double)e.last.jacPasses, (double)e.last.trialPasses, e.last.msDeferred,
+                          (double)e.last.groupJacPasses, (double)e.last.groupTrialPasses};
+    for (uint64_t i = 0; i < n && i < 10; ++i) out[i] = v[i];
+    return 10;
 }
 
 } // extern "C"

@@ -20,89 +20,27 @@
 // damped normal equations (LDL^T of the symmetric matrix), the step and R^2 are FP64.
 #include "pt_common.h"
 #include "pt_kernels.h"
-
-#include <type_traits>
+#include "lm_core.h"
 
 namespace pt {
 namespace {
 
+using lm::Tri;
+using lm::tri;
+using lm::ldl_solve;
+using lm::static_for;
+using lm::Model;
+using lm::NP;
+
 constexpr double kSqrt2d = 1.41421356237309504880;
 constexpr double kPid = 3.14159265358979323846;
 
-template <int MODE> struct NP { static constexpr int v = MODE == 1 ? 4 : MODE == 2 ? 5 : MODE == 3 ? 6 : 7; };
-template <int N> struct Tri { static constexpr int v = N * (N + 1) / 2; };
-// Packed upper triangle of a symmetric N x N matrix: element (i, j) at tri(i, j) == tri(j, i).
-template <int N> __host__ __device__ constexpr int tri(int i, int j)
-{
-    return i <= j ? i * N - i * (i - 1) / 2 + (j - i) : j * N - j * (j - 1) / 2 + (i - j);
-}
 
 __device__ inline uint64_t to_size_f(double v)
 {
     if (!(v == v) || v >= 9223372036854775808.0 || v <= -9223372036854775808.0) return 1ull << 63;
     return (uint64_t)(long long)v;
 }
-
-// GaussianModel (lib/fit/GaussianModel.cpp:19-52) at one parameter vector. The value is FP64:
-// E = exp(-(ax X^2 + ay Y^2 + axy X Y)) with X, Y the pixel offsets from the centre, which for an
-// axis-aligned Gaussian (axy == 0) is the product of a column factor and a row factor. The
-// gradient is FP32, formed from the FP64 E.
-template <int MODE>
-struct Model {
-    double off, amp, ax, ay, axy;
-    float ampf, isx2, isy2, isx, isy, c, s, cs, c2ms2, k;  // k: 1/sigma^3 (mode 2)
-    __device__ void set(const double* p, double sigma, double theta)
-    {
-        off = p[0];
-        amp = p[1];
-        ampf = (float)amp;
-        if constexpr (MODE <= 2) {
-            const double sg = MODE == 1 ? sigma : p[MODE == 1 ? 0 : 4];
-            ax = ay = 1.0 / (2.0 * sg * sg);
-            axy = 0.0;
-            const float sf = (float)sg;
-            isx2 = 1.0f / (sf * sf);
-            k = isx2 / sf;
-        } else {
-            const double th = MODE == 3 ? theta : p[MODE == 3 ? 0 : 6];
-            const double sx = p[4], sy = p[5];
-            const double cd = cos(th), sd = sin(th);
-            const double ix2 = 1.0 / (sx * sx), iy2 = 1.0 / (sy * sy);
-            ax = cd * cd * ix2 + sd * sd * iy2;
-            ay = sd * sd * ix2 + cd * cd * iy2;
-            axy = 2.0 * cd * sd * (iy2 - ix2);
-            isx = (float)(1.0 / sx);
-            isy = (float)(1.0 / sy);
-            isx2 = (float)ix2;
-            isy2 = (float)iy2;
-            c = (float)cd;
-            s = (float)sd;
-            cs = c * s;
-            c2ms2 = c * c - s * s;
-        }
-    }
-    __device__ bool separable() const { return axy == 0.0; }
-    __device__ double E(double X, double Y) const { return exp(-(ax * X * X + ay * Y * Y + axy * X * Y)); }
-    // Gradient at (x, y) given E (same components as GaussianModel::Evaluate).
-    __device__ void grad(float x, float y, float Ef, float* g) const
-    {
-        const float P = ampf * Ef;
-        g[0] = 1.0f;
-        g[1] = Ef;
-        if constexpr (MODE <= 2) {
-            g[2] = P * x * isx2;
-            g[3] = P * y * isx2;
-            if constexpr (MODE == 2) g[4] = P * (x * x + y * y) * k;
-        } else {
-            const float F1 = (x * c - y * s) * isx, F2 = (x * s + y * c) * isy;
-            g[2] = P * (2.0f * (c * c * x - y * cs) * isx2 + 2.0f * (s * s * x + y * cs) * isy2);
-            g[3] = P * (2.0f * (s * s * y - x * cs) * isx2 + 2.0f * (c * c * y + x * cs) * isy2);
-            g[4] = P * 2.0f * F1 * F1 * isx;
-            g[5] = P * 2.0f * F2 * F2 * isy;
-            if constexpr (MODE == 4) g[6] = P * 2.0f * (isx2 - isy2) * (cs * (x * x - y * y) + c2ms2 * x * y);
-        }
-    }
-};
 
 __device__ inline void row_fill(double* pt, int mode, double x, double y, double seg, double surf, double off, double inten,
                                 double sx, double sy, double th, double rss)
@@ -123,59 +61,8 @@ __device__ inline void row_fail(double* pt, int mode, double x, double y, double
     row_fill(pt, mode, x, y, seg, surf, -1, -1, -1, -1, -1, -1);
 }
 
-template <int B, int E, typename F>
-__device__ inline void static_for(F&& f)
-{
-    if constexpr (B < E) {
-        f(std::integral_constant<int, B>{});
-        static_for<B + 1, E>(f);
-    }
-}
-
-// Solve (alpha_R) dp = beta, alpha_R = alpha with diagonal (1 + lambda) * a_ii (1 where a_ii == 0),
-// by LDL^T in FP64. Returns 0 on success, 1 when the damped matrix is not numerically positive
-// definite (the reference's singular-matrix case), 2 when the step is not finite.
-template <int N>
-__device__ inline int ldl_solve(const float (&al)[Tri<N>::v], const float (&be)[N], double lambda, double (&dp)[N])
-{
-    double L[Tri<N>::v], D[N], z[N];
-    static_for<0, Tri<N>::v>([&](auto i) { L[i] = (double)al[i]; });
-    static_for<0, N>([&](auto i) {
-        constexpr int d = tri<N>(i, i);
-        L[d] = L[d] == 0.0 ? 1.0 : (1.0 + lambda) * L[d];
-    });
-    bool pd = true;
-    static_for<0, N>([&](auto j) {
-        const double ajj = L[tri<N>(j, j)];
-        double dj = ajj;
-        static_for<0, j>([&](auto k) { dj -= L[tri<N>(j, k)] * L[tri<N>(j, k)] * D[k]; });
-        pd &= (dj > 2.220446049250313e-16 * N * fabs(ajj)) && isfinite(dj);
-        D[j] = dj;
-        const double inv = 1.0 / dj;
-        static_for<j + 1, N>([&](auto i) {
-            double v = L[tri<N>(i, j)];
-            static_for<0, j>([&](auto k) { v -= L[tri<N>(i, k)] * L[tri<N>(j, k)] * D[k]; });
-            L[tri<N>(i, j)] = v * inv;
-        });
-    });
-    if (!pd) return 1;
-    static_for<0, N>([&](auto i) {
-        double v = (double)be[i];
-        static_for<0, i>([&](auto k) { v -= L[tri<N>(i, k)] * z[k]; });
-        z[i] = v;
-    });
-    static_for<0, N>([&](auto ii) {
-        constexpr int i = N - 1 - ii;
-        double v = z[i] / D[i];
-        static_for<i + 1, N>([&](auto k) { v -= L[tri<N>(k, i)] * dp[k]; });
-        dp[i] = v;
-    });
-    bool ok = true;
-    static_for<0, N>([&](auto i) { ok &= isfinite(dp[i]); });
-    return ok ? 0 : 2;
-}
-
-enum : int { kFetch = 0, kDone = 1, kPassInit = 2, kPassTrial = 3 };
+// kPassRestore: a pass at p after a rejected step, which restores alpha, beta and the model sums at p.
+enum : int { kFetch = 0, kDone = 1, kPassInit = 2, kPassTrial = 3, kPassRestore = 4 };
 
 // Largest ROI side staged in LDS: pixels (15 x 15 x 64 lanes x 2 bytes = 28.8 KB per wave) and the
 // column factors of the separable exponential (15 x 64 x 8 bytes). Larger ROIs read the frame and
@@ -187,21 +74,27 @@ __host__ __device__ constexpr size_t staged_lds_bytes(int roi)
     return (size_t)roi * roi * 64 * sizeof(uint16_t) + (size_t)roi * 64 * sizeof(double);
 }
 
-// Every Jacobian-type pass also accumulates the Jacobian products (alT, beT): the reference
-// evaluates the Jacobian at pTry right after accepting pTry (GaussianFit.cpp:147-151) and chi^2 at
-// pTry is the same sum either way, so an accepted trial pass already holds the next alpha, beta and
-// chi^2.
+// Every pass also accumulates the Jacobian products: the reference evaluates the Jacobian at pTry
+// right after accepting pTry (GaussianFit.cpp:147-151) and chi^2 at pTry is the same sum either way,
+// so an accepted trial pass already holds the next alpha, beta and chi^2. A lane keeps one set of
+// sums (alpha, beta, the model sums of R^2); after a rejected step (under 1 % of the steps) a pass
+// at p restores them.
 // R > 0: the ROI side is the compile-time constant R (pixel loops unrolled, column factors in
 // registers); R = 0: the side is read from fp.roi.
-template <int MODE, bool STAGED, int R>
+template <int MODE, bool STAGED, int R, typename G>
 __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __restrict__ frames, size_t npx, int W, int H,
                                                             const Roi* __restrict__ rois, uint32_t nRoiMax, FitParams fp,
                                                             double* __restrict__ rows, unsigned int* __restrict__ next,
                                                             unsigned long long* __restrict__ passes,
                                                             const uint32_t* __restrict__ frameMap,
-                                                            const uint32_t* __restrict__ count)
+                                                            const uint32_t* __restrict__ count,
+                                                            uint32_t* __restrict__ hand, unsigned int* __restrict__ handCount)
 {
     const uint32_t nRoi = count ? *count : nRoiMax;
+    // With a hand-over list, a ROI leaves this kernel (its index appended to the list, its row left to
+    // the FP64 group kernel) when it reaches fp.handover iterations or the cap, or when its FP32
+    // arithmetic meets a decision that exact arithmetic does not take (see the call sites below).
+    const int limit = hand ? min(fp.maxIter, max(fp.handover, 1)) : fp.maxIter;
     constexpr int N = NP<MODE>::v;
     constexpr int T = Tri<N>::v;
     extern __shared__ __align__(16) unsigned char lds[];
@@ -220,8 +113,9 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
     double relAcc = 0, relRej = 0;     // max |dP| / max(|p|, 1e-12) with p after acceptance / rejection
     double lambda = 0, err = 0, mq = 0;
     double sd = 0, sdd = 0;            // data sums of the ROI (exact integers)
-    double cm = 0, cmm = 0, cdm = 0;   // model sums at p: sum m, sum m^2, sum d*m (R^2)
-    float al[T], be[N];
+    double cm = 0, cmm = 0, cdm = 0;   // model sums of the last pass: sum m, sum m^2, sum d*m (R^2)
+    G al[T], be[N];                    // alpha (packed upper triangle) and beta of the last pass
+    bool satStop = false;              // the rejected step being restored had saturated the damping
     uint32_t nJac = 0, nTrial = 0;
 
     auto pix = [&](int k, int i, int j) -> float {
@@ -230,7 +124,7 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
     };
     // Visit the ROI's pixels in raster order with the model's FP64 exponential E at each:
     // body(k, i, j, X, Y, E) with X, Y the FP64 offsets from the centre (cx, cy).
-    auto for_pixels = [&](const Model<MODE>& m, double cx, double cy, auto&& body) {
+    auto for_pixels = [&](const Model<MODE, G>& m, double cx, double cy, auto&& body) {
         const double bx = (double)(ux - half) - cx, by = (double)(uy - half) - cy;
         if constexpr (R > 0) {
             const bool sep = m.separable();
@@ -366,17 +260,17 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
         double* o = rows + (size_t)idx * kCols;
         // ---- chi^2 and Jacobian products at p (initial pass) or pTry (trial pass) ----
         const bool trial = state == kPassTrial;
-        double e = 0.0, tm = 0.0, tmm = 0.0, tdm = 0.0;
-        float alT[T], beT[N];
+        double e = 0.0;
+        cm = cmm = cdm = 0.0;
 #pragma unroll
-        for (int a = 0; a < T; ++a) alT[a] = 0.0f;
+        for (int a = 0; a < T; ++a) al[a] = G(0);
 #pragma unroll
-        for (int a = 0; a < N; ++a) beT[a] = 0.0f;
+        for (int a = 0; a < N; ++a) be[a] = G(0);
         {
             double q[N];
 #pragma unroll
             for (int a = 0; a < N; ++a) q[a] = trial ? pTry[a] : p[a];
-            Model<MODE> m;
+            Model<MODE, G> m;
             m.set(q, fp.sigmaX, fp.theta);
             bool circ = false;
             if constexpr (MODE == 4) circ = fabs(q[4] - q[5]) <= 2.220446049250313e-16;
@@ -384,30 +278,24 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
                 const double d = (double)pix(k, i, j), mm = m.off + m.amp * E;
                 const double resd = d - mm;
                 e += resd * resd;
-                tm += mm;
-                tmm += mm * mm;
-                tdm += d * mm;
-                const float res = (float)resd;
-                float g[N];
-                m.grad((float)X, (float)Y, (float)E, g);
+                cm += mm;
+                cmm += mm * mm;
+                cdm += d * mm;
+                const G res = (G)resd;
+                G g[N];
+                m.grad((G)X, (G)Y, (G)E, g);
                 if constexpr (MODE == 4)
-                    if (circ) g[6] = 0.0f;
+                    if (circ) g[6] = G(0);
 #pragma unroll
                 for (int a = 0; a < N; ++a) {
-                    beT[a] += g[a] * res;
+                    be[a] += g[a] * res;
 #pragma unroll
-                    for (int b = a; b < N; ++b) alT[tri<N>(a, b)] += g[a] * g[b];
+                    for (int b = a; b < N; ++b) al[tri<N>(a, b)] += g[a] * g[b];
                 }
             });
         }
 
         // ---- advance the LM state machine to its next pass (GaussianFit.cpp:75-165) ----
-        auto take = [&]() {
-#pragma unroll
-            for (int a = 0; a < T; ++a) al[a] = alT[a];
-#pragma unroll
-            for (int a = 0; a < N; ++a) be[a] = beT[a];
-        };
         auto max_abs_beta = [&]() {
             double mb = fabs((double)be[0]);
 #pragma unroll
@@ -420,18 +308,18 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
             ++it;
             return false;
         };
-        bool finish = false, failed = false;
-        auto take_r2 = [&]() { cm = tm; cmm = tmm; cdm = tdm; };
-        if (!trial) {
+        bool finish = false, failed = false, handOff = false;
+        if (state == kPassInit) {
             ++nJac;
-            take();
-            take_r2();
             err = e;
             bool bad = !isfinite(err);
 #pragma unroll
             for (int a = 0; a < N; ++a) bad |= !isfinite(be[a]);
             if (bad) failed = true;
             else it = (max_abs_beta() < 1e-6) ? fp.maxIter : 0;
+        } else if (state == kPassRestore) {
+            // The sums are at p again (chi^2 there is err): the end of the rejected iteration.
+            finish = satStop || stop_after_step(relRej);
         } else {
             ++nTrial;
             const double rho = (err - e) / (2.0 * mq);
@@ -439,23 +327,31 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
                 ++nJac;
 #pragma unroll
                 for (int a = 0; a < N; ++a) p[a] = pTry[a];
-                take();
-                take_r2();
                 err = e;
                 lambda = fmax(lambda / 9, 1e-7);
                 finish = stop_after_step(relAcc);
-            } else if (lambda >= 1e7) {
+            } else if (lambda >= 1e7 && hand) {
                 finish = true;
+                handOff = true;   // damping saturated: rejected steps all the way up
             } else {
+                // Rejected: the stopping rules and the next solve read alpha and beta at p, which this
+                // pass at pTry overwrote; the restore pass recomputes them first.
+                satStop = lambda >= 1e7;
                 lambda = fmin(lambda * 11, 1e7);
-                finish = stop_after_step(relRej);
+                state = kPassRestore;
+                continue;
             }
         }
         // Damped solves until a trial pass is needed (GaussianFit.cpp:83-144).
         while (!finish && !failed) {
-            if (it >= fp.maxIter) { finish = true; break; }
+            if (it >= limit) {
+                finish = true;
+                handOff = hand != nullptr;
+                break;
+            }
             double d[N];
-            const int rc = ldl_solve<N>(al, be, lambda, d);
+            const int rc = ldl_solve<N, G>(al, be, lambda, d);
+            if (rc != 0 && hand) { handOff = true; break; }   // singular damped matrix or non-finite step
             if (rc == 1) {
                 if (lambda < 1e7) { lambda = fmin(lambda * 11, 1e7); ++it; continue; }
                 failed = true;
@@ -480,6 +376,7 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
                 relRej = relRej < rr ? rr : relRej;
                 lin += d[a] * (double)be[a];
             }
+            if (tiny && hand) { handOff = true; break; }
             if (tiny) {
                 if (lambda >= 1e7) { finish = true; break; }
                 lambda = fmin(lambda * 11, 1e7);
@@ -495,6 +392,9 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
                 for (int b = 0; b < N; ++b) ti += (double)al[tri<N>(a, b)] * d[b];
                 quad += d[a] * ti;
             }
+            // A non-positive gain needs a clamped step or rounding: exact arithmetic gives
+            // dP.(beta + lambda D dP) > 0 for the unclamped step.
+            if ((2.0 * lin) - quad <= 0.0 && hand) { handOff = true; break; }
             if ((2.0 * lin) - quad <= 0.0) {
                 if (lambda >= 1e7) { finish = true; break; }
                 lambda = fmin(lambda * 11, 1e7);
@@ -508,15 +408,24 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
             state = kPassTrial;
             break;
         }
-        if (failed) {
-            row_fail(o, MODE, x0, y0, seg, surf);
-            state = kFetch;
-        } else if (finish) {
+        if (failed && hand) handOff = true;
+        double rss = 0.0;
+        if (finish && !handOff) {
             // computeR2 at p (GaussianFit.cpp:270-295): squared Pearson correlation of data and model,
             // NaN -> 0, from the sums of the pass that evaluated p.
             const double vd = sdd - sd * sd / nPix, vm = cmm - cm * cm / nPix, cv = cdm - sd * cm / nPix;
             const double rr = cv / sqrt(vd * vm);
-            const double rss = isnan(rr) ? 0.0 : rr * rr;
+            rss = isnan(rr) ? 0.0 : rr * rr;
+            // A correlation near the failure threshold (1e-7) is decided in FP64.
+            if (rss < 1e-5 && hand) handOff = true;
+        }
+        if (handOff) {
+            hand[atomicAdd(handCount, 1u)] = idx;
+            state = kFetch;
+        } else if (failed) {
+            row_fail(o, MODE, x0, y0, seg, surf);
+            state = kFetch;
+        } else if (finish) {
             if (rss < kEpsilon) {
                 row_fail(o, MODE, p[2], p[3], seg, surf);
             } else {
@@ -544,12 +453,14 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
     }
 }
 
-__global__ void reset_next_kernel(unsigned int* next) { *next = 0u; }
+// Queue heads and the hand-over count of a fit launch start at zero (a fill kernel, so that the
+// sequence can be captured in a graph).
+__global__ void reset_words_kernel(unsigned int* w) { if (threadIdx.x < kFitWords) w[threadIdx.x] = 0u; }
 
 } // namespace
 
 void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, uint32_t nRoi, const FitParams& fp,
-                double* d_rows, bool fp64, unsigned int* d_next, hipStream_t s, unsigned long long* d_passes,
+                double* d_rows, bool fp64, const FitScratch& ws, hipStream_t s, unsigned long long* d_passes,
                 const uint32_t* frameMap, unsigned waves, const uint32_t* d_count)
 {
     if (nRoi == 0) return;
@@ -562,44 +473,59 @@ void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d
         launch_fit_f64(d_frames, npx, W, H, d_rois, nRoi, fp, d_rows, s, d_passes, frameMap, d_count);
         return;
     }
-    if (!d_next) throw Error("launch_fit: the FP32 fit needs a queue word");
-    reset_next_kernel<<<1, 1, 0, s>>>(d_next);
+    if (!ws.words) throw Error("launch_fit: the fast fit needs its queue words");
+    reset_words_kernel<<<1, 64, 0, s>>>(ws.words);
+    unsigned long long* groupPasses = d_passes ? d_passes + 2 : nullptr;
+    // Fit mode 4 (theta free) is fitted whole in FP64 by the group kernel (see lmfit_group.hip).
+    if (fp.mode == 4 && fp.handover > 0) {
+        launch_fit_group(d_frames, npx, W, H, d_rois, nullptr, d_count, nRoi, fp, d_rows, ws.words + 2, s, groupPasses,
+                         frameMap, 0);
+        return;
+    }
+    const bool handing = fp.handover > 0;
+    if (handing && !ws.hand) throw Error("launch_fit: the hand-over needs a list of nRoi words");
     // Persistent waves: by default 16 per CU of the 256-CU part, fewer when there are fewer ROIs.
     const unsigned want = waves ? waves : 4096u;
     const unsigned grid = std::max(1u, std::min(want, ceil_div(nRoi, 64)));
     const bool staged = fp.roi <= kMaxStagedRoi;
     const size_t lds = staged ? staged_lds_bytes(fp.roi) : 0;
+    uint32_t* hand = handing ? ws.hand : nullptr;
     auto go = [&](auto kern) {
-        kern<<<grid, 64, lds, s>>>(d_frames, npx, W, H, d_rois, nRoi, fp, d_rows, d_next, d_passes, frameMap, d_count);
+        kern<<<grid, 64, lds, s>>>(d_frames, npx, W, H, d_rois, nRoi, fp, d_rows, ws.words, d_passes, frameMap, d_count,
+                                   hand, ws.words + 1);
     };
     // ROI sides 7 (the reference default) and 9 (astigmatic ROIs) are compiled out.
     if (fp.roi == 7) {
         switch (fp.mode) {
-        case 1: go(fit_persistent_kernel<1, true, 7>); break;
-        case 2: go(fit_persistent_kernel<2, true, 7>); break;
-        case 3: go(fit_persistent_kernel<3, true, 7>); break;
-        default: go(fit_persistent_kernel<4, true, 7>); break;
+        case 1: go(fit_persistent_kernel<1, true, 7, float>); break;
+        case 2: go(fit_persistent_kernel<2, true, 7, float>); break;
+        case 3: go(fit_persistent_kernel<3, true, 7, float>); break;
+        default: go(fit_persistent_kernel<4, true, 7, float>); break;
         }
     } else if (fp.roi == 9) {
         switch (fp.mode) {
-        case 1: go(fit_persistent_kernel<1, true, 9>); break;
-        case 2: go(fit_persistent_kernel<2, true, 9>); break;
-        case 3: go(fit_persistent_kernel<3, true, 9>); break;
-        default: go(fit_persistent_kernel<4, true, 9>); break;
+        case 1: go(fit_persistent_kernel<1, true, 9, float>); break;
+        case 2: go(fit_persistent_kernel<2, true, 9, float>); break;
+        case 3: go(fit_persistent_kernel<3, true, 9, float>); break;
+        default: go(fit_persistent_kernel<4, true, 9, float>); break;
         }
     } else {
         switch (fp.mode * 2 + (staged ? 1 : 0)) {
-        case 2: go(fit_persistent_kernel<1, false, 0>); break;
-        case 3: go(fit_persistent_kernel<1, true, 0>); break;
-        case 4: go(fit_persistent_kernel<2, false, 0>); break;
-        case 5: go(fit_persistent_kernel<2, true, 0>); break;
-        case 6: go(fit_persistent_kernel<3, false, 0>); break;
-        case 7: go(fit_persistent_kernel<3, true, 0>); break;
-        case 8: go(fit_persistent_kernel<4, false, 0>); break;
-        default: go(fit_persistent_kernel<4, true, 0>); break;
+        case 2: go(fit_persistent_kernel<1, false, 0, float>); break;
+        case 3: go(fit_persistent_kernel<1, true, 0, float>); break;
+        case 4: go(fit_persistent_kernel<2, false, 0, float>); break;
+        case 5: go(fit_persistent_kernel<2, true, 0, float>); break;
+        case 6: go(fit_persistent_kernel<3, false, 0, float>); break;
+        case 7: go(fit_persistent_kernel<3, true, 0, float>); break;
+        case 8: go(fit_persistent_kernel<4, false, 0, float>); break;
+        default: go(fit_persistent_kernel<4, true, 0, float>); break;
         }
     }
     PT_CHECK(hipGetLastError());
+    // The handed-over ROIs, fitted again from the start in FP64 (count read on the device).
+    if (handing)
+        launch_fit_group(d_frames, npx, W, H, d_rois, ws.hand, ws.words + 1, nRoi, fp, d_rows, ws.words + 2, s,
+                         groupPasses, frameMap, 0);
 }
 
 } // namespace pt

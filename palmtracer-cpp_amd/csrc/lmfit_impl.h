@@ -1,15 +1,14 @@
-// Per-ROI Gaussian Levenberg-Marquardt fit on gfx950 (templates; instantiated by lmfit_f32.hip and lmfit_f64.hip).
+// FP64 Gaussian Levenberg-Marquardt fit in the reference's operation order (one ROI per lane), and the
+// model, Gauss-Jordan inverse and row fill that the cooperative FP64 kernel (lmfit_group.hip) shares.
 //
 // Reference: lib/fit/GaussianFit.cpp:48-295 (Fit, lm, gain, getError, computeR2),
 // lib/fit/GaussianModel.cpp:19-52 (Evaluate), lib/fit/Fitting.cpp:61-175 (fill, preCompute, initP),
 // lib/signal_processing/Matrix.cpp:297-355 (InverseGaussJordan).
 //
 // Schedule: one ROI per lane. A lane walks its ROI's pixels in the reference's raster order, so
-// every sum is accumulated in the reference's order; with T = double the only differences from the
-// reference are the device exp/sin/cos. With T = float the model, its Jacobian, alpha, beta and the
-// squared error are evaluated and accumulated in FP32, while parameters, the damped normal equations
-// (Gauss-Jordan with row equilibration and partial pivoting, as the reference), the gain ratio and
-// R^2 stay in FP64. The control flow (lambda schedule, clamps, stopping rules, sentinels) is the
+// every sum is accumulated in the reference's order; the only differences from the reference are
+// the device exp/sin/cos. The control flow (lambda schedule, clamps, stopping rules, sentinels) and
+// the damped normal equations (Gauss-Jordan with row equilibration and partial pivoting) are the
 // reference's, step for step.
 #pragma once
 #include "pt_common.h"
@@ -21,10 +20,6 @@ namespace {
 constexpr double kSqrt2 = 1.41421356237309504880;
 constexpr double kPi = 3.14159265358979323846;
 
-template <typename T> __device__ inline T t_exp(T v);
-template <> __device__ inline float t_exp<float>(float v) { return expf(v); }
-template <> __device__ inline double t_exp<double>(double v) { return exp(v); }
-
 template <int MODE> struct NPar { static constexpr int v = MODE == 1 ? 4 : MODE == 2 ? 5 : MODE == 3 ? 6 : 7; };
 
 // Double to size_t as the reference's x86-64 build converts (truncation; wraps for negatives;
@@ -35,9 +30,10 @@ __device__ inline uint64_t to_size(double v)
     return (uint64_t)(long long)v;
 }
 
-// GaussianModel: isotropic (modes 1, 2) or rotated anisotropic (modes 3, 4).
+// GaussianModel: isotropic (modes 1, 2) or rotated anisotropic (modes 3, 4), FP64.
 template <typename T, int MODE>
 struct Model {
+    static_assert(sizeof(T) == 8, "the reference-order model is FP64");
     T off, amp, sx, sy, sx2, sy2, c, s, c2, s2, cs;
     __device__ Model(const double* p, double sigma, double theta)
     {
@@ -62,7 +58,7 @@ struct Model {
     {
         const T X2 = x * x, Y2 = y * y;
         if constexpr (MODE <= 2) {
-            const T E = t_exp<T>(-(X2 + Y2) / (T(2) * sx2));
+            const T E = exp(-(X2 + Y2) / (T(2) * sx2));
             const T P = amp * E;
             if (g) {
                 g[0] = 1;
@@ -75,7 +71,7 @@ struct Model {
         } else {
         const T XC = x * c, XS = x * s, YC = y * c, YS = y * s;
         const T F1 = (XC - YS) / sx, F2 = (XS + YC) / sy;
-        const T E = t_exp<T>(-((F1 * F1) + (F2 * F2)));
+        const T E = exp(-((F1 * F1) + (F2 * F2)));
         const T P = amp * E;
         if (g) {
             const T XY = x * y, F3 = T(2) * XY * cs, F4 = (T(2) * XY / sx2) - (T(2) * XY / sy2);

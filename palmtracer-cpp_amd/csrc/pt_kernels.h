@@ -153,16 +153,33 @@ struct FitParams {
     int maxIter;     // Fitting::N_ITER (75 in the reference)
     double sigmaX, sigmaY, theta;
     int refill = 16; // FP32 fit: free lanes of a wave that trigger a refill round while others work
+    // Fast fit: iterations after which a ROI still iterating leaves the FP32 kernel for the FP64 group
+    // kernel (0: no hand-over, every ROI and every mode stays in FP32).
+    int handover = 16;
     const SplineParams* spline = nullptr;  // mode 5 (host struct, read by the launcher)
 };
-// LM fit of nRoi ROIs into d_rows (18 columns each). fp64: the reference's arithmetic, one ROI per
-// lane; otherwise the FP32 persistent work-queue kernel, whose queue head is d_next (one word).
-// d_passes (optional) accumulates {Jacobian passes, trial passes}; frameMap maps ROI frames to
-// stack frames.
-// With d_count set, the ROI count is read on the device (nRoi is then an upper bound).
+// Device work space of a fast fit launch: kFitWords queue words ([0] FP32 queue head, [1] ROIs handed
+// to the FP64 group kernel, [2] group queue head) and the hand-over list (one word per ROI).
+constexpr int kFitWords = 4;
+struct FitScratch {
+    unsigned int* words = nullptr;
+    uint32_t* hand = nullptr;
+};
+// LM fit of nRoi ROIs into d_rows (18 columns each). fp64: the reference's arithmetic and operation
+// order, one ROI per lane; otherwise the fast fit: the FP32 persistent work-queue kernel, which hands
+// ROIs that are still iterating after fp.handover iterations (or that meet a decision FP32 cannot
+// take) to the cooperative FP64 group kernel, and the group kernel alone for fit mode 4. ws: the
+// fast fit's work space (nRoi hand-over words). d_passes (optional, 4 words) accumulates {Jacobian
+// passes, trial passes} of the per-lane kernel, then of the group kernel; frameMap maps ROI frames
+// to stack frames. With d_count set, the ROI count is read on the device (nRoi is then an upper bound).
 void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, uint32_t nRoi, const FitParams& fp,
-                double* d_rows, bool fp64, unsigned int* d_next, hipStream_t s, unsigned long long* d_passes = nullptr,
+                double* d_rows, bool fp64, const FitScratch& ws, hipStream_t s, unsigned long long* d_passes = nullptr,
                 const uint32_t* frameMap = nullptr, unsigned waves = 0, const uint32_t* d_count = nullptr);
+// Cooperative FP64 LM fit (eight lanes per ROI) of the ROIs d_list[0 .. *d_count) (d_list null: ROIs
+// 0 .. *d_count, or 0 .. nMax without d_count); d_next: the group queue head (zeroed here).
+void launch_fit_group(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, const uint32_t* d_list,
+                      const uint32_t* d_count, uint32_t nMax, const FitParams& fp, double* d_rows, unsigned int* d_next,
+                      hipStream_t s, unsigned long long* d_passes, const uint32_t* frameMap, unsigned waves);
 void launch_fit_f64(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, uint32_t nRoi,
                     const FitParams& fp, double* d_rows, hipStream_t s, unsigned long long* d_passes,
                     const uint32_t* frameMap, const uint32_t* d_count = nullptr);

@@ -162,20 +162,21 @@ def set_option(name, value):
 
 
 def last_stats():
-    v = (ctypes.c_uint64 * 17)()
-    lib().pt_last_stats(v, 17)
+    v = (ctypes.c_uint64 * 18)()
+    lib().pt_last_stats(v, 18)
     keys = ["frames", "chunks", "rows", "flagged_frames", "multi_components", "deferred_frames", "status",
             "flag_overflow", "flag_unproven", "wave_flood_pops", "wave_flood_max_queue", "rank_too_big",
-            "rank_equal_keys", "rank_unsettled", "wave_queue_overflow", "heap_frames", "rank_two_orders"]
+            "rank_equal_keys", "rank_unsettled", "wave_queue_overflow", "heap_frames", "rank_two_orders",
+            "fit_handed_over"]
     return dict(zip(keys, list(v)))
 
 
 def stage_times():
     """Per-stage device milliseconds of the last Localization (needs set_option('profile', 1))."""
-    v = (ctypes.c_double * 8)()
-    lib().pt_stage_times(v, 8)
+    v = (ctypes.c_double * 10)()
+    lib().pt_stage_times(v, 10)
     keys = ["atrous_ms", "segment_ms", "fit_ms", "assemble_ms", "chunks", "jacobian_passes", "trial_passes",
-            "deferred_ms"]
+            "deferred_ms", "group_jacobian_passes", "group_trial_passes"]
     return dict(zip(keys, list(v)))
 
 

@@ -137,3 +137,43 @@ def spline_params(roi=7, size=(11, 11, 21), delta_z=50.0, sigma=1.3, seed=0, noi
     coef = np.einsum("za,yb,xc->abczyx", pz, py, px).reshape(64, sz * sy * sx)
     coef = coef * (1.0 + noise * np.random.default_rng(seed).standard_normal(coef.shape))
     return np.concatenate([[roi, sx, sy, sz, delta_z], coef.ravel()])
+
+
+def pair_rows_by_plane(actual, expected):
+    """Vectorized pairing for large row sets: within each plane, each actual row takes the expected
+    row nearest in (X, Y). Returns the index array (-1 where the plane has no expected rows)."""
+    out = np.full(len(actual), -1, np.int64)
+    order_a = np.argsort(actual[:, PLANE], kind="stable")
+    order_e = np.argsort(expected[:, PLANE], kind="stable")
+    pa, pe = actual[order_a, PLANE], expected[order_e, PLANE]
+    for plane in np.unique(pa):
+        ia = order_a[np.searchsorted(pa, plane, "left"):np.searchsorted(pa, plane, "right")]
+        ie = order_e[np.searchsorted(pe, plane, "left"):np.searchsorted(pe, plane, "right")]
+        if ie.size == 0:
+            continue
+        d = ((actual[ia][:, None, X:Y + 1] - expected[ie][None, :, X:Y + 1]) ** 2).sum(-1)
+        out[ia] = ie[d.argmin(1)]
+    return out
+
+
+def fit_mismatch_rows(actual, expected):
+    """Indices of actual rows outside FIT_TOL of their expected row (paired by pair_rows_by_plane), with
+    unequal detection columns, fitted on one side only, unpaired, or paired twice. For row sets of
+    tens of thousands of rows."""
+    assert actual.shape == expected.shape, (actual.shape, expected.shape)
+    j = pair_rows_by_plane(actual, expected)
+    bad = j < 0
+    jj = np.where(bad, 0, j)
+    e = expected[jj]
+    dup = np.zeros(len(actual), bool)
+    u, c = np.unique(j[j >= 0], return_counts=True)
+    dup[np.isin(j, u[c > 1])] = True
+    bad |= dup
+    bad |= (actual[:, AMP] > 0) != (e[:, AMP] > 0)
+    for c in EXACT_COLS:
+        if c in (ID, INDEX):
+            continue                   # Id / Index follow the (Y, X) order, checked through the pairing
+        bad |= actual[:, c] != e[:, c]
+    for c, (at, rt) in FIT_TOL.items():
+        bad |= ~(np.abs(actual[:, c] - e[:, c]) <= at + rt * np.abs(e[:, c]))
+    return np.nonzero(bad)[0]
