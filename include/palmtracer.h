@@ -176,6 +176,19 @@ PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n);
  * Writes min(n, 10) values, returns 10. */
 PT_API int PT_CALL pt_stage_times(double* out, uint64_t n);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). Fitting (as above, same
+ * arguments and rows, outputSize dropped) over device memory: d_input planes x height x width u16,
+ * d_positions nPoints x 6 doubles, d_output 18 * nPoints doubles. Its work space is a fixed device
+ * array of the library (about 6.6 MB per GPU, held until the process ends), so the call allocates
+ * nothing; at most 131072 positions and 65535 planes per call, Gaussian fit modes 1-4 only (mode 5
+ * uploads its spline model); larger inputs are refused with a message on stderr. Runs on the legacy
+ * default stream and returns when the rows are written. Calls that overlap on one GPU share that
+ * array and race. Returns 18 * nPoints, or 0 on failure. */
+PT_API uint64_t PT_CALL pt_fitting_noalloc(const uint16_t* d_input, const double* d_positions, uint64_t nPoints,
+                                           double* d_output, uint64_t height, uint64_t width, uint64_t planes,
+                                           uint64_t fit, const double* fit_params);
+
 /* Seeded synthetic stack on the device (benchmark / test input): d_out receives frames x height x
  * width u16. params (15 doubles): bg photons, PSF sigma (px), photon median, photon sigma_log, camera
  * offset (ADU), gain (ADU/photon), astigmatic flag, s0 (px), c (nm), d (nm), zmin, zmax (nm),

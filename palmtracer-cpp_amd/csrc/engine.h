@@ -55,7 +55,21 @@ struct Options {
     int fitHandover = 16;           // fast fit: FP32 iterations before a ROI moves to the FP64 group kernel (0: never)
     int detectRows = 0;             // rows per detection wave (0: chosen from the chunk size)
     int lanes = 2;                  // main lanes the chunks alternate between (1: one stream, for timing)
+    // Defaults with the environment's overrides (PALMTRACER_FIT_PRECISION, PALMTRACER_MAX_ITER,
+    // PALMTRACER_CHUNK_PIXELS).
+    static Options defaults();
 };
+
+// Fit parameters of the Gaussian modes (fit 1-4, and 0 / above 5 as no fit) from fit_params
+// ([roi, sigmaX, sigmaY, theta]; lib/fit/GaussianFit.cpp:39-45, the odd ROI side of
+// lib/fit/Fitting.cpp:35-38) and the options.
+FitParams gaussian_fit_params(uint64_t fit, const double* fp, const Options& opt);
+// Per-plane 1-based index of each position in input order (lib/Palm.cpp:79-84) and the fit jobs of
+// the positions (fit-only entry), on stream s; bad receives 1 when a position's plane is outside
+// [1, planes].
+void launch_positions_to_rois(const double* d_pos, uint64_t n, uint64_t planes, Roi* d_rois, uint32_t* d_bad,
+                              hipStream_t s);
+void launch_positions_ids(const double* d_pos, const uint32_t* d_local, uint64_t n, double* d_rows, hipStream_t s);
 
 // One stream with its own detection buffers. The engine runs the chunks on the main lane and the
 // frames deferred to the exact frame flood on the side lane, concurrently.
@@ -78,6 +92,8 @@ class Engine {
 public:
     // The process-wide engine of the C entry points (lives until the process ends).
     static Engine& get();
+    // The process-wide engine if it exists already, else nullptr (does not create it).
+    static Engine* existing();
     // An independent engine (own streams and buffers), e.g. one per serving slot.
     Engine();
     ~Engine();

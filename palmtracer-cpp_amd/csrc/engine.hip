@@ -2,6 +2,7 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -82,6 +83,21 @@ __global__ void positions_ids(const double* __restrict__ pos, const uint32_t* __
 }
 } // namespace
 
+void launch_positions_to_rois(const double* d_pos, uint64_t n, uint64_t planes, Roi* d_rois, uint32_t* d_bad,
+                              hipStream_t s)
+{
+    const unsigned g = std::max<uint64_t>(1, std::min<uint64_t>(4096, (n + 255) / 256));
+    positions_to_rois<<<g, 256, 0, s>>>(d_pos, n, planes, d_rois, d_bad);
+    PT_CHECK(hipGetLastError());
+}
+
+void launch_positions_ids(const double* d_pos, const uint32_t* d_local, uint64_t n, double* d_rows, hipStream_t s)
+{
+    const unsigned g = std::max<uint64_t>(1, std::min<uint64_t>(4096, (n + 255) / 256));
+    positions_ids<<<g, 256, 0, s>>>(d_pos, d_local, n, d_rows);
+    PT_CHECK(hipGetLastError());
+}
+
 void gather_rois(const Roi* src, uint32_t z0, Roi* dst, const uint32_t* roiBase, uint32_t frames, const uint32_t* base,
                  uint32_t cap, uint32_t* dBase, uint32_t* counters, hipStream_t s)
 {
@@ -111,10 +127,29 @@ void DevBuf::release()
     bytes = 0;
 }
 
+namespace {
+std::atomic<Engine*> g_engine{nullptr};
+}
+
 Engine& Engine::get()
 {
-    static Engine* e = new Engine();  // lives until process exit
+    static Engine* e = [] {
+        Engine* n = new Engine();  // lives until process exit
+        g_engine.store(n);
+        return n;
+    }();
     return *e;
+}
+
+Engine* Engine::existing() { return g_engine.load(); }
+
+Options Options::defaults()
+{
+    Options o;
+    if (const char* v = std::getenv("PALMTRACER_FIT_PRECISION")) o.fitFp64 = std::strcmp(v, "fp32") != 0;
+    if (const char* v = std::getenv("PALMTRACER_MAX_ITER")) o.maxIter = std::max(1, std::atoi(v));
+    if (const char* v = std::getenv("PALMTRACER_CHUNK_PIXELS")) o.chunkPixels = std::max<size_t>(1, std::strtoull(v, nullptr, 10));
+    return o;
 }
 
 Engine::Engine()
@@ -127,9 +162,7 @@ Engine::Engine()
     for (hipEvent_t& e : sideEv_) PT_CHECK(hipEventCreate(&e));
     for (hipEvent_t& e : chunkEv_) PT_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (hipEvent_t& e : fitEv_) PT_CHECK(hipEventCreate(&e));
-    if (const char* v = std::getenv("PALMTRACER_FIT_PRECISION")) opt.fitFp64 = std::strcmp(v, "fp32") != 0;
-    if (const char* v = std::getenv("PALMTRACER_MAX_ITER")) opt.maxIter = std::max(1, std::atoi(v));
-    if (const char* v = std::getenv("PALMTRACER_CHUNK_PIXELS")) opt.chunkPixels = std::max<size_t>(1, std::strtoull(v, nullptr, 10));
+    opt = Options::defaults();
 }
 
 Engine::~Engine()
@@ -299,6 +332,12 @@ FitParams Engine::fit_params(uint64_t fit, const double* fp)
         p.spline = &spline_;
         return p;
     }
+    return gaussian_fit_params(fit, fp, opt);
+}
+
+FitParams gaussian_fit_params(uint64_t fit, const double* fp, const Options& opt)
+{
+    FitParams p{};
     p.mode = (fit >= 1 && fit <= 4) ? (int)fit : 0;
     long roi = 7;
     if (fp) {
@@ -622,14 +661,11 @@ uint64_t Engine::fit_positions(const uint16_t* d_in, uint64_t planes, int H, int
     PT_CHECK(hipMemcpyAsync(dpos.p, hostPos, n * kInCols * 8, hipMemcpyHostToDevice, stream));
     PT_CHECK(hipMemcpyAsync(dloc.p, local.data(), n * 4, hipMemcpyHostToDevice, stream));
     fill_u32_kernel<<<1, 64, 0, stream>>>(dbad.as<uint32_t>(), 1, 0u);
-    const unsigned g = std::min<uint64_t>(4096, (n + 255) / 256);
-    positions_to_rois<<<g, 256, 0, stream>>>(dpos.as<double>(), n, planes, main_.rois.as<Roi>(), dbad.as<uint32_t>());
-    PT_CHECK(hipGetLastError());
+    launch_positions_to_rois(dpos.as<double>(), n, planes, main_.rois.as<Roi>(), dbad.as<uint32_t>(), stream);
     const FitParams fp = fit_params(fit, fitParams);
     launch_fit(d_in, (size_t)W * H, W, H, main_.rois.as<Roi>(), (uint32_t)n, fp, d_out, opt.fitFp64, main_.fit_scratch(n),
                stream, nullptr, nullptr, opt.fitWaves);
-    positions_ids<<<g, 256, 0, stream>>>(dpos.as<double>(), dloc.as<uint32_t>(), n, d_out);
-    PT_CHECK(hipGetLastError());
+    launch_positions_ids(dpos.as<double>(), dloc.as<uint32_t>(), n, d_out, stream);
     uint32_t bad = 0;
     PT_CHECK(hipMemcpyAsync(&bad, dbad.p, 4, hipMemcpyDeviceToHost, stream));
     PT_CHECK(hipStreamSynchronize(stream));

@@ -63,6 +63,8 @@ def lib():
         L.Astigmatism3DEstimation.argtypes = [_vp, _vp, _u64, _f64, _vp, _f64]
         L.pt_astig_estimate_device.restype = _i32
         L.pt_astig_estimate_device.argtypes = [_vp, _vp, _u64, _f64, _vp, _f64]
+        L.pt_fitting_noalloc.restype = _u64
+        L.pt_fitting_noalloc.argtypes = [_vp, _vp, _u64, _vp, _u64, _u64, _u64, _u64, _vp]
         L.pt_server_create.restype = _vp
         L.pt_server_create.argtypes = [_u64, _u64, _u64]
         L.pt_server_submit.restype = ctypes.c_int64
@@ -216,6 +218,16 @@ def localization_device(d_stack, d_out, threshold, watershed_ratio=0.0, fit=2, f
     n = lib().pt_localization_device(d_stack.data_ptr(), d_out.data_ptr(), cap, h, w, z, float(threshold),
                                      float(watershed_ratio), int(fit), p.ctypes.data, mpp)
     return n // NCOL
+
+
+def fitting_noalloc(d_stack, d_positions, d_out, fit=2, fit_params=(7, 1.3, 1.3, 0)):
+    """pt_fitting_noalloc: Fitting over HBM-resident torch tensors (stack (planes, h, w) u16, positions
+    (n, 6) float64, output at least 18 * n float64) without any allocation; returns the number of rows."""
+    z, h, w = d_stack.shape
+    n = d_positions.numel() // 6
+    p = _params(fit_params)
+    return lib().pt_fitting_noalloc(d_stack.data_ptr(), d_positions.data_ptr(), n, d_out.data_ptr(), h, w, z, int(fit),
+                                    p.ctypes.data) // NCOL
 
 
 def astig_estimate(sigmas, pixel_size, model, z_max):
