@@ -257,9 +257,46 @@ __global__ void __launch_bounds__(64) fit_group_kernel(const uint16_t* __restric
             bool circ = false;
             if constexpr (MODE == 4) circ = fabs(q[4] - q[5]) <= 2.220446049250313e-16;
             const double bx = (double)(ux - half) - q[2], by = (double)(uy - half) - q[3];
-            for_pixels([&](int k, double d) {
-                const double X = bx + (double)(k % roi), Y = by + (double)(k / roi);
-                const double E = m.E(X, Y);
+            // An axis-aligned Gaussian (axy == 0) is a column factor times a row factor: lane s of
+            // the group evaluates the factors of columns and rows s and s + 8, and each pixel takes
+            // its two factors from the lanes that hold them (ROI sides up to 16).
+            const int gbase = grp * kGroup;
+            const bool sep = roi <= 2 * kGroup && m.separable();
+            double cf0 = 0.0, cf1 = 0.0, rf0 = 0.0, rf1 = 0.0;
+            if (sep) {
+                const double X0 = bx + (double)sub, Y0 = by + (double)sub;
+                cf0 = exp(-(m.ax * X0 * X0));
+                rf0 = exp(-(m.ay * Y0 * Y0));
+                if (roi > kGroup) {
+                    const double X1 = X0 + (double)kGroup, Y1 = Y0 + (double)kGroup;
+                    cf1 = exp(-(m.ax * X1 * X1));
+                    rf1 = exp(-(m.ay * Y1 * Y1));
+                }
+            }
+            // Every lane of the group runs the same number of pixel slots, so that the shuffles read
+            // factors from active lanes; a slot past the ROI contributes nothing.
+            const int slots = (nPixI + kGroup - 1) / kGroup;
+            for (int m2 = 0; m2 < slots; ++m2) {
+                const int k = sub + m2 * kGroup;
+                const bool valid = k < nPixI;
+                const int kk = valid ? k : 0;
+                const int i = kk / roi, j = kk % roi;
+                const double X = bx + (double)j, Y = by + (double)i;
+                double E;
+                if (sep) {
+                    const double c0 = __shfl(cf0, gbase + (j & (kGroup - 1))), r0 = __shfl(rf0, gbase + (i & (kGroup - 1)));
+                    if (roi > kGroup) {
+                        const double c1 = __shfl(cf1, gbase + (j & (kGroup - 1))), r1 = __shfl(rf1, gbase + (i & (kGroup - 1)));
+                        E = (j < kGroup ? c0 : c1) * (i < kGroup ? r0 : r1);
+                    } else {
+                        E = c0 * r0;
+                    }
+                } else {
+                    E = m.E(X, Y);
+                }
+                if (!valid) continue;
+                const double d = staged ? (double)patch[m2 * 64 + lane]
+                                        : (double)img[(size_t)i * (size_t)W + (size_t)j];
                 const double res = d - (m.off + m.amp * E);
                 e += res * res;
                 double g[N];
@@ -272,7 +309,7 @@ __global__ void __launch_bounds__(64) fit_group_kernel(const uint16_t* __restric
 #pragma unroll
                     for (int b = a; b < N; ++b) al[lm::tri<N>(a, b)] += g[a] * g[b];
                 }
-            });
+            }
         }
         e = group_sum(e);
 #pragma unroll
