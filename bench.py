@@ -5,13 +5,15 @@ Workload (the 512^2 headline configuration): 10,000 seeded frames of 512x512 u16
 frame (ADU = 100 + 2 * Poisson(20 + sum of pixel-integrated Gaussians, sigma 1.3 px, photons
 log-normal median 2000, sigma_log 0.5)), a trous detection with the threshold AutoThreshold returns
 on frame 0, watershed ratio 0, then the 7x7 Gaussian LM fit with free sigma (fit mode 2,
-fit_params [7, 1.3, 1.3, 0], at most 75 iterations). The stack is generated on the device and stays
-in HBM; one step is one Localization over the whole stack (detection, segmentation, fit, row
-assembly), rows written to HBM.
+fit_params [7, 1.3, 1.3, 0], at most 20 iterations as the configuration states; the rate at the
+reference's own cap of 75 iterations is reported beside it). The stack is generated on the device and
+stays in HBM; one step is one Localization over the whole stack (detection, segmentation, fit, row
+assembly), rows written to HBM. `value` counts fitted rows (rows whose fit did not fail).
 
-Prints one JSON line. `roofline` is the dominant stage's kernel measured with HIP events on the
-library's stream; `cpu_baseline` times the compiled reference (oracle/_ref) or, without it, the CPU
-oracle, on a bounded sample of the same frames on this host's cores.
+Prints one JSON line. `roofline` is the dominant kernel (the detection kernel) measured with HIP
+events on the library's stream, `roofline_fit` the fit stage (FP32 kernel and FP64 hand-over kernel);
+`cpu_baseline` times the compiled reference (oracle/_ref, whose cap is 75 iterations) or, without it,
+the CPU oracle, on a bounded sample of the same frames on this host's cores.
 """
 import argparse
 import ctypes
@@ -75,14 +77,13 @@ def cpu_baseline(stack_host, thr, fit_params, budget_s):
     nf = int(min(len(stack_host), max(nf, budget_s / max(per, 1e-6))))
     rows, dt = run(nf)
     return {"value": rows / dt, "unit": "localizations/s", "cores": threads, "kind": kind,
-            "sample": f"first {nf} frames of the same stack ({rows} rows in {dt:.2f} s), OpenMP over frames"}
+            "sample": f"first {nf} frames of the same stack ({rows} rows in {dt:.2f} s), OpenMP over frames, "
+                      f"LM cap 75 iterations (the reference's own)"}
 
 
 def fit_precision(pt, torch, stack, thr, fit_params, frames):
-    """FP32 fit (the bench's) against the FP64 fit (the reference's arithmetic, equal to the CPU oracle
-    to ~1e-14 in the parity tests) on the first frames of the stack: rows paired within each plane by
-    position, largest errors over rows fitted on both sides."""
-    import numpy as np
+    """The fast fit (the bench's) against the reference-order FP64 fit (equal to the CPU oracle within
+    the fit tolerance in the parity tests) on the first frames of the stack."""
     sub = stack[:frames]
     out = torch.empty(frames * 1000 * 18, dtype=torch.float64, device="cuda")
     res = {}
@@ -91,28 +92,7 @@ def fit_precision(pt, torch, stack, thr, fit_params, frames):
         n = pt.localization_device(sub, out, thr, 0.0, 2, fit_params, 1000)
         res[prec] = out[: n * 18].view(-1, 18).cpu().numpy().copy()
     pt.set_option("fit_fp64", 0)
-    a, e = res[0], res[1]
-    X, Y, INT, SX, AMP, OFF = 4, 5, 7, 8, 13, 14
-    pairs = []
-    for p in np.unique(e[:, 1]):
-        A, E = a[a[:, 1] == p], e[e[:, 1] == p]
-        d = ((A[:, None, X:Y + 1] - E[None, :, X:Y + 1]) ** 2).sum(-1)
-        j = d.argmin(1)
-        pairs.append((A, E[j]))
-    A = np.concatenate([q[0] for q in pairs])
-    E = np.concatenate([q[1] for q in pairs])
-    both = (A[:, AMP] > 0) & (E[:, AMP] > 0)
-    one = int(((A[:, AMP] > 0) != (E[:, AMP] > 0)).sum())
-    A, E = A[both], E[both]
-    rel = lambda c: np.abs(A[:, c] - E[:, c]) / np.maximum(np.abs(E[:, c]), 1e-300)
-    within = ((np.abs(A[:, X] - E[:, X]) <= 1e-3) & (np.abs(A[:, Y] - E[:, Y]) <= 1e-3) & (rel(INT) <= 1e-4) &
-              (rel(SX) <= 1e-4) & (rel(AMP) <= 1e-4) & (rel(OFF) <= 1e-4))
-    return {"frames": frames, "rows": int(len(e)), "rows_fp32": int(len(a)), "fitted_on_one_side_only": one,
-            "max_abs_dx_px": float(np.abs(A[:, X] - E[:, X]).max()), "max_abs_dy_px": float(np.abs(A[:, Y] - E[:, Y]).max()),
-            "max_rel_intensity": float(rel(INT).max()), "max_rel_amplitude": float(rel(AMP).max()),
-            "max_rel_sigma": float(rel(SX).max()), "max_rel_offset": float(rel(OFF).max()),
-            "fraction_within_tolerance": float(within.mean()),
-            "tolerance": "x, y 1e-3 px; intensity, amplitude, sigma, offset 1e-4 relative"}
+    return dict(compare_fits(res[0], res[1]), frames=frames)
 
 
 def serve_latency(pt, torch, seconds, slots, n_cams, sizes, fit_params):
@@ -228,6 +208,88 @@ def serve_latency(pt, torch, seconds, slots, n_cams, sizes, fit_params):
     }
 
 
+def astig3d_leg(pt, torch, frames, steps):
+    """Astigmatic 3D workload: `frames` sCMOS-like 2048^2 frames (per-pixel offset ~ N(100, 3) ADU, gain
+    ~ N(2, 0.1)), 2000 emitters per frame with z ~ U[-600, 600] nm and sigma_{x,y} = 1.2 sqrt(1 +
+    ((z -/+ 250) / 400)^2) px; one step = Localization with the 9 x 9 fit of sigma_x / sigma_y (fit
+    mode 3, fast fit) and Astigmatism3DEstimation of every row's widths, both on the device."""
+    import numpy as np
+    cfg = dict(astig=1, s0=1.2, c=250.0, d=400.0, zmin=-600.0, zmax=600.0, scmos=1, sd_offset=3.0, sd_gain=0.1)
+    fit_params = (9.0, 1.4, 1.4, 0.0)
+    model = [250.0, 400.0, 0.0, 0.0, 120.0, -250.0, 400.0, 0.0, 0.0, 120.0]
+    d = torch.empty((frames, 2048, 2048), dtype=torch.uint16, device="cuda")
+    pt.synth_device(d, seed=0x5EED0003, emitters=2000, **cfg)
+    thr = pt.autothreshold_device(d[0], (9.0,))
+    mpp = 4000
+    out = torch.empty(frames * mpp * 18, dtype=torch.float64, device="cuda")
+    zs = torch.empty((frames * mpp, 2), dtype=torch.float64, device="cuda")
+
+    def step():
+        n = pt.localization_device(d, out, thr, 0.0, 3, fit_params, mpp)
+        rows = out[: n * 18].view(-1, 18)
+        sig = rows[:, 8:10].contiguous()
+        pt.astig_estimate_device(sig, zs[:n], 100.0, model, 600.0)
+        return n
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        n = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    st = pt.stage_times()
+    handed = pt.last_stats()["fit_handed_over"]
+    rows = out[: n * 18].view(-1, 18)
+    fitted = int((rows[:, 13] > 0).sum().item())
+    # Fast fit against the reference-order FP64 fit on the first 20 frames of the stack.
+    sub = d[:20]
+    res = {}
+    o2 = torch.empty(20 * mpp * 18, dtype=torch.float64, device="cuda")
+    for prec in (0, 1):
+        pt.set_option("fit_fp64", prec)
+        k = pt.localization_device(sub, o2, thr, 0.0, 3, fit_params, mpp)
+        res[prec] = o2[: k * 18].view(-1, 18).cpu().numpy().copy()
+    pt.set_option("fit_fp64", 0)
+    prec = compare_fits(res[0], res[1], sigma_y=True)
+    return {"workload": f"{frames} frames 2048x2048 u16 sCMOS (offset N(100,3), gain N(2,0.1)), 2000 astigmatic "
+                        f"emitters/frame (z U[-600,600] nm, s0 1.2 px, c 250 nm, d 400 nm), 9x9 fit mode 3 + "
+                        f"Astigmatism3DEstimation", "frames": frames, "steps": steps, "ms_per_step": dt * 1e3,
+            "localizations_per_s": fitted / dt, "rows": n, "fitted_rows": fitted, "threshold": thr,
+            "stage_ms": {"atrous": st["atrous_ms"], "segment": st["segment_ms"], "fit": st["fit_ms"],
+                         "assemble": st["assemble_ms"]},
+            "fit_handed_over": handed, "fit_precision_first_20_frames": prec}
+
+
+def compare_fits(a, e, sigma_y=False):
+    """Fast-fit rows `a` against FP64 rows `e` of the same frames: rows paired within each plane by
+    position, largest errors over rows fitted on both sides."""
+    import numpy as np
+    X, Y, INT, SX, SY, AMP, OFF = 4, 5, 7, 8, 9, 13, 14
+    pairs = []
+    for p in np.unique(e[:, 1]):
+        A, E = a[a[:, 1] == p], e[e[:, 1] == p]
+        d = ((A[:, None, X:Y + 1] - E[None, :, X:Y + 1]) ** 2).sum(-1)
+        pairs.append((A, E[d.argmin(1)]))
+    A = np.concatenate([q[0] for q in pairs])
+    E = np.concatenate([q[1] for q in pairs])
+    both = (A[:, AMP] > 0) & (E[:, AMP] > 0)
+    one = int(((A[:, AMP] > 0) != (E[:, AMP] > 0)).sum())
+    A, E = A[both], E[both]
+    rel = lambda c: np.abs(A[:, c] - E[:, c]) / np.maximum(np.abs(E[:, c]), 1e-300)
+    cols = [SX, SY] if sigma_y else [SX]
+    within = ((np.abs(A[:, X] - E[:, X]) <= 1e-3) & (np.abs(A[:, Y] - E[:, Y]) <= 1e-3) & (rel(INT) <= 1e-4) &
+              (rel(AMP) <= 1e-4) & (rel(OFF) <= 1e-4))
+    for c in cols:
+        within &= rel(c) <= 1e-4
+    return {"rows": int(len(e)), "rows_fast": int(len(a)), "fitted_on_one_side_only": one,
+            "max_abs_dx_px": float(np.abs(A[:, X] - E[:, X]).max()), "max_abs_dy_px": float(np.abs(A[:, Y] - E[:, Y]).max()),
+            "max_rel_intensity": float(rel(INT).max()), "max_rel_amplitude": float(rel(AMP).max()),
+            "max_rel_sigma": float(max(rel(c).max() for c in cols)), "max_rel_offset": float(rel(OFF).max()),
+            "fraction_within_tolerance": float(within.mean()),
+            "tolerance": "x, y 1e-3 px; intensity, amplitude, sigma, offset 1e-4 relative"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,13 +298,18 @@ def main():
     ap.add_argument("--frames", type=int, default=10000)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--emitters", type=int, default=200)
-    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32",
+                    help="fp32: the fast fit (FP32 kernel, FP64 hand-over); fp64: the reference-order FP64 fit")
+    ap.add_argument("--max-iter", type=int, default=20, help="LM iteration cap (the configuration's 20)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=4.0,
                     help="length of the streamed-load phase (0: skip it)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-cameras", type=int, default=16)
+    ap.add_argument("--astig-frames", type=int, default=1000,
+                    help="frames of the astigmatic 3D leg (2048^2 sCMOS; 0: skip it)")
+    ap.add_argument("--astig-steps", type=int, default=3)
     ap.add_argument("--serve-sizes", default="256,512,1024",
                     help="frame sides of the cameras, assigned in turn (e.g. 256,512,1024,2048)")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
@@ -277,7 +344,7 @@ def main():
     max_per_plane = 1000
     out = torch.empty(F * max_per_plane * 18, dtype=torch.float64, device="cuda")
     pt.set_option("fit_fp64", 1 if args.precision == "fp64" else 0)
-    pt.set_option("max_iter", 75)
+    pt.set_option("max_iter", args.max_iter)
     pt.set_option("profile", 1)
     options = {}
     for kv in args.set:
@@ -289,7 +356,8 @@ def main():
         pt.localization_device(stack, out, thr, 0.0, 2, fit_params, max_per_plane)
     torch.cuda.synchronize()
     stages = {"atrous_ms": 0.0, "segment_ms": 0.0, "fit_ms": 0.0, "assemble_ms": 0.0, "chunks": 0.0,
-              "jacobian_passes": 0.0, "trial_passes": 0.0, "deferred_ms": 0.0}
+              "jacobian_passes": 0.0, "trial_passes": 0.0, "deferred_ms": 0.0, "group_jacobian_passes": 0.0,
+              "group_trial_passes": 0.0}
     rows = 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -312,29 +380,50 @@ def main():
     px_per_launch = F * S * S / (stages["chunks"] / args.steps)
     if rows == 0:
         raise SystemExit("no localizations: the pipeline reported an error (see stderr)")
-    # The fit: one launch per Localization call over all ROIs of the call (the deferred frames' fits on
-    # the side lane add their passes, under 1 % of them, to the counters but not their time).
+    # The fit stage of a call: the FP32 kernel over all ROIs of the call, then the FP64 group kernel
+    # over the ROIs it handed over (the deferred frames' fits on the side lane add their passes, under
+    # 1 % of them, to the counters but not their time). FLOPs on the reference's pass schedule.
     fit_ms = stages["fit_ms"] / args.steps
-    flops = (stages["jacobian_passes"] * JAC_FLOP_7x7 + stages["trial_passes"] * TRIAL_FLOP_7x7) / args.steps
-    fit_peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
-    fit_ach = flops / (fit_ms * 1e-3) / 1e12
-    fit_kernel = ("fit_kernel (FP64 LM in the reference's operation order)" if args.precision == "fp64"
-                  else "fit_persistent_kernel (FP32 VALU LM on the reference's pass schedule)")
+    flops32 = (stages["jacobian_passes"] * JAC_FLOP_7x7 + stages["trial_passes"] * TRIAL_FLOP_7x7) / args.steps
+    flops64 = (stages["group_jacobian_passes"] * JAC_FLOP_7x7 + stages["group_trial_passes"] * TRIAL_FLOP_7x7) / args.steps
+    if args.precision == "fp64":
+        fit_kernel, fit_peak = "fit_kernel (FP64 LM in the reference's operation order, one ROI per lane)", FP64_PEAK_TFLOPS
+    else:
+        fit_kernel = ("fit_persistent_kernel (FP32 LM, one ROI per lane) + fit_group_kernel (FP64 LM, eight lanes "
+                      "per ROI, the ROIs handed over)")
+        fit_peak = FP32_PEAK_TFLOPS
+    fit_ach = (flops32 + flops64) / (fit_ms * 1e-3) / 1e12
     roof_fit = {"bound": "valu", "kernel": fit_kernel,
                 "achieved": fit_ach, "peak": fit_peak, "unit": "TFLOP/s", "frac": fit_ach / fit_peak,
-                "launch_ms": fit_ms, "flop_per_launch": flops,
-                "jacobian_passes": stages["jacobian_passes"] / args.steps, "trial_passes": stages["trial_passes"] / args.steps}
-    if stages["fit_ms"] >= stages["atrous_ms"]:
-        roof = dict(roof_fit, bound="mfma", traffic=None)
-    else:
-        bytes_l = ATROUS_BYTES_PER_PX * px_per_launch
-        ach = bytes_l / (atrous_ms * 1e-3) / 1e9
-        tpp = measured_traffic_per_px(S)
-        roof = {"bound": "hbm", "kernel": "atrous_detect_kernel", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                "traffic": None if tpp is None else tpp * px_per_launch, "launch_ms": atrous_ms,
-                "bytes_per_launch": bytes_l}
+                "launch_ms": fit_ms, "flop_per_launch": flops32 + flops64, "flop_fp64_part": flops64,
+                "jacobian_passes": stages["jacobian_passes"] / args.steps,
+                "trial_passes": stages["trial_passes"] / args.steps,
+                "group_jacobian_passes": stages["group_jacobian_passes"] / args.steps,
+                "group_trial_passes": stages["group_trial_passes"] / args.steps,
+                "rois_handed_to_fp64": last["fit_handed_over"]}
+    # The dominant kernel by device time: the detection kernel (HBM-bound).
+    bytes_l = ATROUS_BYTES_PER_PX * px_per_launch
+    ach = bytes_l / (atrous_ms * 1e-3) / 1e9
+    tpp = measured_traffic_per_px(S)
+    roof = {"bound": "hbm", "kernel": "atrous_detect_kernel", "achieved": ach, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": None if tpp is None else tpp * px_per_launch, "launch_ms": atrous_ms,
+            "bytes_per_launch": bytes_l}
     stage_ms = {k: v / args.steps for k, v in stages.items() if k.endswith("_ms")}
+
+    # The same workload at the reference's own iteration cap (75), fitted rows per second.
+    alt_iter = 75 if args.max_iter != 75 else 20
+    pt.set_option("max_iter", alt_iter)
+    pt.localization_device(stack, out, thr, 0.0, 2, fit_params, max_per_plane)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        n_alt = pt.localization_device(stack, out, thr, 0.0, 2, fit_params, max_per_plane)
+    torch.cuda.synchronize()
+    ms_alt = 1e3 * (time.perf_counter() - t1) / args.steps
+    fitted_alt = int((out[: n_alt * 18].view(-1, 18)[:, 13] > 0).sum().item())
+    other_cap = {"max_iter": alt_iter, "ms_per_step": ms_alt, "localizations_per_s": fitted_alt / (ms_alt * 1e-3)}
+    pt.set_option("max_iter", args.max_iter)
 
     precision = fit_precision(pt, torch, stack, thr, fit_params, min(F, 200)) if args.precision == "fp32" else None
 
@@ -343,6 +432,8 @@ def main():
         sizes = [int(v) for v in args.serve_sizes.split(",")]
         serving = serve_latency(pt, torch, args.serve_seconds, args.serve_slots, args.serve_cameras, sizes, fit_params)
 
+    astig = astig3d_leg(pt, torch, args.astig_frames, args.astig_steps) if args.astig_frames > 0 else None
+
     cpu = None
     if not args.no_cpu_baseline:
         host = stack[: min(F, 4000)].cpu().numpy()
@@ -350,7 +441,7 @@ def main():
 
     line = {
         "metric": "fitted localizations/sec at 512^2 frames",
-        "value": rows / (ms_step * 1e-3),
+        "value": fitted / (ms_step * 1e-3),
         "unit": "localizations/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -362,8 +453,9 @@ def main():
         "dtype": "f32" if args.precision == "fp32" else "f64",
         "data": "synthetic: seeded on-device generator (Philox), stack resident in HBM",
         "config": {"workload": f"{F} frames {S}x{S} u16, {args.emitters} emitters/frame, a trous + watershed "
-                               f"(ratio 0) + 7x7 Gaussian LM fit mode 2 (max 75 it), threshold {thr:.6f} "
-                               f"(AutoThreshold of frame 0)",
+                               f"(ratio 0) + 7x7 Gaussian LM fit mode 2 (max {args.max_iter} it), threshold "
+                               f"{thr:.6f} (AutoThreshold of frame 0)", "max_iter": args.max_iter,
+                   "other_iteration_cap": other_cap,
                    "frames": F, "height": S, "width": S, "emitters_per_frame": args.emitters, "fit": 2,
                    "roi": 7, "threshold": thr, "rows": rows, "fitted_rows": fitted, "seed": seed,
                    "stage_ms_per_step": stage_ms, "options": options,
@@ -374,6 +466,7 @@ def main():
         "roofline": roof,
         "roofline_fit": roof_fit,
         "fit_precision": precision,
+        "astig3d": astig,
         "serving": serving,
         "cpu_baseline": cpu,
     }

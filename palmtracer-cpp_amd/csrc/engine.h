@@ -122,7 +122,7 @@ public:
 private:
     Lane main_, mainB_, side_;   // two alternating lanes for the chunks of a call, one for deferred frames
     hipEvent_t sideEv_[2] = {nullptr, nullptr};  // side batch start / done
-    DevBuf atMask_, atPart_, atRows_, passes_, allRois_, allBase_, sideRows_;
+    DevBuf atMask_, atPart_, atRows_, passes_, allRois_, allBase_, sideRois_, sideBase_;
     hipEvent_t fitEv_[3] = {nullptr, nullptr, nullptr};
     hipEvent_t chunkEv_[2] = {nullptr, nullptr};  // a main-lane chunk's counters are on the host
     hipEvent_t scanEv_[2] = {nullptr, nullptr};   // a chunk's ROI scan is done (orders the running total)

@@ -65,6 +65,18 @@ __global__ void gather_rois_kernel(const Roi* __restrict__ src, uint32_t z0, Roi
     for (uint32_t k = t; k < frames; k += stride) dBase[k] = off + roiBase[k];
 }
 
+// ROIs of a side-lane batch appended to the side ROI store with their stack frame numbers.
+__global__ void side_gather_kernel(const Roi* __restrict__ src, const uint32_t* __restrict__ n, const uint32_t* __restrict__ map,
+                                   Roi* __restrict__ dst)
+{
+    const uint32_t m = *n;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        Roi r = src[i];
+        r.frame = map[r.frame];
+        dst[i] = r;
+    }
+}
+
 __global__ void masked_labels(const uint32_t* __restrict__ fin, const uint32_t* __restrict__ P, uint32_t* __restrict__ out,
                               size_t n)
 {
@@ -170,7 +182,7 @@ Engine::~Engine()
     for (Lane* L : {&main_, &mainB_, &side_})
         if (L->stream) (void)hipStreamSynchronize(L->stream);
     for (Lane* L : {&main_, &mainB_, &side_}) L->release();
-    for (DevBuf* b : {&in, &out, &planeInfo, &atMask_, &atPart_, &atRows_, &passes_, &allRois_, &allBase_, &sideRows_,
+    for (DevBuf* b : {&in, &out, &planeInfo, &atMask_, &atPart_, &atRows_, &passes_, &allRois_, &allBase_, &sideRois_, &sideBase_,
                       &run_, &splineRaw_, &splineCv_})
         b->release();
     for (hipEvent_t e : sideEv_) (void)hipEventDestroy(e);
@@ -422,13 +434,13 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     uint32_t* dBase = allBase_.as<uint32_t>();
 
     // Side lane: frames whose component floods could not be proven are re-detected with the exact
-    // frame flood, fitted and sorted on the side stream while the main lane goes on; each batch's
-    // sorted rows are appended to sideRows_ on that stream.
+    // frame flood on the side stream while the main lanes go on. Each batch's ROIs are appended, with
+    // their stack frame numbers, to the side ROI store (in the order of `deferred`), and fitted with the
+    // main lanes' ROIs in the call's one fit launch.
     const size_t SF = std::max<size_t>(1, std::min<size_t>(F, std::max<size_t>(64, (size_t(1) << 24) / npx)));
-    const uint64_t sideRowCap = SF * std::min<uint64_t>(maxPerPlane, npx / 4 + 2);
-    std::vector<uint64_t> sideSrc;      // per deferred entry: first row in sideRows_
+    std::vector<uint64_t> sideSrc;      // per deferred entry: first ROI in the side store
     std::vector<uint32_t> sideCnt;
-    uint64_t sideRowsTotal = 0;
+    uint64_t sideRoiTotal = 0;
     size_t deferredNext = 0;            // next entry of `deferred` not yet sent
     size_t batchFirst = 0, batchCount = 0;
     hipEvent_t sideStart = sideEv_[0], sideDone = sideEv_[1];
@@ -437,8 +449,6 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         batchFirst = deferredNext;
         deferredNext += batchCount;
         side_.prepare(SF, npx);
-        side_.rows.ensure(std::max<uint64_t>(sideRowCap, 1) * kCols * 8);
-        side_.staged.ensure(std::max<uint64_t>(sideRowCap, 1) * kCols * 8);
         uint32_t* h = side_.host(SF + (SF + 1) + kNumCounters);
         std::memcpy(h, deferred.data() + batchFirst, batchCount * 4);
         side_.frameMap.ensure(SF * 4);
@@ -446,17 +456,12 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         PT_CHECK(hipMemcpyAsync(side_.frameMap.p, h, batchCount * 4, hipMemcpyHostToDevice, side_.stream));
         const uint32_t* dmap = side_.frameMap.as<uint32_t>();
         detect_chunk(side_, d_in, batchCount, H, W, thr, ratio, maxPerPlane, kNoEvents, dmap, true);
-        const uint32_t sideFitCap = (uint32_t)std::min<uint64_t>(sideRowCap, 0xFFFFFFFFull);
-        launch_fit(d_in, npx, W, H, side_.rois.as<Roi>(), sideFitCap, fp, side_.rows.as<double>(), opt.fitFp64,
-                   side_.fit_scratch(sideFitCap), side_.stream, passes, dmap, opt.fitWaves, side_.b.roiBase + batchCount);
-        launch_sort_frames(side_.rows.as<double>(), side_.b.roiBase, (int)batchCount, side_.staged.as<double>(),
-                           side_.stream, frameRows);
         PT_CHECK(hipMemcpyAsync(h + SF, side_.b.roiBase, (batchCount + 1) * 4, hipMemcpyDeviceToHost, side_.stream));
         PT_CHECK(hipMemcpyAsync(h + SF + SF + 1, side_.b.counters, kNumCounters * 4, hipMemcpyDeviceToHost, side_.stream));
         PT_CHECK(hipEventRecord(sideDone, side_.stream));
     };
-    // Waits for the batch in flight and appends its rows to sideRows_ (on the side stream, before
-    // the next batch reuses the side lane's buffers).
+    // Waits for the batch in flight and appends its ROIs to the side store (on the side stream,
+    // before the next batch reuses the side lane's buffers).
     auto side_collect = [&]() {
         if (!batchCount) return;
         PT_CHECK(hipEventSynchronize(sideDone));
@@ -479,15 +484,18 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
             last.msDeferred += ms;
         }
         const uint32_t n = sbase[batchCount];
-        grow(sideRows_, (sideRowsTotal + n) * kCols * 8, sideRowsTotal * kCols * 8, side_.stream);
-        if (n)
-            PT_CHECK(hipMemcpyAsync(sideRows_.as<double>() + sideRowsTotal * kCols, side_.staged.p, (size_t)n * kCols * 8,
-                                    hipMemcpyDeviceToDevice, side_.stream));
+        grow(sideRois_, (sideRoiTotal + n) * sizeof(Roi), sideRoiTotal * sizeof(Roi), side_.stream);
+        if (n) {
+            side_gather_kernel<<<std::max(1u, std::min(1024u, ceil_div(n, 256))), 256, 0, side_.stream>>>(
+                side_.rois.as<Roi>(), side_.b.roiBase + batchCount, side_.frameMap.as<uint32_t>(),
+                sideRois_.as<Roi>() + sideRoiTotal);
+            PT_CHECK(hipGetLastError());
+        }
         for (size_t k = 0; k < batchCount; ++k) {
-            sideSrc[batchFirst + k] = sideRowsTotal + sbase[k];
+            sideSrc[batchFirst + k] = sideRoiTotal + sbase[k];
             sideCnt[batchFirst + k] = sbase[k + 1] - sbase[k];
         }
-        sideRowsTotal += n;
+        sideRoiTotal += n;
         batchCount = 0;
     };
     // A batch is sent whenever the side lane is idle, so that the deferred frames of a chunk are
@@ -565,37 +573,54 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         last.chunks++;
     }
     read_meta(chunkNo - 1);
-    set_u32(dBase + planes, (uint32_t)totalRois, main_.stream);
-    // Fit of every ROI, then each plane's rows in (Y, X) order.
-    main_.rows.ensure(std::max<uint64_t>(totalRois, 1) * kCols * 8);
-    grow(main_.staged, std::max<uint64_t>(totalRois, 1) * kCols * 8, 0, main_.stream);
-    if (opt.profile) PT_CHECK(hipEventRecord(fitEv_[0], main_.stream));
-    if (totalRois)
-        launch_fit(d_in, npx, W, H, allRois_.as<Roi>(), (uint32_t)totalRois, fp, main_.rows.as<double>(), opt.fitFp64,
-                   main_.fit_scratch(totalRois), main_.stream, passes, nullptr, opt.fitWaves);
-    if (opt.profile) PT_CHECK(hipEventRecord(fitEv_[1], main_.stream));
-    if (totalRois)
-        launch_sort_frames(main_.rows.as<double>(), dBase, (int)planes, main_.staged.as<double>(), main_.stream, frameRows);
-    if (opt.profile) PT_CHECK(hipEventRecord(fitEv_[2], main_.stream));
+    // Every deferred frame detected and its ROIs in the side store.
     side_poll(true);
     PT_CHECK(hipStreamSynchronize(side_.stream));
     last.deferredFrames = deferred.size();
-
-    // Placement in plane order, planned on the device from the plane bases of the main rows; the
-    // deferred planes take the side lane's rows, placed after the main lane's in the staged array,
-    // and are the only host input. The host reads the row total once, after the last copy.
+    set_u32(dBase + planes, (uint32_t)totalRois, main_.stream);
+    // The fit covers the main lanes' ROIs (plane order) followed by the side store's (deferred order).
+    const uint64_t fitRois = totalRois + sideRoiTotal;
+    if (fitRois > 0xFFFFFFF0ull) throw Error("too many ROIs in one call");
     const uint32_t nDef = (uint32_t)deferred.size();
-    uint64_t* hp = reinterpret_cast<uint64_t*>(main_.host(2 * (3 * (size_t)nDef + 6)));   // + total, LM passes, hand-over
+    // Host block: deferred planes, their first side ROI and count (u64 each), then the row total, the
+    // LM pass counters and the hand-over count, then the side rows' per-frame bases (u32).
+    uint32_t* hw = main_.host(2 * (3 * (size_t)nDef + 6) + nDef + 2);
+    uint64_t* hp = reinterpret_cast<uint64_t*>(hw);
+    uint32_t* hSideBase = hw + 2 * (3 * (size_t)nDef + 6);
     for (uint32_t k = 0; k < nDef; ++k) {
         hp[k] = deferred[k];
         hp[nDef + k] = sideSrc[k];
         hp[2 * (size_t)nDef + k] = sideCnt[k];
+        hSideBase[k] = (uint32_t)sideSrc[k];
     }
-    if (sideRowsTotal) {
-        grow(main_.staged, (totalRois + sideRowsTotal) * kCols * 8, totalRois * kCols * 8, main_.stream);
-        PT_CHECK(hipMemcpyAsync(main_.staged.as<double>() + totalRois * kCols, sideRows_.p, sideRowsTotal * kCols * 8,
+    hSideBase[nDef] = (uint32_t)sideRoiTotal;
+    if (sideRoiTotal) {
+        grow(allRois_, fitRois * sizeof(Roi), totalRois * sizeof(Roi), main_.stream);
+        PT_CHECK(hipMemcpyAsync(allRois_.as<Roi>() + totalRois, sideRois_.p, sideRoiTotal * sizeof(Roi),
                                 hipMemcpyDeviceToDevice, main_.stream));
     }
+    // Fit of every ROI, then each plane's rows in (Y, X) order: the main planes by their plane bases,
+    // the deferred planes' rows (after them) by their side bases.
+    main_.rows.ensure(std::max<uint64_t>(fitRois, 1) * kCols * 8);
+    grow(main_.staged, std::max<uint64_t>(fitRois, 1) * kCols * 8, 0, main_.stream);
+    if (opt.profile) PT_CHECK(hipEventRecord(fitEv_[0], main_.stream));
+    if (fitRois)
+        launch_fit(d_in, npx, W, H, allRois_.as<Roi>(), (uint32_t)fitRois, fp, main_.rows.as<double>(), opt.fitFp64,
+                   main_.fit_scratch(fitRois), main_.stream, passes, nullptr, opt.fitWaves);
+    if (opt.profile) PT_CHECK(hipEventRecord(fitEv_[1], main_.stream));
+    if (totalRois)
+        launch_sort_frames(main_.rows.as<double>(), dBase, (int)planes, main_.staged.as<double>(), main_.stream, frameRows);
+    if (sideRoiTotal) {
+        sideBase_.ensure((nDef + 1) * 4);
+        PT_CHECK(hipMemcpyAsync(sideBase_.p, hSideBase, (nDef + 1) * 4, hipMemcpyHostToDevice, main_.stream));
+        launch_sort_frames(main_.rows.as<double>() + totalRois * kCols, sideBase_.as<uint32_t>(), (int)nDef,
+                           main_.staged.as<double>() + totalRois * kCols, main_.stream, frameRows);
+    }
+    if (opt.profile) PT_CHECK(hipEventRecord(fitEv_[2], main_.stream));
+
+    // Placement in plane order, planned on the device from the plane bases of the main rows; the
+    // deferred planes take the side store's rows, placed after the main lanes' in the staged array,
+    // and are the only host input. The host reads the row total once, after the last copy.
     planeInfo.ensure(planes * 20 + (3 * (uint64_t)nDef + 1) * 8 + 64);
     uint64_t* dSrc = planeInfo.as<uint64_t>();
     uint64_t* dDst = dSrc + planes;
@@ -608,7 +633,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     PT_CHECK(hipMemcpyAsync(hp + 3 * (size_t)nDef, dTotal, 8, hipMemcpyDeviceToHost, main_.stream));
     if (passes) PT_CHECK(hipMemcpyAsync(hp + 3 * (size_t)nDef + 1, passes, 32, hipMemcpyDeviceToHost, main_.stream));
     hp[3 * (size_t)nDef + 5] = 0;
-    const bool fastFit = totalRois && !opt.fitFp64 && fp.mode >= 1 && fp.mode <= 4;
+    const bool fastFit = fitRois && !opt.fitFp64 && fp.mode >= 1 && fp.mode <= 4;
     if (fastFit)
         PT_CHECK(hipMemcpyAsync(hp + 3 * (size_t)nDef + 5, main_.fitNext.as<unsigned int>() + 1, 4, hipMemcpyDeviceToHost,
                                 main_.stream));
@@ -633,7 +658,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         last.groupJacPasses = hp[3 * (size_t)nDef + 3];
         last.groupTrialPasses = hp[3 * (size_t)nDef + 4];
     }
-    last.fitHanded = fastFit ? (fp.mode == 4 && fp.handover > 0 ? totalRois : hp[3 * (size_t)nDef + 5]) : 0;
+    last.fitHanded = fastFit ? (fp.mode == 4 && fp.handover > 0 ? fitRois : hp[3 * (size_t)nDef + 5]) : 0;
     last.frames = planes;
     last.rois = total;
     return total * kCols;
