@@ -102,8 +102,9 @@ PT_API int PT_CALL pt_astig_estimate_device(const double* d_input, double* d_out
 /* Serving loop for several streams of frame chunks (cameras). `slots` independent engines (own
  * streams and chunk buffers; chunkPixels frames x pixels per device chunk, 0: the default) each run
  * one request at a time, so requests on different slots overlap on the GPU. Requests leave the queue
- * in arrival order when a slot is free and the HBM bytes of the requests in flight (input frames +
- * output capacity) plus their own fit in budgetBytes. A request is Localization over device memory
+ * in arrival order when a slot is free and their HBM (input frames, output capacity and the engine
+ * work space they need, see pt_server_request_bytes) fits in budgetBytes beside the requests in flight
+ * and the work space the slots hold (idle slots release theirs when a request needs the room). A request is Localization over device memory
  * (as pt_localization_device; nFitParams doubles of fit_params are copied at submission, so the
  * caller may reuse them); its result equals that call's whatever else is in flight. Options
  * (pt_set_option) in force at creation apply to every slot. Returns NULL on failure. */
@@ -118,6 +119,11 @@ PT_API int64_t PT_CALL pt_server_submit(void* server, const uint16_t* d_input, u
  * submission, start and completion in ms on the server's clock, and the slot that ran it. Returns 1
  * on success, 0 if the request failed (message on stderr). Each id is waited for once. */
 PT_API int PT_CALL pt_server_wait(void* server, int64_t request, uint64_t* doublesWritten, double* times);
+/* HBM the server's admission counts for a request of these sizes: its input frames and output rows
+ * plus the engine work space it needs (chunk buffers, ROI stores, rows; the sizes the engine
+ * allocates). A submission above budgetBytes is refused. Returns 0 on bad arguments. */
+PT_API uint64_t PT_CALL pt_server_request_bytes(void* server, uint64_t planes, uint64_t height, uint64_t width,
+                                                uint64_t outputCapDoubles, uint64_t maxPerPlane);
 /* The server's clock (ms since creation). */
 PT_API double PT_CALL pt_server_now(void* server);
 /* Finishes the queued requests, then releases the slots. */

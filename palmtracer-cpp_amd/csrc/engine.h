@@ -84,6 +84,8 @@ struct Lane {
     void zero();
     // Work space of a fast fit of up to nRoi ROIs on this lane (queue words and hand-over list).
     FitScratch fit_scratch(size_t nRoi);
+    size_t device_bytes() const;   // device memory held by the lane's buffers
+    void release_buffers();        // frees the device buffers (the stream stays)
     uint32_t* host(size_t words);
     void release();   // device and pinned memory and the stream
 };
@@ -116,6 +118,14 @@ public:
     void debug_labels(const uint16_t* d_frame, int H, int W, double thr, double ratio, uint32_t* d_labels);
     // AutoThreshold of one device frame (lib/Palm.cpp:122-172).
     double autothreshold(const uint16_t* d_frame, int H, int W, const double* fitParams);
+    // Frames per device chunk for frames of npx pixels.
+    size_t chunk_frames(uint64_t planes, size_t npx) const;
+    // Device bytes a Localization of `planes` frames of H x W allocates in this engine (its lanes'
+    // chunk buffers, ROI stores and rows; the sizes Lane::prepare and localize use).
+    size_t workspace_bytes(uint64_t planes, int H, int W, uint64_t maxPerPlane, uint64_t outCapRows) const;
+    // Device memory the engine holds now, and a release of all of it (buffers grow again on demand).
+    size_t device_bytes() const;
+    void release_device_buffers();
 
     DevBuf in, out, planeInfo;
 

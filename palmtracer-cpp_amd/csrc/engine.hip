@@ -204,9 +204,36 @@ void Lane::release()
     stream = nullptr;
 }
 
+namespace {
+// Sizes of a lane's buffers for chunks of F frames of npx pixels (Lane::prepare allocates them;
+// Engine::workspace_bytes adds them up for the serving loop's HBM budget).
+struct LaneSizes {
+    size_t n, regionCap, multiCap, fbSlots, rfSlots;
+    LaneSizes(size_t F, size_t npx)
+    {
+        n = F * npx;
+        regionCap = npx / 4 + 2;  // roots are pairwise non-adjacent: at most ceil(H/2)*ceil(W/2)
+        multiCap = n / 64 + 1024;
+        fbSlots = std::clamp<size_t>((size_t(1) << 26) / std::max<size_t>(npx, 1), 4, 64);
+        // Rank-flood slots: up to 128 frames at a time, within 1 GiB of work space.
+        rfSlots = std::clamp<size_t>(std::min<size_t>(F, (size_t(1) << 30) / (rf_slot_words(npx) * 4)), 1, 128);
+    }
+    // Device bytes of a prepared lane plus its ROI buffer (DevBuf allocates at least 256 bytes).
+    size_t bytes(size_t F, size_t npx) const
+    {
+        auto b = [](size_t v) { return std::max<size_t>(v, 256); };
+        return b(n * 4) * 5 + b(n * sizeof(PixAux)) + b((F + 1) * 4 * 9) + b(F * regionCap * sizeof(Region)) +
+               b(F * regionCap * sizeof(Fz)) + b(F * regionCap * 4) + b(multiCap * 8) + b(fbSlots * fb_slot_words(npx) * 4) +
+               b(kNumCounters * 4) + b((F + 1) * 12) + b(rfSlots * rf_slot_words(npx) * 4) + b(F * seed_words(npx) * 4) +
+               b(F * kHistBins * 4) + b(F * regionCap * sizeof(Roi));
+    }
+};
+} // namespace
+
 void Lane::prepare(size_t F, size_t npx)
 {
-    const size_t n = F * npx;
+    const LaneSizes z(F, npx);
+    const size_t n = z.n;
     P.ensure(n * 4);
     lab.ensure(n * 4);
     fin.ensure(n * 4);
@@ -214,18 +241,17 @@ void Lane::prepare(size_t F, size_t npx)
     mask.ensure(n * 4);
     seed.ensure(n * 4);
     perFrame.ensure((F + 1) * 4 * 9);
-    const size_t regionCap = npx / 4 + 2;  // roots are pairwise non-adjacent: at most ceil(H/2)*ceil(W/2)
+    const size_t regionCap = z.regionCap;
     regions.ensure(F * regionCap * sizeof(Region));
     fz.ensure(F * regionCap * sizeof(Fz));
     order.ensure(F * regionCap * 4);
-    const size_t multiCap = n / 64 + 1024;
+    const size_t multiCap = z.multiCap;
     multi.ensure(multiCap * 8);
-    const size_t fbSlots = std::clamp<size_t>((size_t(1) << 26) / std::max<size_t>(npx, 1), 4, 64);
+    const size_t fbSlots = z.fbSlots;
     fb.ensure(fbSlots * fb_slot_words(npx) * 4);
     counters.ensure(kNumCounters * 4);
     flag.ensure((F + 1) * 12);
-    // Rank-flood slots: up to 128 frames at a time, within 1 GiB of work space.
-    const size_t rfSlots = std::clamp<size_t>(std::min<size_t>(F, (size_t(1) << 30) / (rf_slot_words(npx) * 4)), 1, 128);
+    const size_t rfSlots = z.rfSlots;
     rf.ensure(rfSlots * rf_slot_words(npx) * 4);
     sbits.ensure(F * seed_words(npx) * 4);
     hist.ensure(F * kHistBins * 4);
@@ -273,6 +299,64 @@ void Lane::zero()
     zero3_kernel<<<1024, 256, 0, stream>>>(b.maskCnt, 6 * (b.frames + 1), b.counters, kNumCounters, b.seedBits,
                                            b.frames * seed_words(b.npx));
     PT_CHECK(hipGetLastError());
+}
+
+size_t Lane::device_bytes() const
+{
+    size_t t = 0;
+    for (const DevBuf* d : {&P, &lab, &fin, &aux, &mask, &seed, &perFrame, &regions, &fz, &order, &multi, &fb, &counters,
+                            &flag, &rf, &sbits, &hist, &rois, &rows, &staged, &frameMap, &fitNext, &fitHand})
+        t += d->bytes;
+    return t;
+}
+
+void Lane::release_buffers()
+{
+    for (DevBuf* d : {&P, &lab, &fin, &aux, &mask, &seed, &perFrame, &regions, &fz, &order, &multi, &fb, &counters, &flag,
+                      &rf, &sbits, &hist, &rois, &rows, &staged, &frameMap, &fitNext, &fitHand})
+        d->release();
+}
+
+size_t Engine::chunk_frames(uint64_t planes, size_t npx) const
+{
+    return std::max<size_t>(1, std::min<size_t>(planes, std::min<size_t>(65535, opt.chunkPixels / std::max<size_t>(npx, 1))));
+}
+
+size_t Engine::workspace_bytes(uint64_t planes, int H, int W, uint64_t maxPerPlane, uint64_t outCapRows) const
+{
+    if (planes == 0 || H <= 0 || W <= 0) return 0;
+    const size_t npx = (size_t)W * H;
+    const size_t F = chunk_frames(planes, npx);
+    const size_t SF = std::max<size_t>(1, std::min<size_t>(F, std::max<size_t>(64, (size_t(1) << 24) / npx)));
+    size_t t = LaneSizes(F, npx).bytes(F, npx);                                   // first main lane
+    if (planes > F && opt.lanes > 1) t += LaneSizes(F, npx).bytes(F, npx);       // second main lane
+    t += LaneSizes(SF, npx).bytes(SF, npx) + SF * 4;                             // side lane + frame map
+    const uint64_t roiCap = std::min<uint64_t>(outCapRows, planes * std::min<uint64_t>(maxPerPlane, npx / 4 + 2));
+    // ROI store, side ROI store (grown by half again), fit rows, staged rows (grown by half again),
+    // hand-over list, plane bases and placement plan, pinned-sized small buffers.
+    t += roiCap * sizeof(Roi) * 5 / 2 + roiCap * kCols * 8 * 5 / 2 + roiCap * 4 + (planes + 1) * 4 + planes * 28 + 4096;
+    return t;
+}
+
+size_t Engine::device_bytes() const
+{
+    size_t t = 0;
+    for (const Lane* L : {&main_, &mainB_, &side_}) t += L->device_bytes();
+    for (const DevBuf* b : {&in, &out, &planeInfo, &atMask_, &atPart_, &atRows_, &passes_, &allRois_, &allBase_, &sideRois_,
+                            &sideBase_, &run_, &splineRaw_, &splineCv_})
+        t += b->bytes;
+    return t;
+}
+
+void Engine::release_device_buffers()
+{
+    for (Lane* L : {&main_, &mainB_, &side_}) {
+        if (L->stream) PT_CHECK(hipStreamSynchronize(L->stream));
+        L->release_buffers();
+    }
+    for (DevBuf* b : {&in, &out, &planeInfo, &atMask_, &atPart_, &atRows_, &passes_, &allRois_, &allBase_, &sideRois_,
+                      &sideBase_, &run_, &splineRaw_, &splineCv_})
+        b->release();
 }
 
 FitScratch Lane::fit_scratch(size_t nRoi)
@@ -413,7 +497,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     if (planes == 0 || maxPerPlane == 0 || H <= 0 || W <= 0) return 0;
     if (planes >= 0xFFFFFFFFull) throw Error("too many planes");
     const size_t npx = (size_t)W * H;
-    const size_t F = std::max<size_t>(1, std::min<size_t>(planes, std::min<size_t>(65535, opt.chunkPixels / npx)));
+    const size_t F = chunk_frames(planes, npx);
     main_.prepare(F, npx);
     if (planes > F && opt.lanes > 1) mainB_.prepare(F, npx);
     const FitParams fp = fit_params(fit, fitParams);

@@ -4,10 +4,15 @@
 // The reference processes one stack per call (lib/Palm.cpp:55-70); a caller with several cameras
 // serialises their chunks. Here each slot is an independent Engine (its own streams and chunk
 // buffers) driven by one host thread, so requests on different slots run on the device at the same
-// time. A request leaves the queue, in arrival order, as soon as a slot is free and the bytes of the
-// requests in flight (their input frames and output rows, resident in HBM) plus its own stay within
-// the budget. Each request is computed exactly as a Localization call over the same frames: the
-// slots share no device state, so a result does not depend on what else is in flight.
+// time. A request leaves the queue, in arrival order, as soon as a slot is free and the HBM it needs
+// fits the budget beside what is already committed: the input frames and output rows of the requests
+// in flight, and every slot's engine work space (chunk buffers, ROI stores, rows), which a slot keeps
+// between requests and which grows with the largest request it has run. A running slot counts the
+// larger of what it holds and the request's estimated work space (Engine::workspace_bytes, from the
+// sizes the engine allocates); an idle slot counts what it holds, and idle slots release their buffers
+// when the head request needs the room. Each request is computed exactly as a Localization call over
+// the same frames: the slots share no device state, so a result does not depend on what else is in
+// flight.
 #include "engine.h"
 #include "../../include/palmtracer.h"
 
@@ -34,7 +39,8 @@ struct Request {
     std::vector<double> fitParams;
     double* d_out = nullptr;
     uint64_t outCapDoubles = 0, maxPerPlane = 0;
-    uint64_t bytes = 0;              // HBM held by the request while queued or in flight
+    uint64_t io = 0;                 // input frames and output rows (caller's HBM)
+    uint64_t ws = 0;                 // estimated engine work space of the request
     Clock::time_point tSubmit;
 };
 
@@ -55,7 +61,21 @@ public:
             ctx_.back()->opt.profile = false;
             if (chunkPixels) ctx_.back()->opt.chunkPixels = chunkPixels;
         }
+        held_.assign(slots, 0);
+        reserved_.assign(slots, 0);
+        busy_.assign(slots, false);
         for (int i = 0; i < slots; ++i) threads_.emplace_back([this, i] { worker(i); });
+    }
+
+    // HBM admission counts for a request: its input and output plus its estimated work space.
+    uint64_t request_bytes(uint64_t planes, int H, int W, uint64_t outCapDoubles, uint64_t maxPerPlane) const
+    {
+        return planes * (uint64_t)H * (uint64_t)W * 2 + outCapDoubles * 8 +
+               ctx_[0]->workspace_bytes(planes, H, W, maxPerPlane, outCapDoubles / kCols);
+    }
+    uint64_t workspace(const Request& r) const
+    {
+        return ctx_[0]->workspace_bytes(r.planes, r.H, r.W, r.maxPerPlane, r.outCapDoubles / kCols);
     }
 
     ~Server()
@@ -70,7 +90,10 @@ public:
 
     int64_t submit(Request r)
     {
-        if (r.bytes > budget_) throw Error("request larger than the server's HBM budget");
+        r.ws = workspace(r);
+        if (r.io + r.ws > budget_)
+            throw Error("request needs " + std::to_string(r.io + r.ws) + " bytes of HBM (frames, rows and work space), above "
+                        "the server's budget of " + std::to_string(budget_));
         std::lock_guard<std::mutex> lk(mu_);
         if (stop_) throw Error("server is shutting down");
         r.id = nextId_++;
@@ -99,11 +122,36 @@ public:
 private:
     double ms(Clock::time_point t) const { return std::chrono::duration<double, std::milli>(t - t0_).count(); }
 
-    // Admission: FIFO; the head request goes to a free slot when the budget allows it (or nothing
-    // else is in flight).
-    bool admissible() const
+    // HBM committed by the slots other than s: running slots their reservation, idle slots what they hold.
+    uint64_t committed_except(int s) const
     {
-        return !queue_.empty() && (inFlight_ == 0 || inFlight_ + queue_.front().bytes <= budget_);
+        uint64_t t = 0;
+        for (size_t k = 0; k < held_.size(); ++k)
+            if ((int)k != s) t += busy_[k] ? reserved_[k] : held_[k];
+        return t;
+    }
+
+    // Admission (FIFO, under mu_): the head request goes to the free slot s when the requests in
+    // flight, the other slots' work space and its own fit the budget; idle slots release their
+    // buffers (and then s its own) when that makes room. The submission check guarantees that a
+    // request fits once nothing else holds HBM.
+    bool try_admit(int s)
+    {
+        if (queue_.empty()) return false;
+        const Request& r = queue_.front();
+        auto need = [&] { return inFlightIO_ + r.io + committed_except(s) + std::max(held_[s], r.ws); };
+        if (need() <= budget_) return true;
+        for (size_t k = 0; k < held_.size(); ++k) {
+            if ((int)k == s || busy_[k] || held_[k] == 0) continue;
+            ctx_[k]->release_device_buffers();   // that slot's thread is idle and waits for mu_
+            held_[k] = 0;
+            if (need() <= budget_) return true;
+        }
+        if (held_[s]) {
+            ctx_[s]->release_device_buffers();
+            held_[s] = 0;
+        }
+        return need() <= budget_;
     }
 
     void worker(int slot)
@@ -113,11 +161,13 @@ private:
             Request r;
             {
                 std::unique_lock<std::mutex> lk(mu_);
-                cvWork_.wait(lk, [&] { return admissible() || (stop_ && queue_.empty()); });
+                cvWork_.wait(lk, [&] { return try_admit(slot) || (stop_ && queue_.empty()); });
                 if (queue_.empty()) return;
                 r = std::move(queue_.front());
                 queue_.pop_front();
-                inFlight_ += r.bytes;
+                inFlightIO_ += r.io;
+                busy_[slot] = true;
+                reserved_[slot] = std::max(held_[slot], r.ws);
             }
             cvWork_.notify_one();   // another slot may take the next request
             const Clock::time_point tStart = Clock::now();
@@ -132,9 +182,13 @@ private:
                 ok = false;
             }
             const Clock::time_point tDone = Clock::now();
+            const uint64_t holds = e.device_bytes();
             {
                 std::lock_guard<std::mutex> lk(mu_);
-                inFlight_ -= r.bytes;
+                inFlightIO_ -= r.io;
+                busy_[slot] = false;
+                held_[slot] = holds;
+                reserved_[slot] = 0;
                 Result& res = results_[r.id];
                 res.done = true;
                 res.ok = ok;
@@ -154,7 +208,9 @@ private:
     std::unordered_map<int64_t, Result> results_;
     std::vector<std::unique_ptr<Engine>> ctx_;
     std::vector<std::thread> threads_;
-    uint64_t budget_, inFlight_ = 0;
+    uint64_t budget_, inFlightIO_ = 0;
+    std::vector<uint64_t> held_, reserved_;   // per slot: HBM its engine holds; reservation while running
+    std::vector<bool> busy_;
     int64_t nextId_ = 0;
     bool stop_ = false;
     Clock::time_point t0_;
@@ -205,7 +261,7 @@ PT_API int64_t PT_CALL pt_server_submit(void* server, const uint16_t* d_input, u
         r.d_out = d_output;
         r.outCapDoubles = outputCapDoubles;
         r.maxPerPlane = maxPerPlane;
-        r.bytes = planes * height * width * 2 + outputCapDoubles * 8;
+        r.io = planes * height * width * 2 + outputCapDoubles * 8;
         return static_cast<Server*>(server)->submit(std::move(r));
     } catch (const std::exception& e) {
         std::fprintf(stderr, "[palmtracer] pt_server_submit: %s\n", e.what());
@@ -229,6 +285,19 @@ PT_API int PT_CALL pt_server_wait(void* server, int64_t request, uint64_t* doubl
         return ok ? 1 : 0;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "[palmtracer] pt_server_wait: %s\n", e.what());
+        return 0;
+    }
+}
+
+PT_API uint64_t PT_CALL pt_server_request_bytes(void* server, uint64_t planes, uint64_t height, uint64_t width,
+                                                uint64_t outputCapDoubles, uint64_t maxPerPlane)
+{
+    try {
+        if (!server) throw pt::Error("no server");
+        if (height == 0 || width == 0 || height > 0x7FFFFFFF || width > 0x7FFFFFFF) throw pt::Error("bad frame size");
+        return static_cast<Server*>(server)->request_bytes(planes, (int)height, (int)width, outputCapDoubles, maxPerPlane);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[palmtracer] pt_server_request_bytes: %s\n", e.what());
         return 0;
     }
 }

@@ -71,6 +71,8 @@ def lib():
         L.pt_server_submit.argtypes = [_vp, _vp, _u64, _u64, _u64, _f64, _f64, _u64, _vp, _u64, _vp, _u64, _u64]
         L.pt_server_wait.restype = _i32
         L.pt_server_wait.argtypes = [_vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)]
+        L.pt_server_request_bytes.restype = _u64
+        L.pt_server_request_bytes.argtypes = [_vp, _u64, _u64, _u64, _u64, _u64]
         L.pt_server_now.restype = _f64
         L.pt_server_now.argtypes = [_vp]
         L.pt_server_destroy.restype = None
@@ -283,6 +285,11 @@ class Server:
         if not lib().pt_server_wait(self._h, int(rid), ctypes.byref(n), t):
             raise RuntimeError(f"request {rid} failed")
         return n.value // NCOL, (t[0], t[1], t[2], int(t[3]))
+
+    def request_bytes(self, planes, height, width, out_doubles, max_per_plane):
+        """HBM the admission counts for such a request (frames, rows and engine work space)."""
+        return lib().pt_server_request_bytes(self._h, int(planes), int(height), int(width), int(out_doubles),
+                                             int(max_per_plane))
 
     def now(self):
         return lib().pt_server_now(self._h)

@@ -60,22 +60,77 @@ def test_server_results_equal_alone(pt_gpu, cameras, fp64, ratio, fit):
 
 
 def test_server_budget_admits_one_at_a_time(pt_gpu, cameras):
-    """With a budget below two requests' bytes, no two requests are in flight together; a request
-    above the budget is refused."""
+    """With a budget below two requests' HBM (frames, rows and engine work space), no two requests are
+    in flight together; a request above the budget is refused."""
     import torch
     d, thr, mpp = cameras[1]
     cap = d.shape[0] * mpp * 18
-    req_bytes = d.numel() * 2 + cap * 8
+    with pt_gpu.Server(slots=1, budget_bytes=1 << 40, chunk_pixels=1 << 22) as probe:
+        req_bytes = probe.request_bytes(d.shape[0], d.shape[1], d.shape[2], cap, mpp)
+    assert req_bytes > d.numel() * 2 + cap * 8
     with pt_gpu.Server(slots=3, budget_bytes=req_bytes + req_bytes // 2, chunk_pixels=1 << 22) as srv:
+        assert srv.request_bytes(d.shape[0], d.shape[1], d.shape[2], cap, mpp) == req_bytes
         outs = [torch.zeros(cap, dtype=torch.float64, device="cuda") for _ in range(4)]
         ids = [srv.submit(d, o, thr, 0.0, 2, FIT_PARAMS, mpp) for o in outs]
         spans = [srv.wait(r)[1] for r in ids]
         spans.sort(key=lambda t: t[1])
         for a, b in zip(spans, spans[1:]):
             assert b[1] >= a[2], (a, b)
-        big = torch.zeros(cap * 20, dtype=torch.float64, device="cuda")
-        with pytest.raises(RuntimeError):
-            srv.submit(d, big, thr, 0.0, 2, FIT_PARAMS, mpp)
+        # A request whose output alone exceeds the budget is refused at submission (its buffer is
+        # never touched, so the C entry is given a capacity larger than the buffer behind it).
+        params = np.asarray(FIT_PARAMS, np.float64)
+        over = (req_bytes + req_bytes // 2) // 8 + 1
+        assert pt_gpu.lib().pt_server_submit(srv._h, d.data_ptr(), d.shape[0], d.shape[1], d.shape[2], thr, 0.0, 2,
+                                             params.ctypes.data, 4, outs[0].data_ptr(), over, mpp) == -1
         first = outs[0].cpu().numpy()
         for o in outs[1:]:
             assert np.array_equal(o.cpu().numpy(), first)
+
+
+def test_server_budget_at_the_limit(pt_gpu, cameras):
+    """A budget of exactly one request's HBM runs requests of two sizes one at a time (idle slots give
+    their work space back when the other size needs the room); one byte less refuses the request."""
+    import torch
+    (d1, thr1, mpp1), (d2, thr2, mpp2) = cameras[1], cameras[2]
+    caps = [d1.shape[0] * mpp1 * 18, d2.shape[0] * mpp2 * 18]
+    with pt_gpu.Server(slots=1, budget_bytes=1 << 40, chunk_pixels=1 << 22) as probe:
+        rb = [probe.request_bytes(d.shape[0], d.shape[1], d.shape[2], c, m)
+              for (d, _, m), c in zip([cameras[1], cameras[2]], caps)]
+    budget = max(rb)
+    exp = [pt_gpu.localization(d.cpu().numpy(), t, 0.0, 2, FIT_PARAMS, output_size=c)
+           for (d, t, _), c in zip([cameras[1], cameras[2]], caps)]
+    with pt_gpu.Server(slots=2, budget_bytes=budget, chunk_pixels=1 << 22) as srv:
+        jobs = []
+        for k in range(6):
+            i = k % 2
+            d, thr, mpp = cameras[1 + i]
+            o = torch.zeros(caps[i], dtype=torch.float64, device="cuda")
+            jobs.append((i, o, srv.submit(d, o, thr, 0.0, 2, FIT_PARAMS, mpp)))
+        spans = []
+        for i, o, rid in jobs:
+            n, t = srv.wait(rid)
+            spans.append(t)
+            assert np.array_equal(o[: n * 18].view(-1, 18).cpu().numpy(), exp[i])
+        spans.sort(key=lambda t: t[1])
+        for a, b in zip(spans, spans[1:]):
+            assert b[1] >= a[2], (a, b)
+    with pt_gpu.Server(slots=1, budget_bytes=rb[0] - 1, chunk_pixels=1 << 22) as srv:
+        d, thr, mpp = cameras[1]
+        o = torch.zeros(caps[0], dtype=torch.float64, device="cuda")
+        with pytest.raises(RuntimeError):
+            srv.submit(d, o, thr, 0.0, 2, FIT_PARAMS, mpp)
+
+
+def test_server_work_space_estimate_bounds_allocation(pt_gpu, cameras):
+    """The device memory a slot allocates for a request stays within the admission's estimate."""
+    import torch
+    d, thr, mpp = cameras[1]
+    cap = d.shape[0] * mpp * 18
+    o = torch.zeros(cap, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    with pt_gpu.Server(slots=1, budget_bytes=1 << 40, chunk_pixels=1 << 22) as srv:
+        ws = srv.request_bytes(d.shape[0], d.shape[1], d.shape[2], cap, mpp) - d.numel() * 2 - cap * 8
+        free0 = torch.cuda.mem_get_info()[0]
+        srv.wait(srv.submit(d, o, thr, 0.0, 2, FIT_PARAMS, mpp))
+        used = free0 - torch.cuda.mem_get_info()[0]
+    assert used <= ws, (used, ws)
