@@ -162,7 +162,9 @@ PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint6
  * restatement of the reference), "fit_waves" (resident waves of the
  * persistent FP32 fit, 0: default), "fit_refill" (free lanes of an FP32 fit wave that start a refill
  * round, 1-64, default 16), "detect_rows" (rows per detection wave, 0: chosen from the chunk),
- * "lanes" (2: consecutive chunks alternate between two streams; 1: one stream). Returns 1 on success. */
+ * "lanes" (2: consecutive chunks alternate between two streams; 1: one stream), "batch_autotune" (1: tune
+ * new fit classes of pt_batch_localization), "batch_graph" (1: its fit stage as a hipGraph). Returns 1 on
+ * success. */
 PT_API int PT_CALL pt_set_option(const char* name, double value);
 
 /* Counters of the last call: frames, chunks, rows, frames flagged for the exact frame flood,
@@ -181,6 +183,32 @@ PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n);
  * with the main stream), and the LM passes (model + Jacobian, trial) of the FP64 group kernel.
  * Writes min(n, 10) values, returns 10. */
 PT_API int PT_CALL pt_stage_times(double* out, uint64_t n);
+
+/* Batched localization of many stacks (no counterpart in the reference, whose callers run Localization
+ * once per stack). Each stack: device input (planes x height x width u16), threshold, watershed ratio,
+ * fit mode 0-4 with its fit_params (host, [roi, sigmaX, sigmaY, theta]), device output of
+ * outputCapDoubles doubles with at most maxPerPlane rows per plane; doublesWritten is set by the call.
+ * Every stack is detected and segmented on its own; the fit jobs of all stacks are fitted by one
+ * persistent launch per fit class (equal fit parameters), the classes side by side in one captured
+ * hipGraph, with schedules (resident waves, refill threshold, FP32 iterations before the FP64
+ * hand-over) picked by an autotuner the first time a class is seen ("batch_autotune"). A stack's rows
+ * equal pt_localization_device on that stack with the fast fit ("fit_fp64" 0) and the same
+ * hand-over iterations (pt_batch_schedules). Returns 1 on success, 0 on failure. */
+typedef struct {
+    const uint16_t* d_input;
+    uint64_t planes, height, width;
+    double thresholdVal, watershedRatio;
+    uint64_t fit;
+    const double* fit_params;
+    double* d_output;
+    uint64_t outputCapDoubles, maxPerPlane;
+    uint64_t doublesWritten;
+} pt_batch_stack;
+PT_API int PT_CALL pt_batch_localization(pt_batch_stack* stacks, uint64_t nStacks);
+/* The fit classes of the last pt_batch_localization: per class 8 doubles [mode, roi, jobs, waves,
+ * refill, hand-over iterations, autotuner ms of the chosen schedule, 1 if tuned]. Writes min(n, 8 *
+ * classes) values, returns the number of classes. */
+PT_API int PT_CALL pt_batch_schedules(double* out, uint64_t n);
 
 /* No counterpart in the reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc). Fitting (as above, same

@@ -6,7 +6,9 @@
 #pragma once
 #include "pt_kernels.h"
 
+#include <map>
 #include <mutex>
+#include <string>
 #include <vector>
 
 namespace pt {
@@ -55,6 +57,8 @@ struct Options {
     int fitHandover = 16;           // fast fit: FP32 iterations before a ROI moves to the FP64 group kernel (0: never)
     int detectRows = 0;             // rows per detection wave (0: chosen from the chunk size)
     int lanes = 2;                  // main lanes the chunks alternate between (1: one stream, for timing)
+    bool batchAutotune = true;      // batched localization: tune each new fit class's schedule
+    bool batchGraph = true;         // batched localization: the fit stage as a captured hipGraph
     // Defaults with the environment's overrides (PALMTRACER_FIT_PRECISION, PALMTRACER_MAX_ITER,
     // PALMTRACER_CHUNK_PIXELS).
     static Options defaults();
@@ -70,6 +74,32 @@ FitParams gaussian_fit_params(uint64_t fit, const double* fp, const Options& opt
 void launch_positions_to_rois(const double* d_pos, uint64_t n, uint64_t planes, Roi* d_rois, uint32_t* d_bad,
                               hipStream_t s);
 void launch_positions_ids(const double* d_pos, const uint32_t* d_local, uint64_t n, double* d_rows, hipStream_t s);
+
+// One stack of a batched localization (the C ABI's pt_batch_stack).
+struct BatchItem {
+    const uint16_t* d_input;
+    uint64_t planes, height, width;
+    double threshold, watershedRatio;
+    uint64_t fit;
+    const double* fit_params;
+    double* d_output;
+    uint64_t outputCapDoubles, maxPerPlane;
+    uint64_t doublesWritten;
+};
+// Schedule of one fit class in a batch: resident waves and refill threshold of the FP32 kernel,
+// iterations before the FP64 hand-over; the autotuner's time for it.
+struct BatchSchedule {
+    unsigned waves = 4096;
+    int refill = 16;
+    int handover = 16;
+    double ms = 0.0;
+    bool tuned = false;
+};
+struct BatchClassInfo {
+    int mode, roi;
+    uint64_t jobs;
+    BatchSchedule sched;
+};
 
 // One stream with its own detection buffers. The engine runs the chunks on the main lane and the
 // frames deferred to the exact frame flood on the side lane, concurrently.
@@ -110,6 +140,10 @@ public:
     // Returns the number of doubles written (a multiple of 18).
     uint64_t localize(const uint16_t* d_in, uint64_t planes, int H, int W, double thr, double ratio, uint64_t fit,
                       const double* fitParams, uint64_t maxPerPlane, double* d_out, uint64_t outCapRows);
+    // Localization of many device stacks in one call: detection per stack, one fit launch per fit
+    // class over the jobs of every stack (batch.hip).
+    void batch_localize(BatchItem* items, uint64_t nItems);
+    const std::vector<BatchClassInfo>& batch_classes() const { return batchLast_; }
     // Fit-only over caller positions (6 columns each), rows in input order.
     uint64_t fit_positions(const uint16_t* d_in, uint64_t planes, int H, int W, const double* hostPos, uint64_t n,
                            uint64_t fit, const double* fitParams, double* d_out);
@@ -150,6 +184,14 @@ private:
     FitParams fit_params(uint64_t fit, const double* fp);
     SplineParams spline_;
     DevBuf splineRaw_, splineCv_;
+    // Batched localization: jobs, rows, stack table, per-plane bases, placement, counts, fit work space;
+    // tuned schedules by class signature; the captured fit stage and the layout it was captured for.
+    DevBuf bRois_, bRows_, bStaged_, bStacks_, bJobs_, bBases_, bPlan_, bCounts_, bHand_, bWords_;
+    std::map<std::string, BatchSchedule> tuned_;
+    hipGraphExec_t batchGraphExec_ = nullptr;
+    std::string batchGraphKey_;
+    hipEvent_t forkEv_[2] = {nullptr, nullptr};
+    std::vector<BatchClassInfo> batchLast_;
 };
 
 } // namespace pt

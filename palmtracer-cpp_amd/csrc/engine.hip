@@ -174,6 +174,7 @@ Engine::Engine()
     for (hipEvent_t& e : sideEv_) PT_CHECK(hipEventCreate(&e));
     for (hipEvent_t& e : chunkEv_) PT_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (hipEvent_t& e : fitEv_) PT_CHECK(hipEventCreate(&e));
+    for (hipEvent_t& e : forkEv_) PT_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     opt = Options::defaults();
 }
 
@@ -189,6 +190,10 @@ Engine::~Engine()
     for (hipEvent_t e : fitEv_) (void)hipEventDestroy(e);
     for (hipEvent_t e : chunkEv_) (void)hipEventDestroy(e);
     for (hipEvent_t e : scanEv_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : forkEv_) (void)hipEventDestroy(e);
+    if (batchGraphExec_) (void)hipGraphExecDestroy(batchGraphExec_);
+    for (DevBuf* b : {&bRois_, &bRows_, &bStaged_, &bStacks_, &bJobs_, &bBases_, &bPlan_, &bCounts_, &bHand_, &bWords_})
+        b->release();
     for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
 }
 
@@ -343,7 +348,8 @@ size_t Engine::device_bytes() const
     size_t t = 0;
     for (const Lane* L : {&main_, &mainB_, &side_}) t += L->device_bytes();
     for (const DevBuf* b : {&in, &out, &planeInfo, &atMask_, &atPart_, &atRows_, &passes_, &allRois_, &allBase_, &sideRois_,
-                            &sideBase_, &run_, &splineRaw_, &splineCv_})
+                            &sideBase_, &run_, &splineRaw_, &splineCv_, &bRois_, &bRows_, &bStaged_, &bStacks_, &bJobs_,
+                            &bBases_, &bPlan_, &bCounts_, &bHand_, &bWords_})
         t += b->bytes;
     return t;
 }
@@ -355,8 +361,12 @@ void Engine::release_device_buffers()
         L->release_buffers();
     }
     for (DevBuf* b : {&in, &out, &planeInfo, &atMask_, &atPart_, &atRows_, &passes_, &allRois_, &allBase_, &sideRois_,
-                      &sideBase_, &run_, &splineRaw_, &splineCv_})
+                      &sideBase_, &run_, &splineRaw_, &splineCv_, &bRois_, &bRows_, &bStaged_, &bStacks_, &bJobs_,
+                      &bBases_, &bPlan_, &bCounts_, &bHand_, &bWords_})
         b->release();
+    if (batchGraphExec_) PT_CHECK(hipGraphExecDestroy(batchGraphExec_));
+    batchGraphExec_ = nullptr;
+    batchGraphKey_.clear();
 }
 
 FitScratch Lane::fit_scratch(size_t nRoi)

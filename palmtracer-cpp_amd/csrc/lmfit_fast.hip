@@ -108,6 +108,7 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
     int state = kFetch, it = 0;
     uint32_t idx = 0, ux = 0, uy = 0;
     const uint16_t* img = nullptr;     // ROI's top-left pixel in the frame (unstaged reads)
+    size_t stride = (size_t)W;         // row stride of the ROI's frame
     double x0 = 0, y0 = 0;             // detection position (bounds, failure rows)
     double p[N], pTry[N];
     double relAcc = 0, relRej = 0;     // max |dP| / max(|p|, 1e-12) with p after acceptance / rejection
@@ -120,7 +121,7 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
 
     auto pix = [&](int k, int i, int j) -> float {
         if constexpr (STAGED) return (float)patch[k * 64 + lane];
-        else return (float)img[(size_t)i * (size_t)W + j];
+        else return (float)img[(size_t)i * stride + j];
     };
     // Visit the ROI's pixels in raster order with the model's FP64 exponential E at each:
     // body(k, i, j, X, Y, E) with X, Y the FP64 offsets from the centre (cx, cy).
@@ -219,24 +220,35 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
             seg = r.seg;
             surf = r.surf;
             const uint64_t sx = to_size_f(r.x), sy = to_size_f(r.y), h64 = (uint64_t)half;
+            const uint16_t* fbase = frames;
+            size_t fnpx = npx;
+            int Wl = W, Hl = H;
+            if (fp.stacks) {
+                const StackRef sr = fp.stacks[r.slot];
+                fbase = sr.base;
+                fnpx = sr.npx;
+                Wl = sr.W;
+                Hl = sr.H;
+            }
             // Border check with the reference's unsigned arithmetic (GaussianFit.cpp:51-56).
             // A half side beyond the frame (where the reference's unsigned bound wraps and it reads out
             // of the image) gets the border row.
-            if (h64 > (uint64_t)W || h64 > (uint64_t)H || h64 > sx || sx >= (uint64_t)W - h64 || h64 > sy ||
-                sy >= (uint64_t)H - h64) {
+            if (h64 > (uint64_t)Wl || h64 > (uint64_t)Hl || h64 > sx || sx >= (uint64_t)Wl - h64 || h64 > sy ||
+                sy >= (uint64_t)Hl - h64) {
                 row_fail(rows + (size_t)idx * kCols, MODE, r.x, r.y, r.seg, r.surf);
                 continue;  // still kFetch
             }
             ux = (uint32_t)sx;
             uy = (uint32_t)sy;
             const size_t fr = frameMap ? frameMap[r.frame] : r.frame;
-            img = frames + fr * npx + (size_t)(uy - half) * W + (ux - half);
+            stride = (size_t)Wl;
+            img = fbase + fr * fnpx + (size_t)(uy - half) * stride + (ux - half);
             float mn = 3.0e38f, mx = -3.0e38f;
             uint32_t s1 = 0;
             uint64_t s2 = 0;
             for (int i = 0, k = 0; i < roi; ++i)
                 for (int j = 0; j < roi; ++j, ++k) {
-                    const uint16_t u = img[(size_t)i * W + j];
+                    const uint16_t u = img[(size_t)i * stride + j];
                     if constexpr (STAGED) patch[k * 64 + lane] = u;
                     const float v = (float)u;
                     mn = v < mn ? v : mn;

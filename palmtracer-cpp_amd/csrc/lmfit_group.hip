@@ -109,6 +109,7 @@ __global__ void __launch_bounds__(64) fit_group_kernel(const uint16_t* __restric
     uint32_t idx = 0;
     uint64_t ux = 0, uy = 0;
     const uint16_t* img = nullptr;
+    size_t stride = (size_t)W;         // row stride of the ROI's frame
     double x0 = 0, y0 = 0, seg = 0, surf = 0;
     double p[N], pTry[N];
     double al[T], be[N];   // alpha (packed upper triangle) and beta of the last pass
@@ -120,7 +121,7 @@ __global__ void __launch_bounds__(64) fit_group_kernel(const uint16_t* __restric
         if (staged) {
             for (int k = sub, m2 = 0; k < nPixI; k += kGroup, ++m2) f(k, (double)patch[m2 * 64 + lane]);
         } else {
-            for (int k = sub; k < nPixI; k += kGroup) f(k, (double)img[(size_t)(k / roi) * (size_t)W + (size_t)(k % roi)]);
+            for (int k = sub; k < nPixI; k += kGroup) f(k, (double)img[(size_t)(k / roi) * stride + (size_t)(k % roi)]);
         }
     };
     // The group's first lane stores the row.
@@ -171,17 +172,28 @@ __global__ void __launch_bounds__(64) fit_group_kernel(const uint16_t* __restric
             ux = to_size(r.x);
             uy = to_size(r.y);
             const uint64_t h64 = (uint64_t)half;
+            const uint16_t* fbase = frames;
+            size_t fnpx = npx;
+            int Wl = W, Hl = H;
+            if (fp.stacks) {
+                const StackRef sr = fp.stacks[r.slot];
+                fbase = sr.base;
+                fnpx = sr.npx;
+                Wl = sr.W;
+                Hl = sr.H;
+            }
             // Border check with the reference's unsigned arithmetic (GaussianFit.cpp:51-56).
-            if (h64 > (uint64_t)W || h64 > (uint64_t)H || h64 > ux || ux >= (uint64_t)W - h64 || h64 > uy ||
-                uy >= (uint64_t)H - h64) {
+            if (h64 > (uint64_t)Wl || h64 > (uint64_t)Hl || h64 > ux || ux >= (uint64_t)Wl - h64 || h64 > uy ||
+                uy >= (uint64_t)Hl - h64) {
                 fail_row(r.x, r.y);
                 continue;   // still gFetch
             }
             const size_t fr = frameMap ? frameMap[r.frame] : r.frame;
-            img = frames + fr * npx + (size_t)(uy - h64) * W + (ux - h64);
+            stride = (size_t)Wl;
+            img = fbase + fr * fnpx + (size_t)(uy - h64) * stride + (ux - h64);
             double mn = 1.79769313486231570815e308, mx = -1.79769313486231570815e308;
             for (int k = sub, m2 = 0; k < nPixI; k += kGroup, ++m2) {
-                const uint16_t u = img[(size_t)(k / roi) * (size_t)W + (size_t)(k % roi)];
+                const uint16_t u = img[(size_t)(k / roi) * stride + (size_t)(k % roi)];
                 if (staged) patch[m2 * 64 + lane] = (float)u;
                 mn = fmin(mn, (double)u);
                 mx = fmax(mx, (double)u);
@@ -296,7 +308,7 @@ __global__ void __launch_bounds__(64) fit_group_kernel(const uint16_t* __restric
                 }
                 if (!valid) continue;
                 const double d = staged ? (double)patch[m2 * 64 + lane]
-                                        : (double)img[(size_t)i * (size_t)W + (size_t)j];
+                                        : (double)img[(size_t)i * stride + (size_t)j];
                 const double res = d - (m.off + m.amp * E);
                 e += res * res;
                 double g[N];

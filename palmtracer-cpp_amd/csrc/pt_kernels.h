@@ -146,6 +146,13 @@ struct SplineParams {
     const double* cv = nullptr;
 };
 
+// One stack of a batched fit: its frames (frame f at base + f * npx, rows of W pixels).
+struct StackRef {
+    const uint16_t* base;
+    uint64_t npx;
+    int W, H;
+};
+
 // Gaussian LM fit of nRoi ROIs (count read from device memory *d_count when non-null).
 struct FitParams {
     int mode;        // 1..4 Gaussian, 5 spline (0: no fit, rows carry the segmentation only)
@@ -157,6 +164,9 @@ struct FitParams {
     // kernel (0: no hand-over, every ROI and every mode stays in FP32).
     int handover = 16;
     const SplineParams* spline = nullptr;  // mode 5 (host struct, read by the launcher)
+    // Batched fits (device table, optional): ROI r reads stack stacks[r.slot], frame r.frame; the
+    // launch's frames / npx / W / H are then unused.
+    const StackRef* stacks = nullptr;
 };
 // Device work space of a fast fit launch: kFitWords queue words ([0] FP32 queue head, [1] ROIs handed
 // to the FP64 group kernel, [2] group queue head) and the hand-over list (one word per ROI).

@@ -26,6 +26,13 @@ SYNTH_2D = dict(bg=20.0, sigma=1.3, median=2000.0, sigma_log=0.5, offset=100.0, 
                 d=400.0, zmin=-600.0, zmax=600.0, scmos=0, sd_offset=3.0, sd_gain=0.1)
 
 
+class BatchStack(ctypes.Structure):
+    """pt_batch_stack of include/palmtracer.h."""
+    _fields_ = [("d_input", _vp), ("planes", _u64), ("height", _u64), ("width", _u64), ("thresholdVal", _f64),
+                ("watershedRatio", _f64), ("fit", _u64), ("fit_params", _vp), ("d_output", _vp),
+                ("outputCapDoubles", _u64), ("maxPerPlane", _u64), ("doublesWritten", _u64)]
+
+
 def lib():
     """Load the shared library once; raise if it was not built."""
     global _lib
@@ -65,6 +72,10 @@ def lib():
         L.pt_astig_estimate_device.argtypes = [_vp, _vp, _u64, _f64, _vp, _f64]
         L.pt_fitting_noalloc.restype = _u64
         L.pt_fitting_noalloc.argtypes = [_vp, _vp, _u64, _vp, _u64, _u64, _u64, _u64, _vp]
+        L.pt_batch_localization.restype = _i32
+        L.pt_batch_localization.argtypes = [ctypes.POINTER(BatchStack), _u64]
+        L.pt_batch_schedules.restype = _i32
+        L.pt_batch_schedules.argtypes = [_vp, _u64]
         L.pt_server_create.restype = _vp
         L.pt_server_create.argtypes = [_u64, _u64, _u64]
         L.pt_server_submit.restype = ctypes.c_int64
@@ -220,6 +231,43 @@ def localization_device(d_stack, d_out, threshold, watershed_ratio=0.0, fit=2, f
     n = lib().pt_localization_device(d_stack.data_ptr(), d_out.data_ptr(), cap, h, w, z, float(threshold),
                                      float(watershed_ratio), int(fit), p.ctypes.data, mpp)
     return n // NCOL
+
+
+def batch_localization(stacks):
+    """pt_batch_localization over a list of dicts {d_stack (planes, h, w) u16 tensor, d_out float64 tensor,
+    threshold, watershed_ratio, fit, fit_params, max_per_plane}: every stack detected on its own, the fit
+    jobs of all stacks fitted per fit class in one launch each. Returns the rows written per stack."""
+    n = len(stacks)
+    arr = (BatchStack * n)()
+    keep = []
+    for i, st in enumerate(stacks):
+        d, o = st["d_stack"], st["d_out"]
+        z, h, w = d.shape
+        p = _params(st.get("fit_params", (7, 1.3, 1.3, 0)))
+        keep.append(p)
+        cap = o.numel()
+        mpp = st.get("max_per_plane") or (cap // NCOL) // max(z, 1)
+        arr[i] = BatchStack(d.data_ptr(), z, h, w, float(st["threshold"]), float(st.get("watershed_ratio", 0.0)),
+                            int(st.get("fit", 2)), p.ctypes.data, o.data_ptr(), cap, int(mpp), 0)
+    if not lib().pt_batch_localization(arr, n):
+        raise RuntimeError("pt_batch_localization failed (see stderr)")
+    return [arr[i].doublesWritten // NCOL for i in range(n)]
+
+
+def batch_schedules():
+    """Fit classes of the last batch: dicts {mode, roi, jobs, waves, refill, handover, ms, tuned}."""
+    buf = (ctypes.c_double * 8 * 64)()
+    k = lib().pt_batch_schedules(ctypes.cast(buf, _vp), 8 * 64)
+    keys = ["mode", "roi", "jobs", "waves", "refill", "handover", "ms", "tuned"]
+    out = []
+    for i in range(min(k, 64)):
+        row = [buf[i][j] for j in range(8)]
+        d = dict(zip(keys, row))
+        for key in ("mode", "roi", "jobs", "waves", "refill", "handover"):
+            d[key] = int(d[key])
+        d["tuned"] = bool(d["tuned"])
+        out.append(d)
+    return out
 
 
 def fitting_noalloc(d_stack, d_positions, d_out, fit=2, fit_params=(7, 1.3, 1.3, 0)):
