@@ -261,6 +261,111 @@ def astig3d_leg(pt, torch, frames, steps):
             "fit_handed_over": handed, "fit_precision_first_20_frames": prec}
 
 
+def batch_leg(pt, torch, frame_scale, steps, per_frame_sample):
+    """Batched uneven workload: 64 seeded stacks of frame sizes drawn from {128, 256, 512, 1024, 2048}^2,
+    frame counts uniform in 50-2000 (times `frame_scale`), emitter density uniform in 0.1-5 per um^2 at
+    100 nm pixels, capped at 5000 emitters per frame. Each stack gets the 7x7 fit of sigma (mode 2) or
+    the 9x9 astigmatic fit of sigma_x / sigma_y (mode 3), assigned so that the two classes hold about
+    equal numbers of emitters. One step = pt_batch_localization over all stacks (per-stack detection,
+    one fit launch per class in a captured graph, schedules from the autotuner). The same stacks are
+    then run through the plain per-stack path (one Localization per stack) and the plain per-frame path
+    (one Localization per frame, on the first `per_frame_sample` frames of every stack)."""
+    import numpy as np
+    rng = np.random.default_rng(0x5EED0004)
+    astig_cfg = dict(astig=1, s0=1.2, c=250.0, d=400.0, zmin=-600.0, zmax=600.0)
+    p2d, p3d = (7.0, 1.3, 1.3, 0.0), (9.0, 1.4, 1.4, 0.0)
+    load = [0.0, 0.0]
+    stacks = []
+    for k in range(64):
+        side = int(rng.choice([128, 256, 512, 1024, 2048]))
+        frames = max(1, int(round(int(rng.integers(50, 2001)) * frame_scale)))
+        density = float(rng.uniform(0.1, 5.0))
+        em = int(min(5000, round(density * side * side * 1e-2)))
+        cls = 0 if load[0] <= load[1] else 1
+        load[cls] += frames * em
+        d = torch.empty((frames, side, side), dtype=torch.uint16, device="cuda")
+        pt.synth_device(d, seed=0x5EED4000 + k, emitters=em, **(astig_cfg if cls else {}))
+        fp = p3d if cls else p2d
+        thr = pt.autothreshold_device(d[0], (fp[0],))
+        mpp = int(em * 1.25) + 64
+        out = torch.empty(frames * mpp * 18, dtype=torch.float64, device="cuda")
+        stacks.append(dict(d_stack=d, d_out=out, threshold=thr, fit=3 if cls else 2, fit_params=fp, max_per_plane=mpp,
+                           side=side, frames=frames, emitters=em, density=density))
+    torch.cuda.synchronize()
+
+    def fitted_of(rows):
+        n = 0
+        for st, r in zip(stacks, rows):
+            n += int((st["d_out"][: r * 18].view(-1, 18)[:, 13] > 0).sum().item())
+        return n
+
+    pt.batch_localization(stacks)   # warm-up: first sight of each class runs the autotuner
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rows = pt.batch_localization(stacks)
+        st_times = pt.stage_times()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    fitted = fitted_of(rows)
+    sched = pt.batch_schedules()
+    batch_fit_ms = st_times["fit_ms"]
+
+    # Plain per-stack path: one Localization per stack (same fast fit, default schedule).
+    plain_rows = []
+    t1 = time.perf_counter()
+    fit_ms_stacks = 0.0
+    for st in stacks:
+        plain_rows.append(pt.localization_device(st["d_stack"], st["d_out"], st["threshold"], 0.0, st["fit"],
+                                                 st["fit_params"], st["max_per_plane"]))
+        fit_ms_stacks += pt.stage_times()["fit_ms"]
+    torch.cuda.synchronize()
+    dt_stack = time.perf_counter() - t1
+    fitted_stack = fitted_of(plain_rows)
+
+    # Plain per-frame path: one Localization per frame, the first frames of every stack.
+    n_frames, fitted_frame = 0, 0
+    sub = []
+    for st in stacks:
+        m = min(per_frame_sample, st["frames"])
+        o = torch.empty(st["max_per_plane"] * 18, dtype=torch.float64, device="cuda")
+        sub.append((st, m, o))
+    for st, m, o in sub:   # warm every frame size once
+        pt.localization_device(st["d_stack"][:1], o, st["threshold"], 0.0, st["fit"], st["fit_params"], st["max_per_plane"])
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    counts = []
+    for st, m, o in sub:
+        for f in range(m):
+            r = pt.localization_device(st["d_stack"][f: f + 1], o, st["threshold"], 0.0, st["fit"], st["fit_params"],
+                                       st["max_per_plane"])
+            counts.append(int((o[: r * 18].view(-1, 18)[:, 13] > 0).sum().item()) if r else 0)
+            n_frames += 1
+    torch.cuda.synchronize()
+    dt_frame = time.perf_counter() - t2
+    # The sync of each count is part of a per-frame caller's loop (rows come back per call).
+    fitted_frame = sum(counts)
+    rate = fitted / dt
+    rate_stack = fitted_stack / dt_stack
+    rate_frame = fitted_frame / dt_frame
+    total_frames = sum(st["frames"] for st in stacks)
+    gpx = sum(st["frames"] * st["side"] ** 2 for st in stacks) / 1e9
+    return {"workload": "64 stacks, frame sides from {128,256,512,1024,2048}, frame counts U[50,2000] x "
+                        f"{frame_scale:g}, 0.1-5 emitters/um^2 at 100 nm px capped 5000/frame, 7x7 mode 2 / 9x9 "
+                        "astigmatic mode 3 (fast fit), seed 0x5EED0004",
+            "frame_scale": frame_scale, "stacks": len(stacks), "frames": total_frames, "gpx": gpx,
+            "emitters_by_class": {"7x7_mode2": load[0], "9x9_mode3": load[1]},
+            "steps": steps, "ms_per_step": dt * 1e3, "rows": int(sum(rows)), "fitted_rows": fitted,
+            "localizations_per_s": rate, "fit_stage_ms": batch_fit_ms,
+            "schedules": sched,
+            "per_stack": {"ms": dt_stack * 1e3, "fitted_rows": fitted_stack, "localizations_per_s": rate_stack,
+                          "fit_stage_ms_sum": fit_ms_stacks},
+            "per_frame": {"frames": n_frames, "frames_per_stack": per_frame_sample, "ms": dt_frame * 1e3,
+                          "fitted_rows": fitted_frame, "localizations_per_s": rate_frame},
+            "speedup_vs_per_frame": rate / rate_frame, "speedup_vs_per_stack": rate / rate_stack,
+            "stack_sides": [st["side"] for st in stacks], "stack_frames": [st["frames"] for st in stacks]}
+
+
 def compare_fits(a, e, sigma_y=False):
     """Fast-fit rows `a` against FP64 rows `e` of the same frames: rows paired within each plane by
     position, largest errors over rows fitted on both sides."""
@@ -310,6 +415,11 @@ def main():
     ap.add_argument("--astig-frames", type=int, default=1000,
                     help="frames of the astigmatic 3D leg (2048^2 sCMOS; 0: skip it)")
     ap.add_argument("--astig-steps", type=int, default=3)
+    ap.add_argument("--batch-frame-scale", type=float, default=0.1,
+                    help="scale of the batched workload's frame counts (U[50,2000] per stack); 0 skips the leg")
+    ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--batch-frame-sample", type=int, default=4,
+                    help="frames per stack timed through the plain per-frame path")
     ap.add_argument("--serve-sizes", default="256,512,1024",
                     help="frame sides of the cameras, assigned in turn (e.g. 256,512,1024,2048)")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
@@ -434,6 +544,10 @@ def main():
 
     astig = astig3d_leg(pt, torch, args.astig_frames, args.astig_steps) if args.astig_frames > 0 else None
 
+    batched = None
+    if args.batch_frame_scale > 0:
+        batched = batch_leg(pt, torch, args.batch_frame_scale, args.batch_steps, args.batch_frame_sample)
+
     cpu = None
     if not args.no_cpu_baseline:
         host = stack[: min(F, 4000)].cpu().numpy()
@@ -467,6 +581,7 @@ def main():
         "roofline_fit": roof_fit,
         "fit_precision": precision,
         "astig3d": astig,
+        "batched": batched,
         "serving": serving,
         "cpu_baseline": cpu,
     }

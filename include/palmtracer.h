@@ -157,8 +157,9 @@ PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint6
  * iterations of the fast fit, 16; 0: no hand-over, every ROI and mode in FP32, errors as stated for
  * that precision), "max_iter" (LM iteration cap, 75),
  * "chunk_pixels" (frames per device chunk x pixels, 0: default 2^28), "profile" (1: stage timing),
- * "force_sequential_watershed" (1: every frame through the exact frame flood), "force_wave_flood"
- * (1: that flood without the rank flood), "force_heap_watershed" (1: that flood with the heap
+ * "force_sequential_watershed" (1: every frame through the exact frame flood), "force_rank_flood"
+ * (1: that flood from the single-workgroup rank flood, without the grid rank flood), "force_wave_flood"
+ * (1: that flood without the rank floods), "force_heap_watershed" (1: that flood with the heap
  * restatement of the reference), "fit_waves" (resident waves of the
  * persistent FP32 fit, 0: default), "fit_refill" (free lanes of an FP32 fit wave that start a refill
  * round, 1-64, default 16), "detect_rows" (rows per detection wave, 0: chosen from the chunk),
@@ -173,7 +174,10 @@ PT_API int PT_CALL pt_set_option(const char* name, double value);
  * their largest queue, frames the rank flood handed over (too many entries, equal keys, not
  * settled), wave-queue overflows, frames of the heap restatement, frames the rank flood settled by
  * solving both orders of an equal-key pair, ROIs of the fast fit fitted by the FP64 group kernel
- * (main lanes). Writes min(n, 18) values, returns 18. */
+ * (main lanes), frames the grid rank flood resolved, of which frames whose flood has equal keys that
+ * were shown not to change a label, and the frames it handed to the single-workgroup floods: not
+ * settled, equal keys that might change a label, other (work space, push order). Writes min(n, 23)
+ * values, returns 23. */
 PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n);
 
 /* With the "profile" option, device milliseconds of the last Localization per stage (a trous,

@@ -34,6 +34,8 @@ struct RunStats {
     // entries / equal keys / not settled), wave-queue overflows, frames for the heap restatement.
     uint64_t rfTooBig = 0, rfTie = 0, rfUnsettled = 0, fbOverflow = 0, heapFrames = 0;
     uint64_t rfTwoOrders = 0;  // frames the rank flood settled by solving both orders of an equal-key pair
+    uint64_t gridFrames = 0, gridTies = 0;  // frames the grid rank flood resolved; of which with equal keys
+    uint64_t gridUnsettled = 0, gridRejected = 0, gridOther = 0;  // frames it handed over, by reason
     uint32_t status = 0;
     // With Options::profile: device time per stage (HIP events on the engine stream, summed over
     // chunks), the device time of the deferred-frame batches on the side stream, and the LM passes
@@ -51,7 +53,8 @@ struct Options {
     size_t chunkPixels = kDefaultChunkPixels;
     bool profile = false;           // stage timing with HIP events + LM pass counters
     bool forceSequential = false;   // every frame through the exact frame flood
-    int exactPath = 0;              // exact frame flood from: 0 rank flood, 1 wave-queue flood, 2 heap
+    int exactPath = 0;              // exact frame flood from: 0 grid rank flood, 1 wave-queue flood, 2 heap,
+                                    // 3 single-workgroup rank flood
     unsigned fitWaves = 0;          // persistent FP32 fit: resident waves (0: 16 per CU)
     int fitRefill = 16;             // persistent FP32 fit: free lanes that trigger a refill round
     int fitHandover = 16;           // fast fit: FP32 iterations before a ROI moves to the FP64 group kernel (0: never)
@@ -106,7 +109,7 @@ struct BatchClassInfo {
 struct Lane {
     hipStream_t stream = nullptr;
     DetectBuffers b{};
-    DevBuf P, lab, fin, aux, mask, seed, perFrame, regions, fz, order, multi, fb, counters, flag, rf, sbits, hist;
+    DevBuf P, lab, fin, aux, mask, seed, perFrame, regions, fz, order, multi, fb, counters, flag, rf, gpool, gmeta, sbits, hist;
     DevBuf rois, rows, staged, frameMap, fitNext, fitHand;
     uint32_t* pinned = nullptr;  // host-visible: frame map, roiBase and counters of an asynchronous batch
     size_t pinnedWords = 0;

@@ -200,7 +200,7 @@ Engine::~Engine()
 void Lane::release()
 {
     for (DevBuf* d : {&P, &lab, &fin, &aux, &mask, &seed, &perFrame, &regions, &fz, &order, &multi, &fb, &counters, &flag,
-                      &rf, &sbits, &hist, &rois, &rows, &staged, &frameMap, &fitNext, &fitHand})
+                      &rf, &gpool, &gmeta, &sbits, &hist, &rois, &rows, &staged, &frameMap, &fitNext, &fitHand})
         d->release();
     if (pinned) (void)hipHostFree(pinned);
     pinned = nullptr;
@@ -213,15 +213,17 @@ namespace {
 // Sizes of a lane's buffers for chunks of F frames of npx pixels (Lane::prepare allocates them;
 // Engine::workspace_bytes adds them up for the serving loop's HBM budget).
 struct LaneSizes {
-    size_t n, regionCap, multiCap, fbSlots, rfSlots;
+    size_t n, regionCap, multiCap, fbSlots, rfSlots, gridWords;
     LaneSizes(size_t F, size_t npx)
     {
         n = F * npx;
         regionCap = npx / 4 + 2;  // roots are pairwise non-adjacent: at most ceil(H/2)*ceil(W/2)
         multiCap = n / 64 + 1024;
         fbSlots = std::clamp<size_t>((size_t(1) << 26) / std::max<size_t>(npx, 1), 4, 64);
-        // Rank-flood slots: up to 128 frames at a time, within 1 GiB of work space.
-        rfSlots = std::clamp<size_t>(std::min<size_t>(F, (size_t(1) << 30) / (rf_slot_words(npx) * 4)), 1, 128);
+        // Single-workgroup rank-flood slots (frames the grid rank flood hands over): up to 128 frames at
+        // a time, within 256 MiB of work space.
+        rfSlots = std::clamp<size_t>(std::min<size_t>(F, (size_t(1) << 28) / (rf_slot_words(npx) * 4)), 1, 128);
+        gridWords = grid_pool_words(F, npx);
     }
     // Device bytes of a prepared lane plus its ROI buffer (DevBuf allocates at least 256 bytes).
     size_t bytes(size_t F, size_t npx) const
@@ -229,8 +231,8 @@ struct LaneSizes {
         auto b = [](size_t v) { return std::max<size_t>(v, 256); };
         return b(n * 4) * 5 + b(n * sizeof(PixAux)) + b((F + 1) * 4 * 9) + b(F * regionCap * sizeof(Region)) +
                b(F * regionCap * sizeof(Fz)) + b(F * regionCap * 4) + b(multiCap * 8) + b(fbSlots * fb_slot_words(npx) * 4) +
-               b(kNumCounters * 4) + b((F + 1) * 12) + b(rfSlots * rf_slot_words(npx) * 4) + b(F * seed_words(npx) * 4) +
-               b(F * kHistBins * 4) + b(F * regionCap * sizeof(Roi));
+               b(kNumCounters * 4) + b((F + 1) * 20) + b(rfSlots * rf_slot_words(npx) * 4) + b(F * seed_words(npx) * 4) +
+               b(F * kHistBins * 4) + b(F * regionCap * sizeof(Roi)) + b(gridWords * 4) + b(F * kGridMetaWords * 4);
     }
 };
 } // namespace
@@ -255,9 +257,11 @@ void Lane::prepare(size_t F, size_t npx)
     const size_t fbSlots = z.fbSlots;
     fb.ensure(fbSlots * fb_slot_words(npx) * 4);
     counters.ensure(kNumCounters * 4);
-    flag.ensure((F + 1) * 12);
+    flag.ensure((F + 1) * 20);
     const size_t rfSlots = z.rfSlots;
     rf.ensure(rfSlots * rf_slot_words(npx) * 4);
+    gpool.ensure(z.gridWords * 4);
+    gmeta.ensure(F * kGridMetaWords * 4);
     sbits.ensure(F * seed_words(npx) * 4);
     hist.ensure(F * kHistBins * 4);
     uint32_t* pf = perFrame.as<uint32_t>();
@@ -285,6 +289,11 @@ void Lane::prepare(size_t F, size_t npx)
     b.flagList = flag.as<uint32_t>();
     b.slowList = flag.as<uint32_t>() + (F + 1);
     b.waveList = flag.as<uint32_t>() + 2 * (F + 1);
+    b.gridList = flag.as<uint32_t>() + 3 * (F + 1);
+    b.gridRetry = flag.as<uint32_t>() + 4 * (F + 1);
+    b.gridMeta = gmeta.as<uint32_t>();
+    b.gridPool = gpool.as<uint32_t>();
+    b.gridPoolWords = z.gridWords;
     b.rfScratch = rf.as<uint32_t>();
     b.rfSlots = rfSlots;
     b.seedBits = sbits.as<uint32_t>();
@@ -310,7 +319,7 @@ size_t Lane::device_bytes() const
 {
     size_t t = 0;
     for (const DevBuf* d : {&P, &lab, &fin, &aux, &mask, &seed, &perFrame, &regions, &fz, &order, &multi, &fb, &counters,
-                            &flag, &rf, &sbits, &hist, &rois, &rows, &staged, &frameMap, &fitNext, &fitHand})
+                            &flag, &rf, &gpool, &gmeta, &sbits, &hist, &rois, &rows, &staged, &frameMap, &fitNext, &fitHand})
         t += d->bytes;
     return t;
 }
@@ -318,7 +327,7 @@ size_t Lane::device_bytes() const
 void Lane::release_buffers()
 {
     for (DevBuf* d : {&P, &lab, &fin, &aux, &mask, &seed, &perFrame, &regions, &fz, &order, &multi, &fb, &counters, &flag,
-                      &rf, &sbits, &hist, &rois, &rows, &staged, &frameMap, &fitNext, &fitHand})
+                      &rf, &gpool, &gmeta, &sbits, &hist, &rois, &rows, &staged, &frameMap, &fitNext, &fitHand})
         d->release();
 }
 
@@ -570,6 +579,11 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         last.fbOverflow += ctr[kFbOverflow];
         last.heapFrames += ctr[kSlowCnt];
         last.rfTwoOrders += ctr[kRfTwoOrders];
+        last.gridFrames += ctr[kGridOk];
+        last.gridTies += ctr[kGridTies];
+        last.gridUnsettled += ctr[kGridUnsettled];
+        last.gridRejected += ctr[kGridRejected];
+        last.gridOther += ctr[kGridOther];
         last.floodMaxQueue = std::max<uint64_t>(last.floodMaxQueue, ctr[kFloodMaxQueue]);
         if (ctr[kStatus]) throw Error("segmentation status " + std::to_string(ctr[kStatus]));
         if (opt.profile) {
@@ -623,6 +637,20 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         last.multiComponents += ctr[kMultiCnt];
         last.flagOverflow += ctr[kFlagOverflow];
         last.flagUnproven += ctr[kFlagUnproven];
+        // Exact-flood counters: non-zero only for a chunk that resolved its flagged frames itself.
+        last.floodPops += ctr[kFloodPops];
+        last.rfTooBig += ctr[kRfTooBig];
+        last.rfTie += ctr[kRfTie];
+        last.rfUnsettled += ctr[kRfUnsettled];
+        last.fbOverflow += ctr[kFbOverflow];
+        last.heapFrames += ctr[kSlowCnt];
+        last.rfTwoOrders += ctr[kRfTwoOrders];
+        last.gridFrames += ctr[kGridOk];
+        last.gridTies += ctr[kGridTies];
+        last.gridUnsettled += ctr[kGridUnsettled];
+        last.gridRejected += ctr[kGridRejected];
+        last.gridOther += ctr[kGridOther];
+        last.floodMaxQueue = std::max<uint64_t>(last.floodMaxQueue, ctr[kFloodMaxQueue]);
         last.status |= ctr[kStatus];
         if (ctr[kStatus] & 4u) throw Error("output capacity exceeded");
         if (ctr[kStatus]) throw Error("segmentation status " + std::to_string(ctr[kStatus]));
@@ -825,6 +853,11 @@ void Engine::debug_labels(const uint16_t* d_frame, int H, int W, double thr, dou
     last.fbOverflow = ctr[kFbOverflow];
     last.heapFrames = ctr[kSlowCnt];
     last.rfTwoOrders = ctr[kRfTwoOrders];
+    last.gridFrames = ctr[kGridOk];
+    last.gridTies = ctr[kGridTies];
+    last.gridUnsettled = ctr[kGridUnsettled];
+    last.gridRejected = ctr[kGridRejected];
+    last.gridOther = ctr[kGridOther];
     last.status = ctr[kStatus];
 }
 

@@ -213,6 +213,7 @@ PT_API int PT_CALL pt_set_option(const char* name, double value)
         else if (n == "force_sequential_watershed") e.opt.forceSequential = value != 0.0;
         else if (n == "force_heap_watershed") e.opt.exactPath = value != 0.0 ? 2 : 0;
         else if (n == "force_wave_flood") e.opt.exactPath = value != 0.0 ? 1 : 0;
+        else if (n == "force_rank_flood") e.opt.exactPath = value != 0.0 ? 3 : 0;
         else if (n == "fit_waves") e.opt.fitWaves = (unsigned)std::max(0.0, std::min(1048576.0, value));
         else if (n == "fit_refill") e.opt.fitRefill = (int)std::max(1.0, std::min(64.0, value));
         else if (n == "fit_handover") e.opt.fitHandover = (int)std::max(0.0, std::min(1e6, value));
@@ -228,13 +229,14 @@ PT_API int PT_CALL pt_set_option(const char* name, double value)
 PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n)
 {
     Engine& e = Engine::get();
-    const uint64_t v[18] = {e.last.frames, e.last.chunks, e.last.rois, e.last.flaggedFrames, e.last.multiComponents,
+    const uint64_t v[23] = {e.last.frames, e.last.chunks, e.last.rois, e.last.flaggedFrames, e.last.multiComponents,
                             e.last.deferredFrames, e.last.status, e.last.flagOverflow, e.last.flagUnproven,
                             e.last.floodPops, e.last.floodMaxQueue, e.last.rfTooBig, e.last.rfTie,
                             e.last.rfUnsettled, e.last.fbOverflow, e.last.heapFrames, e.last.rfTwoOrders,
-                            e.last.fitHanded};
-    for (uint64_t i = 0; i < n && i < 18; ++i) out[i] = v[i];
-    return 18;
+                            e.last.fitHanded, e.last.gridFrames, e.last.gridTies, e.last.gridUnsettled,
+                            e.last.gridRejected, e.last.gridOther};
+    for (uint64_t i = 0; i < n && i < 23; ++i) out[i] = v[i];
+    return 23;
 }
 
 PT_API int PT_CALL pt_stage_times(double* out, uint64_t n)

@@ -68,6 +68,11 @@ struct DetectBuffers {
     uint32_t* slowList = nullptr; // [F] frames for the heap restatement (equal keys met)
     uint32_t* waveList = nullptr; // [F] frames the rank flood could not settle (wave-queue flood)
     uint32_t* rfScratch = nullptr;// rank-flood work space: rfSlots * rf_slot_words(npx) u32
+    uint32_t* gridList = nullptr; // [F] frames the grid rank flood hands to the single-workgroup floods
+    uint32_t* gridRetry = nullptr;// [F] frames left to the grid rank flood's second round
+    uint32_t* gridMeta = nullptr; // [F * kGridMetaWords] grid rank flood state per flagged frame
+    uint32_t* gridPool = nullptr; // grid rank flood work space (gridPoolWords u32)
+    size_t gridPoolWords = 0;
     uint32_t* seedBits = nullptr; // per frame: bitmap of the local maxima, seed_words(npx) u32
     uint32_t* hist = nullptr;     // per frame: kHistBins counts of mask pixels at or above each level
     uint32_t* runTotal = nullptr; // optional: ROIs of the call so far (shared by the lanes of a call)
@@ -91,7 +96,14 @@ enum Counter : int {
     kRfUnsettled = 14,   // rank flood: frames whose iteration did not settle (or broke T < G)
     kFbOverflow = 15,    // wave-queue flood: frames whose queue overflowed
     kRfTwoOrders = 16,   // rank flood: frames settled by solving their equal-key pairs in both orders
-    kNumCounters = 20
+    kGridCnt = 17,       // frames handed from the grid rank flood to the single-workgroup floods
+    kGridOk = 18,        // frames resolved by the grid rank flood
+    kGridTies = 19,      // of which frames whose solution has equal keys (proven not to matter)
+    kGridUnsettled = 20, // grid rank flood: frames handed over unsettled after kGridMaxIter steps
+    kGridRejected = 21,  // grid rank flood: frames handed over on equal keys that might matter
+    kGridOther = 22,     // grid rank flood: frames handed over for lack of work space or T >= G
+    kGridRetryCnt = 23,  // grid rank flood: frames left to its second round (work space)
+    kNumCounters = 24
 };
 // kStatus bits: 1 = region slots exhausted, 2 = unlabelled mask pixel (internal error).
 
@@ -110,6 +122,9 @@ __host__ __device__ inline size_t seed_words(size_t npx) { return (npx + 31) / 3
 // the work space beyond. Work space per slot: pixel -> entry map, then 25 words per entry. The stride
 // is a power of two, so the padded sort of any admitted frame fits.
 constexpr uint32_t kRankMax = 262144;
+// Grid rank flood: fixed-point steps, state words per flagged frame.
+constexpr int kGridMaxIter = 8;
+constexpr size_t kGridMetaWords = 16;
 constexpr uint32_t kRankLds = 16384;
 __host__ __device__ inline size_t rf_stride(size_t npx)
 {
@@ -134,6 +149,10 @@ int64_t threshold_to_pmin(double thr);
 // pop, the libstdc++-order heap restatement; exactPath = 1 starts at the wave-queue flood, 2 at the
 // heap restatement. With resolve = false, frames whose component floods could not be proven are
 // left flagged (frameFlag / flagList) without rows, for a later pass over them with forceSequential.
+// Grid rank flood of the flagged frames (fz: second watershed); frames it cannot prove go to
+// b.gridList (count in counters[kGridCnt]). grid_pool_words: pool size for chunks of F frames.
+void launch_grid_flood(const DetectBuffers& b, int W, int H, int F, const Fz* fz, hipStream_t s);
+size_t grid_pool_words(size_t F, size_t npx);
 void launch_segment(const DetectBuffers& b, int W, int H, int frames, uint64_t maxPoints, double ratio, Roi* d_rois,
                     hipStream_t s, bool forceSequential = false, int exactPath = 0, bool resolve = true);
 

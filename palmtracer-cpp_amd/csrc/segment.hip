@@ -24,8 +24,7 @@
 // Statistics are exact integer sums of P = 65536 * W1, so they equal the reference's FP64
 // raster-order sums whenever those are exact (sums below 2^53 ulps); larger sums are recomputed in
 // raster order in FP64.
-#include "pt_common.h"
-#include "pt_kernels.h"
+#include "flood_common.h"
 
 namespace pt {
 namespace {
@@ -37,28 +36,6 @@ constexpr int NT = 256;
 __device__ inline uint32_t ld_relaxed(const uint32_t* p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Local maximum of getLocalMax (Segmentation2D.cpp:294-333: P > 0 and no 8-neighbour inside the
-// frame strictly greater), from the bitmap the detection kernel wrote.
-__device__ inline bool seed_bit(const uint32_t* sbf, uint32_t idx) { return (sbf[idx >> 5] >> (idx & 31)) & 1u; }
-
-// A pixel that the watershed would seed: labelled with some 8-neighbour (inside the frame) that is
-// unlabelled (Segmentation2D.cpp:340-359). Unlabelled = below threshold or label 0.
-__device__ inline bool has_unlabelled_nbr(const uint32_t* Pf, const uint32_t* finf, int W, int H, int idx)
-{
-    const int y = idx / W, x = idx - y * W;
-    for (int dy = -1; dy <= 1; ++dy) {
-        const int yy = y + dy;
-        if (yy < 0 || yy >= H) continue;
-        for (int dx = -1; dx <= 1; ++dx) {
-            const int xx = x + dx;
-            if ((dx | dy) == 0 || xx < 0 || xx >= W) continue;
-            const int q = yy * W + xx;
-            if (Pf[q] == 0 || finf[q] == 0) return true;
-        }
-    }
-    return false;
 }
 
 __device__ inline uint32_t uf_find(const uint32_t* lab, uint32_t x)
@@ -119,14 +96,6 @@ struct DHeap {
         return top;
     }
 };
-
-// Reference key of a push at pop count `it`: -output + it * EPSILON, rounded per operation
-// (no fused multiply-add), Segmentation2D.cpp:378.
-__device__ inline double age_key(uint32_t p, uint32_t it)
-{
-    const double o = (double)p * kPScale;
-    return it == 0 ? -o : __dadd_rn(-o, __dmul_rn((double)it, kEpsilon));
-}
 
 __device__ inline void flag_frame(uint32_t* frameFlag, uint32_t* counters, uint32_t* flagList, int f)
 {
@@ -692,14 +661,6 @@ __device__ inline void wave_sync()
     __asm__ __volatile__("" ::: "memory");
 }
 
-// Neighbour j (0..7) of a pixel in the reference's scan order (rows dy, then columns dx).
-__device__ inline void nb_offset(int j, int& dy, int& dx)
-{
-    const int k = j < 4 ? j : j + 1;  // skip the centre
-    dy = k / 3 - 1;
-    dx = k % 3 - 1;
-}
-
 __device__ inline void waveq_put(WaveQ& q, uint32_t pos, uint32_t pix, uint32_t lbl, uint32_t p, uint32_t age)
 {
     q.pa[pos] = ((uint64_t)p << 32) | age;
@@ -969,7 +930,8 @@ constexpr uint32_t RF_PAIRS = 6;              // equal-key pairs per frame (2^6 
 __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ seedBits, uint32_t* __restrict__ fin,
                                                            const uint32_t* __restrict__ maskList,
                                                            const uint32_t* __restrict__ maskCnt,
-                                                           const uint32_t* __restrict__ flagList, uint32_t* __restrict__ counters,
+                                                           const uint32_t* __restrict__ flagList, int countSlot,
+                                                           uint32_t* __restrict__ counters,
                                                            uint32_t* __restrict__ waveList, uint32_t* __restrict__ rfScratch,
                                                            int W, int H, size_t npx, const Fz* __restrict__ fz,
                                                            const PixAux* __restrict__ aux, size_t regionCap)
@@ -1000,7 +962,7 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
         if (tid == 0) sFrame = atomicAdd(&counters[kRfNext], 1u);
         __syncthreads();
         const uint32_t fi = sFrame;
-        if (fi >= counters[kFlagCnt]) return;
+        if (fi >= counters[countSlot]) return;
         const uint32_t f = flagList[fi];
         const uint32_t* Pf = P + (size_t)f * npx;
         uint32_t* fn = fin + (size_t)f * npx;
@@ -1010,18 +972,7 @@ __global__ void __launch_bounds__(RF_NT) rank_flood_kernel(const uint32_t* __res
         // 1. Labels back to the seed state; seed marks in idxOf for now.
         for (uint32_t k = tid; k < cnt; k += RF_NT) {
             const uint32_t p = ml[k];
-            bool seed;
-            if (!fz) {
-                seed = seed_bit(seedBits + (size_t)f * seed_words(npx), p);
-            } else {
-                const uint32_t L = fn[p];
-                seed = false;
-                if (L != 0 && L <= npx) {
-                    const uint32_t sl = aux[(size_t)f * npx + (L - 1)].rslot;
-                    seed = sl < regionCap && fz[(size_t)f * regionCap + sl].alive &&
-                           0xFFFFFFFFu - (uint32_t)fz[(size_t)f * regionCap + sl].maxKey == p;
-                }
-            }
+            const bool seed = flood_seed(seedBits, fn, fz, aux, regionCap, npx, f, p);
             if (!seed) fn[p] = 0u;
             idxOf[p] = seed ? 1u : 0u;
         }
@@ -1414,18 +1365,7 @@ __global__ void __launch_bounds__(64) fallback_fast_kernel(const uint32_t* __res
         // 1. Labels back to the seed state (seeds marked labelled); potentials by rank.
         for (uint32_t k = lane; k < cnt; k += 64) {
             const uint32_t p = ml[k];
-            bool seed;
-            if (!fz) {
-                seed = seed_bit(seedBits + (size_t)f * seed_words(npx), p);
-            } else {
-                const uint32_t L = fn[p];
-                seed = false;
-                if (L != 0 && L <= npx) {
-                    const uint32_t sl = aux[(size_t)f * npx + (L - 1)].rslot;
-                    seed = sl < regionCap && fz[(size_t)f * regionCap + sl].alive &&
-                           0xFFFFFFFFu - (uint32_t)fz[(size_t)f * regionCap + sl].maxKey == p;
-                }
-            }
+            const bool seed = flood_seed(seedBits, fn, fz, aux, regionCap, npx, f, p);
             if (!seed) fn[p] = 0u;
             if (inLds) {
                 const uint32_t r = fr.rank(p);
@@ -1519,18 +1459,7 @@ __global__ void __launch_bounds__(NT) fallback_kernel(const uint32_t* __restrict
         const uint32_t cnt = maskCnt[f];
         for (uint32_t k = threadIdx.x; k < cnt; k += NT) {
             const uint32_t p = maskList[(size_t)f * npx + k];
-            bool seed;
-            if (!fz) {
-                seed = seed_bit(seedBits + (size_t)f * seed_words(npx), p);
-            } else {
-                const uint32_t L = fn[p];
-                seed = false;
-                if (L != 0 && L <= npx) {
-                    const uint32_t sl = aux[(size_t)f * npx + (L - 1)].rslot;
-                    seed = sl < regionCap && fz[(size_t)f * regionCap + sl].alive &&
-                           0xFFFFFFFFu - (uint32_t)fz[(size_t)f * regionCap + sl].maxKey == p;
-                }
-            }
+            const bool seed = flood_seed(seedBits, fn, fz, aux, regionCap, npx, f, p);
             if (!seed) fn[p] = 0u;
         }
         __syncthreads();
@@ -1767,6 +1696,8 @@ __global__ void reset_pass_kernel(uint32_t* __restrict__ counters)
         counters[kFbNext] = 0u;
         counters[kRfNext] = 0u;
         counters[kWaveCnt] = 0u;
+        counters[kGridCnt] = 0u;
+        counters[kGridRetryCnt] = 0u;
     }
 }
 
@@ -1999,11 +1930,16 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
                                          b.counters, b.frameFlag, b.flagList, W, H, npx, pass2 ? b.fz : nullptr,
                                          b.rootCnt, b.regionCap, b.hist);
         if (!resolve) return;
-        // Exact frame flood of the flagged frames: rank flood, then the wave-queue flood for the frames
-        // it hands over, then the heap restatement for frames with equal keys at a pop.
-        const bool rank = exactPath == 0;
+        // Exact frame flood of the flagged frames: the grid rank flood, then the single-workgroup rank
+        // flood for the frames it hands over, the wave-queue flood for the frames that one hands over,
+        // and the heap restatement for frames with equal keys at a pop. exactPath 3 starts at the
+        // single-workgroup rank flood.
+        const bool rank = exactPath == 0 || exactPath == 3;
+        if (exactPath == 0) launch_grid_flood(b, W, H, F, pass2 ? b.fz : nullptr, s);
         if (rank)
-            rank_flood_kernel<<<(unsigned)b.rfSlots, RF_NT, 0, s>>>(b.P, b.seedBits, b.fin, b.maskList, b.maskCnt, b.flagList, b.counters,
+            rank_flood_kernel<<<(unsigned)b.rfSlots, RF_NT, 0, s>>>(b.P, b.seedBits, b.fin, b.maskList, b.maskCnt,
+                                                                   exactPath == 0 ? b.gridList : b.flagList,
+                                                                   exactPath == 0 ? kGridCnt : kFlagCnt, b.counters,
                                                                    b.waveList, b.rfScratch, W, H, npx,
                                                                    pass2 ? b.fz : nullptr, b.aux, b.regionCap);
         // Few frames reach the wave-queue flood when the rank flood runs first: a small persistent grid

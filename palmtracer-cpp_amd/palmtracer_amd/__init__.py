@@ -177,12 +177,12 @@ def set_option(name, value):
 
 
 def last_stats():
-    v = (ctypes.c_uint64 * 18)()
-    lib().pt_last_stats(v, 18)
+    v = (ctypes.c_uint64 * 23)()
+    lib().pt_last_stats(v, 23)
     keys = ["frames", "chunks", "rows", "flagged_frames", "multi_components", "deferred_frames", "status",
             "flag_overflow", "flag_unproven", "wave_flood_pops", "wave_flood_max_queue", "rank_too_big",
             "rank_equal_keys", "rank_unsettled", "wave_queue_overflow", "heap_frames", "rank_two_orders",
-            "fit_handed_over"]
+            "fit_handed_over", "grid_frames", "grid_equal_keys", "grid_unsettled", "grid_rejected", "grid_other"]
     return dict(zip(keys, list(v)))
 
 

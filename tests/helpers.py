@@ -116,6 +116,14 @@ def rank_labels(labels):
     return lut[labels]
 
 
+def filter_small(labels):
+    """FilterData's surface filter (Segmentation2D.cpp:88-141) on an identity-labelled map."""
+    u, c = np.unique(labels[labels > 0], return_counts=True)
+    out = labels.copy()
+    out[np.isin(out, u[c <= 4])] = 0
+    return rank_labels(out)
+
+
 def spline_params(roi=7, size=(11, 11, 21), delta_z=50.0, sigma=1.3, seed=0, noise=1e-3):
     """fit_params of the spline fit (fit mode 5): [roi, sx, sy, sz, deltaZ, coefficients], the
     coefficients of a separable Gaussian PSF interpolated per axis by cubic Hermite polynomials

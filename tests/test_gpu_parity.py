@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 from helpers import (AMP, CIRC, FIT_TOL, ID, INDEX, INTENSITY, OFFSET, PLANE, QXY, SEG, SURF, SX, SY, X, Y, fit_errors,
-                     fit_mismatches, match_rows, rank_labels, theta_to_degrees)
+                     filter_small, fit_mismatches, match_rows, rank_labels, theta_to_degrees)
 
 pytestmark = pytest.mark.gpu
 FIT_PARAMS_CSV = (7.0, 1.0, 2.0, 0.0)
@@ -253,12 +253,7 @@ def test_synthetic_stack_end_to_end(pt_gpu, oracle_mod):
 
 
 # ------------------------------------------------------------------ basin fusion (ratio >= 1) ----
-def _filter_small(labels):
-    """FilterData's surface filter (Segmentation2D.cpp:88-141) on an identity-labelled map."""
-    u, c = np.unique(labels[labels > 0], return_counts=True)
-    out = labels.copy()
-    out[np.isin(out, u[c <= 4])] = 0
-    return rank_labels(out)
+_filter_small = filter_small
 
 
 @pytest.mark.parametrize("name", ["stack", "small", "dense"])
@@ -314,15 +309,17 @@ def frames512(pt_gpu, oracle_mod):
     return st, oracle_mod.autothreshold(st[0])
 
 
-@pytest.mark.parametrize("sequential", [0, 1, 2, 3])
+@pytest.mark.parametrize("sequential", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("ratio", [0.0, 10.0])
 def test_labels_512(pt_gpu, oracle_mod, frames512, sequential, ratio):
-    """0: component floods; every frame through the exact frame flood: 1 rank flood (then the
-    wave-queue flood for frames it hands over), 2 wave-queue flood, 3 heap restatement."""
+    """0: component floods; every frame through the exact frame flood: 1 grid rank flood (then the
+    single-workgroup floods for frames it hands over), 2 wave-queue flood, 3 heap restatement,
+    4 single-workgroup rank flood."""
     st, thr = frames512
     pt_gpu.set_option("force_sequential_watershed", sequential > 0)
     pt_gpu.set_option("force_wave_flood", sequential == 2)
     pt_gpu.set_option("force_heap_watershed", sequential == 3)
+    pt_gpu.set_option("force_rank_flood", sequential == 4)
     try:
         for f in range(st.shape[0]):
             w1 = oracle_mod.atrous(st[f])[1]
@@ -340,6 +337,7 @@ def test_labels_512(pt_gpu, oracle_mod, frames512, sequential, ratio):
         pt_gpu.set_option("force_sequential_watershed", 0)
         pt_gpu.set_option("force_heap_watershed", 0)
         pt_gpu.set_option("force_wave_flood", 0)
+        pt_gpu.set_option("force_rank_flood", 0)
 
 
 def test_localization_512_no_fit(pt_gpu, oracle_mod, frames512):
@@ -412,13 +410,15 @@ def test_fp32_fit_queue_widths(pt_gpu, frames512, waves):
 
 
 @pytest.mark.parametrize("name", ["stack", "small", "dense"])
-@pytest.mark.parametrize("exact", [1, 2, 3])
+@pytest.mark.parametrize("exact", [1, 2, 3, 4])
 def test_exact_floods_bit_exact(pt_gpu, oracle_mod, ref_inputs, ref_vectors, name, exact):
-    """Every frame through each exact frame flood (rank, wave queue, heap): labels equal the oracle's."""
+    """Every frame through each exact frame flood (grid rank flood, wave queue, heap, single-workgroup
+    rank flood): labels equal the oracle's."""
     st, thr = _stacks(ref_inputs, ref_vectors)[name]
     pt_gpu.set_option("force_sequential_watershed", 1)
     pt_gpu.set_option("force_wave_flood", exact == 2)
     pt_gpu.set_option("force_heap_watershed", exact == 3)
+    pt_gpu.set_option("force_rank_flood", exact == 4)
     try:
         for f in range(st.shape[0]):
             w1 = oracle_mod.atrous(st[f])[1]
@@ -429,6 +429,7 @@ def test_exact_floods_bit_exact(pt_gpu, oracle_mod, ref_inputs, ref_vectors, nam
         pt_gpu.set_option("force_sequential_watershed", 0)
         pt_gpu.set_option("force_heap_watershed", 0)
         pt_gpu.set_option("force_wave_flood", 0)
+        pt_gpu.set_option("force_rank_flood", 0)
 
 
 def test_localization_with_all_frames_exact(pt_gpu, oracle_mod, frames512):
@@ -457,20 +458,27 @@ def test_localization_with_all_frames_exact(pt_gpu, oracle_mod, frames512):
 TIE_FRAMES = [(101, 5.0, 1), (102, 4.0, 1), (104, 4.0, 1), (101, 4.0, 2), (103, 4.0, 3)]
 
 
+@pytest.mark.parametrize("grid", [1, 0])
 @pytest.mark.parametrize("seed,thr,pairs", TIE_FRAMES)
-def test_equal_keys_in_the_flood(pt_gpu, oracle_mod, seed, thr, pairs):
+def test_equal_keys_in_the_flood(pt_gpu, oracle_mod, seed, thr, pairs, grid):
+    """The grid rank flood settles a frame whose equal keys cannot change a label and hands the
+    others on; the single-workgroup rank flood (grid 0) solves independent pairs in both orders."""
     import synth_np
     frame = synth_np.frames(1, 256, 256, 60, seed=seed)[0]
     w1 = oracle_mod.atrous(frame)[1]
     exp = oracle_mod.watershed_labels(w1, thr)
     pt_gpu.set_option("force_sequential_watershed", 1)
+    pt_gpu.set_option("force_rank_flood", 1 - grid)
     try:
         got = pt_gpu.debug_labels(frame, thr, 0.0)
         stats = pt_gpu.last_stats()
     finally:
         pt_gpu.set_option("force_sequential_watershed", 0)
+        pt_gpu.set_option("force_rank_flood", 0)
     assert np.array_equal(rank_labels(got), exp)
-    if pairs == 1:
+    if grid and stats["grid_frames"] == 1:
+        assert stats["grid_equal_keys"] == 1, stats
+    elif pairs == 1:
         assert stats["rank_two_orders"] == 1, stats
     else:
         assert stats["rank_two_orders"] == 1 or (stats["rank_equal_keys"] == 1 and stats["heap_frames"] == 1), stats
