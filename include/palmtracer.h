@@ -164,7 +164,9 @@ PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint6
  * persistent FP32 fit, 0: default), "fit_refill" (free lanes of an FP32 fit wave that start a refill
  * round, 1-64, default 16), "detect_rows" (rows per detection wave, 0: chosen from the chunk),
  * "lanes" (2: consecutive chunks alternate between two streams; 1: one stream), "batch_autotune" (1: tune
- * new fit classes of pt_batch_localization), "batch_graph" (1: its fit stage as a hipGraph). Returns 1 on
+ * new fit classes of pt_batch_localization), "batch_graph" (1: its fit stage as a hipGraph), "defer_flagged"
+ * (1: flagged frames of all but the last chunk are re-detected and resolved on the side stream;
+ * 0, the default: every chunk resolves its own). Returns 1 on
  * success. */
 PT_API int PT_CALL pt_set_option(const char* name, double value);
 

@@ -62,6 +62,7 @@ struct Options {
     int lanes = 2;                  // main lanes the chunks alternate between (1: one stream, for timing)
     bool batchAutotune = true;      // batched localization: tune each new fit class's schedule
     bool batchGraph = true;         // batched localization: the fit stage as a captured hipGraph
+    bool deferFlagged = false;      // flagged frames of all but the last chunk: exact flood on the side lane
     // Defaults with the environment's overrides (PALMTRACER_FIT_PRECISION, PALMTRACER_MAX_ITER,
     // PALMTRACER_CHUNK_PIXELS).
     static Options defaults();

@@ -656,7 +656,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         if (ctr[kStatus]) throw Error("segmentation status " + std::to_string(ctr[kStatus]));
         totalRois = m[kNumCounters];
         const uint32_t* fl = m + kNumCounters + 1;
-        if (chunkZ0[c] + F < planes)    // not the last chunk
+        if (opt.deferFlagged && chunkZ0[c] + F < planes)    // deferred (not the last chunk)
             for (uint32_t k = 0; k < std::min<uint32_t>(ctr[kFlagCnt], (uint32_t)F); ++k)
                 deferred.push_back((uint32_t)(chunkZ0[c] + fl[k]));
     };
@@ -677,10 +677,11 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         L.b.runTotal = dRun;
         L.b.scanWait = scanEv_[(chunkNo + 1) & 1];
         L.b.scanDone = scanEv_[chunkNo & 1];
-        // The last chunk resolves its flagged frames itself (exact frame flood in line): a side batch
-        // started after it would only end after the fit of the call.
+        // Each chunk resolves its flagged frames itself (exact frame flood in line), unless they are
+        // deferred to the side lane; the last chunk always does (a side batch started after it would
+        // only end after the fit of the call).
         const bool lastChunk = z0 + F >= planes;
-        detect_chunk(L, d_in + z0 * npx, f, H, W, thr, ratio, maxPerPlane, 3 * chunkNo, nullptr, lastChunk);
+        detect_chunk(L, d_in + z0 * npx, f, H, W, thr, ratio, maxPerPlane, 3 * chunkNo, nullptr, lastChunk || !opt.deferFlagged);
         L.b.runTotal = nullptr;
         L.b.scanWait = L.b.scanDone = nullptr;
         gather_rois(L.rois.as<Roi>(), (uint32_t)z0, allRois_.as<Roi>(), L.b.roiBase, (uint32_t)f, L.b.chunkBase,

@@ -14,31 +14,38 @@
 //
 // Every phase is a kernel over all entries of all flagged frames (grid-stride over frames in y and
 // entries in x), so a 2048^2 frame runs on dozens of workgroups:
-//   setup     per-frame work space from one pool (per-entry arrays, 52 bytes per mask pixel);
+//   setup     per-frame work space from one pool (per-entry arrays, 84 bytes per mask pixel);
 //   reset     labels back to the seeds;           collect  entries with their sort keys;
 //   sort      2048-key tiles in LDS, then merge passes (each element finds its place by one
 //             binary search in the sibling run);
 //   finalize  entries in potential order and their windows; the label map holds each entry's
-//             index (tagged) until the write, the seeds' labels are kept per entry; then the
+//             index (tagged) until the write, the seeds' labels are kept per entry; then a table of
+//             each entry's neighbour entries (read coalesced by the later steps) and the
 //             fixed-point steps;
 //   checks    T < G, equal keys (below), labels by walking the first-popped-neighbour links;
 //   write     labels of the frames that passed; the others get their seed labels back and go to
 //             the single-workgroup floods.
 // Frames that do not fit the pool are given a second round once the first round's frames are done.
 //
-// Equal keys. The heap pops equal keys in an order set by its internal layout. Keys that are equal
-// in the solution form groups of consecutive ranks; a choice of order inside the groups permutes
-// the groups' ranks. The frame is accepted when no choice can change a label:
-//   (1) an entry whose first-popped neighbour lies in a group of k > 1 entries may see its push age
-//       move by up to k - 1 (delta); such an entry is not itself in a group;
-//   (2) no two keys of a window can cross: |K(x) - K(y)| > (delta(x) + delta(y)) * 1e-7 (plus
-//       rounding) for keys that differ;
-//   (3) every entry is pushed before its pop under any choice: T + delta < G, and a group member
-//       is pushed before the group's first rank;
-//   (4) an entry whose first-popped neighbour lies in a group has the same label through each of
-//       its neighbours in that group.
-// Then every choice gives the same ranks outside the groups and the same labels (induction over the
-// pops), the heap's among them.
+// Equal keys. The heap pops equal keys in an order set by its internal layout; the solution above
+// breaks them by entry order. A frame with equal keys is accepted when no order can change a label,
+// shown with intervals that hold for every order:
+//   start   an entry in a group of k equal keys has a rank in [first, first + k - 1] of the group,
+//           every other entry its rank;
+//   step    push ages in [min over neighbours of the lowest rank, min of the highest rank], keys
+//           from those (the key grows with the age), ranks in [1 + entries whose highest key is
+//           below the lowest key, n - entries whose lowest key is above the highest key];
+//   repeat  until the intervals stop growing (at most kGridIvalIter steps, else the frame is
+//           handed over);
+//   accept  every non-seed is pushed before its earliest pop (highest push age < lowest rank), and
+//           every neighbour that can be the first popped one (lowest rank <= highest push age)
+//           carries the label of the solution.
+// By induction over the pops of the flood under any order of the equal keys: the entry popped at
+// step m has m inside its interval and every entry whose highest rank is below m has been popped
+// (an entry with a smaller certain key that is not present at m would need an unpushed entry with
+// a rank interval starting above m, and following certainly-smaller keys down ends at a present
+// one); so the first popped neighbour of every entry is one of the accepted candidates, and the
+// labels are those of the solution.
 #include "flood_common.h"
 
 #include <algorithm>
@@ -62,24 +69,29 @@ struct GfMeta {
     uint32_t tie;      // the solution has equal keys
     uint32_t offLo, offHi;  // pool offset (words)
     uint32_t changed[kGridMaxIter];
+    uint32_t ichanged[kGridIvalIter];   // interval steps that widened some interval
 };
 static_assert(sizeof(GfMeta) <= kGridMetaWords * 4, "grid flood meta");
 
-__host__ __device__ inline size_t gf_even(size_t v) { return (v + 1) & ~size_t(1); }
-// Pool words of a frame of cap mask pixels: 2 u64 arrays and 9 u32 arrays of cap entries.
-constexpr size_t kGfWordsPerEntry = 13;
-__host__ __device__ inline size_t gf_words(size_t cap) { return kGfWordsPerEntry * gf_even(cap); }
+// Entry capacity rounded to 4, so that every array (and each entry's 16-byte neighbour rows) stays
+// 16-byte aligned.
+__host__ __device__ inline size_t gf_round(size_t v) { return (v + 3) & ~size_t(3); }
+// Pool words of a frame of cap mask pixels: 2 u64 arrays, 9 u32 arrays and the 8-word neighbour
+// table of cap entries.
+constexpr size_t kGfWordsPerEntry = 21;
+__host__ __device__ inline size_t gf_words(size_t cap) { return kGfWordsPerEntry * gf_round(cap); }
 
 struct GfView {
     uint64_t* skA;     // sort keys (ping); after the sort: group size / first rank per entry
     uint64_t* skB;     // sort keys (pong)
     double* K;         // keys (the pong buffer, once sorted)
     uint32_t *sp, *spix, *lo, *hi, *Ga, *Gb, *T, *par, *lbl;
+    uint32_t* nbr;     // [entry][8] neighbour entries (none: GF_NONE)
 };
 
 __device__ inline GfView gf_view(uint32_t* pool, const GfMeta& m)
 {
-    const size_t off = ((size_t)m.offHi << 32) | m.offLo, c = gf_even(m.cap);
+    const size_t off = ((size_t)m.offHi << 32) | m.offLo, c = gf_round(m.cap);
     GfView v;
     v.skA = reinterpret_cast<uint64_t*>(pool + off);
     v.skB = v.skA + c;
@@ -94,6 +106,7 @@ __device__ inline GfView gf_view(uint32_t* pool, const GfMeta& m)
     v.T = w + 6 * c;
     v.par = w + 7 * c;
     v.lbl = w + 8 * c;
+    v.nbr = w + 9 * c;
     return v;
 }
 
@@ -352,6 +365,38 @@ __global__ void __launch_bounds__(GF_NT) gf_finalize_kernel(GfArgs a)
     }
 }
 
+// Neighbour table (entries among the 8 neighbours of each entry's pixel).
+__global__ void __launch_bounds__(GF_NT) gf_neighbours_kernel(GfArgs a)
+{
+    const uint32_t cnt = gf_count(a);
+    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+        const GfMeta m = a.meta[k];
+        if (m.fail || m.n == 0) continue;
+        const GfView v = gf_view(a.pool, m);
+        const uint32_t* Pf = a.P + (size_t)a.flagList[k] * a.npx;
+        const uint32_t* fn = a.fin + (size_t)a.flagList[k] * a.npx;
+        for (uint32_t i = blockIdx.x * GF_NT + threadIdx.x; i < m.n; i += gridDim.x * GF_NT) {
+            const uint32_t p = v.spix[i] & ~GF_SEED;
+            const int y = (int)(p / (uint32_t)a.W), x = (int)(p % (uint32_t)a.W);
+            uint32_t e[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) e[j] = gf_nbr(Pf, fn, a.W, a.H, y, x, j);
+            uint4* dst = reinterpret_cast<uint4*>(v.nbr + (size_t)i * 8);
+            dst[0] = make_uint4(e[0], e[1], e[2], e[3]);
+            dst[1] = make_uint4(e[4], e[5], e[6], e[7]);
+        }
+    }
+}
+
+// Neighbour entries of entry i from the table.
+__device__ inline void gf_nbrs(const GfView& v, uint32_t i, uint32_t e[8])
+{
+    const uint4* src = reinterpret_cast<const uint4*>(v.nbr + (size_t)i * 8);
+    const uint4 a = src[0], b = src[1];
+    e[0] = a.x; e[1] = a.y; e[2] = a.z; e[3] = a.w;
+    e[4] = b.x; e[5] = b.y; e[6] = b.z; e[7] = b.w;
+}
+
 // Frames still iterating at step `it` (the previous step changed some rank).
 __device__ inline bool gf_active(const GfMeta& m, int it) { return !m.fail && m.n > 0 && (it == 0 || m.changed[it - 1]); }
 
@@ -364,19 +409,15 @@ __global__ void __launch_bounds__(GF_NT) gf_age_kernel(GfArgs a, int it)
         if (!gf_active(m, it)) continue;
         const GfView v = gf_view(a.pool, m);
         const uint32_t* G = (it & 1) ? v.Gb : v.Ga;
-        const uint32_t* Pf = a.P + (size_t)a.flagList[k] * a.npx;
-        const uint32_t* fn = a.fin + (size_t)a.flagList[k] * a.npx;
         for (uint32_t i = blockIdx.x * GF_NT + threadIdx.x; i < m.n; i += gridDim.x * GF_NT) {
             uint32_t t = 0;
-            const uint32_t sx = v.spix[i];
-            if (!(sx & GF_SEED)) {
+            if (!(v.spix[i] & GF_SEED)) {
                 t = GF_NONE;
-                const int y = (int)(sx / (uint32_t)a.W), x = (int)(sx % (uint32_t)a.W);
+                uint32_t e[8];
+                gf_nbrs(v, i, e);
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const uint32_t e = gf_nbr(Pf, fn, a.W, a.H, y, x, j);
-                    if (e != GF_NONE) t = min(t, G[e]);
-                }
+                for (int j = 0; j < 8; ++j)
+                    if (e[j] != GF_NONE) t = min(t, G[e[j]]);
             }
             v.T[i] = t;
             v.K[i] = t == GF_NONE ? 1.79769313486231570815e308 : age_key(v.sp[i], t);
@@ -418,11 +459,6 @@ __global__ void __launch_bounds__(GF_NT) gf_rank_kernel(GfArgs a, int it)
 // Group of equal keys of entry i: size and first rank, packed in skA.
 __device__ inline uint32_t gf_gsize(const GfView& v, uint32_t i) { return (uint32_t)v.skA[i]; }
 __device__ inline uint32_t gf_gfirst(const GfView& v, uint32_t i) { return (uint32_t)(v.skA[i] >> 32); }
-__device__ inline uint32_t gf_delta(const GfView& v, uint32_t i)
-{
-    return (v.spix[i] & GF_SEED) ? 0u : gf_gsize(v, v.par[i]) - 1u;
-}
-
 // Checks 1: the iteration settled, T < G, first-popped neighbours, groups of equal keys.
 __global__ void __launch_bounds__(GF_NT) gf_check1_kernel(GfArgs a)
 {
@@ -436,8 +472,6 @@ __global__ void __launch_bounds__(GF_NT) gf_check1_kernel(GfArgs a)
         }
         const GfView v = gf_view(a.pool, m);
         const uint32_t* G = v.Ga;     // settled: both rank buffers hold the solution
-        const uint32_t* Pf = a.P + (size_t)a.flagList[k] * a.npx;
-        const uint32_t* fn = a.fin + (size_t)a.flagList[k] * a.npx;
         const uint32_t stride = gridDim.x * GF_NT;
         for (uint32_t b = blockIdx.x * GF_NT + (threadIdx.x & ~63u); b < m.n; b += stride) {
             const uint32_t i = b + (threadIdx.x & 63);
@@ -451,12 +485,11 @@ __global__ void __launch_bounds__(GF_NT) gf_check1_kernel(GfArgs a)
                     if (!(v.T[i] < g)) bad = kGfOrder;
                     uint32_t bg = GF_NONE;
                     pa = GF_NONE;
-                    const int y = (int)(sx / (uint32_t)a.W), x = (int)(sx % (uint32_t)a.W);
+                    uint32_t e[8];
+                    gf_nbrs(v, i, e);
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        const uint32_t e = gf_nbr(Pf, fn, a.W, a.H, y, x, j);
-                        if (e != GF_NONE && G[e] < bg) { bg = G[e]; pa = e; }
-                    }
+                    for (int j = 0; j < 8; ++j)
+                        if (e[j] != GF_NONE && G[e[j]] < bg) { bg = G[e[j]]; pa = e[j]; }
                     if (pa == GF_NONE) bad |= kGfOrder;
                 }
                 v.par[i] = pa;
@@ -476,8 +509,8 @@ __global__ void __launch_bounds__(GF_NT) gf_check1_kernel(GfArgs a)
     }
 }
 
-// Checks 2: conditions (1) and (3) on the equal keys; labels through the first-popped-neighbour
-// links (each link goes to a smaller rank, so every walk ends at a seed).
+// Checks 2: labels through the first-popped-neighbour links (each link goes to a smaller rank, so
+// every walk ends at a seed).
 __global__ void __launch_bounds__(GF_NT) gf_check2_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
@@ -495,12 +528,6 @@ __global__ void __launch_bounds__(GF_NT) gf_check2_kernel(GfArgs a)
                 if (sx & GF_SEED) {
                     L = v.lbl[i];
                 } else {
-                    if (m.tie) {
-                        const uint32_t d = gf_delta(v, i), gs = gf_gsize(v, i);
-                        if (gs > 1 && d > 0) bad |= kGfTie;
-                        if (!((uint64_t)v.T[i] + d < (uint64_t)v.Ga[i])) bad |= kGfTie;
-                        if (gs > 1 && !(v.T[i] < gf_gfirst(v, i))) bad |= kGfTie;
-                    }
                     uint32_t e = v.par[i];
                     for (uint32_t steps = 0; steps < m.n && !(v.spix[e] & GF_SEED); ++steps) e = v.par[e];
                     L = (v.spix[e] & GF_SEED) ? v.lbl[e] : 0u;
@@ -513,43 +540,119 @@ __global__ void __launch_bounds__(GF_NT) gf_check2_kernel(GfArgs a)
     }
 }
 
-// Checks 3 (frames with equal keys): conditions (2) and (4).
-__global__ void __launch_bounds__(GF_NT) gf_check3_kernel(GfArgs a)
+// Intervals (frames with equal keys). Rank intervals in Gb (lowest) and T (highest), key intervals
+// in K (lowest) and skA (highest, as doubles).
+__device__ inline double* gf_khi(const GfView& v) { return reinterpret_cast<double*>(v.skA); }
+
+__global__ void __launch_bounds__(GF_NT) gf_ival_init_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
     for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
         const GfMeta m = a.meta[k];
         if (m.fail || m.n == 0 || !m.tie) continue;
         const GfView v = gf_view(a.pool, m);
-        const uint32_t* Pf = a.P + (size_t)a.flagList[k] * a.npx;
-        const uint32_t* fn = a.fin + (size_t)a.flagList[k] * a.npx;
+        for (uint32_t i = blockIdx.x * GF_NT + threadIdx.x; i < m.n; i += gridDim.x * GF_NT) {
+            const uint32_t first = gf_gfirst(v, i), size = gf_gsize(v, i);
+            v.Gb[i] = first;
+            v.T[i] = first + size - 1;
+        }
+    }
+}
+
+__device__ inline bool gf_ival_active(const GfMeta& m, int it)
+{
+    return !m.fail && m.n > 0 && m.tie && (it == 0 || m.ichanged[it - 1]);
+}
+
+// Interval step, part 1: push-age and key intervals from the neighbours' rank intervals.
+__global__ void __launch_bounds__(GF_NT) gf_ival_key_kernel(GfArgs a, int it)
+{
+    const uint32_t cnt = gf_count(a);
+    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+        const GfMeta m = a.meta[k];
+        if (!gf_ival_active(m, it)) continue;
+        const GfView v = gf_view(a.pool, m);
+        double* khi = gf_khi(v);
+        for (uint32_t i = blockIdx.x * GF_NT + threadIdx.x; i < m.n; i += gridDim.x * GF_NT) {
+            uint32_t tlo = 0, thi = 0;
+            if (!(v.spix[i] & GF_SEED)) {
+                tlo = thi = GF_NONE;
+                uint32_t e[8];
+                gf_nbrs(v, i, e);
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (e[j] != GF_NONE) {
+                        tlo = min(tlo, v.Gb[e[j]]);
+                        thi = min(thi, v.T[e[j]]);
+                    }
+            }
+            v.K[i] = tlo == GF_NONE ? 1.79769313486231570815e308 : age_key(v.sp[i], tlo);
+            khi[i] = thi == GF_NONE ? 1.79769313486231570815e308 : age_key(v.sp[i], thi);
+        }
+    }
+}
+
+// Interval step, part 2: rank intervals from the key intervals of the window (only ever widened).
+__global__ void __launch_bounds__(GF_NT) gf_ival_rank_kernel(GfArgs a, int it)
+{
+    const uint32_t cnt = gf_count(a);
+    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+        const GfMeta m = a.meta[k];
+        if (!gf_ival_active(m, it)) continue;
+        const GfView v = gf_view(a.pool, m);
+        const double* khi = gf_khi(v);
+        const uint32_t stride = gridDim.x * GF_NT;
+        for (uint32_t b = blockIdx.x * GF_NT + (threadIdx.x & ~63u); b < m.n; b += stride) {
+            const uint32_t i = b + (threadIdx.x & 63);
+            bool changed = false;
+            if (i < m.n) {
+                const double klo = v.K[i], kh = khi[i];
+                const uint32_t lo = v.lo[i], hi = v.hi[i];
+                uint32_t before = 0, after = 0;
+                for (uint32_t j = lo; j < hi; ++j) {
+                    if (j == i) continue;
+                    before += khi[j] < klo;
+                    after += v.K[j] > kh;
+                }
+                const uint32_t glo = min(v.Gb[i], 1 + lo + before), ghi = max(v.T[i], hi - after);
+                changed = glo != v.Gb[i] || ghi != v.T[i];
+                v.Gb[i] = glo;
+                v.T[i] = ghi;
+            }
+            gf_or(&a.meta[k].ichanged[it], changed ? 1u : 0u);
+        }
+    }
+}
+
+// Acceptance: intervals settled; every non-seed pushed before its earliest pop; every possible
+// first-popped neighbour carries the label of the solution.
+__global__ void __launch_bounds__(GF_NT) gf_ival_check_kernel(GfArgs a)
+{
+    const uint32_t cnt = gf_count(a);
+    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+        const GfMeta m = a.meta[k];
+        if (m.fail || m.n == 0 || !m.tie) continue;
+        if (m.ichanged[kGridIvalIter - 1]) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.meta[k].fail, kGfTie);
+            continue;
+        }
+        const GfView v = gf_view(a.pool, m);
         const uint32_t stride = gridDim.x * GF_NT;
         for (uint32_t b = blockIdx.x * GF_NT + (threadIdx.x & ~63u); b < m.n; b += stride) {
             const uint32_t i = b + (threadIdx.x & 63);
             uint32_t bad = 0;
-            if (i < m.n) {
-                const double ki = v.K[i];
-                const uint32_t di = gf_delta(v, i);
-                for (uint32_t j = v.lo[i]; j < v.hi[i]; ++j) {
-                    const double kj = v.K[j];
-                    if (kj == ki) continue;
-                    const uint32_t d = di + gf_delta(v, j);
-                    if (d > 0 && fabs(kj - ki) <= (double)d * kEpsilon * (1.0 + 1e-9) + 1e-9) bad |= kGfTie;
-                }
-                const uint32_t sx = v.spix[i];
-                if (!(sx & GF_SEED)) {
-                    const uint32_t pa = v.par[i];
-                    if (gf_gsize(v, pa) > 1) {
-                        const double kp = v.K[pa];
-                        const uint32_t lp = v.lbl[pa];
-                        const int y = (int)(sx / (uint32_t)a.W), x = (int)(sx % (uint32_t)a.W);
+            if (i < m.n && !(v.spix[i] & GF_SEED)) {
+                uint32_t e[8];
+                gf_nbrs(v, i, e);
+                uint32_t thi = GF_NONE;
 #pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            const uint32_t e = gf_nbr(Pf, fn, a.W, a.H, y, x, j);
-                            if (e != GF_NONE && v.K[e] == kp && v.lbl[e] != lp) bad |= kGfTie;
-                        }
-                    }
-                }
+                for (int j = 0; j < 8; ++j)
+                    if (e[j] != GF_NONE) thi = min(thi, v.T[e[j]]);
+                if (!(thi < v.Gb[i])) bad = kGfTie;
+                const uint32_t L = v.lbl[i];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (e[j] != GF_NONE && v.Gb[e[j]] <= thi && v.lbl[e[j]] != L) bad = kGfTie;
             }
             gf_or(&a.meta[k].fail, bad);
         }
@@ -589,7 +692,7 @@ __global__ void __launch_bounds__(GF_NT) gf_write_kernel(GfArgs a)
 
 size_t grid_pool_words(size_t F, size_t npx)
 {
-    // About 8 bytes per pixel of the chunk (a frame needs 52 bytes per mask pixel), 64 MiB to 1 GiB.
+    // About 8 bytes per pixel of the chunk (a frame needs 84 bytes per mask pixel), 64 MiB to 1 GiB.
     const size_t bytes = std::clamp<size_t>(F * npx * 8, size_t(64) << 20, size_t(1) << 30);
     return bytes / 4;
 }
@@ -615,13 +718,19 @@ void launch_grid_flood(const DetectBuffers& b, int W, int H, int F, const Fz* fz
         gf_tile_sort_kernel<<<g, GF_NT, 0, s>>>(a);
         for (int p = 0; p < passes; ++p) gf_merge_kernel<<<g, GF_NT, 0, s>>>(a, p);
         gf_finalize_kernel<<<g, GF_NT, 0, s>>>(a);
+        gf_neighbours_kernel<<<g, GF_NT, 0, s>>>(a);
         for (int it = 0; it < kGridMaxIter; ++it) {
             gf_age_kernel<<<g, GF_NT, 0, s>>>(a, it);
             gf_rank_kernel<<<g, GF_NT, 0, s>>>(a, it);
         }
         gf_check1_kernel<<<g, GF_NT, 0, s>>>(a);
         gf_check2_kernel<<<g, GF_NT, 0, s>>>(a);
-        gf_check3_kernel<<<g, GF_NT, 0, s>>>(a);
+        gf_ival_init_kernel<<<g, GF_NT, 0, s>>>(a);
+        for (int it = 0; it < kGridIvalIter; ++it) {
+            gf_ival_key_kernel<<<g, GF_NT, 0, s>>>(a, it);
+            gf_ival_rank_kernel<<<g, GF_NT, 0, s>>>(a, it);
+        }
+        gf_ival_check_kernel<<<g, GF_NT, 0, s>>>(a);
         gf_write_kernel<<<g, GF_NT, 0, s>>>(a);
     }
     PT_CHECK(hipGetLastError());

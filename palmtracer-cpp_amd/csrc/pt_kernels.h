@@ -122,9 +122,11 @@ __host__ __device__ inline size_t seed_words(size_t npx) { return (npx + 31) / 3
 // the work space beyond. Work space per slot: pixel -> entry map, then 25 words per entry. The stride
 // is a power of two, so the padded sort of any admitted frame fits.
 constexpr uint32_t kRankMax = 262144;
-// Grid rank flood: fixed-point steps, state words per flagged frame.
+// Grid rank flood: fixed-point steps, interval steps (frames with equal keys), state words per
+// flagged frame.
 constexpr int kGridMaxIter = 8;
-constexpr size_t kGridMetaWords = 16;
+constexpr int kGridIvalIter = 6;
+constexpr size_t kGridMetaWords = 32;
 constexpr uint32_t kRankLds = 16384;
 __host__ __device__ inline size_t rf_stride(size_t npx)
 {

@@ -432,22 +432,26 @@ def test_exact_floods_bit_exact(pt_gpu, oracle_mod, ref_inputs, ref_vectors, nam
         pt_gpu.set_option("force_rank_flood", 0)
 
 
-def test_localization_with_all_frames_exact(pt_gpu, oracle_mod, frames512):
-    """Localization with every frame through the exact frame flood: the frames of the first chunks
-    deferred to the side stream, those of the last chunk resolved in line; rows unchanged."""
+@pytest.mark.parametrize("defer", [1, 0])
+def test_localization_with_all_frames_exact(pt_gpu, oracle_mod, frames512, defer):
+    """Localization with every frame through the exact frame flood: the flagged frames of the first
+    chunks deferred to the side stream (defer 1) or resolved in line (defer 0), those of the last chunk
+    resolved in line; rows unchanged."""
     st, thr = frames512
     exp = oracle_mod.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN)
     pt_gpu.set_option("force_sequential_watershed", 1)
     pt_gpu.set_option("chunk_pixels", 5 * 512 * 512)
+    pt_gpu.set_option("defer_flagged", defer)
     try:
         got = pt_gpu.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN)
         stats = pt_gpu.last_stats()
     finally:
         pt_gpu.set_option("force_sequential_watershed", 0)
         pt_gpu.set_option("chunk_pixels", 0)
+        pt_gpu.set_option("defer_flagged", 0)
     last = st.shape[0] - 5 * ((st.shape[0] - 1) // 5)
     assert stats["flagged_frames"] == st.shape[0]
-    assert stats["deferred_frames"] == st.shape[0] - last
+    assert stats["deferred_frames"] == (st.shape[0] - last if defer else 0)
     assert got.shape == exp.shape and np.array_equal(got, exp)
 
 
