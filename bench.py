@@ -95,12 +95,13 @@ def fit_precision(pt, torch, stack, thr, fit_params, frames):
     return dict(compare_fits(res[0], res[1]), frames=frames)
 
 
-def serve_latency(pt, torch, seconds, slots, n_cams, sizes, fit_params):
+def serve_latency(pt, torch, seconds, slots, n_cams, sizes, fit_params, load=1.0):
     """Streamed load: n_cams simulated cameras, each with a fixed frame size (`sizes` in turn), a frame
-    rate uniform in [100, 1000] fps and chunks of 10-100 frames (seeded), the
-    headline workload's emitter density (200 per 512^2 frame, scaled by area). Chunks arrive as Poisson
-    processes (mean rate = fps / chunk frames) and are submitted to a pt_server with `slots` slots; a
-    chunk's latency is the time from its arrival to its rows in HBM, counted once per frame."""
+    rate uniform in [100, 1000] fps (times `load`), chunks of 10-100 frames and an emitter density
+    uniform in 0.5-3 per um^2 at 100 nm pixels, capped at 5000 emitters per frame (all seeded). Chunks
+    arrive as Poisson processes (mean rate = fps / chunk frames) and are submitted to a pt_server with
+    `slots` slots; a chunk's latency is the time from its arrival to its rows in HBM, counted once per
+    frame."""
     import random
     import threading
     import numpy as np
@@ -108,19 +109,20 @@ def serve_latency(pt, torch, seconds, slots, n_cams, sizes, fit_params):
     cams = []
     for c in range(n_cams):
         size = sizes[c % len(sizes)]
-        fps = rng.uniform(100.0, 1000.0)
+        fps = rng.uniform(100.0, 1000.0) * load
         chunk = rng.randint(10, 100)
-        emitters = size * size * 200 // (512 * 512)
+        density = rng.uniform(0.5, 3.0)
+        emitters = int(min(5000, round(density * size * size * 1e-2)))
         pool = torch.empty((128, size, size), dtype=torch.uint16, device="cuda")
         pt.synth_device(pool, seed=0x5EED1000 + c, emitters=emitters)
         thr = pt.autothreshold_device(pool[0])
-        mpp = 2 * emitters + 64
+        mpp = int(emitters * 1.25) + 64
         outs = [torch.empty(chunk * mpp * 18, dtype=torch.float64, device="cuda") for _ in range(8)]
-        cams.append(dict(size=size, fps=fps, chunk=chunk, pool=pool, thr=thr, mpp=mpp, outs=outs, off=0, k=0,
-                         busy=[None] * len(outs)))
+        cams.append(dict(size=size, fps=fps, chunk=chunk, density=density, emitters=emitters, pool=pool, thr=thr,
+                         mpp=mpp, outs=outs, off=0, k=0, busy=[None] * len(outs)))
     torch.cuda.synchronize()
-    budget = 32 << 30
-    srv = pt.Server(slots=slots, budget_bytes=budget, chunk_pixels=1 << 26)
+    budget = 96 << 30
+    srv = pt.Server(slots=slots, budget_bytes=budget, chunk_pixels=1 << 25)
 
     done = set()
 
@@ -174,7 +176,7 @@ def serve_latency(pt, torch, seconds, slots, n_cams, sizes, fit_params):
 
     th = threading.Thread(target=scheduler)
     th.start()
-    lat, weights, rows, lags, i = [], [], 0, [], 0
+    lat, weights, rows, lags, i, sizes_of = [], [], 0, [], 0, []
     while i < len(arrivals):
         if i >= len(submitted):
             time.sleep(0.0002)
@@ -185,26 +187,50 @@ def serve_latency(pt, torch, seconds, slots, n_cams, sizes, fit_params):
         lat.append(t_done - planned)
         lags.append(t_sub - planned)
         weights.append(n)
+        sizes_of.append(cams[arrivals[i][1]]["size"])
         rows += r
         i += 1
     th.join()
     t_end = srv.now()
     srv.close()
-    lat, weights = np.array(lat), np.array(weights, dtype=np.float64)
-    order = np.argsort(lat)
-    cw = np.cumsum(weights[order]) / weights.sum()
-    pct = lambda q: float(lat[order][min(int(np.searchsorted(cw, q)), len(lat) - 1)])
+    lat, weights, sizes_of = np.array(lat), np.array(weights, dtype=np.float64), np.array(sizes_of)
+
+    def pct(sel, q):
+        l, w = lat[sel], weights[sel]
+        order = np.argsort(l)
+        cw = np.cumsum(w[order]) / w.sum()
+        return float(l[order][min(int(np.searchsorted(cw, q)), len(l) - 1)])
+
+    allsel = np.ones(len(lat), bool)
     frames = int(weights.sum())
     px = sum(cams[ci]["chunk"] * cams[ci]["size"] ** 2 for _, ci in arrivals)
+    by_size = {}
+    for sz in sorted(set(sizes_of.tolist())):
+        sel = sizes_of == sz
+        by_size[f"{sz}x{sz}"] = {"p50_ms": pct(sel, 0.5), "p99_ms": pct(sel, 0.99), "frames": int(weights[sel].sum())}
+    # Fast fit against the reference-order FP64 fit on one chunk of every camera size.
+    precision = {}
+    for cam in {c["size"]: c for c in cams}.values():
+        d = cam["pool"][:10]
+        o = torch.empty(10 * cam["mpp"] * 18, dtype=torch.float64, device="cuda")
+        res = {}
+        for prec in (0, 1):
+            pt.set_option("fit_fp64", prec)
+            k = pt.localization_device(d, o, cam["thr"], 0.0, 2, fit_params, cam["mpp"])
+            res[prec] = o[: k * 18].view(-1, 18).cpu().numpy().copy()
+        pt.set_option("fit_fp64", 0)
+        precision[f"{cam['size']}x{cam['size']}"] = compare_fits(res[0], res[1])
     return {
-        "p50_ms": pct(0.50), "p99_ms": pct(0.99), "max_ms": float(lat.max()),
-        "latency": "chunk arrival to its rows in HBM, per frame",
+        "p50_ms": pct(allsel, 0.50), "p99_ms": pct(allsel, 0.99), "max_ms": float(lat.max()),
+        "latency": "chunk arrival to its rows in HBM, per frame", "by_frame_size": by_size,
         "requests": len(arrivals), "frames": frames, "rows": rows,
         "offered_frames_per_s": sum(c["fps"] for c in cams), "offered_gpx_per_s": px / seconds / 1e9,
-        "localizations_per_s": rows / ((t_end - base) * 1e-3),
-        "cameras": n_cams, "frame_sizes": [c["size"] for c in cams], "emitters_per_512x512": 200,
+        "localizations_per_s": rows / ((t_end - base) * 1e-3), "load": load,
+        "cameras": [{"size": c["size"], "fps": round(c["fps"], 1), "chunk": c["chunk"], "density_per_um2": round(c["density"], 2),
+                     "emitters": c["emitters"]} for c in cams],
         "slots": slots, "budget_gb": budget >> 30,
         "seconds": seconds, "max_submit_lag_ms": float(max(lags)),
+        "fit_precision_fast_vs_fp64_10_frames": precision,
     }
 
 
@@ -420,8 +446,10 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--batch-frame-sample", type=int, default=4,
                     help="frames per stack timed through the plain per-frame path")
-    ap.add_argument("--serve-sizes", default="256,512,1024",
-                    help="frame sides of the cameras, assigned in turn (e.g. 256,512,1024,2048)")
+    ap.add_argument("--serve-sizes", default="256,512,1024,2048",
+                    help="frame sides of the cameras, assigned in turn")
+    ap.add_argument("--serve-load", type=float, default=1.0,
+                    help="factor on every camera's frame rate (1: rates uniform in 100-1000 fps)")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
                     help="library option (pt_set_option) applied before the run, e.g. detect_rows=103")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -540,7 +568,8 @@ def main():
     serving = None
     if args.serve_seconds > 0:
         sizes = [int(v) for v in args.serve_sizes.split(",")]
-        serving = serve_latency(pt, torch, args.serve_seconds, args.serve_slots, args.serve_cameras, sizes, fit_params)
+        serving = serve_latency(pt, torch, args.serve_seconds, args.serve_slots, args.serve_cameras, sizes, fit_params,
+                                args.serve_load)
 
     astig = astig3d_leg(pt, torch, args.astig_frames, args.astig_steps) if args.astig_frames > 0 else None
 

@@ -210,6 +210,13 @@ void Lane::release()
 }
 
 namespace {
+// Device bytes of an allocation of v bytes: whole 2 MiB pages.
+size_t alloc_pages(size_t v)
+{
+    constexpr size_t kPage = size_t(2) << 20;
+    return (std::max<size_t>(v, 1) + kPage - 1) / kPage * kPage;
+}
+
 // Sizes of a lane's buffers for chunks of F frames of npx pixels (Lane::prepare allocates them;
 // Engine::workspace_bytes adds them up for the serving loop's HBM budget).
 struct LaneSizes {
@@ -219,16 +226,18 @@ struct LaneSizes {
         n = F * npx;
         regionCap = npx / 4 + 2;  // roots are pairwise non-adjacent: at most ceil(H/2)*ceil(W/2)
         multiCap = n / 64 + 1024;
-        fbSlots = std::clamp<size_t>((size_t(1) << 26) / std::max<size_t>(npx, 1), 4, 64);
+        // Heap restatement slots (frames the other exact floods hand over, rare): about 16M pixels.
+        fbSlots = std::clamp<size_t>((size_t(1) << 24) / std::max<size_t>(npx, 1), 2, 16);
         // Single-workgroup rank-flood slots (frames the grid rank flood hands over): up to 128 frames at
         // a time, within 256 MiB of work space.
         rfSlots = std::clamp<size_t>(std::min<size_t>(F, (size_t(1) << 28) / (rf_slot_words(npx) * 4)), 1, 128);
         gridWords = grid_pool_words(F, npx);
     }
-    // Device bytes of a prepared lane plus its ROI buffer (DevBuf allocates at least 256 bytes).
+    // Device bytes of a prepared lane plus its ROI buffer, each buffer counted in whole 2 MiB pages
+    // (the granularity of device allocations).
     size_t bytes(size_t F, size_t npx) const
     {
-        auto b = [](size_t v) { return std::max<size_t>(v, 256); };
+        auto b = alloc_pages;
         return b(n * 4) * 5 + b(n * sizeof(PixAux)) + b((F + 1) * 4 * 9) + b(F * regionCap * sizeof(Region)) +
                b(F * regionCap * sizeof(Fz)) + b(F * regionCap * 4) + b(multiCap * 8) + b(fbSlots * fb_slot_words(npx) * 4) +
                b(kNumCounters * 4) + b((F + 1) * 20) + b(rfSlots * rf_slot_words(npx) * 4) + b(F * seed_words(npx) * 4) +
@@ -344,11 +353,13 @@ size_t Engine::workspace_bytes(uint64_t planes, int H, int W, uint64_t maxPerPla
     const size_t SF = std::max<size_t>(1, std::min<size_t>(F, std::max<size_t>(64, (size_t(1) << 24) / npx)));
     size_t t = LaneSizes(F, npx).bytes(F, npx);                                   // first main lane
     if (planes > F && opt.lanes > 1) t += LaneSizes(F, npx).bytes(F, npx);       // second main lane
-    t += LaneSizes(SF, npx).bytes(SF, npx) + SF * 4;                             // side lane + frame map
+    if (opt.deferFlagged) t += LaneSizes(SF, npx).bytes(SF, npx) + SF * 4;      // side lane + frame map
     const uint64_t roiCap = std::min<uint64_t>(outCapRows, planes * std::min<uint64_t>(maxPerPlane, npx / 4 + 2));
     // ROI store, side ROI store (grown by half again), fit rows, staged rows (grown by half again),
-    // hand-over list, plane bases and placement plan, pinned-sized small buffers.
-    t += roiCap * sizeof(Roi) * 5 / 2 + roiCap * kCols * 8 * 5 / 2 + roiCap * 4 + (planes + 1) * 4 + planes * 28 + 4096;
+    // hand-over list, plane bases and placement plan, the call's small buffers (counters, fit words).
+    t += alloc_pages(roiCap * sizeof(Roi)) + alloc_pages(roiCap * sizeof(Roi) * 3 / 2) + alloc_pages(roiCap * kCols * 8) +
+         alloc_pages(roiCap * kCols * 8 * 3 / 2) + alloc_pages(roiCap * 4) + alloc_pages((planes + 1) * 4) +
+         alloc_pages(planes * 28) + 4 * alloc_pages(4096);
     return t;
 }
 

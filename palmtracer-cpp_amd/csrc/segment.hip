@@ -26,6 +26,8 @@
 // raster order in FP64.
 #include "flood_common.h"
 
+#include <algorithm>
+
 namespace pt {
 namespace {
 
@@ -560,55 +562,154 @@ __global__ void __launch_bounds__(FL_NT) __attribute__((amdgpu_num_sgpr(80))) fr
                 flag_frame(frameFlag, counters, flagList, f);
             }
         }
-    } else {
-        // Same steps in global memory, over pixel indices (label state in aux[component].cmin).
-        for (uint32_t k = tid; k < n; k += FL_NT) lb[ml[k]] = ml[k];
-        __syncthreads();
-        for (uint32_t k = tid; k < n; k += FL_NT) {
+    }
+    __syncthreads();
+    if (tid == 0 && ldsCap > 0 && n <= ldsCap) rootCnt[f] = nRoots;
+}
+
+// ---------------------------------------------------------------------------------------------
+// The same labelling for frames whose mask exceeds the LDS capacity of frame_label_kernel, in
+// global memory over pixel indices, spread over several workgroups per frame (y: frames, strided;
+// x: mask pixels): union-find with device-scope atomics, label states in aux[component].cmin,
+// region slots from rootCnt. Five passes: parents, unions, flattening, maxima, labels.
+struct FlgArgs {
+    const uint32_t* P;
+    const uint32_t* seedBits;
+    uint32_t* lab;
+    uint32_t* fin;
+    PixAux* aux;
+    const uint32_t* maskList;
+    const uint32_t* maskCnt;
+    const uint32_t* seedList;
+    const uint32_t* seedCnt;
+    uint32_t* rootCnt;
+    Region* regions;
+    size_t regionCap;
+    uint64_t* multiList;
+    size_t multiCap;
+    uint32_t* counters;
+    uint32_t* frameFlag;
+    uint32_t* flagList;
+    int W, H, F;
+    size_t npx;
+    uint32_t ldsCap;
+};
+
+__device__ inline bool flg_global(const FlgArgs& a, uint32_t f) { return !(a.ldsCap > 0 && a.maskCnt[f] <= a.ldsCap); }
+
+__global__ void __launch_bounds__(NT) flg_parents_kernel(FlgArgs a)
+{
+    for (uint32_t f = blockIdx.y; f < (uint32_t)a.F; f += gridDim.y) {
+        if (!flg_global(a, f)) continue;
+        const uint32_t n = a.maskCnt[f];
+        const uint32_t* ml = a.maskList + (size_t)f * a.npx;
+        uint32_t* lb = a.lab + (size_t)f * a.npx;
+        for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < n; k += gridDim.x * NT) lb[ml[k]] = ml[k];
+    }
+}
+
+__global__ void __launch_bounds__(NT) flg_union_kernel(FlgArgs a)
+{
+    for (uint32_t f = blockIdx.y; f < (uint32_t)a.F; f += gridDim.y) {
+        if (!flg_global(a, f)) continue;
+        const uint32_t n = a.maskCnt[f];
+        const uint32_t* ml = a.maskList + (size_t)f * a.npx;
+        const uint32_t* Pf = a.P + (size_t)f * a.npx;
+        uint32_t* lb = a.lab + (size_t)f * a.npx;
+        for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < n; k += gridDim.x * NT) {
             const uint32_t p = ml[k];
-            const int y = (int)(p / (uint32_t)W), x = (int)p - y * W;
+            const int y = (int)(p / (uint32_t)a.W), x = (int)p - y * a.W;
             const int dys[4] = {0, -1, -1, -1}, dxs[4] = {-1, -1, 0, 1};
             for (int j = 0; j < 4; ++j) {
                 const int yy = y + dys[j], xx = x + dxs[j];
-                if (yy < 0 || xx < 0 || xx >= W) continue;
-                const uint32_t q = (uint32_t)(yy * W + xx);
+                if (yy < 0 || xx < 0 || xx >= a.W) continue;
+                const uint32_t q = (uint32_t)(yy * a.W + xx);
                 if (Pf[q] == 0) continue;
-                uint32_t a = p, b = q;
+                uint32_t u = p, v = q;
                 while (true) {
-                    a = uf_find(lb, a);
-                    b = uf_find(lb, b);
-                    if (a == b) break;
-                    if (a > b) { const uint32_t t = a; a = b; b = t; }
-                    const uint32_t old = atomicMin(&lb[b], a);
-                    if (old == b) break;
-                    b = old;
+                    u = uf_find(lb, u);
+                    v = uf_find(lb, v);
+                    if (u == v) break;
+                    if (u > v) { const uint32_t t = u; u = v; v = t; }
+                    const uint32_t old = atomicMin(&lb[v], u);
+                    if (old == v) break;
+                    v = old;
                 }
             }
         }
-        __syncthreads();
-        for (uint32_t k = tid; k < n; k += FL_NT) {
+    }
+}
+
+__global__ void __launch_bounds__(NT) flg_flatten_kernel(FlgArgs a)
+{
+    for (uint32_t f = blockIdx.y; f < (uint32_t)a.F; f += gridDim.y) {
+        if (!flg_global(a, f)) continue;
+        const uint32_t n = a.maskCnt[f];
+        const uint32_t* ml = a.maskList + (size_t)f * a.npx;
+        uint32_t* lb = a.lab + (size_t)f * a.npx;
+        PixAux* ax = a.aux + (size_t)f * a.npx;
+        for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < n; k += gridDim.x * NT) {
             const uint32_t p = ml[k];
             const uint32_t c = uf_find(lb, p);
             if (c == p) ax[p].cmin = 0u;
             else __hip_atomic_store(&lb[p], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        __syncthreads();
-        for (uint32_t k = tid; k < ns; k += FL_NT) {
+    }
+}
+
+__global__ void __launch_bounds__(NT) flg_maxima_kernel(FlgArgs a)
+{
+    for (uint32_t f = blockIdx.y; f < (uint32_t)a.F; f += gridDim.y) {
+        if (!flg_global(a, f)) continue;
+        const uint32_t ns = a.seedCnt[f];
+        const uint32_t* sl = a.seedList + (size_t)f * a.npx;
+        const uint32_t* sbf = a.seedBits + (size_t)f * seed_words(a.npx);
+        uint32_t* lb = a.lab + (size_t)f * a.npx;
+        uint32_t* fn = a.fin + (size_t)f * a.npx;
+        PixAux* ax = a.aux + (size_t)f * a.npx;
+        for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < ns; k += gridDim.x * NT) {
             const uint32_t s = sl[k];
             const uint32_t c = ld_relaxed(&lb[s]);
-            const uint32_t L = label_seed(s);
-            add_label(&ax[c].cmin, L, [](uint32_t* a, uint32_t c, uint32_t v) { return atomicCAS(a, c, v); },
-                      [](uint32_t* a, uint32_t v) { atomicExch(a, v); });
-        }
-        __syncthreads();
-        for (uint32_t k = tid; k < n; k += FL_NT) {
-            const uint32_t p = ml[k];
-            const uint32_t c = ld_relaxed(&lb[p]);
-            assign(p, c, c == p, ld_relaxed(&ax[c].cmin));
+            const uint32_t L = chain_label(sbf, s, a.W);
+            fn[s] = L;
+            if (L - 1u == s) {
+                const uint32_t slot = atomicAdd(&a.rootCnt[f], 1u);
+                ax[s].rslot = slot;
+                if (slot < a.regionCap) a.regions[(size_t)f * a.regionCap + slot] = Region{s, 0u, 0ull, 0ull, 0ull};
+                else atomicOr(&a.counters[kStatus], 1u);
+            }
+            add_label(&ax[c].cmin, L, [](uint32_t* q, uint32_t c0, uint32_t v) { return atomicCAS(q, c0, v); },
+                      [](uint32_t* q, uint32_t v) { atomicExch(q, v); });
         }
     }
-    __syncthreads();
-    if (tid == 0) rootCnt[f] = nRoots;
+}
+
+__global__ void __launch_bounds__(NT) flg_labels_kernel(FlgArgs a)
+{
+    for (uint32_t f = blockIdx.y; f < (uint32_t)a.F; f += gridDim.y) {
+        if (!flg_global(a, f)) continue;
+        const uint32_t n = a.maskCnt[f];
+        const uint32_t* ml = a.maskList + (size_t)f * a.npx;
+        const uint32_t* sbf = a.seedBits + (size_t)f * seed_words(a.npx);
+        uint32_t* lb = a.lab + (size_t)f * a.npx;
+        uint32_t* fn = a.fin + (size_t)f * a.npx;
+        const PixAux* ax = a.aux + (size_t)f * a.npx;
+        for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < n; k += gridDim.x * NT) {
+            const uint32_t p = ml[k];
+            const uint32_t c = ld_relaxed(&lb[p]);
+            const bool first = c == p;
+            const uint32_t st = ld_relaxed(&ax[c].cmin);
+            lb[p] = c | (first ? kCompRoot : 0u);
+            if (st == 0u) { flag_frame(a.frameFlag, a.counters, a.flagList, (int)f); continue; }   // cannot happen
+            if (st != kNone) { fn[p] = st; continue; }
+            if (!seed_bit(sbf, p)) fn[p] = 0u;     // several labels: the flood starts from the maxima
+            if (first) {
+                const uint32_t j = atomicAdd(&a.counters[kMultiCnt], 1u);
+                if (j < a.multiCap) a.multiList[j] = ((uint64_t)f << 32) | c;
+                else flag_frame(a.frameFlag, a.counters, a.flagList, (int)f);
+            }
+        }
+    }
 }
 
 // LDS of frame_label_kernel for frames of npx pixels: the bitmap and prefix counts (later the
@@ -1702,8 +1803,9 @@ __global__ void reset_pass_kernel(uint32_t* __restrict__ counters)
 }
 
 // ComputeStatistics sums (Segmentation2D.cpp:144-175) as exact integers: surface, sum P, sum P*x,
-// sum P*y per region. One workgroup per frame; the frame's regions (slots [0, rootCnt) in the order
-// seed_label_kernel opened them) are accumulated in LDS when they fit, else with global atomics.
+// sum P*y per region. The frame's regions (slots [0, rootCnt) in the order the labelling opened
+// them) are accumulated in LDS by one workgroup per frame when they fit, else with global atomics
+// by all the workgroups of the frame (grid: x workgroups per frame, y frames).
 constexpr int RS_NT = 1024;
 constexpr uint32_t RS_LDS_REGIONS = 2048;
 
@@ -1716,17 +1818,21 @@ __global__ void __launch_bounds__(RS_NT) region_stats_kernel(const uint32_t* __r
 {
     __shared__ unsigned long long sP[RS_LDS_REGIONS], sPx[RS_LDS_REGIONS], sPy[RS_LDS_REGIONS];
     __shared__ uint32_t sSurf[RS_LDS_REGIONS];
-    const int f = blockIdx.x;
+    const int f = blockIdx.y;
     if (skip && skip[f]) return;   // deferred frame: its labels are resolved in a later pass
     const uint32_t cnt = maskCnt[f];
     const uint32_t nReg = (uint32_t)min((size_t)rootCnt[f], regionCap);
     const bool inLds = nReg <= RS_LDS_REGIONS;
+    // LDS accumulation: one workgroup per frame; global atomics: all of the frame's workgroups.
+    if (inLds && blockIdx.x != 0) return;
+    const uint32_t k0 = inLds ? threadIdx.x : blockIdx.x * RS_NT + threadIdx.x;
+    const uint32_t kStride = inLds ? RS_NT : gridDim.x * RS_NT;
     Region* rg = regions + (size_t)f * regionCap;
     if (inLds) {
         for (uint32_t r = threadIdx.x; r < nReg; r += RS_NT) { sP[r] = 0; sPx[r] = 0; sPy[r] = 0; sSurf[r] = 0; }
         __syncthreads();
     }
-    for (uint32_t k = threadIdx.x; k < cnt; k += RS_NT) {
+    for (uint32_t k = k0; k < cnt; k += kStride) {
         const uint32_t p = maskList[(size_t)f * npx + k];
         const uint32_t L = fin[(size_t)f * npx + p];
         if (L == 0 || L > npx) { atomicOr(&counters[kStatus], 2u); continue; }
@@ -1918,9 +2024,23 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
     uint32_t ldsCap = 0, parentOff = 0;
     const size_t flLds = frame_label_lds(npx, ldsCap, parentOff);
     mask_hist_kernel<<<F, MH_NT, 0, s>>>(b.P, b.maskList, b.maskCnt, b.hist, npx);
-    frame_label_kernel<<<F, FL_NT, flLds, s>>>(b.P, b.seedBits, b.lab, b.fin, b.aux, b.maskList, b.maskCnt, b.seedList,
-                                               b.seedCnt, b.rootCnt, b.regions, b.regionCap, b.multiList, b.multiCap,
-                                               b.counters, b.frameFlag, b.flagList, W, H, npx, ldsCap, parentOff, b.hist);
+    if (ldsCap > 0)
+        frame_label_kernel<<<F, FL_NT, flLds, s>>>(b.P, b.seedBits, b.lab, b.fin, b.aux, b.maskList, b.maskCnt, b.seedList,
+                                                   b.seedCnt, b.rootCnt, b.regions, b.regionCap, b.multiList, b.multiCap,
+                                                   b.counters, b.frameFlag, b.flagList, W, H, npx, ldsCap, parentOff, b.hist);
+    {
+        // Frames beyond the LDS capacity (every frame when it is 0).
+        const FlgArgs fa{b.P, b.seedBits, b.lab, b.fin, b.aux, b.maskList, b.maskCnt, b.seedList, b.seedCnt, b.rootCnt,
+                         b.regions, b.regionCap, b.multiList, b.multiCap, b.counters, b.frameFlag, b.flagList, W, H, F,
+                         npx, ldsCap};
+        const unsigned bx = (unsigned)std::clamp<size_t>(npx / 32768, 1, 64);
+        const dim3 gg(bx, (unsigned)std::clamp<size_t>(2048 / bx, 1, (size_t)F));
+        flg_parents_kernel<<<gg, NT, 0, s>>>(fa);
+        flg_union_kernel<<<gg, NT, 0, s>>>(fa);
+        flg_flatten_kernel<<<gg, NT, 0, s>>>(fa);
+        flg_maxima_kernel<<<gg, NT, 0, s>>>(fa);
+        flg_labels_kernel<<<gg, NT, 0, s>>>(fa);
+    }
     auto watershed = [&](int pass2) {
         if (forceSequential) force_flag_kernel<<<ceil_div(F, 256), 256, 0, s>>>(b.counters, b.frameFlag, b.flagList, F);
         if (pass2)
@@ -1962,8 +2082,8 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
         reset_pass_kernel<<<1, 64, 0, s>>>(b.counters);
         watershed(1);
     }
-    region_stats_kernel<<<F, RS_NT, 0, s>>>(b.P, b.fin, b.aux, b.maskList, b.maskCnt, b.rootCnt, b.regions, b.regionCap,
-                                            b.counters, W, npx, skip);
+    region_stats_kernel<<<dim3((unsigned)std::clamp<size_t>(npx / 262144, 1, 16), F), RS_NT, 0, s>>>(
+        b.P, b.fin, b.aux, b.maskList, b.maskCnt, b.rootCnt, b.regions, b.regionCap, b.counters, W, npx, skip);
     region_count_kernel<<<F, NT, 0, s>>>(b.regions, b.regionCap, b.rootCnt, b.found, b.keep, maxPoints, skip);
     if (b.runTotal && b.scanWait) PT_CHECK(hipStreamWaitEvent(s, b.scanWait, 0));
     exclusive_scan_kernel<<<1, 1024, 0, s>>>(b.keep, b.roiBase, F, b.runTotal, b.chunkBase);
