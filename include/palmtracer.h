@@ -165,8 +165,8 @@ PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint6
  * round, 1-64, default 16), "detect_rows" (rows per detection wave, 0: chosen from the chunk),
  * "lanes" (2: consecutive chunks alternate between two streams; 1: one stream), "batch_autotune" (1: tune
  * new fit classes of pt_batch_localization), "batch_graph" (1: its fit stage as a hipGraph), "defer_flagged"
- * (1: flagged frames of all but the last chunk are re-detected and resolved on the side stream;
- * 0, the default: every chunk resolves its own). Returns 1 on
+ * (flagged frames of all but the last chunk re-detected and resolved on the side stream: 0 never, 1
+ * always, 2 (default) when the last chunk whose counters are known flagged at most 2 % of its frames). Returns 1 on
  * success. */
 PT_API int PT_CALL pt_set_option(const char* name, double value);
 

@@ -62,7 +62,8 @@ struct Options {
     int lanes = 2;                  // main lanes the chunks alternate between (1: one stream, for timing)
     bool batchAutotune = true;      // batched localization: tune each new fit class's schedule
     bool batchGraph = true;         // batched localization: the fit stage as a captured hipGraph
-    bool deferFlagged = false;      // flagged frames of all but the last chunk: exact flood on the side lane
+    int deferFlagged = 2;           // flagged frames of all but the last chunk on the side lane: 0 never,
+                                    // 1 always, 2 when the last known chunk flagged at most 2 % of its frames
     // Defaults with the environment's overrides (PALMTRACER_FIT_PRECISION, PALMTRACER_MAX_ITER,
     // PALMTRACER_CHUNK_PIXELS).
     static Options defaults();
@@ -112,6 +113,7 @@ struct Lane {
     DetectBuffers b{};
     DevBuf P, lab, fin, aux, mask, seed, perFrame, regions, fz, order, multi, fb, counters, flag, rf, gpool, gmeta, sbits, hist;
     DevBuf rois, rows, staged, frameMap, fitNext, fitHand;
+    GridFloodGraph gridGraph;    // the grid rank flood of this lane as a captured graph
     uint32_t* pinned = nullptr;  // host-visible: frame map, roiBase and counters of an asynchronous batch
     size_t pinnedWords = 0;
     void prepare(size_t F, size_t npx);

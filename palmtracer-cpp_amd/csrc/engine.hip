@@ -199,6 +199,7 @@ Engine::~Engine()
 
 void Lane::release()
 {
+    gridGraph.release();
     for (DevBuf* d : {&P, &lab, &fin, &aux, &mask, &seed, &perFrame, &regions, &fz, &order, &multi, &fb, &counters, &flag,
                       &rf, &gpool, &gmeta, &sbits, &hist, &rois, &rows, &staged, &frameMap, &fitNext, &fitHand})
         d->release();
@@ -303,6 +304,7 @@ void Lane::prepare(size_t F, size_t npx)
     b.gridMeta = gmeta.as<uint32_t>();
     b.gridPool = gpool.as<uint32_t>();
     b.gridPoolWords = z.gridWords;
+    b.gridGraph = &gridGraph;
     b.rfScratch = rf.as<uint32_t>();
     b.rfSlots = rfSlots;
     b.seedBits = sbits.as<uint32_t>();
@@ -335,6 +337,7 @@ size_t Lane::device_bytes() const
 
 void Lane::release_buffers()
 {
+    gridGraph.release();
     for (DevBuf* d : {&P, &lab, &fin, &aux, &mask, &seed, &perFrame, &regions, &fz, &order, &multi, &fb, &counters, &flag,
                       &rf, &gpool, &gmeta, &sbits, &hist, &rois, &rows, &staged, &frameMap, &fitNext, &fitHand})
         d->release();
@@ -353,7 +356,7 @@ size_t Engine::workspace_bytes(uint64_t planes, int H, int W, uint64_t maxPerPla
     const size_t SF = std::max<size_t>(1, std::min<size_t>(F, std::max<size_t>(64, (size_t(1) << 24) / npx)));
     size_t t = LaneSizes(F, npx).bytes(F, npx);                                   // first main lane
     if (planes > F && opt.lanes > 1) t += LaneSizes(F, npx).bytes(F, npx);       // second main lane
-    if (opt.deferFlagged) t += LaneSizes(SF, npx).bytes(SF, npx) + SF * 4;      // side lane + frame map
+    if (opt.deferFlagged && planes > F) t += LaneSizes(SF, npx).bytes(SF, npx) + SF * 4;   // side lane + frame map
     const uint64_t roiCap = std::min<uint64_t>(outCapRows, planes * std::min<uint64_t>(maxPerPlane, npx / 4 + 2));
     // ROI store, side ROI store (grown by half again), fit rows, staged rows (grown by half again),
     // hand-over list, plane bases and placement plan, the call's small buffers (counters, fit words).
@@ -640,6 +643,8 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
     uint32_t* meta = main_.host(2 * metaWords);
     size_t chunkNo = 0;
     std::vector<uint64_t> chunkZ0;
+    std::vector<bool> chunkInLine;
+    double recentFlagged = 0.0;    // flagged fraction of the last chunk whose counters were read
     auto read_meta = [&](size_t c) {
         PT_CHECK(hipEventSynchronize(chunkEv_[c & 1]));
         const uint32_t* m = meta + (c & 1) * metaWords;
@@ -667,9 +672,11 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         if (ctr[kStatus]) throw Error("segmentation status " + std::to_string(ctr[kStatus]));
         totalRois = m[kNumCounters];
         const uint32_t* fl = m + kNumCounters + 1;
-        if (opt.deferFlagged && chunkZ0[c] + F < planes)    // deferred (not the last chunk)
+        const uint64_t fc = std::min<uint64_t>(F, planes - chunkZ0[c]);
+        if (!chunkInLine[c])
             for (uint32_t k = 0; k < std::min<uint32_t>(ctr[kFlagCnt], (uint32_t)F); ++k)
                 deferred.push_back((uint32_t)(chunkZ0[c] + fl[k]));
+        recentFlagged = (double)ctr[kFlagCnt] / (double)std::max<uint64_t>(fc, 1);
     };
     run_.ensure(16);
     uint32_t* dRun = run_.as<uint32_t>();
@@ -688,11 +695,16 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         L.b.runTotal = dRun;
         L.b.scanWait = scanEv_[(chunkNo + 1) & 1];
         L.b.scanDone = scanEv_[chunkNo & 1];
-        // Each chunk resolves its flagged frames itself (exact frame flood in line), unless they are
-        // deferred to the side lane; the last chunk always does (a side batch started after it would
-        // only end after the fit of the call).
+        // The flagged frames of a chunk are resolved in line (exact frame flood on the chunk's lane) or
+        // deferred to the side lane. The last chunk resolves its own (a side batch started after it
+        // would only end after the fit of the call); the others follow defer_flagged: never, always,
+        // or (2) when the last chunk whose counters are known flagged at most 2 % of its frames, so
+        // that sparse stacks pay for the exact flood once per side batch and dense ones do not
+        // detect their frames twice.
         const bool lastChunk = z0 + F >= planes;
-        detect_chunk(L, d_in + z0 * npx, f, H, W, thr, ratio, maxPerPlane, 3 * chunkNo, nullptr, lastChunk || !opt.deferFlagged);
+        const bool inLine = lastChunk || opt.deferFlagged == 0 || (opt.deferFlagged == 2 && recentFlagged > 0.02);
+        chunkInLine.push_back(inLine);
+        detect_chunk(L, d_in + z0 * npx, f, H, W, thr, ratio, maxPerPlane, 3 * chunkNo, nullptr, inLine);
         L.b.runTotal = nullptr;
         L.b.scanWait = L.b.scanDone = nullptr;
         gather_rois(L.rois.as<Roi>(), (uint32_t)z0, allRois_.as<Roi>(), L.b.roiBase, (uint32_t)f, L.b.chunkBase,

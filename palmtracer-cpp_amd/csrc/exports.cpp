@@ -219,7 +219,7 @@ PT_API int PT_CALL pt_set_option(const char* name, double value)
         else if (n == "fit_handover") e.opt.fitHandover = (int)std::max(0.0, std::min(1e6, value));
         else if (n == "batch_autotune") e.opt.batchAutotune = value != 0.0;
         else if (n == "batch_graph") e.opt.batchGraph = value != 0.0;
-        else if (n == "defer_flagged") e.opt.deferFlagged = value != 0.0;
+        else if (n == "defer_flagged") e.opt.deferFlagged = (int)std::max(0.0, std::min(2.0, value));
         else if (n == "detect_rows") e.opt.detectRows = (int)std::max(0.0, std::min(1048576.0, value));
         else if (n == "lanes") e.opt.lanes = value >= 2.0 ? 2 : 1;
         else throw pt::Error("unknown option '" + n + "'");

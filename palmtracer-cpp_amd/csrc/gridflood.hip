@@ -25,7 +25,7 @@
 //   checks    T < G, equal keys (below), labels by walking the first-popped-neighbour links;
 //   write     labels of the frames that passed; the others get their seed labels back and go to
 //             the single-workgroup floods.
-// Frames that do not fit the pool are given a second round once the first round's frames are done.
+// Frames that do not fit the pool go to the single-workgroup floods.
 //
 // Equal keys. The heap pops equal keys in an order set by its internal layout; the solution above
 // breaks them by entry order. A frame with equal keys is accepted when no order can change a label,
@@ -692,12 +692,17 @@ __global__ void __launch_bounds__(GF_NT) gf_write_kernel(GfArgs a)
 
 size_t grid_pool_words(size_t F, size_t npx)
 {
-    // About 8 bytes per pixel of the chunk (a frame needs 84 bytes per mask pixel), 64 MiB to 1 GiB.
-    const size_t bytes = std::clamp<size_t>(F * npx * 8, size_t(64) << 20, size_t(1) << 30);
+    // About 10 bytes per pixel of the chunk (a frame needs 84 bytes per mask pixel: masks of up to
+    // 12 % of every frame of a chunk), 64 MiB to 1.5 GiB.
+    const size_t bytes = std::clamp<size_t>(F * npx * 10, size_t(64) << 20, size_t(3) << 29);
     return bytes / 4;
 }
 
-void launch_grid_flood(const DetectBuffers& b, int W, int H, int F, const Fz* fz, hipStream_t s)
+namespace {
+
+// The launch sequence of the grid rank flood over the flagged frames of a chunk of up to F frames
+// (the flagged count is read on the device).
+void grid_flood_sequence(const DetectBuffers& b, int W, int H, int F, const Fz* fz, hipStream_t s)
 {
     const size_t npx = b.npx;
     const unsigned bx = (unsigned)std::clamp<size_t>(npx / 16384, 2, 64);
@@ -708,32 +713,72 @@ void launch_grid_flood(const DetectBuffers& b, int W, int H, int F, const Fz* fz
     const size_t maxN = std::min<size_t>(npx, b.gridPoolWords / kGfWordsPerEntry);
     int passes = 0;
     for (size_t w = GF_TILE; w < maxN; w <<= 1) ++passes;
-    for (int round = 0; round < 2; ++round) {
-        GfArgs a{b.P, b.seedBits, b.fin, b.maskList, b.maskCnt, round ? b.gridRetry : b.flagList,
-                 round ? kGridRetryCnt : kFlagCnt, round, b.counters, b.gridList, b.gridRetry,
-                 reinterpret_cast<GfMeta*>(b.gridMeta), b.gridPool, b.gridPoolWords, fz, b.aux, b.regionCap, npx, W, H, F};
-        gf_setup_kernel<<<1, 1024, 0, s>>>(a);
-        gf_reset_kernel<<<g, GF_NT, 0, s>>>(a);
-        gf_collect_kernel<<<g, GF_NT, 0, s>>>(a);
-        gf_tile_sort_kernel<<<g, GF_NT, 0, s>>>(a);
-        for (int p = 0; p < passes; ++p) gf_merge_kernel<<<g, GF_NT, 0, s>>>(a, p);
-        gf_finalize_kernel<<<g, GF_NT, 0, s>>>(a);
-        gf_neighbours_kernel<<<g, GF_NT, 0, s>>>(a);
-        for (int it = 0; it < kGridMaxIter; ++it) {
-            gf_age_kernel<<<g, GF_NT, 0, s>>>(a, it);
-            gf_rank_kernel<<<g, GF_NT, 0, s>>>(a, it);
-        }
-        gf_check1_kernel<<<g, GF_NT, 0, s>>>(a);
-        gf_check2_kernel<<<g, GF_NT, 0, s>>>(a);
-        gf_ival_init_kernel<<<g, GF_NT, 0, s>>>(a);
-        for (int it = 0; it < kGridIvalIter; ++it) {
-            gf_ival_key_kernel<<<g, GF_NT, 0, s>>>(a, it);
-            gf_ival_rank_kernel<<<g, GF_NT, 0, s>>>(a, it);
-        }
-        gf_ival_check_kernel<<<g, GF_NT, 0, s>>>(a);
-        gf_write_kernel<<<g, GF_NT, 0, s>>>(a);
+    GfArgs a{b.P, b.seedBits, b.fin, b.maskList, b.maskCnt, b.flagList, kFlagCnt, 1, b.counters, b.gridList, b.gridRetry,
+             reinterpret_cast<GfMeta*>(b.gridMeta), b.gridPool, b.gridPoolWords, fz, b.aux, b.regionCap, npx, W, H, F};
+    gf_setup_kernel<<<1, 1024, 0, s>>>(a);
+    gf_reset_kernel<<<g, GF_NT, 0, s>>>(a);
+    gf_collect_kernel<<<g, GF_NT, 0, s>>>(a);
+    gf_tile_sort_kernel<<<g, GF_NT, 0, s>>>(a);
+    for (int p = 0; p < passes; ++p) gf_merge_kernel<<<g, GF_NT, 0, s>>>(a, p);
+    gf_finalize_kernel<<<g, GF_NT, 0, s>>>(a);
+    gf_neighbours_kernel<<<g, GF_NT, 0, s>>>(a);
+    for (int it = 0; it < kGridMaxIter; ++it) {
+        gf_age_kernel<<<g, GF_NT, 0, s>>>(a, it);
+        gf_rank_kernel<<<g, GF_NT, 0, s>>>(a, it);
     }
-    PT_CHECK(hipGetLastError());
+    gf_check1_kernel<<<g, GF_NT, 0, s>>>(a);
+    gf_check2_kernel<<<g, GF_NT, 0, s>>>(a);
+    gf_ival_init_kernel<<<g, GF_NT, 0, s>>>(a);
+    for (int it = 0; it < kGridIvalIter; ++it) {
+        gf_ival_key_kernel<<<g, GF_NT, 0, s>>>(a, it);
+        gf_ival_rank_kernel<<<g, GF_NT, 0, s>>>(a, it);
+    }
+    gf_ival_check_kernel<<<g, GF_NT, 0, s>>>(a);
+    gf_write_kernel<<<g, GF_NT, 0, s>>>(a);
+}
+
+} // namespace
+
+void GridFloodGraph::release()
+{
+    for (hipGraphExec_t& e : exec) {
+        if (e) (void)hipGraphExecDestroy(e);
+        e = nullptr;
+    }
+}
+
+void launch_grid_flood(const DetectBuffers& b, int W, int H, int F, const Fz* fz, hipStream_t s)
+{
+    // Sized by the lane's capacity, so that one graph serves chunks (and side batches) of any
+    // number of frames.
+    const int Fcap = (int)std::max<size_t>(b.frames, (size_t)F);
+    GridFloodGraph* cache = b.gridGraph;
+    if (!cache) {
+        grid_flood_sequence(b, W, H, Fcap, fz, s);
+        PT_CHECK(hipGetLastError());
+        return;
+    }
+    const int pass = fz ? 1 : 0;
+    const uint64_t key[24] = {(uint64_t)(uintptr_t)b.P, (uint64_t)(uintptr_t)b.seedBits, (uint64_t)(uintptr_t)b.fin,
+                              (uint64_t)(uintptr_t)b.maskList, (uint64_t)(uintptr_t)b.maskCnt, (uint64_t)(uintptr_t)b.flagList,
+                              (uint64_t)(uintptr_t)b.counters, (uint64_t)(uintptr_t)b.gridList, (uint64_t)(uintptr_t)b.gridRetry,
+                              (uint64_t)(uintptr_t)b.gridMeta, (uint64_t)(uintptr_t)b.gridPool, b.gridPoolWords,
+                              (uint64_t)(uintptr_t)fz, (uint64_t)(uintptr_t)b.aux, b.regionCap, b.npx, (uint64_t)W,
+                              (uint64_t)H, (uint64_t)Fcap, (uint64_t)(uintptr_t)s, 1, 0, 0, 0};
+    bool same = cache->exec[pass] != nullptr;
+    for (int i = 0; i < 24 && same; ++i) same = cache->key[pass][i] == key[i];
+    if (!same) {
+        if (cache->exec[pass]) PT_CHECK(hipGraphExecDestroy(cache->exec[pass]));
+        cache->exec[pass] = nullptr;
+        hipGraph_t graph = nullptr;
+        PT_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        grid_flood_sequence(b, W, H, Fcap, fz, s);
+        PT_CHECK(hipStreamEndCapture(s, &graph));
+        PT_CHECK(hipGraphInstantiate(&cache->exec[pass], graph, nullptr, nullptr, 0));
+        PT_CHECK(hipGraphDestroy(graph));
+        for (int i = 0; i < 24; ++i) cache->key[pass][i] = key[i];
+    }
+    PT_CHECK(hipGraphLaunch(cache->exec[pass], s));
 }
 
 } // namespace pt

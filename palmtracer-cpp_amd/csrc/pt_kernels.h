@@ -42,6 +42,14 @@ struct PixAux {
     uint32_t rslot;       // region slot of a root seed
 };
 
+// The grid rank flood's launch sequence of one lane, captured once per watershed pass as a hipGraph
+// and replayed while the lane's buffers and shapes stay the same (key).
+struct GridFloodGraph {
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    uint64_t key[2][24] = {};
+    void release();
+};
+
 // Buffers of one chunk of F frames of npx pixels. Per-pixel arrays are F*npx long; per-frame
 // lists live in frame segments of capacity npx.
 struct DetectBuffers {
@@ -73,6 +81,7 @@ struct DetectBuffers {
     uint32_t* gridMeta = nullptr; // [F * kGridMetaWords] grid rank flood state per flagged frame
     uint32_t* gridPool = nullptr; // grid rank flood work space (gridPoolWords u32)
     size_t gridPoolWords = 0;
+    GridFloodGraph* gridGraph = nullptr;   // optional: replay the grid rank flood from a captured graph
     uint32_t* seedBits = nullptr; // per frame: bitmap of the local maxima, seed_words(npx) u32
     uint32_t* hist = nullptr;     // per frame: kHistBins counts of mask pixels at or above each level
     uint32_t* runTotal = nullptr; // optional: ROIs of the call so far (shared by the lanes of a call)
@@ -124,8 +133,8 @@ __host__ __device__ inline size_t seed_words(size_t npx) { return (npx + 31) / 3
 constexpr uint32_t kRankMax = 262144;
 // Grid rank flood: fixed-point steps, interval steps (frames with equal keys), state words per
 // flagged frame.
-constexpr int kGridMaxIter = 8;
-constexpr int kGridIvalIter = 6;
+constexpr int kGridMaxIter = 6;
+constexpr int kGridIvalIter = 4;
 constexpr size_t kGridMetaWords = 32;
 constexpr uint32_t kRankLds = 16384;
 __host__ __device__ inline size_t rf_stride(size_t npx)

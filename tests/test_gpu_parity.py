@@ -432,11 +432,13 @@ def test_exact_floods_bit_exact(pt_gpu, oracle_mod, ref_inputs, ref_vectors, nam
         pt_gpu.set_option("force_rank_flood", 0)
 
 
-@pytest.mark.parametrize("defer", [1, 0])
+@pytest.mark.parametrize("defer", [1, 0, 2])
 def test_localization_with_all_frames_exact(pt_gpu, oracle_mod, frames512, defer):
-    """Localization with every frame through the exact frame flood: the flagged frames of the first
-    chunks deferred to the side stream (defer 1) or resolved in line (defer 0), those of the last chunk
-    resolved in line; rows unchanged."""
+    """Localization with every frame through the exact frame flood (chunks of 5 frames): the flagged
+    frames of the first chunks deferred to the side stream (defer 1), resolved in line (defer 0), or
+    deferred until the counters of a chunk show more than 2 % flagged frames (defer 2: chunks 0 and 1
+    are queued before any counters are read); those of the last chunk resolved in line; rows
+    unchanged."""
     st, thr = frames512
     exp = oracle_mod.localization(st, thr, 0.0, 0, FIT_PARAMS_SYN)
     pt_gpu.set_option("force_sequential_watershed", 1)
@@ -448,10 +450,10 @@ def test_localization_with_all_frames_exact(pt_gpu, oracle_mod, frames512, defer
     finally:
         pt_gpu.set_option("force_sequential_watershed", 0)
         pt_gpu.set_option("chunk_pixels", 0)
-        pt_gpu.set_option("defer_flagged", 0)
+        pt_gpu.set_option("defer_flagged", 2)
     last = st.shape[0] - 5 * ((st.shape[0] - 1) // 5)
     assert stats["flagged_frames"] == st.shape[0]
-    assert stats["deferred_frames"] == (st.shape[0] - last if defer else 0)
+    assert stats["deferred_frames"] == {1: st.shape[0] - last, 0: 0, 2: 10}[defer]
     assert got.shape == exp.shape and np.array_equal(got, exp)
 
 
