@@ -392,6 +392,35 @@ def batch_leg(pt, torch, frame_scale, steps, per_frame_sample):
             "stack_sides": [st["side"] for st in stacks], "stack_frames": [st["frames"] for st in stacks]}
 
 
+def host_path_leg(pt, torch, stack, thr, fit_params, max_per_plane, steps):
+    """The reference's own entry point over host buffers: Localization (CPU/main.cpp:112-121) of the
+    whole stack from pageable host memory into a host output, streamed through the device in pieces
+    (pinned staging, uploads and downloads overlapping the computation). End to end, PCIe included."""
+    import numpy as np
+    F, H, W = stack.shape
+    host = stack.cpu().numpy()
+    out = np.empty(F * max_per_plane * 18, np.float64)
+    params = np.asarray(fit_params, np.float64)
+    L = pt.lib()
+
+    def call():
+        return L.Localization(host.ctypes.data, out.ctypes.data, out.size, H, W, F, thr, 0.0, 2, params.ctypes.data)
+
+    call()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        n = call()
+    dt = (time.perf_counter() - t0) / steps
+    st = pt.stage_times()
+    rows = out[:n].reshape(-1, 18)
+    fitted = int((rows[:, 13] > 0).sum())
+    gb = F * H * W * 2 / 1e9
+    return {"entry": "Localization (host buffers)", "frames": F, "steps": steps, "ms_per_call": dt * 1e3,
+            "localizations_per_s": fitted / dt, "fitted_rows": fitted, "input_gb": gb, "input_gb_per_s": gb / dt,
+            "h2d_ms": st["h2d_ms"], "d2h_ms": st["d2h_ms"], "output_gb": n * 8 / 1e9, "pieces": int(st["stream_pieces"]),
+            "note": "h2d/d2h: device time of the copies of the last call, overlapped with the computation"}
+
+
 def compare_fits(a, e, sigma_y=False):
     """Fast-fit rows `a` against FP64 rows `e` of the same frames: rows paired within each plane by
     position, largest errors over rows fitted on both sides."""
@@ -441,6 +470,8 @@ def main():
     ap.add_argument("--astig-frames", type=int, default=1000,
                     help="frames of the astigmatic 3D leg (2048^2 sCMOS; 0: skip it)")
     ap.add_argument("--astig-steps", type=int, default=3)
+    ap.add_argument("--host-steps", type=int, default=2,
+                    help="calls of the host-buffer Localization over the whole stack (0 skips the leg)")
     ap.add_argument("--batch-frame-scale", type=float, default=0.1,
                     help="scale of the batched workload's frame counts (U[50,2000] per stack); 0 skips the leg")
     ap.add_argument("--batch-steps", type=int, default=2)
@@ -565,6 +596,8 @@ def main():
 
     precision = fit_precision(pt, torch, stack, thr, fit_params, min(F, 200)) if args.precision == "fp32" else None
 
+    host_path = host_path_leg(pt, torch, stack, thr, fit_params, max_per_plane, args.host_steps) if args.host_steps > 0 else None
+
     serving = None
     if args.serve_seconds > 0:
         sizes = [int(v) for v in args.serve_sizes.split(",")]
@@ -609,6 +642,7 @@ def main():
         "roofline": roof,
         "roofline_fit": roof_fit,
         "fit_precision": precision,
+        "host_path": host_path,
         "astig3d": astig,
         "batched": batched,
         "serving": serving,

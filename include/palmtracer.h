@@ -40,6 +40,8 @@
  * fit_params = [roi, sigmaX, sigmaY, theta] for 1-4, [roi, spline sizes x, y, z, deltaZ (nm), then
  * 64 * x * y * z coefficients, monomial-major] for 5. watershedRatio <= 0 keeps the
  * maxima separate (reference value 0); (int)ratio >= 1 enables basin fusion.
+ * The stack goes through the device in pieces of at most "stream_pixels" pixels (pinned staging,
+ * uploads and downloads overlapping the computation), so it may be larger than HBM.
  * Returns the number of doubles written. */
 PT_API uint64_t PT_CALL Localization(const uint16_t* input, double* output, uint64_t outputSize, uint64_t height,
                                      uint64_t width, uint64_t planes, double thresholdVal, double watershedRatio,
@@ -48,7 +50,8 @@ PT_API uint64_t PT_CALL Localization(const uint16_t* input, double* output, uint
 /* Replaces Fitting (CPU/main.cpp:141-148 -> lib/Palm.cpp:73-119). positions: nPoints rows
  * [Id, Plane(1-based), X, Y, SegIntensity, Surface] (lib/Constants.hpp:56-64). Rows are written in
  * input order; outputSize is ignored as in the reference. fit in 1..5, fit_params as for
- * Localization. Returns 18 * nPoints. */
+ * Localization. The stack is streamed through the device in pieces as in Localization (each piece
+ * with the positions of its planes). Returns 18 * nPoints. */
 PT_API uint64_t PT_CALL Fitting(const uint16_t* input, const double* positions, uint64_t nPoints, double* output,
                                 uint64_t outputSize, uint64_t height, uint64_t width, uint64_t planes, uint64_t fit,
                                 const double* fit_params);
@@ -164,7 +167,9 @@ PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint6
  * persistent FP32 fit, 0: default), "fit_refill" (free lanes of an FP32 fit wave that start a refill
  * round, 1-64, default 16), "detect_rows" (rows per detection wave, 0: chosen from the chunk),
  * "lanes" (2: consecutive chunks alternate between two streams; 1: one stream), "batch_autotune" (1: tune
- * new fit classes of pt_batch_localization), "batch_graph" (1: its fit stage as a hipGraph), "defer_flagged"
+ * new fit classes of pt_batch_localization), "batch_graph" (1: its fit stage as a hipGraph), "stream_pixels"
+ * (host Localization / Fitting: pixels per piece of the stack, default 2^27), "host_threads" (host
+ * threads of the copies to and from pinned staging, default 8), "defer_flagged"
  * (flagged frames of all but the last chunk re-detected and resolved on the side stream: 0 never, 1
  * always, 2 (default) when the last chunk whose counters are known flagged at most 2 % of its frames). Returns 1 on
  * success. */
@@ -186,8 +191,10 @@ PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n);
  * segmentation, fit, assembly; HIP events on the library's main stream, summed over chunks), the
  * chunk count, the LM passes of the reference's schedule (model + Jacobian, trial) of the per-lane
  * fit kernel, the device milliseconds of the deferred-frame batches on the side stream (overlapped
- * with the main stream), and the LM passes (model + Jacobian, trial) of the FP64 group kernel.
- * Writes min(n, 10) values, returns 10. */
+ * with the main stream), the LM passes (model + Jacobian, trial) of the FP64 group kernel, and for
+ * the host entry points the device milliseconds of the frame uploads and row downloads and the
+ * number of pieces the stack went through the device in (whatever the "profile" option).
+ * Writes min(n, 13) values, returns 13. */
 PT_API int PT_CALL pt_stage_times(double* out, uint64_t n);
 
 /* Batched localization of many stacks (no counterpart in the reference, whose callers run Localization

@@ -44,6 +44,8 @@ struct RunStats {
     uint64_t jacPasses = 0, trialPasses = 0;            // per-lane fit kernel (FP32 or FP64)
     uint64_t groupJacPasses = 0, groupTrialPasses = 0;  // cooperative FP64 group kernel
     uint64_t fitHanded = 0;   // fast fit of the main lanes: ROIs handed from the FP32 kernel to the group kernel
+    double msH2D = 0, msD2H = 0;   // host entry points: device time of the frame uploads and row downloads
+    uint64_t streamPieces = 0;     // host entry points: pieces the stack went through the device in
 };
 
 struct Options {
@@ -62,6 +64,8 @@ struct Options {
     int lanes = 2;                  // main lanes the chunks alternate between (1: one stream, for timing)
     bool batchAutotune = true;      // batched localization: tune each new fit class's schedule
     bool batchGraph = true;         // batched localization: the fit stage as a captured hipGraph
+    size_t streamPixels = size_t(1) << 27;   // host entry points: pixels per piece of the stack
+    int hostThreads = 8;            // host entry points: threads of the copies to and from pinned staging
     int deferFlagged = 2;           // flagged frames of all but the last chunk on the side lane: 0 never,
                                     // 1 always, 2 when the last known chunk flagged at most 2 % of its frames
     // Defaults with the environment's overrides (PALMTRACER_FIT_PRECISION, PALMTRACER_MAX_ITER,
@@ -153,6 +157,13 @@ public:
     // Fit-only over caller positions (6 columns each), rows in input order.
     uint64_t fit_positions(const uint16_t* d_in, uint64_t planes, int H, int W, const double* hostPos, uint64_t n,
                            uint64_t fit, const double* fitParams, double* d_out);
+    // The same two over host buffers, the stack streamed through the device in pieces of at most
+    // opt.streamPixels pixels (pinned staging, uploads and downloads overlapping the computation).
+    // Return doubles written, like localize / fit_positions.
+    uint64_t localize_host(const uint16_t* input, double* output, uint64_t outputSize, int H, int W, uint64_t planes,
+                           double thr, double ratio, uint64_t fit, const double* fitParams);
+    uint64_t fit_positions_host(const uint16_t* input, uint64_t planes, int H, int W, const double* positions, uint64_t n,
+                                uint64_t fit, const double* fitParams, double* output);
     void wavelet(const uint16_t* d_in, uint64_t planes, int H, int W, int level, double* d_out);
     // Label map of one frame after the watershed (labels = first-maximum raster index + 1).
     void debug_labels(const uint16_t* d_frame, int H, int W, double thr, double ratio, uint32_t* d_labels);
@@ -177,6 +188,15 @@ private:
     hipEvent_t chunkEv_[2] = {nullptr, nullptr};  // a main-lane chunk's counters are on the host
     hipEvent_t scanEv_[2] = {nullptr, nullptr};   // a chunk's ROI scan is done (orders the running total)
     DevBuf run_;                                    // ROI running total of the call (device)
+    // Host entry points: staging of the streamed pieces (two each way), their copy stream and events.
+    void stream_prepare(size_t inBytes, size_t outBytes);
+    void stream_release();
+    hipStream_t copyStream_ = nullptr;
+    void* pinIn_[2] = {nullptr, nullptr};
+    void* pinOut_[2] = {nullptr, nullptr};
+    size_t pinInBytes_ = 0, pinOutBytes_ = 0;
+    DevBuf streamIn_[2], streamOut_[2];
+    hipEvent_t inEv_[2] = {}, outEv_[2] = {}, inStartEv_[2] = {}, outStartEv_[2] = {}, computeEv_[2] = {};
     std::vector<hipEvent_t> ev_;
     hipEvent_t event(size_t i);
     // Detection and segmentation of F frames queued on lane L's stream (chunk frame f = input frame

@@ -183,6 +183,7 @@ Engine::~Engine()
     for (Lane* L : {&main_, &mainB_, &side_})
         if (L->stream) (void)hipStreamSynchronize(L->stream);
     for (Lane* L : {&main_, &mainB_, &side_}) L->release();
+    stream_release();
     for (DevBuf* b : {&in, &out, &planeInfo, &atMask_, &atPart_, &atRows_, &passes_, &allRois_, &allBase_, &sideRois_, &sideBase_,
                       &run_, &splineRaw_, &splineCv_})
         b->release();
@@ -372,7 +373,8 @@ size_t Engine::device_bytes() const
     for (const Lane* L : {&main_, &mainB_, &side_}) t += L->device_bytes();
     for (const DevBuf* b : {&in, &out, &planeInfo, &atMask_, &atPart_, &atRows_, &passes_, &allRois_, &allBase_, &sideRois_,
                             &sideBase_, &run_, &splineRaw_, &splineCv_, &bRois_, &bRows_, &bStaged_, &bStacks_, &bJobs_,
-                            &bBases_, &bPlan_, &bCounts_, &bHand_, &bWords_})
+                            &bBases_, &bPlan_, &bCounts_, &bHand_, &bWords_, &streamIn_[0], &streamIn_[1], &streamOut_[0],
+                            &streamOut_[1]})
         t += b->bytes;
     return t;
 }
@@ -385,7 +387,8 @@ void Engine::release_device_buffers()
     }
     for (DevBuf* b : {&in, &out, &planeInfo, &atMask_, &atPart_, &atRows_, &passes_, &allRois_, &allBase_, &sideRois_,
                       &sideBase_, &run_, &splineRaw_, &splineCv_, &bRois_, &bRows_, &bStaged_, &bStacks_, &bJobs_,
-                      &bBases_, &bPlan_, &bCounts_, &bHand_, &bWords_})
+                      &bBases_, &bPlan_, &bCounts_, &bHand_, &bWords_, &streamIn_[0], &streamIn_[1], &streamOut_[0],
+                      &streamOut_[1]})
         b->release();
     if (batchGraphExec_) PT_CHECK(hipGraphExecDestroy(batchGraphExec_));
     batchGraphExec_ = nullptr;

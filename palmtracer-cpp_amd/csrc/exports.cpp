@@ -48,18 +48,9 @@ PT_API uint64_t PT_CALL Localization(const uint16_t* input, double* output, uint
     return guarded("Localization", [&]() -> uint64_t {
         Engine& e = Engine::get();
         std::lock_guard<std::mutex> lk(e.mu);
-        const uint64_t maxPer = planes == 0 ? 0 : (outputSize / pt::kCols) / planes;  // Palm.cpp:51
-        if (maxPer == 0 || height == 0 || width == 0) return 0;
-        const size_t npx = (size_t)height * width;
-        HostToDevice h2d(e.in, input, npx * planes, e.stream);
-        const uint64_t capRows = outputSize / pt::kCols;
-        const uint64_t rowsCap = std::min<uint64_t>(capRows, planes * std::min<uint64_t>(maxPer, npx / 4 + 2));
-        e.out.ensure(std::max<uint64_t>(rowsCap, 1) * pt::kCols * 8);
-        const uint64_t n = e.localize(h2d.dptr, planes, (int)height, (int)width, thresholdVal, watershedRatio, fit,
-                                      fit_params, maxPer, e.out.as<double>(), rowsCap);
-        if (n) PT_CHECK(hipMemcpyAsync(output, e.out.p, n * 8, hipMemcpyDeviceToHost, e.stream));
-        PT_CHECK(hipStreamSynchronize(e.stream));
-        return n;
+        if (height > 0x7FFFFFFFull || width > 0x7FFFFFFFull) throw pt::Error("frame too large");
+        return e.localize_host(input, output, outputSize, (int)height, (int)width, planes, thresholdVal, watershedRatio, fit,
+                               fit_params);
     }, 0);
 }
 
@@ -72,14 +63,8 @@ PT_API uint64_t PT_CALL Fitting(const uint16_t* input, const double* positions, 
         if (nPoints == 0) return 0;
         Engine& e = Engine::get();
         std::lock_guard<std::mutex> lk(e.mu);
-        const size_t npx = (size_t)height * width;
-        HostToDevice h2d(e.in, input, npx * planes, e.stream);
-        e.out.ensure(nPoints * pt::kCols * 8);
-        const uint64_t n = e.fit_positions(h2d.dptr, planes, (int)height, (int)width, positions, nPoints, fit,
-                                           fit_params, e.out.as<double>());
-        PT_CHECK(hipMemcpyAsync(output, e.out.p, n * 8, hipMemcpyDeviceToHost, e.stream));
-        PT_CHECK(hipStreamSynchronize(e.stream));
-        return n;
+        if (height > 0x7FFFFFFFull || width > 0x7FFFFFFFull) throw pt::Error("frame too large");
+        return e.fit_positions_host(input, planes, (int)height, (int)width, positions, nPoints, fit, fit_params, output);
     }, 0);
 }
 
@@ -219,6 +204,8 @@ PT_API int PT_CALL pt_set_option(const char* name, double value)
         else if (n == "fit_handover") e.opt.fitHandover = (int)std::max(0.0, std::min(1e6, value));
         else if (n == "batch_autotune") e.opt.batchAutotune = value != 0.0;
         else if (n == "batch_graph") e.opt.batchGraph = value != 0.0;
+        else if (n == "stream_pixels") e.opt.streamPixels = value >= 1.0 ? (size_t)value : (size_t(1) << 27);
+        else if (n == "host_threads") e.opt.hostThreads = (int)std::max(1.0, std::min(64.0, value));
         else if (n == "defer_flagged") e.opt.deferFlagged = (int)std::max(0.0, std::min(2.0, value));
         else if (n == "detect_rows") e.opt.detectRows = (int)std::max(0.0, std::min(1048576.0, value));
         else if (n == "lanes") e.opt.lanes = value >= 2.0 ? 2 : 1;
@@ -243,11 +230,12 @@ PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n)
 PT_API int PT_CALL pt_stage_times(double* out, uint64_t n)
 {
     Engine& e = Engine::get();
-    const double v[10] = {e.last.msAtrous, e.last.msSegment, e.last.msFit, e.last.msAssemble, (double)e.last.chunks,
+    const double v[13] = {e.last.msAtrous, e.last.msSegment, e.last.msFit, e.last.msAssemble, (double)e.last.chunks,
                           (double)e.last.jacPasses, (double)e.last.trialPasses, e.last.msDeferred,
-                          (double)e.last.groupJacPasses, (double)e.last.groupTrialPasses};
-    for (uint64_t i = 0; i < n && i < 10; ++i) out[i] = v[i];
-    return 10;
+                          (double)e.last.groupJacPasses, (double)e.last.groupTrialPasses, e.last.msH2D, e.last.msD2H,
+                          (double)e.last.streamPieces};
+    for (uint64_t i = 0; i < n && i < 13; ++i) out[i] = v[i];
+    return 13;
 }
 
 } // extern "C"

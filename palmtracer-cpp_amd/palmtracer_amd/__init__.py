@@ -188,10 +188,10 @@ def last_stats():
 
 def stage_times():
     """Per-stage device milliseconds of the last Localization (needs set_option('profile', 1))."""
-    v = (ctypes.c_double * 10)()
-    lib().pt_stage_times(v, 10)
+    v = (ctypes.c_double * 13)()
+    lib().pt_stage_times(v, 13)
     keys = ["atrous_ms", "segment_ms", "fit_ms", "assemble_ms", "chunks", "jacobian_passes", "trial_passes",
-            "deferred_ms", "group_jacobian_passes", "group_trial_passes"]
+            "deferred_ms", "group_jacobian_passes", "group_trial_passes", "h2d_ms", "d2h_ms", "stream_pieces"]
     return dict(zip(keys, list(v)))
 
 
