@@ -197,6 +197,18 @@ PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n);
  * Writes min(n, 13) values, returns 13. */
 PT_API int PT_CALL pt_stage_times(double* out, uint64_t n);
 
+/* Cramer-Rao lower bounds of Gaussian fits (fit 1-4; no counterpart in the reference's Gaussian path,
+ * whose spline fit states them: lib/fit/SplineFit.cpp:168-205). d_rows: nRows rows of 18 doubles on
+ * the device (Localization / Fitting output of frames height x width with this fit and fit_params).
+ * camera (host, 3 doubles; NULL: 1, 0, 0): gain (ADU per photon), offset (ADU) and read-noise
+ * variance (ADU^2) of the noise model var = gain * (model - offset) + readVar. d_out: nRows x 8
+ * doubles on the device, the bounds of X, Y, SigmaX, SigmaY, Theta, Amplitude, Offset, Intensity in
+ * the units of those columns (FP64 Fisher information of the fitted model over the ROI centred on
+ * the fitted pixel, inverted by Cholesky); NaN where the mode does not fit the parameter or the row's
+ * fit failed. Returns 1 on success, 0 on error. */
+PT_API int PT_CALL pt_crlb_device(const double* d_rows, uint64_t nRows, uint64_t height, uint64_t width, uint64_t fit,
+                                  const double* fit_params, const double* camera, double* d_out);
+
 /* Batched localization of many stacks (no counterpart in the reference, whose callers run Localization
  * once per stack). Each stack: device input (planes x height x width u16), threshold, watershed ratio,
  * fit mode 0-4 with its fit_params (host, [roi, sigmaX, sigmaY, theta]), device output of

@@ -227,3 +227,85 @@ def ref_fitting(stack, positions, fit=2, fit_params=(7, 1.3, 1.3, 0)):
     n = R.ref_fitting(s.ctypes.data, pos.ctypes.data, pos.shape[0], out.ctypes.data, out.size, h, w, z, int(fit),
                       p.ctypes.data)
     return out[:n].reshape(-1, NCOL).copy()
+
+
+def gauss_model(mode, p, dx, dy, sigma, theta):
+    """Gaussian model of the fits (lib/fit/GaussianModel.cpp:19-52) over pixel offsets dx, dy from the
+    centre, parameters p = [offset, amplitude, x, y, (sigma | sqrt2 sigmaX, sqrt2 sigmaY, (theta))]
+    (the centre enters through dx, dy: callers pass pixel - p[2], pixel - p[3])."""
+    if mode <= 2:
+        s = sigma if mode == 1 else p[4]
+        return p[0] + p[1] * np.exp(-(dx * dx + dy * dy) / (2.0 * s * s))
+    th = theta if mode == 3 else p[6]
+    c, s = np.cos(th), np.sin(th)
+    ix2, iy2 = 1.0 / (p[4] * p[4]), 1.0 / (p[5] * p[5])
+    ax, ay, axy = c * c * ix2 + s * s * iy2, s * s * ix2 + c * c * iy2, 2.0 * c * s * (iy2 - ix2)
+    return p[0] + p[1] * np.exp(-(ax * dx * dx + ay * dy * dy + axy * dx * dy))
+
+
+def crlb(rows, height, width, fit, fit_params, camera=(1.0, 0.0, 0.0)):
+    """Cramer-Rao bounds of Gaussian-fit rows (the quantities of the library's pt_crlb_device): the
+    Fisher information sum g g^T / var over the fit-side ROI centred on the fitted pixel, with g the
+    model gradient taken here by central finite differences (not the library's analytic gradient),
+    var = gain * (model - offset) + readVar; bounds sqrt(diag(F^-1)) mapped to the row's columns
+    [X, Y, SigmaX, SigmaY, Theta, Amplitude, Offset, Intensity], intensity by the delta method."""
+    rows = np.asarray(rows, np.float64).reshape(-1, NCOL)
+    roi, sig, _, th = float(fit_params[0]), float(fit_params[1]), float(fit_params[2]), float(fit_params[3])
+    half = int(roi) // 2
+    gain, off_cam, rvar = camera
+    npar = {1: 4, 2: 5, 3: 6, 4: 7}[fit]
+    out = np.full((len(rows), 8), np.nan)
+    ii, jj = np.meshgrid(np.arange(-half, half + 1), np.arange(-half, half + 1), indexing="ij")
+    for r, row in enumerate(rows):
+        X, Y, amp, off = row[4], row[5], row[13], row[14]
+        if not (amp > 0) or not np.isfinite(X) or not np.isfinite(Y):
+            continue
+        fx, fy = np.floor(X), np.floor(Y)
+        if fx - half < 0 or fx + half >= width or fy - half < 0 or fy + half >= height:
+            continue
+        p = np.array([off, amp, X, Y, 0.0, 0.0, 0.0][:npar])
+        if fit == 2:
+            p[4] = row[8]
+        if fit >= 3:
+            p[4], p[5] = np.sqrt(2.0) * row[8], np.sqrt(2.0) * row[9]
+        if fit == 4:
+            p[6] = row[10]
+        px, py = fx + jj, fy + ii
+
+        def model(q):
+            return gauss_model(fit, q, px - q[2], py - q[3], sig, th)
+
+        g = np.empty((npar,) + px.shape)
+        for k in range(npar):
+            h = 1e-6 * max(1.0, abs(p[k]))
+            a, b = p.copy(), p.copy()
+            a[k] += h
+            b[k] -= h
+            g[k] = (model(a) - model(b)) / (2 * h)
+        circ = fit == 4 and abs(p[4] - p[5]) <= 2.220446049250313e-16
+        if circ:
+            g[6] = 0.0
+        var = np.maximum(gain * np.maximum(model(p) - off_cam, 0.0) + rvar, 1e-12)
+        F = np.einsum("aij,bij->ab", g, g / var)
+        if circ:
+            F[6, 6] = 1.0
+        try:
+            np.linalg.cholesky(F)
+        except np.linalg.LinAlgError:
+            continue
+        Fi = np.linalg.inv(F)
+        o = out[r]
+        o[0], o[1], o[5], o[6] = np.sqrt(Fi[2, 2]), np.sqrt(Fi[3, 3]), np.sqrt(Fi[1, 1]), np.sqrt(Fi[0, 0])
+        d = np.zeros(npar)
+        if fit == 1:
+            d[1] = 2 * np.pi * sig * sig
+        if fit == 2:
+            o[2] = o[3] = np.sqrt(Fi[4, 4])
+            d[1], d[4] = 2 * np.pi * p[4] ** 2, 4 * np.pi * amp * p[4]
+        if fit >= 3:
+            o[2], o[3] = np.sqrt(Fi[4, 4]) / np.sqrt(2.0), np.sqrt(Fi[5, 5]) / np.sqrt(2.0)
+            d[1], d[4], d[5] = np.pi * p[4] * p[5], np.pi * amp * p[5], np.pi * amp * p[4]
+        if fit == 4:
+            o[4] = np.nan if circ else np.sqrt(Fi[6, 6])
+        o[7] = np.sqrt(max(d @ Fi @ d, 0.0))
+    return out

@@ -72,6 +72,8 @@ def lib():
         L.pt_astig_estimate_device.argtypes = [_vp, _vp, _u64, _f64, _vp, _f64]
         L.pt_fitting_noalloc.restype = _u64
         L.pt_fitting_noalloc.argtypes = [_vp, _vp, _u64, _vp, _u64, _u64, _u64, _u64, _vp]
+        L.pt_crlb_device.restype = _i32
+        L.pt_crlb_device.argtypes = [_vp, _u64, _u64, _u64, _u64, _vp, _vp, _vp]
         L.pt_batch_localization.restype = _i32
         L.pt_batch_localization.argtypes = [ctypes.POINTER(BatchStack), _u64]
         L.pt_batch_schedules.restype = _i32
@@ -231,6 +233,18 @@ def localization_device(d_stack, d_out, threshold, watershed_ratio=0.0, fit=2, f
     n = lib().pt_localization_device(d_stack.data_ptr(), d_out.data_ptr(), cap, h, w, z, float(threshold),
                                      float(watershed_ratio), int(fit), p.ctypes.data, mpp)
     return n // NCOL
+
+
+def crlb_device(d_rows, height, width, fit, fit_params, d_out, camera=(1.0, 0.0, 0.0)):
+    """pt_crlb_device: bounds of X, Y, SigmaX, SigmaY, Theta, Amplitude, Offset, Intensity (n x 8 into the
+    float64 tensor d_out) for the fitted rows in the float64 tensor d_rows (n x 18)."""
+    n = d_rows.numel() // NCOL
+    p = _params(fit_params)
+    cam = np.asarray(camera, np.float64)
+    if not lib().pt_crlb_device(d_rows.data_ptr(), n, height, width, int(fit), p.ctypes.data, cam.ctypes.data,
+                                d_out.data_ptr()):
+        raise RuntimeError("pt_crlb_device failed (see stderr)")
+    return d_out
 
 
 def batch_localization(stacks):
