@@ -171,8 +171,11 @@ PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint6
  * (host Localization / Fitting: pixels per piece of the stack, default 2^27), "host_threads" (host
  * threads of the copies to and from pinned staging, default 8), "defer_flagged"
  * (flagged frames of all but the last chunk re-detected and resolved on the side stream: 0 never, 1
- * always, 2 (default) when the last chunk whose counters are known flagged at most 2 % of its frames). Returns 1 on
- * success. */
+ * always, 2 (default) when the last chunk whose counters are known flagged at most 2 % of its frames),
+ * "fit_objective" (0: the reference's least squares; 1: Poisson maximum likelihood of fit modes 1-4,
+ * FP64, an extension with no counterpart in the reference), "camera_gain" (ADU per photon, 1),
+ * "camera_offset" (ADU, 0) and "camera_read_var" (ADU^2, 0): the camera model of that likelihood.
+ * Returns 1 on success. */
 PT_API int PT_CALL pt_set_option(const char* name, double value);
 
 /* Counters of the last call: frames, chunks, rows, frames flagged for the exact frame flood,

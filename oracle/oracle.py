@@ -309,3 +309,75 @@ def crlb(rows, height, width, fit, fit_params, camera=(1.0, 0.0, 0.0)):
             o[4] = np.nan if circ else np.sqrt(Fi[6, 6])
         o[7] = np.sqrt(max(d @ Fi @ d, 0.0))
     return out
+
+
+def mle_fit(frame, fit, fit_params, x, y, camera=(1.0, 0.0, 0.0)):
+    """Poisson maximum-likelihood fit of one ROI (the library's fit objective 1): the reference's ROI,
+    border rule, initial values and bounds (GaussianFit.cpp:51-56, Fitting.cpp:126-175), the
+    objective sum mu - d log mu with mu = (model - offset) / gain + rv, d = max(0, (v - offset) /
+    gain) + rv, rv = readVar / gain^2, minimised by scipy's bounded L-BFGS-B with numerical gradients
+    (an optimiser independent of the library's Fisher-scoring LM). Returns [offset, amplitude, x, y,
+    sigmaX, sigmaY, theta] in the row's units, or None for the border case."""
+    from scipy.optimize import minimize
+    img = np.asarray(frame, np.float64)
+    H, W = img.shape
+    roi, sig, sigy, th = int(fit_params[0]), float(fit_params[1]), float(fit_params[2]), float(fit_params[3])
+    if roi % 2 == 0:
+        roi -= 1
+    half = roi // 2
+    ux, uy = int(x), int(y)
+    if half > W or half > H or half > ux or ux >= W - half or half > uy or uy >= H - half:
+        return None
+    patch = img[uy - half:uy + half + 1, ux - half:ux + half + 1]
+    gain, c0, rvar = camera
+    rv = rvar / (gain * gain)
+    yy, xx = np.mgrid[uy - half:uy + half + 1, ux - half:ux + half + 1].astype(np.float64)
+    d = np.maximum(0.0, (patch - c0) / gain) + rv
+    p = [patch.min(), patch.max() - patch.min(), x, y]
+    lo, hi = [0.0, 0.0, x - 1, y - 1], [65535.0, 65535.0, x + 1, y + 1]
+    if fit == 2:
+        p += [sig]; lo += [0.1]; hi += [20.0]
+    if fit >= 3:
+        p += [np.sqrt(2) * sig, np.sqrt(2) * sigy]; lo += [0.1, 0.1]; hi += [20.0, 20.0]
+    if fit == 4:
+        p += [th]; lo += [-150.0]; hi += [150.0]
+    p = np.array(p)
+    if p[0] <= c0:
+        p[0] = c0 + 1e-3 * gain
+    # The likelihood is defined where every mu > 0; with amplitude >= 0 that is offset > c0 - rv gain,
+    # so the box keeps the optimiser inside the domain (the library rejects steps that leave it).
+    lo[0] = max(lo[0], c0 - rv * gain + 1e-6 * gain)
+
+    # The Poisson deviance sum mu - d - d log(mu / d): the same minimiser as sum mu - d log mu, but of
+    # the order of the pixel count, so that finite-difference gradients keep their precision.
+    dlogd = np.where(d > 0, d * np.log(np.where(d > 0, d, 1.0)), 0.0)
+
+    def nll(q):
+        mu = (gauss_model(fit, q, xx - q[2], yy - q[3], sig, th) - c0) / gain + rv
+        if np.any(mu <= 0):
+            return 1e300
+        return float(np.sum(mu - d - np.where(d > 0, d * np.log(mu), 0.0) + dlogd))
+
+    scale = np.maximum(np.abs(p), 1.0)
+    bounds = list(zip(np.array(lo) / scale, np.array(hi) / scale))
+    z = p / scale
+    for _ in range(3):  # restarts from the last point until L-BFGS-B stops moving
+        res = minimize(lambda v: nll(v * scale), z, method="L-BFGS-B", jac="3-point", bounds=bounds,
+                       options={"ftol": 0.0, "gtol": 1e-9, "maxiter": 3000})
+        moved = np.abs(res.x - z).max()
+        z = res.x
+        if moved < 1e-9:
+            break
+    q = z * scale
+    sx = sy = sig
+    t = th
+    if fit == 1:
+        t = 0.0
+    if fit == 2:
+        sx = sy = q[4]
+        t = 0.0
+    if fit >= 3:
+        sx, sy = q[4] / np.sqrt(2), q[5] / np.sqrt(2)
+    if fit == 4:
+        t = q[6]
+    return np.array([q[0], q[1], q[2], q[3], sx, sy, t])

@@ -70,8 +70,9 @@ __global__ void plane_bases_kernel(const double* __restrict__ rows, uint64_t n, 
 
 std::string class_key(const FitParams& fp)
 {
-    char b[160];
-    std::snprintf(b, sizeof(b), "%d/%d/%.17g/%.17g/%.17g/%d", fp.mode, fp.roi, fp.sigmaX, fp.sigmaY, fp.theta, fp.maxIter);
+    char b[256];
+    std::snprintf(b, sizeof(b), "%d/%d/%.17g/%.17g/%.17g/%d/%d/%.17g/%.17g/%.17g", fp.mode, fp.roi, fp.sigmaX, fp.sigmaY,
+                  fp.theta, fp.maxIter, fp.objective, fp.camGain, fp.camOffset, fp.camReadVar);
     return b;
 }
 

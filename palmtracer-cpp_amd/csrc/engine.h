@@ -60,6 +60,8 @@ struct Options {
     unsigned fitWaves = 0;          // persistent FP32 fit: resident waves (0: 16 per CU)
     int fitRefill = 16;             // persistent FP32 fit: free lanes that trigger a refill round
     int fitHandover = 16;           // fast fit: FP32 iterations before a ROI moves to the FP64 group kernel (0: never)
+    int fitObjective = 0;           // Gaussian fits: 0 least squares (reference), 1 Poisson likelihood
+    double camGain = 1.0, camOffset = 0.0, camReadVar = 0.0;   // camera model of the Poisson likelihood
     int detectRows = 0;             // rows per detection wave (0: chosen from the chunk size)
     int lanes = 2;                  // main lanes the chunks alternate between (1: one stream, for timing)
     bool batchAutotune = true;      // batched localization: tune each new fit class's schedule

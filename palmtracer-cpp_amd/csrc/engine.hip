@@ -484,6 +484,10 @@ FitParams gaussian_fit_params(uint64_t fit, const double* fp, const Options& opt
     p.maxIter = opt.maxIter;
     p.refill = opt.fitRefill;
     p.handover = opt.fitHandover;
+    p.objective = opt.fitObjective;
+    p.camGain = opt.camGain;
+    p.camOffset = opt.camOffset;
+    p.camReadVar = opt.camReadVar;
     return p;
 }
 

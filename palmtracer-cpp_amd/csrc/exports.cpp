@@ -204,6 +204,10 @@ PT_API int PT_CALL pt_set_option(const char* name, double value)
         else if (n == "fit_handover") e.opt.fitHandover = (int)std::max(0.0, std::min(1e6, value));
         else if (n == "batch_autotune") e.opt.batchAutotune = value != 0.0;
         else if (n == "batch_graph") e.opt.batchGraph = value != 0.0;
+        else if (n == "fit_objective") e.opt.fitObjective = value != 0.0 ? 1 : 0;
+        else if (n == "camera_gain") e.opt.camGain = value;
+        else if (n == "camera_offset") e.opt.camOffset = value;
+        else if (n == "camera_read_var") e.opt.camReadVar = std::max(0.0, value);
         else if (n == "stream_pixels") e.opt.streamPixels = value >= 1.0 ? (size_t)value : (size_t(1) << 27);
         else if (n == "host_threads") e.opt.hostThreads = (int)std::max(1.0, std::min(64.0, value));
         else if (n == "defer_flagged") e.opt.deferFlagged = (int)std::max(0.0, std::min(2.0, value));

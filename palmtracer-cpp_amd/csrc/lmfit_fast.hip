@@ -481,6 +481,10 @@ void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d
         launch_spline_fit(d_frames, npx, W, H, d_rois, nRoi, *fp.spline, d_rows, s, frameMap, d_count);
         return;
     }
+    if (fp.objective == 1 && fp.mode >= 1 && fp.mode <= 4) {
+        launch_fit_mle(d_frames, npx, W, H, d_rois, nRoi, fp, d_rows, s, frameMap, d_count);
+        return;
+    }
     if (fp64 || fp.mode < 1 || fp.mode > 4) {
         launch_fit_f64(d_frames, npx, W, H, d_rois, nRoi, fp, d_rows, s, d_passes, frameMap, d_count);
         return;

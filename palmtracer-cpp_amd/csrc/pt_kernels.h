@@ -197,6 +197,10 @@ struct FitParams {
     // Batched fits (device table, optional): ROI r reads stack stacks[r.slot], frame r.frame; the
     // launch's frames / npx / W / H are then unused.
     const StackRef* stacks = nullptr;
+    // Objective of modes 1-4: 0 least squares (the reference's), 1 Poisson likelihood with the camera
+    // model ADU = camOffset + camGain * photons (+ read noise of variance camReadVar, ADU^2).
+    int objective = 0;
+    double camGain = 1.0, camOffset = 0.0, camReadVar = 0.0;
 };
 // Device work space of a fast fit launch: kFitWords queue words ([0] FP32 queue head, [1] ROIs handed
 // to the FP64 group kernel, [2] group queue head) and the hand-over list (one word per ROI).
@@ -212,6 +216,9 @@ struct FitScratch {
 // fast fit's work space (nRoi hand-over words). d_passes (optional, 4 words) accumulates {Jacobian
 // passes, trial passes} of the per-lane kernel, then of the group kernel; frameMap maps ROI frames
 // to stack frames. With d_count set, the ROI count is read on the device (nRoi is then an upper bound).
+// Poisson-likelihood fit of modes 1-4 (FP64, one ROI per lane; fp.objective == 1).
+void launch_fit_mle(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, uint32_t nRoi, const FitParams& fp,
+                    double* d_rows, hipStream_t s, const uint32_t* frameMap, const uint32_t* d_count);
 void launch_fit(const uint16_t* d_frames, size_t npx, int W, int H, const Roi* d_rois, uint32_t nRoi, const FitParams& fp,
                 double* d_rows, bool fp64, const FitScratch& ws, hipStream_t s, unsigned long long* d_passes = nullptr,
                 const uint32_t* frameMap = nullptr, unsigned waves = 0, const uint32_t* d_count = nullptr);

@@ -49,3 +49,21 @@ def test_oracle_crlb_failed_and_border_rows(oracle_mod):
     c = oracle_mod.crlb(rows, H, W, 2, (7, 1.3, 1.3, 0.0), (2.0, 100.0, 0.0))
     assert np.isfinite(c[0, [0, 1, 2, 3, 5, 6, 7]]).all()
     assert np.isnan(c[1]).all() and np.isnan(c[2]).all()
+
+
+def test_oracle_mle_reaches_crlb(oracle_mod):
+    """The Poisson maximum-likelihood fit is efficient: its spread over 400 realisations equals the
+    bound within sampling error (band [0.88, 1.12]), which pins both the likelihood and the bound."""
+    X0, Y0, amp, off = 10.3, 10.6, 800.0, 140.0
+    p = np.array([off, amp, X0, Y0, 1.3])
+    yy, xx = np.mgrid[0:H, 0:W].astype(float)
+    lam = (oracle_mod.gauss_model(2, p, xx - X0, yy - Y0, 1.3, 0.0) - 100.0) / 2.0
+    rng = np.random.default_rng(21)
+    fits = np.array([oracle_mod.mle_fit((100 + 2 * rng.poisson(lam)).astype(np.uint16), 2, (7, 1.3, 1.3, 0.0), 10.0, 11.0,
+                                        (2.0, 100.0, 0.0)) for _ in range(400)])
+    truth = np.zeros((1, 18))
+    truth[0, [4, 5, 13, 14, 8, 9]] = X0, Y0, amp, off, 1.3, 1.3
+    c = oracle_mod.crlb(truth, H, W, 2, (7, 1.3, 1.3, 0.0), (2.0, 100.0, 0.0))[0]
+    for k, col in ((0, 2), (1, 3), (2, 4), (5, 1), (6, 0)):
+        ratio = fits[:, col].std() / c[k]
+        assert 0.88 <= ratio <= 1.12, (k, ratio)
