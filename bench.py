@@ -580,6 +580,22 @@ def main():
             "bytes_per_launch": bytes_l}
     stage_ms = {k: v / args.steps for k, v in stages.items() if k.endswith("_ms")}
 
+    # The detection launches alone: one main lane, so that no other lane's kernels share the GPU with
+    # them (the timed steps overlap two lanes, and their events include that sharing).
+    pt.set_option("lanes", 1)
+    pt.localization_device(stack, out, thr, 0.0, 2, fit_params, max_per_plane)
+    alone_ms, alone_launches = 0.0, 0.0
+    for _ in range(2):
+        pt.localization_device(stack, out, thr, 0.0, 2, fit_params, max_per_plane)
+        st = pt.stage_times()
+        alone_ms += st["atrous_ms"]
+        alone_launches += st["chunks"]
+    pt.set_option("lanes", options.get("lanes", 2.0))
+    torch.cuda.synchronize()
+    alone = alone_ms / max(alone_launches, 1.0)
+    roof["standalone_launch_ms"] = alone
+    roof["standalone_frac"] = bytes_l / (alone * 1e-3) / 1e9 / HBM_PEAK_GBS
+
     # The same workload at the reference's own iteration cap (75), fitted rows per second.
     alt_iter = 75 if args.max_iter != 75 else 20
     pt.set_option("max_iter", alt_iter)
