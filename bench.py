@@ -28,10 +28,13 @@ sys.path.insert(0, os.path.join(ROOT, "palmtracer-cpp_amd"))
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
-# Algorithmic work (per unit): a trous detection reads the u16 pixel and writes the 4-byte
-# thresholded W1 potential; the 7x7 / 5-parameter LM pass is 49 * 60 FLOP with the Jacobian and
-# alpha/beta accumulation, 49 * 15 FLOP for a trial model evaluation.
-ATROUS_BYTES_PER_PX = 6.0
+# Algorithmic work (per unit): a trous detection reads the u16 pixel, and for each mask pixel
+# (thresholded potential above zero) writes its 4-byte potential and its 4-byte entry of the frame's
+# mask list (the seed list and bitmap add a few bytes per local maximum, left out); the 7x7 /
+# 5-parameter LM pass is 49 * 60 FLOP with the Jacobian and alpha/beta accumulation, 49 * 15 FLOP for
+# a trial model evaluation.
+ATROUS_READ_BYTES_PER_PX = 2.0
+ATROUS_BYTES_PER_MASK_PX = 8.0
 JAC_FLOP_7x7 = 2940.0
 TRIAL_FLOP_7x7 = 735.0
 
@@ -571,13 +574,14 @@ def main():
                 "group_trial_passes": stages["group_trial_passes"] / args.steps,
                 "rois_handed_to_fp64": last["fit_handed_over"]}
     # The dominant kernel by device time: the detection kernel (HBM-bound).
-    bytes_l = ATROUS_BYTES_PER_PX * px_per_launch
+    mask_per_launch = last["mask_pixels"] / max(stages["chunks"] / args.steps, 1.0)
+    bytes_l = ATROUS_READ_BYTES_PER_PX * px_per_launch + ATROUS_BYTES_PER_MASK_PX * mask_per_launch
     ach = bytes_l / (atrous_ms * 1e-3) / 1e9
     tpp = measured_traffic_per_px(S)
     roof = {"bound": "hbm", "kernel": "atrous_detect_kernel", "achieved": ach, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
             "traffic": None if tpp is None else tpp * px_per_launch, "launch_ms": atrous_ms,
-            "bytes_per_launch": bytes_l}
+            "bytes_per_launch": bytes_l, "mask_pixels_per_launch": mask_per_launch}
     stage_ms = {k: v / args.steps for k, v in stages.items() if k.endswith("_ms")}
 
     # The detection launches alone: one main lane, so that no other lane's kernels share the GPU with

@@ -186,8 +186,9 @@ PT_API int PT_CALL pt_set_option(const char* name, double value);
  * solving both orders of an equal-key pair, ROIs of the fast fit fitted by the FP64 group kernel
  * (main lanes), frames the grid rank flood resolved, of which frames whose flood has equal keys that
  * were shown not to change a label, and the frames it handed to the single-workgroup floods: not
- * settled, equal keys that might change a label, other (work space, push order). Writes min(n, 23)
- * values, returns 23. */
+ * settled, equal keys that might change a label, other (work space, push order), and the mask pixels
+ * (thresholded potential above zero) of the main lanes' detections. Writes min(n, 24) values,
+ * returns 24. */
 PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n);
 
 /* With the "profile" option, device milliseconds of the last Localization per stage (a trous,

@@ -154,7 +154,10 @@ __global__ void __launch_bounds__(NT) atrous_plane_kernel(const uint16_t* __rest
 // with compile-time slots, and after it they are back in their shifted layout, so rows never move
 // between registers.
 //
-//   P[frame*npx + i]           dense, every pixel: 65536 * W1 if >= tmin, else 0
+//   P[frame*npx + i]           65536 * W1 where it is >= tmin (the mask pixels), written there only:
+//                              the caller keeps P zero at every other pixel (it clears the mask
+//                              pixels of a chunk once the chunk's segmentation is done), so the
+//                              kernel writes 4 bytes per mask pixel instead of per pixel.
 //   maskList / seedList        per-frame compact lists (capacity npx each, any order) of pixel
 //                              indices with P > 0 / local maxima (Segmentation2D.cpp:294-333
 //                              test: no 8-neighbour inside the frame strictly greater), counts in
@@ -199,7 +202,6 @@ __device__ inline uint32_t b3s(uint32_t m2, uint32_t m1, uint32_t c, uint32_t p1
 
 __device__ inline uint32_t max3u(uint32_t a, uint32_t b, uint32_t c) { return max(max(a, b), c); }
 
-typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
 
 // Number of set bits of m in the lanes below this one.
 __device__ inline uint32_t lanes_below(uint64_t m)
@@ -220,11 +222,7 @@ __device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
 }
 
-// Five waves per SIMD (91 VGPRs): each lane stores its three P columns as one 12-byte vector (a
-// wave's row is one contiguous write) and input rows are loaded two ahead; at six waves (80 VGPRs)
-// the vector store spills. Against six waves with three 4-byte stores per lane and one row of
-// prefetch, the standalone launch is ~19 % shorter (strided 4-byte stores of a row each touched
-// every cache line of the row partially).
+// Five waves per SIMD, input rows loaded two ahead.
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) atrous_detect_kernel(const uint16_t* __restrict__ in, uint32_t* __restrict__ P,
                                                            uint32_t* __restrict__ maskList,
                                                            uint32_t* __restrict__ maskCnt, uint32_t* __restrict__ seedList,
@@ -265,11 +263,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) at
         pastR[k] = x > W - 1;
     }
     const uint32_t outOff0 = (uint32_t)x0 * 4u;   // used only for columns in the frame
-    // A lane whose three columns are all written stores them as one 12-byte vector, so that the
-    // wave's stores of a row are one contiguous 768-byte write; only the lane holding the frame's
-    // last column can have some columns written and others not.
-    const bool outAll = outCol[0] && outCol[1] && outCol[2];
-    const bool outPart = outCol[0] && !outAll;
     // The frame's first column is strip column SHALO of strip 0 (lane SHALO / SC, slot 0); its last
     // column is strip column oR of the last strip.
     const bool fixL = xs < 0;
@@ -312,14 +305,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) at
     auto emit = [&](int y3, const uint32_t (&v)[SC], const uint32_t (&cm)[SC]) {
         const uint32_t rowBase = (uint32_t)y3 * (uint32_t)W;
         const int rowOff = (int)(rowBase * 4u);
-        if (outAll) {
-            const u32x3 pv = {v[0], v[1], v[2]};
-            __builtin_amdgcn_raw_buffer_store_b96(pv, rsP, (int)outOff0, rowOff, 0);
-        }
-        if (fixR && outPart) {
-            __builtin_amdgcn_raw_buffer_store_b32(v[0], rsP, (int)outOff0, rowOff, 0);
-            if (outCol[1]) __builtin_amdgcn_raw_buffer_store_b32(v[1], rsP, (int)(outOff0 + 4u), rowOff, 0);
-        }
         // Pixels with P > 0 (columns outside the frame have P = 0) and the local maxima among them.
         const uint32_t cL = from_left(cm[2]), cR = from_right(cm[0]);
         const bool m0 = outLane && v[0] != 0u, m1 = outLane && v[1] != 0u, m2 = outLane && v[2] != 0u;
@@ -335,6 +320,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) at
         const uint32_t mc0 = (uint32_t)__popcll(mb0), mc1 = (uint32_t)__popcll(mb1);
         const uint32_t mRow = mc0 + mc1 + (uint32_t)__popcll(mb2);
         if (mRow == 0) return;
+        // P of the mask pixels only: the rest of the frame's P is zero already.
+        if (m0) __builtin_amdgcn_raw_buffer_store_b32(v[0], rsP, (int)outOff0, rowOff, 0);
+        if (m1) __builtin_amdgcn_raw_buffer_store_b32(v[1], rsP, (int)(outOff0 + 4u), rowOff, 0);
+        if (m2) __builtin_amdgcn_raw_buffer_store_b32(v[2], rsP, (int)(outOff0 + 8u), rowOff, 0);
         const uint32_t i0 = rowBase + (uint32_t)x0, i1 = i0 + 1u, i2 = i0 + 2u;
         if (mPend + mRow > (uint32_t)MCAP) { flush(sMask, mPend, &maskCnt[f], ml); mPend = 0; }
         if (m0) sMask[mPend + lanes_below(mb0)] = i0;

@@ -26,6 +26,20 @@ __global__ void zero3_kernel(uint32_t* a, size_t na, uint32_t* b, size_t nb, uin
     }
 }
 
+// Zeroes P at the mask pixels of frames 0..F-1 (frames strided in y, list entries in x).
+// Their number is added to counters[kMaskPx].
+__global__ void clear_mask_kernel(uint32_t* __restrict__ P, const uint32_t* __restrict__ maskList,
+                                  const uint32_t* __restrict__ maskCnt, uint32_t F, size_t npx, uint32_t* __restrict__ counters)
+{
+    for (uint32_t f = blockIdx.y; f < F; f += gridDim.y) {
+        const uint32_t n = maskCnt[f];
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[kMaskPx], n);
+        const uint32_t* ml = maskList + (size_t)f * npx;
+        uint32_t* Pf = P + (size_t)f * npx;
+        for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) Pf[ml[k]] = 0u;
+    }
+}
+
 // Fit jobs from caller rows [Id, Plane, X, Y, SegIntensity, Surface] (lib/Palm.cpp:105-109).
 __global__ void positions_to_rois(const double* __restrict__ pos, uint64_t n, uint64_t planes, Roi* __restrict__ rois,
                                   uint32_t* __restrict__ bad)
@@ -252,7 +266,13 @@ void Lane::prepare(size_t F, size_t npx)
 {
     const LaneSizes z(F, npx);
     const size_t n = z.n;
+    void* oldP = P.p;
     P.ensure(n * 4);
+    if (P.p != oldP || pDirty) {
+        // A new buffer, or one whose last detection was not cleared: zero it whole.
+        PT_CHECK(hipMemsetAsync(P.p, 0, P.bytes, stream));
+        pDirty = false;
+    }
     lab.ensure(n * 4);
     fin.ensure(n * 4);
     aux.ensure(n * sizeof(PixAux));
@@ -325,6 +345,16 @@ void Lane::zero()
     zero3_kernel<<<1024, 256, 0, stream>>>(b.maskCnt, 6 * (b.frames + 1), b.counters, kNumCounters, b.seedBits,
                                            b.frames * seed_words(b.npx));
     PT_CHECK(hipGetLastError());
+}
+
+void Lane::clear_mask(size_t F, size_t npx)
+{
+    if (F == 0) return;
+    const unsigned gx = (unsigned)std::max<size_t>(1, std::min<size_t>(64, npx / 65536));
+    const unsigned gy = (unsigned)std::min<size_t>(F, 4096);
+    clear_mask_kernel<<<dim3(gx, gy), 256, 0, stream>>>(b.P, b.maskList, b.maskCnt, (uint32_t)F, npx, b.counters);
+    PT_CHECK(hipGetLastError());
+    pDirty = false;
 }
 
 size_t Lane::device_bytes() const
@@ -508,13 +538,20 @@ void Engine::detect_chunk(Lane& L, const uint16_t* d_frames, size_t F, int H, in
     const hipStream_t st = L.stream;
     L.zero();
     const bool prof = opt.profile && evBase != kNoEvents;
+    if (L.pDirty) {
+        PT_CHECK(hipMemsetAsync(L.P.p, 0, L.P.bytes, st));
+        L.pDirty = false;
+    }
     if (prof) PT_CHECK(hipEventRecord(event(evBase + 0), st));
+    L.pDirty = true;
     launch_atrous_detect(d_frames, L.b, W, H, (int)F, threshold_to_pmin(thr), st, d_map, opt.detectRows);
     if (prof) PT_CHECK(hipEventRecord(event(evBase + 1), st));
     L.rois.ensure(F * (npx / 4 + 2) * sizeof(Roi));
     // Deferred frames (d_map set) go through the exact frame flood.
     const bool forceSeq = opt.forceSequential || d_map != nullptr;
     launch_segment(L.b, W, H, (int)F, maxPoints, ratio, L.rois.as<Roi>(), st, forceSeq, opt.exactPath, resolve);
+    // The segmentation kernels were the last readers of this chunk's P.
+    L.clear_mask(F, npx);
     if (prof) PT_CHECK(hipEventRecord(event(evBase + 2), st));
 }
 
@@ -605,6 +642,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         last.gridUnsettled += ctr[kGridUnsettled];
         last.gridRejected += ctr[kGridRejected];
         last.gridOther += ctr[kGridOther];
+        last.maskPixels += ctr[kMaskPx];
         last.floodMaxQueue = std::max<uint64_t>(last.floodMaxQueue, ctr[kFloodMaxQueue]);
         if (ctr[kStatus]) throw Error("segmentation status " + std::to_string(ctr[kStatus]));
         if (opt.profile) {
@@ -673,6 +711,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         last.gridUnsettled += ctr[kGridUnsettled];
         last.gridRejected += ctr[kGridRejected];
         last.gridOther += ctr[kGridOther];
+        last.maskPixels += ctr[kMaskPx];
         last.floodMaxQueue = std::max<uint64_t>(last.floodMaxQueue, ctr[kFloodMaxQueue]);
         last.status |= ctr[kStatus];
         if (ctr[kStatus] & 4u) throw Error("output capacity exceeded");
@@ -866,12 +905,14 @@ void Engine::debug_labels(const uint16_t* d_frame, int H, int W, double thr, dou
     const size_t npx = (size_t)W * H;
     main_.prepare(1, npx);
     main_.zero();
+    main_.pDirty = true;
     launch_atrous_detect(d_frame, main_.b, W, H, 1, threshold_to_pmin(thr), stream);
     main_.rois.ensure((npx / 4 + 2) * sizeof(Roi));
     launch_segment(main_.b, W, H, 1, ~0ull, ratio, main_.rois.as<Roi>(), stream, opt.forceSequential, opt.exactPath);
     // Labels only where P > 0 (fin is not written elsewhere).
     masked_labels<<<256, 256, 0, stream>>>(main_.b.fin, main_.b.P, d_labels, npx);
     PT_CHECK(hipGetLastError());
+    main_.clear_mask(1, npx);
     uint32_t ctr[kNumCounters];
     PT_CHECK(hipMemcpyAsync(ctr, main_.b.counters, sizeof(ctr), hipMemcpyDeviceToHost, stream));
     PT_CHECK(hipStreamSynchronize(stream));

@@ -221,14 +221,14 @@ PT_API int PT_CALL pt_set_option(const char* name, double value)
 PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n)
 {
     Engine& e = Engine::get();
-    const uint64_t v[23] = {e.last.frames, e.last.chunks, e.last.rois, e.last.flaggedFrames, e.last.multiComponents,
+    const uint64_t v[24] = {e.last.frames, e.last.chunks, e.last.rois, e.last.flaggedFrames, e.last.multiComponents,
                             e.last.deferredFrames, e.last.status, e.last.flagOverflow, e.last.flagUnproven,
                             e.last.floodPops, e.last.floodMaxQueue, e.last.rfTooBig, e.last.rfTie,
                             e.last.rfUnsettled, e.last.fbOverflow, e.last.heapFrames, e.last.rfTwoOrders,
                             e.last.fitHanded, e.last.gridFrames, e.last.gridTies, e.last.gridUnsettled,
-                            e.last.gridRejected, e.last.gridOther};
-    for (uint64_t i = 0; i < n && i < 23; ++i) out[i] = v[i];
-    return 23;
+                            e.last.gridRejected, e.last.gridOther, e.last.maskPixels};
+    for (uint64_t i = 0; i < n && i < 24; ++i) out[i] = v[i];
+    return 24;
 }
 
 PT_API int PT_CALL pt_stage_times(double* out, uint64_t n)

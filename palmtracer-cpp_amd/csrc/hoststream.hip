@@ -182,7 +182,7 @@ uint64_t Engine::localize_host(const uint16_t* input, double* output, uint64_t o
         const RunStats& l = last;
         total.frames += l.frames; total.chunks += l.chunks; total.rois += l.rois; total.flaggedFrames += l.flaggedFrames;
         total.multiComponents += l.multiComponents; total.deferredFrames += l.deferredFrames; total.status |= l.status;
-        total.gridFrames += l.gridFrames; total.fitHanded += l.fitHanded;
+        total.gridFrames += l.gridFrames; total.fitHanded += l.fitHanded; total.maskPixels += l.maskPixels;
         total.msAtrous += l.msAtrous; total.msSegment += l.msSegment; total.msFit += l.msFit; total.msAssemble += l.msAssemble;
         total.jacPasses += l.jacPasses; total.trialPasses += l.trialPasses;
         total.groupJacPasses += l.groupJacPasses; total.groupTrialPasses += l.groupTrialPasses;

@@ -112,7 +112,8 @@ enum Counter : int {
     kGridRejected = 21,  // grid rank flood: frames handed over on equal keys that might matter
     kGridOther = 22,     // grid rank flood: frames handed over for lack of work space or T >= G
     kGridRetryCnt = 23,  // grid rank flood: frames left to its second round (work space)
-    kNumCounters = 24
+    kMaskPx = 24,        // mask pixels of the chunk (P > 0), added when they are cleared
+    kNumCounters = 25
 };
 // kStatus bits: 1 = region slots exhausted, 2 = unlabelled mask pixel (internal error).
 

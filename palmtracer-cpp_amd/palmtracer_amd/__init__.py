@@ -179,12 +179,13 @@ def set_option(name, value):
 
 
 def last_stats():
-    v = (ctypes.c_uint64 * 23)()
-    lib().pt_last_stats(v, 23)
+    v = (ctypes.c_uint64 * 24)()
+    lib().pt_last_stats(v, 24)
     keys = ["frames", "chunks", "rows", "flagged_frames", "multi_components", "deferred_frames", "status",
             "flag_overflow", "flag_unproven", "wave_flood_pops", "wave_flood_max_queue", "rank_too_big",
             "rank_equal_keys", "rank_unsettled", "wave_queue_overflow", "heap_frames", "rank_two_orders",
-            "fit_handed_over", "grid_frames", "grid_equal_keys", "grid_unsettled", "grid_rejected", "grid_other"]
+            "fit_handed_over", "grid_frames", "grid_equal_keys", "grid_unsettled", "grid_rejected", "grid_other",
+            "mask_pixels"]
     return dict(zip(keys, list(v)))
 
 
