@@ -120,15 +120,16 @@ struct Lane {
     DevBuf P, lab, fin, aux, mask, seed, perFrame, regions, fz, order, multi, fb, counters, flag, rf, gpool, gmeta, sbits, hist;
     DevBuf rois, rows, staged, frameMap, fitNext, fitHand;
     GridFloodGraph gridGraph;    // the grid rank flood of this lane as a captured graph
-    // P is zero at every pixel between chunks: the detection kernel writes the mask pixels only, and
-    // clear_mask() zeroes them once the chunk's segmentation is queued. pDirty: a detection was
-    // queued whose mask pixels have not been cleared (then prepare() zeroes the whole buffer).
+    // P and the seed bitmap are zero everywhere between chunks: the detection kernel writes the mask
+    // pixels and the local maxima only, and clear_mask() zeroes them once the chunk's segmentation is
+    // queued. pDirty: a detection was queued whose pixels have not been cleared (then prepare() and
+    // the next detection zero both buffers whole).
     bool pDirty = false;
     uint32_t* pinned = nullptr;  // host-visible: frame map, roiBase and counters of an asynchronous batch
     size_t pinnedWords = 0;
     void prepare(size_t F, size_t npx);
     void zero();
-    void clear_mask(size_t F, size_t npx);   // zeroes P at the mask pixels of the last detection
+    void clear_mask(size_t F, size_t npx);   // zeroes what the last detection set in P and the seed bitmap
     // Work space of a fast fit of up to nRoi ROIs on this lane (queue words and hand-over list).
     FitScratch fit_scratch(size_t nRoi);
     size_t device_bytes() const;   // device memory held by the lane's buffers

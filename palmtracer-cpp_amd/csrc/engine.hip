@@ -26,17 +26,24 @@ __global__ void zero3_kernel(uint32_t* a, size_t na, uint32_t* b, size_t nb, uin
     }
 }
 
-// Zeroes P at the mask pixels of frames 0..F-1 (frames strided in y, list entries in x).
-// Their number is added to counters[kMaskPx].
+// Zeroes P at the mask pixels and the seed-bitmap words of the local maxima of frames 0..F-1 (frames
+// strided in y, list entries in x), so that both are zero everywhere for the lane's next detection.
+// The number of mask pixels is added to counters[kMaskPx].
 __global__ void clear_mask_kernel(uint32_t* __restrict__ P, const uint32_t* __restrict__ maskList,
-                                  const uint32_t* __restrict__ maskCnt, uint32_t F, size_t npx, uint32_t* __restrict__ counters)
+                                  const uint32_t* __restrict__ maskCnt, uint32_t* __restrict__ seedBits,
+                                  const uint32_t* __restrict__ seedList, const uint32_t* __restrict__ seedCnt, uint32_t F,
+                                  size_t npx, uint32_t* __restrict__ counters)
 {
+    const size_t sw = seed_words(npx);
     for (uint32_t f = blockIdx.y; f < F; f += gridDim.y) {
-        const uint32_t n = maskCnt[f];
+        const uint32_t n = maskCnt[f], ns = seedCnt[f];
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[kMaskPx], n);
         const uint32_t* ml = maskList + (size_t)f * npx;
         uint32_t* Pf = P + (size_t)f * npx;
         for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) Pf[ml[k]] = 0u;
+        const uint32_t* sl = seedList + (size_t)f * npx;
+        uint32_t* sbf = seedBits + (size_t)f * sw;
+        for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ns; k += gridDim.x * blockDim.x) sbf[sl[k] >> 5] = 0u;
     }
 }
 
@@ -267,10 +274,13 @@ void Lane::prepare(size_t F, size_t npx)
     const LaneSizes z(F, npx);
     const size_t n = z.n;
     void* oldP = P.p;
+    void* oldBits = sbits.p;
     P.ensure(n * 4);
-    if (P.p != oldP || pDirty) {
-        // A new buffer, or one whose last detection was not cleared: zero it whole.
+    sbits.ensure(F * seed_words(npx) * 4);
+    if (P.p != oldP || sbits.p != oldBits || pDirty) {
+        // New buffers, or buffers whose last detection was not cleared: zero them whole.
         PT_CHECK(hipMemsetAsync(P.p, 0, P.bytes, stream));
+        PT_CHECK(hipMemsetAsync(sbits.p, 0, sbits.bytes, stream));
         pDirty = false;
     }
     lab.ensure(n * 4);
@@ -293,7 +303,6 @@ void Lane::prepare(size_t F, size_t npx)
     rf.ensure(rfSlots * rf_slot_words(npx) * 4);
     gpool.ensure(z.gridWords * 4);
     gmeta.ensure(F * kGridMetaWords * 4);
-    sbits.ensure(F * seed_words(npx) * 4);
     hist.ensure(F * kHistBins * 4);
     uint32_t* pf = perFrame.as<uint32_t>();
     const size_t s = F + 1;
@@ -341,9 +350,9 @@ void Lane::zero()
 {
     // Per-frame counters (6 arrays laid out with a stride of frames + 1 words, frames being the
     // chunk capacity, whatever the number of frames in this chunk) and the chunk counters; a fill
-    // kernel rather than a memset so that the sequence can be captured in a graph.
-    zero3_kernel<<<1024, 256, 0, stream>>>(b.maskCnt, 6 * (b.frames + 1), b.counters, kNumCounters, b.seedBits,
-                                           b.frames * seed_words(b.npx));
+    // kernel rather than a memset so that the sequence can be captured in a graph. (P and the seed
+    // bitmap are zero already: clear_mask() zeroes what a detection set.)
+    zero3_kernel<<<64, 256, 0, stream>>>(b.maskCnt, 6 * (b.frames + 1), b.counters, kNumCounters, nullptr, 0);
     PT_CHECK(hipGetLastError());
 }
 
@@ -352,7 +361,8 @@ void Lane::clear_mask(size_t F, size_t npx)
     if (F == 0) return;
     const unsigned gx = (unsigned)std::max<size_t>(1, std::min<size_t>(64, npx / 65536));
     const unsigned gy = (unsigned)std::min<size_t>(F, 4096);
-    clear_mask_kernel<<<dim3(gx, gy), 256, 0, stream>>>(b.P, b.maskList, b.maskCnt, (uint32_t)F, npx, b.counters);
+    clear_mask_kernel<<<dim3(gx, gy), 256, 0, stream>>>(b.P, b.maskList, b.maskCnt, b.seedBits, b.seedList, b.seedCnt,
+                                                        (uint32_t)F, npx, b.counters);
     PT_CHECK(hipGetLastError());
     pDirty = false;
 }
@@ -540,6 +550,7 @@ void Engine::detect_chunk(Lane& L, const uint16_t* d_frames, size_t F, int H, in
     const bool prof = opt.profile && evBase != kNoEvents;
     if (L.pDirty) {
         PT_CHECK(hipMemsetAsync(L.P.p, 0, L.P.bytes, st));
+        PT_CHECK(hipMemsetAsync(L.sbits.p, 0, L.sbits.bytes, st));
         L.pDirty = false;
     }
     if (prof) PT_CHECK(hipEventRecord(event(evBase + 0), st));
