@@ -69,69 +69,135 @@ __global__ void __launch_bounds__(NT) sort_frames_kernel(const double* __restric
     }
 }
 
-// Frames with up to SORT_MAX rows, one workgroup per frame: (Y, X, position) keys in LDS, a bitonic
-// sort of the row positions, rows written in sorted order. For callers whose frames can hold more
-// rows than the ranking kernel above keeps in LDS.
-constexpr int SORT_MAX = 8192;
-constexpr size_t SORT_LDS = (size_t)SORT_MAX * (8 + 8 + 2);
-__global__ void __launch_bounds__(NT) sort_frames_big_kernel(const double* __restrict__ rows,
-                                                             const uint32_t* __restrict__ roiBase,
-                                                             double* __restrict__ staged)
+// Frames that can hold more rows than the ranking kernel keeps in LDS: a sort spread over many
+// workgroups. Keys (weak Y, weak X, frame-local row) live in a scratch of kSortScratchPerRow bytes per
+// row as structure-of-arrays, two copies for ping-pong: ky[2][n], kx[2][n] (int64), id[2][n] (u32).
+//   1. sort_tiles_kernel: tiles of TILE rows of a frame, bitonic-sorted in LDS (one workgroup each);
+//   2. merge_pass_kernel: sorted runs of L rows merged pairwise; a row's place in the merged run is its
+//      place in its own run plus the rows of the partner run before it (binary search). Keys are
+//      distinct (the row number breaks ties), so the places form a permutation;
+//   3. gather_rows_kernel: rows copied to their sorted place, consecutive threads on consecutive doubles.
+constexpr int TILE = 2048;
+constexpr int TILE_NT = 512;
+constexpr int MERGE_NT = 256;
+constexpr int GATHER_NT = 256;
+constexpr int GATHER_PER_THREAD = 4;
+
+struct SortKeys {
+    int64_t* ky[2];
+    int64_t* kx[2];
+    uint32_t* id[2];
+};
+
+__host__ __device__ inline SortKeys sort_keys(void* scratch, uint64_t n)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
-    int64_t* ky = reinterpret_cast<int64_t*>(smem);
-    int64_t* kx = ky + SORT_MAX;
-    uint16_t* id = reinterpret_cast<uint16_t*>(kx + SORT_MAX);
-    constexpr uint16_t kPad = 0xFFFF;
-    const int f = blockIdx.x;
-    const uint32_t b0 = roiBase[f], n = roiBase[f + 1] - b0;
-    if (n > (uint32_t)SORT_MAX) {
-        // Beyond the LDS: rank every row against all others (global memory).
-        for (uint32_t i = threadIdx.x; i < n; i += NT) {
-            const double* r = rows + (size_t)(b0 + i) * kCols;
-            const int64_t yi = weak_key(r[kY]), xi = weak_key(r[kX]);
-            uint32_t rank = 0;
-            for (uint32_t j = 0; j < n; ++j)
-                rank += before(weak_key(rows[(size_t)(b0 + j) * kCols + kY]), weak_key(rows[(size_t)(b0 + j) * kCols + kX]),
-                               j, yi, xi, i);
-            double* o = staged + (size_t)(b0 + rank) * kCols;
-            for (int c = 0; c < kCols; ++c) o[c] = r[c];
-            o[kIndex] = (double)(rank + 1);
+    SortKeys k;
+    int64_t* w = static_cast<int64_t*>(scratch);
+    k.ky[0] = w;
+    k.ky[1] = w + n;
+    k.kx[0] = w + 2 * n;
+    k.kx[1] = w + 3 * n;
+    uint32_t* u = reinterpret_cast<uint32_t*>(w + 4 * n);
+    k.id[0] = u;
+    k.id[1] = u + n;
+    return k;
+}
+
+__global__ void __launch_bounds__(TILE_NT) sort_tiles_kernel(const double* __restrict__ rows,
+                                                             const uint32_t* __restrict__ roiBase, uint32_t tilesPerFrame,
+                                                             SortKeys keys)
+{
+    __shared__ int64_t ky[TILE], kx[TILE];
+    __shared__ uint32_t id[TILE];
+    const uint32_t f = blockIdx.x / tilesPerFrame, t = blockIdx.x - f * tilesPerFrame;
+    const uint32_t b0 = roiBase[f], n = roiBase[f + 1] - b0, start = t * TILE;
+    if (start >= n) return;
+    const uint32_t m = min<uint32_t>(TILE, n - start);
+    uint32_t m2 = 2;
+    while (m2 < m) m2 <<= 1;
+    for (uint32_t i = threadIdx.x; i < m2; i += TILE_NT) {
+        if (i < m) {
+            const double* r = rows + (size_t)(b0 + start + i) * kCols;
+            ky[i] = weak_key(r[kY]);
+            kx[i] = weak_key(r[kX]);
+            id[i] = start + i;
+        } else {
+            // Padding sorts after every row: largest keys and a row number no frame reaches.
+            ky[i] = INT64_MAX;
+            kx[i] = INT64_MAX;
+            id[i] = 0xFFFFFFFFu;
         }
-        return;
-    }
-    uint32_t n2 = 1;
-    while (n2 < n) n2 <<= 1;
-    for (uint32_t i = threadIdx.x; i < n2; i += NT) {
-        if (i < n) {
-            ky[i] = weak_key(rows[(size_t)(b0 + i) * kCols + kY]);
-            kx[i] = weak_key(rows[(size_t)(b0 + i) * kCols + kX]);
-        }
-        id[i] = i < n ? (uint16_t)i : kPad;
     }
     __syncthreads();
-    auto less = [&](uint16_t a, uint16_t b) {
-        if (a == kPad) return false;
-        if (b == kPad) return true;
-        return before(ky[a], kx[a], a, ky[b], kx[b], b);
-    };
-    for (uint32_t k = 2; k <= n2; k <<= 1) {
+    for (uint32_t k = 2; k <= m2; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < n2; i += NT) {
-                const uint32_t ixj = i ^ j;
-                if (ixj > i) {
-                    const uint16_t a = id[i], b = id[ixj];
-                    if ((i & k) == 0 ? less(b, a) : less(a, b)) { id[i] = b; id[ixj] = a; }
+            for (uint32_t p = threadIdx.x; p < m2 / 2; p += TILE_NT) {
+                const uint32_t i = (p / j) * 2 * j + (p % j), o = i + j;
+                const bool up = (i & k) == 0;
+                const int64_t ya = ky[i], xa = kx[i], yb = ky[o], xb = kx[o];
+                const uint32_t ia = id[i], ib = id[o];
+                if (up ? before(yb, xb, ib, ya, xa, ia) : before(ya, xa, ia, yb, xb, ib)) {
+                    ky[i] = yb; kx[i] = xb; id[i] = ib;
+                    ky[o] = ya; kx[o] = xa; id[o] = ia;
                 }
             }
             __syncthreads();
         }
     }
-    for (uint32_t r = threadIdx.x; r < n; r += NT) {
-        const double* src = rows + (size_t)(b0 + id[r]) * kCols;
-        double* o = staged + (size_t)(b0 + r) * kCols;
-        for (int c = 0; c < kCols; ++c) o[c] = src[c];
-        o[kIndex] = (double)(r + 1);
+    for (uint32_t i = threadIdx.x; i < m; i += TILE_NT) {
+        const size_t g = (size_t)b0 + start + i;
+        keys.ky[0][g] = ky[i];
+        keys.kx[0][g] = kx[i];
+        keys.id[0][g] = id[i];
+    }
+}
+
+// One pass: runs of L rows (sorted, in copy `from`) merged pairwise into copy 1 - from.
+__global__ void __launch_bounds__(MERGE_NT) merge_pass_kernel(const uint32_t* __restrict__ roiBase, uint32_t blocksPerFrame,
+                                                              uint32_t L, int from, SortKeys keys)
+{
+    const uint32_t f = blockIdx.x / blocksPerFrame, c = blockIdx.x - f * blocksPerFrame;
+    const uint32_t b0 = roiBase[f], n = roiBase[f + 1] - b0, q = c * MERGE_NT + threadIdx.x;
+    if (q >= n) return;
+    const int64_t* sy = keys.ky[from] + b0;
+    const int64_t* sx = keys.kx[from] + b0;
+    const uint32_t* si = keys.id[from] + b0;
+    const int64_t y = sy[q], x = sx[q];
+    const uint32_t i = si[q];
+    const uint32_t r = q / L, own = q - r * L, ps = (r ^ 1u) * L;
+    uint32_t before_n = 0;
+    if (ps < n) {
+        // Rows of the partner run that precede (y, x, i): lower bound by binary search.
+        uint32_t lo = ps, hi = min(ps + L, n);
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (before(sy[mid], sx[mid], si[mid], y, x, i)) lo = mid + 1;
+            else hi = mid;
+        }
+        before_n = lo - ps;
+    }
+    const uint32_t d = (r & ~1u) * L + own + before_n;
+    keys.ky[1 - from][b0 + d] = y;
+    keys.kx[1 - from][b0 + d] = x;
+    keys.id[1 - from][b0 + d] = i;
+}
+
+__global__ void __launch_bounds__(GATHER_NT) gather_rows_kernel(const double* __restrict__ rows,
+                                                                const uint32_t* __restrict__ roiBase,
+                                                                uint32_t blocksPerFrame, const uint32_t* __restrict__ order,
+                                                                double* __restrict__ staged)
+{
+    const uint32_t f = blockIdx.x / blocksPerFrame, c = blockIdx.x - f * blocksPerFrame;
+    const uint32_t b0 = roiBase[f], n = roiBase[f + 1] - b0;
+    const uint32_t d0 = c * (GATHER_NT * GATHER_PER_THREAD);
+    if (d0 >= n * kCols) return;
+#pragma unroll
+    for (int k = 0; k < GATHER_PER_THREAD; ++k) {
+        const uint32_t d = d0 + k * GATHER_NT + threadIdx.x;
+        if (d >= n * kCols) break;
+        const uint32_t r = d / kCols, col = d - r * kCols;
+        staged[(size_t)b0 * kCols + d] =
+            col == kIndex ? (double)(r + 1) : rows[((size_t)b0 + order[b0 + r]) * kCols + col];
     }
 }
 
@@ -202,13 +268,34 @@ __global__ void __launch_bounds__(NT) scatter_kernel(const double* __restrict__ 
 } // namespace
 
 void launch_sort_frames(const double* d_rows, const uint32_t* d_roiBase, int frames, double* d_staged, hipStream_t s,
-                        uint64_t maxRows)
+                        uint64_t maxRows, uint64_t rows, void* d_scratch)
 {
     if (frames <= 0) return;
-    if (maxRows <= (uint64_t)LDS_ROWS)
+    if (maxRows <= (uint64_t)LDS_ROWS) {
         sort_frames_kernel<<<frames, NT, 0, s>>>(d_rows, d_roiBase, d_staged);
-    else
-        sort_frames_big_kernel<<<frames, NT, SORT_LDS, s>>>(d_rows, d_roiBase, d_staged);
+        PT_CHECK(hipGetLastError());
+        return;
+    }
+    if (!d_scratch) throw Error("the per-frame sort needs a key scratch");
+    if (maxRows > 0x7FFFFFFFull) throw Error("too many rows per frame for the per-frame sort");
+    // Every row index the kernels use lies below roiBase[frames] <= rows, the rows at d_rows.
+    const uint32_t tiles = (uint32_t)((maxRows + TILE - 1) / TILE);
+    if ((uint64_t)frames * tiles > 0x7FFFFFFFull) throw Error("per-frame sort grid too large");
+    const SortKeys k = sort_keys(d_scratch, rows);
+    sort_tiles_kernel<<<(unsigned)(frames * tiles), TILE_NT, 0, s>>>(d_rows, d_roiBase, tiles, k);
+    PT_CHECK(hipGetLastError());
+    int from = 0;
+    const uint32_t mBlocks = (uint32_t)((maxRows + MERGE_NT - 1) / MERGE_NT);
+    if ((uint64_t)frames * mBlocks > 0x7FFFFFFFull) throw Error("per-frame sort grid too large");
+    for (uint64_t L = TILE; L < maxRows; L <<= 1) {
+        merge_pass_kernel<<<(unsigned)(frames * mBlocks), MERGE_NT, 0, s>>>(d_roiBase, mBlocks, (uint32_t)L, from, k);
+        PT_CHECK(hipGetLastError());
+        from ^= 1;
+    }
+    const uint64_t per = (uint64_t)GATHER_NT * GATHER_PER_THREAD;
+    const uint32_t gBlocks = (uint32_t)((maxRows * kCols + per - 1) / per);
+    if ((uint64_t)frames * gBlocks > 0x7FFFFFFFull) throw Error("per-frame sort grid too large");
+    gather_rows_kernel<<<(unsigned)(frames * gBlocks), GATHER_NT, 0, s>>>(d_rows, d_roiBase, gBlocks, k.id[from], d_staged);
     PT_CHECK(hipGetLastError());
 }
 

@@ -308,7 +308,9 @@ void Engine::batch_localize(BatchItem* items, uint64_t nItems)
         const uint32_t* base = bBases_.as<uint32_t>() + baseOff[s];
         double* rows = bRows_.as<double>() + jobOff[s] * kCols;
         double* staged = bStaged_.as<double>() + jobOff[s] * kCols;
-        launch_sort_frames(rows, base, (int)P, staged, s0, std::min<uint64_t>(it.maxPerPlane, npx / 4 + 2));
+        // Key scratch of the sort: the stack's ROIs, which the fit has read by now.
+        launch_sort_frames(rows, base, (int)P, staged, s0, std::min<uint64_t>(it.maxPerPlane, npx / 4 + 2), nRows[s],
+                           bRois_.as<Roi>() + jobOff[s]);
         uint64_t* src = plan + planOff[s];
         uint64_t* dst = src + P;
         uint32_t* cnt = reinterpret_cast<uint32_t*>(dst + P);

@@ -352,7 +352,11 @@ void Lane::zero()
     // chunk capacity, whatever the number of frames in this chunk) and the chunk counters; a fill
     // kernel rather than a memset so that the sequence can be captured in a graph. (P and the seed
     // bitmap are zero already: clear_mask() zeroes what a detection set.)
-    zero3_kernel<<<64, 256, 0, stream>>>(b.maskCnt, 6 * (b.frames + 1), b.counters, kNumCounters, nullptr, 0);
+    // Frames of at least kHistSplitPx pixels also start with a zero mask histogram (several workgroups
+    // add into it).
+    const bool hz = b.npx >= kHistSplitPx;
+    zero3_kernel<<<64, 256, 0, stream>>>(b.maskCnt, 6 * (b.frames + 1), b.counters, kNumCounters, hz ? b.hist : nullptr,
+                                         hz ? b.frames * kHistBins : 0);
     PT_CHECK(hipGetLastError());
 }
 
@@ -811,13 +815,17 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
         launch_fit(d_in, npx, W, H, allRois_.as<Roi>(), (uint32_t)fitRois, fp, main_.rows.as<double>(), opt.fitFp64,
                    main_.fit_scratch(fitRois), main_.stream, passes, nullptr, opt.fitWaves);
     if (opt.profile) PT_CHECK(hipEventRecord(fitEv_[1], main_.stream));
+    // The sort's key scratch is the ROI store, free once the fit (earlier on this stream) has read it.
+    static_assert(sizeof(Roi) >= kSortScratchPerRow, "the ROI store holds the sort keys");
     if (totalRois)
-        launch_sort_frames(main_.rows.as<double>(), dBase, (int)planes, main_.staged.as<double>(), main_.stream, frameRows);
+        launch_sort_frames(main_.rows.as<double>(), dBase, (int)planes, main_.staged.as<double>(), main_.stream, frameRows,
+                           totalRois, allRois_.as<Roi>());
     if (sideRoiTotal) {
         sideBase_.ensure((nDef + 1) * 4);
         PT_CHECK(hipMemcpyAsync(sideBase_.p, hSideBase, (nDef + 1) * 4, hipMemcpyHostToDevice, main_.stream));
         launch_sort_frames(main_.rows.as<double>() + totalRois * kCols, sideBase_.as<uint32_t>(), (int)nDef,
-                           main_.staged.as<double>() + totalRois * kCols, main_.stream, frameRows);
+                           main_.staged.as<double>() + totalRois * kCols, main_.stream, frameRows, sideRoiTotal,
+                           allRois_.as<Roi>() + totalRois);
     }
     if (opt.profile) PT_CHECK(hipEventRecord(fitEv_[2], main_.stream));
 

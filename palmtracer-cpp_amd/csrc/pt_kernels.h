@@ -119,6 +119,9 @@ enum Counter : int {
 
 // Levels of the per-frame potential histogram: P < 16 one bin each, then 16 bins per octave.
 constexpr uint32_t kHistBins = 512;
+// Frames of at least this many pixels build their mask histogram over several workgroups (the
+// histogram is zeroed with the chunk counters).
+constexpr size_t kHistSplitPx = size_t(1) << 20;
 __host__ __device__ inline uint32_t hist_bin(uint32_t v)
 {
     if (v < 16u) return v;
@@ -240,9 +243,13 @@ void launch_spline_fit(const uint16_t* d_frames, size_t npx, int W, int H, const
 void launch_spline_transpose(const double* d_raw, double* d_cv, uint64_t ssize, hipStream_t s);
 
 // Per-frame (Y, X) sort (rows of frame f at d_rows + roiBase[f]) into d_staged with the same layout;
-// Index is set, Id and Plane are left for launch_scatter. maxRows: the most rows a frame can hold.
+// Index is set, Id and Plane are left for launch_scatter. maxRows: the most rows a frame can hold;
+// rows: the rows at d_rows (roiBase[frames] <= rows). When maxRows is above 2048 the sort needs
+// d_scratch of kSortScratchPerRow bytes per row (8-byte aligned); callers lend the ROI store, which
+// the fit has read by then (a ROI record is at least that large).
+constexpr size_t kSortScratchPerRow = 40;
 void launch_sort_frames(const double* d_rows, const uint32_t* d_roiBase, int frames, double* d_staged, hipStream_t s,
-                        uint64_t maxRows);
+                        uint64_t maxRows, uint64_t rows, void* d_scratch);
 // Placement plan on the device: per plane p, its staged first row src[p] (main rows from the plane
 // bases base[0..planes]; the nDef deferred planes from d_def = {planes, first side rows, counts},
 // offset by sideOff), its row count and its output row dst[p] (prefix of the counts); *d_total the

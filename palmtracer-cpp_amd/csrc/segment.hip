@@ -222,6 +222,41 @@ __global__ void __launch_bounds__(MH_NT) mask_hist_kernel(const uint32_t* __rest
     hist[(size_t)f * kHistBins + t] = h[t];
 }
 
+// Frames of at least kHistSplitPx pixels: the frame's mask list is split over the workgroups of a
+// grid row (x: slices, y: frames); each adds its LDS histogram into the frame's histogram (zeroed with
+// the chunk counters) and mask_hist_suffix_kernel turns the counts into counts at or above each level.
+__global__ void __launch_bounds__(MH_NT) mask_hist_partial_kernel(const uint32_t* __restrict__ P,
+                                                                  const uint32_t* __restrict__ maskList,
+                                                                  const uint32_t* __restrict__ maskCnt,
+                                                                  uint32_t* __restrict__ hist, size_t npx)
+{
+    __shared__ uint32_t h[kHistBins];
+    const int f = blockIdx.y, t = threadIdx.x;
+    h[t] = 0u;
+    __syncthreads();
+    const uint32_t n = maskCnt[f];
+    const uint32_t* ml = maskList + (size_t)f * npx;
+    const uint32_t* Pf = P + (size_t)f * npx;
+    for (uint32_t k = blockIdx.x * MH_NT + t; k < n; k += gridDim.x * MH_NT) atomicAdd(&h[hist_bin(Pf[ml[k]])], 1u);
+    __syncthreads();
+    if (h[t]) atomicAdd(&hist[(size_t)f * kHistBins + t], h[t]);
+}
+
+__global__ void __launch_bounds__(MH_NT) mask_hist_suffix_kernel(uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t h[kHistBins];
+    const int f = blockIdx.x, t = threadIdx.x;
+    h[t] = hist[(size_t)f * kHistBins + t];
+    __syncthreads();
+    for (int o = 1; o < MH_NT; o <<= 1) {
+        const uint32_t v = t + o < MH_NT ? h[t + o] : 0u;
+        __syncthreads();
+        h[t] += v;
+        __syncthreads();
+    }
+    hist[(size_t)f * kHistBins + t] = h[t];
+}
+
 // Value of lane l (wave-uniform l) in every lane, without a round trip through LDS.
 __device__ inline uint32_t from_lane(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
 __device__ inline int from_lane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
@@ -1808,6 +1843,7 @@ __global__ void reset_pass_kernel(uint32_t* __restrict__ counters)
 // by all the workgroups of the frame (grid: x workgroups per frame, y frames).
 constexpr int RS_NT = 1024;
 constexpr uint32_t RS_LDS_REGIONS = 2048;
+constexpr uint32_t RS_RUN = 8;   // list entries per thread and run (global-atomic path)
 
 __global__ void __launch_bounds__(RS_NT) region_stats_kernel(const uint32_t* __restrict__ P, const uint32_t* __restrict__ fin,
                                                              const PixAux* __restrict__ aux, const uint32_t* __restrict__ maskList,
@@ -1825,13 +1861,50 @@ __global__ void __launch_bounds__(RS_NT) region_stats_kernel(const uint32_t* __r
     const bool inLds = nReg <= RS_LDS_REGIONS;
     // LDS accumulation: one workgroup per frame; global atomics: all of the frame's workgroups.
     if (inLds && blockIdx.x != 0) return;
-    const uint32_t k0 = inLds ? threadIdx.x : blockIdx.x * RS_NT + threadIdx.x;
-    const uint32_t kStride = inLds ? RS_NT : gridDim.x * RS_NT;
     Region* rg = regions + (size_t)f * regionCap;
-    if (inLds) {
-        for (uint32_t r = threadIdx.x; r < nReg; r += RS_NT) { sP[r] = 0; sPx[r] = 0; sPy[r] = 0; sSurf[r] = 0; }
-        __syncthreads();
+    if (!inLds) {
+        // Global atomics: each thread sums a run of RS_RUN consecutive list entries (neighbouring
+        // pixels of one detection strip, mostly of one region) and adds a sum per change of region.
+        uint32_t cur = 0xFFFFFFFFu, aS = 0;
+        uint64_t aP = 0, aX = 0, aY = 0;
+        auto flush = [&]() {
+            if (cur == 0xFFFFFFFFu) return;
+            Region* r = rg + cur;
+            atomicAdd(&r->surf, aS);
+            atomicAdd((unsigned long long*)&r->sP, (unsigned long long)aP);
+            atomicAdd((unsigned long long*)&r->sPx, (unsigned long long)aX);
+            atomicAdd((unsigned long long*)&r->sPy, (unsigned long long)aY);
+        };
+        const uint32_t* ml = maskList + (size_t)f * npx;
+        for (uint32_t k0 = (blockIdx.x * RS_NT + threadIdx.x) * RS_RUN; k0 < cnt; k0 += gridDim.x * RS_NT * RS_RUN) {
+            const uint32_t k1 = min(k0 + RS_RUN, cnt);
+            for (uint32_t k = k0; k < k1; ++k) {
+                const uint32_t p = ml[k];
+                const uint32_t L = fin[(size_t)f * npx + p];
+                if (L == 0 || L > npx) { atomicOr(&counters[kStatus], 2u); continue; }
+                const uint32_t slot = aux[(size_t)f * npx + (L - 1)].rslot;
+                if (slot >= nReg) { atomicOr(&counters[kStatus], 2u); continue; }
+                if (slot != cur) {
+                    flush();
+                    cur = slot;
+                    aS = 0;
+                    aP = aX = aY = 0;
+                }
+                const uint64_t v = P[(size_t)f * npx + p];
+                const uint32_t y = p / W, x = p - y * W;
+                aS += 1;
+                aP += v;
+                aX += v * x;
+                aY += v * y;
+            }
+        }
+        flush();
+        return;
     }
+    const uint32_t k0 = threadIdx.x;
+    const uint32_t kStride = RS_NT;
+    for (uint32_t r = threadIdx.x; r < nReg; r += RS_NT) { sP[r] = 0; sPx[r] = 0; sPy[r] = 0; sSurf[r] = 0; }
+    __syncthreads();
     for (uint32_t k = k0; k < cnt; k += kStride) {
         const uint32_t p = maskList[(size_t)f * npx + k];
         const uint32_t L = fin[(size_t)f * npx + p];
@@ -1840,27 +1913,17 @@ __global__ void __launch_bounds__(RS_NT) region_stats_kernel(const uint32_t* __r
         if (slot >= nReg) { atomicOr(&counters[kStatus], 2u); continue; }
         const uint64_t v = P[(size_t)f * npx + p];
         const uint32_t y = p / W, x = p - y * W;
-        if (inLds) {
-            atomicAdd(&sSurf[slot], 1u);
-            atomicAdd(&sP[slot], (unsigned long long)v);
-            atomicAdd(&sPx[slot], (unsigned long long)(v * x));
-            atomicAdd(&sPy[slot], (unsigned long long)(v * y));
-        } else {
-            Region* r = rg + slot;
-            atomicAdd(&r->surf, 1u);
-            atomicAdd((unsigned long long*)&r->sP, (unsigned long long)v);
-            atomicAdd((unsigned long long*)&r->sPx, (unsigned long long)(v * x));
-            atomicAdd((unsigned long long*)&r->sPy, (unsigned long long)(v * y));
-        }
+        atomicAdd(&sSurf[slot], 1u);
+        atomicAdd(&sP[slot], (unsigned long long)v);
+        atomicAdd(&sPx[slot], (unsigned long long)(v * x));
+        atomicAdd(&sPy[slot], (unsigned long long)(v * y));
     }
-    if (inLds) {
-        __syncthreads();
-        for (uint32_t r = threadIdx.x; r < nReg; r += RS_NT) {
-            rg[r].surf = sSurf[r];
-            rg[r].sP = sP[r];
-            rg[r].sPx = sPx[r];
-            rg[r].sPy = sPy[r];
-        }
+    __syncthreads();
+    for (uint32_t r = threadIdx.x; r < nReg; r += RS_NT) {
+        rg[r].surf = sSurf[r];
+        rg[r].sP = sP[r];
+        rg[r].sPx = sPx[r];
+        rg[r].sPy = sPy[r];
     }
 }
 
@@ -2023,7 +2086,14 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
     const uint32_t* skip = resolve ? nullptr : b.frameFlag;
     uint32_t ldsCap = 0, parentOff = 0;
     const size_t flLds = frame_label_lds(npx, ldsCap, parentOff);
-    mask_hist_kernel<<<F, MH_NT, 0, s>>>(b.P, b.maskList, b.maskCnt, b.hist, npx);
+    if (npx >= kHistSplitPx) {
+        // One slice per 16384 mask-list entries the frame could hold at 1/16 of its pixels.
+        const unsigned slices = (unsigned)std::clamp<size_t>(npx / (size_t(16384) * 16), 1, 64);
+        mask_hist_partial_kernel<<<dim3(slices, F), MH_NT, 0, s>>>(b.P, b.maskList, b.maskCnt, b.hist, npx);
+        mask_hist_suffix_kernel<<<F, MH_NT, 0, s>>>(b.hist);
+    } else {
+        mask_hist_kernel<<<F, MH_NT, 0, s>>>(b.P, b.maskList, b.maskCnt, b.hist, npx);
+    }
     if (ldsCap > 0)
         frame_label_kernel<<<F, FL_NT, flLds, s>>>(b.P, b.seedBits, b.lab, b.fin, b.aux, b.maskList, b.maskCnt, b.seedList,
                                                    b.seedCnt, b.rootCnt, b.regions, b.regionCap, b.multiList, b.multiCap,
@@ -2082,7 +2152,7 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
         reset_pass_kernel<<<1, 64, 0, s>>>(b.counters);
         watershed(1);
     }
-    region_stats_kernel<<<dim3((unsigned)std::clamp<size_t>(npx / 262144, 1, 16), F), RS_NT, 0, s>>>(
+    region_stats_kernel<<<dim3((unsigned)std::clamp<size_t>(npx / 131072, 1, 32), F), RS_NT, 0, s>>>(
         b.P, b.fin, b.aux, b.maskList, b.maskCnt, b.rootCnt, b.regions, b.regionCap, b.counters, W, npx, skip);
     region_count_kernel<<<F, NT, 0, s>>>(b.regions, b.regionCap, b.rootCnt, b.found, b.keep, maxPoints, skip);
     if (b.runTotal && b.scanWait) PT_CHECK(hipStreamWaitEvent(s, b.scanWait, 0));
