@@ -68,6 +68,8 @@ enum : int { kFetch = 0, kDone = 1, kPassInit = 2, kPassTrial = 3, kPassRestore 
 // column factors of the separable exponential (15 x 64 x 8 bytes). Larger ROIs read the frame and
 // evaluate the exponential per pixel in every pass.
 constexpr int kMaxStagedRoi = 15;
+// Squared correlation below which a converged FP32 fit is handed to the FP64 group kernel.
+constexpr double kLowCorrelation = 0.05;
 
 __host__ __device__ constexpr size_t staged_lds_bytes(int roi)
 {
@@ -314,13 +316,13 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
             for (int a = 1; a < N; ++a) { const double v = fabs((double)be[a]); mb = mb < v ? v : mb; }
             return mb;
         };
+        bool finish = false, failed = false, handOff = false;
         // End-of-iteration stopping rules (GaussianFit.cpp:155-162); otherwise the next iteration.
         auto stop_after_step = [&](double rel) {
             if (it >= 2 && (rel < 1e-6 || err / nPix < 1e-5 || max_abs_beta() < 1e-6)) return true;
             ++it;
             return false;
         };
-        bool finish = false, failed = false, handOff = false;
         if (state == kPassInit) {
             ++nJac;
             err = e;
@@ -421,6 +423,19 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
             break;
         }
         if (failed && hand) handOff = true;
+        if (finish && !handOff && hand) {
+            // A final parameter at (or within FP32 reach of) one of its bounds: whether the FP64 fit
+            // ends on the bound is a clamping decision of the last steps, so the ROI is handed over.
+            // Bands: 0.5 ADU for offset and amplitude, 0.01 px for the centre, 2 % for the widths.
+#pragma unroll
+            for (int a = 0; a < N; ++a) {
+                if (a == 6) continue;   // theta (mode 4 is fitted by the group kernel)
+                const double lo = a <= 1 ? 0.0 : (a == 2 ? x0 - 1 : (a == 3 ? y0 - 1 : 0.1));
+                const double hi = a <= 1 ? 65535.0 : (a == 2 ? x0 + 1 : (a == 3 ? y0 + 1 : 20.0));
+                const double bl = a <= 1 ? 0.5 : (a <= 3 ? 0.01 : 0.02 * lo), bh = a <= 1 ? 0.5 : (a <= 3 ? 0.01 : 0.02 * hi);
+                if (p[a] - lo <= bl || hi - p[a] <= bh) handOff = true;
+            }
+        }
         double rss = 0.0;
         if (finish && !handOff) {
             // computeR2 at p (GaussianFit.cpp:270-295): squared Pearson correlation of data and model,
@@ -428,8 +443,10 @@ __global__ void __launch_bounds__(64) fit_persistent_kernel(const uint16_t* __re
             const double vd = sdd - sd * sd / nPix, vm = cmm - cm * cm / nPix, cv = cdm - sd * cm / nPix;
             const double rr = cv / sqrt(vd * vm);
             rss = isnan(rr) ? 0.0 : rr * rr;
-            // A correlation near the failure threshold (1e-7) is decided in FP64.
-            if (rss < 1e-5 && hand) handOff = true;
+            // A correlation near the failure threshold (1e-7) is decided in FP64, and so are fits that
+            // barely correlate with the data (noise spikes, tangled spots): their minima are shallow
+            // and ill-conditioned, where FP32 sums end elsewhere in the valley.
+            if (rss < kLowCorrelation && hand) handOff = true;
         }
         if (handOff) {
             hand[atomicAdd(handCount, 1u)] = idx;

@@ -132,3 +132,27 @@ def test_fast_fit_matches_fp64_reference_order(pt_gpu, c2_frames):
     ref = pt_gpu.localization(sub, thr, 0.0, 2, FIT_PARAMS)
     assert fast.shape == ref.shape
     assert fit_mismatch_rows(fast, ref).size == 0
+
+
+# Dense frames of the served and batched workloads (5000 emitters per frame): ROIs holding noise
+# spikes or tangled spots, with fitted widths at their lower bound and barely any correlation, are
+# decided in FP64 (a width at a bound, a squared correlation below 0.05).
+DENSE = [(1024, 0x5EED1002), (2048, 0x5EED1003)]
+
+
+@pytest.mark.parametrize("side,seed", DENSE)
+def test_fast_fit_dense_frames(pt_gpu, oracle_mod, side, seed):
+    """Ten dense frames at the 20-iteration cap: every fast-fit row within the tolerance of the oracle."""
+    import torch
+    d = torch.empty((10, side, side), dtype=torch.uint16, device="cuda")
+    pt_gpu.synth_device(d, seed=seed, emitters=5000)
+    st = d.cpu().numpy()
+    thr = oracle_mod.autothreshold(st[0])
+    pt_gpu.set_option("max_iter", 20)
+    got = pt_gpu.localization(st, thr, 0.0, 2, FIT_PARAMS)
+    handed = pt_gpu.last_stats()["fit_handed_over"]
+    exp = oracle_mod.localization(st, thr, 0.0, 2, FIT_PARAMS, max_iter=20)
+    assert got.shape == exp.shape and len(exp) > 40000
+    bad = fit_mismatch_rows(got, exp)
+    assert bad.size == 0, (bad.size, got[bad[:3]], fit_errors(got, exp))
+    assert 0 < handed < 0.2 * len(exp), handed
