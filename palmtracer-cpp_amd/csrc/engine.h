@@ -68,6 +68,8 @@ struct Options {
     bool batchGraph = true;         // batched localization: the fit stage as a captured hipGraph
     size_t streamPixels = size_t(1) << 27;   // host entry points: pixels per piece of the stack
     int hostThreads = 8;            // host entry points: threads of the copies to and from pinned staging
+    // Frames with more mask pixels go to the exact frame flood without component floods (0: none).
+    uint32_t denseFlagMask = 75000;
     int deferFlagged = 2;           // flagged frames of all but the last chunk on the side lane: 0 never,
                                     // 1 always, 2 when the last known chunk flagged at most 2 % of its frames
     // Defaults with the environment's overrides (PALMTRACER_FIT_PRECISION, PALMTRACER_MAX_ITER,

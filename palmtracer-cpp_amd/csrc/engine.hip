@@ -564,6 +564,7 @@ void Engine::detect_chunk(Lane& L, const uint16_t* d_frames, size_t F, int H, in
     L.rois.ensure(F * (npx / 4 + 2) * sizeof(Roi));
     // Deferred frames (d_map set) go through the exact frame flood.
     const bool forceSeq = opt.forceSequential || d_map != nullptr;
+    L.b.denseMask = opt.denseFlagMask;
     launch_segment(L.b, W, H, (int)F, maxPoints, ratio, L.rois.as<Roi>(), st, forceSeq, opt.exactPath, resolve);
     // The segmentation kernels were the last readers of this chunk's P.
     L.clear_mask(F, npx);
@@ -927,6 +928,7 @@ void Engine::debug_labels(const uint16_t* d_frame, int H, int W, double thr, dou
     main_.pDirty = true;
     launch_atrous_detect(d_frame, main_.b, W, H, 1, threshold_to_pmin(thr), stream);
     main_.rois.ensure((npx / 4 + 2) * sizeof(Roi));
+    main_.b.denseMask = opt.denseFlagMask;
     launch_segment(main_.b, W, H, 1, ~0ull, ratio, main_.rois.as<Roi>(), stream, opt.forceSequential, opt.exactPath);
     // Labels only where P > 0 (fin is not written elsewhere).
     masked_labels<<<256, 256, 0, stream>>>(main_.b.fin, main_.b.P, d_labels, npx);

@@ -210,6 +210,7 @@ PT_API int PT_CALL pt_set_option(const char* name, double value)
         else if (n == "camera_read_var") e.opt.camReadVar = std::max(0.0, value);
         else if (n == "stream_pixels") e.opt.streamPixels = value >= 1.0 ? (size_t)value : (size_t(1) << 27);
         else if (n == "host_threads") e.opt.hostThreads = (int)std::max(1.0, std::min(64.0, value));
+        else if (n == "dense_flag_mask") e.opt.denseFlagMask = (uint32_t)std::max(0.0, std::min(4294967295.0, value));
         else if (n == "defer_flagged") e.opt.deferFlagged = (int)std::max(0.0, std::min(2.0, value));
         else if (n == "detect_rows") e.opt.detectRows = (int)std::max(0.0, std::min(1048576.0, value));
         else if (n == "lanes") e.opt.lanes = value >= 2.0 ? 2 : 1;

@@ -365,7 +365,7 @@ __global__ void __launch_bounds__(GF_NT) gf_finalize_kernel(GfArgs a)
     }
 }
 
-// Neighbour table (entries among the 8 neighbours of each entry's pixel).
+// Neighbour table (entries among the 8 neighbours of each entry's pixel), built from the mask list.
 __global__ void __launch_bounds__(GF_NT) gf_neighbours_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
@@ -373,10 +373,19 @@ __global__ void __launch_bounds__(GF_NT) gf_neighbours_kernel(GfArgs a)
         const GfMeta m = a.meta[k];
         if (m.fail || m.n == 0) continue;
         const GfView v = gf_view(a.pool, m);
-        const uint32_t* Pf = a.P + (size_t)a.flagList[k] * a.npx;
-        const uint32_t* fn = a.fin + (size_t)a.flagList[k] * a.npx;
-        for (uint32_t i = blockIdx.x * GF_NT + threadIdx.x; i < m.n; i += gridDim.x * GF_NT) {
-            const uint32_t p = v.spix[i] & ~GF_SEED;
+        const uint32_t f = a.flagList[k];
+        const uint32_t* Pf = a.P + (size_t)f * a.npx;
+        const uint32_t* fn = a.fin + (size_t)f * a.npx;
+        const uint32_t* ml = a.maskList + (size_t)f * a.npx;
+        const uint32_t mc = a.maskCnt[f];
+        // Pixels in mask-list order (neighbouring pixels of one detection strip are adjacent in the
+        // list), so that the label-map reads of neighbouring threads share cache lines; each entry's
+        // row of the table is one 32-byte write.
+        for (uint32_t q = blockIdx.x * GF_NT + threadIdx.x; q < mc; q += gridDim.x * GF_NT) {
+            const uint32_t p = ml[q];
+            const uint32_t t = fn[p];
+            if (!(t & GF_TAG)) continue;   // not an entry (a seed with no unlabelled neighbour)
+            const uint32_t i = t & ~GF_TAG;
             const int y = (int)(p / (uint32_t)a.W), x = (int)(p % (uint32_t)a.W);
             uint32_t e[8];
 #pragma unroll

@@ -89,6 +89,9 @@ struct DetectBuffers {
     hipEvent_t scanWait = nullptr;// with runTotal: the ROI scan waits for it (previous chunk's scan)
     hipEvent_t scanDone = nullptr;// with runTotal: recorded after this chunk's ROI scan
     size_t frames = 0, npx = 0, regionCap = 0, multiCap = 0, fbSlots = 0, rfSlots = 0;
+    // Frames with more mask pixels than this go to the exact frame flood without component floods
+    // (their component-flood proofs fail: the age bound grows with the mask); 0: no such limit.
+    uint32_t denseMask = 0;
 };
 
 // Device counters of one chunk (u32 words).

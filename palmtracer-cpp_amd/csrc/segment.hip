@@ -917,7 +917,7 @@ __global__ void __launch_bounds__(64) flood_kernel(const uint32_t* __restrict__ 
                                                    uint32_t* __restrict__ counters, uint32_t* __restrict__ frameFlag,
                                                    uint32_t* __restrict__ flagList, int W, int H, size_t npx,
                                                    const Fz* __restrict__ fz, const uint32_t* __restrict__ rootCnt,
-                                                   size_t regionCap, const uint32_t* __restrict__ hist)
+                                                   size_t regionCap, const uint32_t* __restrict__ hist, uint32_t denseMask)
 {
     __shared__ uint64_t sPa[QCAP_FLOOD];
     __shared__ double sKey[QCAP_FLOOD];
@@ -930,6 +930,10 @@ __global__ void __launch_bounds__(64) flood_kernel(const uint32_t* __restrict__ 
         const int f = (int)(e >> 32);
         const uint32_t c = (uint32_t)e;
         if (__hip_atomic_load(&frameFlag[f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
+        if (denseMask && maskCnt[f] > denseMask) {
+            if (lane == 0) flag_frame(frameFlag, counters, flagList, f);
+            continue;
+        }
         const uint32_t* Pf = P + (size_t)f * npx;
         const uint32_t* lb = lab + (size_t)f * npx;
         uint32_t* fn = fin + (size_t)f * npx;
@@ -963,7 +967,13 @@ __global__ void __launch_bounds__(64) flood_kernel(const uint32_t* __restrict__ 
         const uint32_t D = age_quanta(maskCnt[f]);
         const uint32_t* cum = hist + (size_t)f * kHistBins;
         uint32_t it = 0;
+        bool dropped = false;
         while (ok && n > 0) {
+            // Another wave flagged the frame: its labels come from the exact frame flood.
+            if ((it & 63u) == 63u && __hip_atomic_load(&frameFlag[f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                dropped = true;
+                break;
+            }
             uint32_t a, la;
             bool proven = true;
             // b (potential ob, age tb) stays behind the popped a (oa, ta) in the frame-global order
@@ -980,7 +990,7 @@ __global__ void __launch_bounds__(64) flood_kernel(const uint32_t* __restrict__ 
             ++it;
             if (!waveq_expand(q, n, Pf, fn, W, H, a, la, it)) { ok = false; overflow = true; break; }
         }
-        if (!ok && lane == 0) {
+        if (!ok && !dropped && lane == 0) {
             atomicAdd(&counters[overflow ? kFlagOverflow : kFlagUnproven], 1u);
             flag_frame(frameFlag, counters, flagList, f);
         }
@@ -2118,7 +2128,7 @@ void launch_segment(const DetectBuffers& b, int W, int H, int F, uint64_t maxPoi
                                                  b.counters, b.frameFlag, b.flagList, npx);
         flood_kernel<<<4096, 64, 0, s>>>(b.P, b.lab, b.fin, b.seedList, b.seedCnt, b.maskCnt, b.multiList, b.multiCap,
                                          b.counters, b.frameFlag, b.flagList, W, H, npx, pass2 ? b.fz : nullptr,
-                                         b.rootCnt, b.regionCap, b.hist);
+                                         b.rootCnt, b.regionCap, b.hist, pass2 ? 0u : b.denseMask);
         if (!resolve) return;
         // Exact frame flood of the flagged frames: the grid rank flood, then the single-workgroup rank
         // flood for the frames it hands over, the wave-queue flood for the frames that one hands over,
