@@ -22,6 +22,10 @@ import os
 import sys
 import time
 
+# The serving leg runs four slot engines of three streams each; HIP maps streams onto this many
+# hardware queues (4 by default), and streams that share a queue wait for each other.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "palmtracer-cpp_amd"))
 
@@ -98,7 +102,7 @@ def fit_precision(pt, torch, stack, thr, fit_params, frames):
     return dict(compare_fits(res[0], res[1]), frames=frames)
 
 
-def serve_latency(pt, torch, seconds, slots, n_cams, sizes, fit_params, load=1.0):
+def serve_latency(pt, torch, seconds, slots, n_cams, sizes, fit_params, load=1.0, chunk_pixels=1 << 27, budget_gb=192):
     """Streamed load: n_cams simulated cameras, each with a fixed frame size (`sizes` in turn), a frame
     rate uniform in [100, 1000] fps (times `load`), chunks of 10-100 frames and an emitter density
     uniform in 0.5-3 per um^2 at 100 nm pixels, capped at 5000 emitters per frame (all seeded). Chunks
@@ -124,8 +128,8 @@ def serve_latency(pt, torch, seconds, slots, n_cams, sizes, fit_params, load=1.0
         cams.append(dict(size=size, fps=fps, chunk=chunk, density=density, emitters=emitters, pool=pool, thr=thr,
                          mpp=mpp, outs=outs, off=0, k=0, busy=[None] * len(outs)))
     torch.cuda.synchronize()
-    budget = 96 << 30
-    srv = pt.Server(slots=slots, budget_bytes=budget, chunk_pixels=1 << 25)
+    budget = int(budget_gb) << 30
+    srv = pt.Server(slots=slots, budget_bytes=budget, chunk_pixels=chunk_pixels)
 
     done = set()
 
@@ -231,7 +235,7 @@ def serve_latency(pt, torch, seconds, slots, n_cams, sizes, fit_params, load=1.0
         "localizations_per_s": rows / ((t_end - base) * 1e-3), "load": load,
         "cameras": [{"size": c["size"], "fps": round(c["fps"], 1), "chunk": c["chunk"], "density_per_um2": round(c["density"], 2),
                      "emitters": c["emitters"]} for c in cams],
-        "slots": slots, "budget_gb": budget >> 30,
+        "slots": slots, "budget_gb": budget >> 30, "chunk_pixels": chunk_pixels,
         "seconds": seconds, "max_submit_lag_ms": float(max(lags)),
         "fit_precision_fast_vs_fp64_10_frames": precision,
     }
@@ -480,6 +484,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--batch-frame-sample", type=int, default=4,
                     help="frames per stack timed through the plain per-frame path")
+    ap.add_argument("--serve-chunk-pixels", type=int, default=1 << 27,
+                    help="pixels per engine chunk of a serving slot")
+    ap.add_argument("--serve-budget-gb", type=int, default=192, help="HBM budget of the serving loop")
     ap.add_argument("--serve-sizes", default="256,512,1024,2048",
                     help="frame sides of the cameras, assigned in turn")
     ap.add_argument("--serve-load", type=float, default=1.0,
@@ -622,7 +629,7 @@ def main():
     if args.serve_seconds > 0:
         sizes = [int(v) for v in args.serve_sizes.split(",")]
         serving = serve_latency(pt, torch, args.serve_seconds, args.serve_slots, args.serve_cameras, sizes, fit_params,
-                                args.serve_load)
+                                args.serve_load, args.serve_chunk_pixels, args.serve_budget_gb)
 
     astig = astig3d_leg(pt, torch, args.astig_frames, args.astig_steps) if args.astig_frames > 0 else None
 

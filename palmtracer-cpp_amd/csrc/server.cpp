@@ -59,6 +59,10 @@ public:
             ctx_.push_back(std::make_unique<Engine>());
             ctx_.back()->opt = base;
             ctx_.back()->opt.profile = false;
+            // A request is a chunk of at most a few hundred frames: deferring its flagged frames to the
+            // side lane (which detects them again) costs more than it overlaps, so slots resolve them
+            // in line.
+            ctx_.back()->opt.deferFlagged = 0;
             if (chunkPixels) ctx_.back()->opt.chunkPixels = chunkPixels;
         }
         held_.assign(slots, 0);
