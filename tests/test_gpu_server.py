@@ -134,3 +134,39 @@ def test_server_work_space_estimate_bounds_allocation(pt_gpu, cameras):
         srv.wait(srv.submit(d, o, thr, 0.0, 2, FIT_PARAMS, mpp))
         used = free0 - torch.cuda.mem_get_info()[0]
     assert used <= ws, (used, ws)
+
+
+def test_served_dense_cameras_match_oracle(pt_gpu, oracle_mod):
+    """Requests at the served load's density, in flight together on two slots with the fast fit and
+    the 20-iteration cap: a 2048^2 camera chunk at 5000 emitters per frame (every frame flagged and
+    flooded whole) and a 256^2 one at 1500; every served row within the fit tolerance of the oracle."""
+    import torch
+    from helpers import fit_errors, fit_mismatch_rows
+    specs = [(2048, 3, 5000, 0x5EED6001), (256, 10, 1500, 0x5EED6002)]
+    chunks = []
+    for side, frames, emitters, seed in specs:
+        d = torch.empty((frames, side, side), dtype=torch.uint16, device="cuda")
+        pt_gpu.synth_device(d, seed=seed, emitters=emitters)
+        st = d.cpu().numpy()
+        chunks.append((d, st, oracle_mod.autothreshold(st[0]), int(emitters * 1.25) + 64))
+    pt_gpu.set_option("fit_fp64", 0)
+    pt_gpu.set_option("max_iter", 20)
+    try:
+        with pt_gpu.Server(slots=2, budget_bytes=64 << 30, chunk_pixels=1 << 25) as srv:
+            outs, ids = [], []
+            for d, st, thr, mpp in chunks:
+                o = torch.full((d.shape[0] * mpp * 18,), -7.0, dtype=torch.float64, device="cuda")
+                outs.append(o)
+                ids.append(srv.submit(d, o, thr, 0.0, 2, FIT_PARAMS, mpp))
+            got = []
+            for rid, o in zip(ids, outs):
+                n, _ = srv.wait(rid)
+                got.append(o[: n * 18].view(-1, 18).cpu().numpy())
+    finally:
+        pt_gpu.set_option("fit_fp64", 1)
+        pt_gpu.set_option("max_iter", 75)
+    for (d, st, thr, mpp), g in zip(chunks, got):
+        exp = oracle_mod.localization(st, thr, 0.0, 2, FIT_PARAMS, output_size=st.shape[0] * mpp * 18, max_iter=20)
+        assert g.shape == exp.shape and len(exp) > 0, (g.shape, exp.shape)
+        bad = fit_mismatch_rows(g, exp)
+        assert bad.size == 0, (bad.size, fit_errors(g, exp))
