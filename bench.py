@@ -102,6 +102,66 @@ def fit_precision(pt, torch, stack, thr, fit_params, frames):
     return dict(compare_fits(res[0], res[1]), frames=frames)
 
 
+def serve_cameras(pt, torch, n_cams, sizes, load, rng, pool_frames=128, only=None):
+    """The simulated cameras of the streamed load (seeded): frame side `sizes[c % len(sizes)]`, a frame
+    rate uniform in [100, 1000] fps (times `load`), chunks of 10-100 frames, 0.5-3 emitters per um^2 at
+    100 nm pixels capped at 5000 per frame, a pool of generated frames on the device. `only`: frame
+    sides to build (the others are drawn but skipped)."""
+    cams = []
+    for c in range(n_cams):
+        size = sizes[c % len(sizes)]
+        fps = rng.uniform(100.0, 1000.0) * load
+        chunk = rng.randint(10, 100)
+        density = rng.uniform(0.5, 3.0)
+        emitters = int(min(5000, round(density * size * size * 1e-2)))
+        if only and size not in only:
+            continue
+        pool = torch.empty((max(pool_frames, chunk), size, size), dtype=torch.uint16, device="cuda")
+        pt.synth_device(pool, seed=0x5EED1000 + c, emitters=emitters)
+        thr = pt.autothreshold_device(pool[0])
+        mpp = int(emitters * 1.25) + 64
+        outs = [torch.empty(chunk * mpp * 18, dtype=torch.float64, device="cuda") for _ in range(8)]
+        cams.append(dict(cam=c, size=size, fps=fps, chunk=chunk, density=density, emitters=emitters, pool=pool, thr=thr,
+                         mpp=mpp, outs=outs, off=0, k=0, busy=[None] * len(outs)))
+    torch.cuda.synchronize()
+    return cams
+
+
+def serve_alone(pt, torch, n_cams, sizes, fit_params, reps, chunk_pixels, only=None):
+    """Each camera's chunk alone: one Localization over one chunk with the GPU otherwise idle, in the
+    serving slots' configuration (chunk_pixels, flagged frames resolved in line); the sum over the
+    cameras of (alone time x chunks per second) is the GPU time the offered load asks for."""
+    import random
+    rng = random.Random(0x5EED0005)
+    cams = serve_cameras(pt, torch, n_cams, sizes, 1.0, rng, pool_frames=0, only=only)
+    pt.set_option("chunk_pixels", chunk_pixels)
+    pt.set_option("defer_flagged", 0)
+    out = []
+    try:
+        for cam in cams:
+            d = cam["pool"][: cam["chunk"]]
+            o = cam["outs"][0]
+            pt.localization_device(d, o, cam["thr"], 0.0, 2, fit_params, cam["mpp"])
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                n = pt.localization_device(d, o, cam["thr"], 0.0, 2, fit_params, cam["mpp"])
+                torch.cuda.synchronize()
+                ts.append(1e3 * (time.perf_counter() - t0))
+            st = pt.last_stats()
+            ms = min(ts)
+            out.append({"camera": cam["cam"], "size": cam["size"], "frames": cam["chunk"], "emitters": cam["emitters"],
+                        "rows": n, "ms": ms, "gpu_ms_per_s": ms * cam["fps"] / cam["chunk"],
+                        "flagged_frames": st["flagged_frames"], "mask_pixels_per_frame": st["mask_pixels"] / cam["chunk"]})
+            del cam["pool"], cam["outs"]
+    finally:
+        pt.set_option("chunk_pixels", 0)
+        pt.set_option("defer_flagged", 2)
+    return {"requests": out, "gpu_ms_per_s": sum(r["gpu_ms_per_s"] for r in out),
+            "note": "each camera's chunk alone (min of the repetitions); gpu_ms_per_s = ms x the camera's chunks per second"}
+
+
 def serve_latency(pt, torch, seconds, slots, n_cams, sizes, fit_params, load=1.0, chunk_pixels=1 << 27, budget_gb=192):
     """Streamed load: n_cams simulated cameras, each with a fixed frame size (`sizes` in turn), a frame
     rate uniform in [100, 1000] fps (times `load`), chunks of 10-100 frames and an emitter density
@@ -113,21 +173,7 @@ def serve_latency(pt, torch, seconds, slots, n_cams, sizes, fit_params, load=1.0
     import threading
     import numpy as np
     rng = random.Random(0x5EED0005)
-    cams = []
-    for c in range(n_cams):
-        size = sizes[c % len(sizes)]
-        fps = rng.uniform(100.0, 1000.0) * load
-        chunk = rng.randint(10, 100)
-        density = rng.uniform(0.5, 3.0)
-        emitters = int(min(5000, round(density * size * size * 1e-2)))
-        pool = torch.empty((128, size, size), dtype=torch.uint16, device="cuda")
-        pt.synth_device(pool, seed=0x5EED1000 + c, emitters=emitters)
-        thr = pt.autothreshold_device(pool[0])
-        mpp = int(emitters * 1.25) + 64
-        outs = [torch.empty(chunk * mpp * 18, dtype=torch.float64, device="cuda") for _ in range(8)]
-        cams.append(dict(size=size, fps=fps, chunk=chunk, density=density, emitters=emitters, pool=pool, thr=thr,
-                         mpp=mpp, outs=outs, off=0, k=0, busy=[None] * len(outs)))
-    torch.cuda.synchronize()
+    cams = serve_cameras(pt, torch, n_cams, sizes, load, rng)
     budget = int(budget_gb) << 30
     srv = pt.Server(slots=slots, budget_bytes=budget, chunk_pixels=chunk_pixels)
 
@@ -487,6 +533,10 @@ def main():
     ap.add_argument("--serve-chunk-pixels", type=int, default=1 << 27,
                     help="pixels per engine chunk of a serving slot")
     ap.add_argument("--serve-budget-gb", type=int, default=192, help="HBM budget of the serving loop")
+    ap.add_argument("--serve-alone", type=int, default=0, metavar="REPS",
+                    help="only time each serving camera's chunk alone (REPS repetitions) and print that, then exit")
+    ap.add_argument("--serve-alone-sizes", default="",
+                    help="with --serve-alone: frame sides to time (default: every camera)")
     ap.add_argument("--serve-sizes", default="256,512,1024,2048",
                     help="frame sides of the cameras, assigned in turn")
     ap.add_argument("--serve-load", type=float, default=1.0,
@@ -512,6 +562,18 @@ def main():
         print(json.dumps({"copy_bytes": args.copy_bytes}))
         return
     import palmtracer_amd as pt
+
+    if args.serve_alone > 0:
+        pt.set_option("fit_fp64", 0)
+        pt.set_option("max_iter", args.max_iter)
+        for kv in args.set:
+            name, value = kv.split("=", 1)
+            pt.set_option(name, float(value))
+        sizes = [int(v) for v in args.serve_sizes.split(",")]
+        only = [int(v) for v in args.serve_alone_sizes.split(",")] if args.serve_alone_sizes else None
+        print(json.dumps({"serve_alone": serve_alone(pt, torch, args.serve_cameras, sizes, (7.0, 1.3, 1.3, 0.0),
+                                                     args.serve_alone, args.serve_chunk_pixels, only)}))
+        return
 
     F, S = args.frames, args.size
     fit_params = (7.0, 1.3, 1.3, 0.0)
