@@ -406,6 +406,18 @@ __device__ inline void gf_nbrs(const GfView& v, uint32_t i, uint32_t e[8])
     e[4] = b.x; e[5] = b.y; e[6] = b.z; e[7] = b.w;
 }
 
+// a[e[j]] for the 8 neighbour entries, every load issued before any is used (no per-neighbour
+// branch, which would wait for each load in turn); a neighbour without an entry reads entry 0 (every
+// frame that runs these steps has one) and gets GF_NONE.
+__device__ inline void gf_gather8(const uint32_t* a, const uint32_t e[8], uint32_t out[8])
+{
+    uint32_t raw[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) raw[j] = a[e[j] == GF_NONE ? 0u : e[j]];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) out[j] = e[j] == GF_NONE ? GF_NONE : raw[j];
+}
+
 // Frames still iterating at step `it` (the previous step changed some rank).
 __device__ inline bool gf_active(const GfMeta& m, int it) { return !m.fail && m.n > 0 && (it == 0 || m.changed[it - 1]); }
 
@@ -421,12 +433,12 @@ __global__ void __launch_bounds__(GF_NT) gf_age_kernel(GfArgs a, int it)
         for (uint32_t i = blockIdx.x * GF_NT + threadIdx.x; i < m.n; i += gridDim.x * GF_NT) {
             uint32_t t = 0;
             if (!(v.spix[i] & GF_SEED)) {
-                t = GF_NONE;
-                uint32_t e[8];
+                uint32_t e[8], g[8];
                 gf_nbrs(v, i, e);
+                gf_gather8(G, e, g);
+                t = GF_NONE;
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (e[j] != GF_NONE) t = min(t, G[e[j]]);
+                for (int j = 0; j < 8; ++j) t = min(t, g[j]);
             }
             v.T[i] = t;
             v.K[i] = t == GF_NONE ? 1.79769313486231570815e308 : age_key(v.sp[i], t);
@@ -494,11 +506,12 @@ __global__ void __launch_bounds__(GF_NT) gf_check1_kernel(GfArgs a)
                     if (!(v.T[i] < g)) bad = kGfOrder;
                     uint32_t bg = GF_NONE;
                     pa = GF_NONE;
-                    uint32_t e[8];
+                    uint32_t e[8], g[8];
                     gf_nbrs(v, i, e);
+                    gf_gather8(G, e, g);
 #pragma unroll
                     for (int j = 0; j < 8; ++j)
-                        if (e[j] != GF_NONE && G[e[j]] < bg) { bg = G[e[j]]; pa = e[j]; }
+                        if (g[j] < bg) { bg = g[j]; pa = e[j]; }
                     if (pa == GF_NONE) bad |= kGfOrder;
                 }
                 v.par[i] = pa;
@@ -585,15 +598,16 @@ __global__ void __launch_bounds__(GF_NT) gf_ival_key_kernel(GfArgs a, int it)
         for (uint32_t i = blockIdx.x * GF_NT + threadIdx.x; i < m.n; i += gridDim.x * GF_NT) {
             uint32_t tlo = 0, thi = 0;
             if (!(v.spix[i] & GF_SEED)) {
-                tlo = thi = GF_NONE;
-                uint32_t e[8];
+                uint32_t e[8], glo[8], ghi[8];
                 gf_nbrs(v, i, e);
+                gf_gather8(v.Gb, e, glo);
+                gf_gather8(v.T, e, ghi);
+                tlo = thi = GF_NONE;
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (e[j] != GF_NONE) {
-                        tlo = min(tlo, v.Gb[e[j]]);
-                        thi = min(thi, v.T[e[j]]);
-                    }
+                for (int j = 0; j < 8; ++j) {
+                    tlo = min(tlo, glo[j]);
+                    thi = min(thi, ghi[j]);
+                }
             }
             v.K[i] = tlo == GF_NONE ? 1.79769313486231570815e308 : age_key(v.sp[i], tlo);
             khi[i] = thi == GF_NONE ? 1.79769313486231570815e308 : age_key(v.sp[i], thi);
@@ -651,17 +665,19 @@ __global__ void __launch_bounds__(GF_NT) gf_ival_check_kernel(GfArgs a)
             const uint32_t i = b + (threadIdx.x & 63);
             uint32_t bad = 0;
             if (i < m.n && !(v.spix[i] & GF_SEED)) {
-                uint32_t e[8];
+                uint32_t e[8], th[8], lo[8], lb[8];
                 gf_nbrs(v, i, e);
+                gf_gather8(v.T, e, th);
+                gf_gather8(v.Gb, e, lo);
+                gf_gather8(v.lbl, e, lb);
                 uint32_t thi = GF_NONE;
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (e[j] != GF_NONE) thi = min(thi, v.T[e[j]]);
+                for (int j = 0; j < 8; ++j) thi = min(thi, th[j]);
                 if (!(thi < v.Gb[i])) bad = kGfTie;
                 const uint32_t L = v.lbl[i];
 #pragma unroll
                 for (int j = 0; j < 8; ++j)
-                    if (e[j] != GF_NONE && v.Gb[e[j]] <= thi && v.lbl[e[j]] != L) bad = kGfTie;
+                    if (e[j] != GF_NONE && lo[j] <= thi && lb[j] != L) bad = kGfTie;
             }
             gf_or(&a.meta[k].fail, bad);
         }
