@@ -175,10 +175,11 @@ PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint6
  * "fit_objective" (0: the reference's least squares; 1: Poisson maximum likelihood of fit modes 1-4,
  * FP64, an extension with no counterpart in the reference), "camera_gain" (ADU per photon, 1),
  * "camera_offset" (ADU, 0) and "camera_read_var" (ADU^2, 0): the camera model of that likelihood.
- * Returns 1 on success. */
+ * Options apply to every call that starts after this one returns, whichever engine serves it
+ * (concurrent calls run on separate engines). Returns 1 on success. */
 PT_API int PT_CALL pt_set_option(const char* name, double value);
 
-/* Counters of the last call: frames, chunks, rows, frames flagged for the exact frame flood,
+/* Counters of the call that finished last (of concurrent calls, the last to return): frames, chunks, rows, frames flagged for the exact frame flood,
  * multi-label components, frames deferred to the final pass, status, component floods given up on a
  * full queue, component floods given up on an unproven order, pops of the wave-queue frame floods,
  * their largest queue, frames the rank flood handed over (too many entries, equal keys, not

@@ -337,14 +337,19 @@ void Engine::batch_localize(BatchItem* items, uint64_t nItems)
 extern "C" PT_API int PT_CALL pt_batch_localization(pt_batch_stack* stacks, uint64_t nStacks)
 {
     try {
-        pt::Engine& e = pt::Engine::get();
+        pt::Engine& e = pt::Engine::get();   // batches keep their tuned schedules and captured graph here
         std::lock_guard<std::mutex> lk(e.mu);
+        e.opt = pt::Engine::shared_options();
         std::vector<pt::BatchItem> items(nStacks);
         for (uint64_t s = 0; s < nStacks; ++s) {
             const pt_batch_stack& b = stacks[s];
             items[s] = pt::BatchItem{b.d_input, b.planes, b.height, b.width, b.thresholdVal, b.watershedRatio, b.fit,
                                      b.fit_params, b.d_output, b.outputCapDoubles, b.maxPerPlane, 0};
         }
+        struct Publish {
+            pt::Engine& e;
+            ~Publish() { pt::Engine::publish(e.last); }
+        } publish{e};
         e.batch_localize(items.data(), nStacks);
         for (uint64_t s = 0; s < nStacks; ++s) stacks[s].doublesWritten = items[s].doublesWritten;
         return 1;

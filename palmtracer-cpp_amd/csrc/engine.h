@@ -144,10 +144,15 @@ class Engine {
 public:
     // The process-wide engine of the C entry points (lives until the process ends).
     static Engine& get();
-    // The process-wide engine if it exists already, else nullptr (does not create it).
-    static Engine* existing();
     // An independent engine (own streams and buffers), e.g. one per serving slot.
     Engine();
+    // Options of the C entry points (pt_set_option; the environment's defaults until then). Every
+    // engine a call leases starts from them.
+    static Options shared_options();
+    static void set_shared_options(const Options& o);
+    // Counters of the call that finished last, whichever engine ran it (pt_last_stats, pt_stage_times).
+    static void publish(const RunStats& s);
+    static RunStats published();
     ~Engine();
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
@@ -228,6 +233,26 @@ private:
     std::string batchGraphKey_;
     hipEvent_t forkEv_[2] = {nullptr, nullptr};
     std::vector<BatchClassInfo> batchLast_;
+};
+
+// The engine of one call of a C entry point. Concurrent calls run independently: a call takes the
+// process-wide engine when it is free, else a free pooled engine (own streams and buffers, created
+// on first need, at most kPoolEngines of them, kept until the process ends), else it waits for the
+// first of them to come free. The leased engine starts from the shared options; its counters are
+// published when the lease ends.
+class Lease {
+public:
+    static constexpr int kPoolEngines = 4;
+    Lease();
+    ~Lease();
+    Lease(const Lease&) = delete;
+    Lease& operator=(const Lease&) = delete;
+    Engine& operator*() const { return *e_; }
+    Engine* operator->() const { return e_; }
+
+private:
+    Engine* e_ = nullptr;
+    int slot_ = -1;   // pool slot, or -1 for the process-wide engine (its mutex is held)
 };
 
 } // namespace pt

@@ -2,7 +2,8 @@
 #include "engine.h"
 
 #include <algorithm>
-#include <atomic>
+#include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -160,21 +161,110 @@ void DevBuf::release()
     bytes = 0;
 }
 
-namespace {
-std::atomic<Engine*> g_engine{nullptr};
-}
-
 Engine& Engine::get()
 {
-    static Engine* e = [] {
-        Engine* n = new Engine();  // lives until process exit
-        g_engine.store(n);
-        return n;
-    }();
+    static Engine* e = new Engine();  // lives until process exit
     return *e;
 }
 
-Engine* Engine::existing() { return g_engine.load(); }
+namespace {
+struct Shared {
+    std::mutex optMu, statMu, poolMu;
+    Options opt = Options::defaults();
+    RunStats stat;
+    std::condition_variable poolCv;
+    Engine* pool[Lease::kPoolEngines] = {};
+    bool busy[Lease::kPoolEngines] = {};
+};
+Shared& shared()
+{
+    static Shared* s = new Shared();   // lives until process exit, like the engines it lists
+    return *s;
+}
+} // namespace
+
+Options Engine::shared_options()
+{
+    Shared& sh = shared();
+    std::lock_guard<std::mutex> lk(sh.optMu);
+    return sh.opt;
+}
+
+void Engine::set_shared_options(const Options& o)
+{
+    Shared& sh = shared();
+    std::lock_guard<std::mutex> lk(sh.optMu);
+    sh.opt = o;
+}
+
+void Engine::publish(const RunStats& s)
+{
+    Shared& sh = shared();
+    std::lock_guard<std::mutex> lk(sh.statMu);
+    sh.stat = s;
+}
+
+RunStats Engine::published()
+{
+    Shared& sh = shared();
+    std::lock_guard<std::mutex> lk(sh.statMu);
+    return sh.stat;
+}
+
+Lease::Lease()
+{
+    Engine& g = Engine::get();
+    Shared& sh = shared();
+    bool create = false;
+    for (;;) {
+        if (g.mu.try_lock()) {
+            e_ = &g;
+            break;
+        }
+        std::unique_lock<std::mutex> lk(sh.poolMu);
+        int pick = -1;
+        for (int i = 0; i < Lease::kPoolEngines && pick < 0; ++i)   // an engine that exists already first
+            if (!sh.busy[i] && sh.pool[i]) pick = i;
+        for (int i = 0; i < Lease::kPoolEngines && pick < 0; ++i)
+            if (!sh.busy[i]) pick = i;
+        if (pick >= 0) {
+            sh.busy[pick] = true;
+            slot_ = pick;
+            e_ = sh.pool[pick];
+            create = e_ == nullptr;
+            break;
+        }
+        // The process-wide engine's mutex is also taken by calls that do not lease (options, batches),
+        // which do not notify: wake now and then to try it again.
+        sh.poolCv.wait_for(lk, std::chrono::milliseconds(2));
+    }
+    if (create) {
+        try {
+            e_ = new Engine();
+        } catch (...) {
+            std::lock_guard<std::mutex> lk(sh.poolMu);
+            sh.busy[slot_] = false;
+            sh.poolCv.notify_all();
+            throw;
+        }
+        std::lock_guard<std::mutex> lk(sh.poolMu);
+        sh.pool[slot_] = e_;
+    }
+    e_->opt = Engine::shared_options();
+}
+
+Lease::~Lease()
+{
+    Engine::publish(e_->last);
+    Shared& sh = shared();
+    if (slot_ < 0) {
+        e_->mu.unlock();
+    } else {
+        std::lock_guard<std::mutex> lk(sh.poolMu);
+        sh.busy[slot_] = false;
+    }
+    sh.poolCv.notify_all();
+}
 
 Options Options::defaults()
 {

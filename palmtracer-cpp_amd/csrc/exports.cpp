@@ -46,8 +46,8 @@ PT_API uint64_t PT_CALL Localization(const uint16_t* input, double* output, uint
                                      uint64_t fit, const double* fit_params)
 {
     return guarded("Localization", [&]() -> uint64_t {
-        Engine& e = Engine::get();
-        std::lock_guard<std::mutex> lk(e.mu);
+        pt::Lease lease;
+        Engine& e = *lease;
         if (height > 0x7FFFFFFFull || width > 0x7FFFFFFFull) throw pt::Error("frame too large");
         return e.localize_host(input, output, outputSize, (int)height, (int)width, planes, thresholdVal, watershedRatio, fit,
                                fit_params);
@@ -61,8 +61,8 @@ PT_API uint64_t PT_CALL Fitting(const uint16_t* input, const double* positions, 
     (void)outputSize;  // the reference ignores it too (lib/Palm.cpp:74)
     return guarded("Fitting", [&]() -> uint64_t {
         if (nPoints == 0) return 0;
-        Engine& e = Engine::get();
-        std::lock_guard<std::mutex> lk(e.mu);
+        pt::Lease lease;
+        Engine& e = *lease;
         if (height > 0x7FFFFFFFull || width > 0x7FFFFFFFull) throw pt::Error("frame too large");
         return e.fit_positions_host(input, planes, (int)height, (int)width, positions, nPoints, fit, fit_params, output);
     }, 0);
@@ -75,8 +75,8 @@ PT_API void PT_CALL Wavelett(const uint16_t* input, double* output, uint64_t hei
         if (planes == 0 || height == 0 || width == 0) return 0;
         if (levelOutput == 0) levelOutput = 1;                   // CPU/main.cpp:262
         const int level = (int)std::min<uint64_t>(levelOutput - 1, 2);
-        Engine& e = Engine::get();
-        std::lock_guard<std::mutex> lk(e.mu);
+        pt::Lease lease;
+        Engine& e = *lease;
         const size_t npx = (size_t)height * width;
         // Bounded device footprint: planes in chunks.
         const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(planes, (size_t(256) << 20) / (npx * 10)));
@@ -95,8 +95,8 @@ PT_API void PT_CALL Wavelett(const uint16_t* input, double* output, uint64_t hei
 PT_API double PT_CALL AutoThreshold(const uint16_t* input, uint64_t height, uint64_t width, const double* fit_params)
 {
     return guarded("AutoThreshold", [&]() -> double {
-        Engine& e = Engine::get();
-        std::lock_guard<std::mutex> lk(e.mu);
+        pt::Lease lease;
+        Engine& e = *lease;
         if (height == 0 || width == 0) return std::nan("");
         HostToDevice h2d(e.in, input, (size_t)height * width, e.stream);
         return e.autothreshold(h2d.dptr, (int)height, (int)width, fit_params);
@@ -108,8 +108,8 @@ PT_API void PT_CALL Astigmatism3DEstimation(const double* input, double* output,
 {
     guarded("Astigmatism3DEstimation", [&]() -> int {
         if (size == 0) return 0;
-        Engine& e = Engine::get();
-        std::lock_guard<std::mutex> lk(e.mu);
+        pt::Lease lease;
+        Engine& e = *lease;
         e.in.ensure(size * 16);
         e.out.ensure(size * 16);
         PT_CHECK(hipMemcpyAsync(e.in.p, input, size * 16, hipMemcpyHostToDevice, e.stream));
@@ -124,8 +124,8 @@ PT_API int PT_CALL pt_astig_estimate_device(const double* d_input, double* d_out
                                             const double* model, double zMax)
 {
     return guarded("pt_astig_estimate_device", [&]() -> int {
-        Engine& e = Engine::get();
-        std::lock_guard<std::mutex> lk(e.mu);
+        pt::Lease lease;
+        Engine& e = *lease;
         pt::launch_astig_estimate(d_input, d_output, size, pixelSize, model, zMax, e.stream);
         PT_CHECK(hipStreamSynchronize(e.stream));
         return 1;
@@ -136,8 +136,8 @@ PT_API double PT_CALL pt_autothreshold_device(const uint16_t* d_input, uint64_t 
                                               const double* fit_params)
 {
     return guarded("pt_autothreshold_device", [&]() -> double {
-        Engine& e = Engine::get();
-        std::lock_guard<std::mutex> lk(e.mu);
+        pt::Lease lease;
+        Engine& e = *lease;
         return e.autothreshold(d_input, (int)height, (int)width, fit_params);
     }, std::nan(""));
 }
@@ -150,8 +150,8 @@ PT_API uint64_t PT_CALL pt_localization_device(const uint16_t* d_input, double* 
                                                uint64_t maxPerPlane)
 {
     return guarded("pt_localization_device", [&]() -> uint64_t {
-        Engine& e = Engine::get();
-        std::lock_guard<std::mutex> lk(e.mu);
+        pt::Lease lease;
+        Engine& e = *lease;
         return e.localize(d_input, planes, (int)height, (int)width, thresholdVal, watershedRatio, fit, fit_params,
                           maxPerPlane, d_output, outputCapDoubles / pt::kCols);
     }, 0);
@@ -161,8 +161,8 @@ PT_API int PT_CALL pt_wavelet_device(const uint16_t* d_input, double* d_output, 
                                      uint64_t planes, uint64_t levelOutput)
 {
     return guarded("pt_wavelet_device", [&]() -> int {
-        Engine& e = Engine::get();
-        std::lock_guard<std::mutex> lk(e.mu);
+        pt::Lease lease;
+        Engine& e = *lease;
         if (levelOutput == 0) levelOutput = 1;
         e.wavelet(d_input, planes, (int)height, (int)width, (int)std::min<uint64_t>(levelOutput - 1, 2), d_output);
         return 1;
@@ -173,8 +173,8 @@ PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint6
                                    double watershedRatio, uint32_t* labels)
 {
     return guarded("pt_debug_labels", [&]() -> int {
-        Engine& e = Engine::get();
-        std::lock_guard<std::mutex> lk(e.mu);
+        pt::Lease lease;
+        Engine& e = *lease;
         const size_t npx = (size_t)height * width;
         HostToDevice h2d(e.in, image, npx, e.stream);
         e.out.ensure(npx * 4);
@@ -188,57 +188,59 @@ PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint6
 PT_API int PT_CALL pt_set_option(const char* name, double value)
 {
     return guarded("pt_set_option", [&]() -> int {
-        Engine& e = Engine::get();
-        std::lock_guard<std::mutex> lk(e.mu);
+        static std::mutex setMu;   // one read-modify-write of the shared options at a time
+        std::lock_guard<std::mutex> lk(setMu);
+        pt::Options o = Engine::shared_options();
         const std::string n(name ? name : "");
-        if (n == "fit_fp64") e.opt.fitFp64 = value != 0.0;
-        else if (n == "max_iter") e.opt.maxIter = std::max(1, (int)value);
-        else if (n == "chunk_pixels") e.opt.chunkPixels = value > 0.0 ? (size_t)value : pt::Options::kDefaultChunkPixels;
-        else if (n == "profile") e.opt.profile = value != 0.0;
-        else if (n == "force_sequential_watershed") e.opt.forceSequential = value != 0.0;
-        else if (n == "force_heap_watershed") e.opt.exactPath = value != 0.0 ? 2 : 0;
-        else if (n == "force_wave_flood") e.opt.exactPath = value != 0.0 ? 1 : 0;
-        else if (n == "force_rank_flood") e.opt.exactPath = value != 0.0 ? 3 : 0;
-        else if (n == "fit_waves") e.opt.fitWaves = (unsigned)std::max(0.0, std::min(1048576.0, value));
-        else if (n == "fit_refill") e.opt.fitRefill = (int)std::max(1.0, std::min(64.0, value));
-        else if (n == "fit_handover") e.opt.fitHandover = (int)std::max(0.0, std::min(1e6, value));
-        else if (n == "batch_autotune") e.opt.batchAutotune = value != 0.0;
-        else if (n == "batch_graph") e.opt.batchGraph = value != 0.0;
-        else if (n == "fit_objective") e.opt.fitObjective = value != 0.0 ? 1 : 0;
-        else if (n == "camera_gain") e.opt.camGain = value;
-        else if (n == "camera_offset") e.opt.camOffset = value;
-        else if (n == "camera_read_var") e.opt.camReadVar = std::max(0.0, value);
-        else if (n == "stream_pixels") e.opt.streamPixels = value >= 1.0 ? (size_t)value : (size_t(1) << 27);
-        else if (n == "host_threads") e.opt.hostThreads = (int)std::max(1.0, std::min(64.0, value));
-        else if (n == "dense_flag_mask") e.opt.denseFlagMask = (uint32_t)std::max(0.0, std::min(4294967295.0, value));
-        else if (n == "defer_flagged") e.opt.deferFlagged = (int)std::max(0.0, std::min(2.0, value));
-        else if (n == "detect_rows") e.opt.detectRows = (int)std::max(0.0, std::min(1048576.0, value));
-        else if (n == "lanes") e.opt.lanes = value >= 2.0 ? 2 : 1;
+        if (n == "fit_fp64") o.fitFp64 = value != 0.0;
+        else if (n == "max_iter") o.maxIter = std::max(1, (int)value);
+        else if (n == "chunk_pixels") o.chunkPixels = value > 0.0 ? (size_t)value : pt::Options::kDefaultChunkPixels;
+        else if (n == "profile") o.profile = value != 0.0;
+        else if (n == "force_sequential_watershed") o.forceSequential = value != 0.0;
+        else if (n == "force_heap_watershed") o.exactPath = value != 0.0 ? 2 : 0;
+        else if (n == "force_wave_flood") o.exactPath = value != 0.0 ? 1 : 0;
+        else if (n == "force_rank_flood") o.exactPath = value != 0.0 ? 3 : 0;
+        else if (n == "fit_waves") o.fitWaves = (unsigned)std::max(0.0, std::min(1048576.0, value));
+        else if (n == "fit_refill") o.fitRefill = (int)std::max(1.0, std::min(64.0, value));
+        else if (n == "fit_handover") o.fitHandover = (int)std::max(0.0, std::min(1e6, value));
+        else if (n == "batch_autotune") o.batchAutotune = value != 0.0;
+        else if (n == "batch_graph") o.batchGraph = value != 0.0;
+        else if (n == "fit_objective") o.fitObjective = value != 0.0 ? 1 : 0;
+        else if (n == "camera_gain") o.camGain = value;
+        else if (n == "camera_offset") o.camOffset = value;
+        else if (n == "camera_read_var") o.camReadVar = std::max(0.0, value);
+        else if (n == "stream_pixels") o.streamPixels = value >= 1.0 ? (size_t)value : (size_t(1) << 27);
+        else if (n == "host_threads") o.hostThreads = (int)std::max(1.0, std::min(64.0, value));
+        else if (n == "dense_flag_mask") o.denseFlagMask = (uint32_t)std::max(0.0, std::min(4294967295.0, value));
+        else if (n == "defer_flagged") o.deferFlagged = (int)std::max(0.0, std::min(2.0, value));
+        else if (n == "detect_rows") o.detectRows = (int)std::max(0.0, std::min(1048576.0, value));
+        else if (n == "lanes") o.lanes = value >= 2.0 ? 2 : 1;
         else throw pt::Error("unknown option '" + n + "'");
+        Engine::set_shared_options(o);
         return 1;
     }, 0);
 }
 
 PT_API int PT_CALL pt_last_stats(uint64_t* out, uint64_t n)
 {
-    Engine& e = Engine::get();
-    const uint64_t v[24] = {e.last.frames, e.last.chunks, e.last.rois, e.last.flaggedFrames, e.last.multiComponents,
-                            e.last.deferredFrames, e.last.status, e.last.flagOverflow, e.last.flagUnproven,
-                            e.last.floodPops, e.last.floodMaxQueue, e.last.rfTooBig, e.last.rfTie,
-                            e.last.rfUnsettled, e.last.fbOverflow, e.last.heapFrames, e.last.rfTwoOrders,
-                            e.last.fitHanded, e.last.gridFrames, e.last.gridTies, e.last.gridUnsettled,
-                            e.last.gridRejected, e.last.gridOther, e.last.maskPixels};
+    const pt::RunStats last = Engine::published();
+    const uint64_t v[24] = {last.frames, last.chunks, last.rois, last.flaggedFrames, last.multiComponents,
+                            last.deferredFrames, last.status, last.flagOverflow, last.flagUnproven,
+                            last.floodPops, last.floodMaxQueue, last.rfTooBig, last.rfTie,
+                            last.rfUnsettled, last.fbOverflow, last.heapFrames, last.rfTwoOrders,
+                            last.fitHanded, last.gridFrames, last.gridTies, last.gridUnsettled,
+                            last.gridRejected, last.gridOther, last.maskPixels};
     for (uint64_t i = 0; i < n && i < 24; ++i) out[i] = v[i];
     return 24;
 }
 
 PT_API int PT_CALL pt_stage_times(double* out, uint64_t n)
 {
-    Engine& e = Engine::get();
-    const double v[13] = {e.last.msAtrous, e.last.msSegment, e.last.msFit, e.last.msAssemble, (double)e.last.chunks,
-                          (double)e.last.jacPasses, (double)e.last.trialPasses, e.last.msDeferred,
-                          (double)e.last.groupJacPasses, (double)e.last.groupTrialPasses, e.last.msH2D, e.last.msD2H,
-                          (double)e.last.streamPieces};
+    const pt::RunStats last = Engine::published();
+    const double v[13] = {last.msAtrous, last.msSegment, last.msFit, last.msAssemble, (double)last.chunks,
+                          (double)last.jacPasses, (double)last.trialPasses, last.msDeferred,
+                          (double)last.groupJacPasses, (double)last.groupTrialPasses, last.msH2D, last.msD2H,
+                          (double)last.streamPieces};
     for (uint64_t i = 0; i < n && i < 13; ++i) out[i] = v[i];
     return 13;
 }

@@ -135,13 +135,7 @@ extern "C" PT_API uint64_t PT_CALL pt_fitting_noalloc(const uint16_t* d_input, c
             throw Error("planes must be in [1, " + std::to_string(kNoAllocMaxPlanes) + "] (got " + std::to_string(planes) + ")");
         if (fit < 1 || fit > 4) throw Error("the Gaussian fit modes 1-4 only (the spline model of mode 5 is uploaded by Fitting)");
         if (height == 0 || width == 0 || height > 0x7FFFFFFFull || width > 0x7FFFFFFFull) throw Error("bad frame size");
-        Options opt;
-        if (Engine* e = Engine::existing()) {
-            std::lock_guard<std::mutex> lk(e->mu);
-            opt = e->opt;
-        } else {
-            opt = Options::defaults();
-        }
+        const Options opt = Engine::shared_options();
         const FitParams fp = gaussian_fit_params(fit, fit_params, opt);
         NoAllocScratch* ws = device_scratch();
         const hipStream_t s = nullptr;   // legacy default stream: no stream is created

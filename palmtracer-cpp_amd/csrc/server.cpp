@@ -231,12 +231,7 @@ PT_API void* PT_CALL pt_server_create(uint64_t slots, uint64_t budgetBytes, uint
 {
     try {
         if (slots < 1 || slots > 64) throw pt::Error("slots must be in [1, 64]");
-        pt::Options base;
-        {
-            pt::Engine& g = pt::Engine::get();
-            std::lock_guard<std::mutex> lk(g.mu);
-            base = g.opt;
-        }
+        const pt::Options base = pt::Engine::shared_options();
         return new Server((int)slots, budgetBytes, chunkPixels, base);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "[palmtracer] pt_server_create: %s\n", e.what());
