@@ -1396,3 +1396,8 @@ double or_astig_quality(const double* model, double z, double sx, double sy, dou
 {
     return ast_quality(model, z, sx, sy, pixelSize);
 }
+
+void or_exp(const double* x, uint64_t n, double* y)
+{
+    for (uint64_t i = 0; i < n; ++i) y[i] = exp(x[i]);
+}

@@ -47,6 +47,8 @@ def lib():
         L.or_astig_estimate.argtypes = [_vp, _vp, _u64, _f64, _vp, _f64]
         L.or_astig_quality.restype = _f64
         L.or_astig_quality.argtypes = [_vp, _f64, _f64, _f64, _f64]
+        L.or_exp.restype = None
+        L.or_exp.argtypes = [_vp, _u64, _vp]
         _or = L
     return _or
 
@@ -185,6 +187,14 @@ def astig_quality(model, z, sx, sy, pixel_size):
     """AstigFit::getQuality restated."""
     m = np.ascontiguousarray(model, np.float64).ravel()
     return lib().or_astig_quality(m.ctypes.data, float(z), float(sx), float(sy), float(pixel_size))
+
+
+def libm_exp(x):
+    """exp of the C library the oracle's fits call, elementwise."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty_like(x)
+    lib().or_exp(x.ctypes.data, x.size, y.ctypes.data)
+    return y
 
 
 def ref_astig_estimate(sigmas, pixel_size, model, z_max):

@@ -6,12 +6,14 @@
 // lib/signal_processing/Matrix.cpp:297-355 (InverseGaussJordan).
 //
 // Schedule: one ROI per lane. A lane walks its ROI's pixels in the reference's raster order, so
-// every sum is accumulated in the reference's order; the only differences from the reference are
-// the device exp/sin/cos. The control flow (lambda schedule, clamps, stopping rules, sentinels) and
+// every sum is accumulated in the reference's order, and exp is bit-equal to the x86-64 glibc exp
+// the reference's Linux build calls (pt_exp.h); the only differences from the reference are the
+// device sin/cos of mode 4's fitted angle. The control flow (lambda schedule, clamps, stopping rules, sentinels) and
 // the damped normal equations (Gauss-Jordan with row equilibration and partial pivoting) are the
 // reference's, step for step.
 #pragma once
 #include "pt_common.h"
+#include "pt_exp.h"
 #include "pt_kernels.h"
 
 namespace pt {
@@ -58,7 +60,7 @@ struct Model {
     {
         const T X2 = x * x, Y2 = y * y;
         if constexpr (MODE <= 2) {
-            const T E = exp(-(X2 + Y2) / (T(2) * sx2));
+            const T E = libm_exp(-(X2 + Y2) / (T(2) * sx2));
             const T P = amp * E;
             if (g) {
                 g[0] = 1;
@@ -71,7 +73,7 @@ struct Model {
         } else {
         const T XC = x * c, XS = x * s, YC = y * c, YS = y * s;
         const T F1 = (XC - YS) / sx, F2 = (XS + YC) / sy;
-        const T E = exp(-((F1 * F1) + (F2 * F2)));
+        const T E = libm_exp(-((F1 * F1) + (F2 * F2)));
         const T P = amp * E;
         if (g) {
             const T XY = x * y, F3 = T(2) * XY * cs, F4 = (T(2) * XY / sx2) - (T(2) * XY / sy2);

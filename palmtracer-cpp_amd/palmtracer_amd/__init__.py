@@ -52,6 +52,8 @@ def lib():
         L.pt_wavelet_device.argtypes = [_vp, _vp, _u64, _u64, _u64, _u64]
         L.pt_debug_labels.restype = _i32
         L.pt_debug_labels.argtypes = [_vp, _u64, _u64, _f64, _f64, _vp]
+        L.pt_debug_exp.restype = _i32
+        L.pt_debug_exp.argtypes = [_vp, _vp, _u64]
         L.AutoThreshold.restype = _f64
         L.AutoThreshold.argtypes = [_vp, _u64, _u64, _vp]
         L.pt_autothreshold_device.restype = _f64
@@ -171,6 +173,13 @@ def debug_labels(frame, threshold, watershed_ratio=0.0):
     if not lib().pt_debug_labels(f.ctypes.data, h, w, float(threshold), float(watershed_ratio), out.ctypes.data):
         raise RuntimeError("pt_debug_labels failed")
     return out
+
+
+def debug_exp(d_x, d_y):
+    """d_y = exp(d_x) on the device with the fit model's exp (float64 torch tensors of one size)."""
+    assert d_x.dtype == d_y.dtype and d_x.numel() == d_y.numel() and d_x.is_contiguous() and d_y.is_contiguous()
+    if not lib().pt_debug_exp(d_x.data_ptr(), d_y.data_ptr(), d_x.numel()):
+        raise RuntimeError("pt_debug_exp failed")
 
 
 def set_option(name, value):
