@@ -147,10 +147,10 @@ PT_API uint64_t PT_CALL pt_localization_device(const uint16_t* d_input, double* 
 PT_API int PT_CALL pt_wavelet_device(const uint16_t* d_input, double* d_output, uint64_t height, uint64_t width,
                                      uint64_t planes, uint64_t levelOutput);
 
-/* The fit model's exp over n device doubles (d_y[i] = exp(d_x[i]), bit-equal to the x86-64 glibc
- * exp of the reference's Linux build); runs on the legacy default stream and returns after it
- * completes. Returns 1 on success. */
-PT_API int PT_CALL pt_debug_exp(const double* d_x, double* d_y, uint64_t n);
+/* The fit model's exp (fn 0) or the sine (1) or cosine (2) of its sincos over n device doubles,
+ * d_y[i] = f(d_x[i]), bit-equal to the x86-64 glibc exp and sincos of the reference's Linux build;
+ * runs on the legacy default stream and returns after it completes. Returns 1 on success. */
+PT_API int PT_CALL pt_debug_libm(uint64_t fn, const double* d_x, double* d_y, uint64_t n);
 
 /* Watershed label map of one host frame: labels[i] = raster index + 1 of the first local maximum
  * of the pixel's label (its region's identity), 0 below threshold. With (int)watershedRatio >= 1

@@ -30,6 +30,7 @@
  * Floating-point: built with -ffp-contract=off so that every product and sum rounds as in
  * the reference's x86-64 build (which has no FMA at its default target).
  */
+#define _GNU_SOURCE   /* sincos */
 #include "oracle.h"
 
 #include <float.h>
@@ -479,7 +480,10 @@ static gmodel prepare_model(int mode, const double* p, double sigma, double thet
         const double th = mode == 3 ? theta : p[6];
         m.sx = p[4]; m.sy = p[5];
         m.sx2 = p[4] * p[4]; m.sy2 = p[5] * p[5];
-        m.c = cos(th); m.s = sin(th);
+        /* The reference's std::cos / std::sin of one angle, which gcc merges into one sincos call
+         * (the compiled reference imports sincos only); glibc's sincos can differ from its sin and
+         * cos in the last bit. */
+        sincos(th, &m.s, &m.c);
         m.c2 = m.c * m.c; m.s2 = m.s * m.s; m.cs = m.c * m.s;
         m.iso = 0; m.hasSigma = 0; m.hasTheta = mode == 4;
     }
@@ -1397,7 +1401,15 @@ double or_astig_quality(const double* model, double z, double sx, double sy, dou
     return ast_quality(model, z, sx, sy, pixelSize);
 }
 
-void or_exp(const double* x, uint64_t n, double* y)
+void or_libm(int fn, const double* x, uint64_t n, double* y)
 {
-    for (uint64_t i = 0; i < n; ++i) y[i] = exp(x[i]);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (fn == 0) {
+            y[i] = exp(x[i]);
+        } else {
+            double s, c;
+            sincos(x[i], &s, &c);
+            y[i] = fn == 1 ? s : c;
+        }
+    }
 }

@@ -65,9 +65,10 @@ void or_astig_estimate(const double* input, double* output, uint64_t size, doubl
 /* AstigFit::getQuality (lib/fit/AstigFit.cpp:99-108) of one observed pair at z. */
 double or_astig_quality(const double* model, double z, double sx, double sy, double pixelSize);
 
-/* The C library's exp over n doubles: std::exp of the reference's Gaussian model
- * (lib/fit/GaussianModel.cpp:19-52), the function every Gaussian fit of this oracle calls. */
-void or_exp(const double* x, uint64_t n, double* y);
+/* The C library's exp (fn 0) or the sine (1) or cosine (2) of its sincos over n doubles: std::exp
+ * and the std::cos / std::sin pair of the reference's Gaussian model (lib/fit/GaussianModel.cpp:
+ * 19-52), which gcc turns into one sincos call, in the reference's build and in this oracle's. */
+void or_libm(int fn, const double* x, uint64_t n, double* y);
 
 #ifdef __cplusplus
 }

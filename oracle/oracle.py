@@ -47,8 +47,8 @@ def lib():
         L.or_astig_estimate.argtypes = [_vp, _vp, _u64, _f64, _vp, _f64]
         L.or_astig_quality.restype = _f64
         L.or_astig_quality.argtypes = [_vp, _f64, _f64, _f64, _f64]
-        L.or_exp.restype = None
-        L.or_exp.argtypes = [_vp, _u64, _vp]
+        L.or_libm.restype = None
+        L.or_libm.argtypes = [_i32, _vp, _u64, _vp]
         _or = L
     return _or
 
@@ -189,11 +189,12 @@ def astig_quality(model, z, sx, sy, pixel_size):
     return lib().or_astig_quality(m.ctypes.data, float(z), float(sx), float(sy), float(pixel_size))
 
 
-def libm_exp(x):
-    """exp of the C library the oracle's fits call, elementwise."""
+def libm(name, x):
+    """exp, sin or cos of the C library the oracle's fits call, elementwise."""
+    fn = {"exp": 0, "sin": 1, "cos": 2}[name]
     x = np.ascontiguousarray(x, np.float64)
     y = np.empty_like(x)
-    lib().or_exp(x.ctypes.data, x.size, y.ctypes.data)
+    lib().or_libm(fn, x.ctypes.data, x.size, y.ctypes.data)
     return y
 
 

@@ -6,14 +6,13 @@
 // lib/signal_processing/Matrix.cpp:297-355 (InverseGaussJordan).
 //
 // Schedule: one ROI per lane. A lane walks its ROI's pixels in the reference's raster order, so
-// every sum is accumulated in the reference's order, and exp is bit-equal to the x86-64 glibc exp
-// the reference's Linux build calls (pt_exp.h); the only differences from the reference are the
-// device sin/cos of mode 4's fitted angle. The control flow (lambda schedule, clamps, stopping rules, sentinels) and
-// the damped normal equations (Gauss-Jordan with row equilibration and partial pivoting) are the
-// reference's, step for step.
+// every sum is accumulated in the reference's order, and exp and sincos are bit-equal to the x86-64
+// glibc functions the reference's Linux build calls (pt_libm.h). The control flow (lambda
+// schedule, clamps, stopping rules, sentinels) and the damped normal equations (Gauss-Jordan with
+// row equilibration and partial pivoting) are the reference's, step for step.
 #pragma once
 #include "pt_common.h"
-#include "pt_exp.h"
+#include "pt_libm.h"
 #include "pt_kernels.h"
 
 namespace pt {
@@ -51,8 +50,10 @@ struct Model {
             sy = (T)p[5];
             sx2 = sx * sx;
             sy2 = sy * sy;
-            c = (T)cos(th);
-            s = (T)sin(th);
+            double sn, cn;
+            libm_sincos(th, sn, cn);
+            c = (T)cn;
+            s = (T)sn;
             c2 = c * c; s2 = s * s; cs = c * s;
         }
     }

@@ -52,8 +52,8 @@ def lib():
         L.pt_wavelet_device.argtypes = [_vp, _vp, _u64, _u64, _u64, _u64]
         L.pt_debug_labels.restype = _i32
         L.pt_debug_labels.argtypes = [_vp, _u64, _u64, _f64, _f64, _vp]
-        L.pt_debug_exp.restype = _i32
-        L.pt_debug_exp.argtypes = [_vp, _vp, _u64]
+        L.pt_debug_libm.restype = _i32
+        L.pt_debug_libm.argtypes = [_u64, _vp, _vp, _u64]
         L.AutoThreshold.restype = _f64
         L.AutoThreshold.argtypes = [_vp, _u64, _u64, _vp]
         L.pt_autothreshold_device.restype = _f64
@@ -175,11 +175,12 @@ def debug_labels(frame, threshold, watershed_ratio=0.0):
     return out
 
 
-def debug_exp(d_x, d_y):
-    """d_y = exp(d_x) on the device with the fit model's exp (float64 torch tensors of one size)."""
+def debug_libm(name, d_x, d_y):
+    """d_y = f(d_x) on the device with the fit model's exp, sin or cos (float64 tensors of one size)."""
+    fn = {"exp": 0, "sin": 1, "cos": 2}[name]
     assert d_x.dtype == d_y.dtype and d_x.numel() == d_y.numel() and d_x.is_contiguous() and d_y.is_contiguous()
-    if not lib().pt_debug_exp(d_x.data_ptr(), d_y.data_ptr(), d_x.numel()):
-        raise RuntimeError("pt_debug_exp failed")
+    if not lib().pt_debug_libm(fn, d_x.data_ptr(), d_y.data_ptr(), d_x.numel()):
+        raise RuntimeError("pt_debug_libm failed")
 
 
 def set_option(name, value):

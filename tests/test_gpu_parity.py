@@ -11,19 +11,11 @@ FIT_PARAMS_SYN = (7.0, 1.3, 1.3, 0.0)
 
 
 def assert_fp64_fit(got, exp, fit):
-    """FP64 fits run the reference's arithmetic in its order with an exp bit-equal to the C
-    library's, so modes 1-3 equal the oracle bit for bit. Mode 4 also evaluates the device sin/cos of
-    its fitted angle, whose last bit can differ from the C library's: every row within the fit
-    tolerance and at least 99 % of rows within 1e-7."""
+    """FP64 fits run the reference's arithmetic in its order with exp, sin and cos bit-equal to the C
+    library's, so every row equals the oracle's bit for bit."""
     assert got.shape == exp.shape
-    if fit in (1, 2, 3):
-        same = np.all((got == exp) | (np.isnan(got) & np.isnan(exp)), axis=1)
-        assert same.all(), (np.flatnonzero(~same)[:5], fit_errors(got, exp))
-        return
-    bad = fit_mismatches(got, exp)
-    assert bad == [], bad[:5]
-    near = np.all(np.isclose(got, exp, atol=1e-7, rtol=1e-7, equal_nan=True), axis=1)
-    assert near.mean() >= 0.99, (near.mean(), fit_errors(got, exp))
+    same = np.all((got == exp) | (np.isnan(got) & np.isnan(exp)), axis=1)
+    assert same.all(), (fit, np.flatnonzero(~same)[:5], fit_errors(got, exp))
 
 
 @pytest.fixture(autouse=True)
@@ -136,6 +128,24 @@ def test_localization_fp64_matches_oracle(pt_gpu, oracle_mod, ref_vectors, name,
     got = pt_gpu.localization(st, thr, 0.0, fit, FIT_PARAMS_SYN)
     exp = oracle_mod.localization(st, thr, 0.0, fit, FIT_PARAMS_SYN)
     assert_fp64_fit(got, exp, fit)
+
+
+@pytest.mark.parametrize("fit", [1, 2, 3, 4])
+@pytest.mark.parametrize("name", ["stack", "dense"])
+def test_localization_fp64_bit_equal_to_compiled_reference(pt_gpu, oracle_mod, ref_inputs, ref_vectors, name, fit):
+    """FP64 rows against the reference's own code (lib/*.cpp compiled in place by oracle/Makefile
+    into oracle/_ref, which travels with the built library): bit for bit, every column."""
+    if oracle_mod.ref_lib() is None:
+        pytest.skip("the compiled reference (oracle/_ref) was not built")
+    if name == "stack":
+        st, thr, fp = ref_inputs["stack"], 103.6, FIT_PARAMS_CSV
+    else:
+        st, thr, fp = ref_vectors[name], float(ref_vectors[f"{name}_autothreshold"][0]), FIT_PARAMS_SYN
+    got = pt_gpu.localization(st, thr, 0.0, fit, fp)
+    exp = oracle_mod.ref_localization(st, thr, 0.0, fit, fp)
+    assert got.shape == exp.shape and got.shape[0] > 100
+    same = np.all((got == exp) | (np.isnan(got) & np.isnan(exp)), axis=1)
+    assert same.all(), (np.flatnonzero(~same)[:5], fit_errors(got, exp))
 
 
 @pytest.mark.parametrize("fit", [0, 1, 2, 3, 4])
@@ -285,20 +295,27 @@ def test_localization_fusion_matches_oracle(pt_gpu, oracle_mod, ref_vectors, nam
     assert_fp64_fit(got, exp, 2)
 
 
-def test_fit_exp_bit_equal_to_libm(pt_gpu, oracle_mod):
-    """The fit model's exp on the device returns the oracle's C-library exp bit for bit: over the
-    fit's argument range, the scaled path (|x| >= 512), the subnormal range and special values."""
+@pytest.mark.parametrize("name", ["exp", "sin", "cos"])
+def test_fit_libm_bit_equal_to_c_library(pt_gpu, oracle_mod, name):
+    """The fit model's exp, sin and cos on the device return the oracle's C-library values bit for
+    bit: over the fit's argument range, every reduction path and special values."""
     import torch
     rng = np.random.default_rng(0x5EED0E)
-    specials = [0.0, -0.0, 1e-300, -1e-300, 2.0 ** -54, -(2.0 ** -55), 512.0, -512.0, 709.782712893384, 709.79,
-                -745.1332191019411, -745.14, -1024.0, 1024.0, np.inf, -np.inf, np.nan]
-    x = np.concatenate([rng.uniform(-40.0, 1.0, 1 << 22), rng.uniform(-760.0, 720.0, 1 << 20),
-                        rng.uniform(-745.2, -700.0, 1 << 18), np.array(specials)])
+    if name == "exp":
+        specials = [0.0, -0.0, 1e-300, -1e-300, 2.0 ** -54, -(2.0 ** -55), 512.0, -512.0, 709.782712893384, 709.79,
+                    -745.1332191019411, -745.14, -1024.0, 1024.0, np.inf, -np.inf, np.nan]
+        parts = [rng.uniform(-40.0, 1.0, 1 << 22), rng.uniform(-760.0, 720.0, 1 << 20), rng.uniform(-745.2, -700.0, 1 << 18)]
+    else:
+        specials = [0.0, -0.0, 2.0 ** -27, -(2.0 ** -26), 0.126, -0.126, 0.85546875, -0.8554687, 2.426265, -2.4262651,
+                    np.pi / 2, np.pi, 105414349.0, -105414349.0, np.inf, -np.inf, np.nan]
+        parts = [rng.uniform(-0.126, 0.126, 1 << 20), rng.uniform(-2.43, 2.43, 1 << 22), rng.uniform(-100.0, 100.0, 1 << 20),
+                 rng.uniform(-1e8, 1e8, 1 << 18)]
+    x = np.concatenate(parts + [np.array(specials)])
     d_x = torch.from_numpy(x).cuda()
     d_y = torch.empty_like(d_x)
-    pt_gpu.debug_exp(d_x, d_y)
+    pt_gpu.debug_libm(name, d_x, d_y)
     got = d_y.cpu().numpy()
-    exp = oracle_mod.libm_exp(x)
+    exp = oracle_mod.libm(name, x)
     same = (got.view(np.uint64) == exp.view(np.uint64)) | (np.isnan(got) & np.isnan(exp))
     assert same.all(), list(zip(x[~same][:5], got[~same][:5], exp[~same][:5]))
 
