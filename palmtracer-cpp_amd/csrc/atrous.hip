@@ -233,8 +233,17 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) at
     __shared__ uint32_t sMask[MCAP];
     __shared__ uint32_t sSeed[SCAP];
     const int lane = threadIdx.x;
-    const int f = blockIdx.y;
-    const int strip = blockIdx.x % strips, seg = blockIdx.x / strips;
+    // Workgroups go to the 8 XCDs in turn (workgroup i to XCD i mod 8), each XCD with its own L2.
+    // Each XCD takes a contiguous range of tiles instead (whole frames, tiles in row-segment and
+    // strip order), so that neighbouring strips and segments, which read the same halo columns and
+    // rows, run side by side on one XCD and share those lines through its L2.
+    const uint32_t nTiles = gridDim.x * gridDim.y;
+    const uint32_t lin = blockIdx.y * gridDim.x + blockIdx.x;
+    const uint32_t xcd = lin & 7u, per = nTiles >> 3, rem = nTiles & 7u;
+    const uint32_t tile = xcd * per + min(xcd, rem) + (lin >> 3);
+    const int f = (int)(tile / gridDim.x);
+    const uint32_t bx = tile % gridDim.x;
+    const int strip = (int)bx % strips, seg = (int)bx / strips;
     const uint32_t npx = (uint32_t)W * (uint32_t)H;     // < 2^30 (checked by the launcher)
     const __amdgpu_buffer_rsrc_t rsI = make_rsrc(in + (size_t)(frameMap ? frameMap[f] : (uint32_t)f) * npx, npx * 2u);
     const __amdgpu_buffer_rsrc_t rsP = make_rsrc(P + (size_t)f * npx, npx * 4u);
