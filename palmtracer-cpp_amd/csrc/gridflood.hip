@@ -159,6 +159,27 @@ struct GfArgs {
 
 __device__ inline uint32_t gf_count(const GfArgs& a) { return min(a.counters[a.countSlot], (uint32_t)a.F); }
 
+// A workgroup's place in the grid: (x, y) of (nx, ny), x striding over a frame's entries and y over
+// the flagged frames. Workgroups go to the 8 XCDs in turn (workgroup i to XCD i mod 8), each XCD
+// with its own L2. With at least 8 flagged frames and a multiple of 8 frame slots, frame slot y is
+// given to the workgroups of XCD y mod 8 only, so that the random gathers into a frame's per-entry
+// arrays hit one L2; with fewer frames the hardware order spreads each frame over every XCD. Both
+// are one-to-one maps of the grid.
+struct GfGrid {
+    uint32_t x, y, nx, ny;
+};
+__device__ inline GfGrid gf_grid(uint32_t cnt)
+{
+    GfGrid g{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y};
+    if (cnt >= 8u && (gridDim.y & 7u) == 0u) {
+        const uint32_t lin = blockIdx.y * gridDim.x + blockIdx.x;   // dispatch order
+        const uint32_t turn = lin >> 3;                              // this workgroup's turn on its XCD
+        g.x = turn % gridDim.x;
+        g.y = (lin & 7u) + 8u * (turn / gridDim.x);
+    }
+    return g;
+}
+
 // Wave-aggregated atomic OR of a per-lane flag into *dst.
 __device__ inline void gf_or(uint32_t* dst, uint32_t bits)
 {
@@ -203,13 +224,14 @@ __global__ void __launch_bounds__(1024) gf_setup_kernel(GfArgs a)
 __global__ void __launch_bounds__(GF_NT) gf_reset_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
-    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+    const GfGrid gg = gf_grid(cnt);
+    for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         if (a.meta[k].fail) continue;
         const uint32_t f = a.flagList[k];
         const uint32_t* ml = a.maskList + (size_t)f * a.npx;
         uint32_t* fn = a.fin + (size_t)f * a.npx;
         const uint32_t mc = a.maskCnt[f];
-        for (uint32_t i = blockIdx.x * GF_NT + threadIdx.x; i < mc; i += gridDim.x * GF_NT) {
+        for (uint32_t i = gg.x * GF_NT + threadIdx.x; i < mc; i += gg.nx * GF_NT) {
             const uint32_t p = ml[i];
             if (!flood_seed(a.seedBits, fn, a.fz, a.aux, a.regionCap, a.npx, f, p)) fn[p] = 0u;
         }
@@ -221,8 +243,9 @@ __global__ void __launch_bounds__(GF_NT) gf_reset_kernel(GfArgs a)
 __global__ void __launch_bounds__(GF_NT) gf_collect_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
+    const GfGrid gg = gf_grid(cnt);
     const int lane = threadIdx.x & 63;
-    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+    for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (m.fail) continue;
         const uint32_t f = a.flagList[k];
@@ -231,8 +254,8 @@ __global__ void __launch_bounds__(GF_NT) gf_collect_kernel(GfArgs a)
         const uint32_t* Pf = a.P + (size_t)f * a.npx;
         const uint32_t* fn = a.fin + (size_t)f * a.npx;
         const uint32_t mc = a.maskCnt[f];
-        const uint32_t stride = gridDim.x * GF_NT;
-        for (uint32_t b = blockIdx.x * GF_NT + (threadIdx.x & ~63u); b < mc; b += stride) {
+        const uint32_t stride = gg.nx * GF_NT;
+        for (uint32_t b = gg.x * GF_NT + (threadIdx.x & ~63u); b < mc; b += stride) {
             const uint32_t i = b + lane;
             bool entry = false;
             uint32_t p = 0;
@@ -257,12 +280,13 @@ __global__ void __launch_bounds__(GF_NT) gf_tile_sort_kernel(GfArgs a)
 {
     __shared__ uint64_t s[GF_TILE];
     const uint32_t cnt = gf_count(a);
-    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+    const GfGrid gg = gf_grid(cnt);
+    for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (m.fail || m.n == 0) continue;
         const GfView v = gf_view(a.pool, m);
         const uint32_t tiles = (m.n + GF_TILE - 1) / GF_TILE;
-        for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        for (uint32_t t = gg.x; t < tiles; t += gg.nx) {
             const uint32_t t0 = t * GF_TILE, len = min((uint32_t)GF_TILE, m.n - t0);
             for (uint32_t i = threadIdx.x; i < GF_TILE; i += GF_NT) s[i] = i < len ? v.skA[t0 + i] : ~0ull;
             __syncthreads();
@@ -300,15 +324,16 @@ __device__ inline uint32_t gf_lower(const uint64_t* src, uint32_t b, uint32_t e,
 __global__ void __launch_bounds__(GF_NT) gf_merge_kernel(GfArgs a, int pass)
 {
     const uint32_t cnt = gf_count(a);
+    const GfGrid gg = gf_grid(cnt);
     const uint64_t w = (uint64_t)GF_TILE << pass;
-    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+    for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (m.fail || m.n <= w) continue;
         const GfView v = gf_view(a.pool, m);
         const uint64_t* src = (pass & 1) ? v.skB : v.skA;
         uint64_t* dst = (pass & 1) ? v.skA : v.skB;
         const uint32_t n = m.n;
-        for (uint32_t i = blockIdx.x * GF_NT + threadIdx.x; i < n; i += gridDim.x * GF_NT) {
+        for (uint32_t i = gg.x * GF_NT + threadIdx.x; i < n; i += gg.nx * GF_NT) {
             const uint64_t key = src[i];
             const uint64_t r = i / w, base = r * w;
             uint64_t out;
@@ -330,7 +355,8 @@ __global__ void __launch_bounds__(GF_NT) gf_merge_kernel(GfArgs a, int pass)
 __global__ void __launch_bounds__(GF_NT) gf_finalize_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
-    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+    const GfGrid gg = gf_grid(cnt);
+    for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (m.fail || m.n == 0) continue;
         const GfView v = gf_view(a.pool, m);
@@ -339,7 +365,7 @@ __global__ void __launch_bounds__(GF_NT) gf_finalize_kernel(GfArgs a)
         const uint32_t n = m.n;
         const uint64_t* S = (gf_passes(n) & 1) ? v.skB : v.skA;
         const uint64_t D = gf_window(n);
-        for (uint32_t i = blockIdx.x * GF_NT + threadIdx.x; i < n; i += gridDim.x * GF_NT) {
+        for (uint32_t i = gg.x * GF_NT + threadIdx.x; i < n; i += gg.nx * GF_NT) {
             const uint64_t key = S[i];
             const uint32_t P = ~(uint32_t)(key >> 32), p = (uint32_t)key;
             const bool seed = flood_seed(a.seedBits, fn, a.fz, a.aux, a.regionCap, a.npx, f, p);
@@ -369,7 +395,8 @@ __global__ void __launch_bounds__(GF_NT) gf_finalize_kernel(GfArgs a)
 __global__ void __launch_bounds__(GF_NT) gf_neighbours_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
-    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+    const GfGrid gg = gf_grid(cnt);
+    for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (m.fail || m.n == 0) continue;
         const GfView v = gf_view(a.pool, m);
@@ -381,7 +408,7 @@ __global__ void __launch_bounds__(GF_NT) gf_neighbours_kernel(GfArgs a)
         // Pixels in mask-list order (neighbouring pixels of one detection strip are adjacent in the
         // list), so that the label-map reads of neighbouring threads share cache lines; each entry's
         // row of the table is one 32-byte write.
-        for (uint32_t q = blockIdx.x * GF_NT + threadIdx.x; q < mc; q += gridDim.x * GF_NT) {
+        for (uint32_t q = gg.x * GF_NT + threadIdx.x; q < mc; q += gg.nx * GF_NT) {
             const uint32_t p = ml[q];
             const uint32_t t = fn[p];
             if (!(t & GF_TAG)) continue;   // not an entry (a seed with no unlabelled neighbour)
@@ -425,12 +452,13 @@ __device__ inline bool gf_active(const GfMeta& m, int it) { return !m.fail && m.
 __global__ void __launch_bounds__(GF_NT) gf_age_kernel(GfArgs a, int it)
 {
     const uint32_t cnt = gf_count(a);
-    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+    const GfGrid gg = gf_grid(cnt);
+    for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (!gf_active(m, it)) continue;
         const GfView v = gf_view(a.pool, m);
         const uint32_t* G = (it & 1) ? v.Gb : v.Ga;
-        for (uint32_t i = blockIdx.x * GF_NT + threadIdx.x; i < m.n; i += gridDim.x * GF_NT) {
+        for (uint32_t i = gg.x * GF_NT + threadIdx.x; i < m.n; i += gg.nx * GF_NT) {
             uint32_t t = 0;
             if (!(v.spix[i] & GF_SEED)) {
                 uint32_t e[8], g[8];
@@ -450,14 +478,15 @@ __global__ void __launch_bounds__(GF_NT) gf_age_kernel(GfArgs a, int it)
 __global__ void __launch_bounds__(GF_NT) gf_rank_kernel(GfArgs a, int it)
 {
     const uint32_t cnt = gf_count(a);
-    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+    const GfGrid gg = gf_grid(cnt);
+    for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (!gf_active(m, it)) continue;
         const GfView v = gf_view(a.pool, m);
         const uint32_t* G = (it & 1) ? v.Gb : v.Ga;
         uint32_t* Gn = (it & 1) ? v.Ga : v.Gb;
-        const uint32_t stride = gridDim.x * GF_NT;
-        for (uint32_t b = blockIdx.x * GF_NT + (threadIdx.x & ~63u); b < m.n; b += stride) {
+        const uint32_t stride = gg.nx * GF_NT;
+        for (uint32_t b = gg.x * GF_NT + (threadIdx.x & ~63u); b < m.n; b += stride) {
             const uint32_t i = b + (threadIdx.x & 63);
             bool changed = false;
             if (i < m.n) {
@@ -484,17 +513,18 @@ __device__ inline uint32_t gf_gfirst(const GfView& v, uint32_t i) { return (uint
 __global__ void __launch_bounds__(GF_NT) gf_check1_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
-    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+    const GfGrid gg = gf_grid(cnt);
+    for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (m.fail || m.n == 0) continue;
         if (m.changed[kGridMaxIter - 1]) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.meta[k].fail, kGfUnsettled);
+            if (gg.x == 0 && threadIdx.x == 0) atomicOr(&a.meta[k].fail, kGfUnsettled);
             continue;
         }
         const GfView v = gf_view(a.pool, m);
         const uint32_t* G = v.Ga;     // settled: both rank buffers hold the solution
-        const uint32_t stride = gridDim.x * GF_NT;
-        for (uint32_t b = blockIdx.x * GF_NT + (threadIdx.x & ~63u); b < m.n; b += stride) {
+        const uint32_t stride = gg.nx * GF_NT;
+        for (uint32_t b = gg.x * GF_NT + (threadIdx.x & ~63u); b < m.n; b += stride) {
             const uint32_t i = b + (threadIdx.x & 63);
             uint32_t bad = 0, tie = 0;
             if (i < m.n) {
@@ -536,12 +566,13 @@ __global__ void __launch_bounds__(GF_NT) gf_check1_kernel(GfArgs a)
 __global__ void __launch_bounds__(GF_NT) gf_check2_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
-    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+    const GfGrid gg = gf_grid(cnt);
+    for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (m.fail || m.n == 0) continue;
         const GfView v = gf_view(a.pool, m);
-        const uint32_t stride = gridDim.x * GF_NT;
-        for (uint32_t b = blockIdx.x * GF_NT + (threadIdx.x & ~63u); b < m.n; b += stride) {
+        const uint32_t stride = gg.nx * GF_NT;
+        for (uint32_t b = gg.x * GF_NT + (threadIdx.x & ~63u); b < m.n; b += stride) {
             const uint32_t i = b + (threadIdx.x & 63);
             uint32_t bad = 0;
             if (i < m.n) {
@@ -569,11 +600,12 @@ __device__ inline double* gf_khi(const GfView& v) { return reinterpret_cast<doub
 __global__ void __launch_bounds__(GF_NT) gf_ival_init_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
-    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+    const GfGrid gg = gf_grid(cnt);
+    for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (m.fail || m.n == 0 || !m.tie) continue;
         const GfView v = gf_view(a.pool, m);
-        for (uint32_t i = blockIdx.x * GF_NT + threadIdx.x; i < m.n; i += gridDim.x * GF_NT) {
+        for (uint32_t i = gg.x * GF_NT + threadIdx.x; i < m.n; i += gg.nx * GF_NT) {
             const uint32_t first = gf_gfirst(v, i), size = gf_gsize(v, i);
             v.Gb[i] = first;
             v.T[i] = first + size - 1;
@@ -590,12 +622,13 @@ __device__ inline bool gf_ival_active(const GfMeta& m, int it)
 __global__ void __launch_bounds__(GF_NT) gf_ival_key_kernel(GfArgs a, int it)
 {
     const uint32_t cnt = gf_count(a);
-    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+    const GfGrid gg = gf_grid(cnt);
+    for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (!gf_ival_active(m, it)) continue;
         const GfView v = gf_view(a.pool, m);
         double* khi = gf_khi(v);
-        for (uint32_t i = blockIdx.x * GF_NT + threadIdx.x; i < m.n; i += gridDim.x * GF_NT) {
+        for (uint32_t i = gg.x * GF_NT + threadIdx.x; i < m.n; i += gg.nx * GF_NT) {
             uint32_t tlo = 0, thi = 0;
             if (!(v.spix[i] & GF_SEED)) {
                 uint32_t e[8], glo[8], ghi[8];
@@ -619,13 +652,14 @@ __global__ void __launch_bounds__(GF_NT) gf_ival_key_kernel(GfArgs a, int it)
 __global__ void __launch_bounds__(GF_NT) gf_ival_rank_kernel(GfArgs a, int it)
 {
     const uint32_t cnt = gf_count(a);
-    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+    const GfGrid gg = gf_grid(cnt);
+    for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (!gf_ival_active(m, it)) continue;
         const GfView v = gf_view(a.pool, m);
         const double* khi = gf_khi(v);
-        const uint32_t stride = gridDim.x * GF_NT;
-        for (uint32_t b = blockIdx.x * GF_NT + (threadIdx.x & ~63u); b < m.n; b += stride) {
+        const uint32_t stride = gg.nx * GF_NT;
+        for (uint32_t b = gg.x * GF_NT + (threadIdx.x & ~63u); b < m.n; b += stride) {
             const uint32_t i = b + (threadIdx.x & 63);
             bool changed = false;
             if (i < m.n) {
@@ -652,16 +686,17 @@ __global__ void __launch_bounds__(GF_NT) gf_ival_rank_kernel(GfArgs a, int it)
 __global__ void __launch_bounds__(GF_NT) gf_ival_check_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
-    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+    const GfGrid gg = gf_grid(cnt);
+    for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (m.fail || m.n == 0 || !m.tie) continue;
         if (m.ichanged[kGridIvalIter - 1]) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.meta[k].fail, kGfTie);
+            if (gg.x == 0 && threadIdx.x == 0) atomicOr(&a.meta[k].fail, kGfTie);
             continue;
         }
         const GfView v = gf_view(a.pool, m);
-        const uint32_t stride = gridDim.x * GF_NT;
-        for (uint32_t b = blockIdx.x * GF_NT + (threadIdx.x & ~63u); b < m.n; b += stride) {
+        const uint32_t stride = gg.nx * GF_NT;
+        for (uint32_t b = gg.x * GF_NT + (threadIdx.x & ~63u); b < m.n; b += stride) {
             const uint32_t i = b + (threadIdx.x & 63);
             uint32_t bad = 0;
             if (i < m.n && !(v.spix[i] & GF_SEED)) {
@@ -689,10 +724,11 @@ __global__ void __launch_bounds__(GF_NT) gf_ival_check_kernel(GfArgs a)
 __global__ void __launch_bounds__(GF_NT) gf_write_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
-    for (uint32_t k = blockIdx.y; k < cnt; k += gridDim.y) {
+    const GfGrid gg = gf_grid(cnt);
+    for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         const uint32_t f = a.flagList[k];
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (gg.x == 0 && threadIdx.x == 0) {
             if (m.fail == kGfPool && a.round == 0) {
                 a.retryList[atomicAdd(&a.counters[kGridRetryCnt], 1u)] = f;
             } else if (m.fail) {
@@ -706,7 +742,7 @@ __global__ void __launch_bounds__(GF_NT) gf_write_kernel(GfArgs a)
         if ((m.fail & kGfPool) || m.n == 0) continue;
         const GfView v = gf_view(a.pool, m);
         uint32_t* fn = a.fin + (size_t)f * a.npx;
-        for (uint32_t i = blockIdx.x * GF_NT + threadIdx.x; i < m.n; i += gridDim.x * GF_NT) {
+        for (uint32_t i = gg.x * GF_NT + threadIdx.x; i < m.n; i += gg.nx * GF_NT) {
             const uint32_t sx = v.spix[i];
             fn[sx & ~GF_SEED] = (m.fail && !(sx & GF_SEED)) ? 0u : v.lbl[i];
         }
