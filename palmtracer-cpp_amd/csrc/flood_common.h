@@ -58,4 +58,38 @@ __device__ inline bool flood_seed(const uint32_t* seedBits, const uint32_t* fn, 
            0xFFFFFFFFu - (uint32_t)fz[(size_t)f * regionCap + sl].maxKey == p;
 }
 
+// Workgroups go to the 8 XCDs in turn (workgroup i to XCD i mod 8), each XCD with its own L2. The
+// two maps below keep a frame's workgroups on one XCD, so that their scattered accesses into the
+// frame's arrays share one L2; both are one-to-one maps of the grid.
+//
+// Grid-stride kernels ((x, y) of (nx, ny): x strides over a frame's items, y over frames): with at
+// least 8 frames and a multiple of 8 frame slots, frame slot y goes to the workgroups of XCD y mod 8;
+// with fewer frames the hardware order spreads each frame over every XCD.
+struct FrameGrid {
+    uint32_t x, y, nx, ny;
+};
+__device__ inline FrameGrid frame_grid(uint32_t frames)
+{
+    FrameGrid g{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y};
+    if (frames >= 8u && (gridDim.y & 7u) == 0u) {
+        const uint32_t lin = blockIdx.y * gridDim.x + blockIdx.x;   // dispatch order
+        const uint32_t turn = lin >> 3;                              // this workgroup's turn on its XCD
+        g.x = turn % gridDim.x;
+        g.y = (lin & 7u) + 8u * (turn / gridDim.x);
+    }
+    return g;
+}
+
+// Grids of one workgroup per (x, frame) with no striding: each XCD takes a contiguous range of the
+// (frame, x) pairs in frame order.
+__device__ inline void frame_tile(uint32_t& x, uint32_t& frame)
+{
+    const uint32_t n = gridDim.x * gridDim.y;
+    const uint32_t lin = blockIdx.y * gridDim.x + blockIdx.x;
+    const uint32_t xcd = lin & 7u, per = n >> 3, rem = n & 7u;
+    const uint32_t t = xcd * per + min(xcd, rem) + (lin >> 3);
+    frame = t / gridDim.x;
+    x = t % gridDim.x;
+}
+
 } // namespace pt

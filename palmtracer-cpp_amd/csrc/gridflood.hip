@@ -159,27 +159,6 @@ struct GfArgs {
 
 __device__ inline uint32_t gf_count(const GfArgs& a) { return min(a.counters[a.countSlot], (uint32_t)a.F); }
 
-// A workgroup's place in the grid: (x, y) of (nx, ny), x striding over a frame's entries and y over
-// the flagged frames. Workgroups go to the 8 XCDs in turn (workgroup i to XCD i mod 8), each XCD
-// with its own L2. With at least 8 flagged frames and a multiple of 8 frame slots, frame slot y is
-// given to the workgroups of XCD y mod 8 only, so that the random gathers into a frame's per-entry
-// arrays hit one L2; with fewer frames the hardware order spreads each frame over every XCD. Both
-// are one-to-one maps of the grid.
-struct GfGrid {
-    uint32_t x, y, nx, ny;
-};
-__device__ inline GfGrid gf_grid(uint32_t cnt)
-{
-    GfGrid g{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y};
-    if (cnt >= 8u && (gridDim.y & 7u) == 0u) {
-        const uint32_t lin = blockIdx.y * gridDim.x + blockIdx.x;   // dispatch order
-        const uint32_t turn = lin >> 3;                              // this workgroup's turn on its XCD
-        g.x = turn % gridDim.x;
-        g.y = (lin & 7u) + 8u * (turn / gridDim.x);
-    }
-    return g;
-}
-
 // Wave-aggregated atomic OR of a per-lane flag into *dst.
 __device__ inline void gf_or(uint32_t* dst, uint32_t bits)
 {
@@ -224,7 +203,7 @@ __global__ void __launch_bounds__(1024) gf_setup_kernel(GfArgs a)
 __global__ void __launch_bounds__(GF_NT) gf_reset_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
-    const GfGrid gg = gf_grid(cnt);
+    const FrameGrid gg = frame_grid(cnt);
     for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         if (a.meta[k].fail) continue;
         const uint32_t f = a.flagList[k];
@@ -243,7 +222,7 @@ __global__ void __launch_bounds__(GF_NT) gf_reset_kernel(GfArgs a)
 __global__ void __launch_bounds__(GF_NT) gf_collect_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
-    const GfGrid gg = gf_grid(cnt);
+    const FrameGrid gg = frame_grid(cnt);
     const int lane = threadIdx.x & 63;
     for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
@@ -280,7 +259,7 @@ __global__ void __launch_bounds__(GF_NT) gf_tile_sort_kernel(GfArgs a)
 {
     __shared__ uint64_t s[GF_TILE];
     const uint32_t cnt = gf_count(a);
-    const GfGrid gg = gf_grid(cnt);
+    const FrameGrid gg = frame_grid(cnt);
     for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (m.fail || m.n == 0) continue;
@@ -324,7 +303,7 @@ __device__ inline uint32_t gf_lower(const uint64_t* src, uint32_t b, uint32_t e,
 __global__ void __launch_bounds__(GF_NT) gf_merge_kernel(GfArgs a, int pass)
 {
     const uint32_t cnt = gf_count(a);
-    const GfGrid gg = gf_grid(cnt);
+    const FrameGrid gg = frame_grid(cnt);
     const uint64_t w = (uint64_t)GF_TILE << pass;
     for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
@@ -355,7 +334,7 @@ __global__ void __launch_bounds__(GF_NT) gf_merge_kernel(GfArgs a, int pass)
 __global__ void __launch_bounds__(GF_NT) gf_finalize_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
-    const GfGrid gg = gf_grid(cnt);
+    const FrameGrid gg = frame_grid(cnt);
     for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (m.fail || m.n == 0) continue;
@@ -395,7 +374,7 @@ __global__ void __launch_bounds__(GF_NT) gf_finalize_kernel(GfArgs a)
 __global__ void __launch_bounds__(GF_NT) gf_neighbours_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
-    const GfGrid gg = gf_grid(cnt);
+    const FrameGrid gg = frame_grid(cnt);
     for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (m.fail || m.n == 0) continue;
@@ -452,7 +431,7 @@ __device__ inline bool gf_active(const GfMeta& m, int it) { return !m.fail && m.
 __global__ void __launch_bounds__(GF_NT) gf_age_kernel(GfArgs a, int it)
 {
     const uint32_t cnt = gf_count(a);
-    const GfGrid gg = gf_grid(cnt);
+    const FrameGrid gg = frame_grid(cnt);
     for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (!gf_active(m, it)) continue;
@@ -478,7 +457,7 @@ __global__ void __launch_bounds__(GF_NT) gf_age_kernel(GfArgs a, int it)
 __global__ void __launch_bounds__(GF_NT) gf_rank_kernel(GfArgs a, int it)
 {
     const uint32_t cnt = gf_count(a);
-    const GfGrid gg = gf_grid(cnt);
+    const FrameGrid gg = frame_grid(cnt);
     for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (!gf_active(m, it)) continue;
@@ -513,7 +492,7 @@ __device__ inline uint32_t gf_gfirst(const GfView& v, uint32_t i) { return (uint
 __global__ void __launch_bounds__(GF_NT) gf_check1_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
-    const GfGrid gg = gf_grid(cnt);
+    const FrameGrid gg = frame_grid(cnt);
     for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (m.fail || m.n == 0) continue;
@@ -566,7 +545,7 @@ __global__ void __launch_bounds__(GF_NT) gf_check1_kernel(GfArgs a)
 __global__ void __launch_bounds__(GF_NT) gf_check2_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
-    const GfGrid gg = gf_grid(cnt);
+    const FrameGrid gg = frame_grid(cnt);
     for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (m.fail || m.n == 0) continue;
@@ -600,7 +579,7 @@ __device__ inline double* gf_khi(const GfView& v) { return reinterpret_cast<doub
 __global__ void __launch_bounds__(GF_NT) gf_ival_init_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
-    const GfGrid gg = gf_grid(cnt);
+    const FrameGrid gg = frame_grid(cnt);
     for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (m.fail || m.n == 0 || !m.tie) continue;
@@ -622,7 +601,7 @@ __device__ inline bool gf_ival_active(const GfMeta& m, int it)
 __global__ void __launch_bounds__(GF_NT) gf_ival_key_kernel(GfArgs a, int it)
 {
     const uint32_t cnt = gf_count(a);
-    const GfGrid gg = gf_grid(cnt);
+    const FrameGrid gg = frame_grid(cnt);
     for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (!gf_ival_active(m, it)) continue;
@@ -652,7 +631,7 @@ __global__ void __launch_bounds__(GF_NT) gf_ival_key_kernel(GfArgs a, int it)
 __global__ void __launch_bounds__(GF_NT) gf_ival_rank_kernel(GfArgs a, int it)
 {
     const uint32_t cnt = gf_count(a);
-    const GfGrid gg = gf_grid(cnt);
+    const FrameGrid gg = frame_grid(cnt);
     for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (!gf_ival_active(m, it)) continue;
@@ -686,7 +665,7 @@ __global__ void __launch_bounds__(GF_NT) gf_ival_rank_kernel(GfArgs a, int it)
 __global__ void __launch_bounds__(GF_NT) gf_ival_check_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
-    const GfGrid gg = gf_grid(cnt);
+    const FrameGrid gg = frame_grid(cnt);
     for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         if (m.fail || m.n == 0 || !m.tie) continue;
@@ -724,7 +703,7 @@ __global__ void __launch_bounds__(GF_NT) gf_ival_check_kernel(GfArgs a)
 __global__ void __launch_bounds__(GF_NT) gf_write_kernel(GfArgs a)
 {
     const uint32_t cnt = gf_count(a);
-    const GfGrid gg = gf_grid(cnt);
+    const FrameGrid gg = frame_grid(cnt);
     for (uint32_t k = gg.y; k < cnt; k += gg.ny) {
         const GfMeta m = a.meta[k];
         const uint32_t f = a.flagList[k];

@@ -634,24 +634,26 @@ __device__ inline bool flg_global(const FlgArgs& a, uint32_t f) { return !(a.lds
 
 __global__ void __launch_bounds__(NT) flg_parents_kernel(FlgArgs a)
 {
-    for (uint32_t f = blockIdx.y; f < (uint32_t)a.F; f += gridDim.y) {
+    const FrameGrid gg = frame_grid((uint32_t)a.F);
+    for (uint32_t f = gg.y; f < (uint32_t)a.F; f += gg.ny) {
         if (!flg_global(a, f)) continue;
         const uint32_t n = a.maskCnt[f];
         const uint32_t* ml = a.maskList + (size_t)f * a.npx;
         uint32_t* lb = a.lab + (size_t)f * a.npx;
-        for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < n; k += gridDim.x * NT) lb[ml[k]] = ml[k];
+        for (uint32_t k = gg.x * NT + threadIdx.x; k < n; k += gg.nx * NT) lb[ml[k]] = ml[k];
     }
 }
 
 __global__ void __launch_bounds__(NT) flg_union_kernel(FlgArgs a)
 {
-    for (uint32_t f = blockIdx.y; f < (uint32_t)a.F; f += gridDim.y) {
+    const FrameGrid gg = frame_grid((uint32_t)a.F);
+    for (uint32_t f = gg.y; f < (uint32_t)a.F; f += gg.ny) {
         if (!flg_global(a, f)) continue;
         const uint32_t n = a.maskCnt[f];
         const uint32_t* ml = a.maskList + (size_t)f * a.npx;
         const uint32_t* Pf = a.P + (size_t)f * a.npx;
         uint32_t* lb = a.lab + (size_t)f * a.npx;
-        for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < n; k += gridDim.x * NT) {
+        for (uint32_t k = gg.x * NT + threadIdx.x; k < n; k += gg.nx * NT) {
             const uint32_t p = ml[k];
             const int y = (int)(p / (uint32_t)a.W), x = (int)p - y * a.W;
             const int dys[4] = {0, -1, -1, -1}, dxs[4] = {-1, -1, 0, 1};
@@ -677,13 +679,14 @@ __global__ void __launch_bounds__(NT) flg_union_kernel(FlgArgs a)
 
 __global__ void __launch_bounds__(NT) flg_flatten_kernel(FlgArgs a)
 {
-    for (uint32_t f = blockIdx.y; f < (uint32_t)a.F; f += gridDim.y) {
+    const FrameGrid gg = frame_grid((uint32_t)a.F);
+    for (uint32_t f = gg.y; f < (uint32_t)a.F; f += gg.ny) {
         if (!flg_global(a, f)) continue;
         const uint32_t n = a.maskCnt[f];
         const uint32_t* ml = a.maskList + (size_t)f * a.npx;
         uint32_t* lb = a.lab + (size_t)f * a.npx;
         PixAux* ax = a.aux + (size_t)f * a.npx;
-        for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < n; k += gridDim.x * NT) {
+        for (uint32_t k = gg.x * NT + threadIdx.x; k < n; k += gg.nx * NT) {
             const uint32_t p = ml[k];
             const uint32_t c = uf_find(lb, p);
             if (c == p) ax[p].cmin = 0u;
@@ -694,7 +697,8 @@ __global__ void __launch_bounds__(NT) flg_flatten_kernel(FlgArgs a)
 
 __global__ void __launch_bounds__(NT) flg_maxima_kernel(FlgArgs a)
 {
-    for (uint32_t f = blockIdx.y; f < (uint32_t)a.F; f += gridDim.y) {
+    const FrameGrid gg = frame_grid((uint32_t)a.F);
+    for (uint32_t f = gg.y; f < (uint32_t)a.F; f += gg.ny) {
         if (!flg_global(a, f)) continue;
         const uint32_t ns = a.seedCnt[f];
         const uint32_t* sl = a.seedList + (size_t)f * a.npx;
@@ -702,7 +706,7 @@ __global__ void __launch_bounds__(NT) flg_maxima_kernel(FlgArgs a)
         uint32_t* lb = a.lab + (size_t)f * a.npx;
         uint32_t* fn = a.fin + (size_t)f * a.npx;
         PixAux* ax = a.aux + (size_t)f * a.npx;
-        for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < ns; k += gridDim.x * NT) {
+        for (uint32_t k = gg.x * NT + threadIdx.x; k < ns; k += gg.nx * NT) {
             const uint32_t s = sl[k];
             const uint32_t c = ld_relaxed(&lb[s]);
             const uint32_t L = chain_label(sbf, s, a.W);
@@ -721,7 +725,8 @@ __global__ void __launch_bounds__(NT) flg_maxima_kernel(FlgArgs a)
 
 __global__ void __launch_bounds__(NT) flg_labels_kernel(FlgArgs a)
 {
-    for (uint32_t f = blockIdx.y; f < (uint32_t)a.F; f += gridDim.y) {
+    const FrameGrid gg = frame_grid((uint32_t)a.F);
+    for (uint32_t f = gg.y; f < (uint32_t)a.F; f += gg.ny) {
         if (!flg_global(a, f)) continue;
         const uint32_t n = a.maskCnt[f];
         const uint32_t* ml = a.maskList + (size_t)f * a.npx;
@@ -729,7 +734,7 @@ __global__ void __launch_bounds__(NT) flg_labels_kernel(FlgArgs a)
         uint32_t* lb = a.lab + (size_t)f * a.npx;
         uint32_t* fn = a.fin + (size_t)f * a.npx;
         const PixAux* ax = a.aux + (size_t)f * a.npx;
-        for (uint32_t k = blockIdx.x * NT + threadIdx.x; k < n; k += gridDim.x * NT) {
+        for (uint32_t k = gg.x * NT + threadIdx.x; k < n; k += gg.nx * NT) {
             const uint32_t p = ml[k];
             const uint32_t c = ld_relaxed(&lb[p]);
             const bool first = c == p;
@@ -1864,13 +1869,15 @@ __global__ void __launch_bounds__(RS_NT) region_stats_kernel(const uint32_t* __r
 {
     __shared__ unsigned long long sP[RS_LDS_REGIONS], sPx[RS_LDS_REGIONS], sPy[RS_LDS_REGIONS];
     __shared__ uint32_t sSurf[RS_LDS_REGIONS];
-    const int f = blockIdx.y;
+    uint32_t bx, fu;
+    frame_tile(bx, fu);   // each XCD a contiguous range of frames
+    const int f = (int)fu;
     if (skip && skip[f]) return;   // deferred frame: its labels are resolved in a later pass
     const uint32_t cnt = maskCnt[f];
     const uint32_t nReg = (uint32_t)min((size_t)rootCnt[f], regionCap);
     const bool inLds = nReg <= RS_LDS_REGIONS;
     // LDS accumulation: one workgroup per frame; global atomics: all of the frame's workgroups.
-    if (inLds && blockIdx.x != 0) return;
+    if (inLds && bx != 0) return;
     Region* rg = regions + (size_t)f * regionCap;
     if (!inLds) {
         // Global atomics: each thread sums a run of RS_RUN consecutive list entries (neighbouring
@@ -1886,7 +1893,7 @@ __global__ void __launch_bounds__(RS_NT) region_stats_kernel(const uint32_t* __r
             atomicAdd((unsigned long long*)&r->sPy, (unsigned long long)aY);
         };
         const uint32_t* ml = maskList + (size_t)f * npx;
-        for (uint32_t k0 = (blockIdx.x * RS_NT + threadIdx.x) * RS_RUN; k0 < cnt; k0 += gridDim.x * RS_NT * RS_RUN) {
+        for (uint32_t k0 = (bx * RS_NT + threadIdx.x) * RS_RUN; k0 < cnt; k0 += gridDim.x * RS_NT * RS_RUN) {
             const uint32_t k1 = min(k0 + RS_RUN, cnt);
             for (uint32_t k = k0; k < k1; ++k) {
                 const uint32_t p = ml[k];
