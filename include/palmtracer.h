@@ -184,8 +184,9 @@ PT_API int PT_CALL pt_debug_labels(const uint16_t* image, uint64_t height, uint6
  * (concurrent calls run on separate engines). Returns 1 on success. */
 PT_API int PT_CALL pt_set_option(const char* name, double value);
 
-/* Counters of the call that finished last (of concurrent calls, the last to return): frames, chunks, rows, frames flagged for the exact frame flood,
- * multi-label components, frames deferred to the final pass, status, component floods given up on a
+/* Counters of the Localization, Fitting, AutoThreshold, label-map or batch call that finished last
+ * (of concurrent calls, the last to return; Fitting fills frames and rows only): frames, chunks,
+ * rows, frames flagged for the exact frame flood, multi-label components, frames deferred to the final pass, status, component floods given up on a
  * full queue, component floods given up on an unproven order, pops of the wave-queue frame floods,
  * their largest queue, frames the rank flood handed over (too many entries, equal keys, not
  * settled), wave-queue overflows, frames of the heap restatement, frames the rank flood settled by

@@ -163,7 +163,7 @@ extern "C" PT_API int PT_CALL pt_crlb_device(const double* d_rows, uint64_t nRow
         if (fp.roi < 3 || fp.roi > 31) throw pt::Error("CRLB: roi side must be in [3, 31]");
         const pt::Camera cam{camera ? camera[0] : 1.0, camera ? camera[1] : 0.0, camera ? camera[2] : 0.0};
         if (!(cam.gain > 0.0)) throw pt::Error("CRLB: camera gain must be positive");
-        pt::Lease lease;
+        pt::Lease lease(false);   // no counters
         pt::Engine& e = *lease;
         const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(16384, (nRows + 63) / 64));
         switch (fit) {

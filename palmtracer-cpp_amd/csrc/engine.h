@@ -238,12 +238,12 @@ private:
 // The engine of one call of a C entry point. Concurrent calls run independently: a call takes the
 // process-wide engine when it is free, else a free pooled engine (own streams and buffers, created
 // on first need, at most kPoolEngines of them, kept until the process ends), else it waits for the
-// first of them to come free. The leased engine starts from the shared options; its counters are
-// published when the lease ends.
+// first of them to come free. The leased engine starts from the shared options; the counters of a
+// call that computes them (stats = true) are published when the lease ends.
 class Lease {
 public:
     static constexpr int kPoolEngines = 4;
-    Lease();
+    explicit Lease(bool stats = true);
     ~Lease();
     Lease(const Lease&) = delete;
     Lease& operator=(const Lease&) = delete;
@@ -253,6 +253,7 @@ public:
 private:
     Engine* e_ = nullptr;
     int slot_ = -1;   // pool slot, or -1 for the process-wide engine (its mutex is held)
+    bool stats_ = true;
 };
 
 } // namespace pt

@@ -211,7 +211,7 @@ RunStats Engine::published()
     return sh.stat;
 }
 
-Lease::Lease()
+Lease::Lease(bool stats) : stats_(stats)
 {
     Engine& g = Engine::get();
     Shared& sh = shared();
@@ -255,7 +255,7 @@ Lease::Lease()
 
 Lease::~Lease()
 {
-    Engine::publish(e_->last);
+    if (stats_) Engine::publish(e_->last);
     Shared& sh = shared();
     if (slot_ < 0) {
         e_->mu.unlock();
@@ -969,6 +969,7 @@ uint64_t Engine::localize(const uint16_t* d_in, uint64_t planes, int H, int W, d
 uint64_t Engine::fit_positions(const uint16_t* d_in, uint64_t planes, int H, int W, const double* hostPos, uint64_t n,
                                uint64_t fit, const double* fitParams, double* d_out)
 {
+    last = RunStats{};
     if (n == 0) return 0;
     if (fit < 1 || fit > 5) throw Error("Fitting supports the Gaussian modes 1-4 and the spline mode 5");
     if (n > 0xFFFFFFFFull) throw Error("too many positions");
@@ -1001,6 +1002,8 @@ uint64_t Engine::fit_positions(const uint16_t* d_in, uint64_t planes, int H, int
     dpos.release();
     dloc.release();
     dbad.release();
+    last.frames = planes;
+    last.rois = n;
     return n * kCols;
 }
 

@@ -75,7 +75,7 @@ PT_API void PT_CALL Wavelett(const uint16_t* input, double* output, uint64_t hei
         if (planes == 0 || height == 0 || width == 0) return 0;
         if (levelOutput == 0) levelOutput = 1;                   // CPU/main.cpp:262
         const int level = (int)std::min<uint64_t>(levelOutput - 1, 2);
-        pt::Lease lease;
+        pt::Lease lease(false);   // no counters
         Engine& e = *lease;
         const size_t npx = (size_t)height * width;
         // Bounded device footprint: planes in chunks.
@@ -108,7 +108,7 @@ PT_API void PT_CALL Astigmatism3DEstimation(const double* input, double* output,
 {
     guarded("Astigmatism3DEstimation", [&]() -> int {
         if (size == 0) return 0;
-        pt::Lease lease;
+        pt::Lease lease(false);   // no counters
         Engine& e = *lease;
         e.in.ensure(size * 16);
         e.out.ensure(size * 16);
@@ -124,7 +124,7 @@ PT_API int PT_CALL pt_astig_estimate_device(const double* d_input, double* d_out
                                             const double* model, double zMax)
 {
     return guarded("pt_astig_estimate_device", [&]() -> int {
-        pt::Lease lease;
+        pt::Lease lease(false);   // no counters
         Engine& e = *lease;
         pt::launch_astig_estimate(d_input, d_output, size, pixelSize, model, zMax, e.stream);
         PT_CHECK(hipStreamSynchronize(e.stream));
@@ -161,7 +161,7 @@ PT_API int PT_CALL pt_wavelet_device(const uint16_t* d_input, double* d_output, 
                                      uint64_t planes, uint64_t levelOutput)
 {
     return guarded("pt_wavelet_device", [&]() -> int {
-        pt::Lease lease;
+        pt::Lease lease(false);   // no counters
         Engine& e = *lease;
         if (levelOutput == 0) levelOutput = 1;
         e.wavelet(d_input, planes, (int)height, (int)width, (int)std::min<uint64_t>(levelOutput - 1, 2), d_output);

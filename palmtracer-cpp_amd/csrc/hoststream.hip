@@ -265,6 +265,9 @@ uint64_t Engine::fit_positions_host(const uint16_t* input, uint64_t planes, int 
             dst[kPlane] = positions[idx[p][k] * kInCols + kInPlane];   // the caller's plane
         }
     }
+    last = RunStats{};
+    last.frames = planes;
+    last.rois = n;
     last.msH2D = msH2D;
     last.msD2H = msD2H;
     last.streamPieces = pieces;

@@ -106,3 +106,20 @@ def test_options_reach_every_engine(pt_gpu, stacks):
         pt_gpu.set_option("max_iter", 75)   # lib/fit/Fitting.hpp:80
     full = pt_gpu.localization(st, thr, 0.0, 2, FIT_PARAMS)
     assert not np.array_equal(full, one, equal_nan=True)
+
+
+def test_counters_follow_the_last_counting_call(pt_gpu, stacks):
+    """pt_last_stats reports the last call that computes counters, whichever engine ran it: a
+    Localization's rows, then a Fitting's frames and positions; Wavelett and the astigmatism
+    estimate leave them as they were."""
+    st, _, thr, pos = stacks[1]
+    rows = pt_gpu.localization(st, thr, 0.0, 2, FIT_PARAMS)
+    s = pt_gpu.last_stats()
+    assert s["rows"] == len(rows) and s["frames"] == st.shape[0]
+    pt_gpu.wavelett(st, 2)
+    pt_gpu.astig_estimate(np.full((4, 2), 1.3), 100.0, [0.0, 1.0, 0.0, 0.0, 1.0] * 2, 600.0)
+    assert pt_gpu.last_stats() == s
+    sub = pos[: len(pos) // 2]
+    pt_gpu.fitting(st, sub, 2, FIT_PARAMS)
+    s2 = pt_gpu.last_stats()
+    assert s2["rows"] == len(sub) and s2["frames"] == st.shape[0] and s2["flagged_frames"] == 0
