@@ -722,6 +722,24 @@ def main():
     ms_alt = 1e3 * (time.perf_counter() - t1) / args.steps
     fitted_alt = int((out[: n_alt * 18].view(-1, 18)[:, 13] > 0).sum().item())
     other_cap = {"max_iter": alt_iter, "ms_per_step": ms_alt, "localizations_per_s": fitted_alt / (ms_alt * 1e-3)}
+
+    # The library's default fit on the same workload: FP64 in the reference's operation order (rows
+    # bit-equal to the reference's) at the reference's own cap, the drop-in Localization's rate.
+    pt.set_option("fit_fp64", 1)
+    pt.set_option("max_iter", 75)
+    pt.localization_device(stack, out, thr, 0.0, 2, fit_params, max_per_plane)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ref_steps = max(1, min(args.steps, 2))
+    for _ in range(ref_steps):
+        n_ref = pt.localization_device(stack, out, thr, 0.0, 2, fit_params, max_per_plane)
+    torch.cuda.synchronize()
+    ms_ref = 1e3 * (time.perf_counter() - t1) / ref_steps
+    fitted_ref = int((out[: n_ref * 18].view(-1, 18)[:, 13] > 0).sum().item())
+    library_default = {"fit": "FP64, the reference's operation order (library default)", "max_iter": 75,
+                       "steps": ref_steps, "ms_per_step": ms_ref, "localizations_per_s": fitted_ref / (ms_ref * 1e-3),
+                       "fitted_rows": fitted_ref}
+    pt.set_option("fit_fp64", 0 if args.precision == "fp32" else 1)
     pt.set_option("max_iter", args.max_iter)
 
     precision = fit_precision(pt, torch, stack, thr, fit_params, min(F, 200)) if args.precision == "fp32" else None
@@ -761,7 +779,7 @@ def main():
         "config": {"workload": f"{F} frames {S}x{S} u16, {args.emitters} emitters/frame, a trous + watershed "
                                f"(ratio 0) + 7x7 Gaussian LM fit mode 2 (max {args.max_iter} it), threshold "
                                f"{thr:.6f} (AutoThreshold of frame 0)", "max_iter": args.max_iter,
-                   "other_iteration_cap": other_cap,
+                   "other_iteration_cap": other_cap, "library_default_fit": library_default,
                    "frames": F, "height": S, "width": S, "emitters_per_frame": args.emitters, "fit": 2,
                    "roi": 7, "threshold": thr, "rows": rows, "fitted_rows": fitted, "seed": seed,
                    "stage_ms_per_step": stage_ms, "options": options,

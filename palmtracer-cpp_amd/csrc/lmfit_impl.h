@@ -174,12 +174,25 @@ __device__ inline void fill_row(int mode, double* pt, double x, double y, double
     pt[kCircularity] = (sx > kEpsilon && sy > kEpsilon) ? (sy < sx ? sy : sx) / (sx < sy ? sy : sx) : 1;
 }
 
+// A ROI's pixels: pixel (i, j), i and j in [-half, half] around the centre pixel (ux, uy), at
+// base[(i + half) * rs + (j + half) * cs] (in the frame: rs = W, cs = 1; staged: see kStageMax).
 struct Patch {
-    const uint16_t* img;
-    int W, half, roi;
+    const uint16_t* base;
+    uint32_t rs, cs;
+    int half, roi;
     uint64_t ux, uy;
-    __device__ double v(int i, int j) const { return (double)img[(uy + i) * (uint64_t)W + (ux + j)]; }
+    __device__ double v(int i, int j) const { return (double)base[(size_t)(i + half) * rs + (size_t)(j + half) * cs]; }
 };
+
+__device__ inline Patch frame_patch(const uint16_t* img, int W, int half, int roi, uint64_t ux, uint64_t uy)
+{
+    return Patch{img + (uy - (uint64_t)half) * (uint64_t)W + (ux - (uint64_t)half), (uint32_t)W, 1u, half, roi, ux, uy};
+}
+
+// ROIs of up to kStageMax pixels are copied once per ROI into LDS, pixel k of lane l at
+// stage[k * 64 + l] (consecutive lanes in consecutive halfwords), so that the passes over the ROI
+// read LDS rather than the frame.
+constexpr int kStageMax = 121;   // 11 x 11
 
 // GaussianFit::lm: model, residuals, alpha (upper triangle), beta and chi^2 at p.
 template <typename T, int MODE>
@@ -262,13 +275,38 @@ __device__ inline double lm_r2(const Patch& pa, const double* p, double sigma, d
     return isnan(r) ? 0.0 : r * r;
 }
 
+// GaussianFit::Fit (GaussianFit.cpp:48-215) as three steps over a per-ROI state, so that a lane can
+// run its ROIs one after the other at its own pace: lm_start (border check, initP, the first model
+// and Jacobian pass), lm_iterate (one pass of the LM loop body) and lm_finish (R^2 and the row).
 template <typename T, int MODE>
-__device__ void fit_one(const uint16_t* img, int W, int H, const Roi& r, const FitParams& fp, double* row, uint32_t& nJac,
-                        uint32_t& nTrial)
+struct LmState {
+    static constexpr int NP = NPar<MODE>::v;
+    Patch pa;
+    double p[NP], lo[NP], hi[NP];
+    T al[NP][NP], be[NP];
+    double dP[NP];
+    double err, lambda, m;
+    int it;
+};
+
+// Outcomes of a step: the loop goes on at s.it + 1; the loop ends (lm_finish writes the row); the
+// row is written (failure); the trial point was accepted and the model and Jacobian pass at the new
+// point is due (lm_pass, then lm_accepted).
+enum : int { kLmContinue = 0, kLmFinish = 1, kLmDone = 2, kLmAccept = 3 };
+
+__device__ inline void fail_row(int mode, const Roi& r, double* row)
 {
-    constexpr int NP = NPar<MODE>::v;
+    fill_row(mode, row, r.x, r.y, 0.0, r.seg, r.surf, -1, -1, -1, -1, -1, -1, -1);
+}
+
+// Border check and initP: kLmDone when the row is written already (border), else kLmContinue with
+// the state ready for the first model and Jacobian pass (lm_pass), after which lm_started runs. With
+// a stage (this lane's slot of the workgroup's LDS, see kStageMax) a small ROI is copied there.
+template <typename T, int MODE>
+__device__ inline int lm_prepare(const uint16_t* img, int W, int H, const Roi& r, const FitParams& fp, double* row,
+                                 LmState<T, MODE>& s, uint16_t* stage)
+{
     const int roi = fp.roi, half = roi / 2;
-    const double zero = 0.0;
     const uint64_t ux = to_size(r.x), uy = to_size(r.y);
     const uint64_t h64 = (uint64_t)half;
     // Border check (GaussianFit.cpp:51-56), with the reference's unsigned arithmetic.
@@ -276,19 +314,29 @@ __device__ void fit_one(const uint16_t* img, int W, int H, const Roi& r, const F
     // image) gets the border row.
     if (h64 > (uint64_t)W || h64 > (uint64_t)H || h64 > ux || ux >= (uint64_t)W - h64 || h64 > uy ||
         uy >= (uint64_t)H - h64) {
-        fill_row(MODE, row, r.x, r.y, zero, r.seg, r.surf, -1, -1, -1, -1, -1, -1, -1);
-        return;
+        fail_row(MODE, r, row);
+        return kLmDone;
     }
-    const Patch pa{img, W, half, roi, ux, uy};
+    s.pa = frame_patch(img, W, half, roi, ux, uy);
+    if (stage && roi * roi <= kStageMax) {
+        const Patch src = s.pa;
+        s.pa.base = stage;
+        s.pa.rs = (uint32_t)roi * 64u;
+        s.pa.cs = 64u;
+        for (int i = 0; i < roi; ++i)
+            for (int j = 0; j < roi; ++j) stage[(i * roi + j) * 64] = src.base[(size_t)i * src.rs + j];
+    }
     double minI = 1.79769313486231570815e308, maxI = -1.79769313486231570815e308;
     for (int i = -half; i <= half; ++i)
         for (int j = -half; j <= half; ++j) {
-            const double v = pa.v(i, j);
+            const double v = s.pa.v(i, j);
             if (v < minI) minI = v;
             if (maxI < v) maxI = v;
         }
     // initP (Fitting.cpp:126-175)
-    double p[NP], lo[NP], hi[NP], pTry[NP], dP[NP];
+    double* p = s.p;
+    double* lo = s.lo;
+    double* hi = s.hi;
     p[0] = minI; p[1] = maxI - minI; p[2] = r.x; p[3] = r.y;
     lo[0] = 0; lo[1] = 0; lo[2] = r.x - 1; lo[3] = r.y - 1;
     hi[0] = 65535.0; hi[1] = 65535.0; hi[2] = r.x + 1; hi[3] = r.y + 1;
@@ -298,116 +346,157 @@ __device__ void fit_one(const uint16_t* img, int W, int H, const Roi& r, const F
         lo[4] = lo[5] = 0.1; hi[4] = hi[5] = 20.0;
     }
     if constexpr (MODE == 4) { p[6] = fp.theta; lo[6] = -150; hi[6] = 150; }
+    return kLmContinue;
+}
 
-    T al[NP][NP], be[NP];
-    double err = (double)lm_pass<T, MODE>(pa, p, fp.sigmaX, fp.theta, al, be);
-    ++nJac;
-    bool bad = !isfinite(err);
+// After the first pass (s.err, s.al, s.be at the initial point): kLmDone when the row is written
+// (non-finite), else kLmContinue with s.it at the loop's first value.
+template <typename T, int MODE>
+__device__ inline int lm_started(LmState<T, MODE>& s, const Roi& r, const FitParams& fp, double* row)
+{
+    constexpr int NP = NPar<MODE>::v;
+    bool bad = !isfinite(s.err);
 #pragma unroll
-    for (int a = 0; a < NP; ++a) bad |= !isfinite((double)be[a]);
+    for (int a = 0; a < NP; ++a) bad |= !isfinite((double)s.be[a]);
     if (bad) {
-        fill_row(MODE, row, r.x, r.y, zero, r.seg, r.surf, -1, -1, -1, -1, -1, -1, -1);
-        return;
+        fail_row(MODE, r, row);
+        return kLmDone;
     }
-    auto max_abs_beta = [&]() {
-        double m = fabs((double)be[0]);
+    double mb = fabs((double)s.be[0]);
 #pragma unroll
-        for (int a = 1; a < NP; ++a) { const double v = fabs((double)be[a]); if (m < v) m = v; }
-        return m;
-    };
-    auto rel_step = [&]() {
-        double m = 0.0;
+    for (int a = 1; a < NP; ++a) { const double v = fabs((double)s.be[a]); if (mb < v) mb = v; }
+    s.lambda = 1e-2;
+    s.m = mb;
+    s.it = (mb < 1e-6) ? fp.maxIter : 0;
+    return kLmContinue;
+}
+
+template <typename T, int MODE>
+__device__ inline double lm_rel_step(const LmState<T, MODE>& s)
+{
+    constexpr int NP = NPar<MODE>::v;
+    double m = 0.0;
 #pragma unroll
-        for (int a = 0; a < NP; ++a) {
-            const double ap = fabs(p[a]);
-            const double q = fabs(dP[a]) / (ap < 1e-12 ? 1e-12 : ap);
-            if (m < q) m = q;
-        }
-        return m;
-    };
-    double lambda = 1e-2;
-    double m = max_abs_beta();
-    for (int it = (m < 1e-6) ? fp.maxIter : 0; it < fp.maxIter; ++it) {
-        double A[NP][NP], Ai[NP][NP];
-#pragma unroll
-        for (int a = 0; a < NP; ++a)
-#pragma unroll
-            for (int b = 0; b < NP; ++b) A[a][b] = (double)(a <= b ? al[a][b] : al[b][a]);
-#pragma unroll
-        for (int a = 0; a < NP; ++a) A[a][a] = A[a][a] == 0.0 ? 1.0 : (1.0 + lambda) * A[a][a];
-        if (!gauss_jordan<NP>(A, Ai)) {
-            if (lambda < 1e7) { lambda = fmin(lambda * 11, 1e7); continue; }
-            fill_row(MODE, row, r.x, r.y, zero, r.seg, r.surf, -1, -1, -1, -1, -1, -1, -1);
-            return;
-        }
-        bool nonfinite = false;
-#pragma unroll
-        for (int a = 0; a < NP; ++a) {
-            double d = 0.0;
-#pragma unroll
-            for (int b = 0; b < NP; ++b) d += Ai[a][b] * (double)be[b];
-            dP[a] = d;
-            nonfinite |= !isfinite(d);
-            const double v = p[a] + d;
-            pTry[a] = v < lo[a] ? lo[a] : (hi[a] < v ? hi[a] : v);
-        }
-        if (nonfinite) {
-            fill_row(MODE, row, r.x, r.y, zero, r.seg, r.surf, -1, -1, -1, -1, -1, -1, -1);
-            return;
-        }
-        bool tiny = true;
-#pragma unroll
-        for (int a = 0; a < NP; ++a) { dP[a] = pTry[a] - p[a]; tiny &= fabs(dP[a]) < 1e-15; }
-        if (tiny) {
-            if (lambda >= 1e7) break;
-            lambda = fmin(lambda * 11, 1e7);
-            continue;
-        }
-        // gain = 2 dP.beta - dP.alpha.dP (GaussianFit.cpp:248-259)
-        double quad = 0.0, lin = 0.0;
-#pragma unroll
-        for (int a = 0; a < NP; ++a) {
-            double ti = 0.0;
-#pragma unroll
-            for (int b = 0; b < NP; ++b) ti += (double)(a <= b ? al[a][b] : al[b][a]) * dP[b];
-            quad += dP[a] * ti;
-            lin += dP[a] * (double)be[a];
-        }
-        const double gain = (2.0 * lin) - quad;
-        if (gain <= 0.0) {
-            if (lambda >= 1e7) break;
-            lambda = fmin(lambda * 11, 1e7);
-            m = rel_step();
-            if (m < 1e-6 && it >= 2) break;
-            continue;
-        }
-        m = 0.0;
-#pragma unroll
-        for (int a = 0; a < NP; ++a) m = m + dP[a] * ((lambda * dP[a]) + (double)be[a]);
-        const double errTry = (double)lm_error<T, MODE>(pa, pTry, fp.sigmaX, fp.theta);
-        ++nTrial;
-        const double rho = (err - errTry) / (2.0 * m);
-        if (rho > 1e-2) {
-#pragma unroll
-            for (int a = 0; a < NP; ++a) p[a] = pTry[a];
-            err = (double)lm_pass<T, MODE>(pa, p, fp.sigmaX, fp.theta, al, be);
-            ++nJac;
-            lambda = fmax(lambda / 9, 1e-7);
-        } else {
-            if (lambda >= 1e7) break;
-            lambda = fmin(lambda * 11, 1e7);
-        }
-        m = rel_step();
-        if (it >= 2) {
-            if (m < 1e-6) break;
-            if (err / (double)(roi * roi) < 1e-5) break;
-            if (max_abs_beta() < 1e-6) break;
-        }
+    for (int a = 0; a < NP; ++a) {
+        const double ap = fabs(s.p[a]);
+        const double q = fabs(s.dP[a]) / (ap < 1e-12 ? 1e-12 : ap);
+        if (m < q) m = q;
     }
+    return m;
+}
+
+// The end of the loop body (after the accept / reject): the stopping rules.
+template <typename T, int MODE>
+__device__ inline int lm_tail(LmState<T, MODE>& s, const FitParams& fp)
+{
+    constexpr int NP = NPar<MODE>::v;
+    s.m = lm_rel_step<T, MODE>(s);
+    if (s.it >= 2) {
+        if (s.m < 1e-6) return kLmFinish;
+        if (s.err / (double)(fp.roi * fp.roi) < 1e-5) return kLmFinish;
+        double mb = fabs((double)s.be[0]);
+#pragma unroll
+        for (int a = 1; a < NP; ++a) { const double v = fabs((double)s.be[a]); if (mb < v) mb = v; }
+        if (mb < 1e-6) return kLmFinish;
+    }
+    return kLmContinue;
+}
+
+// The loop body at s.it up to the trial decision: kLmContinue, kLmFinish, kLmDone, or kLmAccept
+// (s.p moved to the trial point; the pass there and lm_accepted follow).
+template <typename T, int MODE>
+__device__ inline int lm_iterate(LmState<T, MODE>& s, const Roi& r, const FitParams& fp, double* row, uint32_t& nTrial)
+{
+    constexpr int NP = NPar<MODE>::v;
+    double* p = s.p;
+    double* dP = s.dP;
+    double pTry[NP];
+    double A[NP][NP], Ai[NP][NP];
+#pragma unroll
+    for (int a = 0; a < NP; ++a)
+#pragma unroll
+        for (int b = 0; b < NP; ++b) A[a][b] = (double)(a <= b ? s.al[a][b] : s.al[b][a]);
+#pragma unroll
+    for (int a = 0; a < NP; ++a) A[a][a] = A[a][a] == 0.0 ? 1.0 : (1.0 + s.lambda) * A[a][a];
+    if (!gauss_jordan<NP>(A, Ai)) {
+        if (s.lambda < 1e7) { s.lambda = fmin(s.lambda * 11, 1e7); return kLmContinue; }
+        fail_row(MODE, r, row);
+        return kLmDone;
+    }
+    bool nonfinite = false;
+#pragma unroll
+    for (int a = 0; a < NP; ++a) {
+        double d = 0.0;
+#pragma unroll
+        for (int b = 0; b < NP; ++b) d += Ai[a][b] * (double)s.be[b];
+        dP[a] = d;
+        nonfinite |= !isfinite(d);
+        const double v = p[a] + d;
+        pTry[a] = v < s.lo[a] ? s.lo[a] : (s.hi[a] < v ? s.hi[a] : v);
+    }
+    if (nonfinite) {
+        fail_row(MODE, r, row);
+        return kLmDone;
+    }
+    bool tiny = true;
+#pragma unroll
+    for (int a = 0; a < NP; ++a) { dP[a] = pTry[a] - p[a]; tiny &= fabs(dP[a]) < 1e-15; }
+    if (tiny) {
+        if (s.lambda >= 1e7) return kLmFinish;
+        s.lambda = fmin(s.lambda * 11, 1e7);
+        return kLmContinue;
+    }
+    // gain = 2 dP.beta - dP.alpha.dP (GaussianFit.cpp:248-259)
+    double quad = 0.0, lin = 0.0;
+#pragma unroll
+    for (int a = 0; a < NP; ++a) {
+        double ti = 0.0;
+#pragma unroll
+        for (int b = 0; b < NP; ++b) ti += (double)(a <= b ? s.al[a][b] : s.al[b][a]) * dP[b];
+        quad += dP[a] * ti;
+        lin += dP[a] * (double)s.be[a];
+    }
+    const double gain = (2.0 * lin) - quad;
+    if (gain <= 0.0) {
+        if (s.lambda >= 1e7) return kLmFinish;
+        s.lambda = fmin(s.lambda * 11, 1e7);
+        s.m = lm_rel_step<T, MODE>(s);
+        if (s.m < 1e-6 && s.it >= 2) return kLmFinish;
+        return kLmContinue;
+    }
+    double m = 0.0;
+#pragma unroll
+    for (int a = 0; a < NP; ++a) m = m + dP[a] * ((s.lambda * dP[a]) + (double)s.be[a]);
+    s.m = m;
+    const double errTry = (double)lm_error<T, MODE>(s.pa, pTry, fp.sigmaX, fp.theta);
+    ++nTrial;
+    const double rho = (s.err - errTry) / (2.0 * s.m);
+    if (rho > 1e-2) {
+#pragma unroll
+        for (int a = 0; a < NP; ++a) p[a] = pTry[a];
+        return kLmAccept;
+    }
+    if (s.lambda >= 1e7) return kLmFinish;
+    s.lambda = fmin(s.lambda * 11, 1e7);
+    return lm_tail<T, MODE>(s, fp);
+}
+
+// After the pass at an accepted point (s.err, s.al, s.be there).
+template <typename T, int MODE>
+__device__ inline int lm_accepted(LmState<T, MODE>& s, const FitParams& fp)
+{
+    s.lambda = fmax(s.lambda / 9, 1e-7);
+    return lm_tail<T, MODE>(s, fp);
+}
+
+template <typename T, int MODE>
+__device__ inline void lm_finish(const LmState<T, MODE>& s, const Roi& r, const FitParams& fp, double* row)
+{
+    const double* p = s.p;
     const double off = p[0], inten = p[1], x = p[2], y = p[3];
-    const double rss = lm_r2<T, MODE>(pa, p, fp.sigmaX, fp.theta);
+    const double rss = lm_r2<T, MODE>(s.pa, p, fp.sigmaX, fp.theta);
     if (rss < kEpsilon) {
-        fill_row(MODE, row, x, y, zero, r.seg, r.surf, -1, -1, -1, -1, -1, -1, -1);
+        fill_row(MODE, row, x, y, 0.0, r.seg, r.surf, -1, -1, -1, -1, -1, -1, -1);
         return;
     }
     double sx = fp.sigmaX, sy = fp.sigmaY, th = fp.theta;
@@ -415,9 +504,20 @@ __device__ void fit_one(const uint16_t* img, int W, int H, const Roi& r, const F
     if constexpr (MODE == 2) { sx = sy = p[4]; th = 0; }
     if constexpr (MODE >= 3) { sx = p[4] / kSqrt2; sy = p[5] / kSqrt2; }
     if constexpr (MODE == 4) th = p[6];
-    fill_row(MODE, row, x, y, zero, r.seg, r.surf, off, inten, sx, sy, th, rss, -1);
+    fill_row(MODE, row, x, y, 0.0, r.seg, r.surf, off, inten, sx, sy, th, rss, -1);
 }
 
+__device__ inline void store_row(double* rows, uint32_t i, const double* row)
+{
+    double* o = rows + (size_t)i * kCols;
+#pragma unroll
+    for (int c = 0; c < kCols; ++c) o[c] = row[c];
+}
+
+// One ROI per lane, each lane running its ROIs (i, i + stride, ...) one after the other at its own
+// pace: a lane whose ROI ends takes its next one while the other lanes of the wave go on iterating,
+// so a wave is not held by its slowest ROI of every round. Every ROI's arithmetic is that of the
+// reference, in its order, whatever the schedule.
 template <typename T, int MODE>
 __global__ void __launch_bounds__(64) fit_kernel(const uint16_t* __restrict__ frames, size_t npx, int W, int H,
                                                  const Roi* __restrict__ rois, const uint32_t* __restrict__ count,
@@ -425,19 +525,76 @@ __global__ void __launch_bounds__(64) fit_kernel(const uint16_t* __restrict__ fr
                                                  unsigned long long* __restrict__ passes, const uint32_t* __restrict__ frameMap)
 {
     const uint32_t n = count ? *count : nHost;
+    const uint32_t stride = gridDim.x * 64;
     uint32_t nJac = 0, nTrial = 0;
-    for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i < n; i += gridDim.x * 64) {
-        const Roi r = rois[i];
-        double row[kCols];
-        if constexpr (MODE == 0) {
+    uint32_t i = blockIdx.x * 64 + threadIdx.x;
+    if constexpr (MODE == 0) {
+        for (; i < n; i += stride) {
+            const Roi r = rois[i];
+            double row[kCols];
             fill_row(0, row, r.x, r.y, 0.0, r.seg, r.surf, 0, 0, -1, -1, -1, -1, -1);
-        } else {
-            const size_t fr = frameMap ? frameMap[r.frame] : r.frame;
-            fit_one<T, MODE == 0 ? 1 : MODE>(frames + fr * npx, W, H, r, fp, row, nJac, nTrial);
+            store_row(rows, i, row);
         }
-        double* o = rows + (size_t)i * kCols;
-#pragma unroll
-        for (int c = 0; c < kCols; ++c) o[c] = row[c];
+    } else {
+        __shared__ uint16_t sStage[kStageMax * 64];
+        uint16_t* stage = sStage + threadIdx.x;
+        LmState<T, MODE> s;
+        Roi r{};
+        int phase = 0;   // 0: needs a ROI, 1: in the LM loop, 2: loop ended (its row is due)
+        while (true) {
+            double row[kCols];
+            bool pass = false;    // the model and Jacobian pass at s.p is due in this step
+            bool fresh = false;   // ... for a ROI started in this step
+            // One loop body for the lanes in the loop.
+            if (phase == 1) {
+                const int st = lm_iterate<T, MODE>(s, r, fp, row, nTrial);
+                if (st == kLmAccept) {
+                    pass = true;
+                } else if (st == kLmDone) {
+                    store_row(rows, i, row);
+                    i += stride;
+                    phase = 0;
+                } else if (st == kLmFinish || ++s.it >= fp.maxIter) {
+                    phase = 2;
+                }
+            }
+            // Rows of the loops that ended.
+            if (phase == 2) {
+                lm_finish<T, MODE>(s, r, fp, row);
+                store_row(rows, i, row);
+                i += stride;
+                phase = 0;
+            }
+            // Next ROI: border check and initP (border rows written on the way).
+            while (phase == 0 && i < n) {
+                r = rois[i];
+                const size_t fr = frameMap ? frameMap[r.frame] : r.frame;
+                if (lm_prepare<T, MODE>(frames + fr * npx, W, H, r, fp, row, s, stage) == kLmDone) {
+                    store_row(rows, i, row);
+                    i += stride;
+                } else {
+                    pass = fresh = true;
+                    phase = 1;
+                }
+            }
+            if (__ballot(phase != 0) == 0) break;
+            // The model and Jacobian passes of this step, of accepted points and new ROIs together.
+            if (pass) {
+                s.err = (double)lm_pass<T, MODE>(s.pa, s.p, fp.sigmaX, fp.theta, s.al, s.be);
+                ++nJac;
+            }
+            if (fresh) {
+                if (lm_started<T, MODE>(s, r, fp, row) == kLmDone) {
+                    store_row(rows, i, row);
+                    i += stride;
+                    phase = 0;
+                } else if (s.it >= fp.maxIter) {
+                    phase = 2;
+                }
+            } else if (pass && (lm_accepted<T, MODE>(s, fp) == kLmFinish || ++s.it >= fp.maxIter)) {
+                phase = 2;
+            }
+        }
     }
     if (passes) {
         // Passes over the ROI actually executed (model + Jacobian, and trial model), for the FLOP count.

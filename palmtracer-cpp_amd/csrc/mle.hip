@@ -52,7 +52,7 @@ __device__ void mle_one(const uint16_t* img, int W, int H, const Roi& r, const F
         fill_row(MODE, row, r.x, r.y, 0.0, r.seg, r.surf, -1, -1, -1, -1, -1, -1, -1);
         return;
     }
-    const Patch pa{img, W, half, roi, ux, uy};
+    const Patch pa = frame_patch(img, W, half, roi, ux, uy);
     double minI = 1.79769313486231570815e308, maxI = -1.79769313486231570815e308;
     for (int i = -half; i <= half; ++i)
         for (int j = -half; j <= half; ++j) {
